@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Throughput of the stable-fluids step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): decaying 2D
+turbulence on a 1024^2 periodic MAC grid, FP32 fields, nu = 1e-4, MacCormack
+advection + explicit diffusion, dt at CFL 0.5 evaluated on the device each step,
+projection by multigrid V(2,2) red-black, 9 levels (minSize 4), 40 coarse
+sweeps, iterated until the FP64 residual max|div u* - L p| <= 1e-6 after every
+projection. The initial state is seeded random-phase (synthetic).
+
+A step is one full advect/diffuse/project pass. `value` is cell-steps/s over the
+timed steps (inputs already resident on the device). The roofline object is
+measured with HIP events on the solver's own stream over a second, profiled run
+of the same number of steps; the CPU baseline is the f64 oracle timed on this
+host on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-sim_amd"))
+
+METRIC = "cell-steps/s at 1024² to max|∇·u|≤1e-6, learned-MG vs plain MG; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic (compulsory) bytes per cell of one launch scope, FP32 fields, FP64 pressure
+KINDS = {
+    0: ("advection (SL forward + MacCormack correction)", 16 + 24),
+    1: ("level-0 red-black colour pass", 6),
+    2: ("FP64 refinement residual", 32),
+    3: ("FP64-pressure gradient subtraction", 24),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=12)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    return ap.parse_args()
+
+
+def make_solver(fluidsim, n, tol):
+    return fluidsim.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                                   pressureIterations=60, tolerance=tol, mgMinSize=4, mgPre=2, mgPost=2,
+                                   mgCoarse=40)
+
+
+def cpu_baseline(n, seconds):
+    """f64 oracle (single thread, then all host cores) on the same workload, exact FFT projection."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    h = 2 * math.pi / n
+    out = {}
+    for label, threads in (("single", 1), ("omp", min(16, os.cpu_count() or 1))):
+        O.set_threads(threads)
+        s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+        s.u[:] = u0.astype(np.float32)
+        s.v[:] = v0.astype(np.float32)
+        budget = seconds if label == "single" else max(2.0, seconds / 4)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            dt = 0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h)
+            s.step(dt)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= budget or steps >= 200:
+                break
+        out[label] = (n * n * steps / el, threads, steps, el)
+    O.set_threads(1)
+    v, c, st, el = out["single"]
+    vo, co, sto, elo = out["omp"]
+    return {"value": v, "unit": "cell-steps/s", "cores": c, "kind": "port",
+            "sample": f"f64 oracle (oracle/fsoracle.c), same {n}^2 workload with exact FFT projection, "
+                      f"{st} steps in {el:.1f} s on 1 thread",
+            "omp": {"value": vo, "cores": co, "sample": f"{sto} steps in {elo:.1f} s"}}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); this build has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+    import fluidsim
+    from fluidsim import initial
+    dev = fluidsim.createDevice()
+    n = a.n
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    g = make_solver(fluidsim, n, a.tol)
+    g.uploadVelocity(u0, v0)
+    print(f"# device: {dev['info']['description']}; lib {fluidsim.library_path()}", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        g.stepCFL(0.5)
+    g.sync()
+    st0 = g.stats()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.stepCFL(0.5)
+    g.sync()
+    el = time.perf_counter() - t0
+    st1 = g.stats()
+    cycles = (st1["total_vcycles"] - st0["total_vcycles"]) / max(a.steps, 1)
+    value = n * n * a.steps / el
+    d = g.diagnostics()
+    print(f"# timed {a.steps} steps in {el:.3f} s: {value:.4g} cell-steps/s, {cycles:.2f} V-cycles/step, "
+          f"last residual {st1['last_residual']:.3g}, stored-field max|div| {d['max_divergence']:.3g}",
+          file=sys.stderr, flush=True)
+
+    roof = None
+    per_kind = {}
+    if not a.no_roofline:
+        psteps = max(10, min(a.steps, 200))
+        for kind in KINDS:
+            g.profileEnable(kind)
+            for _ in range(psteps):
+                g.stepCFL(0.5)
+            ms, launches = g.profileRead()
+            per_kind[kind] = (ms, launches, psteps)
+        g.profileEnable(-1)
+        dom = max(per_kind, key=lambda k: per_kind[k][0])
+        ms, launches, ps = per_kind[dom]
+        avg_s = ms / 1e3 / max(launches, 1)
+        bytes_per_launch = KINDS[dom][1] * n * n
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": KINDS[dom][0], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / ps,
+                "algorithmic_bytes_per_launch": bytes_per_launch,
+                "share_of_step": ms / (el / a.steps * ps * 1e3)}
+        for k, (ms_k, l_k, ps_k) in per_kind.items():
+            print(f"# kind {k} {KINDS[k][0]}: {ms_k / max(l_k, 1) * 1e3:.2f} us/launch, "
+                  f"{l_k / ps_k:.1f} launches/step, {ms_k / ps_k:.3f} ms/step, "
+                  f"{KINDS[k][1] * n * n / (ms_k / 1e3 / max(l_k, 1)) / 1e9:.0f} GB/s algorithmic",
+                  file=sys.stderr, flush=True)
+    g.destroy()
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(n, a.cpu_seconds)
+        print(f"# cpu baseline: {cpu['value']:.4g} cell-steps/s (1 thread), "
+              f"{cpu['omp']['value']:.4g} ({cpu['omp']['cores']} threads)", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"decaying 2D turbulence {n}^2 periodic MAC grid, nu=1e-4, MacCormack + explicit "
+                        "diffusion, CFL 0.5, plain multigrid V(2,2) red-black 9 levels (minSize 4, 40 coarse "
+                        "sweeps) to FP64 max|div u* - L p| <= 1e-6 every step, zero initial guess",
+            "n": n,
+            "residual_tolerance": a.tol,
+            "residual_dtype": "f64",
+            "vcycles_per_step": cycles,
+            "guess": "zero",
+            "stored_field_max_div": d["max_divergence"],
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

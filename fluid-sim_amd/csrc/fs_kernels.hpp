@@ -1,0 +1,526 @@
+// fs_kernels.hpp -- HIP kernels (gfx950) of the stable-fluids step on a periodic MAC grid.
+//
+// Storage is SoA FP32, row-major with i fastest: u[j*nx+i] on the left face of
+// cell (i,j), v[j*nx+i] on its bottom face, pressure at the centre
+// (the index map of src/core/grid.js:83-105). Launches use 64-wide rows so one
+// wavefront reads 64 consecutive faces: every stencil load is coalesced.
+//
+// Arithmetic is written with explicit fmaf where a fused multiply-add is meant
+// and the library is compiled with -ffp-contract=off, so two kernels that
+// evaluate the same formula produce the same bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsk {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  if ((unsigned)i < (unsigned)n) return i;
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ int nxt(int i, int n) { return (i + 1 == n) ? 0 : i + 1; }
+__device__ __forceinline__ int prv(int i, int n) { return (i == 0) ? n - 1 : i - 1; }
+
+// Periodic bilinear sample at grid coordinates (gx, gy); the CPU form
+// a + (b-a)*f of src/core/grid.js:147-165.
+__device__ __forceinline__ float bilerp(const float* __restrict__ a, int nx, int ny, float gx, float gy) {
+  float fi = floorf(gx), fj = floorf(gy);
+  int i0 = wrapi((int)fi, nx), j0 = wrapi((int)fj, ny);
+  float fx = gx - fi, fy = gy - fj;
+  int i1 = nxt(i0, nx), j1 = nxt(j0, ny);
+  const float* r0 = a + (size_t)j0 * nx;
+  const float* r1 = a + (size_t)j1 * nx;
+  float A = r0[i0], B = r0[i1], Cc = r1[i0], D = r1[i1];
+  float top = fmaf(B - A, fx, A);
+  float bot = fmaf(D - Cc, fx, Cc);
+  return fmaf(bot - top, fy, top);
+}
+
+// min/max of the four nodes a bilinear sample at (gx, gy) reads
+// (interpBounds, src/cpu/advect.js:113-121).
+__device__ __forceinline__ void bounds4(const float* __restrict__ a, int nx, int ny, float gx, float gy,
+                                        float& lo, float& hi) {
+  int i0 = wrapi((int)floorf(gx), nx), j0 = wrapi((int)floorf(gy), ny);
+  int i1 = nxt(i0, nx), j1 = nxt(j0, ny);
+  float A = a[(size_t)j0 * nx + i0], B = a[(size_t)j0 * nx + i1];
+  float Cc = a[(size_t)j1 * nx + i0], D = a[(size_t)j1 * nx + i1];
+  lo = fminf(fminf(A, B), fminf(Cc, D));
+  hi = fmaxf(fmaxf(A, B), fmaxf(Cc, D));
+}
+
+// Samplers in cell coordinates X = x/dx, Y = y/dy (src/core/grid.js:167-180):
+// u nodes sit at (i, j+1/2), v nodes at (i+1/2, j).
+__device__ __forceinline__ float sU(const float* __restrict__ u, int nx, int ny, float X, float Y) {
+  return bilerp(u, nx, ny, X, Y - 0.5f);
+}
+__device__ __forceinline__ float sV(const float* __restrict__ v, int nx, int ny, float X, float Y) {
+  return bilerp(v, nx, ny, X - 0.5f, Y);
+}
+
+// RK2 midpoint backward trace (src/cpu/advect.js:46-54) in cell coordinates;
+// cx = dt/dx, cy = dt/dy.
+__device__ __forceinline__ void trace(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
+                                      float X, float Y, float cx, float cy, float& Xb, float& Yb) {
+  float u1 = sU(u, nx, ny, X, Y), v1 = sV(v, nx, ny, X, Y);
+  float Xm = fmaf(-0.5f * cx, u1, X), Ym = fmaf(-0.5f * cy, v1, Y);
+  float u2 = sU(u, nx, ny, Xm, Ym), v2 = sV(v, nx, ny, Xm, Ym);
+  Xb = fmaf(-cx, u2, X);
+  Yb = fmaf(-cy, v2, Y);
+}
+
+// ------------------------------------------------------------------ advection
+
+// Semi-Lagrangian self-advection of both components, each traced from its own
+// face (src/cpu/advect.js:74-97; ADVECT_VELOCITY src/gpu/shaders.js:126-146).
+__global__ void __launch_bounds__(BX * BY)
+k_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
+            float* __restrict__ ov, int nx, int ny, float cx, float cy) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  float Xb, Yb;
+  trace(u, v, nx, ny, (float)i, j + 0.5f, cx, cy, Xb, Yb);
+  ou[(size_t)j * nx + i] = sU(u, nx, ny, Xb, Yb);
+  trace(u, v, nx, ny, i + 0.5f, (float)j, cx, cy, Xb, Yb);
+  ov[(size_t)j * nx + i] = sV(v, nx, ny, Xb, Yb);
+}
+
+// MacCormack correction + limiter (src/cpu/advect.js:149-206;
+// ADVECT_FIELD_BY + MACCORMACK_COMBINE src/gpu/shaders.js:155-231):
+// back = A^-1(fwd) traced through the ORIGINAL flow at -dt, out = fwd + (orig-back)/2
+// clamped to the four nodes around the forward trace.
+__global__ void __launch_bounds__(BX * BY)
+k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ fu,
+            const float* __restrict__ fv, float* __restrict__ ou, float* __restrict__ ov, int nx, int ny,
+            float cx, float cy) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float Xb, Yb, lo, hi;
+  {  // u face at (i, j+1/2)
+    float X = (float)i, Y = j + 0.5f;
+    trace(u, v, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+    float back = bilerp(fu, nx, ny, Xb, Yb - 0.5f);
+    float c = fmaf(0.5f, u[k] - back, fu[k]);
+    trace(u, v, nx, ny, X, Y, cx, cy, Xb, Yb);
+    bounds4(u, nx, ny, Xb, Yb - 0.5f, lo, hi);
+    ou[k] = c < lo ? lo : (c > hi ? hi : c);
+  }
+  {  // v face at (i+1/2, j)
+    float X = i + 0.5f, Y = (float)j;
+    trace(u, v, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+    float back = bilerp(fv, nx, ny, Xb - 0.5f, Yb);
+    float c = fmaf(0.5f, v[k] - back, fv[k]);
+    trace(u, v, nx, ny, X, Y, cx, cy, Xb, Yb);
+    bounds4(v, nx, ny, Xb - 0.5f, Yb, lo, hi);
+    ov[k] = c < lo ? lo : (c > hi ? hi : c);
+  }
+}
+
+// Explicit 5-point diffusion of both components (src/cpu/solver.js:56-69;
+// DIFFUSE src/gpu/shaders.js:401-425); a = nu dt/dx^2, b = nu dt/dy^2.
+__global__ void __launch_bounds__(BX * BY)
+k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
+          float* __restrict__ ov, int nx, int ny, float a, float b) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  int im = prv(i, nx), ip = nxt(i, nx), jm = prv(j, ny), jp = nxt(j, ny);
+  size_t k = (size_t)j * nx + i;
+  size_t kl = (size_t)j * nx + im, kr = (size_t)j * nx + ip, kb = (size_t)jm * nx + i, kt = (size_t)jp * nx + i;
+  float c = u[k];
+  ou[k] = c + a * (u[kl] - 2.f * c + u[kr]) + b * (u[kb] - 2.f * c + u[kt]);
+  c = v[k];
+  ov[k] = c + a * (v[kl] - 2.f * c + v[kr]) + b * (v[kb] - 2.f * c + v[kt]);
+}
+
+// Band forcing then linear drag, in place (src/cpu/solver.js:200-237 semantics).
+// modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi), coef) in turns per cell;
+// coefu/coefv carry scale*ky/|k| and -scale*kx/|k|. The argument is reduced to
+// [0,1) turns in FP64 before cospi, so large wavenumbers keep full FP32 accuracy.
+struct Mode { double tx, ty, tph; float cu, cv; };
+constexpr int MAX_MODES = 24;
+struct ModeSet { Mode m[MAX_MODES]; int n; };
+
+__global__ void __launch_bounds__(BX * BY)
+k_forcing(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms, float drag_factor) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float fu = 0.f, fv = 0.f;
+  for (int m = 0; m < ms.n; m++) {
+    const Mode& md = ms.m[m];
+    double tu = md.tx * i + md.ty * (j + 0.5) + md.tph;
+    double tv = md.tx * (i + 0.5) + md.ty * j + md.tph;
+    tu -= floor(tu);
+    tv -= floor(tv);
+    fu = fmaf(md.cu, cospif(2.f * (float)tu), fu);
+    fv = fmaf(md.cv, cospif(2.f * (float)tv), fv);
+  }
+  u[k] = (u[k] + fu) * drag_factor;
+  v[k] = (v[k] + fv) * drag_factor;
+}
+
+// --------------------------------------------------- projection, FP32 paths
+
+// Divergence at centres (src/core/grid.js:191-206; DIVERGENCE src/gpu/shaders.js:269-288).
+__global__ void __launch_bounds__(BX * BY)
+k_divergence(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ div, int nx, int ny,
+             float idx, float idy) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float du = u[(size_t)j * nx + nxt(i, nx)] - u[k];
+  float dv = v[(size_t)nxt(j, ny) * nx + i] - v[k];
+  div[k] = du * idx + dv * idy;
+}
+
+__global__ void k_fill_f32(float* __restrict__ a, size_t n, float val) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = val;
+}
+__global__ void k_fill_f64(double* __restrict__ a, size_t n, double val) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = val;
+}
+
+// One Jacobi sweep, ping-pong (JACOBI src/gpu/shaders.js:302-325).
+__global__ void __launch_bounds__(BX * BY)
+k_jacobi(const float* __restrict__ p, const float* __restrict__ rhs, float* __restrict__ q, int nx, int ny,
+         float h2, float omega) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float s = p[(size_t)j * nx + prv(i, nx)] + p[(size_t)j * nx + nxt(i, nx)] + p[(size_t)prv(j, ny) * nx + i] +
+            p[(size_t)nxt(j, ny) * nx + i];
+  float jac = (s - h2 * rhs[k]) * 0.25f;
+  float cur = p[k];
+  q[k] = (omega == 1.f) ? jac : fmaf(omega, jac - cur, cur);
+}
+
+// One colour of red-black Gauss-Seidel, in place (RED_BLACK_GS src/gpu/shaders.js:340-372).
+// Cells of the other colour are untouched, which equals the reference's copy-through.
+// Each thread owns the cell of its colour in a horizontal pair, so all lanes work.
+// x_zero: the iterate is logically zero (its stored values are stale) -- used for
+// the first pass of a correction that starts from zero (src/gpu/multigrid.js:306-307).
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_rb_colour(T* __restrict__ x, const T* __restrict__ rhs, int nx, int ny, float h2, float omega, int colour,
+            int x_zero) {
+  int ip = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int i = 2 * ip + ((j + colour) & 1);
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float s = 0.f;
+  if (!x_zero)
+    s = (float)x[(size_t)j * nx + prv(i, nx)] + (float)x[(size_t)j * nx + nxt(i, nx)] +
+        (float)x[(size_t)prv(j, ny) * nx + i] + (float)x[(size_t)nxt(j, ny) * nx + i];
+  float gs = (s - h2 * (float)rhs[k]) * 0.25f;
+  float cur = x_zero ? 0.f : (float)x[k];
+  x[k] = (T)((omega == 1.f) ? gs : fmaf(omega, gs - cur, cur));
+}
+
+// u -= grad p with FP32 pressure (SUBTRACT_GRADIENT src/gpu/shaders.js:375-398).
+__global__ void __launch_bounds__(BX * BY)
+k_gradient32(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ p,
+             float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, float idx, float idy) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t k = (size_t)j * nx + i;
+  float pc = p[k];
+  ou[k] = u[k] - (pc - p[(size_t)j * nx + prv(i, nx)]) * idx;
+  ov[k] = v[k] - (pc - p[(size_t)prv(j, ny) * nx + i]) * idy;
+}
+
+// ------------------------------------------------------------ reductions
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// block-wide max of a non-negative double, folded into *dst with an integer atomic
+// (non-negative IEEE doubles order like their bit patterns).
+__device__ __forceinline__ void block_max_atomic(double x, double* dst) {
+  __shared__ double sm[16];
+  int lane = threadIdx.x + threadIdx.y * blockDim.x;
+  int nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();  // a previous call in this kernel may still be reading sm
+  x = wave_max(x);
+  if ((lane & 63) == 0) sm[lane >> 6] = x;
+  __syncthreads();
+  if (lane == 0) {
+    double m = sm[0];
+    for (int w = 1; w < nw; w++) m = fmax(m, sm[w]);
+    if (m != m) m = __longlong_as_double(0x7ff0000000000000LL);  // NaN -> +inf
+    atomicMax((unsigned long long*)dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ----------------------------------------------- FP64 refinement residual
+
+// r = div(u*) - L(p + e) in FP64, p' = p + e stored FP64, r stored FP32 as the
+// multigrid right-hand side, max|r| folded into rmax[0] and rmax[1] cleared for
+// the next iteration. Skipped when the previous iteration already met tol
+// (skip != nullptr && *skip <= tol). e == nullptr means a zero correction,
+// p == nullptr a zero initial pressure. The divergence and Laplacian follow
+// src/core/grid.js:191-206 and src/cpu/projection.js:48-62.
+__global__ void __launch_bounds__(BX * BY)
+k_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
+             const float* __restrict__ e, double* __restrict__ pout, float* __restrict__ r32, int nx, int ny,
+             double idx, double idy, double idx2, double idy2, double* rmax, const double* skip, double tol) {
+  if (skip && *skip <= tol) return;
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double amax = 0.0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) rmax[1] = 0.0;
+  if (i < nx && j < ny) {
+    int im = prv(i, nx), ip = nxt(i, nx), jm = prv(j, ny), jp = nxt(j, ny);
+    size_t k = (size_t)j * nx + i;
+    size_t kl = (size_t)j * nx + im, kr = (size_t)j * nx + ip, kb = (size_t)jm * nx + i, kt = (size_t)jp * nx + i;
+    double pc = 0, pl = 0, pr = 0, pb = 0, pt = 0;
+    if (p) { pc = p[k]; pl = p[kl]; pr = p[kr]; pb = p[kb]; pt = p[kt]; }
+    if (e) { pc += e[k]; pl += e[kl]; pr += e[kr]; pb += e[kb]; pt += e[kt]; }
+    double lap = (pl - 2 * pc + pr) * idx2 + (pb - 2 * pc + pt) * idy2;
+    double dv = ((double)u[kr] - (double)u[k]) * idx + ((double)v[kt] - (double)v[k]) * idy;
+    double r = dv - lap;
+    pout[k] = pc;
+    r32[k] = (float)r;
+    amax = fabs(r);
+  }
+  block_max_atomic(amax, rmax);
+}
+
+// ------------------------------------------------------------- multigrid
+// Cell-centred V-cycle of src/gpu/multigrid.js:277-330: red-black smoothing,
+// full-weighting 2x2 restriction of the residual, bilinear prolongation added
+// to the finer correction. The residual and the restriction are fused: each
+// coarse cell evaluates the residual of its four children (RESIDUAL :48-66,
+// RESTRICT :72-93) and writes their mean. Every kernel returns immediately once
+// the FP64 residual of this refinement iteration met the tolerance.
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_mg_rb(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int colour, int x_zero,
+        const double* __restrict__ conv, double tol) {
+  if (conv && *conv <= tol) return;
+  int ip = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int i = 2 * ip + ((j + colour) & 1);
+  if (i >= n || j >= n) return;
+  size_t k = (size_t)j * n + i;
+  float s = 0.f;
+  if (!x_zero)
+    s = (float)x[(size_t)j * n + prv(i, n)] + (float)x[(size_t)j * n + nxt(i, n)] +
+        (float)x[(size_t)prv(j, n) * n + i] + (float)x[(size_t)nxt(j, n) * n + i];
+  x[k] = (T)((s - h2 * (float)rhs[k]) * 0.25f);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_mg_restrict(const T* __restrict__ x, const T* __restrict__ rhs, int n, float ih2, T* __restrict__ crhs,
+              int x_zero, const double* __restrict__ conv, double tol) {
+  if (conv && *conv <= tol) return;
+  int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+  int cn = n >> 1;
+  if (I >= cn || J >= cn) return;
+  float s = 0.f;
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      int i = 2 * I + a, j = 2 * J + b;
+      size_t k = (size_t)j * n + i;
+      float lap = 0.f;
+      if (!x_zero) {
+        float xc = (float)x[k];
+        lap = ((float)x[(size_t)j * n + prv(i, n)] + (float)x[(size_t)j * n + nxt(i, n)] +
+               (float)x[(size_t)prv(j, n) * n + i] + (float)x[(size_t)nxt(j, n) * n + i] - 4.f * xc) * ih2;
+      }
+      s += (float)rhs[k] - lap;
+    }
+  crhs[(size_t)J * cn + I] = (T)(s * 0.25f);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX * BY)
+k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero, const double* __restrict__ conv,
+             double tol) {
+  if (conv && *conv <= tol) return;
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= n || j >= n) return;
+  int cn = n >> 1;
+  // fine centre (i+1/2) sits at coarse coordinate (i+1/2)/2 - 1/2 (PROLONG :99-131)
+  int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+  int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+  float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+  int x0 = wrapi(i0, cn), x1 = nxt(x0, cn), y0 = wrapi(j0, cn), y1 = nxt(y0, cn);
+  float a = (float)xc[(size_t)y0 * cn + x0], b = (float)xc[(size_t)y0 * cn + x1];
+  float c = (float)xc[(size_t)y1 * cn + x0], d = (float)xc[(size_t)y1 * cn + x1];
+  float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+  float corr = top * (1.f - fy) + bot * fy;
+  size_t k = (size_t)j * n + i;
+  x[k] = (T)((x_zero ? 0.f : (float)x[k]) + corr);
+}
+
+// Coarsest level solved by one workgroup in LDS: the rhs mean is removed (the
+// periodic compatibility condition, src/cpu/projection.js:91-102) and `sweeps`
+// red-black sweeps run from a zero start (src/gpu/multigrid.js:312-313).
+constexpr int COARSE_MAX = 64;
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_mg_coarse(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int sweeps,
+            const double* __restrict__ conv, double tol) {
+  if (conv && *conv <= tol) return;
+  __shared__ float sx[COARSE_MAX * COARSE_MAX];
+  __shared__ float sr[COARSE_MAX * COARSE_MAX];
+  __shared__ double red[4];
+  int tid = threadIdx.x, cells = n * n;
+  double s = 0.0;
+  for (int k = tid; k < cells; k += 256) {
+    float r = (float)rhs[k];
+    sr[k] = r;
+    sx[k] = 0.f;
+    s += r;
+  }
+  s = wave_sum(s);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  float mean = (float)((red[0] + red[1] + red[2] + red[3]) / cells);
+  for (int k = tid; k < cells; k += 256) sr[k] -= mean;
+  __syncthreads();
+  for (int sw = 0; sw < sweeps; sw++) {
+    for (int colour = 0; colour < 2; colour++) {
+      for (int q = tid; q < cells; q += 256) {
+        int j = q / n, i = q - j * n;
+        if (((i + j) & 1) != colour) continue;
+        float sum = sx[j * n + prv(i, n)] + sx[j * n + nxt(i, n)] + sx[prv(j, n) * n + i] + sx[nxt(j, n) * n + i];
+        sx[j * n + i] = (sum - h2 * sr[j * n + i]) * 0.25f;
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = tid; k < cells; k += 256) x[k] = (T)sx[k];
+}
+
+// u = u* - grad p with FP64 pressure, stored FP32; max|u|, max|v| of the result
+// are folded into red[0], red[1] for the CFL time step of the next step.
+__global__ void __launch_bounds__(BX * BY)
+k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
+             float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, double idx, double idy,
+             double* __restrict__ red) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double mu = 0.0, mv = 0.0;
+  if (i < nx && j < ny) {
+    size_t k = (size_t)j * nx + i;
+    double pc = p[k];
+    float nu_ = (float)((double)u[k] - (pc - p[(size_t)j * nx + prv(i, nx)]) * idx);
+    float nv_ = (float)((double)v[k] - (pc - p[(size_t)prv(j, ny) * nx + i]) * idy);
+    ou[k] = nu_;
+    ov[k] = nv_;
+    mu = fabs((double)nu_);
+    mv = fabs((double)nv_);
+  }
+  block_max_atomic(mu, red + 0);
+  block_max_atomic(mv, red + 1);
+}
+
+// max|u|, max|v| of a field (for a CFL step before any projection ran).
+__global__ void __launch_bounds__(BX * BY)
+k_maxabs(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double* __restrict__ red) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double mu = 0.0, mv = 0.0;
+  if (i < nx && j < ny) {
+    size_t k = (size_t)j * nx + i;
+    mu = fabs((double)u[k]);
+    mv = fabs((double)v[k]);
+  }
+  block_max_atomic(mu, red + 0);
+  block_max_atomic(mv, red + 1);
+}
+
+// dt = cfl / (max|u|/dx + max|v|/dy) (src/core/grid.js:322-327), written to
+// dt_out; the accumulators are cleared for the next projection.
+__global__ void k_cfl_dt(double* __restrict__ red, double cfl, double idx, double idy, double dt_max,
+                         double* __restrict__ dt_out) {
+  double rate = red[0] * idx + red[1] * idy;
+  double dt = rate > 0 ? cfl / rate : dt_max;
+  if (!(dt <= dt_max)) dt = dt_max;
+  *dt_out = dt;
+  red[0] = 0.0;
+  red[1] = 0.0;
+}
+
+// Per-block FP64 diagnostics of the stored FP32 field (src/core/grid.js:290-319):
+// part[b*8 + {0..7}] = {sum KE_c, max|div|, sum div^2, sum u, sum v, max|u|, max|v|, nonfinite}.
+constexpr int DIAG_W = 8;
+__global__ void __launch_bounds__(BX * BY)
+k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double idx, double idy,
+       double* __restrict__ part) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double ke = 0, md = 0, d2 = 0, su = 0, sv = 0, mu = 0, mv = 0, nf = 0;
+  if (i < nx && j < ny) {
+    size_t k = (size_t)j * nx + i;
+    double uc = u[k], vc = v[k];
+    double ur = u[(size_t)j * nx + nxt(i, nx)], vt = v[(size_t)nxt(j, ny) * nx + i];
+    double cu = 0.5 * (uc + ur), cv = 0.5 * (vc + vt);
+    ke = cu * cu + cv * cv;
+    double dv = (ur - uc) * idx + (vt - vc) * idy;
+    md = fabs(dv);
+    d2 = dv * dv;
+    su = uc;
+    sv = vc;
+    mu = fabs(uc);
+    mv = fabs(vc);
+    nf = (isfinite(uc) ? 0 : 1) + (isfinite(vc) ? 0 : 1);
+    if (md != md) md = __longlong_as_double(0x7ff0000000000000LL);
+  }
+  __shared__ double sm[8][4];
+  double vals[8] = {ke, md, d2, su, sv, mu, mv, nf};
+  int lane = threadIdx.x + threadIdx.y * BX;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    bool isMax = (q == 1 || q == 5 || q == 6);
+    double x = isMax ? wave_max(vals[q]) : wave_sum(vals[q]);
+    if ((lane & 63) == 0) sm[q][lane >> 6] = x;
+  }
+  __syncthreads();
+  if (lane < 8) {
+    int q = lane;
+    bool isMax = (q == 1 || q == 5 || q == 6);
+    double x = sm[q][0];
+    for (int w = 1; w < 4; w++) x = isMax ? fmax(x, sm[q][w]) : x + sm[q][w];
+    part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * DIAG_W + q] = x;
+  }
+}
+
+// Deterministic fold of the per-block partials (one workgroup, fixed order).
+__global__ void __launch_bounds__(256)
+k_diag_final(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  __shared__ double sm[8][256];
+  int t = threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = t; b < nblocks; b += 256)
+    for (int q = 0; q < 8; q++) {
+      double x = part[(size_t)b * DIAG_W + q];
+      bool isMax = (q == 1 || q == 5 || q == 6);
+      acc[q] = isMax ? fmax(acc[q], x) : acc[q] + x;
+    }
+  for (int q = 0; q < 8; q++) sm[q][t] = acc[q];
+  __syncthreads();
+  if (t < 8) {
+    int q = t;
+    bool isMax = (q == 1 || q == 5 || q == 6);
+    double x = sm[q][0];
+    for (int k = 1; k < 256; k++) x = isMax ? fmax(x, sm[q][k]) : x + sm[q][k];
+    out[q] = x;
+  }
+}
+
+}  // namespace fsk

@@ -1,0 +1,162 @@
+"""ctypes binding of libfluidsim.so (the C ABI declared in include/fluidsim.h).
+
+No fallback exists: if the HIP library is missing, importing the solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build as _build
+
+FS_OK = 0
+FS_E_INVALID_ARG = -1
+FS_E_DEVICE = -2
+FS_E_OOM = -3
+FS_E_NONFINITE = -4
+FS_E_NOT_CONVERGED = -5
+FS_E_UNSUPPORTED = -6
+
+ADVECTION = {"semi-lagrangian": 0, "maccormack": 1}
+PRESSURE = {"jacobi": 0, "red-black": 1, "multigrid": 2}
+PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+GUESS = {"zero": 0, "previous": 1, "learned": 2}
+FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
+KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5}
+
+
+class FsConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double),
+        ("nu", C.c_double),
+        ("advection", C.c_int),
+        ("pressure_solver", C.c_int),
+        ("pressure_iterations", C.c_int),
+        ("tolerance", C.c_double),
+        ("mg_min_size", C.c_int),
+        ("mg_pre", C.c_int), ("mg_post", C.c_int),
+        ("mg_coarse", C.c_int),
+        ("mg_precision", C.c_int),
+        ("guess", C.c_int),
+        ("solid_mask_width", C.c_double),
+        ("device", C.c_int),
+    ]
+
+
+class FsStats(C.Structure):
+    _fields_ = [
+        ("t", C.c_double),
+        ("steps", C.c_int64),
+        ("last_vcycles", C.c_int),
+        ("total_vcycles", C.c_int64),
+        ("last_residual", C.c_double),
+        ("last_dt", C.c_double),
+    ]
+
+
+class FsDiag(C.Structure):
+    _fields_ = [
+        ("kinetic_energy", C.c_double),
+        ("max_divergence", C.c_double),
+        ("l2_divergence", C.c_double),
+        ("momentum_x", C.c_double), ("momentum_y", C.c_double),
+        ("max_abs_u", C.c_double), ("max_abs_v", C.c_double),
+        ("nonfinite", C.c_int),
+    ]
+
+
+class FsForcing(C.Structure):
+    _fields_ = [
+        ("kf", C.c_double), ("width", C.c_double), ("amp", C.c_double),
+        ("drag", C.c_double),
+        ("seed", C.c_uint32),
+    ]
+
+
+class FluidSimError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (status {code})")
+        self.code = code
+
+
+class NotConverged(FluidSimError):
+    pass
+
+
+_VP = C.c_void_p
+_SIGNATURES = {
+    "fs_abi_version": (C.c_int, []),
+    "fs_last_error": (C.c_char_p, []),
+    "fs_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "fs_device_name": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
+    "fs_config_default": (None, [C.POINTER(FsConfig)]),
+    "fs_solver_create": (C.c_int, [C.POINTER(FsConfig), C.POINTER(_VP)]),
+    "fs_solver_destroy": (C.c_int, [_VP]),
+    "fs_upload_velocity_f64": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+    "fs_upload_velocity_f32": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+    "fs_read_velocity": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+    "fs_read_field": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
+    "fs_step": (C.c_int, [_VP, C.c_double]),
+    "fs_step_cfl": (C.c_int, [_VP, C.c_double]),
+    "fs_sync": (C.c_int, [_VP]),
+    "fs_max_divergence": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "fs_poisson_residual_after": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
+    "fs_set_pressure_iterations": (C.c_int, [_VP, C.c_int]),
+    "fs_set_tolerance": (C.c_int, [_VP, C.c_double]),
+    "fs_set_nu": (C.c_int, [_VP, C.c_double]),
+    "fs_set_guess": (C.c_int, [_VP, C.c_int]),
+    "fs_set_forcing": (C.c_int, [_VP, C.POINTER(FsForcing)]),
+    "fs_get_stats": (C.c_int, [_VP, C.POINTER(FsStats)]),
+    "fs_diagnostics": (C.c_int, [_VP, C.POINTER(FsDiag)]),
+    "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),
+    "fs_profile_enable": (C.c_int, [_VP, C.c_int]),
+    "fs_profile_read": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def load():
+    """Load the in-tree libfluidsim.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libfluidsim.so not found at {path}; build it with "
+            "`python fluid-sim_amd/fluidsim/build.py` (hipcc, gfx950)")
+    L = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = load().fs_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(code: int):
+    if code == FS_OK:
+        return
+    msg = last_error()
+    if code == FS_E_NOT_CONVERGED:
+        raise NotConverged(code, msg)
+    if code == FS_E_INVALID_ARG:
+        raise ValueError(f"{msg} (status {code})")
+    raise FluidSimError(code, msg)
+
+
+def default_config() -> FsConfig:
+    cfg = FsConfig()
+    load().fs_config_default(C.byref(cfg))
+    return cfg

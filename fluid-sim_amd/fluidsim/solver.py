@@ -1,0 +1,252 @@
+"""Python mirror of the reference's GPU solver API (src/gpu/solver.js) over libfluidsim.so.
+
+`createDevice()` and `GPUFluidSolver` keep the reference's names, options,
+defaults and error behaviour, so callers and parity tests read like the
+reference's own harness (tools/gputest.html, tools/bench.html). The differences
+are deliberate and listed on each method: unknown selectors raise instead of
+silently running Jacobi, readbacks return numpy arrays instead of Promises, and
+the multigrid projection can iterate to an FP64 residual tolerance.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import FsConfig, FsDiag, FsForcing, FsStats, check, load
+
+TWO_PI = 2.0 * math.pi
+
+
+def createDevice(requireTimestamps: bool = False, device: int = 0):
+    """createDevice() (src/gpu/solver.js:43-62): the HIP device and its name.
+
+    Timestamps are always available (HIP events), so `requireTimestamps` never fails.
+    """
+    L = load()
+    n = C.c_int()
+    check(L.fs_device_count(C.byref(n)))
+    if n.value == 0:
+        raise RuntimeError("no HIP device available")
+    buf = C.create_string_buffer(256)
+    check(L.fs_device_name(device, buf, 256))
+    return {"device": device, "hasTimestamps": True,
+            "info": {"vendor": "amd", "description": buf.value.decode()}}
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class GPUFluidSolver:
+    """new GPUFluidSolver(device, opts) (src/gpu/solver.js:64-98).
+
+    Options and defaults follow the reference: n=256, lx=ly=2pi, nu=0,
+    advection="maccormack", pressureSolver="jacobi", pressureIterations=40,
+    multigrid V(2,2) with 40 coarse sweeps and minSize 8. Extra options:
+    tolerance (FP64 residual gate for multigrid; 0 = fixed V-cycle count),
+    mgMinSize, mgPrecision, guess ("zero" | "previous").
+    """
+
+    def __init__(self, device=0, *, n=256, nx=None, ny=None, lx=TWO_PI, ly=TWO_PI, nu=0.0,
+                 confinement=0, dyeFade=0, advection="maccormack", solidMaskWidth=1.5,
+                 pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
+                 tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
+                 mgPrecision="fp32", guess="zero"):
+        if isinstance(device, dict):
+            device = device.get("device", 0)
+        nx = n if nx is None else nx
+        ny = n if ny is None else ny
+        if advection not in _lib.ADVECTION:
+            raise ValueError(f"unknown advection scheme {advection}; have {', '.join(_lib.ADVECTION)}")
+        if pressureSolver not in _lib.PRESSURE:
+            raise ValueError(f"unknown pressure solver {pressureSolver}; have {', '.join(_lib.PRESSURE)}")
+        if mgPrecision not in _lib.PRECISION:
+            raise ValueError(f"unknown multigrid precision {mgPrecision}")
+        if guess not in _lib.GUESS:
+            raise ValueError(f"unknown guess mode {guess}")
+        if confinement:
+            raise ValueError("vorticity confinement is a display-only force and is not built here")
+        cfg = _lib.default_config()
+        cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.nu = nx, ny, lx, ly, nu
+        cfg.advection = _lib.ADVECTION[advection]
+        cfg.pressure_solver = _lib.PRESSURE[pressureSolver]
+        cfg.pressure_iterations = int(pressureIterations)
+        cfg.tolerance = float(tolerance)
+        cfg.mg_min_size, cfg.mg_pre, cfg.mg_post, cfg.mg_coarse = mgMinSize, mgPre, mgPost, mgCoarse
+        cfg.mg_precision = _lib.PRECISION[mgPrecision]
+        cfg.guess = _lib.GUESS[guess]
+        cfg.solid_mask_width = solidMaskWidth
+        cfg.device = int(device)
+        self._L = load()
+        h = C.c_void_p()
+        check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.nx, self.ny = nx, ny
+        self.dx, self.dy = lx / nx, ly / ny
+        self.lx, self.ly = lx, ly
+        self._nu = nu
+        self.advection = advection
+        self.pressureSolver = pressureSolver
+        self._pressureIterations = int(pressureIterations)
+        self.hasTimestamps = True
+        self.solid = None
+
+    # ---------------------------------------------------------------- state
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("solver has been destroyed")
+        return self._h
+
+    @property
+    def nu(self):
+        return self._nu
+
+    @nu.setter
+    def nu(self, value):
+        check(self._L.fs_set_nu(self.handle, float(value)))
+        self._nu = float(value)
+
+    @property
+    def pressureIterations(self):
+        return self._pressureIterations
+
+    @pressureIterations.setter
+    def pressureIterations(self, value):  # mutable, as index.html:472 uses it
+        check(self._L.fs_set_pressure_iterations(self.handle, int(value)))
+        self._pressureIterations = int(value)
+
+    def setTolerance(self, tol: float):
+        check(self._L.fs_set_tolerance(self.handle, float(tol)))
+
+    def setGuess(self, guess: str):
+        if guess not in _lib.GUESS:
+            raise ValueError(f"unknown guess mode {guess}")
+        check(self._L.fs_set_guess(self.handle, _lib.GUESS[guess]))
+
+    def stats(self) -> dict:
+        st = FsStats()
+        check(self._L.fs_get_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in FsStats._fields_}
+
+    @property
+    def t(self):
+        return self.stats()["t"]
+
+    @property
+    def steps(self):
+        return self.stats()["steps"]
+
+    # ------------------------------------------------------------ data in/out
+    def uploadVelocity(self, u, v):
+        """uploadVelocity(u, v) (src/gpu/solver.js:301-315): float64 or float32, nx*ny MAC order."""
+        u = np.ascontiguousarray(u).reshape(-1)
+        v = np.ascontiguousarray(v).reshape(-1)
+        n = self.nx * self.ny
+        if u.size != n or v.size != n:
+            raise ValueError(f"expected {n} values per component, got {u.size}, {v.size}")
+        if u.dtype == np.float32 and v.dtype == np.float32:
+            check(self._L.fs_upload_velocity_f32(self.handle, _ptr(u), _ptr(v), n))
+        else:
+            u = np.ascontiguousarray(u, np.float64)
+            v = np.ascontiguousarray(v, np.float64)
+            check(self._L.fs_upload_velocity_f64(self.handle, _ptr(u), _ptr(v), n))
+
+    def readVelocity(self):
+        """readVelocity() (src/gpu/solver.js:356-362) -> {"u": float32[nx*ny], "v": ...}."""
+        n = self.nx * self.ny
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        check(self._L.fs_read_velocity(self.handle, _ptr(u), _ptr(v), n))
+        return {"u": u, "v": v}
+
+    def readField(self, name: str):
+        """readTexture(tex) (src/gpu/solver.js:336-353) by field name."""
+        if name not in _lib.FIELD:
+            raise ValueError(f"unknown field {name}; have {', '.join(_lib.FIELD)}")
+        dtype = np.float64 if name == "pressure" else np.float32
+        out = np.empty(self.nx * self.ny, dtype)
+        check(self._L.fs_read_field(self.handle, _lib.FIELD[name], _ptr(out), out.nbytes))
+        return out
+
+    # ------------------------------------------------------------------ step
+    def step(self, dt, splats=()):
+        """step(dt) (src/gpu/solver.js:518-591). Raises NotConverged if a
+        tolerance was set and not met within pressureIterations V-cycles."""
+        if splats:
+            raise ValueError("splats are a display-only interaction and are not built here")
+        check(self._L.fs_step(self.handle, float(dt)))
+
+    def stepCFL(self, cfl=0.5):
+        """One step with dt = cfl / (max|u|/dx + max|v|/dy), computed on the device."""
+        check(self._L.fs_step_cfl(self.handle, float(cfl)))
+        return self.stats()["last_dt"]
+
+    def sync(self):
+        """device.queue.onSubmittedWorkDone()."""
+        check(self._L.fs_sync(self.handle))
+
+    def setForcing(self, forcing):
+        """Band forcing with the CPU's seeded semantics (src/cpu/solver.js:179-237):
+        dict(kf, width, amp, drag, seed) or None to disable; step k uses seed + k."""
+        if forcing is None:
+            check(self._L.fs_set_forcing(self.handle, None))
+            return
+        f = FsForcing(float(forcing.get("kf", 28)), float(forcing.get("width", 2)),
+                      float(forcing.get("amp", 0.4)), float(forcing.get("drag", 0.0)),
+                      int(forcing.get("seed", 1)) & 0xFFFFFFFF)
+        check(self._L.fs_set_forcing(self.handle, C.byref(f)))
+
+    # ------------------------------------------------------------ diagnostics
+    def maxDivergence(self):
+        """maxDivergence() (src/gpu/solver.js:835-850), reduced on the device in FP64."""
+        out = C.c_double()
+        check(self._L.fs_max_divergence(self.handle, C.byref(out)))
+        return out.value
+
+    def poissonResidualAfter(self, iterations, dt=0.01):
+        """poissonResidualAfter(it, dt) (src/gpu/solver.js:857-896): ||L p - div|| / ||div||."""
+        out = C.c_double()
+        check(self._L.fs_poisson_residual_after(self.handle, int(iterations), C.byref(out)))
+        return out.value
+
+    def diagnostics(self) -> dict:
+        d = FsDiag()
+        check(self._L.fs_diagnostics(self.handle, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in FsDiag._fields_}
+
+    def runKernel(self, name: str, dt: float = 0.0):
+        if name not in _lib.KERNEL:
+            raise ValueError(f"unknown kernel {name}")
+        check(self._L.fs_run_kernel(self.handle, _lib.KERNEL[name], float(dt)))
+
+    def profileEnable(self, kind: int):
+        check(self._L.fs_profile_enable(self.handle, int(kind)))
+
+    def profileRead(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(self._L.fs_profile_read(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # -------------------------------------------------------------- lifetime
+    def destroy(self):
+        """destroy() (src/gpu/solver.js:898-911); idempotent."""
+        if getattr(self, "_h", None) is not None:
+            self._L.fs_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()

@@ -1,0 +1,177 @@
+/*
+ * fluidsim.h -- C ABI of the MI355X stable-fluids solver (libfluidsim.so).
+ *
+ * Plain C: no torch or HIP types cross this boundary. Every entry point returns
+ * an int status (FS_OK = 0, negative on failure) and sets a thread-local message
+ * readable with fs_last_error(). A handle is not thread-safe: callers serialise
+ * calls per handle (the N-API shim queues them), handles on different threads
+ * are independent.
+ *
+ * Each declaration names the reference interface it replaces (abho7/fluid-sim,
+ * src/gpu/solver.js unless stated otherwise).
+ */
+#ifndef FLUIDSIM_H
+#define FLUIDSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_ABI_VERSION 1
+
+/* status codes */
+enum {
+  FS_OK = 0,
+  FS_E_INVALID_ARG = -1,   /* bad size, selector or pointer */
+  FS_E_DEVICE = -2,        /* a HIP call failed (message names it) */
+  FS_E_OOM = -3,           /* device allocation failed */
+  FS_E_NONFINITE = -4,     /* NaN/Inf found by an FP64 reduction */
+  FS_E_NOT_CONVERGED = -5, /* tolerance not reached within the V-cycle cap */
+  FS_E_UNSUPPORTED = -6    /* valid request this build does not implement */
+};
+
+/* advection selector: GPUFluidSolver({advection}) src/gpu/solver.js:64-72, :604-647 */
+enum { FS_ADV_SEMI_LAGRANGIAN = 0, FS_ADV_MACCORMACK = 1 };
+
+/* pressure selector: GPUFluidSolver({pressureSolver}) src/gpu/solver.js:686-738.
+ * Unknown selectors are rejected (the reference silently falls back to Jacobi,
+ * src/gpu/solver.js:725; the CPU solver throws, src/cpu/solver.js:288-290). */
+enum { FS_P_JACOBI = 0, FS_P_RED_BLACK = 1, FS_P_MULTIGRID = 2 };
+
+/* multigrid smoother storage precision (accumulation is always FP32) */
+enum { FS_MG_FP32 = 0, FS_MG_FP16 = 1, FS_MG_BF16 = 2 };
+
+/* initial pressure guess for the multigrid projection */
+enum { FS_GUESS_ZERO = 0, FS_GUESS_PREVIOUS = 1, FS_GUESS_LEARNED = 2 };
+
+/* readable fields for fs_read_field */
+enum {
+  FS_F_U = 0,        /* float[nx*ny], current u (left faces) */
+  FS_F_V = 1,        /* float[nx*ny], current v (bottom faces) */
+  FS_F_DIV = 2,      /* float[nx*ny], divergence written by the last divergence pass */
+  FS_F_PRESSURE = 3, /* double[nx*ny], pressure of the last projection */
+  FS_F_RESIDUAL = 4  /* float[nx*ny], level-0 multigrid right-hand side of the last cycle */
+};
+
+/* single kernels for per-kernel parity tests (tools/gputest.html:90-100, :121-129) */
+enum {
+  FS_K_ADVECT_SL = 0,   /* velocity <- semi-Lagrangian(velocity), dt */
+  FS_K_ADVECT_MC = 1,   /* velocity <- limited MacCormack(velocity), dt */
+  FS_K_DIVERGENCE = 2,  /* div <- divergence(velocity) */
+  FS_K_DIFFUSE = 3,     /* velocity <- explicit diffusion(velocity), dt, nu */
+  FS_K_PROJECT = 4,     /* one projection with the configured solver */
+  FS_K_GRADIENT = 5     /* velocity <- velocity - grad(pressure) */
+};
+
+/* GPUFluidSolver constructor options, src/gpu/solver.js:64-98 */
+typedef struct fs_config {
+  int nx, ny;                 /* cells; reference default n = 256 */
+  double lx, ly;              /* domain size; default 2*pi */
+  double nu;                  /* kinematic viscosity; default 0 */
+  int advection;              /* FS_ADV_*; default MacCormack */
+  int pressure_solver;        /* FS_P_*; default Jacobi */
+  int pressure_iterations;    /* sweeps (Jacobi, red-black) or V-cycles; default 40.
+                                 With tolerance > 0 it caps the V-cycles per step. */
+  double tolerance;           /* > 0 (multigrid only): iterate until the FP64 residual
+                                 max|div u* - L p| <= tolerance; 0 = fixed count */
+  int mg_min_size;            /* stop coarsening below this size; default 8 */
+  int mg_pre, mg_post;        /* red-black sweeps per level; default 2, 2 */
+  int mg_coarse;              /* sweeps on the coarsest level; default 40 */
+  int mg_precision;           /* FS_MG_*; default FP32 */
+  int guess;                  /* FS_GUESS_*; default zero (the reference zeroes p) */
+  double solid_mask_width;    /* in cells; default 1.5 */
+  int device;                 /* HIP device ordinal; default 0 */
+} fs_config;
+
+/* per-step statistics (new: the reference reports none) */
+typedef struct fs_stats {
+  double t;                   /* simulated time */
+  int64_t steps;              /* steps taken */
+  int last_vcycles;           /* V-cycles in the last projection */
+  int64_t total_vcycles;      /* V-cycles since creation */
+  double last_residual;       /* FP64 max|div u* - L p| at the end of the last projection */
+  double last_dt;             /* dt of the last step */
+} fs_stats;
+
+/* FP64 diagnostics of the current velocity (src/core/grid.js:290-319) */
+typedef struct fs_diag {
+  double kinetic_energy;      /* 1/2 <u_c^2 + v_c^2>, centre-interpolated (grid.js:290-295) */
+  double max_divergence;      /* max |div u| of the stored FP32 field, in FP64 */
+  double l2_divergence;       /* sqrt(<div^2>) */
+  double momentum_x, momentum_y; /* sum u dA, sum v dA (src/cpu/solid.js:264-270) */
+  double max_abs_u, max_abs_v;
+  int nonfinite;              /* count of non-finite faces */
+} fs_diag;
+
+/* band forcing (src/cpu/solver.js:179-237 semantics: force, then drag) */
+typedef struct fs_forcing {
+  double kf, width, amp;      /* shell centre, half width, amplitude */
+  double drag;                /* linear drag alpha; u *= exp(-alpha dt) after the force */
+  uint32_t seed;              /* seed of step k is seed + k */
+} fs_forcing;
+
+int         fs_abi_version(void);
+const char* fs_last_error(void);
+
+/* createDevice() src/gpu/solver.js:43-62: device count and name */
+int fs_device_count(int* count);
+int fs_device_name(int device, char* buf, size_t len);
+
+/* reference defaults, src/gpu/solver.js:64-98 and src/gpu/multigrid.js:159-161 */
+void fs_config_default(fs_config* cfg);
+
+/* new GPUFluidSolver(device, opts) src/gpu/solver.js:64-98 */
+int fs_solver_create(const fs_config* cfg, void** solver);
+/* destroy() src/gpu/solver.js:898-911; NULL is accepted */
+int fs_solver_destroy(void* solver);
+
+/* uploadVelocity(u, v) src/gpu/solver.js:301-315; n = nx*ny values each */
+int fs_upload_velocity_f64(void* solver, const double* u, const double* v, size_t n);
+int fs_upload_velocity_f32(void* solver, const float* u, const float* v, size_t n);
+/* readVelocity() src/gpu/solver.js:356-362 */
+int fs_read_velocity(void* solver, float* u, float* v, size_t n);
+/* readTexture(tex) src/gpu/solver.js:336-353, by field id; bytes must match the field */
+int fs_read_field(void* solver, int field, void* out, size_t bytes);
+
+/* step(dt) src/gpu/solver.js:518-591: enqueue one step on the solver's stream */
+int fs_step(void* solver, double dt);
+/* step with dt = cfl / (max|u|/dx + max|v|/dy), the CFL definition of
+ * src/core/grid.js:322-327, evaluated on the device from the last projection */
+int fs_step_cfl(void* solver, double cfl);
+/* device.queue.onSubmittedWorkDone() */
+int fs_sync(void* solver);
+
+/* maxDivergence() src/gpu/solver.js:835-850 (FP64 reduction on the device) */
+int fs_max_divergence(void* solver, double* out);
+/* poissonResidualAfter(it, dt) src/gpu/solver.js:857-896: ||L p - div|| / ||div|| */
+int fs_poisson_residual_after(void* solver, int iterations, double* out);
+
+/* mutable fields (index.html:472 mutates pressureIterations) */
+int fs_set_pressure_iterations(void* solver, int iterations);
+int fs_set_tolerance(void* solver, double tolerance);
+int fs_set_nu(void* solver, double nu);
+int fs_set_guess(void* solver, int guess);
+
+/* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's
+ * seeded band-forcing semantics; NULL disables */
+int fs_set_forcing(void* solver, const fs_forcing* forcing);
+
+int fs_get_stats(void* solver, fs_stats* out);
+int fs_diagnostics(void* solver, fs_diag* out);
+
+/* per-kernel hook replacing gputest's use of internals */
+int fs_run_kernel(void* solver, int kernel, double dt);
+
+/* measurement: HIP-event time of one kernel class on the solver's stream.
+ * kind: 0 = advection, 1 = level-0 smoother, 2 = residual, 3 = gradient.
+ * Enabled timing adds an event pair around every launch of that class. */
+int fs_profile_enable(void* solver, int kind);
+int fs_profile_read(void* solver, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,863 @@
+/*
+ * fsoracle.c -- f64 CPU oracle for the stable-fluids step (TEST INFRASTRUCTURE).
+ *
+ * This is the checker, not the product: the HIP library in fluid-sim_amd/ never
+ * links or loads it. Each function names the reference file:line whose
+ * algorithm it restates; the restatement is written from the mathematics the
+ * reference documents, in C, with the same operation order where that order
+ * decides the result (sampling offsets, limiter bounds, colour order).
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   LCG, divergence       reference-held max|div| of the seed-4242 field
+ *   Jacobi / red-black /  reference-held residual curves of the seed-777 field
+ *   multigrid             (results/gpu-verify.json poissonCurves)
+ *   SL / MacCormack       reference-held energies after 200 TG steps
+ *                         (results/gpu-verify.json schemeEnergy), integer-shift
+ *                         exactness, translating-Gaussian order
+ *   FFT / Poisson         naive DFT, single-mode known answers, Parseval
+ *   diffusion             closed-form checkerboard and sine-mode factors
+ *   forcing               exact integer LCG, curl structure
+ */
+#include "fsoracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int or_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* test/core.test.js:94-98 -- the product s*1103515245 exceeds 2^53, so it is
+ * rounded to double before ToInt32 takes it mod 2^32; fmod on doubles is exact. */
+void or_js_lcg_fill(int64_t *state, int n, double *out) {
+  int64_t s = *state;
+  for (int k = 0; k < n; k++) {
+    double prod = (double)s * 1103515245.0;
+    double sum = prod + 12345.0;
+    double m = fmod(sum, 4294967296.0);
+    if (m < 0) m += 4294967296.0;
+    uint32_t u32 = (uint32_t)m;
+    s = (int64_t)(u32 & 0x7fffffffu);
+    out[k] = (double)s / 2147483647.0 - 0.5;
+  }
+  *state = s;
+}
+
+/* src/core/grid.js:40-42 */
+int or_wrap(int i, int n) { return ((i % n) + n) % n; }
+
+/* src/core/grid.js:147-165 */
+double or_bilerp(const double *a, int nx, int ny, double gx, double gy) {
+  double fi = floor(gx), fj = floor(gy);
+  int i0 = (int)fi, j0 = (int)fj;
+  double fx = gx - fi, fy = gy - fj;
+  int i0w = or_wrap(i0, nx), i1w = or_wrap(i0 + 1, nx);
+  int j0w = or_wrap(j0, ny), j1w = or_wrap(j0 + 1, ny);
+  double A = a[j0w * nx + i0w], B = a[j0w * nx + i1w];
+  double C = a[j1w * nx + i0w], D = a[j1w * nx + i1w];
+  double top = A + (B - A) * fx;
+  double bot = C + (D - C) * fx;
+  return top + (bot - top) * fy;
+}
+
+/* src/core/grid.js:167-180 */
+double or_sample_u(int nx, int ny, double lx, double ly, const double *u, double x, double y) {
+  double dx = lx / nx, dy = ly / ny;
+  return or_bilerp(u, nx, ny, x / dx, y / dy - 0.5);
+}
+double or_sample_v(int nx, int ny, double lx, double ly, const double *v, double x, double y) {
+  double dx = lx / nx, dy = ly / ny;
+  return or_bilerp(v, nx, ny, x / dx - 0.5, y / dy);
+}
+double or_sample_p(int nx, int ny, double lx, double ly, const double *p, double x, double y) {
+  double dx = lx / nx, dy = ly / ny;
+  return or_bilerp(p, nx, ny, x / dx - 0.5, y / dy - 0.5);
+}
+
+/* src/core/grid.js:191-206 */
+void or_divergence(int nx, int ny, double lx, double ly, const double *u, const double *v, double *out) {
+  double dx = lx / nx, dy = ly / ny;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    int jp = (j + 1 == ny) ? 0 : j + 1;
+    for (int i = 0; i < nx; i++) {
+      int ip = (i + 1 == nx) ? 0 : i + 1;
+      double uL = u[j * nx + i], uR = u[j * nx + ip];
+      double vB = v[j * nx + i], vT = v[jp * nx + i];
+      out[j * nx + i] = (uR - uL) / dx + (vT - vB) / dy;
+    }
+  }
+}
+
+/* src/core/grid.js:209-218 */
+void or_subtract_gradient(int nx, int ny, double lx, double ly, double *u, double *v,
+                          const double *p, double scale) {
+  double dx = lx / nx, dy = ly / ny;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    int jm = (j == 0) ? ny - 1 : j - 1;
+    for (int i = 0; i < nx; i++) {
+      int im = (i == 0) ? nx - 1 : i - 1;
+      u[j * nx + i] -= scale * (p[j * nx + i] - p[j * nx + im]) / dx;
+      v[j * nx + i] -= scale * (p[j * nx + i] - p[jm * nx + i]) / dy;
+    }
+  }
+}
+
+/* src/cpu/projection.js:48-62 */
+void or_laplacian(int nx, int ny, double lx, double ly, const double *p, double *out) {
+  double dx = lx / nx, dy = ly / ny;
+  double ix = 1.0 / (dx * dx), iy = 1.0 / (dy * dy);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    int jm = or_wrap(j - 1, ny) * nx, jp = or_wrap(j + 1, ny) * nx, j0 = j * nx;
+    for (int i = 0; i < nx; i++) {
+      int im = or_wrap(i - 1, nx), ip = or_wrap(i + 1, nx);
+      out[j0 + i] = (p[j0 + im] - 2 * p[j0 + i] + p[j0 + ip]) * ix +
+                    (p[jm + i] - 2 * p[j0 + i] + p[jp + i]) * iy;
+    }
+  }
+}
+
+/* src/core/grid.js:229-240 */
+void or_vorticity_corner(int nx, int ny, double lx, double ly, const double *u, const double *v, double *out) {
+  double dx = lx / nx, dy = ly / ny;
+  for (int j = 0; j < ny; j++) {
+    int jm = or_wrap(j - 1, ny);
+    for (int i = 0; i < nx; i++) {
+      int im = or_wrap(i - 1, nx);
+      double dvdx = (v[j * nx + i] - v[j * nx + im]) / dx;
+      double dudy = (u[j * nx + i] - u[jm * nx + i]) / dy;
+      out[j * nx + i] = dvdx - dudy;
+    }
+  }
+}
+
+/* src/core/grid.js:267-279 */
+void or_velocity_at_centers(int nx, int ny, const double *u, const double *v, double *cu, double *cv) {
+  for (int j = 0; j < ny; j++) {
+    int jp = or_wrap(j + 1, ny);
+    for (int i = 0; i < nx; i++) {
+      int ip = or_wrap(i + 1, nx);
+      cu[j * nx + i] = 0.5 * (u[j * nx + i] + u[j * nx + ip]);
+      cv[j * nx + i] = 0.5 * (v[j * nx + i] + v[jp * nx + i]);
+    }
+  }
+}
+
+/* src/core/grid.js:290-295 (centre-interpolated) */
+double or_kinetic_energy(int nx, int ny, const double *u, const double *v) {
+  double s = 0;
+  for (int j = 0; j < ny; j++) {
+    int jp = or_wrap(j + 1, ny);
+    for (int i = 0; i < nx; i++) {
+      int ip = or_wrap(i + 1, nx);
+      double cu = 0.5 * (u[j * nx + i] + u[j * nx + ip]);
+      double cv = 0.5 * (v[j * nx + i] + v[jp * nx + i]);
+      s += cu * cu + cv * cv;
+    }
+  }
+  return 0.5 * s / ((double)nx * ny);
+}
+
+/* tools/gputest.html:313-316 -- raw face values, a different definition */
+double or_kinetic_energy_faces(int n, const double *u, const double *v) {
+  double s = 0;
+  for (int k = 0; k < n; k++) s += u[k] * u[k] + v[k] * v[k];
+  return 0.5 * s / n;
+}
+
+/* src/core/grid.js:298-303 with vorticityCenter :245-262 */
+double or_enstrophy(int nx, int ny, double lx, double ly, const double *u, const double *v) {
+  double *w = (double *)malloc(sizeof(double) * nx * ny);
+  or_vorticity_corner(nx, ny, lx, ly, u, v, w);
+  double s = 0;
+  for (int j = 0; j < ny; j++) {
+    int jp = or_wrap(j + 1, ny);
+    for (int i = 0; i < nx; i++) {
+      int ip = or_wrap(i + 1, nx);
+      double c = 0.25 * (w[j * nx + i] + w[j * nx + ip] + w[jp * nx + i] + w[jp * nx + ip]);
+      s += c * c;
+    }
+  }
+  free(w);
+  return 0.5 * s / ((double)nx * ny);
+}
+
+/* src/core/grid.js:306-311 */
+double or_max_divergence(int nx, int ny, double lx, double ly, const double *u, const double *v) {
+  double *d = (double *)malloc(sizeof(double) * nx * ny);
+  or_divergence(nx, ny, lx, ly, u, v, d);
+  double m = 0;
+  for (int k = 0; k < nx * ny; k++) m = fmax(m, fabs(d[k]));
+  free(d);
+  return m;
+}
+
+/* src/core/grid.js:322-327 */
+double or_cfl(int nx, int ny, double lx, double ly, const double *u, const double *v, double dt) {
+  double mu = 0, mv = 0;
+  for (int k = 0; k < nx * ny; k++) { mu = fmax(mu, fabs(u[k])); mv = fmax(mv, fabs(v[k])); }
+  return dt * (mu / (lx / nx) + mv / (ly / ny));
+}
+
+/* src/cpu/projection.js:251-261 */
+double or_poisson_residual_rel(int nx, int ny, double lx, double ly, const double *p, const double *rhs) {
+  double *Ap = (double *)malloc(sizeof(double) * nx * ny);
+  or_laplacian(nx, ny, lx, ly, p, Ap);
+  double num = 0, den = 0;
+  for (int k = 0; k < nx * ny; k++) {
+    double d = Ap[k] - rhs[k];
+    num += d * d;
+    den += rhs[k] * rhs[k];
+  }
+  free(Ap);
+  return sqrt(num) / (den > 0 ? sqrt(den) : 1.0);
+}
+
+/* ------------------------------------------------------------------ advection */
+
+typedef struct { int nx, ny; double dx, dy; } G;
+
+static inline double su(const G *g, const double *u, double x, double y) {
+  return or_bilerp(u, g->nx, g->ny, x / g->dx, y / g->dy - 0.5);
+}
+static inline double sv(const G *g, const double *v, double x, double y) {
+  return or_bilerp(v, g->nx, g->ny, x / g->dx - 0.5, y / g->dy);
+}
+
+/* src/cpu/advect.js:46-54 -- RK2 midpoint backward trace */
+static inline void trace_back(const G *g, const double *u, const double *v, double x, double y,
+                              double dt, double *ox, double *oy) {
+  double u1 = su(g, u, x, y), v1 = sv(g, v, x, y);
+  double xm = x - 0.5 * dt * u1, ym = y - 0.5 * dt * v1;
+  double u2 = su(g, u, xm, ym), v2 = sv(g, v, xm, ym);
+  *ox = x - dt * u2;
+  *oy = y - dt * v2;
+}
+
+/* src/cpu/advect.js:74-97 */
+void or_advect_velocity_sl(int nx, int ny, double lx, double ly, const double *u, const double *v,
+                           double *ou, double *ov, double dt) {
+  G g = {nx, ny, lx / nx, ly / ny};
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      double x, y;
+      trace_back(&g, u, v, i * g.dx, (j + 0.5) * g.dy, dt, &x, &y);
+      ou[j * nx + i] = su(&g, u, x, y);
+      trace_back(&g, u, v, (i + 0.5) * g.dx, j * g.dy, dt, &x, &y);
+      ov[j * nx + i] = sv(&g, v, x, y);
+    }
+  }
+}
+
+/* src/cpu/advect.js:113-121 */
+static inline void interp_bounds(const double *a, int nx, int ny, double gx, double gy,
+                                 double *lo, double *hi) {
+  int i0 = (int)floor(gx), j0 = (int)floor(gy);
+  int i0w = or_wrap(i0, nx), i1w = or_wrap(i0 + 1, nx);
+  int j0w = or_wrap(j0, ny), j1w = or_wrap(j0 + 1, ny);
+  double A = a[j0w * nx + i0w], B = a[j0w * nx + i1w];
+  double C = a[j1w * nx + i0w], D = a[j1w * nx + i1w];
+  *lo = fmin(fmin(A, B), fmin(C, D));
+  *hi = fmax(fmax(A, B), fmax(C, D));
+}
+
+static inline double clampd(double c, double lo, double hi) { return c < lo ? lo : (c > hi ? hi : c); }
+
+/* src/cpu/advect.js:149-185 with advectComponentWith :192-206 */
+void or_advect_velocity_mc(int nx, int ny, double lx, double ly, const double *u, const double *v,
+                           double *ou, double *ov, double dt) {
+  G g = {nx, ny, lx / nx, ly / ny};
+  size_t n = (size_t)nx * ny;
+  double *fu = (double *)malloc(sizeof(double) * n), *fv = (double *)malloc(sizeof(double) * n);
+  or_advect_velocity_sl(nx, ny, lx, ly, u, v, fu, fv, dt);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      int k = j * nx + i;
+      double x, y, lo, hi;
+      /* u face: backward pass of the forward result through the ORIGINAL flow */
+      double px = i * g.dx, py = (j + 0.5) * g.dy;
+      trace_back(&g, u, v, px, py, -dt, &x, &y);
+      double bu = or_bilerp(fu, nx, ny, x / g.dx, y / g.dy - 0.5);
+      double cu = fu[k] + 0.5 * (u[k] - bu);
+      trace_back(&g, u, v, px, py, dt, &x, &y);
+      interp_bounds(u, nx, ny, x / g.dx, y / g.dy - 0.5, &lo, &hi);
+      ou[k] = clampd(cu, lo, hi);
+      /* v face */
+      px = (i + 0.5) * g.dx; py = j * g.dy;
+      trace_back(&g, u, v, px, py, -dt, &x, &y);
+      double bv = or_bilerp(fv, nx, ny, x / g.dx - 0.5, y / g.dy);
+      double cv = fv[k] + 0.5 * (v[k] - bv);
+      trace_back(&g, u, v, px, py, dt, &x, &y);
+      interp_bounds(v, nx, ny, x / g.dx - 0.5, y / g.dy, &lo, &hi);
+      ov[k] = clampd(cv, lo, hi);
+    }
+  }
+  free(fu); free(fv);
+}
+
+/* src/cpu/advect.js:61-71 */
+void or_advect_scalar_sl(int nx, int ny, double lx, double ly, const double *u, const double *v,
+                         const double *q, double *out, double dt) {
+  G g = {nx, ny, lx / nx, ly / ny};
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      double x, y;
+      trace_back(&g, u, v, (i + 0.5) * g.dx, (j + 0.5) * g.dy, dt, &x, &y);
+      out[j * nx + i] = or_bilerp(q, nx, ny, x / g.dx - 0.5, y / g.dy - 0.5);
+    }
+  }
+}
+
+/* src/cpu/advect.js:124-146 */
+void or_advect_scalar_mc(int nx, int ny, double lx, double ly, const double *u, const double *v,
+                         const double *q, double *out, double dt) {
+  G g = {nx, ny, lx / nx, ly / ny};
+  size_t n = (size_t)nx * ny;
+  double *fwd = (double *)malloc(sizeof(double) * n), *back = (double *)malloc(sizeof(double) * n);
+  or_advect_scalar_sl(nx, ny, lx, ly, u, v, q, fwd, dt);
+  or_advect_scalar_sl(nx, ny, lx, ly, u, v, fwd, back, -dt);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      int k = j * nx + i;
+      double corrected = fwd[k] + 0.5 * (q[k] - back[k]);
+      double x, y, lo, hi;
+      trace_back(&g, u, v, (i + 0.5) * g.dx, (j + 0.5) * g.dy, dt, &x, &y);
+      interp_bounds(q, nx, ny, x / g.dx - 0.5, y / g.dy - 0.5, &lo, &hi);
+      out[k] = clampd(corrected, lo, hi);
+    }
+  }
+  free(fwd); free(back);
+}
+
+/* ------------------------------------------------------------------ diffusion */
+
+/* src/cpu/solver.js:56-69 */
+void or_diffuse_explicit(int nx, int ny, double lx, double ly, const double *f, double *out,
+                         double nu, double dt) {
+  double dx = lx / nx, dy = ly / ny;
+  if (nu <= 0) { memcpy(out, f, sizeof(double) * nx * ny); return; }
+  double alpha = nu * dt / (dx * dx), beta = nu * dt / (dy * dy);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    int jm = or_wrap(j - 1, ny) * nx, jp = or_wrap(j + 1, ny) * nx, j0 = j * nx;
+    for (int i = 0; i < nx; i++) {
+      int im = or_wrap(i - 1, nx), ip = or_wrap(i + 1, nx);
+      out[j0 + i] = f[j0 + i] + alpha * (f[j0 + im] - 2 * f[j0 + i] + f[j0 + ip]) +
+                    beta * (f[jm + i] - 2 * f[j0 + i] + f[jp + i]);
+    }
+  }
+}
+
+/* src/cpu/solver.js:71-111 -- CG on (I - nu dt Lap), warm-started from f */
+int or_diffuse_implicit(int nx, int ny, double lx, double ly, const double *f, double *out,
+                        double nu, double dt, double tol) {
+  size_t n = (size_t)nx * ny;
+  if (nu <= 0) { memcpy(out, f, sizeof(double) * n); return 0; }
+  double k = nu * dt;
+  double *r = (double *)malloc(sizeof(double) * n), *d = (double *)malloc(sizeof(double) * n);
+  double *Ap = (double *)malloc(sizeof(double) * n);
+  memcpy(out, f, sizeof(double) * n);
+  or_laplacian(nx, ny, lx, ly, out, Ap);
+  for (size_t i = 0; i < n; i++) { Ap[i] = out[i] - k * Ap[i]; r[i] = f[i] - Ap[i]; d[i] = r[i]; }
+  double rr = 0, bb = 0;
+  for (size_t i = 0; i < n; i++) { rr += r[i] * r[i]; bb += f[i] * f[i]; }
+  double bnorm = bb > 0 ? sqrt(bb) : 1.0;
+  int it = 0;
+  for (; it < 400; it++) {
+    if (sqrt(rr) / bnorm <= tol) break;
+    or_laplacian(nx, ny, lx, ly, d, Ap);
+    double dAd = 0;
+    for (size_t i = 0; i < n; i++) { Ap[i] = d[i] - k * Ap[i]; dAd += d[i] * Ap[i]; }
+    if (fabs(dAd) < 1e-300) break;
+    double alpha = rr / dAd, rrn = 0;
+    for (size_t i = 0; i < n; i++) { out[i] += alpha * d[i]; r[i] -= alpha * Ap[i]; rrn += r[i] * r[i]; }
+    double beta = rrn / rr;
+    for (size_t i = 0; i < n; i++) d[i] = r[i] + beta * d[i];
+    rr = rrn;
+  }
+  free(r); free(d); free(Ap);
+  return it;
+}
+
+/* ------------------------------------------------------------------------ FFT */
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* src/core/fft.js:33-69 -- radix-2, sign -1 forward, +1 inverse (unnormalised).
+ * Twiddles are evaluated directly rather than by recurrence. */
+int or_fft(int n, double *re, double *im, int sign) {
+  if (!is_pow2(n)) return -1;
+  for (int i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double t = re[i]; re[i] = re[j]; re[j] = t;
+      t = im[i]; im[i] = im[j]; im[j] = t;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len / 2;
+    for (int k = 0; k < half; k++) {
+      double ang = sign * 2.0 * M_PI * k / len;
+      double cr = cos(ang), ci = sin(ang);
+      for (int i = 0; i < n; i += len) {
+        double ar = re[i + k], ai = im[i + k];
+        double br = re[i + k + half], bi = im[i + k + half];
+        double dr = br * cr - bi * ci, di = br * ci + bi * cr;
+        re[i + k] = ar + dr; im[i + k] = ai + di;
+        re[i + k + half] = ar - dr; im[i + k + half] = ai - di;
+      }
+    }
+  }
+  return 0;
+}
+
+/* src/core/fft.js:112-160 -- rows then columns; inverse scaled by 1/(nx*ny) */
+int or_fft2(int nx, int ny, double *re, double *im, int sign) {
+  if (!is_pow2(nx) || !is_pow2(ny)) return -1;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) or_fft(nx, re + (size_t)j * nx, im + (size_t)j * nx, sign);
+#pragma omp parallel
+  {
+    double *cr = (double *)malloc(sizeof(double) * ny), *ci = (double *)malloc(sizeof(double) * ny);
+#pragma omp for schedule(static)
+    for (int i = 0; i < nx; i++) {
+      for (int j = 0; j < ny; j++) { cr[j] = re[(size_t)j * nx + i]; ci[j] = im[(size_t)j * nx + i]; }
+      or_fft(ny, cr, ci, sign);
+      for (int j = 0; j < ny; j++) { re[(size_t)j * nx + i] = cr[j]; im[(size_t)j * nx + i] = ci[j]; }
+    }
+    free(cr); free(ci);
+  }
+  if (sign > 0) {
+    double N = (double)nx * ny;
+    for (size_t k = 0; k < (size_t)nx * ny; k++) { re[k] /= N; im[k] /= N; }
+  }
+  return 0;
+}
+
+/* src/cpu/projection.js:208-236 -- divide by the discrete 5-point eigenvalue */
+int or_poisson_fft(int nx, int ny, double lx, double ly, const double *rhs, double *p) {
+  if (!is_pow2(nx) || !is_pow2(ny)) return -1;
+  double dx = lx / nx, dy = ly / ny;
+  size_t n = (size_t)nx * ny;
+  double *re = (double *)malloc(sizeof(double) * n), *im = (double *)calloc(n, sizeof(double));
+  memcpy(re, rhs, sizeof(double) * n);
+  or_fft2(nx, ny, re, im, -1);
+  for (int j = 0; j < ny; j++) {
+    double sy = sin(M_PI * j / ny);
+    double lyv = -4 * sy * sy / (dy * dy);
+    for (int i = 0; i < nx; i++) {
+      double sx = sin(M_PI * i / nx);
+      double lam = -4 * sx * sx / (dx * dx) + lyv;
+      size_t k = (size_t)j * nx + i;
+      if (i == 0 && j == 0) { re[k] = 0; im[k] = 0; }
+      else { re[k] /= lam; im[k] /= lam; }
+    }
+  }
+  or_fft2(nx, ny, re, im, +1);
+  memcpy(p, re, sizeof(double) * n);
+  free(re); free(im);
+  return 0;
+}
+
+static double dotv(const double *a, const double *b, size_t n) {
+  double s = 0;
+  for (size_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+static void remove_mean(double *a, size_t n) {
+  double s = 0;
+  for (size_t i = 0; i < n; i++) s += a[i];
+  double m = s / n;
+  for (size_t i = 0; i < n; i++) a[i] -= m;
+}
+
+/* src/cpu/projection.js:113-149 */
+int or_poisson_cg(int nx, int ny, double lx, double ly, const double *rhs, double *p,
+                  double tol, int maxit, double *residual_out) {
+  size_t n = (size_t)nx * ny;
+  double *work = (double *)malloc(sizeof(double) * n), *r = (double *)malloc(sizeof(double) * n);
+  double *d = (double *)malloc(sizeof(double) * n), *Ap = (double *)malloc(sizeof(double) * n);
+  memcpy(work, rhs, sizeof(double) * n);
+  remove_mean(work, n);
+  memcpy(r, work, sizeof(double) * n);
+  or_laplacian(nx, ny, lx, ly, p, Ap);
+  for (size_t i = 0; i < n; i++) r[i] -= Ap[i];
+  remove_mean(r, n);
+  memcpy(d, r, sizeof(double) * n);
+  double rr = dotv(r, r, n);
+  double rn = sqrt(dotv(work, work, n));
+  double rhs_norm = rn > 0 ? rn : 1.0;
+  double residual = sqrt(rr) / rhs_norm;
+  int it = 0;
+  if (residual > tol) {
+    for (; it < maxit; it++) {
+      or_laplacian(nx, ny, lx, ly, d, Ap);
+      remove_mean(Ap, n);
+      double dAd = dotv(d, Ap, n);
+      if (fabs(dAd) < 1e-300) break;
+      double alpha = rr / dAd;
+      for (size_t i = 0; i < n; i++) { p[i] += alpha * d[i]; r[i] -= alpha * Ap[i]; }
+      double rrn = dotv(r, r, n);
+      residual = sqrt(rrn) / rhs_norm;
+      if (residual <= tol) { it++; break; }
+      double beta = rrn / rr;
+      for (size_t i = 0; i < n; i++) d[i] = r[i] + beta * d[i];
+      rr = rrn;
+    }
+    remove_mean(p, n);
+  }
+  if (residual_out) *residual_out = residual;
+  free(work); free(r); free(d); free(Ap);
+  return it;
+}
+
+/* ------------------------------------------- GPU pressure-solver semantics */
+
+/* src/gpu/shaders.js:302-325 (JACOBI) with the ping-pong of src/gpu/solver.js:726-737:
+ * p' = p + omega*((l+r+b+t - dx^2*rhs)/4 - p); square cells assumed (h^2 = dx^2). */
+void or_jacobi(int nx, int ny, double lx, double ly, const double *rhs, double *p, int iters, double omega) {
+  double dx = lx / nx, h2 = dx * dx;
+  (void)ly;
+  size_t n = (size_t)nx * ny;
+  double *a = p, *b = (double *)malloc(sizeof(double) * n);
+  for (int it = 0; it < iters; it++) {
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; j++) {
+      int jm = or_wrap(j - 1, ny) * nx, jp = or_wrap(j + 1, ny) * nx, j0 = j * nx;
+      for (int i = 0; i < nx; i++) {
+        int im = or_wrap(i - 1, nx), ip = or_wrap(i + 1, nx);
+        double jac = (a[j0 + im] + a[j0 + ip] + a[jm + i] + a[jp + i] - h2 * rhs[j0 + i]) * 0.25;
+        b[j0 + i] = a[j0 + i] + omega * (jac - a[j0 + i]);
+      }
+    }
+    double *t = a; a = b; b = t;
+  }
+  if (a != p) { memcpy(p, a, sizeof(double) * n); b = a; }
+  free(b);
+}
+
+/* one colour pass, src/gpu/shaders.js:340-372: colour = (i+j)%2; the other
+ * colour is unchanged, so an in-place update equals the reference's copy-through. */
+static void rb_pass(int nx, int ny, double h2, const double *rhs, double *p, int colour, double omega) {
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    int jm = or_wrap(j - 1, ny) * nx, jp = or_wrap(j + 1, ny) * nx, j0 = j * nx;
+    for (int i = ((j + colour) & 1); i < nx; i += 2) {
+      int im = or_wrap(i - 1, nx), ip = or_wrap(i + 1, nx);
+      double gs = (p[j0 + im] + p[j0 + ip] + p[jm + i] + p[jp + i] - h2 * rhs[j0 + i]) * 0.25;
+      p[j0 + i] = p[j0 + i] + omega * (gs - p[j0 + i]);
+    }
+  }
+}
+
+/* src/gpu/solver.js:706-724 -- colour 0 then colour 1 per sweep */
+void or_redblack(int nx, int ny, double lx, double ly, const double *rhs, double *p, int sweeps, double omega) {
+  double dx = lx / nx;
+  (void)ly;
+  for (int s = 0; s < sweeps; s++) {
+    rb_pass(nx, ny, dx * dx, rhs, p, 0, omega);
+    rb_pass(nx, ny, dx * dx, rhs, p, 1, omega);
+  }
+}
+
+/* src/gpu/multigrid.js:182-197 -- halve while n >= min_size */
+int or_mg_levels(int n, int min_size) {
+  int L = 0;
+  while (n >= min_size) {
+    L++;
+    if (n / 2 < min_size || (n & 1)) break;
+    n /= 2;
+  }
+  return L;
+}
+
+typedef struct { int n; double dx; double *x, *rhs, *res; } MGL;
+
+/* src/gpu/multigrid.js:48-66 */
+static void mg_residual(const MGL *l) {
+  int n = l->n;
+  double idx2 = 1.0 / (l->dx * l->dx);
+  for (int j = 0; j < n; j++) {
+    int jm = or_wrap(j - 1, n) * n, jp = or_wrap(j + 1, n) * n, j0 = j * n;
+    for (int i = 0; i < n; i++) {
+      int im = or_wrap(i - 1, n), ip = or_wrap(i + 1, n);
+      double lap = (l->x[j0 + im] + l->x[j0 + ip] + l->x[jm + i] + l->x[jp + i] - 4.0 * l->x[j0 + i]) * idx2;
+      l->res[j0 + i] = l->rhs[j0 + i] - lap;
+    }
+  }
+}
+
+/* src/gpu/multigrid.js:72-93 -- 2x2 average */
+static void mg_restrict(const MGL *f, MGL *c) {
+  int n = c->n, fn = f->n;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      double s = f->res[(2 * j) * fn + 2 * i] + f->res[(2 * j) * fn + 2 * i + 1] +
+                 f->res[(2 * j + 1) * fn + 2 * i] + f->res[(2 * j + 1) * fn + 2 * i + 1];
+      c->rhs[j * n + i] = s * 0.25;
+    }
+}
+
+static inline double mixd(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+
+/* src/gpu/multigrid.js:99-131 -- bilinear cell-centred, added to the fine field */
+static void mg_prolong(const MGL *c, MGL *f) {
+  int n = f->n, cn = c->n;
+  for (int j = 0; j < n; j++) {
+    double gy = (j + 0.5) * 0.5 - 0.5;
+    int j0 = (int)floor(gy);
+    double fy = gy - floor(gy);
+    int y0 = or_wrap(j0, cn), y1 = or_wrap(j0 + 1, cn);
+    for (int i = 0; i < n; i++) {
+      double gx = (i + 0.5) * 0.5 - 0.5;
+      int i0 = (int)floor(gx);
+      double fx = gx - floor(gx);
+      int x0 = or_wrap(i0, cn), x1 = or_wrap(i0 + 1, cn);
+      double a = c->x[y0 * cn + x0], b = c->x[y0 * cn + x1];
+      double cc = c->x[y1 * cn + x0], d = c->x[y1 * cn + x1];
+      double corr = mixd(mixd(a, b, fx), mixd(cc, d, fx), fy);
+      f->x[j * n + i] += corr;
+    }
+  }
+}
+
+/* src/gpu/multigrid.js:256-270 */
+static void mg_smooth(MGL *l, int sweeps) {
+  double h2 = l->dx * l->dx;
+  for (int s = 0; s < sweeps; s++) {
+    rb_pass(l->n, l->n, h2, l->rhs, l->x, 0, 1.0);
+    rb_pass(l->n, l->n, h2, l->rhs, l->x, 1, 1.0);
+  }
+}
+
+/* src/gpu/multigrid.js:277-330 (V-cycle) and src/gpu/solver.js:686-703 (k cycles
+ * accumulating into the finest solution). p is the finest x on entry and exit;
+ * history[k] (if given) receives ||Lp - rhs|| / ||rhs|| after cycle k+1. */
+int or_mg_vcycles(int n, double lx, const double *rhs, double *p, int cycles,
+                  int min_size, int pre, int post, int coarse, double *history) {
+  int L = or_mg_levels(n, min_size);
+  if (L < 1) return -1;
+  MGL *lv = (MGL *)calloc(L, sizeof(MGL));
+  int m = n;
+  for (int i = 0; i < L; i++) {
+    size_t c = (size_t)m * m;
+    lv[i].n = m;
+    lv[i].dx = lx / m;
+    lv[i].x = i == 0 ? p : (double *)calloc(c, sizeof(double));
+    lv[i].rhs = i == 0 ? (double *)rhs : (double *)calloc(c, sizeof(double));
+    lv[i].res = (double *)calloc(c, sizeof(double));
+    m /= 2;
+  }
+  for (int k = 0; k < cycles; k++) {
+    for (int i = 0; i < L - 1; i++) {
+      mg_smooth(&lv[i], pre);
+      mg_residual(&lv[i]);
+      mg_restrict(&lv[i], &lv[i + 1]);
+      memset(lv[i + 1].x, 0, sizeof(double) * lv[i + 1].n * lv[i + 1].n);
+    }
+    mg_smooth(&lv[L - 1], coarse);
+    for (int i = L - 2; i >= 0; i--) {
+      mg_prolong(&lv[i + 1], &lv[i]);
+      mg_smooth(&lv[i], post);
+    }
+    if (history) history[k] = or_poisson_residual_rel(n, n, lx, lx, p, rhs);
+  }
+  for (int i = 0; i < L; i++) {
+    if (i > 0) { free(lv[i].x); free(lv[i].rhs); }
+    free(lv[i].res);
+  }
+  free(lv);
+  return L;
+}
+
+/* -------------------------------------------------------------------- forcing */
+
+static double js_round(double x) {
+  double r = floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
+}
+
+/* src/cpu/solver.js:179-199 -- 24 modes from the exact uint32 LCG */
+int or_band_modes(uint32_t seed, double kf, double width, int *kx, int *ky, double *phase) {
+  uint32_t s = seed;
+#define RND() (s = s * 1664525u + 1013904223u, (double)s / 4294967296.0)
+  for (int m = 0; m < 24; m++) {
+    double theta = 2 * M_PI * RND();
+    double kmag = kf + (RND() - 0.5) * 2 * width;
+    kx[m] = (int)js_round(kmag * cos(theta));
+    ky[m] = (int)js_round(kmag * sin(theta));
+    phase[m] = 2 * M_PI * RND();
+  }
+#undef RND
+  return 24;
+}
+
+/* src/cpu/solver.js:200-222 -- curl of a streamfunction, added at the faces */
+void or_band_forcing(int nx, int ny, double lx, double ly, double *u, double *v,
+                     double kf, double width, double amp, uint32_t seed, double dt) {
+  int kx[24], ky[24];
+  double ph[24];
+  int M = or_band_modes(seed, kf, width, kx, ky, ph);
+  double dx = lx / nx, dy = ly / ny;
+  double scale = amp * dt / sqrt((double)M);
+  for (int m = 0; m < M; m++) {
+    if (kx[m] == 0 && ky[m] == 0) continue;
+    double kk = (double)kx[m] * kx[m] + (double)ky[m] * ky[m];
+    double cu = scale * (ky[m] / sqrt(kk)), cv = scale * (-kx[m] / sqrt(kk));
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; j++) {
+      for (int i = 0; i < nx; i++) {
+        double xu = i * dx, yu = (j + 0.5) * dy;
+        u[j * nx + i] += cu * cos(kx[m] * xu + ky[m] * yu + ph[m]);
+        double xv = (i + 0.5) * dx, yv = j * dy;
+        v[j * nx + i] += cv * cos(kx[m] * xv + ky[m] * yv + ph[m]);
+      }
+    }
+  }
+}
+
+/* src/cpu/solver.js:233-237 */
+void or_linear_drag(int n, double *u, double *v, double alpha, double dt) {
+  if (alpha <= 0) return;
+  double f = exp(-alpha * dt);
+  for (int i = 0; i < n; i++) { u[i] *= f; v[i] *= f; }
+}
+
+/* ---------------------------------------------------------------- closed forms */
+
+/* src/core/analytic.js:38-72 */
+void or_taylor_green(int nx, int ny, double lx, double ly, double t, double nu, double *u, double *v) {
+  double dx = lx / nx, dy = ly / ny, a = exp(-2 * nu * t);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      double xu = i * dx, yu = (j + 0.5) * dy, xv = (i + 0.5) * dx, yv = j * dy;
+      u[j * nx + i] = sin(xu) * cos(yu) * a;
+      v[j * nx + i] = -cos(xv) * sin(yv) * a;
+    }
+}
+
+/* src/core/analytic.js:195-206 */
+double or_velocity_error_tg(int nx, int ny, double lx, double ly, const double *u, const double *v,
+                            double t, double nu) {
+  size_t n = (size_t)nx * ny;
+  double *eu = (double *)malloc(sizeof(double) * n), *ev = (double *)malloc(sizeof(double) * n);
+  or_taylor_green(nx, ny, lx, ly, t, nu, eu, ev);
+  double num = 0, den = 0;
+  for (size_t k = 0; k < n; k++) {
+    double du = u[k] - eu[k], dv = v[k] - ev[k];
+    num += du * du + dv * dv;
+    den += eu[k] * eu[k] + ev[k] * ev[k];
+  }
+  free(eu); free(ev);
+  if (den < 1e-300) return sqrt(num / (2.0 * n));
+  return sqrt(num / den);
+}
+
+/* src/core/fft.js:196-234 */
+int or_energy_spectrum(int nx, int ny, const double *cu, const double *cv, double *E, int *counts) {
+  if (!is_pow2(nx) || !is_pow2(ny)) return -1;
+  size_t n = (size_t)nx * ny;
+  double N = (double)n;
+  double *ur = (double *)malloc(sizeof(double) * n), *ui = (double *)calloc(n, sizeof(double));
+  double *vr = (double *)malloc(sizeof(double) * n), *vi = (double *)calloc(n, sizeof(double));
+  memcpy(ur, cu, sizeof(double) * n);
+  memcpy(vr, cv, sizeof(double) * n);
+  or_fft2(nx, ny, ur, ui, -1);
+  or_fft2(nx, ny, vr, vi, -1);
+  int kmax = (nx < ny ? nx : ny) / 2;
+  for (int k = 0; k <= kmax; k++) { E[k] = 0; counts[k] = 0; }
+  for (int j = 0; j < ny; j++) {
+    int ky = j <= ny / 2 ? j : j - ny;
+    for (int i = 0; i < nx; i++) {
+      int kx = i <= nx / 2 ? i : i - nx;
+      size_t id = (size_t)j * nx + i;
+      double pu = (ur[id] * ur[id] + ui[id] * ui[id]) / (N * N);
+      double pv = (vr[id] * vr[id] + vi[id] * vi[id]) / (N * N);
+      double e = 0.5 * (pu + pv);
+      int shell = (int)js_round(sqrt((double)kx * kx + (double)ky * ky));
+      if (shell <= kmax) { E[shell] += e; counts[shell]++; }
+    }
+  }
+  free(ur); free(ui); free(vr); free(vi);
+  return kmax;
+}
+
+/* ------------------------------------------------------------------- the step */
+
+/* src/cpu/solver.js:336-398: advect -> diffuse -> force -> drag -> project */
+int or_step(const or_step_cfg *c, double *u, double *v, double *p_out, double dt, uint32_t force_seed) {
+  int nx = c->nx, ny = c->ny;
+  size_t n = (size_t)nx * ny;
+  double *tu = (double *)malloc(sizeof(double) * n), *tv = (double *)malloc(sizeof(double) * n);
+  double *div = (double *)malloc(sizeof(double) * n), *p = (double *)calloc(n, sizeof(double));
+  int rc = 0;
+  if (c->advection == OR_ADV_MC) or_advect_velocity_mc(nx, ny, c->lx, c->ly, u, v, tu, tv, dt);
+  else if (c->advection == OR_ADV_SL) or_advect_velocity_sl(nx, ny, c->lx, c->ly, u, v, tu, tv, dt);
+  else { rc = -1; goto done; }
+  if (c->nu > 0) {
+    if (c->explicit_diffusion) {
+      or_diffuse_explicit(nx, ny, c->lx, c->ly, tu, u, c->nu, dt);
+      or_diffuse_explicit(nx, ny, c->lx, c->ly, tv, v, c->nu, dt);
+    } else {
+      or_diffuse_implicit(nx, ny, c->lx, c->ly, tu, u, c->nu, dt, 1e-12);
+      or_diffuse_implicit(nx, ny, c->lx, c->ly, tv, v, c->nu, dt, 1e-12);
+    }
+  } else {
+    memcpy(u, tu, sizeof(double) * n);
+    memcpy(v, tv, sizeof(double) * n);
+  }
+  if (c->forcing)
+    or_band_forcing(nx, ny, c->lx, c->ly, u, v, c->kf, c->width, c->amp, force_seed, dt);
+  if (c->drag > 0) or_linear_drag((int)n, u, v, c->drag, dt);
+
+  if (c->projection != OR_PROJ_NONE) {
+    or_divergence(nx, ny, c->lx, c->ly, u, v, div);
+    switch (c->projection) {
+      case OR_PROJ_FFT: rc = or_poisson_fft(nx, ny, c->lx, c->ly, div, p); break;
+      case OR_PROJ_CG: or_poisson_cg(nx, ny, c->lx, c->ly, div, p, c->cg_tol, c->cg_maxit, NULL); break;
+      case OR_PROJ_JACOBI: or_jacobi(nx, ny, c->lx, c->ly, div, p, c->iterations, 1.0); break;
+      case OR_PROJ_REDBLACK: or_redblack(nx, ny, c->lx, c->ly, div, p, c->iterations, 1.0); break;
+      case OR_PROJ_MULTIGRID:
+        if (nx != ny) { rc = -2; goto done; }
+        if (or_mg_vcycles(nx, c->lx, div, p, c->iterations, c->mg_min_size, c->mg_pre, c->mg_post,
+                          c->mg_coarse, NULL) < 1) { rc = -2; goto done; }
+        break;
+      default: rc = -1; goto done;
+    }
+    if (rc) goto done;
+    or_subtract_gradient(nx, ny, c->lx, c->ly, u, v, p, 1.0);
+  }
+  if (p_out) memcpy(p_out, p, sizeof(double) * n);
+done:
+  free(tu); free(tv); free(div); free(p);
+  return rc;
+}

@@ -1,0 +1,335 @@
+"""ctypes binding of the f64 CPU oracle (oracle/fsoracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. The product library in
+fluid-sim_amd/ never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfsoracle.so")
+
+ADV = {"semi-lagrangian": 0, "maccormack": 1}
+PROJ = {"fft": 0, "cg": 1, "jacobi": 2, "red-black": 3, "multigrid": 4, "none": 5}
+
+
+class StepCfg(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double),
+        ("nu", C.c_double),
+        ("advection", C.c_int),
+        ("explicit_diffusion", C.c_int),
+        ("projection", C.c_int),
+        ("iterations", C.c_int),
+        ("mg_min_size", C.c_int), ("mg_pre", C.c_int), ("mg_post", C.c_int), ("mg_coarse", C.c_int),
+        ("cg_tol", C.c_double),
+        ("cg_maxit", C.c_int),
+        ("forcing", C.c_int),
+        ("kf", C.c_double), ("width", C.c_double), ("amp", C.c_double), ("drag", C.c_double),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile fsoracle.c with gcc into oracle/_build/ (make recipe in oracle/Makefile)."""
+    cmd = ["make", "-s", "-f", os.path.join(HERE, "Makefile")]
+    if force:
+        cmd.append("-B")
+    subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+_lib = None
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    i, d, p = C.c_int, C.c_double, _D
+    sig = {
+        "or_set_threads": (None, [i]),
+        "or_get_threads": (i, []),
+        "or_js_lcg_fill": (None, [C.POINTER(C.c_int64), i, p]),
+        "or_wrap": (i, [i, i]),
+        "or_bilerp": (d, [p, i, i, d, d]),
+        "or_sample_u": (d, [i, i, d, d, p, d, d]),
+        "or_sample_v": (d, [i, i, d, d, p, d, d]),
+        "or_sample_p": (d, [i, i, d, d, p, d, d]),
+        "or_divergence": (None, [i, i, d, d, p, p, p]),
+        "or_subtract_gradient": (None, [i, i, d, d, p, p, p, d]),
+        "or_laplacian": (None, [i, i, d, d, p, p]),
+        "or_vorticity_corner": (None, [i, i, d, d, p, p, p]),
+        "or_velocity_at_centers": (None, [i, i, p, p, p, p]),
+        "or_kinetic_energy": (d, [i, i, p, p]),
+        "or_kinetic_energy_faces": (d, [i, p, p]),
+        "or_enstrophy": (d, [i, i, d, d, p, p]),
+        "or_max_divergence": (d, [i, i, d, d, p, p]),
+        "or_cfl": (d, [i, i, d, d, p, p, d]),
+        "or_poisson_residual_rel": (d, [i, i, d, d, p, p]),
+        "or_advect_velocity_sl": (None, [i, i, d, d, p, p, p, p, d]),
+        "or_advect_velocity_mc": (None, [i, i, d, d, p, p, p, p, d]),
+        "or_advect_scalar_sl": (None, [i, i, d, d, p, p, p, p, d]),
+        "or_advect_scalar_mc": (None, [i, i, d, d, p, p, p, p, d]),
+        "or_diffuse_explicit": (None, [i, i, d, d, p, p, d, d]),
+        "or_diffuse_implicit": (i, [i, i, d, d, p, p, d, d, d]),
+        "or_fft": (i, [i, p, p, i]),
+        "or_fft2": (i, [i, i, p, p, i]),
+        "or_poisson_fft": (i, [i, i, d, d, p, p]),
+        "or_poisson_cg": (i, [i, i, d, d, p, p, d, i, C.POINTER(C.c_double)]),
+        "or_jacobi": (None, [i, i, d, d, p, p, i, d]),
+        "or_redblack": (None, [i, i, d, d, p, p, i, d]),
+        "or_mg_levels": (i, [i, i]),
+        "or_mg_vcycles": (i, [i, d, p, p, i, i, i, i, i, C.c_void_p]),
+        "or_band_modes": (i, [C.c_uint32, d, d, _I, _I, p]),
+        "or_band_forcing": (None, [i, i, d, d, p, p, d, d, d, C.c_uint32, d]),
+        "or_linear_drag": (None, [i, p, p, d, d]),
+        "or_taylor_green": (None, [i, i, d, d, d, d, p, p]),
+        "or_velocity_error_tg": (d, [i, i, d, d, p, p, d, d]),
+        "or_energy_spectrum": (i, [i, i, p, p, p, _I]),
+        "or_step": (i, [C.POINTER(StepCfg), p, p, C.c_void_p, d, C.c_uint32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+TWO_PI = 2.0 * np.pi
+
+
+# ---------------------------------------------------------------- generators
+def js_lcg(seed: int, n: int):
+    """Values of the reference tests' LCG (test/core.test.js:94-98); returns (values, state)."""
+    st = C.c_int64(seed)
+    out = np.empty(n, np.float64)
+    lib().or_js_lcg_fill(C.byref(st), n, out)
+    return out, st.value
+
+
+def js_random_velocity(n: int, seed: int):
+    """u then v filled from one LCG stream, as tools/gputest.html:155-157 does."""
+    vals, _ = js_lcg(seed, 2 * n * n)
+    u = np.empty(n * n)
+    v = np.empty(n * n)
+    # the harness interleaves: u0[i] = rnd(); v0[i] = rnd();
+    u[:] = vals[0::2]
+    v[:] = vals[1::2]
+    return u, v
+
+
+# ---------------------------------------------------------------- operators
+def divergence(nx, ny, u, v, lx=TWO_PI, ly=TWO_PI):
+    out = np.empty(nx * ny)
+    lib().or_divergence(nx, ny, lx, ly, _f(u), _f(v), out)
+    return out
+
+
+def subtract_gradient(nx, ny, u, v, p, scale=1.0, lx=TWO_PI, ly=TWO_PI):
+    u = _f(u).copy()
+    v = _f(v).copy()
+    lib().or_subtract_gradient(nx, ny, lx, ly, u, v, _f(p), scale)
+    return u, v
+
+
+def laplacian(nx, ny, p, lx=TWO_PI, ly=TWO_PI):
+    out = np.empty(nx * ny)
+    lib().or_laplacian(nx, ny, lx, ly, _f(p), out)
+    return out
+
+
+def bilerp(a, nx, ny, gx, gy):
+    return lib().or_bilerp(_f(a), nx, ny, gx, gy)
+
+
+def sample(kind, nx, ny, arr, x, y, lx=TWO_PI, ly=TWO_PI):
+    fn = {"u": lib().or_sample_u, "v": lib().or_sample_v, "p": lib().or_sample_p}[kind]
+    return fn(nx, ny, lx, ly, _f(arr), x, y)
+
+
+def advect_velocity(scheme, nx, ny, u, v, dt, lx=TWO_PI, ly=TWO_PI):
+    ou = np.empty(nx * ny)
+    ov = np.empty(nx * ny)
+    fn = lib().or_advect_velocity_mc if scheme == "maccormack" else lib().or_advect_velocity_sl
+    if scheme not in ADV:
+        raise ValueError(f"unknown advection scheme {scheme}")
+    fn(nx, ny, lx, ly, _f(u), _f(v), ou, ov, dt)
+    return ou, ov
+
+
+def advect_scalar(scheme, nx, ny, u, v, q, dt, lx=TWO_PI, ly=TWO_PI):
+    out = np.empty(nx * ny)
+    fn = lib().or_advect_scalar_mc if scheme == "maccormack" else lib().or_advect_scalar_sl
+    fn(nx, ny, lx, ly, _f(u), _f(v), _f(q), out, dt)
+    return out
+
+
+def diffuse_explicit(nx, ny, f, nu, dt, lx=TWO_PI, ly=TWO_PI):
+    out = np.empty(nx * ny)
+    lib().or_diffuse_explicit(nx, ny, lx, ly, _f(f), out, nu, dt)
+    return out
+
+
+def poisson_fft(nx, ny, rhs, lx=TWO_PI, ly=TWO_PI):
+    p = np.empty(nx * ny)
+    rc = lib().or_poisson_fft(nx, ny, lx, ly, _f(rhs), p)
+    if rc:
+        raise ValueError("fft projection needs power-of-two dimensions")
+    return p
+
+
+def poisson_cg(nx, ny, rhs, tol=1e-10, maxit=2000, lx=TWO_PI, ly=TWO_PI):
+    p = np.zeros(nx * ny)
+    res = C.c_double()
+    it = lib().or_poisson_cg(nx, ny, lx, ly, _f(rhs), p, tol, maxit, C.byref(res))
+    return p, it, res.value
+
+
+def jacobi(nx, ny, rhs, iters, omega=1.0, p=None, lx=TWO_PI, ly=TWO_PI):
+    p = np.zeros(nx * ny) if p is None else _f(p).copy()
+    lib().or_jacobi(nx, ny, lx, ly, _f(rhs), p, iters, omega)
+    return p
+
+
+def redblack(nx, ny, rhs, sweeps, omega=1.0, p=None, lx=TWO_PI, ly=TWO_PI):
+    p = np.zeros(nx * ny) if p is None else _f(p).copy()
+    lib().or_redblack(nx, ny, lx, ly, _f(rhs), p, sweeps, omega)
+    return p
+
+
+def mg_levels(n, min_size=8):
+    return lib().or_mg_levels(n, min_size)
+
+
+def mg_vcycles(n, rhs, cycles, p=None, min_size=8, pre=2, post=2, coarse=40, lx=TWO_PI):
+    """Returns (p, history) where history[k] = ||Lp - rhs||/||rhs|| after cycle k+1."""
+    p = np.zeros(n * n) if p is None else _f(p).copy()
+    hist = np.zeros(max(cycles, 1))
+    L = lib().or_mg_vcycles(n, lx, _f(rhs), p, cycles, min_size, pre, post, coarse,
+                            hist.ctypes.data_as(C.c_void_p))
+    if L < 1:
+        raise ValueError("bad multigrid configuration")
+    return p, hist[:cycles]
+
+
+def poisson_residual_rel(nx, ny, p, rhs, lx=TWO_PI, ly=TWO_PI):
+    return lib().or_poisson_residual_rel(nx, ny, lx, ly, _f(p), _f(rhs))
+
+
+def band_modes(seed, kf, width):
+    kx = np.zeros(24, np.int32)
+    ky = np.zeros(24, np.int32)
+    ph = np.zeros(24)
+    lib().or_band_modes(seed & 0xFFFFFFFF, kf, width, kx, ky, ph)
+    return kx, ky, ph
+
+
+def band_forcing(nx, ny, u, v, kf, width, amp, seed, dt, lx=TWO_PI, ly=TWO_PI):
+    u = _f(u).copy()
+    v = _f(v).copy()
+    lib().or_band_forcing(nx, ny, lx, ly, u, v, kf, width, amp, seed & 0xFFFFFFFF, dt)
+    return u, v
+
+
+def taylor_green(nx, ny, t=0.0, nu=0.0, lx=TWO_PI, ly=TWO_PI):
+    u = np.empty(nx * ny)
+    v = np.empty(nx * ny)
+    lib().or_taylor_green(nx, ny, lx, ly, t, nu, u, v)
+    return u, v
+
+
+def velocity_error_tg(nx, ny, u, v, t, nu, lx=TWO_PI, ly=TWO_PI):
+    return lib().or_velocity_error_tg(nx, ny, lx, ly, _f(u), _f(v), t, nu)
+
+
+def kinetic_energy(nx, ny, u, v):
+    return lib().or_kinetic_energy(nx, ny, _f(u), _f(v))
+
+
+def kinetic_energy_faces(u, v):
+    return lib().or_kinetic_energy_faces(len(u), _f(u), _f(v))
+
+
+def enstrophy(nx, ny, u, v, lx=TWO_PI, ly=TWO_PI):
+    return lib().or_enstrophy(nx, ny, lx, ly, _f(u), _f(v))
+
+
+def max_divergence(nx, ny, u, v, lx=TWO_PI, ly=TWO_PI):
+    return lib().or_max_divergence(nx, ny, lx, ly, _f(u), _f(v))
+
+
+def velocity_at_centers(nx, ny, u, v):
+    cu = np.empty(nx * ny)
+    cv = np.empty(nx * ny)
+    lib().or_velocity_at_centers(nx, ny, _f(u), _f(v), cu, cv)
+    return cu, cv
+
+
+def energy_spectrum(nx, ny, cu, cv):
+    kmax = min(nx, ny) // 2
+    E = np.zeros(kmax + 1)
+    cnt = np.zeros(kmax + 1, np.int32)
+    lib().or_energy_spectrum(nx, ny, _f(cu), _f(cv), E, cnt)
+    return E, cnt
+
+
+def fft2(re, im, nx, ny, sign=-1):
+    re = _f(re).copy()
+    im = _f(im).copy()
+    if lib().or_fft2(nx, ny, re, im, sign):
+        raise ValueError("fft2 needs power-of-two dimensions")
+    return re, im
+
+
+def set_threads(n):
+    lib().or_set_threads(int(n))
+
+
+class Solver:
+    """Oracle counterpart of FluidSolver (src/cpu/solver.js:241-418), f64."""
+
+    def __init__(self, n=64, nx=None, ny=None, lx=TWO_PI, ly=TWO_PI, nu=0.0,
+                 advection="maccormack", explicit_diffusion=True, projection="fft",
+                 iterations=40, mg_min_size=8, mg_pre=2, mg_post=2, mg_coarse=40,
+                 cg_tol=1e-10, cg_maxit=2000, forcing=None, drag=0.0):
+        nx = nx or n
+        ny = ny or n
+        if advection not in ADV:
+            raise ValueError(f"unknown advection scheme {advection}")
+        if projection not in PROJ:
+            raise ValueError(f"unknown projection {projection}")
+        f = forcing or {}
+        self.cfg = StepCfg(nx, ny, lx, ly, nu, ADV[advection], int(explicit_diffusion), PROJ[projection],
+                           iterations, mg_min_size, mg_pre, mg_post, mg_coarse, cg_tol, cg_maxit,
+                           1 if forcing else 0, f.get("kf", 0.0), f.get("width", 0.0), f.get("amp", 0.0), drag)
+        self.nx, self.ny, self.lx, self.ly = nx, ny, lx, ly
+        self.u = np.zeros(nx * ny)
+        self.v = np.zeros(nx * ny)
+        self.p = np.zeros(nx * ny)
+        self.t = 0.0
+        self.steps = 0
+
+    def step(self, dt, force_seed=0):
+        rc = lib().or_step(C.byref(self.cfg), self.u, self.v, self.p.ctypes.data_as(C.c_void_p), dt,
+                           force_seed & 0xFFFFFFFF)
+        if rc:
+            raise RuntimeError(f"oracle step failed ({rc})")
+        self.t += dt
+        self.steps += 1

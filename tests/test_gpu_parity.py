@@ -1,0 +1,266 @@
+"""GPU parity: the HIP path through the C ABI against the f64 oracle on the same
+seeded inputs, restating tools/gputest.html checks 1-7 plus the tolerance-gated
+projection. FP32 kernels are compared with the tolerance written in each test;
+the FP64 refinement residual is checked against its gate exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+TWO_PI = 2 * math.pi
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return math.sqrt(np.sum((a - b) ** 2) / max(np.sum(b * b), 1e-300))
+
+
+def f32(a):
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def fs():
+    import fluidsim
+    fluidsim.createDevice()
+    return fluidsim
+
+
+def solver(fs, **kw):
+    return fs.GPUFluidSolver(**kw)
+
+
+# ------------------------------------------------------------- single kernels
+
+def test_advection_sl_kernel_matches_oracle(fs, O):
+    # gputest check 1: one SL step of TG at 64^2, dt = 0.01 (reference relL2 2.3e-7)
+    u0, v0 = O.taylor_green(N, N)
+    with solver(fs, n=N, advection="semi-lagrangian", pressureIterations=0) as g:
+        g.uploadVelocity(u0, v0)
+        g.runKernel("advect-sl", 0.01)
+        out = g.readVelocity()
+    ou, ov = O.advect_velocity("semi-lagrangian", N, N, f32(u0), f32(v0), 0.01)
+    assert max(rel(out["u"], ou), rel(out["v"], ov)) < 1e-6
+
+
+@pytest.mark.parametrize("seed", [991, 31337])
+def test_advection_mc_kernel_matches_oracle_on_rough_field(fs, O, seed):
+    # random field: the limiter is active at most faces
+    u0, v0 = O.js_random_velocity(N, seed)
+    dt = 0.05
+    with solver(fs, n=N, advection="maccormack") as g:
+        g.uploadVelocity(u0, v0)
+        g.runKernel("advect-mc", dt)
+        out = g.readVelocity()
+    ou, ov = O.advect_velocity("maccormack", N, N, f32(u0), f32(v0), dt)
+    assert max(rel(out["u"], ou), rel(out["v"], ov)) < 1e-5
+
+
+def test_divergence_kernel_matches_oracle(fs, O):
+    # gputest check 2 (seed 991; reference relL2 5.8e-8)
+    u0, v0 = O.js_random_velocity(N, 991)
+    with solver(fs, n=N) as g:
+        g.uploadVelocity(u0, v0)
+        g.runKernel("divergence")
+        d = g.readField("div")
+    ref = O.divergence(N, N, f32(u0), f32(v0))
+    assert rel(d, ref) < 1e-6
+
+
+def test_diffusion_kernel_matches_oracle(fs, O):
+    u0, v0 = O.js_random_velocity(N, 4242)
+    nu, dt = 0.01, 0.02
+    with solver(fs, n=N, nu=nu) as g:
+        g.uploadVelocity(u0, v0)
+        g.runKernel("diffuse", dt)
+        out = g.readVelocity()
+    ou = O.diffuse_explicit(N, N, f32(u0), nu, dt)
+    ov = O.diffuse_explicit(N, N, f32(v0), nu, dt)
+    assert max(rel(out["u"], ou), rel(out["v"], ov)) < 1e-6
+
+
+def test_max_divergence_reduction(fs, O, golden):
+    u0, v0 = O.js_random_velocity(N, 4242)
+    with solver(fs, n=N) as g:
+        g.uploadVelocity(u0, v0)
+        md = g.maxDivergence()
+        d = g.diagnostics()
+    ref = O.max_divergence(N, N, f32(u0), f32(v0))
+    assert abs(md - ref) <= 1e-12 * ref
+    assert abs(md - golden["projection"]["red-black"]["before"]) / md < 2e-7
+    assert abs(d["kinetic_energy"] - O.kinetic_energy(N, N, f32(u0), f32(v0))) < 1e-12
+    assert abs(d["momentum_x"] - f32(u0).sum() * (TWO_PI / N) ** 2) < 1e-9
+
+
+# ---------------------------------------------------- projection and solvers
+
+@pytest.mark.parametrize("solver_name", ["jacobi", "red-black"])
+def test_projection_after_200_sweeps(fs, O, golden, solver_name):
+    # gputest check 3 (seed 4242, MacCormack default, 200 sweeps)
+    u0, v0 = O.js_random_velocity(N, 4242)
+    with solver(fs, n=N, pressureSolver=solver_name, pressureIterations=200) as g:
+        g.uploadVelocity(u0, v0)
+        before = g.maxDivergence()
+        g.step(0.01)
+        after = g.maxDivergence()
+    s = O.Solver(n=N, nu=0.0, advection="maccormack", projection=solver_name, iterations=200)
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    s.step(0.01)
+    ref = O.max_divergence(N, N, s.u, s.v)
+    assert abs(after - ref) / ref < 1e-3
+    assert abs(after - golden["projection"][solver_name]["after"]) / ref < 1e-3
+    assert before / after > (1000 if solver_name == "red-black" else 25)
+
+
+@pytest.mark.parametrize("solver_name", ["jacobi", "red-black", "multigrid"])
+def test_poisson_curves(fs, O, golden, solver_name):
+    # gputest check 4 (seed 777): residual after k iterations from a zeroed pressure
+    u0, v0 = O.js_random_velocity(N, 777)
+    d = O.divergence(N, N, f32(u0), f32(v0))
+    with solver(fs, n=N, pressureSolver=solver_name) as g:
+        g.uploadVelocity(u0, v0)
+        for it, ref in golden["poissonCurves"][solver_name]:
+            r = g.poissonResidualAfter(it)
+            if it == 0:
+                assert abs(r - 1.0) < 1e-12
+                continue
+            if solver_name == "jacobi":
+                p = O.jacobi(N, N, d, it)
+            elif solver_name == "red-black":
+                p = O.redblack(N, N, d, it)
+            else:
+                p, _ = O.mg_vcycles(N, d, it)
+            ro = O.poisson_residual_rel(N, N, p, d)
+            if ro > 1e-5:
+                assert abs(r - ro) / ro < 1e-3, (it, r, ro)
+                assert abs(r - ref) / ref < 1e-3, (it, r, ref)
+            else:
+                # below the FP32 floor of the reference's plateau (~9e-8)
+                assert r < 2e-6, (it, r)
+
+
+# ------------------------------------------------------------- full steps
+
+@pytest.mark.parametrize("scheme,solver_name,iters,steps,tol", [
+    ("semi-lagrangian", "red-black", 400, 10, 1e-5),   # gputest check 5 (ref 3.3e-6 vs FFT)
+    ("maccormack", "multigrid", 4, 10, 1e-5),          # gputest check 6 (ref 3.5e-7 vs FFT)
+])
+def test_full_steps_match_oracle(fs, O, scheme, solver_name, iters, steps, tol):
+    u0, v0 = O.taylor_green(N, N)
+    with solver(fs, n=N, advection=scheme, pressureSolver=solver_name, pressureIterations=iters) as g:
+        g.uploadVelocity(u0, v0)
+        for _ in range(steps):
+            g.step(0.01)
+        out = g.readVelocity()
+    s = O.Solver(n=N, nu=0.0, advection=scheme, projection=solver_name, iterations=iters)
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        s.step(0.01)
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < tol
+    sf = O.Solver(n=N, nu=0.0, advection=scheme, projection="fft")
+    sf.u[:], sf.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        sf.step(0.01)
+    assert max(rel(out["u"], sf.u), rel(out["v"], sf.v)) < 5e-3
+
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_scheme_energy_after_200_steps(fs, O, golden, scheme):
+    # gputest check 7: MacCormack keeps more energy than SL at nu = 0
+    u0, v0 = O.taylor_green(N, N)
+    with solver(fs, n=N, advection=scheme, pressureSolver="multigrid", pressureIterations=3) as g:
+        g.uploadVelocity(u0, v0)
+        for _ in range(200):
+            g.step(0.01)
+        out = g.readVelocity()
+    ke = 0.5 * np.mean(out["u"].astype(np.float64) ** 2 + out["v"].astype(np.float64) ** 2)
+    assert abs(ke - golden["schemeEnergy"][scheme]) / ke < 2e-5
+
+
+def test_taylor_green_200_steps_c1(fs, O):
+    # TG 64^2, nu = 0.01, MacCormack + explicit diffusion, projection to
+    # an FP64 residual of 1e-9, dt at CFL 0.5; FP32 GPU against the f64 oracle (exact FFT)
+    n, nu = 64, 0.01
+    u0, v0 = O.taylor_green(n, n)
+    h = TWO_PI / n
+    dt = 0.5 / (np.max(np.abs(u0)) / h + np.max(np.abs(v0)) / h)
+    with solver(fs, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=30, tolerance=1e-9) as g:
+        g.uploadVelocity(u0, v0)
+        for _ in range(200):
+            g.step(dt)
+            assert g.stats()["last_residual"] <= 1e-9
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    for _ in range(200):
+        s.step(dt)
+    err = max(rel(out["u"], s.u), rel(out["v"], s.v))
+    assert err < 1e-4, err
+    T = 200 * dt
+    ke0 = O.kinetic_energy(n, n, f32(u0), f32(v0))
+    ke = O.kinetic_energy(n, n, out["u"].astype(np.float64), out["v"].astype(np.float64))
+    rate_gpu = -math.log(ke / ke0) / T
+    ke_o = O.kinetic_energy(n, n, s.u, s.v)
+    rate_o = -math.log(ke_o / ke0) / T
+    assert abs(rate_gpu - rate_o) / rate_o < 1e-3
+    assert 4 * nu < rate_gpu < 1.35 * 4 * nu
+
+
+def test_tolerance_gate_on_turbulence(fs, O):
+    # C2 at 256^2: every projection meets max|div u* - L p| <= 1e-6 in FP64, and the
+    # velocity tracks the oracle's exact projection
+    from fluidsim import initial
+    n, nu = 256, 1e-4
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    with solver(fs, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-6, mgMinSize=4) as g:
+        g.uploadVelocity(u0, v0)
+        dts = []
+        for _ in range(5):
+            dts.append(g.stepCFL(0.5))
+            st = g.stats()
+            assert st["last_residual"] <= 1e-6
+            assert st["last_vcycles"] >= 1
+        out = g.readVelocity()
+        d = g.diagnostics()
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for k in range(5):
+        dt = 0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h)
+        assert abs(dt - dts[k]) / dt < 1e-5
+        s.step(dts[k])
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+    # the stored FP32 field's own divergence sits at the FP32 rounding floor
+    assert d["max_divergence"] < 5e-4
+
+
+def test_runs_are_bitwise_deterministic(fs, O):
+    # test/solid.test.js:339-358 style: two identical runs, identical bits
+    u0, v0 = O.js_random_velocity(N, 31337)
+    outs = []
+    for _ in range(2):
+        with solver(fs, n=N, nu=1e-3, advection="maccormack", pressureSolver="multigrid",
+                    pressureIterations=20, tolerance=1e-7) as g:
+            g.uploadVelocity(u0 * 0.1, v0 * 0.1)
+            for _ in range(5):
+                g.step(0.01)
+            outs.append(g.readVelocity())
+    assert (outs[0]["u"] == outs[1]["u"]).all() and (outs[0]["v"] == outs[1]["v"]).all()
+
+
+def test_nonfinite_velocity_is_reported(fs):
+    import fluidsim
+    u = np.zeros(N * N)
+    v = np.zeros(N * N)
+    u[123] = np.nan
+    with solver(fs, n=N) as g:
+        g.uploadVelocity(u, v)
+        with pytest.raises(fluidsim.FluidSimError, match="non-finite"):
+            g.diagnostics()

@@ -1,0 +1,368 @@
+"""Pins of the f64 CPU oracle: reference-held records, the reference's own
+known-answer tests, and closed forms chosen so a plausible mistake (wrong
+offset, sign, factor or colour order) fails one of them. CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+N = 64
+TWO_PI = 2 * math.pi
+
+
+def f32(a):
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+# ------------------------------------------------ reference-held records (GPU harness)
+
+def test_lcg_and_divergence_reproduce_reference_max_div(O, golden):
+    # tools/gputest.html:155-167 + results/gpu-verify.json projection.before
+    u, v = O.js_random_velocity(N, 4242)
+    md = O.max_divergence(N, N, f32(u), f32(v))
+    ref = golden["projection"]["red-black"]["before"]
+    assert abs(md - ref) / ref < 2e-7
+
+
+@pytest.mark.parametrize("solver", ["jacobi", "red-black"])
+def test_projection_after_200_sweeps_matches_reference(O, golden, solver):
+    # gputest block 3: maccormack advection (the default), nu=0, 200 sweeps, dt=0.01
+    u, v = O.js_random_velocity(N, 4242)
+    s = O.Solver(n=N, nu=0.0, advection="maccormack", projection=solver, iterations=200)
+    s.u[:] = f32(u)
+    s.v[:] = f32(v)
+    s.step(0.01)
+    after = O.max_divergence(N, N, s.u, s.v)
+    ref = golden["projection"][solver]["after"]
+    assert abs(after - ref) / ref < 2e-5
+
+
+@pytest.mark.parametrize("solver", ["jacobi", "red-black", "multigrid"])
+def test_poisson_curves_match_reference(O, golden, solver):
+    # gputest block 4 (tools/gputest.html:205-246): seed 777, residual after k iterations
+    u, v = O.js_random_velocity(N, 777)
+    d = O.divergence(N, N, f32(u), f32(v))
+    for it, ref in golden["poissonCurves"][solver]:
+        if solver == "jacobi":
+            p = O.jacobi(N, N, d, it)
+        elif solver == "red-black":
+            p = O.redblack(N, N, d, it)
+        else:
+            p, _ = O.mg_vcycles(N, d, it, min_size=8, pre=2, post=2, coarse=40)
+        r = O.poisson_residual_rel(N, N, p, d) if it > 0 else 1.0
+        if solver == "multigrid" and it >= 5:
+            # the reference's FP32 plateau (~9e-8); the f64 oracle keeps converging
+            assert r < ref
+            continue
+        tol = 5e-3 if (solver == "multigrid" and it == 4) else 1e-4
+        assert abs(r - ref) / ref < tol, (solver, it, r, ref)
+
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_scheme_energy_after_200_tg_steps(O, golden, scheme):
+    # gputest block 7: TG 64^2, nu=0, dt=0.01, multigrid 3 V-cycles, KE of raw faces
+    s = O.Solver(n=N, nu=0.0, advection=scheme, projection="multigrid", iterations=3)
+    u, v = O.taylor_green(N, N)
+    s.u[:] = f32(u)
+    s.v[:] = f32(v)
+    for _ in range(200):
+        s.step(0.01)
+    ke = O.kinetic_energy_faces(s.u, s.v)
+    ref = golden["schemeEnergy"][scheme]
+    assert abs(ke - ref) / ref < 1e-5
+    assert golden["schemeEnergy"]["maccormack"] > golden["schemeEnergy"]["semi-lagrangian"]
+
+
+# ------------------------------------------------ the reference's known-answer tests
+
+def test_wrap_known_answers(O):
+    # test/core.test.js:35-42
+    L = O.lib()
+    assert [L.or_wrap(i, 8) for i in (0, 7, 8, -1, -9, 17)] == [0, 7, 0, 7, 7, 1]
+
+
+def test_bilerp_known_answers_and_wrap(O):
+    # test/core.test.js:44-64
+    a = np.tile(np.arange(4, dtype=np.float64), 4)
+    assert O.bilerp(a, 4, 4, 1, 1) == 1
+    assert O.bilerp(a, 4, 4, 2, 0) == 2
+    assert abs(O.bilerp(a, 4, 4, 1.5, 0) - 1.5) < 1e-15
+    assert abs(O.bilerp(a, 4, 4, 1.25, 0) - 1.25) < 1e-15
+    assert O.bilerp(a, 4, 4, 4, 0) == O.bilerp(a, 4, 4, 0, 0)
+    assert O.bilerp(a, 4, 4, -1, 0) == O.bilerp(a, 4, 4, 3, 0)
+
+
+def test_samplers_round_trip_their_own_positions(O):
+    # test/core.test.js:66-83: catches a half-cell MAC offset
+    n = 16
+    dx = TWO_PI / n
+    i = np.arange(n)[None, :]
+    j = np.arange(n)[:, None]
+    u = (np.sin(i * dx) * np.cos((j + 0.5) * dx)).reshape(-1)
+    v = (np.cos((i + 0.5) * dx) * np.sin(j * dx)).reshape(-1)
+    p = np.sin((i + 0.5) * dx + (j + 0.5) * dx).reshape(-1)
+    for (a, b) in [(0, 0), (3, 5), (15, 15), (8, 2)]:
+        assert abs(O.sample("u", n, n, u, a * dx, (b + 0.5) * dx) - u[b * n + a]) < 1e-12
+        assert abs(O.sample("v", n, n, v, (a + 0.5) * dx, b * dx) - v[b * n + a]) < 1e-12
+        assert abs(O.sample("p", n, n, p, (a + 0.5) * dx, (b + 0.5) * dx) - p[b * n + a]) < 1e-12
+
+
+def test_divergence_gradient_exact_negative_adjoints(O):
+    # test/core.test.js:85-120 (seed 12345, 16^2)
+    n = 16
+    vals, _ = O.js_lcg(12345, 3 * n * n)
+    wu, wv, q = vals[: n * n], vals[n * n: 2 * n * n], vals[2 * n * n:]
+    lhs = float(np.dot(O.divergence(n, n, wu, wv), q))
+    gu, gv = O.subtract_gradient(n, n, np.zeros(n * n), np.zeros(n * n), q, scale=-1.0)
+    rhs = float(np.dot(wu, gu) + np.dot(wv, gv))
+    assert abs(lhs + rhs) / max(abs(lhs), abs(rhs)) < 1e-12
+
+
+def test_taylor_green_discretely_divergence_free(O):
+    # test/core.test.js:138-149
+    for n in (16, 32, 64):
+        u, v = O.taylor_green(n, n, 0.0, 0.01)
+        assert O.max_divergence(n, n, u, v) < 1e-14
+    u = np.full(32 * 32, 1.3)
+    v = np.full(32 * 32, -0.7)
+    assert O.max_divergence(32, 32, u, v) < 1e-14
+
+
+def test_laplacian_symmetric_and_annihilates_constants(O):
+    # test/solver.test.js:31-58
+    n = 16
+    vals, _ = O.js_lcg(4242, 2 * n * n)
+    x, y = vals[: n * n], vals[n * n:]
+    lhs = np.dot(O.laplacian(n, n, x), y)
+    rhs = np.dot(x, O.laplacian(n, n, y))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+    assert np.max(np.abs(O.laplacian(n, n, np.full(n * n, 3.7)))) < 1e-12
+
+
+def test_kinetic_energy_of_taylor_green(O):
+    # test/core.test.js:160-173
+    u, v = O.taylor_green(128, 128, 0.7, 0.03)
+    exact = 0.25 * math.exp(-4 * 0.03 * 0.7)
+    assert abs(O.kinetic_energy(128, 128, u, v) - exact) / exact < 2e-3
+
+
+# ------------------------------------------------------------------ FFT / Poisson
+
+def test_fft_matches_naive_dft(O):
+    n = 32
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    re, im = x.copy(), y.copy()
+    assert O.lib().or_fft(n, re, im, -1) == 0
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    ref = W @ (x + 1j * y)
+    assert np.max(np.abs(re + 1j * im - ref)) < 1e-12
+
+
+def test_fft2_single_cosine_mode(O):
+    # test/core.test.js:298-316
+    nx = ny = 32
+    k0 = 5
+    f = np.cos(2 * np.pi * k0 * np.arange(nx) / nx)[None, :].repeat(ny, 0).reshape(-1)
+    re, im = O.fft2(f, np.zeros(nx * ny), nx, ny)
+    mag = np.hypot(re, im).reshape(ny, nx) / (nx * ny)
+    expect = np.zeros((ny, nx))
+    expect[0, k0] = expect[0, nx - k0] = 0.5
+    assert np.max(np.abs(mag - expect)) < 1e-12
+
+
+def test_energy_spectrum_parseval_for_taylor_green(O):
+    # test/core.test.js:318-327 (no corner modes for TG, so equality is exact)
+    u, v = O.taylor_green(64, 64, 0.0, 0.01)
+    cu, cv = O.velocity_at_centers(64, 64, u, v)
+    E, _ = O.energy_spectrum(64, 64, cu, cv)
+    phys = 0.5 * np.mean(cu * cu + cv * cv)
+    assert abs(E.sum() - phys) / phys < 1e-12
+
+
+def test_fft_poisson_single_mode_closed_form(O):
+    # rhs = sin(kx x) sin(ky y) at centres solves exactly to rhs / lambda(k)
+    n = 32
+    h = TWO_PI / n
+    c = (np.arange(n) + 0.5) * h
+    kx, ky = 3, 5
+    rhs = (np.sin(kx * c)[None, :] * np.sin(ky * c)[:, None]).reshape(-1)
+    lam = -4 * math.sin(math.pi * kx / n) ** 2 / h ** 2 - 4 * math.sin(math.pi * ky / n) ** 2 / h ** 2
+    p = O.poisson_fft(n, n, rhs)
+    assert np.max(np.abs(p - rhs / lam)) < 1e-13
+    assert np.max(np.abs(O.laplacian(n, n, p) - rhs)) < 1e-11
+
+
+def test_cg_equals_fft_projection(O):
+    # test/validation.test.js:201-221
+    n = 32
+    u, v = O.js_random_velocity(n, 777)
+    d = O.divergence(n, n, u, v)
+    pf = O.poisson_fft(n, n, d)
+    pc, it, res = O.poisson_cg(n, n, d, tol=1e-12, maxit=2000)
+    pf -= pf.mean()
+    assert res < 1e-12
+    assert np.max(np.abs(pf - pc)) < 1e-9
+
+
+def test_projection_divergence_free_and_idempotent(O):
+    # test/solver.test.js:133-181
+    n = 32
+    u, v = O.js_random_velocity(n, 777)
+    p = O.poisson_fft(n, n, O.divergence(n, n, u, v))
+    u1, v1 = O.subtract_gradient(n, n, u, v, p)
+    assert O.max_divergence(n, n, u1, v1) < 1e-10
+    p2 = O.poisson_fft(n, n, O.divergence(n, n, u1, v1))
+    u2, v2 = O.subtract_gradient(n, n, u1, v1, p2)
+    assert np.max(np.abs(u2 - u1)) < 1e-12 and np.max(np.abs(v2 - v1)) < 1e-12
+
+
+# ------------------------------------------------------------------- diffusion
+
+def test_explicit_diffusion_checkerboard_and_sine_factors(O):
+    # checkerboard amplification |1 - 8d| (test/validation.test.js:122-146)
+    n = 16
+    h = TWO_PI / n
+    for d in (0.1, 0.25, 0.26):
+        nu, dt = 1.0, d * h * h
+        cb = np.fromfunction(lambda j, i: (-1.0) ** (i + j), (n, n)).reshape(-1)
+        out = O.diffuse_explicit(n, n, cb, nu, dt)
+        assert np.max(np.abs(out - (1 - 8 * d) * cb)) < 1e-12
+    # a sine mode decays by 1 - 2a(1 - cos k h) - 2b(1 - cos l h), and the mean is kept
+    k, l = 2, 3
+    c = np.arange(n) * h
+    f = (np.sin(k * c)[None, :] * np.cos(l * c)[:, None]).reshape(-1) + 0.3
+    nu, dt = 0.02, 0.05
+    a = nu * dt / h ** 2
+    fac = 1 - 2 * a * (1 - math.cos(k * h)) - 2 * a * (1 - math.cos(l * h))
+    out = O.diffuse_explicit(n, n, f, nu, dt)
+    assert np.max(np.abs((out - 0.3) - fac * (f - 0.3))) < 1e-13
+    assert abs(out.mean() - f.mean()) < 1e-14
+
+
+# ------------------------------------------------------------------- advection
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_uniform_integer_shift_is_exact(O, scheme):
+    # a uniform flow moving a whole number of cells per step transports exactly
+    n = 32
+    h = TWO_PI / n
+    dt = 0.1
+    U, V = 2 * h / dt, -1 * h / dt
+    q = O.js_lcg(31337, n * n)[0]
+    u = np.full(n * n, U)
+    v = np.full(n * n, V)
+    out = O.advect_scalar(scheme, n, n, u, v, q, dt)
+    expect = np.roll(np.roll(q.reshape(n, n), 2, axis=1), -1, axis=0).reshape(-1)
+    assert np.max(np.abs(out - expect)) < 1e-12
+    # the velocity itself is a constant and must stay exactly constant
+    ou, ov = O.advect_velocity(scheme, n, n, u, v, dt)
+    assert np.max(np.abs(ou - U)) < 1e-12 and np.max(np.abs(ov - V)) < 1e-12
+
+
+def _gaussian_error(O, scheme, n):
+    # translating Gaussian (src/core/analytic.js:100-140) after T=1 with CFL ~0.5
+    h = TWO_PI / n
+    ax, ay, sig = 1.0, 0.6, 0.5
+    c = (np.arange(n) + 0.5) * h
+
+    def blob(t):
+        out = np.zeros((n, n))
+        for mx in (-1, 0, 1):
+            for my in (-1, 0, 1):
+                X = c[None, :] - (math.pi + ax * t) + mx * TWO_PI
+                Y = c[:, None] - (math.pi + ay * t) + my * TWO_PI
+                out += np.exp(-(X * X + Y * Y) / (2 * sig * sig))
+        return out.reshape(-1)
+
+    dt = 0.5 * h / (ax + ay)
+    steps = int(round(1.0 / dt))
+    dt = 1.0 / steps
+    u = np.full(n * n, ax)
+    v = np.full(n * n, ay)
+    q = blob(0.0)
+    for _ in range(steps):
+        q = O.advect_scalar(scheme, n, n, u, v, q, dt)
+    e = q - blob(1.0)
+    return math.sqrt(np.mean(e * e))
+
+
+def test_maccormack_second_order_and_less_diffusive(O):
+    e_sl = [_gaussian_error(O, "semi-lagrangian", n) for n in (32, 64)]
+    e_mc = [_gaussian_error(O, "maccormack", n) for n in (32, 64)]
+    order_sl = math.log2(e_sl[0] / e_sl[1])
+    order_mc = math.log2(e_mc[0] / e_mc[1])
+    assert 0.7 < order_sl < 1.4
+    assert order_mc > 1.6
+    assert e_mc[1] < e_sl[1] / 4
+
+
+def test_maccormack_limiter_keeps_bounds(O):
+    # test/solver.test.js:275-295: a [0,1] step function stays in [0,1]
+    n = 32
+    u, v = O.taylor_green(n, n)
+    q = np.zeros((n, n))
+    q[8:20, 10:24] = 1.0
+    q = q.reshape(-1)
+    for _ in range(20):
+        q = O.advect_scalar("maccormack", n, n, u, v, q, 0.05)
+    assert q.min() >= -1e-15 and q.max() <= 1 + 1e-15
+
+
+# ------------------------------------------------------------------- multigrid
+
+def test_multigrid_levels_follow_reference_rule(O):
+    # src/gpu/multigrid.js:182-197
+    assert O.mg_levels(64, 8) == 4
+    assert O.mg_levels(1024, 4) == 9
+    assert O.mg_levels(1024, 8) == 8
+
+
+# ------------------------------------------------------------------- full step
+
+def test_taylor_green_decay_c1(O):
+    # TG 64^2, nu=0.01, MacCormack, explicit diffusion, exact projection,
+    # dt at CFL 0.5 (src/core/grid.js:322-327), 200 steps
+    n, nu = 64, 0.01
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = O.taylor_green(n, n)
+    h = TWO_PI / n
+    dt = 0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h)
+    ke0 = O.kinetic_energy(n, n, s.u, s.v)
+    for _ in range(200):
+        s.step(dt)
+    T = 200 * dt
+    err = O.velocity_error_tg(n, n, s.u, s.v, T, nu)
+    ke = O.kinetic_energy(n, n, s.u, s.v)
+    rate = -math.log(ke / ke0) / T
+    # first order in time (Lie splitting) at d = nu dt / h^2 = 0.0255: the scheme's own
+    # dissipation adds to 4 nu, it never removes from it
+    assert err < 0.05
+    assert 4 * nu < rate < 1.35 * 4 * nu
+    assert O.max_divergence(n, n, s.u, s.v) < 1e-12
+
+
+def test_taylor_green_reference_gate(O):
+    # test/solver.test.js:339-361 with the explicit diffusion the GPU path uses
+    n, nu, dt = 64, 0.05, 0.01
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = O.taylor_green(n, n)
+    for _ in range(50):
+        s.step(dt)
+    assert O.velocity_error_tg(n, n, s.u, s.v, 0.5, nu) < 0.05
+    ke = O.kinetic_energy(n, n, s.u, s.v)
+    assert ke < 0.25 * math.exp(-4 * nu * 0.5) * 1.02
+
+
+def test_band_forcing_modes_deterministic_and_in_shell(O):
+    kx, ky, ph = O.band_modes(1000, 32.0, 2.0)
+    kx2, ky2, ph2 = O.band_modes(1000, 32.0, 2.0)
+    assert (kx == kx2).all() and (ky == ky2).all() and (ph == ph2).all()
+    k = np.hypot(kx, ky)
+    assert (k >= 30 - 1).all() and (k <= 34 + 1).all()
+    assert (ph >= 0).all() and (ph < TWO_PI).all()
+    kx3, _, _ = O.band_modes(1001, 32.0, 2.0)
+    assert not (kx3 == kx).all()
