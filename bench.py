@@ -29,12 +29,14 @@ sys.path.insert(0, os.path.join(ROOT, "fluid-sim_amd"))
 METRIC = "cell-steps/s at 1024² to max|∇·u|≤1e-6, learned-MG vs plain MG; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic (compulsory) bytes per cell of one launch scope, FP32 fields, FP64 pressure
+# algorithmic (compulsory) bytes per cell of one launch scope: every distinct input
+# element read once and every output written once at its storage precision
+# (FP32 fields, FP64 pressure); the coarse right-hand side is 1/4 cell per cell.
 KINDS = {
-    0: ("advection (SL forward + MacCormack correction)", 16 + 24),
-    1: ("level-0 red-black colour pass", 6),
-    2: ("FP64 refinement residual", 32),
-    3: ("FP64-pressure gradient subtraction", 24),
+    0: ("advection: SL forward pass + MacCormack correction (2 launches)", 16 + 24),
+    1: ("level-0 up-sweep: prolongation + post-smoothing (k_mgt_up)", 4 + 4 + 1 + 4),
+    2: ("level-0 down-sweep fused with the FP64 residual (k_mgt_down0)", 8 + 8 + 4 + 8 + 4 + 4 + 1),
+    3: ("gradient subtraction with FP64 pressure (k_gradient64)", 8 + 8 + 8),
 }
 
 

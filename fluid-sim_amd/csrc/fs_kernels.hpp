@@ -72,15 +72,24 @@ __device__ __forceinline__ void trace(const float* __restrict__ u, const float* 
   Yb = fmaf(-cy, v2, Y);
 }
 
+// Per-step coefficients, kept on the device so a CFL time step never has to
+// visit the host: cx = dt/dx, cy = dt/dy, da = nu dt/dx^2, db = nu dt/dy^2,
+// fdt = dt (forcing amplitude factor), drag = exp(-alpha dt).
+struct StepCoef {
+  double dt;
+  float cx, cy, da, db, fdt, drag;
+};
+
 // ------------------------------------------------------------------ advection
 
 // Semi-Lagrangian self-advection of both components, each traced from its own
 // face (src/cpu/advect.js:74-97; ADVECT_VELOCITY src/gpu/shaders.js:126-146).
 __global__ void __launch_bounds__(BX * BY)
 k_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
-            float* __restrict__ ov, int nx, int ny, float cx, float cy) {
+            float* __restrict__ ov, int nx, int ny, const StepCoef* __restrict__ cf, float sign) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
+  float cx = sign * cf->cx, cy = sign * cf->cy;
   float Xb, Yb;
   trace(u, v, nx, ny, (float)i, j + 0.5f, cx, cy, Xb, Yb);
   ou[(size_t)j * nx + i] = sU(u, nx, ny, Xb, Yb);
@@ -95,9 +104,10 @@ k_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float* __r
 __global__ void __launch_bounds__(BX * BY)
 k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ fu,
             const float* __restrict__ fv, float* __restrict__ ou, float* __restrict__ ov, int nx, int ny,
-            float cx, float cy) {
+            const StepCoef* __restrict__ cf) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
+  float cx = cf->cx, cy = cf->cy;
   size_t k = (size_t)j * nx + i;
   float Xb, Yb, lo, hi;
   {  // u face at (i, j+1/2)
@@ -124,9 +134,10 @@ k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const floa
 // DIFFUSE src/gpu/shaders.js:401-425); a = nu dt/dx^2, b = nu dt/dy^2.
 __global__ void __launch_bounds__(BX * BY)
 k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
-          float* __restrict__ ov, int nx, int ny, float a, float b) {
+          float* __restrict__ ov, int nx, int ny, const StepCoef* __restrict__ cf) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
+  float a = cf->da, b = cf->db;
   int im = prv(i, nx), ip = nxt(i, nx), jm = prv(j, ny), jp = nxt(j, ny);
   size_t k = (size_t)j * nx + i;
   size_t kl = (size_t)j * nx + im, kr = (size_t)j * nx + ip, kb = (size_t)jm * nx + i, kt = (size_t)jp * nx + i;
@@ -138,14 +149,15 @@ k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __res
 
 // Band forcing then linear drag, in place (src/cpu/solver.js:200-237 semantics).
 // modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi), coef) in turns per cell;
-// coefu/coefv carry scale*ky/|k| and -scale*kx/|k|. The argument is reduced to
+// cu/cv carry amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device). The argument is reduced to
 // [0,1) turns in FP64 before cospi, so large wavenumbers keep full FP32 accuracy.
 struct Mode { double tx, ty, tph; float cu, cv; };
 constexpr int MAX_MODES = 24;
 struct ModeSet { Mode m[MAX_MODES]; int n; };
 
 __global__ void __launch_bounds__(BX * BY)
-k_forcing(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms, float drag_factor) {
+k_forcing(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms,
+          const StepCoef* __restrict__ cf) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   size_t k = (size_t)j * nx + i;
@@ -159,8 +171,9 @@ k_forcing(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const Mo
     fu = fmaf(md.cu, cospif(2.f * (float)tu), fu);
     fv = fmaf(md.cv, cospif(2.f * (float)tv), fv);
   }
-  u[k] = (u[k] + fu) * drag_factor;
-  v[k] = (v[k] + fv) * drag_factor;
+  float fdt = cf->fdt, drag = cf->drag;
+  u[k] = (u[k] + fu * fdt) * drag;
+  v[k] = (v[k] + fv * fdt) * drag;
 }
 
 // --------------------------------------------------- projection, FP32 paths
@@ -247,9 +260,10 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
-// block-wide max of a non-negative double, folded into *dst with an integer atomic
-// (non-negative IEEE doubles order like their bit patterns).
-__device__ __forceinline__ void block_max_atomic(double x, double* dst) {
+// Block-wide max of a non-negative double, stored to dst[block] (one plain store per
+// block: a single-address atomic from thousands of blocks serialises at memory).
+// NaN is turned into +inf by the callers because fmax drops NaN operands.
+__device__ __forceinline__ double block_max(double x) {
   __shared__ double sm[16];
   int lane = threadIdx.x + threadIdx.y * blockDim.x;
   int nw = (blockDim.x * blockDim.y + 63) / 64;
@@ -257,30 +271,29 @@ __device__ __forceinline__ void block_max_atomic(double x, double* dst) {
   x = wave_max(x);
   if ((lane & 63) == 0) sm[lane >> 6] = x;
   __syncthreads();
-  if (lane == 0) {
-    double m = sm[0];
-    for (int w = 1; w < nw; w++) m = fmax(m, sm[w]);
-    if (m != m) m = __longlong_as_double(0x7ff0000000000000LL);  // NaN -> +inf
-    atomicMax((unsigned long long*)dst, (unsigned long long)__double_as_longlong(m));
-  }
+  double m = sm[0];
+  for (int w = 1; w < nw; w++) m = fmax(m, sm[w]);
+  return m;
+}
+
+__device__ __forceinline__ int flat_block() { return blockIdx.x + blockIdx.y * gridDim.x; }
+__device__ __forceinline__ bool thread0() { return threadIdx.x == 0 && threadIdx.y == 0; }
+__device__ __forceinline__ double nan_to_inf(double a) {
+  return (a != a) ? __longlong_as_double(0x7ff0000000000000LL) : a;
 }
 
 // ----------------------------------------------- FP64 refinement residual
 
 // r = div(u*) - L(p + e) in FP64, p' = p + e stored FP64, r stored FP32 as the
-// multigrid right-hand side, max|r| folded into rmax[0] and rmax[1] cleared for
-// the next iteration. Skipped when the previous iteration already met tol
-// (skip != nullptr && *skip <= tol). e == nullptr means a zero correction,
-// p == nullptr a zero initial pressure. The divergence and Laplacian follow
+// multigrid right-hand side, the block's max|r| stored to rpart[block].
+// e == nullptr means a zero correction, p == nullptr a zero initial pressure. The divergence and Laplacian follow
 // src/core/grid.js:191-206 and src/cpu/projection.js:48-62.
 __global__ void __launch_bounds__(BX * BY)
 k_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
              const float* __restrict__ e, double* __restrict__ pout, float* __restrict__ r32, int nx, int ny,
-             double idx, double idy, double idx2, double idy2, double* rmax, const double* skip, double tol) {
-  if (skip && *skip <= tol) return;
+             double idx, double idy, double idx2, double idy2, double* __restrict__ rpart) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double amax = 0.0;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) rmax[1] = 0.0;
   if (i < nx && j < ny) {
     int im = prv(i, nx), ip = nxt(i, nx), jm = prv(j, ny), jp = nxt(j, ny);
     size_t k = (size_t)j * nx + i;
@@ -293,9 +306,10 @@ k_residual64(const float* __restrict__ u, const float* __restrict__ v, const dou
     double r = dv - lap;
     pout[k] = pc;
     r32[k] = (float)r;
-    amax = fabs(r);
+    amax = nan_to_inf(fabs(r));
   }
-  block_max_atomic(amax, rmax);
+  double m = block_max(amax);
+  if (thread0()) rpart[flat_block()] = m;
 }
 
 // ------------------------------------------------------------- multigrid
@@ -303,14 +317,11 @@ k_residual64(const float* __restrict__ u, const float* __restrict__ v, const dou
 // full-weighting 2x2 restriction of the residual, bilinear prolongation added
 // to the finer correction. The residual and the restriction are fused: each
 // coarse cell evaluates the residual of its four children (RESIDUAL :48-66,
-// RESTRICT :72-93) and writes their mean. Every kernel returns immediately once
-// the FP64 residual of this refinement iteration met the tolerance.
+// RESTRICT :72-93) and writes their mean.
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
-k_mg_rb(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int colour, int x_zero,
-        const double* __restrict__ conv, double tol) {
-  if (conv && *conv <= tol) return;
+k_mg_rb(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int colour, int x_zero) {
   int ip = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   int i = 2 * ip + ((j + colour) & 1);
   if (i >= n || j >= n) return;
@@ -325,8 +336,7 @@ k_mg_rb(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int colou
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
 k_mg_restrict(const T* __restrict__ x, const T* __restrict__ rhs, int n, float ih2, T* __restrict__ crhs,
-              int x_zero, const double* __restrict__ conv, double tol) {
-  if (conv && *conv <= tol) return;
+              int x_zero) {
   int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
   int cn = n >> 1;
   if (I >= cn || J >= cn) return;
@@ -350,9 +360,7 @@ k_mg_restrict(const T* __restrict__ x, const T* __restrict__ rhs, int n, float i
 
 template <typename T>
 __global__ void __launch_bounds__(BX * BY)
-k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero, const double* __restrict__ conv,
-             double tol) {
-  if (conv && *conv <= tol) return;
+k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= n || j >= n) return;
   int cn = n >> 1;
@@ -369,52 +377,12 @@ k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero, con
   x[k] = (T)((x_zero ? 0.f : (float)x[k]) + corr);
 }
 
-// Coarsest level solved by one workgroup in LDS: the rhs mean is removed (the
-// periodic compatibility condition, src/cpu/projection.js:91-102) and `sweeps`
-// red-black sweeps run from a zero start (src/gpu/multigrid.js:312-313).
-constexpr int COARSE_MAX = 64;
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_mg_coarse(T* __restrict__ x, const T* __restrict__ rhs, int n, float h2, int sweeps,
-            const double* __restrict__ conv, double tol) {
-  if (conv && *conv <= tol) return;
-  __shared__ float sx[COARSE_MAX * COARSE_MAX];
-  __shared__ float sr[COARSE_MAX * COARSE_MAX];
-  __shared__ double red[4];
-  int tid = threadIdx.x, cells = n * n;
-  double s = 0.0;
-  for (int k = tid; k < cells; k += 256) {
-    float r = (float)rhs[k];
-    sr[k] = r;
-    sx[k] = 0.f;
-    s += r;
-  }
-  s = wave_sum(s);
-  if ((tid & 63) == 0) red[tid >> 6] = s;
-  __syncthreads();
-  float mean = (float)((red[0] + red[1] + red[2] + red[3]) / cells);
-  for (int k = tid; k < cells; k += 256) sr[k] -= mean;
-  __syncthreads();
-  for (int sw = 0; sw < sweeps; sw++) {
-    for (int colour = 0; colour < 2; colour++) {
-      for (int q = tid; q < cells; q += 256) {
-        int j = q / n, i = q - j * n;
-        if (((i + j) & 1) != colour) continue;
-        float sum = sx[j * n + prv(i, n)] + sx[j * n + nxt(i, n)] + sx[prv(j, n) * n + i] + sx[nxt(j, n) * n + i];
-        sx[j * n + i] = (sum - h2 * sr[j * n + i]) * 0.25f;
-      }
-      __syncthreads();
-    }
-  }
-  for (int k = tid; k < cells; k += 256) x[k] = (T)sx[k];
-}
-
-// u = u* - grad p with FP64 pressure, stored FP32; max|u|, max|v| of the result
-// are folded into red[0], red[1] for the CFL time step of the next step.
+// u = u* - grad p with FP64 pressure, stored FP32; the block maxima of |u|, |v|
+// of the result go to part[2*block + {0,1}] for the CFL time step of the next step.
 __global__ void __launch_bounds__(BX * BY)
 k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
              float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, double idx, double idy,
-             double* __restrict__ red) {
+             double* __restrict__ part) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double mu = 0.0, mv = 0.0;
   if (i < nx && j < ny) {
@@ -424,37 +392,79 @@ k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const dou
     float nv_ = (float)((double)v[k] - (pc - p[(size_t)prv(j, ny) * nx + i]) * idy);
     ou[k] = nu_;
     ov[k] = nv_;
-    mu = fabs((double)nu_);
-    mv = fabs((double)nv_);
+    mu = nan_to_inf(fabs((double)nu_));
+    mv = nan_to_inf(fabs((double)nv_));
   }
-  block_max_atomic(mu, red + 0);
-  block_max_atomic(mv, red + 1);
+  double a = block_max(mu), b = block_max(mv);
+  if (thread0()) {
+    part[2 * flat_block()] = a;
+    part[2 * flat_block() + 1] = b;
+  }
 }
 
-// max|u|, max|v| of a field (for a CFL step before any projection ran).
+// block maxima of |u|, |v| of a field (for a CFL step before any projection ran).
 __global__ void __launch_bounds__(BX * BY)
-k_maxabs(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double* __restrict__ red) {
+k_maxabs(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double* __restrict__ part) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double mu = 0.0, mv = 0.0;
   if (i < nx && j < ny) {
     size_t k = (size_t)j * nx + i;
-    mu = fabs((double)u[k]);
-    mv = fabs((double)v[k]);
+    mu = nan_to_inf(fabs((double)u[k]));
+    mv = nan_to_inf(fabs((double)v[k]));
   }
-  block_max_atomic(mu, red + 0);
-  block_max_atomic(mv, red + 1);
+  double a = block_max(mu), b = block_max(mv);
+  if (thread0()) {
+    part[2 * flat_block()] = a;
+    part[2 * flat_block() + 1] = b;
+  }
 }
 
-// dt = cfl / (max|u|/dx + max|v|/dy) (src/core/grid.js:322-327), written to
-// dt_out; the accumulators are cleared for the next projection.
-__global__ void k_cfl_dt(double* __restrict__ red, double cfl, double idx, double idy, double dt_max,
-                         double* __restrict__ dt_out) {
-  double rate = red[0] * idx + red[1] * idy;
-  double dt = rate > 0 ? cfl / rate : dt_max;
-  if (!(dt <= dt_max)) dt = dt_max;
-  *dt_out = dt;
-  red[0] = 0.0;
-  red[1] = 0.0;
+__device__ __forceinline__ void fill_coef(StepCoef* cf, double dt, double dx, double dy, double nu, double drag) {
+  cf->dt = dt;
+  cf->cx = (float)(dt / dx);
+  cf->cy = (float)(dt / dy);
+  cf->da = (float)(nu * dt / (dx * dx));
+  cf->db = (float)(nu * dt / (dy * dy));
+  cf->fdt = (float)dt;
+  cf->drag = drag > 0 ? (float)exp(-drag * dt) : 1.0f;
+}
+
+// Coefficients of a step with a dt chosen on the host.
+__global__ void k_set_coef(StepCoef* cf, double dt, double dx, double dy, double nu, double drag) {
+  if (threadIdx.x == 0) fill_coef(cf, dt, dx, dy, nu, drag);
+}
+
+// dt = cfl / (max|u|/dx + max|v|/dy) (src/core/grid.js:322-327) from the block
+// maxima part[2*b + {0,1}] of the last gradient (or maxabs) pass; one workgroup.
+__global__ void __launch_bounds__(1024)
+k_cfl_dt(const double* __restrict__ part, int nparts, double cfl, double dx, double dy, double nu, double drag,
+         double dt_max, StepCoef* __restrict__ cf) {
+  __shared__ double su[16], sv[16];
+  double mu = 0.0, mv = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 1024) {
+    mu = fmax(mu, part[2 * b]);
+    mv = fmax(mv, part[2 * b + 1]);
+  }
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  if ((threadIdx.x & 63) == 0) {
+    su[threadIdx.x >> 6] = mu;
+    sv[threadIdx.x >> 6] = mv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; w++) {
+      mu = fmax(mu, su[w]);
+      mv = fmax(mv, sv[w]);
+    }
+    mu = fmax(mu, su[0]);
+    mv = fmax(mv, sv[0]);
+    double rate = mu / dx + mv / dy;
+    double dt;
+    if (!(rate <= 1.7976931348623157e308)) dt = __longlong_as_double(0x7ff8000000000000LL);  // non-finite field
+    else dt = rate > 0 ? fmin(cfl / rate, dt_max) : dt_max;
+    fill_coef(cf, dt, dx, dy, nu, drag);
+  }
 }
 
 // Per-block FP64 diagnostics of the stored FP32 field (src/core/grid.js:290-319):
@@ -476,8 +486,8 @@ k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
     d2 = dv * dv;
     su = uc;
     sv = vc;
-    mu = fabs(uc);
-    mv = fabs(vc);
+    mu = nan_to_inf(fabs(uc));
+    mv = nan_to_inf(fabs(vc));
     nf = (isfinite(uc) ? 0 : 1) + (isfinite(vc) ? 0 : 1);
     if (md != md) md = __longlong_as_double(0x7ff0000000000000LL);
   }

@@ -3,7 +3,9 @@
 // One solver owns one HIP stream (default priority) and all of its device
 // memory; fs_step only enqueues. The step follows src/gpu/solver.js:518-591
 // (advect -> diffuse -> force/drag -> project) with the CPU forcing semantics
-// of src/cpu/solver.js:353-358 (force, then drag).
+// of src/cpu/solver.js:353-358 (force, then drag). The per-step coefficients
+// (dt, dt/dx, nu dt/dx^2, drag factor) live on the device, so a CFL time step
+// is computed and consumed there without a host round trip.
 //
 // Projection modes
 //   jacobi, red-black   FP32 pressure, fixed sweep count, p zeroed per step
@@ -12,8 +14,13 @@
 //                       V-cycle on the FP32 correction e (L e = r), p += e in
 //                       FP64. A V-cycle is affine in its start value, so k
 //                       refinement iterations equal the reference's k V-cycles
-//                       on p (src/gpu/solver.js:686-703) up to rounding; with a
-//                       tolerance the loop stops once max|r| <= tol.
+//                       on p (src/gpu/solver.js:686-703) up to rounding.
+//                       With a tolerance, a predicted number of V-cycles is
+//                       enqueued, the per-block residual maxima of every
+//                       iteration are read back once, and further V-cycles are
+//                       added only if the last residual is still above the gate.
+//                       Extra V-cycles only lower the residual, so an
+//                       over-prediction costs time, never accuracy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +33,7 @@
 
 #include "../../include/fluidsim.h"
 #include "fs_kernels.hpp"
+#include "fs_mg_fused.hpp"
 
 namespace {
 
@@ -42,27 +50,28 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                              \
-  do {                                                                                            \
-    hipError_t e_ = (expr);                                                                       \
-    if (e_ != hipSuccess)                                                                         \
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
       return fail(e_ == hipErrorOutOfMemory ? FS_E_OOM : FS_E_DEVICE, "%s failed: %s (%s:%d)", #expr, \
-                  hipGetErrorString(e_), __FILE__, __LINE__);                                    \
+                  hipGetErrorString(e_), __FILE__, __LINE__);                                         \
   } while (0)
 
-#define LAUNCHCHK()                                                                              \
-  do {                                                                                           \
-    hipError_t e_ = hipGetLastError();                                                           \
-    if (e_ != hipSuccess)                                                                        \
+#define LAUNCHCHK()                                                                                  \
+  do {                                                                                               \
+    hipError_t e_ = hipGetLastError();                                                               \
+    if (e_ != hipSuccess)                                                                            \
       return fail(FS_E_DEVICE, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, \
-                  __LINE__);                                                                     \
+                  __LINE__);                                                                         \
   } while (0)
 
 struct Level {
   int n = 0;
   double dx = 0;
-  float* x = nullptr;    // correction (level 0: e)
+  float* x = nullptr;    // correction (level 0: e); fused path: after pre-smoothing
   float* rhs = nullptr;  // right-hand side (level 0: r)
+  float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e)
 };
 
 struct Profile {
@@ -72,6 +81,8 @@ struct Profile {
   double total_ms = 0;
   int64_t launches = 0;
 };
+
+constexpr int DT_RING = 4096;  // steps whose dt may be pending a host read
 
 }  // namespace
 
@@ -89,20 +100,28 @@ struct fs_solver_t {
   float* fv = nullptr;
   float* div = nullptr;
   float* p32[2] = {nullptr, nullptr};
+  int p32idx = 0;            // p32 buffer holding the last flat-solver pressure
   double* p64[2] = {nullptr, nullptr};
-  int pidx = 0;             // p64 buffer holding the last projection's pressure
-  bool have_p = false;      // a previous pressure exists (warm start)
-  std::vector<Level> lv;    // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
-  double* scal = nullptr;   // red[4] followed by rmax[rmax_len], cleared by one fill
-  double* red = nullptr;    // scal + 0: [0] max|u|, [1] max|v|, [2] dt
-  double* rmax = nullptr;   // scal + 4: per-iteration FP64 residual maxima
-  int rmax_len = 0;
-  int p32idx = 0;           // p32 buffer holding the last flat-solver pressure
+  int pidx = 0;              // p64 buffer holding the last projection's pressure
+  bool have_p = false;       // a previous pressure exists (warm start)
+  std::vector<Level> lv;     // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
+  bool fused = false;        // LDS-tiled V-cycle in use
+  int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
+  // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
+  double* rpart = nullptr;
+  double* rpart_host = nullptr;  // pinned mirror
+  int rp_rows = 0, rp_cols = 0;
+  int resid_row_pending = -1;    // fixed-count projection whose residual is not yet read
+  // |u|, |v| block maxima of the last gradient pass, for the CFL step
+  double* mpart = nullptr;
+  int mp_count = 0;
+  bool have_maxabs = false;
+  fsk::StepCoef* coef = nullptr;  // device step coefficients
+  double* dt_ring = nullptr;      // pinned: dt of steps not yet folded into t
+  int dt_pending = 0;
   double* diag_part = nullptr;
   double* diag_out = nullptr;
-  int diag_blocks = 0;
-  double* host = nullptr;   // pinned staging for scalars
-  bool have_maxabs = false; // red[0..1] hold max|u|, max|v| of the current field
+  double* host = nullptr;         // pinned staging for small reads
   // forcing
   bool forcing_on = false;
   fs_forcing forcing{};
@@ -113,7 +132,7 @@ struct fs_solver_t {
   int64_t total_vcycles = 0;
   double last_residual = 0;
   double last_dt = 0;
-  int last_pred = 2;        // speculative V-cycle count for the next projection
+  int last_pred = 2;  // predicted V-cycles for the next projection
   Profile prof;
 };
 
@@ -126,21 +145,39 @@ inline dim3 grid_pairs(int nx, int ny) {
   int half = (nx + 1) / 2;
   return dim3((unsigned)((half + fsk::BX - 1) / fsk::BX), (unsigned)((ny + fsk::BY - 1) / fsk::BY));
 }
+inline int grid_blocks(int nx, int ny) {
+  dim3 g = grid2(nx, ny);
+  return (int)(g.x * g.y);
+}
 const dim3 BLK(fsk::BX, fsk::BY);
 
 template <typename T>
 int dalloc(T** p, size_t count) {
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T) > 0 ? count * sizeof(T) : 1);
+  size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 1;
+  hipError_t e = hipMalloc((void**)p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
-    return fail(e == hipErrorOutOfMemory ? FS_E_OOM : FS_E_DEVICE, "hipMalloc(%zu bytes) failed: %s",
-                count * sizeof(T), hipGetErrorString(e));
+    return fail(e == hipErrorOutOfMemory ? FS_E_OOM : FS_E_DEVICE, "hipMalloc(%zu bytes) failed: %s", bytes,
+                hipGetErrorString(e));
+  }
+  return FS_OK;
+}
+
+template <typename T>
+int halloc(T** p, size_t count) {
+  size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 1;
+  if (hipHostMalloc((void**)p, bytes, hipHostMallocDefault) != hipSuccess) {
+    *p = nullptr;
+    return fail(FS_E_OOM, "hipHostMalloc(%zu bytes) failed", bytes);
   }
   return FS_OK;
 }
 
 void dfree(void* p) {
   if (p) (void)hipFree(p);
+}
+void hfree(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 
 // event pair around launches of the profiled kernel class
@@ -171,7 +208,8 @@ struct ProfScope {
   }
 };
 
-void prof_collect(fs_solver_t* s) {
+// Everything the host learns lazily after a stream synchronisation.
+void after_sync(fs_solver_t* s) {
   for (auto& pr : s->prof.pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
@@ -182,6 +220,25 @@ void prof_collect(fs_solver_t* s) {
     s->prof.pool.push_back(pr.second);
   }
   s->prof.pending.clear();
+  for (int k = 0; k < s->dt_pending; k++) {
+    s->t += s->dt_ring[k];
+    s->last_dt = s->dt_ring[k];
+  }
+  s->dt_pending = 0;
+  if (s->resid_row_pending >= 0) {
+    // the fixed-count residual row was copied before the synchronisation
+    double m = 0;
+    const double* row = s->rpart_host + (size_t)s->resid_row_pending * s->rp_cols;
+    for (int b = 0; b < s->rp_cols; b++) m = std::max(m, row[b]);
+    s->last_residual = m;
+    s->resid_row_pending = -1;
+  }
+}
+
+int sync_stream(fs_solver_t* s) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  after_sync(s);
+  return FS_OK;
 }
 
 int check_solver(void* h) {
@@ -194,34 +251,62 @@ int set_device(fs_solver_t* s) {
   return FS_OK;
 }
 
+int fill_f32(fs_solver_t* s, float* a, size_t n, float val) {
+  fsk::k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(a, n, val);
+  LAUNCHCHK();
+  return FS_OK;
+}
+int fill_f64(fs_solver_t* s, double* a, size_t n, double val) {
+  fsk::k_fill_f64<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(a, n, val);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+double drag_alpha(const fs_solver_t* s) { return s->forcing_on ? s->forcing.drag : 0.0; }
+
 // ------------------------------------------------------------- step pieces
 
-int enqueue_advection(fs_solver_t* s, double dt) {
+int enqueue_coef_fixed(fs_solver_t* s, double dt) {
+  fsk::k_set_coef<<<1, 64, 0, s->stream>>>(s->coef, dt, s->dx, s->dy, s->cfg.nu, drag_alpha(s));
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+int enqueue_coef_cfl(fs_solver_t* s, double cfl) {
+  if (!s->have_maxabs) {
+    fsk::k_maxabs<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[s->cur], s->v[s->cur], s->nx, s->ny, s->mpart);
+    LAUNCHCHK();
+  }
+  fsk::k_cfl_dt<<<1, 1024, 0, s->stream>>>(s->mpart, s->mp_count, cfl, s->dx, s->dy, s->cfg.nu, drag_alpha(s), 1e30,
+                                           s->coef);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+int enqueue_advection(fs_solver_t* s) {
   ProfScope ps(s, 0);
-  float cx = (float)(dt / s->dx), cy = (float)(dt / s->dy);
   int c = s->cur;
   if (s->cfg.advection == FS_ADV_MACCORMACK) {
     fsk::k_advect_sl<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->fu, s->fv, s->nx, s->ny,
-                                                                   cx, cy);
+                                                                   s->coef, 1.0f);
     LAUNCHCHK();
     fsk::k_advect_mc<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->fu, s->fv, s->u[1 - c],
-                                                                   s->v[1 - c], s->nx, s->ny, cx, cy);
+                                                                   s->v[1 - c], s->nx, s->ny, s->coef);
     LAUNCHCHK();
   } else {
     fsk::k_advect_sl<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->u[1 - c], s->v[1 - c],
-                                                                   s->nx, s->ny, cx, cy);
+                                                                   s->nx, s->ny, s->coef, 1.0f);
     LAUNCHCHK();
   }
   s->cur = 1 - c;
   return FS_OK;
 }
 
-int enqueue_diffusion(fs_solver_t* s, double dt) {
+int enqueue_diffusion(fs_solver_t* s) {
   if (!(s->cfg.nu > 0)) return FS_OK;
-  float a = (float)(s->cfg.nu * dt / (s->dx * s->dx)), b = (float)(s->cfg.nu * dt / (s->dy * s->dy));
   int c = s->cur;
   fsk::k_diffuse<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->u[1 - c], s->v[1 - c], s->nx,
-                                                               s->ny, a, b);
+                                                               s->ny, s->coef);
   LAUNCHCHK();
   s->cur = 1 - c;
   return FS_OK;
@@ -233,8 +318,9 @@ double js_round(double x) {
   return r;
 }
 
-// 24 modes of src/cpu/solver.js:181-199 from the exact uint32 LCG.
-int enqueue_forcing(fs_solver_t* s, double dt) {
+// 24 modes of src/cpu/solver.js:181-199 from the exact uint32 LCG; the amplitude
+// factor dt and the drag factor come from the device coefficients.
+int enqueue_forcing(fs_solver_t* s) {
   if (!s->forcing_on) return FS_OK;
   const fs_forcing& f = s->forcing;
   uint32_t st = f.seed + (uint32_t)s->steps;
@@ -244,7 +330,7 @@ int enqueue_forcing(fs_solver_t* s, double dt) {
   };
   fsk::ModeSet ms{};
   const int M = 24;
-  double scale = f.amp * dt / std::sqrt((double)M);
+  double scale = f.amp / std::sqrt((double)M);
   int k = 0;
   for (int m = 0; m < M; m++) {
     double theta = 2 * M_PI * rnd();
@@ -262,20 +348,8 @@ int enqueue_forcing(fs_solver_t* s, double dt) {
     md.cv = (float)(scale * (-kx / kk));
   }
   ms.n = k;
-  float drag = f.drag > 0 ? (float)std::exp(-f.drag * dt) : 1.0f;
   int c = s->cur;
-  fsk::k_forcing<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, ms, drag);
-  LAUNCHCHK();
-  return FS_OK;
-}
-
-int fill_f32(fs_solver_t* s, float* a, size_t n, float val) {
-  fsk::k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(a, n, val);
-  LAUNCHCHK();
-  return FS_OK;
-}
-int fill_f64(fs_solver_t* s, double* a, size_t n, double val) {
-  fsk::k_fill_f64<<<(unsigned)((n + 255) / 256), 256, 0, s->stream>>>(a, n, val);
+  fsk::k_forcing<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, ms, s->coef);
   LAUNCHCHK();
   return FS_OK;
 }
@@ -288,7 +362,7 @@ int enqueue_divergence(fs_solver_t* s) {
   return FS_OK;
 }
 
-// Jacobi / red-black with FP32 pressure; result in p32[result index], returns index via out.
+// Jacobi / red-black with FP32 pressure; the result index comes back in out_idx.
 int solve_flat(fs_solver_t* s, int iterations, int* out_idx) {
   float h2 = (float)(s->dx * s->dx);
   int rc = fill_f32(s, s->p32[0], s->n, 0.f);
@@ -315,15 +389,26 @@ int solve_flat(fs_solver_t* s, int iterations, int* out_idx) {
   return FS_OK;
 }
 
-// Smooth `sweeps` red-black sweeps on level l; x_zero marks a logically-zero start.
-int mg_smooth(fs_solver_t* s, int l, int sweeps, bool& x_zero, const double* conv, double tol) {
+// ---------------------------------------------------- per-operator V-cycle
+
+fsk::LevelSpacing level_spacing(const fs_solver_t* s, int from) {
+  fsk::LevelSpacing sp{};
+  for (int l = from; l < (int)s->lv.size() && l - from < fsk::CML_MAXL; l++) {
+    double dx = s->lv[l].dx;
+    sp.h2[l - from] = (float)(dx * dx);
+    sp.ih2[l - from] = (float)(1.0 / (dx * dx));
+  }
+  return sp;
+}
+
+// `sweeps` red-black sweeps on level l; x_zero marks a logically-zero start.
+int mg_smooth(fs_solver_t* s, int l, int sweeps, bool& x_zero) {
   Level& L = s->lv[l];
   float h2 = (float)(L.dx * L.dx);
   for (int k = 0; k < sweeps; k++) {
     for (int colour = 0; colour < 2; colour++) {
       ProfScope ps(s, l == 0 ? 1 : -2);
-      fsk::k_mg_rb<float><<<grid_pairs(L.n, L.n), BLK, 0, s->stream>>>(L.x, L.rhs, L.n, h2, colour, x_zero ? 1 : 0,
-                                                                        conv, tol);
+      fsk::k_mg_rb<float><<<grid_pairs(L.n, L.n), BLK, 0, s->stream>>>(L.x, L.rhs, L.n, h2, colour, x_zero ? 1 : 0);
       LAUNCHCHK();
       // after the red pass from zero, black cells are still stale but the black
       // pass never reads its own value, and red cells are now valid
@@ -333,198 +418,273 @@ int mg_smooth(fs_solver_t* s, int l, int sweeps, bool& x_zero, const double* con
   return FS_OK;
 }
 
-// One V-cycle on the correction (lv[0].x = e, lv[0].rhs = r), predicated on conv.
-int mg_vcycle(fs_solver_t* s, const double* conv, double tol) {
+// One V-cycle on the correction (lv[0].x = e, lv[0].rhs = r), one launch per operator.
+int mg_vcycle(fs_solver_t* s) {
   int Lc = (int)s->lv.size();
   std::vector<char> zero(Lc, 1);  // every level's correction starts from zero
   for (int l = 0; l < Lc - 1; l++) {
     bool z = zero[l];
-    int rc = mg_smooth(s, l, s->cfg.mg_pre, z, conv, tol);
+    int rc = mg_smooth(s, l, s->cfg.mg_pre, z);
     if (rc) return rc;
     zero[l] = z;
     Level& F = s->lv[l];
     Level& Cl = s->lv[l + 1];
-    fsk::k_mg_restrict<float><<<grid2(Cl.n, Cl.n), BLK, 0, s->stream>>>(
-        F.x, F.rhs, F.n, (float)(1.0 / (F.dx * F.dx)), Cl.rhs, zero[l] ? 1 : 0, conv, tol);
+    fsk::k_mg_restrict<float><<<grid2(Cl.n, Cl.n), BLK, 0, s->stream>>>(F.x, F.rhs, F.n, (float)(1.0 / (F.dx * F.dx)),
+                                                                         Cl.rhs, zero[l] ? 1 : 0);
     LAUNCHCHK();
   }
   {
     Level& C = s->lv[Lc - 1];
-    float h2 = (float)(C.dx * C.dx);
-    if (C.n <= fsk::COARSE_MAX) {
-      fsk::k_mg_coarse<float><<<1, 256, 0, s->stream>>>(C.x, C.rhs, C.n, h2, s->cfg.mg_coarse, conv, tol);
+    if (C.n <= fsk::COARSE_N) {
+      fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.x, C.rhs, C.n, 1, level_spacing(s, Lc - 1),
+                                                         s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse);
       LAUNCHCHK();
     } else {
       bool z = true;
-      int rc = mg_smooth(s, Lc - 1, s->cfg.mg_coarse, z, conv, tol);
+      int rc = mg_smooth(s, Lc - 1, s->cfg.mg_coarse, z);
       if (rc) return rc;
       if (z) {  // zero sweeps: the coarse correction is zero
-        int rc2 = fill_f32(s, C.x, (size_t)C.n * C.n, 0.f);
-        if (rc2) return rc2;
+        rc = fill_f32(s, C.x, (size_t)C.n * C.n, 0.f);
+        if (rc) return rc;
       }
     }
   }
   for (int l = Lc - 2; l >= 0; l--) {
     Level& F = s->lv[l];
     Level& Cl = s->lv[l + 1];
-    fsk::k_mg_prolong<float><<<grid2(F.n, F.n), BLK, 0, s->stream>>>(Cl.x, F.x, F.n, zero[l] ? 1 : 0, conv, tol);
+    fsk::k_mg_prolong<float><<<grid2(F.n, F.n), BLK, 0, s->stream>>>(Cl.x, F.x, F.n, zero[l] ? 1 : 0);
     LAUNCHCHK();
     bool z = false;
-    int rc = mg_smooth(s, l, s->cfg.mg_post, z, conv, tol);
+    int rc = mg_smooth(s, l, s->cfg.mg_post, z);
     if (rc) return rc;
   }
   return FS_OK;
 }
 
-// FP64 residual of iteration `it`: reads p64[pin] (+ e if use_e), writes p64[1-pin], r.
-int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, bool use_e, double tol) {
-  ProfScope ps(s, 2);
+// ----------------------------------------------------------- fused V-cycle
+
+inline dim3 tiles_grid(int n, int* tiles_x, int* nblocks) {
+  int tx = (n + fsk::TX - 1) / fsk::TX, ty = (n + fsk::TY - 1) / fsk::TY;
+  *tiles_x = tx;
+  *nblocks = tx * ty;
+  return dim3((unsigned)(tx * ty));
+}
+
+template <int PRE>
+int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+  Level& L0 = s->lv[0];
+  int tx, nb;
+  dim3 g = tiles_grid(L0.n, &tx, &nb);
+  double idx = 1.0 / s->dx;
   int c = s->cur;
-  const double* skip = it > 0 ? s->rmax + (it - 1) : nullptr;
-  double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  fsk::k_residual64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
-      s->u[c], s->v[c], zero_p ? nullptr : s->p64[pin], use_e ? s->lv[0].x : nullptr, s->p64[1 - pin],
-      s->lv[0].rhs, s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rmax + it, skip, tol);
+  fsk::k_mgt_down0<PRE><<<g, fsk::NT, 0, s->stream>>>(
+      s->u[c], s->v[c], zero_p ? nullptr : s->p64[pin], it > 0 ? L0.xb : nullptr, s->p64[1 - pin], L0.rhs, L0.x,
+      s->lv[1].rhs, L0.n, idx, idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx, nb,
+      s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
   LAUNCHCHK();
   return FS_OK;
 }
 
-// Multigrid projection by iterative refinement. Returns the V-cycle count via *cycles
-// and leaves the final pressure in p64[s->pidx].
-int solve_multigrid(fs_solver_t* s, int* cycles_out, double* resid_out) {
-  const double tol_cfg = s->cfg.tolerance;
-  const bool tol_mode = tol_cfg > 0;
-  const double tol = tol_mode ? tol_cfg : -1.0;  // -1: never converged (fixed count)
-  const int maxc = std::max(0, s->cfg.pressure_iterations);
-  // clears max|u|, max|v| for the gradient kernel and every residual slot
-  int rc = fill_f64(s, s->scal, (size_t)s->rmax_len + 4, 0.0);
-  if (rc) return rc;
-  bool zero_p = !(s->cfg.guess == FS_GUESS_PREVIOUS && s->have_p);
-  int a0 = s->pidx;
-  int it = 0;
-  int cycles = 0;
-  if (!tol_mode) {
-    // k V-cycles: residual(0), V(0), residual(1), ..., V(k-1), residual(k)
-    for (it = 0; it <= maxc; it++) {
-      int pin = (a0 + it) & 1;
-      rc = enqueue_residual(s, it, pin, it == 0 && zero_p, it > 0, tol);
-      if (rc) return rc;
-      if (it < maxc) {
-        rc = mg_vcycle(s, s->rmax + it, tol);
-        if (rc) return rc;
-      }
-    }
-    cycles = maxc;
-    s->pidx = (a0 + maxc + 1) & 1;
-    HIPCHK(hipMemcpyAsync(s->host, s->rmax + maxc, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    prof_collect(s);
-    *resid_out = s->host[0];
-  } else {
-    // speculate last_pred iterations, then check; continue one at a time
-    int done_at = -1;
-    int batch = std::max(1, std::min(s->last_pred, maxc + 1));
-    while (done_at < 0) {
-      int start = it;
-      for (; it < start + batch && it <= maxc; it++) {
-        int pin = (a0 + it) & 1;
-        rc = enqueue_residual(s, it, pin, it == 0 && zero_p, it > 0, tol);
-        if (rc) return rc;
-        if (it < maxc) {
-          rc = mg_vcycle(s, s->rmax + it, tol);
-          if (rc) return rc;
-        }
-      }
-      HIPCHK(hipMemcpyAsync(s->host, s->rmax, sizeof(double) * it, hipMemcpyDeviceToHost, s->stream));
-      HIPCHK(hipStreamSynchronize(s->stream));
-      prof_collect(s);
-      for (int k = 0; k < it; k++) {
-        double r = s->host[k];
-        if (std::isnan(r) || std::isinf(r)) {
-          return fail(FS_E_NONFINITE, "non-finite residual in projection (iteration %d)", k);
-        }
-        if (r <= tol) {
-          done_at = k;
-          break;
-        }
-      }
-      if (done_at < 0 && it > maxc) {
-        s->pidx = (a0 + maxc + 1) & 1;
-        s->last_vcycles = maxc;
-        *cycles_out = maxc;
-        *resid_out = s->host[maxc];
-        s->last_pred = maxc + 1;
-        return fail(FS_E_NOT_CONVERGED, "multigrid did not reach tolerance %.3g in %d V-cycles (residual %.3g)",
-                    tol_cfg, maxc, s->host[maxc]);
-      }
-      batch = 1;
-    }
-    cycles = done_at;
-    s->pidx = (a0 + done_at + 1) & 1;
-    *resid_out = s->host[done_at];
-    s->last_pred = done_at + 1;
-  }
-  *cycles_out = cycles;
+template <int PRE>
+int launch_down(fs_solver_t* s, int l) {
+  Level& F = s->lv[l];
+  int tx, nb;
+  dim3 g = tiles_grid(F.n, &tx, &nb);
+  fsk::k_mgt_down<PRE><<<g, fsk::NT, 0, s->stream>>>(F.x, F.rhs, F.n, (float)(F.dx * F.dx),
+                                                      (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
+  LAUNCHCHK();
   return FS_OK;
 }
 
-int enqueue_projection(fs_solver_t* s, int* cycles, double* resid) {
-  *cycles = 0;
-  *resid = 0;
+template <int POST>
+int launch_up(fs_solver_t* s, int l) {
+  Level& F = s->lv[l];
+  int tx, nb;
+  dim3 g = tiles_grid(F.n, &tx, &nb);
+  fsk::k_mgt_up<POST><<<g, fsk::NT, 0, s->stream>>>(F.x, F.xb, F.rhs, s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx),
+                                                     tx, nb);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+int launch_coarse_ml(fs_solver_t* s, int from) {
+  Level& C = s->lv[from];
+  fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.xb, C.rhs, C.n, (int)s->lv.size() - from,
+                                                     level_spacing(s, from), s->cfg.mg_pre, s->cfg.mg_post,
+                                                     s->cfg.mg_coarse);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+#define DISPATCH_SW(sw, fn, ...) \
+  ((sw) == 1 ? fn<1>(__VA_ARGS__) : (sw) == 2 ? fn<2>(__VA_ARGS__) : fn<3>(__VA_ARGS__))
+
+// FP64 residual `it` on the per-operator path (and the fused path when the whole
+// hierarchy fits one workgroup): reads p64[pin] (+ e), writes p64[1-pin] and r.
+int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
+  ProfScope ps(s, 2);
   int c = s->cur;
-  if (s->cfg.pressure_solver == FS_P_MULTIGRID) {
-    int rc = solve_multigrid(s, cycles, resid);
-    if (rc && rc != FS_E_NOT_CONVERGED) return rc;
-    s->have_p = true;
-    {
-      ProfScope ps(s, 3);
-      double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-      fsk::k_gradient64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p64[s->pidx],
-                                                                      s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
-                                                                      idy, s->red);
-      LAUNCHCHK();
-    }
-    s->have_maxabs = true;
-    s->cur = 1 - c;
-    return rc;
+  double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
+  fsk::k_residual64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
+      s->u[c], s->v[c], zero_p ? nullptr : s->p64[pin], e, s->p64[1 - pin], s->lv[0].rhs, s->nx, s->ny, idx, idy,
+      idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+// Residual `it`, then (unless residual_only) the V-cycle on its correction.
+int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+  int rc;
+  if (!s->fused) {
+    if ((rc = enqueue_residual(s, it, pin, zero_p, it > 0 ? s->lv[0].x : nullptr))) return rc;
+    return residual_only ? FS_OK : mg_vcycle(s);
   }
-  int rc = enqueue_divergence(s);
-  if (rc) return rc;
-  int pi = 0;
-  rc = solve_flat(s, s->cfg.pressure_iterations, &pi);
-  if (rc) return rc;
-  s->p32idx = pi;
+  int cs = s->coarse_start;
+  if (cs == 0) {
+    if ((rc = enqueue_residual(s, it, pin, zero_p, it > 0 ? s->lv[0].xb : nullptr))) return rc;
+    return residual_only ? FS_OK : launch_coarse_ml(s, 0);
+  }
+  int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
   {
-    ProfScope ps(s, 3);
-    fsk::k_gradient32<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p32[pi], s->u[1 - c],
-                                                                    s->v[1 - c], s->nx, s->ny,
-                                                                    (float)(1.0 / s->dx), (float)(1.0 / s->dy));
-    LAUNCHCHK();
+    ProfScope ps(s, 2);
+    if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only))) return rc;
   }
-  s->have_maxabs = false;
+  if (residual_only) return FS_OK;
+  for (int l = 1; l < cs; l++)
+    if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
+  if ((rc = launch_coarse_ml(s, cs))) return rc;
+  for (int l = cs - 1; l >= 0; l--) {
+    ProfScope ps(s, l == 0 ? 1 : -2);
+    if ((rc = DISPATCH_SW(post, launch_up, s, l))) return rc;
+  }
+  return FS_OK;
+}
+
+// Read back residual rows [0, rows) and return the row maxima.
+int read_residuals(fs_solver_t* s, int rows, std::vector<double>& rmax) {
+  HIPCHK(hipMemcpyAsync(s->rpart_host, s->rpart, sizeof(double) * (size_t)rows * s->rp_cols,
+                        hipMemcpyDeviceToHost, s->stream));
+  int rc = sync_stream(s);
+  if (rc) return rc;
+  rmax.assign(rows, 0.0);
+  for (int k = 0; k < rows; k++) {
+    const double* row = s->rpart_host + (size_t)k * s->rp_cols;
+    double m = 0;
+    for (int b = 0; b < s->rp_cols; b++) m = std::max(m, row[b]);
+    rmax[k] = m;
+  }
+  return FS_OK;
+}
+
+// Multigrid projection by iterative refinement; leaves the pressure in p64[s->pidx].
+int solve_multigrid(fs_solver_t* s, int* cycles_out) {
+  const double tol = s->cfg.tolerance;
+  const int maxc = std::max(0, s->cfg.pressure_iterations);
+  const bool zero_p = !(s->cfg.guess == FS_GUESS_PREVIOUS && s->have_p);
+  const int a0 = s->pidx;
+  int rc;
+  if (!(tol > 0)) {
+    // k V-cycles: residual(0), V(0), ..., residual(k-1), V(k-1), residual(k); no host round trip
+    for (int it = 0; it <= maxc; it++)
+      if ((rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, it == maxc))) return rc;
+    HIPCHK(hipMemcpyAsync(s->rpart_host + (size_t)maxc * s->rp_cols, s->rpart + (size_t)maxc * s->rp_cols,
+                          sizeof(double) * s->rp_cols, hipMemcpyDeviceToHost, s->stream));
+    s->resid_row_pending = maxc;
+    s->pidx = (a0 + maxc + 1) & 1;
+    *cycles_out = maxc;
+    return FS_OK;
+  }
+  // predicted V-cycles, then one read-back; add V-cycles one at a time if needed
+  int c = std::max(0, std::min(s->last_pred, maxc));
+  for (int it = 0; it < c; it++)
+    if ((rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, false))) return rc;
+  if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, c == 0 && zero_p, true))) return rc;
+  std::vector<double> rmax;
+  for (;;) {
+    if ((rc = read_residuals(s, c + 1, rmax))) return rc;
+    double last = rmax[c];
+    if (std::isnan(last) || std::isinf(last)) {
+      s->pidx = (a0 + c + 1) & 1;
+      *cycles_out = c;
+      s->last_residual = last;
+      return fail(FS_E_NONFINITE, "non-finite residual in the projection (V-cycle %d)", c);
+    }
+    if (last <= tol || c >= maxc) {
+      int need = c;
+      for (int k = 0; k <= c; k++)
+        if (rmax[k] <= tol) {
+          need = k;
+          break;
+        }
+      s->last_pred = need;
+      s->pidx = (a0 + c + 1) & 1;
+      s->last_residual = last;
+      *cycles_out = c;
+      if (last > tol)
+        return fail(FS_E_NOT_CONVERGED, "multigrid did not reach tolerance %.3g in %d V-cycles (residual %.3g)",
+                    tol, maxc, last);
+      return FS_OK;
+    }
+    // residual c was enqueued residual-only: rerun it with its V-cycle, then probe c+1
+    if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, c == 0 && zero_p, false))) return rc;
+    c++;
+    if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, false, true))) return rc;
+  }
+}
+
+int enqueue_gradient64(fs_solver_t* s) {
+  ProfScope ps(s, 3);
+  int c = s->cur;
+  double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
+  fsk::k_gradient64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p64[s->pidx], s->u[1 - c],
+                                                                  s->v[1 - c], s->nx, s->ny, idx, idy, s->mpart);
+  LAUNCHCHK();
+  s->have_maxabs = true;
   s->cur = 1 - c;
   return FS_OK;
 }
 
-int do_step(fs_solver_t* s, double dt) {
-  int rc = set_device(s);
+int enqueue_projection(fs_solver_t* s, int* cycles) {
+  *cycles = 0;
+  if (s->cfg.pressure_solver == FS_P_MULTIGRID) {
+    int rc = solve_multigrid(s, cycles);
+    if (rc && rc != FS_E_NOT_CONVERGED) return rc;
+    s->have_p = true;
+    int rc2 = enqueue_gradient64(s);
+    return rc2 ? rc2 : rc;
+  }
+  int rc = enqueue_divergence(s);
   if (rc) return rc;
-  if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0, got %g", dt);
-  rc = enqueue_advection(s, dt);
-  if (rc) return rc;
-  rc = enqueue_diffusion(s, dt);
-  if (rc) return rc;
-  rc = enqueue_forcing(s, dt);
-  if (rc) return rc;
+  int pi = 0;
+  if ((rc = solve_flat(s, s->cfg.pressure_iterations, &pi))) return rc;
+  s->p32idx = pi;
+  {
+    ProfScope ps(s, 3);
+    int c = s->cur;
+    fsk::k_gradient32<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p32[pi], s->u[1 - c],
+                                                                    s->v[1 - c], s->nx, s->ny, (float)(1.0 / s->dx),
+                                                                    (float)(1.0 / s->dy));
+    LAUNCHCHK();
+    s->cur = 1 - c;
+  }
+  s->have_maxabs = false;
+  return FS_OK;
+}
+
+// The step after its coefficients were enqueued.
+int do_step(fs_solver_t* s) {
+  int rc;
+  if (s->dt_pending >= DT_RING && (rc = sync_stream(s))) return rc;
+  HIPCHK(hipMemcpyAsync(s->dt_ring + s->dt_pending, &s->coef->dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  s->dt_pending++;
+  if ((rc = enqueue_advection(s))) return rc;
+  if ((rc = enqueue_diffusion(s))) return rc;
+  if ((rc = enqueue_forcing(s))) return rc;
   int cycles = 0;
-  double resid = 0;
-  int prc = enqueue_projection(s, &cycles, &resid);
+  int prc = enqueue_projection(s, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
   s->last_vcycles = cycles;
   s->total_vcycles += cycles;
-  s->last_residual = resid;
-  s->last_dt = dt;
-  s->t += dt;
   s->steps++;
   return prc;
 }
@@ -546,14 +706,59 @@ int build_levels(fs_solver_t* s) {
     if (n / 2 < minS || (n & 1)) break;
     n /= 2;
   }
+  // fused path: grid-wide levels are even (restriction, tile parity), the tail from
+  // the first level <= COARSE_N fits the one-workgroup kernel, and the sweep counts
+  // have a compiled tile kernel
+  int Lc = (int)s->lv.size();
+  int cs = 0;
+  while (cs < Lc && s->lv[cs].n > fsk::COARSE_N) cs++;
+  s->fused = s->cfg.mg_fused && cs < Lc && Lc - cs <= fsk::CML_MAXL && s->cfg.mg_pre >= 1 && s->cfg.mg_pre <= 3 &&
+             s->cfg.mg_post >= 1 && s->cfg.mg_post <= 3;
+  s->coarse_start = cs;
   for (size_t l = 0; l < s->lv.size(); l++) {
     size_t c = (size_t)s->lv[l].n * s->lv[l].n;
     int rc = dalloc(&s->lv[l].x, c);
     if (rc) return rc;
-    rc = dalloc(&s->lv[l].rhs, c);
-    if (rc) return rc;
+    if ((rc = dalloc(&s->lv[l].rhs, c))) return rc;
+    if (s->fused && (rc = dalloc(&s->lv[l].xb, c))) return rc;
   }
   return FS_OK;
+}
+
+// residual partial rows for `iterations` + 2 refinement iterations
+int alloc_rpart(fs_solver_t* s, int iterations) {
+  int rows = iterations + 2;
+  int cols;
+  if (s->fused && s->coarse_start > 0) {
+    int tx, nb;
+    tiles_grid(s->nx, &tx, &nb);
+    cols = nb;
+  } else {
+    cols = grid_blocks(s->nx, s->ny);
+  }
+  double* d = nullptr;
+  double* h = nullptr;
+  int rc = dalloc(&d, (size_t)rows * cols);
+  if (rc) return rc;
+  if ((rc = halloc(&h, (size_t)rows * cols))) {
+    dfree(d);
+    return rc;
+  }
+  dfree(s->rpart);
+  hfree(s->rpart_host);
+  s->rpart = d;
+  s->rpart_host = h;
+  s->rp_rows = rows;
+  s->rp_cols = cols;
+  return FS_OK;
+}
+
+int ensure_rpart(fs_solver_t* s, int iterations) {
+  if (s->cfg.pressure_solver != FS_P_MULTIGRID || iterations + 2 <= s->rp_rows) return FS_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = sync_stream(s))) return rc;
+  return alloc_rpart(s, iterations);
 }
 
 void release(fs_solver_t* s) {
@@ -572,11 +777,16 @@ void release(fs_solver_t* s) {
   for (auto& l : s->lv) {
     dfree(l.x);
     dfree(l.rhs);
+    dfree(l.xb);
   }
-  dfree(s->scal);
+  dfree(s->rpart);
+  hfree(s->rpart_host);
+  dfree(s->mpart);
+  dfree(s->coef);
+  hfree(s->dt_ring);
   dfree(s->diag_part);
   dfree(s->diag_out);
-  if (s->host) (void)hipHostFree(s->host);
+  hfree(s->host);
   for (auto& pr : s->prof.pending) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
@@ -594,36 +804,9 @@ int run_diag(fs_solver_t* s, double out[8]) {
   fsk::k_diag_final<<<1, 256, 0, s->stream>>>(s->diag_part, (int)(g.x * g.y), s->diag_out);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(s->host, s->diag_out, sizeof(double) * 8, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
-  for (int k = 0; k < 8; k++) out[k] = s->host[k];
-  return FS_OK;
-}
-
-// grow the residual-slot buffer (and its pinned mirror) to hold `iterations` + 2 slots
-int ensure_rmax(fs_solver_t* s, int iterations) {
-  if (iterations + 2 <= s->rmax_len) return FS_OK;
-  int rc = set_device(s);
+  int rc = sync_stream(s);
   if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  double* nr = nullptr;
-  if ((rc = dalloc(&nr, (size_t)iterations + 2 + 4))) return rc;
-  if (hipMemcpy(nr, s->scal, 4 * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess) {
-    dfree(nr);
-    return fail(FS_E_DEVICE, "copy of the scalar block failed");
-  }
-  double* nh = nullptr;
-  if (hipHostMalloc((void**)&nh, ((size_t)iterations + 10) * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    dfree(nr);
-    return fail(FS_E_OOM, "hipHostMalloc failed");
-  }
-  dfree(s->scal);
-  (void)hipHostFree(s->host);
-  s->scal = nr;
-  s->red = nr;
-  s->rmax = nr + 4;
-  s->host = nh;
-  s->rmax_len = iterations + 2;
+  for (int k = 0; k < 8; k++) out[k] = s->host[k];
   return FS_OK;
 }
 
@@ -675,6 +858,7 @@ void fs_config_default(fs_config* c) {
   c->guess = FS_GUESS_ZERO;
   c->solid_mask_width = 1.5;
   c->device = 0;
+  c->mg_fused = 1;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -713,7 +897,6 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   s->n = (size_t)c.nx * c.ny;
   s->dx = c.lx / c.nx;
   s->dy = c.ly / c.ny;
-  int rc = FS_OK;
   auto bail = [&](int code) {
     std::string msg = g_err;
     release(s);
@@ -723,6 +906,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (hipSetDevice(s->dev) != hipSuccess) return bail(fail(FS_E_DEVICE, "hipSetDevice(%d) failed", s->dev));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(FS_E_DEVICE, "hipStreamCreate failed"));
+  int rc = FS_OK;
   for (int k = 0; k < 2 && !rc; k++) {
     rc = dalloc(&s->u[k], s->n);
     if (!rc) rc = dalloc(&s->v[k], s->n);
@@ -739,32 +923,25 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (!rc) rc = dalloc(&s->p64[0], s->n);
   if (!rc) rc = dalloc(&s->p64[1], s->n);
   if (!rc) rc = build_levels(s);
-  s->rmax_len = std::max(c.pressure_iterations, 1) + 2;
-  if (!rc) rc = dalloc(&s->scal, (size_t)s->rmax_len + 4);
-  if (!rc) {
-    s->red = s->scal;
-    s->rmax = s->scal + 4;
-  }
-  dim3 g = grid2(s->nx, s->ny);
-  s->diag_blocks = (int)(g.x * g.y);
-  if (!rc) rc = dalloc(&s->diag_part, (size_t)s->diag_blocks * fsk::DIAG_W);
+  if (!rc && c.pressure_solver == FS_P_MULTIGRID) rc = alloc_rpart(s, std::max(c.pressure_iterations, 1));
+  s->mp_count = grid_blocks(s->nx, s->ny);
+  if (!rc) rc = dalloc(&s->mpart, (size_t)2 * s->mp_count);
+  if (!rc) rc = dalloc(&s->coef, 1);
+  if (!rc) rc = halloc(&s->dt_ring, DT_RING);
+  if (!rc) rc = dalloc(&s->diag_part, (size_t)grid_blocks(s->nx, s->ny) * fsk::DIAG_W);
   if (!rc) rc = dalloc(&s->diag_out, 8);
-  if (!rc) {
-    size_t hn = std::max<size_t>(64, (size_t)s->rmax_len + 8);
-    if (hipHostMalloc((void**)&s->host, hn * sizeof(double), hipHostMallocDefault) != hipSuccess)
-      rc = fail(FS_E_OOM, "hipHostMalloc of the staging buffer failed");
-  }
+  if (!rc) rc = halloc(&s->host, 64);
   if (rc) return bail(rc);
-  // zero state: velocity, pressure, accumulators
+  // zero state: velocity, pressure, multigrid buffers
   if (!rc) rc = fill_f32(s, s->u[0], s->n, 0.f);
   if (!rc) rc = fill_f32(s, s->v[0], s->n, 0.f);
   if (!rc) rc = fill_f64(s, s->p64[0], s->n, 0.0);
   if (!rc) rc = fill_f64(s, s->p64[1], s->n, 0.0);
   if (!rc) rc = fill_f32(s, s->div, s->n, 0.f);
-  if (!rc) rc = fill_f64(s, s->scal, (size_t)s->rmax_len + 4, 0.0);
   for (auto& l : s->lv) {
     if (!rc) rc = fill_f32(s, l.x, (size_t)l.n * l.n, 0.f);
     if (!rc) rc = fill_f32(s, l.rhs, (size_t)l.n * l.n, 0.f);
+    if (!rc && l.xb) rc = fill_f32(s, l.xb, (size_t)l.n * l.n, 0.f);
   }
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "initial fill failed");
   if (rc) return bail(rc);
@@ -786,7 +963,7 @@ int fs_upload_velocity_f32(void* h, const float* u, const float* v, size_t n) {
   if ((rc = set_device(s))) return rc;
   HIPCHK(hipMemcpyAsync(s->u[s->cur], u, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->v[s->cur], v, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  if ((rc = sync_stream(s))) return rc;
   s->have_maxabs = false;
   s->have_p = false;
   return FS_OK;
@@ -815,9 +992,7 @@ int fs_read_velocity(void* h, float* u, float* v, size_t n) {
   if ((rc = set_device(s))) return rc;
   HIPCHK(hipMemcpyAsync(u, s->u[s->cur], n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(v, s->v[s->cur], n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
-  return FS_OK;
+  return sync_stream(s);
 }
 
 int fs_read_field(void* h, int field, void* out, size_t bytes) {
@@ -838,7 +1013,7 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
         if ((rc = set_device(s))) return rc;
         std::vector<float> pf(s->n);
         HIPCHK(hipMemcpyAsync(pf.data(), s->p32[s->p32idx], s->n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-        HIPCHK(hipStreamSynchronize(s->stream));
+        if ((rc = sync_stream(s))) return rc;
         for (size_t k = 0; k < s->n; k++) ((double*)out)[k] = pf[k];
         return FS_OK;
       }
@@ -846,20 +1021,25 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
       break;
     case FS_F_RESIDUAL:
       if (s->lv.empty()) return fail(FS_E_INVALID_ARG, "no multigrid residual without the multigrid solver");
-      src = s->lv[0].rhs; need = s->n * sizeof(float); break;
+      src = s->lv[0].rhs;
+      need = s->n * sizeof(float);
+      break;
     default: return fail(FS_E_INVALID_ARG, "unknown field %d", field);
   }
   if (bytes != need) return fail(FS_E_INVALID_ARG, "field %d needs %zu bytes, got %zu", field, need, bytes);
   if ((rc = set_device(s))) return rc;
   HIPCHK(hipMemcpyAsync(out, src, need, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return FS_OK;
+  return sync_stream(s);
 }
 
 int fs_step(void* h, double dt) {
   int rc = check_solver(h);
   if (rc) return rc;
-  return do_step((fs_solver_t*)h, dt);
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0, got %g", dt);
+  if ((rc = set_device(s))) return rc;
+  if ((rc = enqueue_coef_fixed(s, dt))) return rc;
+  return do_step(s);
 }
 
 int fs_step_cfl(void* h, double cfl) {
@@ -868,19 +1048,8 @@ int fs_step_cfl(void* h, double cfl) {
   fs_solver_t* s = (fs_solver_t*)h;
   if (!(cfl > 0) || !std::isfinite(cfl)) return fail(FS_E_INVALID_ARG, "cfl must be > 0");
   if ((rc = set_device(s))) return rc;
-  if (!s->have_maxabs) {
-    fill_f64(s, s->red, 2, 0.0);
-    fsk::k_maxabs<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[s->cur], s->v[s->cur], s->nx, s->ny, s->red);
-    LAUNCHCHK();
-  }
-  // dt is needed on the host for the kernel arguments of this step
-  fsk::k_cfl_dt<<<1, 1, 0, s->stream>>>(s->red, cfl, 1.0 / s->dx, 1.0 / s->dy, 1e30, s->red + 2);
-  LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(s->host, s->red + 2, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  double dt = s->host[0];
-  if (!std::isfinite(dt) || dt >= 1e30) return fail(FS_E_NONFINITE, "CFL time step undefined (zero or non-finite velocity)");
-  return do_step(s, dt);
+  if ((rc = enqueue_coef_cfl(s, cfl))) return rc;
+  return do_step(s);
 }
 
 int fs_sync(void* h) {
@@ -888,9 +1057,7 @@ int fs_sync(void* h) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
-  return FS_OK;
+  return sync_stream(s);
 }
 
 int fs_max_divergence(void* h, double* out) {
@@ -919,14 +1086,13 @@ int fs_poisson_residual_after(void* h, int iterations, double* out) {
   std::vector<float> dv(s->n);
   std::vector<double> p(s->n);
   if (s->cfg.pressure_solver == FS_P_MULTIGRID) {
-    if ((rc = ensure_rmax(s, iterations))) return rc;
+    if ((rc = ensure_rpart(s, iterations))) return rc;
     fs_config saved = s->cfg;
     s->cfg.tolerance = 0;
     s->cfg.pressure_iterations = iterations;
     s->cfg.guess = FS_GUESS_ZERO;
     int cyc = 0;
-    double res = 0;
-    rc = solve_multigrid(s, &cyc, &res);
+    rc = solve_multigrid(s, &cyc);
     s->cfg = saved;
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(p.data(), s->p64[s->pidx], s->n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -935,12 +1101,11 @@ int fs_poisson_residual_after(void* h, int iterations, double* out) {
     if ((rc = solve_flat(s, iterations, &pi))) return rc;
     std::vector<float> pf(s->n);
     HIPCHK(hipMemcpyAsync(pf.data(), s->p32[pi], s->n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
+    if ((rc = sync_stream(s))) return rc;
     for (size_t k = 0; k < s->n; k++) p[k] = pf[k];
   }
   HIPCHK(hipMemcpyAsync(dv.data(), s->div, s->n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
+  if ((rc = sync_stream(s))) return rc;
   int nx = s->nx, ny = s->ny;
   double h2 = s->dx * s->dx;
   double num = 0, den = 0;
@@ -963,7 +1128,7 @@ int fs_set_pressure_iterations(void* h, int iterations) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if (iterations < 0) return fail(FS_E_INVALID_ARG, "iterations must be >= 0");
-  if ((rc = ensure_rmax(s, iterations))) return rc;
+  if ((rc = ensure_rpart(s, iterations))) return rc;
   s->cfg.pressure_iterations = iterations;
   return FS_OK;
 }
@@ -1004,7 +1169,8 @@ int fs_set_forcing(void* h, const fs_forcing* f) {
     s->forcing_on = false;
     return FS_OK;
   }
-  if (!std::isfinite(f->kf) || !std::isfinite(f->width) || !std::isfinite(f->amp) || !(f->drag >= 0))
+  if (!std::isfinite(f->kf) || !std::isfinite(f->width) || !std::isfinite(f->amp) || !(f->drag >= 0) ||
+      !std::isfinite(f->drag))
     return fail(FS_E_INVALID_ARG, "forcing parameters must be finite, drag >= 0");
   s->forcing = *f;
   s->forcing_on = true;
@@ -1016,6 +1182,10 @@ int fs_get_stats(void* h, fs_stats* out) {
   if (rc) return rc;
   if (!out) return fail(FS_E_INVALID_ARG, "out is NULL");
   fs_solver_t* s = (fs_solver_t*)h;
+  if (s->dt_pending > 0 || s->resid_row_pending >= 0) {
+    if ((rc = set_device(s))) return rc;
+    if ((rc = sync_stream(s))) return rc;
+  }
   out->t = s->t;
   out->steps = s->steps;
   out->last_vcycles = s->last_vcycles;
@@ -1051,44 +1221,41 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
+  if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE) {
+    if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0");
+    if ((rc = enqueue_coef_fixed(s, dt))) return rc;
+  }
   switch (kernel) {
     case FS_K_ADVECT_SL: {
       int saved = s->cfg.advection;
       s->cfg.advection = FS_ADV_SEMI_LAGRANGIAN;
-      rc = enqueue_advection(s, dt);
+      rc = enqueue_advection(s);
       s->cfg.advection = saved;
       break;
     }
     case FS_K_ADVECT_MC:
       if (s->cfg.advection != FS_ADV_MACCORMACK)
         return fail(FS_E_INVALID_ARG, "MacCormack scratch exists only on a maccormack solver");
-      rc = enqueue_advection(s, dt);
+      rc = enqueue_advection(s);
       break;
     case FS_K_DIVERGENCE: rc = enqueue_divergence(s); break;
-    case FS_K_DIFFUSE: rc = enqueue_diffusion(s, dt); break;
+    case FS_K_DIFFUSE: rc = enqueue_diffusion(s); break;
     case FS_K_PROJECT: {
       int cyc = 0;
-      double res = 0;
-      rc = enqueue_projection(s, &cyc, &res);
+      rc = enqueue_projection(s, &cyc);
       s->last_vcycles = cyc;
-      s->last_residual = res;
       break;
     }
-    case FS_K_GRADIENT: {
-      int c = s->cur;
-      double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-      fsk::k_gradient64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p64[s->pidx], s->u[1 - c],
-                                                                      s->v[1 - c], s->nx, s->ny, idx, idy, s->red);
-      LAUNCHCHK();
-      s->cur = 1 - c;
+    case FS_K_GRADIENT:
+      if (s->cfg.pressure_solver != FS_P_MULTIGRID)
+        return fail(FS_E_INVALID_ARG, "the FP64 gradient needs the multigrid pressure");
+      rc = enqueue_gradient64(s);
       break;
-    }
     default: return fail(FS_E_INVALID_ARG, "unknown kernel %d", kernel);
   }
   if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
-  return FS_OK;
+  s->have_maxabs = s->have_maxabs && kernel == FS_K_GRADIENT;
+  return sync_stream(s);
 }
 
 int fs_profile_enable(void* h, int kind) {
@@ -1097,8 +1264,7 @@ int fs_profile_enable(void* h, int kind) {
   fs_solver_t* s = (fs_solver_t*)h;
   if (kind < -1 || kind > 3) return fail(FS_E_INVALID_ARG, "unknown profile kind %d", kind);
   if ((rc = set_device(s))) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
+  if ((rc = sync_stream(s))) return rc;
   s->prof.kind = kind;
   s->prof.total_ms = 0;
   s->prof.launches = 0;
@@ -1110,8 +1276,7 @@ int fs_profile_read(void* h, double* total_ms, int64_t* launches) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  prof_collect(s);
+  if ((rc = sync_stream(s))) return rc;
   if (total_ms) *total_ms = s->prof.total_ms;
   if (launches) *launches = s->prof.launches;
   return FS_OK;

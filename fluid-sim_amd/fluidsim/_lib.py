@@ -41,6 +41,7 @@ class FsConfig(C.Structure):
         ("guess", C.c_int),
         ("solid_mask_width", C.c_double),
         ("device", C.c_int),
+        ("mg_fused", C.c_int),
     ]
 
 

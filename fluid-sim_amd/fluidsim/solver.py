@@ -54,7 +54,7 @@ class GPUFluidSolver:
                  confinement=0, dyeFade=0, advection="maccormack", solidMaskWidth=1.5,
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
-                 mgPrecision="fp32", guess="zero"):
+                 mgPrecision="fp32", guess="zero", mgFused=True):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -80,6 +80,7 @@ class GPUFluidSolver:
         cfg.guess = _lib.GUESS[guess]
         cfg.solid_mask_width = solidMaskWidth
         cfg.device = int(device)
+        cfg.mg_fused = 1 if mgFused else 0
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))

@@ -84,6 +84,8 @@ typedef struct fs_config {
   int guess;                  /* FS_GUESS_*; default zero (the reference zeroes p) */
   double solid_mask_width;    /* in cells; default 1.5 */
   int device;                 /* HIP device ordinal; default 0 */
+  int mg_fused;               /* 1: LDS-tiled V-cycle (two launches per level, levels <= 64
+                                 in one workgroup); 0: one launch per operator. Same bits. */
 } fs_config;
 
 /* per-step statistics (new: the reference reports none) */

@@ -264,3 +264,43 @@ def test_nonfinite_velocity_is_reported(fs):
         g.uploadVelocity(u, v)
         with pytest.raises(fluidsim.FluidSimError, match="non-finite"):
             g.diagnostics()
+
+
+# ------------------------------------------- fused V-cycle == per-operator V-cycle
+
+@pytest.mark.parametrize("n,pre,post,min_size", [(64, 2, 2, 8), (256, 2, 2, 4), (256, 1, 3, 8),
+                                                 (512, 3, 1, 4), (192, 2, 2, 4), (256, 2, 2, 2),
+                                                 (256, 2, 2, 16), (1024, 2, 2, 4)])
+def test_fused_vcycle_bitwise_equals_per_operator_path(fs, n, pre, post, min_size):
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=3)
+    outs = []
+    for fused in (True, False):
+        with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
+                    mgMinSize=min_size, mgPre=pre, mgPost=post, mgFused=fused) as g:
+            g.uploadVelocity(u0, v0)
+            for _ in range(3):
+                g.step(0.004)
+            outs.append((g.readVelocity(), g.readField("pressure"), g.stats()["last_residual"]))
+    (a, pa, ra), (b, pb, rb) = outs
+    assert (a["u"] == b["u"]).all() and (a["v"] == b["v"]).all()
+    assert (pa == pb).all()
+    assert ra == rb
+
+
+def test_fused_tolerance_mode_matches_per_operator_counts(fs):
+    from fluidsim import initial
+    n = 256
+    u0, v0 = initial.decaying_turbulence(n, seed=5)
+    res = []
+    for fused in (True, False):
+        with solver(fs, n=n, nu=1e-4, pressureSolver="multigrid", pressureIterations=40, tolerance=1e-6,
+                    mgMinSize=4, mgFused=fused) as g:
+            g.uploadVelocity(u0, v0)
+            cyc = []
+            for _ in range(4):
+                g.stepCFL(0.5)
+                cyc.append(g.stats()["last_vcycles"])
+            res.append((cyc, g.readVelocity()))
+    assert res[0][0] == res[1][0]
+    assert (res[0][1]["u"] == res[1][1]["u"]).all()
