@@ -50,13 +50,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--guess", default="zero", choices=["zero", "previous"])
     return ap.parse_args()
 
 
-def make_solver(fluidsim, n, tol):
+def make_solver(fluidsim, n, tol, guess="zero"):
     return fluidsim.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
                                    pressureIterations=60, tolerance=tol, mgMinSize=4, mgPre=2, mgPost=2,
-                                   mgCoarse=40)
+                                   mgCoarse=40, guess=guess)
 
 
 def cpu_baseline(n, seconds):
@@ -104,7 +105,7 @@ def main():
     dev = fluidsim.createDevice()
     n = a.n
     u0, v0 = initial.decaying_turbulence(n, seed=1)
-    g = make_solver(fluidsim, n, a.tol)
+    g = make_solver(fluidsim, n, a.tol, a.guess)
     g.uploadVelocity(u0, v0)
     print(f"# device: {dev['info']['description']}; lib {fluidsim.library_path()}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
@@ -174,12 +175,13 @@ def main():
         "config": {
             "workload": f"decaying 2D turbulence {n}^2 periodic MAC grid, nu=1e-4, MacCormack + explicit "
                         "diffusion, CFL 0.5, plain multigrid V(2,2) red-black 9 levels (minSize 4, 40 coarse "
-                        "sweeps) to FP64 max|div u* - L p| <= 1e-6 every step, zero initial guess",
+                        "sweeps) to FP64 max|div u* - L p| <= 1e-6 every step, "
+                        f"{a.guess} initial guess",
             "n": n,
             "residual_tolerance": a.tol,
             "residual_dtype": "f64",
             "vcycles_per_step": cycles,
-            "guess": "zero",
+            "guess": a.guess,
             "stored_field_max_div": d["max_divergence"],
         },
         "roofline": roof,

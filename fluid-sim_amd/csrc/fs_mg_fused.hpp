@@ -41,21 +41,71 @@ __device__ __forceinline__ int wrapn(int i, int n) {
   return i < 0 ? i + n : (i >= n ? i - n : i);
 }
 
-// One red-black half-sweep on the LDS region [lo, RW-lo) x [lo, RH-lo), visiting
-// only the cells of `colour`. par0 is the colour parity of LDS cell (0,0).
-template <int RW, int RH>
-__device__ __forceinline__ void lds_half_sweep(float* __restrict__ sx, const float* __restrict__ sr, int lo,
-                                               int colour, int par0, float h2) {
-  const int w = RW - 2 * lo, hgt = RH - 2 * lo, hw = w >> 1;  // w is even
-  for (int q = threadIdx.x; q < hw * hgt; q += NT) {
-    int ry = q / hw, rx = q - ry * hw;
-    int ly = lo + ry;
-    int lx = lo + 2 * rx + ((colour + par0 + lo + ly) & 1);
-    int k = ly * RW + lx;
-    float sum = sx[k - 1] + sx[k + 1] + sx[k - RW] + sx[k + RW];
-    sx[k] = (sum - h2 * sr[k]) * 0.25f;
+// Load a periodic RW x RH window with origin (ox, oy) of an n x n field into LDS.
+// Every load a thread owns is issued before any of them is stored, so the
+// window costs one memory latency instead of one per row of threads.
+template <int RW, int RH, typename T>
+__device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __restrict__ src, int n, int ox, int oy) {
+  constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+  float tmp[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) {
+      int ly = q / RW, lx = q - ly * RW;
+      tmp[k] = (float)src[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) dst[q] = tmp[k];
   }
 }
+
+// One red-black half-sweep on the LDS region [LO, RW-LO) x [LO, RH-LO), visiting
+// only the cells of `colour` (par0 = colour parity of LDS cell (0,0)). A colour pass
+// reads only the other colour, so all of a thread's reads are issued before its writes.
+template <int RW, int RH, int LO>
+__device__ __forceinline__ void lds_half_sweep(float* __restrict__ sx, const float* __restrict__ sr, int colour,
+                                               int par0, float h2) {
+  constexpr int w = RW - 2 * LO, hgt = RH - 2 * LO, hw = w / 2;  // w is even
+  constexpr int CNT = hw * hgt, K = (CNT + NT - 1) / NT;
+  float acc[K];
+  int kk[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) {
+      int ry = q / hw, rx = q - ry * hw;
+      int ly = LO + ry;
+      int lx = LO + 2 * rx + ((colour + par0 + LO + ly) & 1);
+      int c = ly * RW + lx;
+      kk[k] = c;
+      float sum = sx[c - 1] + sx[c + 1] + sx[c - RW] + sx[c + RW];
+      acc[k] = (sum - h2 * sr[c]) * 0.25f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) sx[kk[k]] = acc[k];
+  }
+}
+
+// Half-sweeps S .. SEND-1 on regions LO = S + LOOFF (one cell smaller each time).
+template <int RW, int RH, int S, int SEND, int LOOFF>
+struct HalfSweeps {
+  static __device__ __forceinline__ void run(float* sx, const float* sr, int par0, float h2) {
+    lds_half_sweep<RW, RH, S + LOOFF>(sx, sr, S & 1, par0, h2);
+    __syncthreads();
+    HalfSweeps<RW, RH, S + 1, SEND, LOOFF>::run(sx, sr, par0, h2);
+  }
+};
+template <int RW, int RH, int SEND, int LOOFF>
+struct HalfSweeps<RW, RH, SEND, SEND, LOOFF> {
+  static __device__ __forceinline__ void run(float*, const float*, int, float) {}
+};
 
 // Pre-smoothing from zero: red half-sweep on the whole region, then 2*SW-1
 // half-sweeps on shrinking regions.
@@ -68,11 +118,7 @@ __device__ __forceinline__ void tile_smooth_from_zero(float* __restrict__ sx, co
     sx[q] = red ? (0.f - h2 * sr[q]) * 0.25f : 0.f;
   }
   __syncthreads();
-#pragma unroll
-  for (int s = 1; s < 2 * SW; s++) {
-    lds_half_sweep<RW, RH>(sx, sr, s, s & 1, par0, h2);
-    __syncthreads();
-  }
+  HalfSweeps<RW, RH, 1, 2 * SW, 0>::run(sx, sr, par0, h2);
 }
 
 // Store the tile interior and the restricted residual of each 2x2 block of it.
@@ -117,10 +163,7 @@ k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2
   int t = xcd_tile(nblocks);
   int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   int ox = tx0 - H, oy = ty0 - H;  // global coordinate of LDS (0,0)
-  for (int q = threadIdx.x; q < RW * RH; q += NT) {
-    int ly = q / RW, lx = q - ly * RW;
-    sr[q] = rhs[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
-  }
+  load_window<RW, RH>(sr, rhs, n, ox, oy);
   __syncthreads();
   tile_smooth_from_zero<SW, RW, RH>(sx, sr, (ox + oy) & 1, h2);
   tile_store_and_restrict<H, RW>(sx, sr, x, crhs, n, tx0, ty0, ih2);
@@ -149,32 +192,63 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
   int t = xcd_tile(nblocks);
   int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   int ox = tx0 - H, oy = ty0 - H;
-  for (int q = threadIdx.x; q < PW * PH; q += NT) {
-    int ly = q / PW, lx = q - ly * PW;
-    size_t g = (size_t)wrapn(oy - 1 + ly, n) * n + wrapn(ox - 1 + lx, n);
-    double pv = p ? p[g] : 0.0;
-    if (e_prev) pv += (double)e_prev[g];
-    sp[q] = pv;
+  {
+    constexpr int CNT = PW * PH, K = (CNT + NT - 1) / NT;
+    double tp[K];
+    float te[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        int ly = q / PW, lx = q - ly * PW;
+        size_t g = (size_t)wrapn(oy - 1 + ly, n) * n + wrapn(ox - 1 + lx, n);
+        tp[k] = p ? p[g] : 0.0;
+        te[k] = e_prev ? e_prev[g] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) sp[q] = e_prev ? tp[k] + (double)te[k] : tp[k];
+    }
   }
   __syncthreads();
   double amax = 0.0;
-  for (int q = threadIdx.x; q < RW * RH; q += NT) {
-    int ly = q / RW, lx = q - ly * RW;
-    int gi = wrapn(ox + lx, n), gj = wrapn(oy + ly, n);
-    int gip = gi + 1 == n ? 0 : gi + 1, gjp = gj + 1 == n ? 0 : gj + 1;
-    int kp = (ly + 1) * PW + lx + 1;
-    double pc = sp[kp];
-    double lap = (sp[kp - 1] - 2 * pc + sp[kp + 1]) * idx2 + (sp[kp - PW] - 2 * pc + sp[kp + PW]) * idx2;
-    double dv = ((double)u[(size_t)gj * n + gip] - (double)u[(size_t)gj * n + gi]) * idx +
-                ((double)v[(size_t)gjp * n + gi] - (double)v[(size_t)gj * n + gi]) * idx;
-    double r = dv - lap;
-    sr[q] = (float)r;
-    bool inner = lx >= H && lx < H + TX && ly >= H && ly < H + TY && ox + lx < n && oy + ly < n;
-    if (inner) {
-      size_t g = (size_t)gj * n + gi;
-      pout[g] = pc;
-      r32[g] = (float)r;
-      amax = fmax(amax, nan_to_inf(fabs(r)));
+  {
+    constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+    float uL[K], uR[K], vB[K], vT[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        int ly = q / RW, lx = q - ly * RW;
+        int gi = wrapn(ox + lx, n), gj = wrapn(oy + ly, n);
+        int gip = gi + 1 == n ? 0 : gi + 1, gjp = gj + 1 == n ? 0 : gj + 1;
+        uL[k] = u[(size_t)gj * n + gi];
+        uR[k] = u[(size_t)gj * n + gip];
+        vB[k] = v[(size_t)gj * n + gi];
+        vT[k] = v[(size_t)gjp * n + gi];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        int ly = q / RW, lx = q - ly * RW;
+        int kp = (ly + 1) * PW + lx + 1;
+        double pc = sp[kp];
+        double lap = (sp[kp - 1] - 2 * pc + sp[kp + 1]) * idx2 + (sp[kp - PW] - 2 * pc + sp[kp + PW]) * idx2;
+        double dv = ((double)uR[k] - (double)uL[k]) * idx + ((double)vT[k] - (double)vB[k]) * idx;
+        double r = dv - lap;
+        sr[q] = (float)r;
+        bool inner = lx >= H && lx < H + TX && ly >= H && ly < H + TY && ox + lx < n && oy + ly < n;
+        if (inner) {
+          size_t g = (size_t)(oy + ly) * n + (ox + lx);
+          pout[g] = pc;
+          r32[g] = (float)r;
+          amax = fmax(amax, nan_to_inf(fabs(r)));
+        }
+      }
     }
   }
   {
@@ -213,36 +287,37 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
   int cn = n >> 1;
   // lowest coarse node any region cell interpolates from (ox, oy are even)
   int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
-  for (int q = threadIdx.x; q < CW * CH; q += NT) {
-    int ly = q / CW, lx = q - ly * CW;
-    sc[q] = xc[(size_t)wrapn(coy + ly, cn) * cn + wrapn(cox + lx, cn)];
-  }
-  for (int q = threadIdx.x; q < RW * RH; q += NT) {
-    int ly = q / RW, lx = q - ly * RW;
-    size_t g = (size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n);
-    sr[q] = rhs[g];
-    sx[q] = xin[g];
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < RW * RH; q += NT) {
-    int ly = q / RW, lx = q - ly * RW;
-    int i = ox + lx, j = oy + ly;  // unwrapped fine coordinates (parity is what matters)
-    int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
-    int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
-    float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
-    int cx = i0 - cox, cy = j0 - coy;
-    float a = sc[cy * CW + cx], b = sc[cy * CW + cx + 1];
-    float c = sc[(cy + 1) * CW + cx], d = sc[(cy + 1) * CW + cx + 1];
-    float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-    sx[q] = sx[q] + (top * (1.f - fy) + bot * fy);
-  }
-  __syncthreads();
-  int par0 = (ox + oy) & 1;
+  constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+  float xv[K];
 #pragma unroll
-  for (int s = 0; s < 2 * SW; s++) {
-    lds_half_sweep<RW, RH>(sx, sr, s + 1, s & 1, par0, h2);
-    __syncthreads();
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) {
+      int ly = q / RW, lx = q - ly * RW;
+      xv[k] = xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
+    }
   }
+  load_window<CW, CH>(sc, xc, cn, cox, coy);
+  load_window<RW, RH>(sr, rhs, n, ox, oy);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) {
+      int ly = q / RW, lx = q - ly * RW;
+      int i = ox + lx, j = oy + ly;  // unwrapped fine coordinates (parity is what matters)
+      int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+      int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+      float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+      int cx = i0 - cox, cy = j0 - coy;
+      float a = sc[cy * CW + cx], b = sc[cy * CW + cx + 1];
+      float c = sc[(cy + 1) * CW + cx], d = sc[(cy + 1) * CW + cx + 1];
+      float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+      sx[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+    }
+  }
+  __syncthreads();
+  HalfSweeps<RW, RH, 0, 2 * SW, 1>::run(sx, sr, (ox + oy) & 1, h2);
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
@@ -262,26 +337,63 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
 // Reads the level-n0 rhs and writes the level-n0 correction.
 constexpr int CML_CELLS = 4096 + 1024 + 256 + 64 + 16 + 4 + 1 + 1;
 constexpr int CTH = 1024;
+constexpr int CK = (COARSE_N * COARSE_N / 4 + CTH - 1) / CTH;  // coarse cells per thread (restriction)
+constexpr int CKF = (COARSE_N * COARSE_N + CTH - 1) / CTH;     // fine cells per thread (prolongation)
 constexpr int CML_MAXL = 12;
 constexpr int REG_N = 8;       // levels with n <= REG_N run in registers
 constexpr int REG_MAXL = 4;    // 8, 4, 2, 1
-struct LevelSpacing {
+// Per-level table passed by value (kernel arguments are uniform and indexable).
+struct LevelTable {
   float h2[CML_MAXL];   // (float)(dx_l * dx_l)
   float ih2[CML_MAXL];  // (float)(1 / (dx_l * dx_l))
+  int n[CML_MAXL];      // level size
+  int off[CML_MAXL];    // offset of the level in the LDS arrays
+  int levels;           // levels in this kernel
+  int tail;             // first level run in registers (== levels: none)
 };
 
-// one colour pass on an LDS level (n <= 64): lane -> column, 16 rows per pass
+// One colour pass on an LDS level (n <= 64). Threads map to the cells of the
+// colour only (row j, pair m, i = 2m + ((j + colour) & 1)), so every active lane
+// does work and small levels occupy few wavefronts; odd n falls back to a
+// full-grid mapping with a colour test. Reads of a pass touch only the other
+// colour, so they are all issued before the writes.
+constexpr int CKP = (COARSE_N * COARSE_N / 2 + CTH - 1) / CTH;
 __device__ __forceinline__ void cml_colour(float* __restrict__ x, const float* __restrict__ r, int n, float h2,
                                            int colour, bool from_zero) {
-  int i = threadIdx.x & 63;
-  if (i >= n) return;
-  int im = i == 0 ? n - 1 : i - 1, ip = i + 1 == n ? 0 : i + 1;
-  for (int j = threadIdx.x >> 6; j < n; j += CTH / 64) {
-    if (((i + j) & 1) != colour) continue;
-    float sum = 0.f;
-    if (!from_zero)
-      sum = x[j * n + im] + x[j * n + ip] + x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i];
-    x[j * n + i] = (sum - h2 * r[j * n + i]) * 0.25f;
+  if (n & 1) {
+    for (int q = threadIdx.x; q < n * n; q += CTH) {
+      int j = q / n, i = q - j * n;
+      if (((i + j) & 1) != colour) continue;
+      float sum = 0.f;
+      if (!from_zero)
+        sum = x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
+              x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i];
+      x[q] = (sum - h2 * r[q]) * 0.25f;
+    }
+    return;
+  }
+  const int hn = n >> 1, cells = hn * n;
+  float acc[CKP];
+  int cc[CKP];
+#pragma unroll
+  for (int k = 0; k < CKP; k++) {
+    int q = threadIdx.x + k * CTH;
+    if (q < cells) {
+      int j = q / hn, m = q - j * hn;
+      int i = 2 * m + ((j + colour) & 1);
+      int c = j * n + i;
+      cc[k] = c;
+      float sum = 0.f;
+      if (!from_zero)
+        sum = x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
+              x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i];
+      acc[k] = (sum - h2 * r[c]) * 0.25f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CKP; k++) {
+    int q = threadIdx.x + k * CTH;
+    if (q < cells) x[cc[k]] = acc[k];
   }
 }
 
@@ -307,34 +419,35 @@ __device__ __forceinline__ float reg_residual(float x, float r, int n, int i, in
 }
 
 __global__ void __launch_bounds__(CTH)
-k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, int n0, int levels, const LevelSpacing sp,
-               int pre, int post, int coarse) {
+k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const LevelTable tb, int pre, int post,
+               int coarse, const float* __restrict__ pinv) {
   __shared__ float bx[CML_CELLS];
   __shared__ float br[CML_CELLS];
   __shared__ double red[CTH / 64];
-  int off[CML_MAXL];
-  int nl[CML_MAXL];
-  int tail = levels;  // first level handled in registers
+  const int levels = tb.levels, tail = tb.tail, last = levels - 1;
+  const int n0 = tb.n[0];
   {
-    int o = 0, m = n0;
-    for (int l = 0; l < levels; l++) {
-      off[l] = o;
-      nl[l] = m;
-      if (tail == levels && m <= REG_N) tail = l;
-      o += m * m;
-      m >>= 1;
+    constexpr int K = CKF;
+    float tmp[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * CTH;
+      if (q < n0 * n0) tmp[k] = rg[q];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * CTH;
+      if (q < n0 * n0) br[q] = tmp[k];
     }
   }
-  for (int q = threadIdx.x; q < n0 * n0; q += CTH) br[q] = rg[q];
   __syncthreads();
-  const int last = levels - 1;
   // ---- LDS levels above the register tail (or above the LDS coarsest): down
   const int ldsDown = tail < levels ? tail : last;
   for (int l = 0; l < ldsDown; l++) {
-    int n = nl[l];
-    float h2 = sp.h2[l], ih2 = sp.ih2[l];
-    float* x = bx + off[l];
-    float* r = br + off[l];
+    int n = tb.n[l];
+    float h2 = tb.h2[l], ih2 = tb.ih2[l];
+    float* x = bx + tb.off[l];
+    float* r = br + tb.off[l];
     if (pre == 0) {
       for (int q = threadIdx.x; q < n * n; q += CTH) x[q] = 0.f;
       __syncthreads();
@@ -346,31 +459,43 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, int n0, int
       __syncthreads();
     }
     int cn = n >> 1;
-    float* rc = br + off[l + 1];
-    int I = threadIdx.x & 63;
-    if (I < cn) {
-      for (int J = threadIdx.x >> 6; J < cn; J += CTH / 64) {
-        float s = 0.f;
-        for (int b = 0; b < 2; b++)
-          for (int a = 0; a < 2; a++) {
-            int i = 2 * I + a, j = 2 * J + b;
-            int k = j * n + i;
-            float lap = (x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
-                         x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i] - 4.f * x[k]) *
-                        ih2;
-            s += r[k] - lap;
-          }
-        rc[J * cn + I] = s * 0.25f;
+    float* rc = br + tb.off[l + 1];
+    {
+      float acc[CK];
+#pragma unroll
+      for (int k = 0; k < CK; k++) {
+        int q = threadIdx.x + k * CTH;
+        if (q < cn * cn) {
+          int J = q / cn, I = q - J * cn;
+          float s = 0.f;
+#pragma unroll
+          for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+              int i = 2 * I + a, j = 2 * J + b;
+              int c = j * n + i;
+              float lap = (x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
+                           x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i] - 4.f * x[c]) *
+                          ih2;
+              s += r[c] - lap;
+            }
+          acc[k] = s * 0.25f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CK; k++) {
+        int q = threadIdx.x + k * CTH;
+        if (q < cn * cn) rc[q] = acc[k];
       }
     }
     __syncthreads();
   }
   if (tail == levels) {
     // the coarsest level is an LDS level: remove the rhs mean, then sweep
-    int n = nl[last];
-    float h2 = sp.h2[last];
-    float* x = bx + off[last];
-    float* r = br + off[last];
+    int n = tb.n[last];
+    float h2 = tb.h2[last];
+    float* x = bx + tb.off[last];
+    float* r = br + tb.off[last];
     double s = 0.0;
     for (int q = threadIdx.x; q < n * n; q += CTH) s += r[q];
     s = wave_sum(s);
@@ -398,26 +523,36 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, int n0, int
 #pragma unroll
     for (int k = 0; k < REG_MAXL; k++) xr[k] = rr[k] = 0.f;
     {
-      int n = nl[tail];
-      rr[0] = lane < n * n ? br[off[tail] + lane] : 0.f;
+      int n = tb.n[tail];
+      rr[0] = lane < n * n ? br[tb.off[tail] + lane] : 0.f;
     }
     const int nt = last - tail + 1;  // register levels (<= REG_MAXL)
 #pragma unroll
     for (int k = 0; k < REG_MAXL; k++) {
       if (k < nt) {
-        int l = tail + k, n = nl[l];
+        int l = tail + k, n = tb.n[l];
         bool act = lane < n * n;
         int j = act ? lane / n : 0, i = act ? lane - j * n : 0;
-        float h2 = sp.h2[l], ih2 = sp.ih2[l];
+        float h2 = tb.h2[l], ih2 = tb.ih2[l];
         if (k == nt - 1) {
           double s = act ? (double)rr[k] : 0.0;
           s = wave_sum(s);
           float mean = (float)(s / (n * n));
           rr[k] = rr[k] - mean;
           float x = 0.f;
-          for (int sw = 0; sw < coarse; sw++) {
-            x = reg_colour(x, rr[k], n, i, j, act, h2, 0, false);
-            x = reg_colour(x, rr[k], n, i, j, act, h2, 1, false);
+          if (pinv) {
+            // the sweeps would converge below FP32 rounding: apply the pseudo-inverse
+            // of the periodic Laplacian instead (pinv[c*cells + lane], columns broadcast)
+            int cells = n * n;
+            for (int c = 0; c < cells; c++) {
+              float bc = __shfl(rr[k], c, 64);
+              x = fmaf(act ? pinv[c * cells + lane] : 0.f, bc, x);
+            }
+          } else {
+            for (int sw = 0; sw < coarse; sw++) {
+              x = reg_colour(x, rr[k], n, i, j, act, h2, 0, false);
+              x = reg_colour(x, rr[k], n, i, j, act, h2, 1, false);
+            }
           }
           xr[k] = x;
         } else {
@@ -448,10 +583,10 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, int n0, int
 #pragma unroll
     for (int k = REG_MAXL - 2; k >= 0; k--) {
       if (k < nt - 1) {
-        int l = tail + k, n = nl[l], cn = n >> 1;
+        int l = tail + k, n = tb.n[l], cn = n >> 1;
         bool act = lane < n * n;
         int j = act ? lane / n : 0, i = act ? lane - j * n : 0;
-        float h2 = sp.h2[l];
+        float h2 = tb.h2[l];
         int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
         int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
         float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
@@ -469,31 +604,38 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, int n0, int
         xr[k] = x;
       }
     }
-    int n = nl[tail];
-    if (lane < n * n) bx[off[tail] + lane] = xr[0];
+    int n = tb.n[tail];
+    if (lane < n * n) bx[tb.off[tail] + lane] = xr[0];
   }
   __syncthreads();
   // ---- LDS levels: up
   for (int l = ldsDown - 1; l >= 0; l--) {
-    int n = nl[l], cn = n >> 1;
-    float h2 = sp.h2[l];
-    float* x = bx + off[l];
-    float* r = br + off[l];
-    const float* xc = bx + off[l + 1];
-    int i = threadIdx.x & 63;
-    if (i < n) {
-      int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
-      float fx = (i & 1) ? 0.25f : 0.75f;
-      int x0 = i0 < 0 ? i0 + cn : i0;
-      int x1 = x0 + 1 == cn ? 0 : x0 + 1;
-      for (int j = threadIdx.x >> 6; j < n; j += CTH / 64) {
-        int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
-        float fy = (j & 1) ? 0.25f : 0.75f;
-        int y0 = j0 < 0 ? j0 + cn : j0;
-        int y1 = y0 + 1 == cn ? 0 : y0 + 1;
-        float a = xc[y0 * cn + x0], b = xc[y0 * cn + x1], c = xc[y1 * cn + x0], d = xc[y1 * cn + x1];
-        float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-        x[j * n + i] = x[j * n + i] + (top * (1.f - fy) + bot * fy);
+    int n = tb.n[l], cn = n >> 1;
+    float h2 = tb.h2[l];
+    float* x = bx + tb.off[l];
+    float* r = br + tb.off[l];
+    const float* xc = bx + tb.off[l + 1];
+    {
+      float acc[CKF];
+#pragma unroll
+      for (int k = 0; k < CKF; k++) {
+        int q = threadIdx.x + k * CTH;
+        if (q < n * n) {
+          int j = q / n, i = q - j * n;
+          int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+          int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+          float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+          int x0 = i0 < 0 ? i0 + cn : i0, y0 = j0 < 0 ? j0 + cn : j0;
+          int x1 = x0 + 1 == cn ? 0 : x0 + 1, y1 = y0 + 1 == cn ? 0 : y0 + 1;
+          float a = xc[y0 * cn + x0], b = xc[y0 * cn + x1], c = xc[y1 * cn + x0], d = xc[y1 * cn + x1];
+          float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+          acc[k] = x[q] + (top * (1.f - fy) + bot * fy);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CKF; k++) {
+        int q = threadIdx.x + k * CTH;
+        if (q < n * n) x[q] = acc[k];
       }
     }
     __syncthreads();

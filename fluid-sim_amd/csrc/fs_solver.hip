@@ -106,6 +106,7 @@ struct fs_solver_t {
   bool have_p = false;       // a previous pressure exists (warm start)
   std::vector<Level> lv;     // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
   bool fused = false;        // LDS-tiled V-cycle in use
+  float* pinv = nullptr;     // coarsest-level pseudo-inverse (null: run the sweeps)
   int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
   double* rpart = nullptr;
@@ -391,14 +392,23 @@ int solve_flat(fs_solver_t* s, int iterations, int* out_idx) {
 
 // ---------------------------------------------------- per-operator V-cycle
 
-fsk::LevelSpacing level_spacing(const fs_solver_t* s, int from) {
-  fsk::LevelSpacing sp{};
-  for (int l = from; l < (int)s->lv.size() && l - from < fsk::CML_MAXL; l++) {
+// Table of levels from..end-1 for the one-workgroup coarse kernel.
+fsk::LevelTable level_table(const fs_solver_t* s, int from, int end) {
+  fsk::LevelTable tb{};
+  int o = 0;
+  tb.levels = end - from;
+  tb.tail = tb.levels;
+  for (int l = from; l < end; l++) {
+    int k = l - from;
     double dx = s->lv[l].dx;
-    sp.h2[l - from] = (float)(dx * dx);
-    sp.ih2[l - from] = (float)(1.0 / (dx * dx));
+    tb.h2[k] = (float)(dx * dx);
+    tb.ih2[k] = (float)(1.0 / (dx * dx));
+    tb.n[k] = s->lv[l].n;
+    tb.off[k] = o;
+    o += s->lv[l].n * s->lv[l].n;
+    if (tb.tail == tb.levels && s->lv[l].n <= fsk::REG_N) tb.tail = k;
   }
-  return sp;
+  return tb;
 }
 
 // `sweeps` red-black sweeps on level l; x_zero marks a logically-zero start.
@@ -436,8 +446,8 @@ int mg_vcycle(fs_solver_t* s) {
   {
     Level& C = s->lv[Lc - 1];
     if (C.n <= fsk::COARSE_N) {
-      fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.x, C.rhs, C.n, 1, level_spacing(s, Lc - 1),
-                                                         s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse);
+      fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.x, C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre,
+                                                         s->cfg.mg_post, s->cfg.mg_coarse, s->pinv);
       LAUNCHCHK();
     } else {
       bool z = true;
@@ -509,9 +519,8 @@ int launch_up(fs_solver_t* s, int l) {
 
 int launch_coarse_ml(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
-  fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.xb, C.rhs, C.n, (int)s->lv.size() - from,
-                                                     level_spacing(s, from), s->cfg.mg_pre, s->cfg.mg_post,
-                                                     s->cfg.mg_coarse);
+  fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.xb, C.rhs, level_table(s, from, (int)s->lv.size()),
+                                                     s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, s->pinv);
   LAUNCHCHK();
   return FS_OK;
 }
@@ -689,6 +698,57 @@ int do_step(fs_solver_t* s) {
   return prc;
 }
 
+
+// The coarsest level's red-black sweeps converge geometrically with factor
+// rho = ((1 + cos(2 pi / n)) / 2)^2 (the smoothest non-constant mode). When
+// rho^sweeps is below FP32 rounding the sweeps are an exact solve, and the
+// pseudo-inverse of the periodic 5-point Laplacian gives the same answer (up to
+// the null-space constant, which no residual sees) in one pass. Built in FP64:
+// A+ = (A + 11^T/N)^-1 - 11^T/N.
+int build_coarse_inverse(fs_solver_t* s) {
+  const Level& C = s->lv.back();
+  int n = C.n, N = n * n;
+  if (n > fsk::REG_N || n < 2) return FS_OK;
+  double rho = std::pow((1.0 + std::cos(2 * M_PI / n)) / 2.0, 2.0);
+  if (!(std::pow(rho, (double)s->cfg.mg_coarse) < 1e-7)) return FS_OK;
+  double h2 = C.dx * C.dx;
+  std::vector<double> M((size_t)N * 2 * N, 0.0);  // [A + 11^T/N | I]
+  auto at = [&](int r, int c) -> double& { return M[(size_t)r * 2 * N + c]; };
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      int r = j * n + i;
+      int nb[4] = {j * n + (i + n - 1) % n, j * n + (i + 1) % n, ((j + n - 1) % n) * n + i, ((j + 1) % n) * n + i};
+      at(r, r) += -4.0 / h2;
+      for (int q = 0; q < 4; q++) at(r, nb[q]) += 1.0 / h2;
+      for (int c = 0; c < N; c++) at(r, c) += 1.0 / N;
+      at(r, N + r) = 1.0;
+    }
+  for (int col = 0; col < N; col++) {  // Gauss-Jordan with partial pivoting
+    int piv = col;
+    for (int r = col + 1; r < N; r++)
+      if (std::fabs(at(r, col)) > std::fabs(at(piv, col))) piv = r;
+    if (std::fabs(at(piv, col)) < 1e-300) return fail(FS_E_DEVICE, "singular coarse-level matrix");
+    if (piv != col)
+      for (int c = 0; c < 2 * N; c++) std::swap(at(piv, c), at(col, c));
+    double d = at(col, col);
+    for (int c = 0; c < 2 * N; c++) at(col, c) /= d;
+    for (int r = 0; r < N; r++)
+      if (r != col) {
+        double f = at(r, col);
+        if (f != 0.0)
+          for (int c = 0; c < 2 * N; c++) at(r, c) -= f * at(col, c);
+      }
+  }
+  // stored transposed: pinv[c*N + r] = A+(r, c), so lanes read consecutive words
+  std::vector<float> pt((size_t)N * N);
+  for (int r = 0; r < N; r++)
+    for (int c = 0; c < N; c++) pt[(size_t)c * N + r] = (float)(at(r, N + c) - 1.0 / N);
+  int rc = dalloc(&s->pinv, (size_t)N * N);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(s->pinv, pt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
 int build_levels(fs_solver_t* s) {
   if (s->cfg.pressure_solver != FS_P_MULTIGRID) return FS_OK;
   if (s->nx != s->ny) return fail(FS_E_INVALID_ARG, "multigrid here assumes a square grid, got %dx%d", s->nx, s->ny);
@@ -722,7 +782,7 @@ int build_levels(fs_solver_t* s) {
     if ((rc = dalloc(&s->lv[l].rhs, c))) return rc;
     if (s->fused && (rc = dalloc(&s->lv[l].xb, c))) return rc;
   }
-  return FS_OK;
+  return build_coarse_inverse(s);
 }
 
 // residual partial rows for `iterations` + 2 refinement iterations
@@ -781,6 +841,7 @@ void release(fs_solver_t* s) {
   }
   dfree(s->rpart);
   hfree(s->rpart_host);
+  dfree(s->pinv);
   dfree(s->mpart);
   dfree(s->coef);
   hfree(s->dt_ring);

@@ -34,6 +34,28 @@ def sources():
     return [os.path.join(CSRC, "fs_solver.hip")]
 
 
+NODE_DIR = os.path.join(PKG, "node")
+NODE_ADDON = os.path.join(NODE_DIR, "fluidsim.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_node(verbose: bool = False):
+    """Build the Node-API addon (fluidsim.node) over libfluidsim.so with gcc.
+
+    Returns the addon path, or None when the Node-API headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = NODE_ADDON + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-I", NODE_INCLUDE, "-I", os.path.join(ROOT, "include"),
+           os.path.join(NODE_DIR, "fluidsim_napi.c"), "-o", tmp, "-L", LIBDIR, "-lfluidsim",
+           "-Wl,-rpath,$ORIGIN/../lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, NODE_ADDON)
+    return NODE_ADDON
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = LIB + ".tmp"
@@ -42,6 +64,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_node(verbose)
     return LIB
 
 

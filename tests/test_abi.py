@@ -97,3 +97,21 @@ def test_initial_turbulence_field_is_discretely_divergence_free(O):
     assert abs(np.sqrt(np.mean(u * u) + np.mean(v * v)) - 1.0) < 1e-12
     u2, _ = initial.decaying_turbulence(128, seed=1)
     assert (u == u2).all()
+
+
+def test_node_addon_loads_and_exports_the_shim_surface():
+    # the Node-API addon over the C ABI (drop-in for src/gpu/solver.js); no device call
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed")
+    addon = os.path.join(ROOT, "fluid-sim_amd", "node", "fluidsim.node")
+    out = subprocess.run([node, "-e", f"console.log(Object.keys(require({addon!r})).join(','))"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    names = set(out.stdout.strip().split(","))
+    assert {"create", "destroy", "uploadVelocity", "readVelocity", "step", "stepCFL", "sync", "maxDivergence",
+            "poissonResidualAfter", "setParam", "setForcing", "stats"} <= names
+    shim = open(os.path.join(ROOT, "fluid-sim_amd", "node", "solver.mjs")).read()
+    assert "export async function createDevice" in shim and "export class GPUFluidSolver" in shim

@@ -1,0 +1,47 @@
+"""The Node drop-in (fluid-sim_amd/node/solver.mjs -> fluidsim.node -> libfluidsim.so)
+runs the same step as the Python binding and produces the same bits."""
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_node_shim_matches_python_binding_bitwise():
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed on this machine")
+    out = subprocess.run([node, os.path.join(ROOT, "fluid-sim_amd", "node", "smoke.mjs")],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    js = json.loads(out.stdout.strip().splitlines()[-1])
+    import fluidsim
+    n = 64
+    h = 2 * math.pi / n
+    i = np.arange(n)[None, :]
+    j = np.arange(n)[:, None]
+    u = (np.sin(i * h) * np.cos((j + 0.5) * h)).reshape(-1)
+    v = (-np.cos((i + 0.5) * h) * np.sin(j * h)).reshape(-1)
+    with fluidsim.GPUFluidSolver(0, n=n, nu=0.01, advection="maccormack", pressureSolver="multigrid",
+                                 pressureIterations=20, tolerance=1e-6) as g:
+        g.uploadVelocity(u, v)
+        for _ in range(5):
+            g.step(0.01)
+        vel = g.readVelocity()
+        st = g.stats()
+    k = np.arange(n * n)
+    su = float(np.sum(vel["u"].astype(np.float64) * (k % 97 + 1)))
+    sv = float(np.sum(vel["v"].astype(np.float64) * (k % 89 + 1)))
+    # same library, same inputs: identical velocities; the JS and numpy sums of them
+    # differ only in summation order
+    assert abs(js["checksumU"] - su) <= 1e-9 * max(1.0, abs(su))
+    assert abs(js["checksumV"] - sv) <= 1e-9 * max(1.0, abs(sv))
+    assert js["steps"] == 5 and abs(js["t"] - 0.05) < 1e-15
+    assert js["lastResidual"] == st["last_residual"] and js["lastResidual"] <= 1e-6
+    assert js["unknownSelectorThrows"] is True
