@@ -29,14 +29,21 @@ sys.path.insert(0, os.path.join(ROOT, "fluid-sim_amd"))
 METRIC = "cell-steps/s at 1024² to max|∇·u|≤1e-6, learned-MG vs plain MG; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic (compulsory) bytes per cell of one launch scope: every distinct input
-# element read once and every output written once at its storage precision
-# (FP32 fields, FP64 pressure); the coarse right-hand side is 1/4 cell per cell.
-KINDS = {
-    0: ("advection: SL forward pass + MacCormack correction (2 launches)", 16 + 24),
-    1: ("level-0 up-sweep: prolongation + post-smoothing (k_mgt_up)", 4 + 4 + 1 + 4),
-    2: ("level-0 down-sweep fused with the FP64 residual (k_mgt_down0)", 8 + 8 + 4 + 8 + 4 + 4 + 1),
-    3: ("gradient subtraction with FP64 pressure (k_gradient64)", 8 + 8 + 8),
+# Kernel classes timed for the roofline. The library counts each launch's
+# algorithmic bytes: every distinct input element read once and every output
+# written once at its storage precision (FP32 velocity and correction, FP64
+# pressure; the coarse right-hand side is a quarter cell per cell).
+KIND_TEXT = {
+    "advect_sl": "semi-Lagrangian forward trace (k_advect_sl), 16 B/cell",
+    "advect_mc": "MacCormack back-trace + limiter (k_advect_mc), 24 B/cell",
+    "diffuse": "explicit diffusion (k_diffuse), 16 B/cell",
+    "down0": "level-0 residual in FP64 + pre-smoothing + restriction (k_mgt_down0), 37-45 B/cell",
+    "down": "level>0 pre-smoothing + restriction (k_mgt_down), 9 B/cell of the level",
+    "coarse": "coarse levels in one workgroup (k_mg_coarse_ml), 8 B/cell of its top level",
+    "up": "prolongation + post-smoothing (k_mgt_up), 13 B/cell of the level",
+    "gradient": "FP64 pressure-gradient subtraction (k_gradient64), 24 B/cell",
+    "residual": "FP64 residual only (k_residual64), 20-32 B/cell",
+    "rb": "red-black colour sweep (k_mg_rb), 6 B/cell",
 }
 
 
@@ -126,31 +133,33 @@ def main():
           file=sys.stderr, flush=True)
 
     roof = None
-    per_kind = {}
     if not a.no_roofline:
         psteps = max(10, min(a.steps, 200))
-        for kind in KINDS:
-            g.profileEnable(kind)
-            for _ in range(psteps):
-                g.stepCFL(0.5)
-            ms, launches = g.profileRead()
-            per_kind[kind] = (ms, launches, psteps)
-        g.profileEnable(-1)
-        dom = max(per_kind, key=lambda k: per_kind[k][0])
-        ms, launches, ps = per_kind[dom]
+        g.profileEnable()          # every kernel class, one launch-timed pass
+        for _ in range(psteps):
+            g.stepCFL(0.5)
+        g.sync()
+        prof = g.profileRead()
+        g.profileEnable([])
+        step_ms = el / a.steps * 1e3
+        for k, (ms_k, l_k, b_k) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+            if l_k:
+                print(f"# {k:10s} {ms_k / l_k * 1e3:8.2f} us/launch {l_k / psteps:6.2f} launches/step "
+                      f"{ms_k / psteps:7.4f} ms/step ({ms_k / psteps / step_ms:5.1%}) "
+                      f"{b_k / (ms_k / 1e3) / 1e9:7.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+        # dominant kernel = the class with the largest share of the step's time
+        dom = max(prof, key=lambda k: prof[k][0])
+        ms, launches, nbytes = prof[dom]
         avg_s = ms / 1e3 / max(launches, 1)
-        bytes_per_launch = KINDS[dom][1] * n * n
-        achieved = bytes_per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": KINDS[dom][0], "achieved": achieved, "peak": HBM_PEAK_GBS,
+        achieved = nbytes / (ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": KIND_TEXT[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / ps,
-                "algorithmic_bytes_per_launch": bytes_per_launch,
-                "share_of_step": ms / (el / a.steps * ps * 1e3)}
-        for k, (ms_k, l_k, ps_k) in per_kind.items():
-            print(f"# kind {k} {KINDS[k][0]}: {ms_k / max(l_k, 1) * 1e3:.2f} us/launch, "
-                  f"{l_k / ps_k:.1f} launches/step, {ms_k / ps_k:.3f} ms/step, "
-                  f"{KINDS[k][1] * n * n / (ms_k / 1e3 / max(l_k, 1)) / 1e9:.0f} GB/s algorithmic",
-                  file=sys.stderr, flush=True)
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
+                "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
+                "share_of_step": ms / psteps / step_ms,
+                "all_kernels": {k: {"us_per_launch": v[0] / max(v[1], 1) * 1e3, "launches_per_step": v[1] / psteps,
+                                    "gbs": v[2] / (v[0] / 1e3) / 1e9 if v[0] else 0.0}
+                                for k, v in prof.items() if v[1]}}
     g.destroy()
 
     cpu = None

@@ -352,49 +352,67 @@ struct LevelTable {
   int tail;             // first level run in registers (== levels: none)
 };
 
-// One colour pass on an LDS level (n <= 64). Threads map to the cells of the
-// colour only (row j, pair m, i = 2m + ((j + colour) & 1)), so every active lane
-// does work and small levels occupy few wavefronts; odd n falls back to a
-// full-grid mapping with a colour test. Reads of a pass touch only the other
-// colour, so they are all issued before the writes.
-constexpr int CKP = (COARSE_N * COARSE_N / 2 + CTH - 1) / CTH;
-__device__ __forceinline__ void cml_colour(float* __restrict__ x, const float* __restrict__ r, int n, float h2,
-                                           int colour, bool from_zero) {
-  if (n & 1) {
-    for (int q = threadIdx.x; q < n * n; q += CTH) {
-      int j = q / n, i = q - j * n;
-      if (((i + j) & 1) != colour) continue;
-      float sum = 0.f;
-      if (!from_zero)
-        sum = x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
-              x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i];
-      x[q] = (sum - h2 * r[q]) * 0.25f;
+// Cells of one colour owned by this thread on an LDS level (n <= 64), with the
+// addresses of their four neighbours. Threads map to the cells of the colour only
+// (row j, pair m, i = 2m + ((j + colour) & 1)), so every active lane works and small
+// levels occupy few wavefronts. Computed once per level and reused by every pass:
+// the passes run on one CU, where per-pass index arithmetic would dominate.
+constexpr int CKP = (COARSE_N * COARSE_N / 2 + CTH - 1) / CTH;  // slots for an even level
+constexpr int CKS = (COARSE_N * COARSE_N + CTH - 1) / CTH;      // slots for an odd level
+struct ColourCells {
+  int c[CKS], l[CKS], r[CKS], b[CKS], t[CKS];  // c < 0: no cell in this slot
+};
+
+__device__ __forceinline__ void colour_cells(ColourCells& cs, int n, int colour) {
+#pragma unroll
+  for (int k = 0; k < CKS; k++) cs.c[k] = -1;
+  if (n & 1) {  // odd level: full-grid mapping with a colour test
+#pragma unroll
+    for (int k = 0; k < CKS; k++) {
+      int q = threadIdx.x + k * CTH;
+      if (q < n * n) {
+        int j = q / n, i = q - j * n;
+        if (((i + j) & 1) == colour) {
+          cs.c[k] = q;
+          cs.l[k] = j * n + (i == 0 ? n - 1 : i - 1);
+          cs.r[k] = j * n + (i + 1 == n ? 0 : i + 1);
+          cs.b[k] = (j == 0 ? n - 1 : j - 1) * n + i;
+          cs.t[k] = (j + 1 == n ? 0 : j + 1) * n + i;
+        }
+      }
     }
     return;
   }
   const int hn = n >> 1, cells = hn * n;
-  float acc[CKP];
-  int cc[CKP];
 #pragma unroll
   for (int k = 0; k < CKP; k++) {
     int q = threadIdx.x + k * CTH;
     if (q < cells) {
       int j = q / hn, m = q - j * hn;
       int i = 2 * m + ((j + colour) & 1);
-      int c = j * n + i;
-      cc[k] = c;
-      float sum = 0.f;
-      if (!from_zero)
-        sum = x[j * n + (i == 0 ? n - 1 : i - 1)] + x[j * n + (i + 1 == n ? 0 : i + 1)] +
-              x[(j == 0 ? n - 1 : j - 1) * n + i] + x[(j + 1 == n ? 0 : j + 1) * n + i];
-      acc[k] = (sum - h2 * r[c]) * 0.25f;
+      cs.c[k] = j * n + i;
+      cs.l[k] = j * n + (i == 0 ? n - 1 : i - 1);
+      cs.r[k] = j * n + (i + 1 == n ? 0 : i + 1);
+      cs.b[k] = (j == 0 ? n - 1 : j - 1) * n + i;
+      cs.t[k] = (j + 1 == n ? 0 : j + 1) * n + i;
+    }
+  }
+}
+
+// One colour pass; it reads only the other colour, so all reads precede the writes.
+__device__ __forceinline__ void cml_pass(float* __restrict__ x, const float* __restrict__ r, const ColourCells& cs,
+                                         float h2, bool from_zero) {
+  float acc[CKS];
+#pragma unroll
+  for (int k = 0; k < CKS; k++) {
+    if (cs.c[k] >= 0) {
+      float sum = from_zero ? 0.f : x[cs.l[k]] + x[cs.r[k]] + x[cs.b[k]] + x[cs.t[k]];
+      acc[k] = (sum - h2 * r[cs.c[k]]) * 0.25f;
     }
   }
 #pragma unroll
-  for (int k = 0; k < CKP; k++) {
-    int q = threadIdx.x + k * CTH;
-    if (q < cells) x[cc[k]] = acc[k];
-  }
+  for (int k = 0; k < CKS; k++)
+    if (cs.c[k] >= 0) x[cs.c[k]] = acc[k];
 }
 
 // register-level red-black pass: lane = j*n + i holds x of its cell
@@ -452,11 +470,16 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       for (int q = threadIdx.x; q < n * n; q += CTH) x[q] = 0.f;
       __syncthreads();
     }
-    for (int s = 0; s < pre; s++) {
-      cml_colour(x, r, n, h2, 0, s == 0);  // from zero: black cells are stale but never read
-      __syncthreads();
-      cml_colour(x, r, n, h2, 1, false);
-      __syncthreads();
+    if (pre > 0) {
+      ColourCells red, blk;
+      colour_cells(red, n, 0);
+      colour_cells(blk, n, 1);
+      for (int s = 0; s < pre; s++) {
+        cml_pass(x, r, red, h2, s == 0);  // from zero: black cells are stale but never read
+        __syncthreads();
+        cml_pass(x, r, blk, h2, false);
+        __syncthreads();
+      }
     }
     int cn = n >> 1;
     float* rc = br + tb.off[l + 1];
@@ -510,10 +533,13 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       x[q] = 0.f;
     }
     __syncthreads();
+    ColourCells red, blk;
+    colour_cells(red, n, 0);
+    colour_cells(blk, n, 1);
     for (int sw = 0; sw < coarse; sw++) {
-      cml_colour(x, r, n, h2, 0, false);
+      cml_pass(x, r, red, h2, false);
       __syncthreads();
-      cml_colour(x, r, n, h2, 1, false);
+      cml_pass(x, r, blk, h2, false);
       __syncthreads();
     }
   } else if (threadIdx.x < 64) {
@@ -639,11 +665,16 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       }
     }
     __syncthreads();
-    for (int s = 0; s < post; s++) {
-      cml_colour(x, r, n, h2, 0, false);
-      __syncthreads();
-      cml_colour(x, r, n, h2, 1, false);
-      __syncthreads();
+    if (post > 0) {
+      ColourCells red, blk;
+      colour_cells(red, n, 0);
+      colour_cells(blk, n, 1);
+      for (int s = 0; s < post; s++) {
+        cml_pass(x, r, red, h2, false);
+        __syncthreads();
+        cml_pass(x, r, blk, h2, false);
+        __syncthreads();
+      }
     }
   }
   for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = bx[q];

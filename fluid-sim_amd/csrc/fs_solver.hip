@@ -21,6 +21,7 @@
 //                       added only if the last residual is still above the gate.
 //                       Extra V-cycles only lower the residual, so an
 //                       over-prediction costs time, never accuracy.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,12 +75,26 @@ struct Level {
   float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e)
 };
 
+// Kernel classes timed by fs_profile_*: each profiled launch goes through
+// hipExtLaunchKernelGGL with a start/stop event pair, which time the dispatch
+// itself (not the gap before it), and carries its algorithmic byte count.
+enum {
+  PK_ADVECT_SL = 0, PK_ADVECT_MC, PK_DIFFUSE, PK_DOWN0, PK_DOWN, PK_COARSE, PK_UP, PK_GRADIENT, PK_RESIDUAL,
+  PK_RB, PK_COUNT
+};
+
 struct Profile {
-  int kind = -1;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  unsigned mask = 0;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  double total_ms = 0;
-  int64_t launches = 0;
+  double ms[PK_COUNT] = {};
+  int64_t launches[PK_COUNT] = {};
+  double bytes[PK_COUNT] = {};
 };
 
 constexpr int DT_RING = 4096;  // steps whose dt may be pending a host read
@@ -181,44 +196,44 @@ void hfree(void* p) {
   if (p) (void)hipHostFree(p);
 }
 
-// event pair around launches of the profiled kernel class
-struct ProfScope {
-  fs_solver_t* s;
-  bool on;
-  hipEvent_t a{}, b{};
-  ProfScope(fs_solver_t* s_, int kind) : s(s_), on(s_->prof.kind == kind) {
-    if (!on) return;
-    a = take();
-    b = take();
-    (void)hipEventRecord(a, s->stream);
-  }
-  hipEvent_t take() {
-    if (!s->prof.pool.empty()) {
-      hipEvent_t e = s->prof.pool.back();
-      s->prof.pool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
+hipEvent_t take_event(fs_solver_t* s) {
+  if (!s->prof.pool.empty()) {
+    hipEvent_t e = s->prof.pool.back();
+    s->prof.pool.pop_back();
     return e;
   }
-  ~ProfScope() {
-    if (!on) return;
-    (void)hipEventRecord(b, s->stream);
-    s->prof.pending.push_back({a, b});
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Launch on the solver's stream; when `kind` is being profiled the dispatch is
+// timed by events recorded at its own start and end.
+template <typename K, typename... Args>
+int launch(fs_solver_t* s, int kind, double bytes, K kern, dim3 g, dim3 b, Args... args) {
+  if (kind >= 0 && ((s->prof.mask >> kind) & 1u)) {
+    hipEvent_t e0 = take_event(s), e1 = take_event(s);
+    hipExtLaunchKernelGGL(kern, g, b, 0, s->stream, e0, e1, 0, args...);
+    s->prof.pending.push_back({kind, e0, e1, bytes});
+  } else {
+    kern<<<g, b, 0, s->stream>>>(args...);
   }
-};
+  hipError_t e_ = hipGetLastError();
+  if (e_ != hipSuccess) return fail(FS_E_DEVICE, "kernel launch failed: %s", hipGetErrorString(e_));
+  return FS_OK;
+}
 
 // Everything the host learns lazily after a stream synchronisation.
 void after_sync(fs_solver_t* s) {
   for (auto& pr : s->prof.pending) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
-      s->prof.total_ms += ms;
-      s->prof.launches++;
+    if (hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
+      s->prof.ms[pr.kind] += ms;
+      s->prof.launches[pr.kind]++;
+      s->prof.bytes[pr.kind] += pr.bytes;
     }
-    s->prof.pool.push_back(pr.first);
-    s->prof.pool.push_back(pr.second);
+    s->prof.pool.push_back(pr.a);
+    s->prof.pool.push_back(pr.b);
   }
   s->prof.pending.clear();
   for (int k = 0; k < s->dt_pending; k++) {
@@ -285,19 +300,21 @@ int enqueue_coef_cfl(fs_solver_t* s, double cfl) {
 }
 
 int enqueue_advection(fs_solver_t* s) {
-  ProfScope ps(s, 0);
-  int c = s->cur;
+  int c = s->cur, rc;
+  const double N = (double)s->n;
   if (s->cfg.advection == FS_ADV_MACCORMACK) {
-    fsk::k_advect_sl<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->fu, s->fv, s->nx, s->ny,
-                                                                   s->coef, 1.0f);
-    LAUNCHCHK();
-    fsk::k_advect_mc<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->fu, s->fv, s->u[1 - c],
-                                                                   s->v[1 - c], s->nx, s->ny, s->coef);
-    LAUNCHCHK();
+    if ((rc = launch(s, PK_ADVECT_SL, 16 * N, fsk::k_advect_sl, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                     (const float*)s->v[c], s->fu, s->fv, s->nx, s->ny, (const fsk::StepCoef*)s->coef, 1.0f)))
+      return rc;
+    if ((rc = launch(s, PK_ADVECT_MC, 24 * N, fsk::k_advect_mc, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                     (const float*)s->v[c], (const float*)s->fu, (const float*)s->fv, s->u[1 - c], s->v[1 - c], s->nx,
+                     s->ny, (const fsk::StepCoef*)s->coef)))
+      return rc;
   } else {
-    fsk::k_advect_sl<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->u[1 - c], s->v[1 - c],
-                                                                   s->nx, s->ny, s->coef, 1.0f);
-    LAUNCHCHK();
+    if ((rc = launch(s, PK_ADVECT_SL, 16 * N, fsk::k_advect_sl, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                     (const float*)s->v[c], s->u[1 - c], s->v[1 - c], s->nx, s->ny, (const fsk::StepCoef*)s->coef,
+                     1.0f)))
+      return rc;
   }
   s->cur = 1 - c;
   return FS_OK;
@@ -306,9 +323,9 @@ int enqueue_advection(fs_solver_t* s) {
 int enqueue_diffusion(fs_solver_t* s) {
   if (!(s->cfg.nu > 0)) return FS_OK;
   int c = s->cur;
-  fsk::k_diffuse<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->u[1 - c], s->v[1 - c], s->nx,
-                                                               s->ny, s->coef);
-  LAUNCHCHK();
+  int rc = launch(s, PK_DIFFUSE, 16.0 * s->n, fsk::k_diffuse, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                  (const float*)s->v[c], s->u[1 - c], s->v[1 - c], s->nx, s->ny, (const fsk::StepCoef*)s->coef);
+  if (rc) return rc;
   s->cur = 1 - c;
   return FS_OK;
 }
@@ -379,10 +396,9 @@ int solve_flat(fs_solver_t* s, int iterations, int* out_idx) {
   } else {
     for (int k = 0; k < iterations; k++) {
       for (int colour = 0; colour < 2; colour++) {
-        ProfScope ps(s, 1);
-        fsk::k_rb_colour<float><<<grid_pairs(s->nx, s->ny), BLK, 0, s->stream>>>(s->p32[0], s->div, s->nx, s->ny,
-                                                                                  h2, 1.0f, colour, 0);
-        LAUNCHCHK();
+        rc = launch(s, PK_RB, 6.0 * s->n, fsk::k_rb_colour<float>, grid_pairs(s->nx, s->ny), BLK, s->p32[0],
+                    (const float*)s->div, s->nx, s->ny, h2, 1.0f, colour, 0);
+        if (rc) return rc;
       }
     }
   }
@@ -417,9 +433,9 @@ int mg_smooth(fs_solver_t* s, int l, int sweeps, bool& x_zero) {
   float h2 = (float)(L.dx * L.dx);
   for (int k = 0; k < sweeps; k++) {
     for (int colour = 0; colour < 2; colour++) {
-      ProfScope ps(s, l == 0 ? 1 : -2);
-      fsk::k_mg_rb<float><<<grid_pairs(L.n, L.n), BLK, 0, s->stream>>>(L.x, L.rhs, L.n, h2, colour, x_zero ? 1 : 0);
-      LAUNCHCHK();
+      int rc = launch(s, PK_RB, 6.0 * L.n * L.n, fsk::k_mg_rb<float>, grid_pairs(L.n, L.n), BLK, L.x,
+                      (const float*)L.rhs, L.n, h2, colour, x_zero ? 1 : 0);
+      if (rc) return rc;
       // after the red pass from zero, black cells are still stale but the black
       // pass never reads its own value, and red cells are now valid
       x_zero = false;
@@ -446,9 +462,10 @@ int mg_vcycle(fs_solver_t* s) {
   {
     Level& C = s->lv[Lc - 1];
     if (C.n <= fsk::COARSE_N) {
-      fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.x, C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre,
-                                                         s->cfg.mg_post, s->cfg.mg_coarse, s->pinv);
-      LAUNCHCHK();
+      int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml, dim3(1), dim3(fsk::CTH), C.x,
+                      (const float*)C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre, s->cfg.mg_post,
+                      s->cfg.mg_coarse, (const float*)s->pinv);
+      if (rc) return rc;
     } else {
       bool z = true;
       int rc = mg_smooth(s, Lc - 1, s->cfg.mg_coarse, z);
@@ -487,12 +504,13 @@ int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_onl
   dim3 g = tiles_grid(L0.n, &tx, &nb);
   double idx = 1.0 / s->dx;
   int c = s->cur;
-  fsk::k_mgt_down0<PRE><<<g, fsk::NT, 0, s->stream>>>(
-      s->u[c], s->v[c], zero_p ? nullptr : s->p64[pin], it > 0 ? L0.xb : nullptr, s->p64[1 - pin], L0.rhs, L0.x,
-      s->lv[1].rhs, L0.n, idx, idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx, nb,
-      s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
-  LAUNCHCHK();
-  return FS_OK;
+  // u, v (8) + p (8) + previous correction (4) read; p (8) + r (4) written; then x (4) and the coarse rhs (1)
+  double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? 4 : 0) + 8 + 4 + (residual_only ? 0 : 4 + 0.25 * 4);
+  return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE>, g, dim3(fsk::NT), (const float*)s->u[c],
+                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
+                (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], L0.rhs, L0.x, s->lv[1].rhs, L0.n, idx,
+                idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx, nb,
+                s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
 }
 
 template <int PRE>
@@ -500,10 +518,9 @@ int launch_down(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid(F.n, &tx, &nb);
-  fsk::k_mgt_down<PRE><<<g, fsk::NT, 0, s->stream>>>(F.x, F.rhs, F.n, (float)(F.dx * F.dx),
-                                                      (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
-  LAUNCHCHK();
-  return FS_OK;
+  // rhs (4) read; x (4) and the coarse rhs (1 per 4 cells) written
+  return launch(s, PK_DOWN, 9.0 * F.n * F.n, fsk::k_mgt_down<PRE>, g, dim3(fsk::NT), F.x, (const float*)F.rhs, F.n,
+                (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
 }
 
 template <int POST>
@@ -511,18 +528,16 @@ int launch_up(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid(F.n, &tx, &nb);
-  fsk::k_mgt_up<POST><<<g, fsk::NT, 0, s->stream>>>(F.x, F.xb, F.rhs, s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx),
-                                                     tx, nb);
-  LAUNCHCHK();
-  return FS_OK;
+  // x (4), rhs (4), coarse x (1 per 4 cells) read; x (4) written
+  return launch(s, PK_UP, 13.0 * F.n * F.n, fsk::k_mgt_up<POST>, g, dim3(fsk::NT), (const float*)F.x, F.xb,
+                (const float*)F.rhs, (const float*)s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx), tx, nb);
 }
 
 int launch_coarse_ml(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
-  fsk::k_mg_coarse_ml<<<1, fsk::CTH, 0, s->stream>>>(C.xb, C.rhs, level_table(s, from, (int)s->lv.size()),
-                                                     s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, s->pinv);
-  LAUNCHCHK();
-  return FS_OK;
+  return launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml, dim3(1), dim3(fsk::CTH), C.xb,
+                (const float*)C.rhs, level_table(s, from, (int)s->lv.size()), s->cfg.mg_pre, s->cfg.mg_post,
+                s->cfg.mg_coarse, (const float*)s->pinv);
 }
 
 #define DISPATCH_SW(sw, fn, ...) \
@@ -531,14 +546,12 @@ int launch_coarse_ml(fs_solver_t* s, int from) {
 // FP64 residual `it` on the per-operator path (and the fused path when the whole
 // hierarchy fits one workgroup): reads p64[pin] (+ e), writes p64[1-pin] and r.
 int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
-  ProfScope ps(s, 2);
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  fsk::k_residual64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
-      s->u[c], s->v[c], zero_p ? nullptr : s->p64[pin], e, s->p64[1 - pin], s->lv[0].rhs, s->nx, s->ny, idx, idy,
-      idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
-  LAUNCHCHK();
-  return FS_OK;
+  double bpc = 8 + (zero_p ? 0 : 8) + (e ? 4 : 0) + 8 + 4;
+  return launch(s, PK_RESIDUAL, bpc * s->n, fsk::k_residual64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), e, s->p64[1 - pin],
+                s->lv[0].rhs, s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
 }
 
 // Residual `it`, then (unless residual_only) the V-cycle on its correction.
@@ -554,18 +567,13 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
     return residual_only ? FS_OK : launch_coarse_ml(s, 0);
   }
   int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
-  {
-    ProfScope ps(s, 2);
-    if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only))) return rc;
-  }
+  if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only))) return rc;
   if (residual_only) return FS_OK;
   for (int l = 1; l < cs; l++)
     if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
   if ((rc = launch_coarse_ml(s, cs))) return rc;
-  for (int l = cs - 1; l >= 0; l--) {
-    ProfScope ps(s, l == 0 ? 1 : -2);
+  for (int l = cs - 1; l >= 0; l--)
     if ((rc = DISPATCH_SW(post, launch_up, s, l))) return rc;
-  }
   return FS_OK;
 }
 
@@ -642,12 +650,12 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
 }
 
 int enqueue_gradient64(fs_solver_t* s) {
-  ProfScope ps(s, 3);
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  fsk::k_gradient64<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p64[s->pidx], s->u[1 - c],
-                                                                  s->v[1 - c], s->nx, s->ny, idx, idy, s->mpart);
-  LAUNCHCHK();
+  int rc = launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                  (const float*)s->v[c], (const double*)s->p64[s->pidx], s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
+                  idy, s->mpart);
+  if (rc) return rc;
   s->have_maxabs = true;
   s->cur = 1 - c;
   return FS_OK;
@@ -668,7 +676,6 @@ int enqueue_projection(fs_solver_t* s, int* cycles) {
   if ((rc = solve_flat(s, s->cfg.pressure_iterations, &pi))) return rc;
   s->p32idx = pi;
   {
-    ProfScope ps(s, 3);
     int c = s->cur;
     fsk::k_gradient32<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->p32[pi], s->u[1 - c],
                                                                     s->v[1 - c], s->nx, s->ny, (float)(1.0 / s->dx),
@@ -757,13 +764,14 @@ int build_levels(fs_solver_t* s) {
   int n = s->nx;
   if (n < minS) return fail(FS_E_INVALID_ARG, "grid %d is smaller than mg_min_size %d", n, minS);
   double L = s->cfg.lx;
-  // src/gpu/multigrid.js:182-197: halve while n >= minSize (and n stays even)
+  // src/gpu/multigrid.js:182-197: halve while n >= minSize. Every level stays even:
+  // on a periodic grid of odd size the red-black colouring meets itself at the seam.
   while (n >= minS) {
     Level lvl;
     lvl.n = n;
     lvl.dx = L / n;
     s->lv.push_back(lvl);
-    if (n / 2 < minS || (n & 1)) break;
+    if (n / 2 < minS || (n & 1) || ((n / 2) & 1)) break;
     n /= 2;
   }
   // fused path: grid-wide levels are even (restriction, tile parity), the tail from
@@ -849,8 +857,8 @@ void release(fs_solver_t* s) {
   dfree(s->diag_out);
   hfree(s->host);
   for (auto& pr : s->prof.pending) {
-    (void)hipEventDestroy(pr.first);
-    (void)hipEventDestroy(pr.second);
+    (void)hipEventDestroy(pr.a);
+    (void)hipEventDestroy(pr.b);
   }
   for (auto e : s->prof.pool) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -936,6 +944,9 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     return fail(FS_E_INVALID_ARG, "unknown pressure solver %d; have jacobi(0), red-black(1), multigrid(2)",
                 c.pressure_solver);
   if (c.pressure_iterations < 0) return fail(FS_E_INVALID_ARG, "pressure_iterations must be >= 0");
+  if ((c.pressure_solver == FS_P_RED_BLACK || c.pressure_solver == FS_P_MULTIGRID) && ((c.nx | c.ny) & 1))
+    return fail(FS_E_INVALID_ARG,
+                "red-black ordering needs even grid dimensions on a periodic grid, got %dx%d", c.nx, c.ny);
   if (c.tolerance < 0 || !std::isfinite(c.tolerance)) return fail(FS_E_INVALID_ARG, "tolerance must be >= 0");
   if (c.tolerance > 0 && c.pressure_solver != FS_P_MULTIGRID)
     return fail(FS_E_INVALID_ARG, "a residual tolerance needs the multigrid solver");
@@ -1319,27 +1330,32 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   return sync_stream(s);
 }
 
-int fs_profile_enable(void* h, int kind) {
+int fs_profile_enable(void* h, unsigned mask) {
   int rc = check_solver(h);
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
-  if (kind < -1 || kind > 3) return fail(FS_E_INVALID_ARG, "unknown profile kind %d", kind);
+  if (mask >> PK_COUNT) return fail(FS_E_INVALID_ARG, "unknown profile kinds in mask 0x%x", mask);
   if ((rc = set_device(s))) return rc;
   if ((rc = sync_stream(s))) return rc;
-  s->prof.kind = kind;
-  s->prof.total_ms = 0;
-  s->prof.launches = 0;
+  s->prof.mask = mask;
+  for (int k = 0; k < PK_COUNT; k++) {
+    s->prof.ms[k] = 0;
+    s->prof.launches[k] = 0;
+    s->prof.bytes[k] = 0;
+  }
   return FS_OK;
 }
 
-int fs_profile_read(void* h, double* total_ms, int64_t* launches) {
+int fs_profile_read(void* h, int kind, double* total_ms, int64_t* launches, double* bytes) {
   int rc = check_solver(h);
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
+  if (kind < 0 || kind >= PK_COUNT) return fail(FS_E_INVALID_ARG, "unknown profile kind %d", kind);
   if ((rc = set_device(s))) return rc;
   if ((rc = sync_stream(s))) return rc;
-  if (total_ms) *total_ms = s->prof.total_ms;
-  if (launches) *launches = s->prof.launches;
+  if (total_ms) *total_ms = s->prof.ms[kind];
+  if (launches) *launches = s->prof.launches[kind];
+  if (bytes) *bytes = s->prof.bytes[kind];
   return FS_OK;
 }
 

@@ -22,6 +22,8 @@ PRESSURE = {"jacobi": 0, "red-black": 1, "multigrid": 2}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GUESS = {"zero": 0, "previous": 1, "learned": 2}
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
+PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
+                 "gradient": 7, "residual": 8, "rb": 9}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5}
 
 
@@ -111,8 +113,9 @@ _SIGNATURES = {
     "fs_get_stats": (C.c_int, [_VP, C.POINTER(FsStats)]),
     "fs_diagnostics": (C.c_int, [_VP, C.POINTER(FsDiag)]),
     "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),
-    "fs_profile_enable": (C.c_int, [_VP, C.c_int]),
-    "fs_profile_read": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
+    "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -224,14 +224,22 @@ class GPUFluidSolver:
             raise ValueError(f"unknown kernel {name}")
         check(self._L.fs_run_kernel(self.handle, _lib.KERNEL[name], float(dt)))
 
-    def profileEnable(self, kind: int):
-        check(self._L.fs_profile_enable(self.handle, int(kind)))
+    def profileEnable(self, kinds=None):
+        """Time every launch of the named kernel classes (all when None, none when [])."""
+        names = _lib.PROFILE_KINDS if kinds is None else kinds
+        mask = 0
+        for k in names:
+            mask |= 1 << _lib.PROFILE_KINDS[k]
+        check(self._L.fs_profile_enable(self.handle, mask))
 
-    def profileRead(self):
-        ms = C.c_double()
-        n = C.c_int64()
-        check(self._L.fs_profile_read(self.handle, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+    def profileRead(self) -> dict:
+        """{kind: (total_ms, launches, algorithmic_bytes)} since profileEnable."""
+        out = {}
+        for name, k in _lib.PROFILE_KINDS.items():
+            ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+            check(self._L.fs_profile_read(self.handle, k, C.byref(ms), C.byref(n), C.byref(b)))
+            out[name] = (ms.value, n.value, b.value)
+        return out
 
     # -------------------------------------------------------------- lifetime
     def destroy(self):

@@ -167,11 +167,17 @@ int fs_diagnostics(void* solver, fs_diag* out);
 /* per-kernel hook replacing gputest's use of internals */
 int fs_run_kernel(void* solver, int kernel, double dt);
 
-/* measurement: HIP-event time of one kernel class on the solver's stream.
- * kind: 0 = advection, 1 = level-0 smoother, 2 = residual, 3 = gradient.
- * Enabled timing adds an event pair around every launch of that class. */
-int fs_profile_enable(void* solver, int kind);
-int fs_profile_read(void* solver, double* total_ms, int64_t* launches);
+/* measurement (new: the reference allocates a timestamp query set and never writes
+ * it, src/gpu/solver.js:202-210). Kernel classes: */
+enum {
+  FS_PK_ADVECT_SL = 0, FS_PK_ADVECT_MC = 1, FS_PK_DIFFUSE = 2, FS_PK_DOWN0 = 3, FS_PK_DOWN = 4,
+  FS_PK_COARSE = 5, FS_PK_UP = 6, FS_PK_GRADIENT = 7, FS_PK_RESIDUAL = 8, FS_PK_RB = 9, FS_PK_COUNT = 10
+};
+/* Time every launch of the classes in `mask` (bit k = class k; 0 = off) with HIP
+ * events recorded at the dispatch's own start and end on the solver's stream, and
+ * accumulate each launch's algorithmic bytes (inputs read once, outputs written once). */
+int fs_profile_enable(void* solver, unsigned mask);
+int fs_profile_read(void* solver, int kind, double* total_ms, int64_t* launches, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -588,12 +588,13 @@ void or_redblack(int nx, int ny, double lx, double ly, const double *rhs, double
   }
 }
 
-/* src/gpu/multigrid.js:182-197 -- halve while n >= min_size */
+/* src/gpu/multigrid.js:182-197 -- halve while n >= min_size; every level stays even
+ * (an odd periodic level has same-colour neighbours at its seam) */
 int or_mg_levels(int n, int min_size) {
   int L = 0;
   while (n >= min_size) {
     L++;
-    if (n / 2 < min_size || (n & 1)) break;
+    if (n / 2 < min_size || (n & 1) || ((n / 2) & 1)) break;
     n /= 2;
   }
   return L;

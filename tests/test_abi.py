@@ -62,6 +62,18 @@ def test_invalid_config_is_rejected_with_a_message(field, value, msg):
     assert h.value is None
 
 
+def test_odd_grid_rejected_for_red_black_orderings():
+    from fluidsim import _lib
+    L = _lib.load()
+    for solver in ("red-black", "multigrid"):
+        c = _lib.default_config()
+        c.pressure_solver = _lib.PRESSURE[solver]
+        c.nx = c.ny = 63
+        h = C.c_void_p()
+        assert L.fs_solver_create(C.byref(c), C.byref(h)) == _lib.FS_E_INVALID_ARG
+        assert "even grid" in _lib.last_error()
+
+
 def test_tolerance_needs_multigrid():
     from fluidsim import _lib
     L = _lib.load()

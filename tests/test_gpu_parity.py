@@ -304,3 +304,44 @@ def test_fused_tolerance_mode_matches_per_operator_counts(fs):
             res.append((cyc, g.readVelocity()))
     assert res[0][0] == res[1][0]
     assert (res[0][1]["u"] == res[1][1]["u"]).all()
+
+
+@pytest.mark.parametrize("n,min_size", [(96, 4), (192, 4), (256, 4), (128, 2), (256, 8), (100, 4)])
+def test_vcycle_residual_curve_matches_oracle(fs, O, n, min_size):
+    # non-power-of-two levels (96 -> 6, 192 -> 6, 100 stops at 50), the direct coarsest
+    # solve (6x6, 4x4, 2x2), and 8x8 swept: residual after k V-cycles vs the f64 oracle
+    u0, v0 = O.js_random_velocity(n, 777)
+    d = O.divergence(n, n, f32(u0), f32(v0))
+    with solver(fs, n=n, pressureSolver="multigrid", mgMinSize=min_size) as g:
+        g.uploadVelocity(u0, v0)
+        for it in (1, 2, 3):
+            r = g.poissonResidualAfter(it)
+            _, hist = O.mg_vcycles(n, d, it, min_size=min_size)
+            ro = hist[-1]
+            assert abs(r - ro) / ro < 2e-3, (n, it, r, ro)
+
+
+def test_profile_counts_every_launch_and_its_bytes(fs, O):
+    # one fused V(2,2) step at 256^2, minSize 4: levels 256..4, tile kernels on 256
+    # and 128, the one-workgroup kernel from 64 down; a fixed 3 V-cycles
+    n = 256
+    u0, v0 = O.js_random_velocity(n, 7)
+    with solver(fs, n=n, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
+                mgMinSize=4) as g:
+        g.uploadVelocity(u0, v0)
+        g.profileEnable()
+        g.step(1e-3)
+        g.sync()
+        p = g.profileRead()
+        g.profileEnable([])
+        g.step(1e-3)
+        assert g.profileRead()["advect_sl"][1] == 0
+    ms, launches, nbytes = p["advect_sl"]
+    assert launches == 1 and ms > 0 and nbytes == 16 * n * n
+    assert p["advect_mc"][1] == 1 and p["advect_mc"][2] == 24 * n * n
+    assert p["gradient"][1] == 1 and p["gradient"][2] == 24 * n * n
+    # three V-cycles: one level-0 down-sweep each, plus the final residual-only pass
+    assert p["down0"][1] + p["residual"][1] == 4
+    assert p["up"][1] == 3 * 2 and p["down"][1] == 3 and p["coarse"][1] == 3
+    for k, (ms_k, l_k, b_k) in p.items():
+        assert (ms_k > 0) == (l_k > 0) and (b_k > 0) == (l_k > 0), k

@@ -319,6 +319,8 @@ def test_multigrid_levels_follow_reference_rule(O):
     assert O.mg_levels(64, 8) == 4
     assert O.mg_levels(1024, 4) == 9
     assert O.mg_levels(1024, 8) == 8
+    assert O.mg_levels(96, 4) == 5     # 96, 48, 24, 12, 6: 3 would be odd
+    assert O.mg_levels(100, 4) == 2    # 100, 50: 25 would be odd
 
 
 # ------------------------------------------------------------------- full step
