@@ -46,6 +46,16 @@ KIND_TEXT = {
     "rb": "red-black colour sweep (k_mg_rb), 6 B/cell",
 }
 
+# HBM bytes per launch at 1024^2 measured by `rocprofv3 --pmc FETCH_SIZE` and
+# `--pmc WRITE_SIZE` runs of this script (--steps 20 --warmup 2 --no-roofline),
+# each scaled by what the counter reads for a plain copy of 256 MiB
+# (`bench.py --copy-calibration`: read = 2.0 x FETCH_SIZE KiB, write = 1.0 x
+# WRITE_SIZE KiB). For a class that runs on several levels, the mean over them.
+TRAFFIC_1024 = {
+    "up": 5.04e6, "down": 1.24e6, "advect_sl": 24.19e6, "advect_mc": 38.02e6, "diffuse": 25.19e6,
+    "gradient": 27.44e6, "down0": 37.85e6, "coarse": 0.10e6,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -58,7 +68,29 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--guess", default="zero", choices=["zero", "previous"])
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the second timed run with the other pressure guess")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 256 MiB (counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Five plain device copies of 256 MiB through the HIP runtime, for scaling HBM counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, C.c_size_t(nbytes))
+    for _ in range(5):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"copy calibration: 5 copies of {nbytes} bytes", flush=True)
 
 
 def make_solver(fluidsim, n, tol, guess="zero"):
@@ -103,6 +135,9 @@ def cpu_baseline(n, seconds):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); this build has no multi-GPU path. "
               "Run with --gpus 1.", file=sys.stderr)
@@ -152,8 +187,9 @@ def main():
         ms, launches, nbytes = prof[dom]
         avg_s = ms / 1e3 / max(launches, 1)
         achieved = nbytes / (ms / 1e3) / 1e9
+        traffic = TRAFFIC_1024.get(dom) if n == 1024 else None
         roof = {"bound": "hbm", "kernel": KIND_TEXT[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
                 "share_of_step": ms / psteps / step_ms,
@@ -161,6 +197,29 @@ def main():
                                     "gbs": v[2] / (v[0] / 1e3) / 1e9 if v[0] else 0.0}
                                 for k, v in prof.items() if v[1]}}
     g.destroy()
+
+    # ablation: the same workload and timing with the other initial pressure guess
+    ablation = None
+    if not a.no_ablation:
+        other = "previous" if a.guess == "zero" else "zero"
+        g2 = make_solver(fluidsim, n, a.tol, other)
+        g2.uploadVelocity(u0, v0)
+        for _ in range(a.warmup):
+            g2.stepCFL(0.5)
+        g2.sync()
+        s0 = g2.stats()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            g2.stepCFL(0.5)
+        g2.sync()
+        el2 = time.perf_counter() - t0
+        s1 = g2.stats()
+        c2 = (s1["total_vcycles"] - s0["total_vcycles"]) / max(a.steps, 1)
+        ablation = {"guess": other, "value": n * n * a.steps / el2, "ms_per_step": el2 / a.steps * 1e3,
+                    "vcycles_per_step": c2, "last_residual": s1["last_residual"]}
+        g2.destroy()
+        print(f"# {other} guess: {ablation['value']:.4g} cell-steps/s, {c2:.2f} V-cycles/step",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -195,6 +254,7 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ablation": ablation,
     }
     print(json.dumps(line), flush=True)
     return 0

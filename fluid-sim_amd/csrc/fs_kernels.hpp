@@ -533,4 +533,90 @@ k_diag_final(const double* __restrict__ part, int nblocks, double* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------ immersed disk
+
+// Disk state for one step: centre, radius, rigid velocity, mask width in world
+// units, cell area, domain size and the number of forcing passes.
+struct Disk {
+  double x, y, r, vx, vy, omega, width, dA, lx, ly, hx, hy;
+  int passes;
+};
+
+__device__ __forceinline__ double wrap_delta(double d, double l) { return d - l * floor(d / l + 0.5); }
+
+// chi = 0.5 (1 - tanh((|p - c| - r) / width)) (src/cpu/solid.js:101-107); 0 when
+// it would fall below the 1e-6 cut of applyCoupling (src/cpu/solid.js:189-190).
+// Beyond t = 8, chi < 1.2e-7, so tanh is not evaluated there.
+__device__ __forceinline__ double disk_chi(const Disk& d, double ddx, double ddy) {
+  double t = (hypot(ddx, ddy) - d.r) / d.width;
+  if (t > 8.0) return 0.0;
+  double chi = 0.5 * (1.0 - tanh(t));
+  return chi < 1e-6 ? 0.0 : chi;
+}
+
+// Direct forcing u += chi (U_s - u) on this cell's u-face and v-face, `passes`
+// times in FP64 registers, then rounded once to FP32 (src/cpu/solid.js:165-233;
+// SOLID_COUPLE src/gpu/shaders.js:587-653). The fluid impulse (x, y) and the
+// torque impulse about the centre, dA included, are reduced per block in FP64.
+__global__ void __launch_bounds__(BX * BY)
+k_solid(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const Disk d, double* __restrict__ part) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double fx = 0, fy = 0, tq = 0;
+  if (i < nx && j < ny) {
+    size_t k = (size_t)j * nx + i;
+    // u-face at (i hx, (j + 1/2) hy)
+    double ux = wrap_delta(i * d.hx - d.x, d.lx), uy = wrap_delta((j + 0.5) * d.hy - d.y, d.ly);
+    double cu = disk_chi(d, ux, uy);
+    if (cu > 0) {
+      double su = d.vx - d.omega * uy;
+      double q = u[k], q0 = q;
+      for (int p = 0; p < d.passes; p++) q += cu * (su - q);
+      u[k] = (float)q;
+      fx = (q - q0) * d.dA;
+      tq = -uy * (q - q0) * d.dA;
+    }
+    // v-face at ((i + 1/2) hx, j hy)
+    double vx = wrap_delta((i + 0.5) * d.hx - d.x, d.lx), vy = wrap_delta(j * d.hy - d.y, d.ly);
+    double cv = disk_chi(d, vx, vy);
+    if (cv > 0) {
+      double sv = d.vy + d.omega * vx;
+      double q = v[k], q0 = q;
+      for (int p = 0; p < d.passes; p++) q += cv * (sv - q);
+      v[k] = (float)q;
+      fy = (q - q0) * d.dA;
+      tq += vx * (q - q0) * d.dA;
+    }
+  }
+  __shared__ double sm[3][4];
+  int lane = threadIdx.x + threadIdx.y * BX;
+  double a = wave_sum(fx), b = wave_sum(fy), c = wave_sum(tq);
+  if ((lane & 63) == 0) {
+    sm[0][lane >> 6] = a;
+    sm[1][lane >> 6] = b;
+    sm[2][lane >> 6] = c;
+  }
+  __syncthreads();
+  if (lane < 3) {
+    double x = sm[lane][0] + sm[lane][1] + sm[lane][2] + sm[lane][3];
+    part[(size_t)flat_block() * 3 + lane] = x;
+  }
+}
+
+// Fixed-order fold of k_solid's per-block partials into out[3] (one workgroup).
+__global__ void __launch_bounds__(256)
+k_solid_final(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  __shared__ double sm[3][256];
+  int t = threadIdx.x;
+  double acc[3] = {0, 0, 0};
+  for (int b = t; b < nblocks; b += 256)
+    for (int q = 0; q < 3; q++) acc[q] += part[(size_t)b * 3 + q];
+  for (int q = 0; q < 3; q++) sm[q][t] = acc[q];
+  __syncthreads();
+  if (t < 3) {
+    double x = 0;
+    for (int k = 0; k < 256; k++) x += sm[t][k];
+    out[t] = x;
+  }
+}
+
 }  // namespace fsk

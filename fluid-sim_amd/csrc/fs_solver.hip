@@ -141,6 +141,11 @@ struct fs_solver_t {
   // forcing
   bool forcing_on = false;
   fs_forcing forcing{};
+  // immersed disk: state for the next step, per-block impulse partials, FP64 total
+  bool solid_on = false;
+  fs_disk solid{};
+  double* solid_part = nullptr;
+  double* solid_imp = nullptr;
   // stats
   double t = 0;
   int64_t steps = 0;
@@ -368,6 +373,34 @@ int enqueue_forcing(fs_solver_t* s) {
   ms.n = k;
   int c = s->cur;
   fsk::k_forcing<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, ms, s->coef);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+// Direct-forcing coupling of the disk, after forcing and before the projection
+// (src/cpu/solver.js:373-383; src/gpu/solver.js:570-573).
+int enqueue_solid(fs_solver_t* s) {
+  if (!s->solid_on) return FS_OK;
+  const fs_disk& d = s->solid;
+  fsk::Disk k{};
+  k.x = d.x;
+  k.y = d.y;
+  k.r = d.r;
+  k.vx = d.vx;
+  k.vy = d.vy;
+  k.omega = d.omega;
+  k.width = s->cfg.solid_mask_width * s->dx;
+  k.dA = s->dx * s->dy;
+  k.lx = s->cfg.lx;
+  k.ly = s->cfg.ly;
+  k.hx = s->dx;
+  k.hy = s->dy;
+  k.passes = d.passes;
+  int c = s->cur;
+  dim3 g = grid2(s->nx, s->ny);
+  fsk::k_solid<<<g, BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, k, s->solid_part);
+  LAUNCHCHK();
+  fsk::k_solid_final<<<1, 256, 0, s->stream>>>(s->solid_part, (int)(g.x * g.y), s->solid_imp);
   LAUNCHCHK();
   return FS_OK;
 }
@@ -696,6 +729,7 @@ int do_step(fs_solver_t* s) {
   if ((rc = enqueue_advection(s))) return rc;
   if ((rc = enqueue_diffusion(s))) return rc;
   if ((rc = enqueue_forcing(s))) return rc;
+  if ((rc = enqueue_solid(s))) return rc;
   int cycles = 0;
   int prc = enqueue_projection(s, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
@@ -855,6 +889,8 @@ void release(fs_solver_t* s) {
   hfree(s->dt_ring);
   dfree(s->diag_part);
   dfree(s->diag_out);
+  dfree(s->solid_part);
+  dfree(s->solid_imp);
   hfree(s->host);
   for (auto& pr : s->prof.pending) {
     (void)hipEventDestroy(pr.a);
@@ -1002,6 +1038,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (!rc) rc = halloc(&s->dt_ring, DT_RING);
   if (!rc) rc = dalloc(&s->diag_part, (size_t)grid_blocks(s->nx, s->ny) * fsk::DIAG_W);
   if (!rc) rc = dalloc(&s->diag_out, 8);
+  if (!rc) rc = dalloc(&s->solid_part, (size_t)grid_blocks(s->nx, s->ny) * 3);
+  if (!rc) rc = dalloc(&s->solid_imp, 3);
   if (!rc) rc = halloc(&s->host, 64);
   if (rc) return bail(rc);
   // zero state: velocity, pressure, multigrid buffers
@@ -1249,6 +1287,44 @@ int fs_set_forcing(void* h, const fs_forcing* f) {
   return FS_OK;
 }
 
+int fs_set_solid(void* h, const fs_disk* d) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (!d) {
+    s->solid_on = false;
+    return FS_OK;
+  }
+  if (!std::isfinite(d->x) || !std::isfinite(d->y) || !(d->r > 0) || !std::isfinite(d->r) ||
+      !std::isfinite(d->vx) || !std::isfinite(d->vy) || !std::isfinite(d->omega))
+    return fail(FS_E_INVALID_ARG, "disk centre, velocity must be finite and radius > 0");
+  if (d->passes < 1 || d->passes > 16) return fail(FS_E_INVALID_ARG, "passes must be in 1..16, got %d", d->passes);
+  if (!(s->cfg.solid_mask_width > 0)) return fail(FS_E_INVALID_ARG, "solid_mask_width must be > 0");
+  if (!s->solid_on) {
+    // no step has coupled yet: the impulse of "the last step" is zero
+    if ((rc = set_device(s))) return rc;
+    if ((rc = fill_f64(s, s->solid_imp, 3, 0.0))) return rc;
+  }
+  s->solid = *d;
+  s->solid_on = true;
+  return FS_OK;
+}
+
+int fs_solid_impulse(void* h, double* out) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  if (!out) return fail(FS_E_INVALID_ARG, "out is NULL");
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (!s->solid_on) return fail(FS_E_INVALID_ARG, "no disk is set");
+  if ((rc = set_device(s))) return rc;
+  HIPCHK(hipMemcpyAsync(s->host, s->solid_imp, sizeof(double) * 3, hipMemcpyDeviceToHost, s->stream));
+  if ((rc = sync_stream(s))) return rc;
+  out[0] = s->host[0];
+  out[1] = s->host[1];
+  out[2] = s->host[2];
+  return FS_OK;
+}
+
 int fs_get_stats(void* h, fs_stats* out) {
   int rc = check_solver(h);
   if (rc) return rc;
@@ -1318,6 +1394,10 @@ int fs_run_kernel(void* h, int kernel, double dt) {
       s->last_vcycles = cyc;
       break;
     }
+    case FS_K_SOLID:
+      if (!s->solid_on) return fail(FS_E_INVALID_ARG, "no disk is set");
+      rc = enqueue_solid(s);
+      break;
     case FS_K_GRADIENT:
       if (s->cfg.pressure_solver != FS_P_MULTIGRID)
         return fail(FS_E_INVALID_ARG, "the FP64 gradient needs the multigrid pressure");

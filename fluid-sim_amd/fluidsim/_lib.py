@@ -24,7 +24,7 @@ GUESS = {"zero": 0, "previous": 1, "learned": 2}
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9}
-KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5}
+KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6}
 
 
 class FsConfig(C.Structure):
@@ -77,6 +77,14 @@ class FsForcing(C.Structure):
     ]
 
 
+class FsDisk(C.Structure):
+    _fields_ = [
+        ("x", C.c_double), ("y", C.c_double), ("r", C.c_double),
+        ("vx", C.c_double), ("vy", C.c_double), ("omega", C.c_double),
+        ("passes", C.c_int),
+    ]
+
+
 class FluidSimError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"{message} (status {code})")
@@ -110,6 +118,8 @@ _SIGNATURES = {
     "fs_set_nu": (C.c_int, [_VP, C.c_double]),
     "fs_set_guess": (C.c_int, [_VP, C.c_int]),
     "fs_set_forcing": (C.c_int, [_VP, C.POINTER(FsForcing)]),
+    "fs_set_solid": (C.c_int, [_VP, C.POINTER(FsDisk)]),
+    "fs_solid_impulse": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "fs_get_stats": (C.c_int, [_VP, C.POINTER(FsStats)]),
     "fs_diagnostics": (C.c_int, [_VP, C.POINTER(FsDiag)]),
     "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),

@@ -179,12 +179,53 @@ class GPUFluidSolver:
         tolerance was set and not met within pressureIterations V-cycles."""
         if splats:
             raise ValueError("splats are a display-only interaction and are not built here")
+        self._push_solid()
         check(self._L.fs_step(self.handle, float(dt)))
 
     def stepCFL(self, cfl=0.5):
         """One step with dt = cfl / (max|u|/dx + max|v|/dy), computed on the device."""
+        self._push_solid()
         check(self._L.fs_step_cfl(self.handle, float(cfl)))
         return self.stats()["last_dt"]
+
+    # ----------------------------------------------------------- immersed disk
+    def setSolid(self, disk, passes=1):
+        """setSolid(disk | None) (src/gpu/solver.js:425-440). `disk` is a
+        fluidsim.solid.RigidDisk (or anything with x, y, r, vx, vy, omega); its
+        current state is sent before every step, as _encodeSolid does
+        (:442-452). `passes` forcing passes per step (reference GPU: 1, CPU: 2)."""
+        self.solid = disk
+        self._solid_passes = int(passes)
+        if disk is None:
+            check(self._L.fs_set_solid(self.handle, None))
+        else:
+            self._push_solid()
+
+    def _push_solid(self):
+        d = getattr(self, "solid", None)
+        if d is None:
+            return
+        fd = _lib.FsDisk(float(d.x), float(d.y), float(d.r), float(d.vx), float(d.vy), float(d.omega),
+                         self._solid_passes)
+        check(self._L.fs_set_solid(self.handle, C.byref(fd)))
+
+    def solidImpulse(self):
+        """FP64 device sum of the last coupling: (fluid impulse x, y, torque impulse), dA included."""
+        out = (C.c_double * 3)()
+        check(self._L.fs_solid_impulse(self.handle, out))
+        return [out[0], out[1], out[2]]
+
+    def integrateSolid(self, dt, gravity=0.0):
+        """integrateSolid(dt) (src/gpu/solver.js:454-510) without the one-step lag: the
+        impulse of the step just taken, Newton's third law, explicit Euler, and the
+        centre wrapped into the periodic box."""
+        d = getattr(self, "solid", None)
+        if d is None:
+            return
+        d.apply_reaction(self.solidImpulse(), dt)
+        d.integrate(dt, gravity=gravity)
+        d.x %= self.lx
+        d.y %= self.ly
 
     def sync(self):
         """device.queue.onSubmittedWorkDone()."""

@@ -306,6 +306,48 @@ static napi_value js_forcing(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setSolid(h, {x, y, r, vx, vy, omega, passes} | null) */
+static napi_value js_solid(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  napi_valuetype t;
+  NAPI_CALL(env, napi_typeof(env, argv[1], &t));
+  int rc;
+  if (t == napi_null || t == napi_undefined) {
+    rc = fs_set_solid(h->solver, NULL);
+  } else {
+    fs_disk d = {0, 0, 0, 0, 0, 0, 1};
+    double x;
+    if (prop_double(env, argv[1], "x", &x)) d.x = x;
+    if (prop_double(env, argv[1], "y", &x)) d.y = x;
+    if (prop_double(env, argv[1], "r", &x)) d.r = x;
+    if (prop_double(env, argv[1], "vx", &x)) d.vx = x;
+    if (prop_double(env, argv[1], "vy", &x)) d.vy = x;
+    if (prop_double(env, argv[1], "omega", &x)) d.omega = x;
+    if (prop_double(env, argv[1], "passes", &x)) d.passes = (int)x;
+    rc = fs_set_solid(h->solver, &d);
+  }
+  if (rc) return throw_fs(env, rc);
+  return NULL;
+}
+
+/* solidImpulse(h) -> [fx, fy, torque] of the last coupling, dA included */
+static napi_value js_solid_impulse(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  double imp[3];
+  int rc = fs_solid_impulse(h->solver, imp);
+  if (rc) return throw_fs(env, rc);
+  napi_value arr;
+  NAPI_CALL(env, napi_create_array_with_length(env, 3, &arr));
+  for (uint32_t k = 0; k < 3; k++) NAPI_CALL(env, napi_set_element(env, arr, k, make_number(env, imp[k])));
+  return arr;
+}
+
 static napi_value js_stats(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -367,7 +409,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readVelocity", js_read}, {"step", js_step},             {"stepCFL", js_step_cfl},
       {"sync", js_sync},       {"maxDivergence", js_max_div},   {"poissonResidualAfter", js_residual_after},
       {"setParam", js_set},    {"setForcing", js_forcing},      {"stats", js_stats},
-      {"diagnostics", js_diag}, {"device", js_device},
+      {"diagnostics", js_diag}, {"device", js_device},         {"setSolid", js_solid},
+      {"solidImpulse", js_solid_impulse},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

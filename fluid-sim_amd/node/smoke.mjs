@@ -25,9 +25,35 @@ async function main() {
   const st = gpu.stats();
   let threw = false;
   try { new GPUFluidSolver(device, { n, pressureSolver: "sor" }); } catch (e) { threw = true; }
+  // a free disk (density 2.5, the RigidDisk update of src/cpu/solid.js) pushed by the
+  // same flow for five steps through setSolid / integrateSolid
+  const mass = 2.5 * Math.PI * 0.6 * 0.6;
+  const disk = {
+    x: 2.0, y: 3.5, r: 0.6, vx: 0.1, vy: 0, omega: 0, force: [0, 0], torque: 0,
+    mass, inertia: 0.5 * mass * 0.6 * 0.6,
+    integrate(dt) {
+      this.vx += (this.force[0] / this.mass) * dt;
+      this.vy += (this.force[1] / this.mass) * dt;
+      this.omega += (this.torque / this.inertia) * dt;
+      this.x += this.vx * dt;
+      this.y += this.vy * dt;
+    },
+  };
+  const g2 = new GPUFluidSolver(device, {
+    n, nu: 0.01, advection: "maccormack", pressureSolver: "multigrid",
+    pressureIterations: 20, tolerance: 1e-6,
+  });
+  g2.uploadVelocity(u, v);
+  g2.setSolid(disk, { passes: 2 });
+  for (let k = 0; k < 5; k++) { g2.step(0.01); await g2.integrateSolid(0.01); }
+  const vel2 = await g2.readVelocity();
+  let du = 0;
+  for (let k = 0; k < n * n; k++) du += vel2.u[k] * (k % 97 + 1);
+  g2.destroy();
   console.log(JSON.stringify({
     device: info.description, checksumU: su, checksumV: sv, t: st.t, steps: st.steps,
     lastResidual: st.lastResidual, maxDivergence: await gpu.maxDivergence(), unknownSelectorThrows: threw,
+    disk: { checksumU: du, x: disk.x, y: disk.y, vx: disk.vx, vy: disk.vy, omega: disk.omega },
   }));
   gpu.destroy();
 }

@@ -99,10 +99,41 @@ export class GPUFluidSolver {
 
   step(dt, { splats = [] } = {}) {
     if (splats.length) throw new Error("splats are display-only and not built here");
+    this._pushSolid();
     addon.step(this._h, dt);
   }
 
-  stepCFL(cfl = 0.5) { addon.stepCFL(this._h, cfl); return addon.stats(this._h).lastDt; }
+  stepCFL(cfl = 0.5) { this._pushSolid(); addon.stepCFL(this._h, cfl); return addon.stats(this._h).lastDt; }
+
+  // setSolid(disk | null) (src/gpu/solver.js:425-440): a RigidDisk from
+  // src/cpu/solid.js or any {x, y, r, vx, vy, omega}; its state is sent before
+  // every step, as _encodeSolid does.
+  setSolid(disk, { passes = 1 } = {}) {
+    this.solid = disk || null;
+    this._solidPasses = passes;
+    if (!disk) addon.setSolid(this._h, null); else this._pushSolid();
+  }
+
+  _pushSolid() {
+    const d = this.solid;
+    if (!d) return;
+    addon.setSolid(this._h, { x: d.x, y: d.y, r: d.r, vx: d.vx, vy: d.vy, omega: d.omega,
+      passes: this._solidPasses });
+  }
+
+  // integrateSolid(dt) (src/gpu/solver.js:454-510): the impulse of the step just
+  // taken (FP64 device sum), Newton's third law, the body's own integrate().
+  async integrateSolid(dt) {
+    const d = this.solid;
+    if (!d) return;
+    const [fx, fy, tq] = addon.solidImpulse(this._h);
+    d.force = [-fx / dt, -fy / dt];
+    d.torque = -tq / dt;
+    d.integrate(dt);
+    const L = this.nx * this.dx;
+    d.x = ((d.x % L) + L) % L;
+    d.y = ((d.y % L) + L) % L;
+  }
 
   sync() { addon.sync(this._h); }
 

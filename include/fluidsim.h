@@ -63,7 +63,8 @@ enum {
   FS_K_DIVERGENCE = 2,  /* div <- divergence(velocity) */
   FS_K_DIFFUSE = 3,     /* velocity <- explicit diffusion(velocity), dt, nu */
   FS_K_PROJECT = 4,     /* one projection with the configured solver */
-  FS_K_GRADIENT = 5     /* velocity <- velocity - grad(pressure) */
+  FS_K_GRADIENT = 5,    /* velocity <- velocity - grad(pressure) */
+  FS_K_SOLID = 6        /* velocity <- direct forcing toward the disk's rigid motion */
 };
 
 /* GPUFluidSolver constructor options, src/gpu/solver.js:64-98 */
@@ -115,6 +116,16 @@ typedef struct fs_forcing {
   uint32_t seed;              /* seed of step k is seed + k */
 } fs_forcing;
 
+/* immersed rigid disk for one step (RigidDisk, src/cpu/solid.js:52-66): centre,
+ * radius, rigid velocity; `passes` direct-forcing passes (the CPU default is 2,
+ * src/cpu/solver.js:258; the reference GPU does 1). The mask width is
+ * fs_config.solid_mask_width cells. */
+typedef struct fs_disk {
+  double x, y, r;
+  double vx, vy, omega;
+  int passes;
+} fs_disk;
+
 int         fs_abi_version(void);
 const char* fs_last_error(void);
 
@@ -160,6 +171,15 @@ int fs_set_guess(void* solver, int guess);
 /* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's
  * seeded band-forcing semantics; NULL disables */
 int fs_set_forcing(void* solver, const fs_forcing* forcing);
+
+/* setSolid(disk) src/gpu/solver.js:425-440 with the state _encodeSolid uploads each
+ * step (:442-452): the next steps couple this disk; NULL removes it. Callers that
+ * move the disk call this again before each step. */
+int fs_set_solid(void* solver, const fs_disk* disk);
+/* the sum integrateSolid reads back (src/gpu/solver.js:454-510), reduced on the
+ * device in FP64 for the most recent coupling: out[0..1] = fluid impulse
+ * sum(du dA), sum(dv dA); out[2] = torque impulse about the centre */
+int fs_solid_impulse(void* solver, double* out);
 
 int fs_get_stats(void* solver, fs_stats* out);
 int fs_diagnostics(void* solver, fs_diag* out);

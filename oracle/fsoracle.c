@@ -812,9 +812,66 @@ int or_energy_spectrum(int nx, int ny, const double *cu, const double *cv, doubl
   return kmax;
 }
 
+/* ---------------------------------------------------------- immersed disk */
+
+/* shortest signed separation on a periodic axis (src/cpu/solid.js:38-40);
+ * Math.round rounds halves up, so floor(x + 0.5) */
+double or_wrap_delta(double d, double l) { return d - l * floor(d / l + 0.5); }
+
+/* smoothed indicator 0.5 (1 - tanh((|x - c| - r) / width)) (src/cpu/solid.js:101-107) */
+double or_disk_mask(const or_disk *s, double px, double py, double lx, double ly, double width) {
+  double dx = or_wrap_delta(px - s->x, lx), dy = or_wrap_delta(py - s->y, ly);
+  double d = hypot(dx, dy) - s->r;
+  return 0.5 * (1.0 - tanh(d / width));
+}
+
+/* rigid-body velocity: translation + omega x r (src/cpu/solid.js:110-115) */
+void or_disk_velocity(const or_disk *s, double px, double py, double lx, double ly, double *vx, double *vy) {
+  double dx = or_wrap_delta(px - s->x, lx), dy = or_wrap_delta(py - s->y, ly);
+  *vx = s->vx - s->omega * dy;
+  *vy = s->vy + s->omega * dx;
+}
+
+/* direct forcing u += chi (U_s - u) on u-faces then v-faces, `passes` times,
+ * faces with chi < 1e-6 skipped (src/cpu/solid.js:165-233) */
+void or_apply_coupling(int nx, int ny, double lx, double ly, double *u, double *v, const or_disk *s,
+                       double mask_width, int passes, double *imp) {
+  double hx = lx / nx, hy = ly / ny, dA = hx * hy, width = mask_width * hx;
+  double fx = 0, fy = 0, tq = 0;
+  for (int pass = 0; pass < passes; pass++) {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double px = i * hx, py = (j + 0.5) * hy;
+        double chi = or_disk_mask(s, px, py, lx, ly, width);
+        if (chi < 1e-6) continue;
+        double su, sv;
+        or_disk_velocity(s, px, py, lx, ly, &su, &sv);
+        size_t k = (size_t)j * nx + i;
+        double du = chi * (su - u[k]);
+        u[k] += du;
+        fx += du * dA;
+        tq += -or_wrap_delta(py - s->y, ly) * du * dA;
+      }
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double px = (i + 0.5) * hx, py = j * hy;
+        double chi = or_disk_mask(s, px, py, lx, ly, width);
+        if (chi < 1e-6) continue;
+        double su, sv;
+        or_disk_velocity(s, px, py, lx, ly, &su, &sv);
+        size_t k = (size_t)j * nx + i;
+        double dv = chi * (sv - v[k]);
+        v[k] += dv;
+        fy += dv * dA;
+        tq += or_wrap_delta(px - s->x, lx) * dv * dA;
+      }
+  }
+  if (imp) { imp[0] = fx; imp[1] = fy; imp[2] = tq; }
+}
+
 /* ------------------------------------------------------------------- the step */
 
-/* src/cpu/solver.js:336-398: advect -> diffuse -> force -> drag -> project */
+/* src/cpu/solver.js:336-398: advect -> diffuse -> force -> drag -> solid -> project */
 int or_step(const or_step_cfg *c, double *u, double *v, double *p_out, double dt, uint32_t force_seed) {
   int nx = c->nx, ny = c->ny;
   size_t n = (size_t)nx * ny;
@@ -839,6 +896,10 @@ int or_step(const or_step_cfg *c, double *u, double *v, double *p_out, double dt
   if (c->forcing)
     or_band_forcing(nx, ny, c->lx, c->ly, u, v, c->kf, c->width, c->amp, force_seed, dt);
   if (c->drag > 0) or_linear_drag((int)n, u, v, c->drag, dt);
+  if (c->solid) {
+    or_disk d = {c->sx, c->sy, c->sr, c->svx, c->svy, c->somega};
+    or_apply_coupling(nx, ny, c->lx, c->ly, u, v, &d, c->mask_width, c->passes, NULL);
+  }
 
   if (c->projection != OR_PROJ_NONE) {
     or_divergence(nx, ny, c->lx, c->ly, u, v, div);

@@ -51,7 +51,16 @@ typedef struct {
   /* band forcing (src/cpu/solver.js:179-237); off when forcing == 0 */
   int forcing;
   double kf, width, amp, drag;
+  /* one immersed disk with prescribed rigid motion (0 = none) */
+  int solid;
+  double sx, sy, sr, svx, svy, somega, mask_width;
+  int passes;
 } or_step_cfg;
+
+/* rigid disk for the direct-forcing coupling (src/cpu/solid.js:52-114) */
+typedef struct {
+  double x, y, r, vx, vy, omega;
+} or_disk;
 
 void   or_set_threads(int n);
 int    or_get_threads(void);
@@ -119,6 +128,14 @@ double or_velocity_error_tg(int nx, int ny, double lx, double ly, const double *
 
 /* shell-binned energy spectrum (src/core/fft.js:196-234); E has kmax+1 entries */
 int    or_energy_spectrum(int nx, int ny, const double *cu, const double *cv, double *E, int *counts);
+
+/* immersed disk (src/cpu/solid.js:38-40, :101-114, :165-233) */
+double or_wrap_delta(double d, double l);
+double or_disk_mask(const or_disk *s, double px, double py, double lx, double ly, double width);
+void   or_disk_velocity(const or_disk *s, double px, double py, double lx, double ly, double *vx, double *vy);
+/* `passes` direct-forcing passes in place; imp = fluid impulse (x, y) and torque impulse, dA included */
+void   or_apply_coupling(int nx, int ny, double lx, double ly, double *u, double *v, const or_disk *s,
+                         double mask_width, int passes, double *imp);
 
 /* one full step (src/cpu/solver.js:336-398); p_out may be NULL */
 int    or_step(const or_step_cfg *cfg, double *u, double *v, double *p_out, double dt, uint32_t force_seed);

@@ -33,7 +33,17 @@ class StepCfg(C.Structure):
         ("cg_maxit", C.c_int),
         ("forcing", C.c_int),
         ("kf", C.c_double), ("width", C.c_double), ("amp", C.c_double), ("drag", C.c_double),
+        ("solid", C.c_int),
+        ("sx", C.c_double), ("sy", C.c_double), ("sr", C.c_double),
+        ("svx", C.c_double), ("svy", C.c_double), ("somega", C.c_double), ("mask_width", C.c_double),
+        ("passes", C.c_int),
     ]
+
+
+class Disk(C.Structure):
+    """or_disk: centre, radius and rigid velocity (src/cpu/solid.js:52-66)."""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("r", C.c_double),
+                ("vx", C.c_double), ("vy", C.c_double), ("omega", C.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -99,6 +109,10 @@ def lib():
         "or_velocity_error_tg": (d, [i, i, d, d, p, p, d, d]),
         "or_energy_spectrum": (i, [i, i, p, p, p, _I]),
         "or_step": (i, [C.POINTER(StepCfg), p, p, C.c_void_p, d, C.c_uint32]),
+        "or_wrap_delta": (d, [d, d]),
+        "or_disk_mask": (d, [C.POINTER(Disk), d, d, d, d, d]),
+        "or_disk_velocity": (None, [C.POINTER(Disk), d, d, d, d, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "or_apply_coupling": (None, [i, i, d, d, p, p, C.POINTER(Disk), d, i, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -302,13 +316,44 @@ def set_threads(n):
     lib().or_set_threads(int(n))
 
 
+def _disk_tuple(d):
+    return (float(d["x"]), float(d["y"]), float(d["r"]), float(d.get("vx", 0.0)), float(d.get("vy", 0.0)),
+            float(d.get("omega", 0.0)))
+
+
+def make_disk(d):
+    return Disk(*_disk_tuple(d))
+
+
+def wrap_delta(d, l):
+    return lib().or_wrap_delta(d, l)
+
+
+def disk_mask(d, px, py, lx, ly, width):
+    return lib().or_disk_mask(C.byref(make_disk(d)), px, py, lx, ly, width)
+
+
+def disk_velocity(d, px, py, lx, ly):
+    a, b = C.c_double(), C.c_double()
+    lib().or_disk_velocity(C.byref(make_disk(d)), px, py, lx, ly, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def apply_coupling(nx, ny, u, v, d, mask_width=1.5, passes=1, lx=TWO_PI, ly=TWO_PI):
+    """In place on float64 u, v; returns the fluid impulse (fx, fy, torque), dA included."""
+    imp = np.zeros(3)
+    lib().or_apply_coupling(nx, ny, lx, ly, u, v, C.byref(make_disk(d)), mask_width, passes, imp)
+    return imp
+
+
 class Solver:
     """Oracle counterpart of FluidSolver (src/cpu/solver.js:241-418), f64."""
 
     def __init__(self, n=64, nx=None, ny=None, lx=TWO_PI, ly=TWO_PI, nu=0.0,
                  advection="maccormack", explicit_diffusion=True, projection="fft",
                  iterations=40, mg_min_size=8, mg_pre=2, mg_post=2, mg_coarse=40,
-                 cg_tol=1e-10, cg_maxit=2000, forcing=None, drag=0.0):
+                 cg_tol=1e-10, cg_maxit=2000, forcing=None, drag=0.0, disk=None, mask_width=1.5,
+                 passes=1):
         nx = nx or n
         ny = ny or n
         if advection not in ADV:
@@ -318,7 +363,8 @@ class Solver:
         f = forcing or {}
         self.cfg = StepCfg(nx, ny, lx, ly, nu, ADV[advection], int(explicit_diffusion), PROJ[projection],
                            iterations, mg_min_size, mg_pre, mg_post, mg_coarse, cg_tol, cg_maxit,
-                           1 if forcing else 0, f.get("kf", 0.0), f.get("width", 0.0), f.get("amp", 0.0), drag)
+                           1 if forcing else 0, f.get("kf", 0.0), f.get("width", 0.0), f.get("amp", 0.0), drag,
+                           1 if disk else 0, *(_disk_tuple(disk) if disk else (0.0,) * 6), mask_width, passes)
         self.nx, self.ny, self.lx, self.ly = nx, ny, lx, ly
         self.u = np.zeros(nx * ny)
         self.v = np.zeros(nx * ny)

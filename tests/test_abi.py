@@ -124,6 +124,6 @@ def test_node_addon_loads_and_exports_the_shim_surface():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     assert {"create", "destroy", "uploadVelocity", "readVelocity", "step", "stepCFL", "sync", "maxDivergence",
-            "poissonResidualAfter", "setParam", "setForcing", "stats"} <= names
+            "poissonResidualAfter", "setParam", "setForcing", "stats", "setSolid", "solidImpulse"} <= names
     shim = open(os.path.join(ROOT, "fluid-sim_amd", "node", "solver.mjs")).read()
     assert "export async function createDevice" in shim and "export class GPUFluidSolver" in shim

@@ -45,3 +45,18 @@ def test_node_shim_matches_python_binding_bitwise():
     assert js["steps"] == 5 and abs(js["t"] - 0.05) < 1e-15
     assert js["lastResidual"] == st["last_residual"] and js["lastResidual"] <= 1e-6
     assert js["unknownSelectorThrows"] is True
+    # the disk path: same library, the disk state pushed and integrated by each binding
+    from fluidsim.solid import RigidDisk
+    disk = RigidDisk(x=2.0, y=3.5, r=0.6, density=2.5, vx=0.1)
+    with fluidsim.GPUFluidSolver(0, n=n, nu=0.01, advection="maccormack", pressureSolver="multigrid",
+                                 pressureIterations=20, tolerance=1e-6) as g:
+        g.uploadVelocity(u, v)
+        g.setSolid(disk, passes=2)
+        for _ in range(5):
+            g.step(0.01)
+            g.integrateSolid(0.01)
+        vel = g.readVelocity()
+    du = float(np.sum(vel["u"].astype(np.float64) * (k % 97 + 1)))
+    assert abs(js["disk"]["checksumU"] - du) <= 1e-9 * max(1.0, abs(du))
+    for key in ("x", "y", "vx", "vy", "omega"):
+        assert abs(js["disk"][key] - getattr(disk, key)) <= 1e-12 * max(1.0, abs(getattr(disk, key))), key

@@ -241,6 +241,37 @@ def test_tolerance_gate_on_turbulence(fs, O):
     assert d["max_divergence"] < 5e-4
 
 
+@pytest.mark.parametrize("n,kf,steps", [(64, 8, 20), (256, 16, 5)])
+def test_forced_steps_match_oracle(fs, O, n, kf, steps):
+    # band forcing + linear drag (src/cpu/solver.js:179-237, order :354-358): step k
+    # uses LCG seed 1000 + k; FP32 GPU with an FP64-gated projection against the f64
+    # oracle with the exact FFT projection
+    nu, dt = 0.01, 0.01
+    forcing = dict(kf=kf, width=2.0, amp=0.5, drag=0.05, seed=1000)
+    u0, v0 = O.taylor_green(n, n)
+    with solver(fs, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-9, mgMinSize=4) as g:
+        g.setForcing(forcing)
+        g.uploadVelocity(u0, v0)
+        for _ in range(steps):
+            g.step(dt)
+            assert g.stats()["last_residual"] <= 1e-9
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft",
+                 forcing=dict(kf=kf, width=2.0, amp=0.5), drag=0.05)
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    for k in range(steps):
+        s.step(dt, force_seed=1000 + k)
+    err = max(rel(out["u"], s.u), rel(out["v"], s.v))
+    assert err < 2e-5, err
+    # not vacuous: the unforced run is far from the forced one
+    s0 = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s0.u[:], s0.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        s0.step(dt)
+    assert rel(s0.u, s.u) > 100 * err
+
+
 def test_runs_are_bitwise_deterministic(fs, O):
     # test/solid.test.js:339-358 style: two identical runs, identical bits
     u0, v0 = O.js_random_velocity(N, 31337)
@@ -345,3 +376,82 @@ def test_profile_counts_every_launch_and_its_bytes(fs, O):
     assert p["up"][1] == 3 * 2 and p["down"][1] == 3 and p["coarse"][1] == 3
     for k, (ms_k, l_k, b_k) in p.items():
         assert (ms_k > 0) == (l_k > 0) and (b_k > 0) == (l_k > 0), k
+
+
+# ------------------------------------------------------------- immersed disk
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_solid_kernel_matches_oracle(fs, O, passes):
+    # one coupling pass over a rough field, moving and rotating disk across the seam
+    from fluidsim.solid import RigidDisk
+    n = 64
+    u0, v0 = O.js_random_velocity(n, 4242)
+    disk = RigidDisk(x=0.3, y=3.0, r=0.9, vx=-0.4, vy=0.25, omega=1.3)
+    with solver(fs, n=n) as g:
+        g.uploadVelocity(u0, v0)
+        g.setSolid(disk, passes=passes)
+        g.runKernel("solid", 0.0)
+        out = g.readVelocity()
+        imp = g.solidImpulse()
+    u, v = f32(u0), f32(v0)
+    ref = O.apply_coupling(n, n, u, v, disk.as_dict(), passes=passes)
+    assert np.max(np.abs(out["u"] - u)) < 4e-7 and np.max(np.abs(out["v"] - v)) < 4e-7
+    for a, b in zip(imp, ref):
+        assert abs(a - b) <= 1e-9 * max(abs(b), 1e-3), (imp, ref)
+
+
+def test_pinned_disk_steps_match_oracle(fs, O):
+    # fixed disk in a Taylor-Green flow: advect, diffuse, couple (2 passes), project
+    from fluidsim.solid import RigidDisk
+    n, nu, dt, steps = 64, 0.01, 0.01, 20
+    u0, v0 = O.taylor_green(n, n)
+    disk = RigidDisk(x=2.0, y=3.5, r=0.6, fixed=True)
+    with solver(fs, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-9, mgMinSize=4) as g:
+        g.uploadVelocity(u0, v0)
+        g.setSolid(disk, passes=2)
+        for _ in range(steps):
+            g.step(dt)
+            g.integrateSolid(dt)
+        out = g.readVelocity()
+        d = g.diagnostics()
+    assert disk.x == 2.0 and disk.vx == 0.0
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft",
+                 disk=disk.as_dict(), passes=2)
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        s.step(dt)
+    err = max(rel(out["u"], s.u), rel(out["v"], s.v))
+    assert err < 2e-5, err
+    s0 = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s0.u[:], s0.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        s0.step(dt)
+    assert rel(s0.u, s.u) > 100 * err
+    assert d["max_divergence"] < 1e-4
+
+
+def test_free_disk_exchanges_momentum_with_the_fluid(fs, O):
+    # Newton's third law through the device reduction: the fluid's momentum change
+    # over a coupling pass equals the reported impulse, and the disk takes the opposite
+    from fluidsim.solid import RigidDisk
+    n, dt = 128, 0.01
+    u0 = np.full(n * n, 1.0)
+    v0 = np.full(n * n, -0.5)
+    disk = RigidDisk(x=3.0, y=3.0, r=0.7, density=2.5, vx=-0.3, vy=0.2)
+    with solver(fs, n=n) as g:
+        g.uploadVelocity(u0, v0)
+        p0 = g.diagnostics()
+        g.setSolid(disk)
+        g.runKernel("solid", dt)
+        p1 = g.diagnostics()
+        imp = g.solidImpulse()
+        m0 = disk.momentum()
+        disk.apply_reaction(imp, dt)
+        disk.integrate(dt)
+        m1 = disk.momentum()
+    dpx = p1["momentum_x"] - p0["momentum_x"]
+    dpy = p1["momentum_y"] - p0["momentum_y"]
+    # the stored field is FP32: its momentum change carries the rounding of each face
+    assert abs(dpx - imp[0]) < 1e-5 * abs(imp[0]) and abs(dpy - imp[1]) < 1e-5 * abs(imp[1])
+    assert abs((m1[0] - m0[0]) + imp[0]) < 1e-12 and abs((m1[1] - m0[1]) + imp[1]) < 1e-12

@@ -368,3 +368,117 @@ def test_band_forcing_modes_deterministic_and_in_shell(O):
     assert (ph >= 0).all() and (ph < TWO_PI).all()
     kx3, _, _ = O.band_modes(1001, 32.0, 2.0)
     assert not (kx3 == kx).all()
+
+
+# ------------------------------------------------- immersed disk (src/cpu/solid.js)
+# restating test/solid.test.js:29-160 on the oracle's coupling
+
+def _disk(**kw):
+    from fluidsim.solid import RigidDisk
+    return RigidDisk(**kw)
+
+
+def test_disk_mask_known_answers_and_wrap(O):
+    d = dict(x=math.pi, y=math.pi, r=0.8)
+    w = 1.5 * (TWO_PI / 128)
+    assert O.disk_mask(d, math.pi, math.pi, TWO_PI, TWO_PI, w) > 0.999
+    assert O.disk_mask(d, math.pi + 2.0, math.pi, TWO_PI, TWO_PI, w) < 1e-6
+    assert abs(O.disk_mask(d, math.pi + 0.8, math.pi, TWO_PI, TWO_PI, w) - 0.5) < 1e-9
+    seam = dict(x=0.1, y=math.pi, r=0.5)
+    assert O.disk_mask(seam, TWO_PI - 0.1, math.pi, TWO_PI, TWO_PI, 0.05) > 0.9
+    # Math.round semantics of the periodic separation: halves round up
+    assert O.wrap_delta(math.pi, TWO_PI) == -math.pi
+    assert O.wrap_delta(-math.pi, TWO_PI) == -math.pi
+
+
+def test_disk_velocity_includes_rotation(O):
+    d = dict(x=1, y=1, r=0.4, vx=2, vy=-1, omega=3)
+    assert O.disk_velocity(d, 1, 1, TWO_PI, TWO_PI) == (2.0, -1.0)
+    vx, vy = O.disk_velocity(d, 1.2, 1, TWO_PI, TWO_PI)
+    assert abs(vx - 2) < 1e-12 and abs(vy - (-1 + 0.6)) < 1e-12
+
+
+def _fluid_momentum(n, u, v):
+    dA = (TWO_PI / n) ** 2
+    return u.sum() * dA, v.sum() * dA
+
+
+def test_coupling_conserves_total_momentum_with_a_free_disk(O):
+    n = 64
+    vals, _ = O.js_lcg(4242, 2 * n * n)
+    # test/solid.test.js:64-69 uses its own LCG; any seeded field serves the invariant
+    u = 1 + (vals[0::2] - 0.5)
+    v = vals[1::2] - 0.5
+    disk = _disk(x=math.pi, y=math.pi, r=0.7, density=2.5, vx=-0.3, vy=0.2)
+    dt = 0.01
+    f0, s0 = _fluid_momentum(n, u, v), disk.momentum()
+    imp = O.apply_coupling(n, n, u, v, disk.as_dict())
+    disk.apply_reaction(imp, dt)
+    disk.integrate(dt)
+    f1, s1 = _fluid_momentum(n, u, v), disk.momentum()
+    before = math.hypot(f0[0] + s0[0], f0[1] + s0[1])
+    drift = math.hypot(f1[0] + s1[0] - f0[0] - s0[0], f1[1] + s1[1] - f0[1] - s0[1]) / before
+    assert drift < 1e-12
+
+
+def test_reported_impulse_is_the_fluid_momentum_change(O):
+    n = 48
+    u = np.full(n * n, 1.4)
+    v = np.full(n * n, -0.6)
+    dt = 0.02
+    f0 = _fluid_momentum(n, u, v)
+    imp = O.apply_coupling(n, n, u, v, dict(x=3, y=3, r=0.6))
+    f1 = _fluid_momentum(n, u, v)
+    assert abs((f1[0] - f0[0]) - imp[0]) < 1e-12
+    assert abs((f1[1] - f0[1]) - imp[1]) < 1e-12
+    disk = _disk(x=3, y=3, r=0.6, density=1.7)
+    disk.apply_reaction(imp, dt)
+    assert abs(disk.force[0] * dt + imp[0]) < 1e-12 and abs(disk.force[1] * dt + imp[1]) < 1e-12
+
+
+def test_pinned_disk_absorbs_momentum(O):
+    n = 48
+    u = np.ones(n * n)
+    v = np.zeros(n * n)
+    f0 = _fluid_momentum(n, u, v)[0]
+    O.apply_coupling(n, n, u, v, dict(x=3, y=3, r=0.7))
+    assert _fluid_momentum(n, u, v)[0] < f0 - 1e-6
+
+
+@pytest.mark.parametrize("passes", [1, 2, 3])
+def test_slip_falls_geometrically_with_passes(O, passes):
+    # uniform flow past a disk at rest: after p passes u = (1 - chi)^p exactly, with
+    # chi evaluated independently from the tanh mask at each u-face
+    n = 64
+    h = TWO_PI / n
+    u = np.ones(n * n)
+    v = np.zeros(n * n)
+    d = dict(x=3.0, y=2.5, r=0.9)
+    O.apply_coupling(n, n, u, v, d, passes=passes)
+    i = np.arange(n)
+    X, Y = np.meshgrid(i * h, (i + 0.5) * h)
+    dx = (X - 3.0) - TWO_PI * np.floor((X - 3.0) / TWO_PI + 0.5)
+    dy = (Y - 2.5) - TWO_PI * np.floor((Y - 2.5) / TWO_PI + 0.5)
+    chi = 0.5 * (1 - np.tanh((np.hypot(dx, dy) - 0.9) / (1.5 * h)))
+    expect = np.where(chi < 1e-6, 1.0, (1 - chi) ** passes).ravel()
+    assert np.max(np.abs(u - expect)) < 1e-14
+    assert np.all(v == 0)
+
+
+def test_torque_impulse_is_the_angular_momentum_change(O):
+    # solid-body rotation about the disk centre, disk at rest: the fluid loses
+    # angular momentum and the reported torque impulse is exactly that change
+    n = 64
+    h = TWO_PI / n
+    cx, cy, om = 3.1, 2.9, 0.7
+    i = np.arange(n)
+    Xu, Yu = np.meshgrid(i * h, (i + 0.5) * h)
+    Xv, Yv = np.meshgrid((i + 0.5) * h, i * h)
+    wd = lambda d: d - TWO_PI * np.floor(d / TWO_PI + 0.5)
+    u = (-om * wd(Yu - cy)).ravel()
+    v = (om * wd(Xv - cx)).ravel()
+    L = lambda u, v: (np.sum(-wd(Yu - cy).ravel() * u) + np.sum(wd(Xv - cx).ravel() * v)) * h * h
+    L0 = L(u, v)
+    imp = O.apply_coupling(n, n, u, v, dict(x=cx, y=cy, r=0.8), passes=2)
+    assert imp[2] < 0
+    assert abs((L(u, v) - L0) - imp[2]) < 1e-12 * abs(L0)
