@@ -23,10 +23,19 @@
 
 namespace fsk {
 
-constexpr int TX = 64;
-constexpr int TY = 32;
-constexpr int NT = 256;
 constexpr int COARSE_N = 64;   // levels at or below this size are solved by one workgroup
+
+// Tile shape: TX x TY cells per workgroup of NT threads. Each grid-wide level picks
+// the largest shape that still gives every CU a tile (256 tiles), so the small
+// levels, which are latency-bound, are spread over the whole chip.
+template <int TX_, int TY_, int NT_>
+struct TileShape {
+  static constexpr int TX = TX_, TY = TY_, NT = NT_;
+};
+using Tile64 = TileShape<64, 32, 256>;
+using Tile32 = TileShape<32, 32, 256>;
+using Tile16 = TileShape<16, 16, 128>;
+using Tile8 = TileShape<8, 8, 64>;
 
 // Blocks are issued round-robin over the 8 XCDs; give each XCD a contiguous band of
 // tiles so neighbouring tiles (which share halos) share an L2.
@@ -44,7 +53,7 @@ __device__ __forceinline__ int wrapn(int i, int n) {
 // Load a periodic RW x RH window with origin (ox, oy) of an n x n field into LDS.
 // Every load a thread owns is issued before any of them is stored, so the
 // window costs one memory latency instead of one per row of threads.
-template <int RW, int RH, typename T>
+template <int RW, int RH, int NT, typename T>
 __device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __restrict__ src, int n, int ox, int oy) {
   constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
   float tmp[K];
@@ -66,7 +75,7 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __
 // One red-black half-sweep on the LDS region [LO, RW-LO) x [LO, RH-LO), visiting
 // only the cells of `colour` (par0 = colour parity of LDS cell (0,0)). A colour pass
 // reads only the other colour, so all of a thread's reads are issued before its writes.
-template <int RW, int RH, int LO>
+template <int RW, int RH, int LO, int NT>
 __device__ __forceinline__ void lds_half_sweep(float* __restrict__ sx, const float* __restrict__ sr, int colour,
                                                int par0, float h2) {
   constexpr int w = RW - 2 * LO, hgt = RH - 2 * LO, hw = w / 2;  // w is even
@@ -94,22 +103,22 @@ __device__ __forceinline__ void lds_half_sweep(float* __restrict__ sx, const flo
 }
 
 // Half-sweeps S .. SEND-1 on regions LO = S + LOOFF (one cell smaller each time).
-template <int RW, int RH, int S, int SEND, int LOOFF>
+template <int RW, int RH, int S, int SEND, int LOOFF, int NT>
 struct HalfSweeps {
   static __device__ __forceinline__ void run(float* sx, const float* sr, int par0, float h2) {
-    lds_half_sweep<RW, RH, S + LOOFF>(sx, sr, S & 1, par0, h2);
+    lds_half_sweep<RW, RH, S + LOOFF, NT>(sx, sr, S & 1, par0, h2);
     __syncthreads();
-    HalfSweeps<RW, RH, S + 1, SEND, LOOFF>::run(sx, sr, par0, h2);
+    HalfSweeps<RW, RH, S + 1, SEND, LOOFF, NT>::run(sx, sr, par0, h2);
   }
 };
-template <int RW, int RH, int SEND, int LOOFF>
-struct HalfSweeps<RW, RH, SEND, SEND, LOOFF> {
+template <int RW, int RH, int SEND, int LOOFF, int NT>
+struct HalfSweeps<RW, RH, SEND, SEND, LOOFF, NT> {
   static __device__ __forceinline__ void run(float*, const float*, int, float) {}
 };
 
 // Pre-smoothing from zero: red half-sweep on the whole region, then 2*SW-1
 // half-sweeps on shrinking regions.
-template <int SW, int RW, int RH>
+template <int SW, int RW, int RH, int NT>
 __device__ __forceinline__ void tile_smooth_from_zero(float* __restrict__ sx, const float* __restrict__ sr, int par0,
                                                       float h2) {
   for (int q = threadIdx.x; q < RW * RH; q += NT) {
@@ -118,14 +127,15 @@ __device__ __forceinline__ void tile_smooth_from_zero(float* __restrict__ sx, co
     sx[q] = red ? (0.f - h2 * sr[q]) * 0.25f : 0.f;
   }
   __syncthreads();
-  HalfSweeps<RW, RH, 1, 2 * SW, 0>::run(sx, sr, par0, h2);
+  HalfSweeps<RW, RH, 1, 2 * SW, 0, NT>::run(sx, sr, par0, h2);
 }
 
 // Store the tile interior and the restricted residual of each 2x2 block of it.
-template <int H, int RW>
+template <int H, int RW, class TS>
 __device__ __forceinline__ void tile_store_and_restrict(const float* __restrict__ sx, const float* __restrict__ sr,
                                                         float* __restrict__ x, float* __restrict__ crhs, int n,
                                                         int tx0, int ty0, float ih2) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
@@ -152,10 +162,11 @@ __device__ __forceinline__ void tile_store_and_restrict(const float* __restrict_
 
 // -------------------------------------------------------------- level >= 1 down
 
-template <int SW>
-__global__ void __launch_bounds__(NT)
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
 k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2, float ih2,
            float* __restrict__ crhs, int tiles_x, int nblocks) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
   __shared__ float sx[RH * RW];
@@ -163,10 +174,10 @@ k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2
   int t = xcd_tile(nblocks);
   int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   int ox = tx0 - H, oy = ty0 - H;  // global coordinate of LDS (0,0)
-  load_window<RW, RH>(sr, rhs, n, ox, oy);
+  load_window<RW, RH, NT>(sr, rhs, n, ox, oy);
   __syncthreads();
-  tile_smooth_from_zero<SW, RW, RH>(sx, sr, (ox + oy) & 1, h2);
-  tile_store_and_restrict<H, RW>(sx, sr, x, crhs, n, tx0, ty0, ih2);
+  tile_smooth_from_zero<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
+  tile_store_and_restrict<H, RW, TS>(sx, sr, x, crhs, n, tx0, ty0, ih2);
 }
 
 // -------------------------------------------------------------- level 0 down
@@ -176,12 +187,13 @@ k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2
 // r = div u* - L p' (src/core/grid.js:191-206, src/cpu/projection.js:48-62) on the
 // tile + halo 2*SW; p' and r are stored for the interior and the tile's max|r| goes
 // to rpart[tile]. residual_only stops after that.
-template <int SW>
-__global__ void __launch_bounds__(NT)
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
 k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
             const float* __restrict__ e_prev, double* __restrict__ pout, float* __restrict__ r32,
             float* __restrict__ x, float* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2,
             int tiles_x, int nblocks, double* __restrict__ rpart, int residual_only) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
   constexpr int PW = RW + 2, PH = RH + 2;  // p' needs one more ring
@@ -262,8 +274,8 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
     }
   }
   if (residual_only) return;
-  tile_smooth_from_zero<SW, RW, RH>(sx, sr, (ox + oy) & 1, h2);
-  tile_store_and_restrict<H, RW>(sx, sr, x, crhs, n, tx0, ty0, ih2);
+  tile_smooth_from_zero<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
+  tile_store_and_restrict<H, RW, TS>(sx, sr, x, crhs, n, tx0, ty0, ih2);
 }
 
 // -------------------------------------------------------------------- up
@@ -271,10 +283,11 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
 // x += bilinear prolongation of the coarse correction (PROLONG, src/gpu/multigrid.js:99-131),
 // then SW red-black sweeps. Reads xin and writes xout: a tile's halo is another
 // tile's interior, so the update cannot be in place across blocks.
-template <int SW>
-__global__ void __launch_bounds__(NT)
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
 k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* __restrict__ rhs,
          const float* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
   constexpr int CW = RW / 2 + 3, CH = RH / 2 + 3;  // coarse footprint of the region
@@ -297,8 +310,8 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
       xv[k] = xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
     }
   }
-  load_window<CW, CH>(sc, xc, cn, cox, coy);
-  load_window<RW, RH>(sr, rhs, n, ox, oy);
+  load_window<CW, CH, NT>(sc, xc, cn, cox, coy);
+  load_window<RW, RH, NT>(sr, rhs, n, ox, oy);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -317,7 +330,7 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
     }
   }
   __syncthreads();
-  HalfSweeps<RW, RH, 0, 2 * SW, 1>::run(sx, sr, (ox + oy) & 1, h2);
+  HalfSweeps<RW, RH, 0, 2 * SW, 1, NT>::run(sx, sr, (ox + oy) & 1, h2);
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;

@@ -523,15 +523,36 @@ int mg_vcycle(fs_solver_t* s) {
 
 // ----------------------------------------------------------- fused V-cycle
 
+// Tile shape of a grid-wide level: the largest one that still yields a tile per CU.
+enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8 };
+static const int kTileW[4] = {64, 32, 16, 8}, kTileH[4] = {32, 32, 16, 8};
+
+inline int tile_kind(int n) {
+  for (int k = TILE64; k < TILE8; k++) {
+    long tiles = (long)((n + kTileW[k] - 1) / kTileW[k]) * ((n + kTileH[k] - 1) / kTileH[k]);
+    if (tiles >= 256) return k;
+  }
+  return TILE8;
+}
+
 inline dim3 tiles_grid(int n, int* tiles_x, int* nblocks) {
-  int tx = (n + fsk::TX - 1) / fsk::TX, ty = (n + fsk::TY - 1) / fsk::TY;
+  int k = tile_kind(n);
+  int tx = (n + kTileW[k] - 1) / kTileW[k], ty = (n + kTileH[k] - 1) / kTileH[k];
   *tiles_x = tx;
   *nblocks = tx * ty;
   return dim3((unsigned)(tx * ty));
 }
 
+#define DISPATCH_TILE(n, fn, SW, ...)                                                        \
+  (tile_kind(n) == TILE64   ? fn<SW, fsk::Tile64>(__VA_ARGS__)                               \
+   : tile_kind(n) == TILE32 ? fn<SW, fsk::Tile32>(__VA_ARGS__)                               \
+   : tile_kind(n) == TILE16 ? fn<SW, fsk::Tile16>(__VA_ARGS__)                               \
+                            : fn<SW, fsk::Tile8>(__VA_ARGS__))
+
 template <int PRE>
-int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only);
+template <int PRE, class TS>
+int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
   Level& L0 = s->lv[0];
   int tx, nb;
   dim3 g = tiles_grid(L0.n, &tx, &nb);
@@ -539,31 +560,43 @@ int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_onl
   int c = s->cur;
   // u, v (8) + p (8) + previous correction (4) read; p (8) + r (4) written; then x (4) and the coarse rhs (1)
   double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? 4 : 0) + 8 + 4 + (residual_only ? 0 : 4 + 0.25 * 4);
-  return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE>, g, dim3(fsk::NT), (const float*)s->u[c],
+  return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE, TS>, g, dim3(TS::NT), (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
                 (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], L0.rhs, L0.x, s->lv[1].rhs, L0.n, idx,
                 idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx, nb,
                 s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
 }
-
 template <int PRE>
-int launch_down(fs_solver_t* s, int l) {
+int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+  return DISPATCH_TILE(s->lv[0].n, launch_down0_t, PRE, s, it, pin, zero_p, residual_only);
+}
+
+template <int PRE, class TS>
+int launch_down_t(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid(F.n, &tx, &nb);
   // rhs (4) read; x (4) and the coarse rhs (1 per 4 cells) written
-  return launch(s, PK_DOWN, 9.0 * F.n * F.n, fsk::k_mgt_down<PRE>, g, dim3(fsk::NT), F.x, (const float*)F.rhs, F.n,
-                (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
+  return launch(s, PK_DOWN, 9.0 * F.n * F.n, fsk::k_mgt_down<PRE, TS>, g, dim3(TS::NT), F.x, (const float*)F.rhs,
+                F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
+}
+template <int PRE>
+int launch_down(fs_solver_t* s, int l) {
+  return DISPATCH_TILE(s->lv[l].n, launch_down_t, PRE, s, l);
 }
 
-template <int POST>
-int launch_up(fs_solver_t* s, int l) {
+template <int POST, class TS>
+int launch_up_t(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid(F.n, &tx, &nb);
   // x (4), rhs (4), coarse x (1 per 4 cells) read; x (4) written
-  return launch(s, PK_UP, 13.0 * F.n * F.n, fsk::k_mgt_up<POST>, g, dim3(fsk::NT), (const float*)F.x, F.xb,
+  return launch(s, PK_UP, 13.0 * F.n * F.n, fsk::k_mgt_up<POST, TS>, g, dim3(TS::NT), (const float*)F.x, F.xb,
                 (const float*)F.rhs, (const float*)s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx), tx, nb);
+}
+template <int POST>
+int launch_up(fs_solver_t* s, int l) {
+  return DISPATCH_TILE(s->lv[l].n, launch_up_t, POST, s, l);
 }
 
 int launch_coarse_ml(fs_solver_t* s, int from) {
