@@ -361,71 +361,53 @@ struct LevelTable {
   float ih2[CML_MAXL];  // (float)(1 / (dx_l * dx_l))
   int n[CML_MAXL];      // level size
   int off[CML_MAXL];    // offset of the level in the LDS arrays
+  int mn[CML_MAXL];     // qdiv magic for division by n
+  int mh[CML_MAXL];     // qdiv magic for division by n / 2
   int levels;           // levels in this kernel
   int tail;             // first level run in registers (== levels: none)
 };
 
-// Cells of one colour owned by this thread on an LDS level (n <= 64), with the
-// addresses of their four neighbours. Threads map to the cells of the colour only
-// (row j, pair m, i = 2m + ((j + colour) & 1)), so every active lane works and small
-// levels occupy few wavefronts. Computed once per level and reused by every pass:
-// the passes run on one CU, where per-pass index arithmetic would dominate.
-constexpr int CKP = (COARSE_N * COARSE_N / 2 + CTH - 1) / CTH;  // slots for an even level
-constexpr int CKS = (COARSE_N * COARSE_N + CTH - 1) / CTH;      // slots for an odd level
-struct ColourCells {
-  int c[CKS], l[CKS], r[CKS], b[CKS], t[CKS];  // c < 0: no cell in this slot
-};
+// q / d for 0 <= q < 4096 and 1 <= d <= 64 as one multiply and shift, with
+// m = ceil(2^18 / d): q (m d - 2^18) < 2^18 over that range makes the floor exact.
+// Runtime integer division costs tens of instructions and the coarse levels run
+// on one CU, where that index arithmetic is on the critical path.
+__host__ __device__ inline int qdiv_magic(int d) { return (int)(((1u << 18) + (unsigned)d - 1) / (unsigned)d); }
+__device__ __forceinline__ int qdiv(int q, int m) { return (int)(((unsigned)q * (unsigned)m) >> 18); }
 
-__device__ __forceinline__ void colour_cells(ColourCells& cs, int n, int colour) {
-#pragma unroll
-  for (int k = 0; k < CKS; k++) cs.c[k] = -1;
-  if (n & 1) {  // odd level: full-grid mapping with a colour test
-#pragma unroll
-    for (int k = 0; k < CKS; k++) {
-      int q = threadIdx.x + k * CTH;
-      if (q < n * n) {
-        int j = q / n, i = q - j * n;
-        if (((i + j) & 1) == colour) {
-          cs.c[k] = q;
-          cs.l[k] = j * n + (i == 0 ? n - 1 : i - 1);
-          cs.r[k] = j * n + (i + 1 == n ? 0 : i + 1);
-          cs.b[k] = (j == 0 ? n - 1 : j - 1) * n + i;
-          cs.t[k] = (j + 1 == n ? 0 : j + 1) * n + i;
-        }
-      }
-    }
-    return;
-  }
+// One colour pass on an LDS level (n even, n <= 64). Threads map to the cells of
+// the colour only (row j, pair m, i = 2m + ((j + colour) & 1)), so every active
+// lane works and small levels occupy few wavefronts; the indices come from one
+// multiply-shift per cell. A pass reads only the other colour, so all reads precede
+// the writes. Levels handed to this kernel are even (the hierarchy stops before an
+// odd level).
+constexpr int CKP = (COARSE_N * COARSE_N / 2 + CTH - 1) / CTH;  // cells of one colour per thread
+__device__ __forceinline__ void cml_pass(float* __restrict__ x, const float* __restrict__ r, int n, int mh,
+                                         int colour, float h2, bool from_zero) {
   const int hn = n >> 1, cells = hn * n;
+  float acc[CKP];
+  int cc[CKP];
 #pragma unroll
   for (int k = 0; k < CKP; k++) {
     int q = threadIdx.x + k * CTH;
     if (q < cells) {
-      int j = q / hn, m = q - j * hn;
+      int j = qdiv(q, mh), m = q - j * hn;
       int i = 2 * m + ((j + colour) & 1);
-      cs.c[k] = j * n + i;
-      cs.l[k] = j * n + (i == 0 ? n - 1 : i - 1);
-      cs.r[k] = j * n + (i + 1 == n ? 0 : i + 1);
-      cs.b[k] = (j == 0 ? n - 1 : j - 1) * n + i;
-      cs.t[k] = (j + 1 == n ? 0 : j + 1) * n + i;
-    }
-  }
-}
-
-// One colour pass; it reads only the other colour, so all reads precede the writes.
-__device__ __forceinline__ void cml_pass(float* __restrict__ x, const float* __restrict__ r, const ColourCells& cs,
-                                         float h2, bool from_zero) {
-  float acc[CKS];
-#pragma unroll
-  for (int k = 0; k < CKS; k++) {
-    if (cs.c[k] >= 0) {
-      float sum = from_zero ? 0.f : x[cs.l[k]] + x[cs.r[k]] + x[cs.b[k]] + x[cs.t[k]];
-      acc[k] = (sum - h2 * r[cs.c[k]]) * 0.25f;
+      int row = j * n, c = row + i;
+      cc[k] = c;
+      float sum = 0.f;
+      if (!from_zero) {
+        int il = i == 0 ? n - 1 : i - 1, ir = i + 1 == n ? 0 : i + 1;
+        int jb = j == 0 ? n - 1 : j - 1, jt = j + 1 == n ? 0 : j + 1;
+        sum = x[row + il] + x[row + ir] + x[jb * n + i] + x[jt * n + i];
+      }
+      acc[k] = (sum - h2 * r[c]) * 0.25f;
     }
   }
 #pragma unroll
-  for (int k = 0; k < CKS; k++)
-    if (cs.c[k] >= 0) x[cs.c[k]] = acc[k];
+  for (int k = 0; k < CKP; k++) {
+    int q = threadIdx.x + k * CTH;
+    if (q < cells) x[cc[k]] = acc[k];
+  }
 }
 
 // register-level red-black pass: lane = j*n + i holds x of its cell
@@ -454,21 +436,35 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
                int coarse, const float* __restrict__ pinv) {
   __shared__ float bx[CML_CELLS];
   __shared__ float br[CML_CELLS];
+  __shared__ float spv[REG_N * REG_N * REG_N * REG_N];  // pseudo-inverse of the coarsest register level
+  __shared__ float srr[REG_N * REG_N];                  // its right-hand side, read back as broadcasts
   __shared__ double red[CTH / 64];
   const int levels = tb.levels, tail = tb.tail, last = levels - 1;
   const int n0 = tb.n[0];
+  const int ncells = tb.n[last] * tb.n[last];
+  const bool use_pinv = pinv != nullptr && tail < levels;
   {
-    constexpr int K = CKF;
-    float tmp[K];
+    constexpr int K = CKF, KP = (REG_N * REG_N * REG_N * REG_N + CTH - 1) / CTH;
+    float tmp[K], tp[KP];
 #pragma unroll
     for (int k = 0; k < K; k++) {
       int q = threadIdx.x + k * CTH;
       if (q < n0 * n0) tmp[k] = rg[q];
     }
 #pragma unroll
+    for (int k = 0; k < KP; k++) {
+      int q = threadIdx.x + k * CTH;
+      if (use_pinv && q < ncells * ncells) tp[k] = pinv[q];
+    }
+#pragma unroll
     for (int k = 0; k < K; k++) {
       int q = threadIdx.x + k * CTH;
       if (q < n0 * n0) br[q] = tmp[k];
+    }
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      int q = threadIdx.x + k * CTH;
+      if (use_pinv && q < ncells * ncells) spv[q] = tp[k];
     }
   }
   __syncthreads();
@@ -484,13 +480,10 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       __syncthreads();
     }
     if (pre > 0) {
-      ColourCells red, blk;
-      colour_cells(red, n, 0);
-      colour_cells(blk, n, 1);
       for (int s = 0; s < pre; s++) {
-        cml_pass(x, r, red, h2, s == 0);  // from zero: black cells are stale but never read
+        cml_pass(x, r, n, tb.mh[l], 0, h2, s == 0);  // from zero: black cells are stale but never read
         __syncthreads();
-        cml_pass(x, r, blk, h2, false);
+        cml_pass(x, r, n, tb.mh[l], 1, h2, false);
         __syncthreads();
       }
     }
@@ -502,7 +495,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       for (int k = 0; k < CK; k++) {
         int q = threadIdx.x + k * CTH;
         if (q < cn * cn) {
-          int J = q / cn, I = q - J * cn;
+          int J = qdiv(q, tb.mn[l + 1]), I = q - J * cn;
           float s = 0.f;
 #pragma unroll
           for (int b = 0; b < 2; b++)
@@ -546,13 +539,10 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       x[q] = 0.f;
     }
     __syncthreads();
-    ColourCells red, blk;
-    colour_cells(red, n, 0);
-    colour_cells(blk, n, 1);
     for (int sw = 0; sw < coarse; sw++) {
-      cml_pass(x, r, red, h2, false);
+      cml_pass(x, r, n, tb.mh[last], 0, h2, false);
       __syncthreads();
-      cml_pass(x, r, blk, h2, false);
+      cml_pass(x, r, n, tb.mh[last], 1, h2, false);
       __syncthreads();
     }
   } else if (threadIdx.x < 64) {
@@ -571,7 +561,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       if (k < nt) {
         int l = tail + k, n = tb.n[l];
         bool act = lane < n * n;
-        int j = act ? lane / n : 0, i = act ? lane - j * n : 0;
+        int j = act ? qdiv(lane, tb.mn[l]) : 0, i = act ? lane - j * n : 0;
         float h2 = tb.h2[l], ih2 = tb.ih2[l];
         if (k == nt - 1) {
           double s = act ? (double)rr[k] : 0.0;
@@ -579,14 +569,19 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
           float mean = (float)(s / (n * n));
           rr[k] = rr[k] - mean;
           float x = 0.f;
-          if (pinv) {
+          if (use_pinv) {
             // the sweeps would converge below FP32 rounding: apply the pseudo-inverse
-            // of the periodic Laplacian instead (pinv[c*cells + lane], columns broadcast)
+            // of the periodic Laplacian instead (spv[c*cells + lane]); the rhs goes
+            // through LDS so that every column weight is an independent broadcast read.
+            // Only wavefront 0 runs here, and one wavefront's LDS accesses complete in
+            // issue order, so its own writes are visible to its following reads.
             int cells = n * n;
-            for (int c = 0; c < cells; c++) {
-              float bc = __shfl(rr[k], c, 64);
-              x = fmaf(act ? pinv[c * cells + lane] : 0.f, bc, x);
-            }
+            if (act) srr[lane] = rr[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 8
+            for (int c = 0; c < cells; c++) x = fmaf(act ? spv[c * cells + lane] : 0.f, srr[c], x);
           } else {
             for (int sw = 0; sw < coarse; sw++) {
               x = reg_colour(x, rr[k], n, i, j, act, h2, 0, false);
@@ -609,7 +604,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
           float res = reg_residual(x, rr[k], n, i, j, ih2);
           int cn = n >> 1;
           bool cact = lane < cn * cn;
-          int J = cact ? lane / cn : 0, I = cact ? lane - J * cn : 0;
+          int J = cact ? qdiv(lane, tb.mn[l + 1]) : 0, I = cact ? lane - J * cn : 0;
           float s = 0.f;
           s += __shfl(res, (2 * J) * n + 2 * I, 64);
           s += __shfl(res, (2 * J) * n + 2 * I + 1, 64);
@@ -624,7 +619,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       if (k < nt - 1) {
         int l = tail + k, n = tb.n[l], cn = n >> 1;
         bool act = lane < n * n;
-        int j = act ? lane / n : 0, i = act ? lane - j * n : 0;
+        int j = act ? qdiv(lane, tb.mn[l]) : 0, i = act ? lane - j * n : 0;
         float h2 = tb.h2[l];
         int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
         int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
@@ -660,7 +655,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       for (int k = 0; k < CKF; k++) {
         int q = threadIdx.x + k * CTH;
         if (q < n * n) {
-          int j = q / n, i = q - j * n;
+          int j = qdiv(q, tb.mn[l]), i = q - j * n;
           int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
           int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
           float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
@@ -679,13 +674,10 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
     }
     __syncthreads();
     if (post > 0) {
-      ColourCells red, blk;
-      colour_cells(red, n, 0);
-      colour_cells(blk, n, 1);
       for (int s = 0; s < post; s++) {
-        cml_pass(x, r, red, h2, false);
+        cml_pass(x, r, n, tb.mh[l], 0, h2, false);
         __syncthreads();
-        cml_pass(x, r, blk, h2, false);
+        cml_pass(x, r, n, tb.mh[l], 1, h2, false);
         __syncthreads();
       }
     }

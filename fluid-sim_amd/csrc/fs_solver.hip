@@ -453,6 +453,8 @@ fsk::LevelTable level_table(const fs_solver_t* s, int from, int end) {
     tb.h2[k] = (float)(dx * dx);
     tb.ih2[k] = (float)(1.0 / (dx * dx));
     tb.n[k] = s->lv[l].n;
+    tb.mn[k] = fsk::qdiv_magic(s->lv[l].n);
+    tb.mh[k] = fsk::qdiv_magic(std::max(1, s->lv[l].n / 2));
     tb.off[k] = o;
     o += s->lv[l].n * s->lv[l].n;
     if (tb.tail == tb.levels && s->lv[l].n <= fsk::REG_N) tb.tail = k;
