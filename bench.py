@@ -150,6 +150,9 @@ def main():
     g = make_solver(fluidsim, n, a.tol, a.guess)
     g.uploadVelocity(u0, v0)
     print(f"# device: {dev['info']['description']}; lib {fluidsim.library_path()}", file=sys.stderr, flush=True)
+    tiles = g.tileReport()
+    for ln in tiles.strip().splitlines():
+        print(f"# tiles {ln}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         g.stepCFL(0.5)
     g.sync()
@@ -251,6 +254,7 @@ def main():
             "vcycles_per_step": cycles,
             "guess": a.guess,
             "stored_field_max_div": d["max_divergence"],
+            "tiles": [ln.split(" | ")[0] for ln in tiles.strip().splitlines()],
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -73,6 +73,8 @@ struct Level {
   float* x = nullptr;    // correction (level 0: e); fused path: after pre-smoothing
   float* rhs = nullptr;  // right-hand side (level 0: r)
   float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e)
+  int tile[2] = {0, 0};  // fused path: tile shape (TileKind) of the down and up kernels
+  float tune_us[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
 };
 
 // Kernel classes timed by fs_profile_*: each profiled launch goes through
@@ -525,31 +527,35 @@ int mg_vcycle(fs_solver_t* s) {
 
 // ----------------------------------------------------------- fused V-cycle
 
-// Tile shape of a grid-wide level: the largest one that still yields a tile per CU.
-enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8 };
-static const int kTileW[4] = {64, 32, 16, 8}, kTileH[4] = {32, 32, 16, 8};
+// Tile shapes of the grid-wide levels. The default is the largest shape that still
+// yields a tile per CU; the autotuner replaces it with the fastest measured one.
+// Every shape computes the same bits (each cell sees the same operands in the same
+// order), so the choice affects time only.
+enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS };
+static const int kTileW[TILE_KINDS] = {64, 32, 16, 8}, kTileH[TILE_KINDS] = {32, 32, 16, 8};
+static const char* kTileName[TILE_KINDS] = {"64x32", "32x32", "16x16", "8x8"};
+
+inline int tile_count(int k, int n) { return ((n + kTileW[k] - 1) / kTileW[k]) * ((n + kTileH[k] - 1) / kTileH[k]); }
 
 inline int tile_kind(int n) {
-  for (int k = TILE64; k < TILE8; k++) {
-    long tiles = (long)((n + kTileW[k] - 1) / kTileW[k]) * ((n + kTileH[k] - 1) / kTileH[k]);
-    if (tiles >= 256) return k;
-  }
+  for (int k = TILE64; k < TILE8; k++)
+    if (tile_count(k, n) >= 256) return k;
   return TILE8;
 }
 
-inline dim3 tiles_grid(int n, int* tiles_x, int* nblocks) {
-  int k = tile_kind(n);
-  int tx = (n + kTileW[k] - 1) / kTileW[k], ty = (n + kTileH[k] - 1) / kTileH[k];
+template <class TS>
+inline dim3 tiles_grid_t(int n, int* tiles_x, int* nblocks) {
+  int tx = (n + TS::TX - 1) / TS::TX, ty = (n + TS::TY - 1) / TS::TY;
   *tiles_x = tx;
   *nblocks = tx * ty;
   return dim3((unsigned)(tx * ty));
 }
 
-#define DISPATCH_TILE(n, fn, SW, ...)                                                        \
-  (tile_kind(n) == TILE64   ? fn<SW, fsk::Tile64>(__VA_ARGS__)                               \
-   : tile_kind(n) == TILE32 ? fn<SW, fsk::Tile32>(__VA_ARGS__)                               \
-   : tile_kind(n) == TILE16 ? fn<SW, fsk::Tile16>(__VA_ARGS__)                               \
-                            : fn<SW, fsk::Tile8>(__VA_ARGS__))
+#define DISPATCH_TILE(k, fn, SW, ...)                                                        \
+  ((k) == TILE64   ? fn<SW, fsk::Tile64>(__VA_ARGS__)                                         \
+   : (k) == TILE32 ? fn<SW, fsk::Tile32>(__VA_ARGS__)                                         \
+   : (k) == TILE16 ? fn<SW, fsk::Tile16>(__VA_ARGS__)                                         \
+                   : fn<SW, fsk::Tile8>(__VA_ARGS__))
 
 template <int PRE>
 int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only);
@@ -557,7 +563,7 @@ template <int PRE, class TS>
 int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
   Level& L0 = s->lv[0];
   int tx, nb;
-  dim3 g = tiles_grid(L0.n, &tx, &nb);
+  dim3 g = tiles_grid_t<TS>(L0.n, &tx, &nb);
   double idx = 1.0 / s->dx;
   int c = s->cur;
   // u, v (8) + p (8) + previous correction (4) read; p (8) + r (4) written; then x (4) and the coarse rhs (1)
@@ -570,35 +576,35 @@ int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_o
 }
 template <int PRE>
 int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
-  return DISPATCH_TILE(s->lv[0].n, launch_down0_t, PRE, s, it, pin, zero_p, residual_only);
+  return DISPATCH_TILE(s->lv[0].tile[0], launch_down0_t, PRE, s, it, pin, zero_p, residual_only);
 }
 
 template <int PRE, class TS>
 int launch_down_t(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
-  dim3 g = tiles_grid(F.n, &tx, &nb);
+  dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
   // rhs (4) read; x (4) and the coarse rhs (1 per 4 cells) written
   return launch(s, PK_DOWN, 9.0 * F.n * F.n, fsk::k_mgt_down<PRE, TS>, g, dim3(TS::NT), F.x, (const float*)F.rhs,
                 F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
 }
 template <int PRE>
 int launch_down(fs_solver_t* s, int l) {
-  return DISPATCH_TILE(s->lv[l].n, launch_down_t, PRE, s, l);
+  return DISPATCH_TILE(s->lv[l].tile[0], launch_down_t, PRE, s, l);
 }
 
 template <int POST, class TS>
 int launch_up_t(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
-  dim3 g = tiles_grid(F.n, &tx, &nb);
+  dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
   // x (4), rhs (4), coarse x (1 per 4 cells) read; x (4) written
   return launch(s, PK_UP, 13.0 * F.n * F.n, fsk::k_mgt_up<POST, TS>, g, dim3(TS::NT), (const float*)F.x, F.xb,
                 (const float*)F.rhs, (const float*)s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx), tx, nb);
 }
 template <int POST>
 int launch_up(fs_solver_t* s, int l) {
-  return DISPATCH_TILE(s->lv[l].n, launch_up_t, POST, s, l);
+  return DISPATCH_TILE(s->lv[l].tile[1], launch_up_t, POST, s, l);
 }
 
 int launch_coarse_ml(fs_solver_t* s, int from) {
@@ -852,6 +858,7 @@ int build_levels(fs_solver_t* s) {
   s->fused = s->cfg.mg_fused && cs < Lc && Lc - cs <= fsk::CML_MAXL && s->cfg.mg_pre >= 1 && s->cfg.mg_pre <= 3 &&
              s->cfg.mg_post >= 1 && s->cfg.mg_post <= 3;
   s->coarse_start = cs;
+  for (auto& L : s->lv) L.tile[0] = L.tile[1] = tile_kind(L.n);
   for (size_t l = 0; l < s->lv.size(); l++) {
     size_t c = (size_t)s->lv[l].n * s->lv[l].n;
     int rc = dalloc(&s->lv[l].x, c);
@@ -862,17 +869,60 @@ int build_levels(fs_solver_t* s) {
   return build_coarse_inverse(s);
 }
 
+// Time every tile shape of the down and up kernels on each grid-wide level (one
+// warm-up and five timed launches on the zeroed buffers of a new solver) and keep
+// the fastest. Level 0's down kernel is the fused residual + down-sweep; its
+// per-tile partials go to a scratch row sized for the finest shape.
+int autotune_tiles(fs_solver_t* s) {
+  const int cs = s->coarse_start;
+  if (!s->fused || cs == 0) return FS_OK;
+  double* tmp = nullptr;
+  int rc = dalloc(&tmp, (size_t)tile_count(TILE8, s->lv[0].n));
+  if (rc) return rc;
+  double* saved = s->rpart;
+  s->rpart = tmp;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventCreate");
+  const int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
+  for (int l = 0; l < cs && !rc; l++) {
+    Level& L = s->lv[l];
+    for (int dir = 0; dir < 2 && !rc; dir++) {
+      auto run = [&]() -> int {
+        if (dir == 1) return DISPATCH_SW(post, launch_up, s, l);
+        if (l == 0) return DISPATCH_SW(pre, launch_down0, s, 0, 0, true, false);
+        return DISPATCH_SW(pre, launch_down, s, l);
+      };
+      int best = L.tile[dir];
+      float best_us = 0;
+      for (int k = TILE64; k < TILE_KINDS && !rc; k++) {
+        L.tile[dir] = k;
+        if ((rc = run())) break;
+        if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
+        for (int r = 0; r < 5 && !rc; r++) rc = run();
+        if (!rc && hipEventRecord(b, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
+        float ms = 0;
+        if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess))
+          rc = fail(FS_E_DEVICE, "autotune timing failed");
+        L.tune_us[dir][k] = ms * 1000.f / 5.f;
+        if (!rc && (best_us == 0 || L.tune_us[dir][k] < best_us)) {
+          best_us = L.tune_us[dir][k];
+          best = k;
+        }
+      }
+      L.tile[dir] = best;
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  s->rpart = saved;
+  dfree(tmp);
+  return rc;
+}
+
 // residual partial rows for `iterations` + 2 refinement iterations
 int alloc_rpart(fs_solver_t* s, int iterations) {
   int rows = iterations + 2;
-  int cols;
-  if (s->fused && s->coarse_start > 0) {
-    int tx, nb;
-    tiles_grid(s->nx, &tx, &nb);
-    cols = nb;
-  } else {
-    cols = grid_blocks(s->nx, s->ny);
-  }
+  int cols = s->fused && s->coarse_start > 0 ? tile_count(s->lv[0].tile[0], s->nx) : grid_blocks(s->nx, s->ny);
   double* d = nullptr;
   double* h = nullptr;
   int rc = dalloc(&d, (size_t)rows * cols);
@@ -997,6 +1047,7 @@ void fs_config_default(fs_config* c) {
   c->mg_precision = FS_MG_FP32;
   c->guess = FS_GUESS_ZERO;
   c->solid_mask_width = 1.5;
+  c->autotune = 1;
   c->device = 0;
   c->mg_fused = 1;
 }
@@ -1066,7 +1117,6 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (!rc) rc = dalloc(&s->p64[0], s->n);
   if (!rc) rc = dalloc(&s->p64[1], s->n);
   if (!rc) rc = build_levels(s);
-  if (!rc && c.pressure_solver == FS_P_MULTIGRID) rc = alloc_rpart(s, std::max(c.pressure_iterations, 1));
   s->mp_count = grid_blocks(s->nx, s->ny);
   if (!rc) rc = dalloc(&s->mpart, (size_t)2 * s->mp_count);
   if (!rc) rc = dalloc(&s->coef, 1);
@@ -1088,6 +1138,9 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     if (!rc) rc = fill_f32(s, l.rhs, (size_t)l.n * l.n, 0.f);
     if (!rc && l.xb) rc = fill_f32(s, l.xb, (size_t)l.n * l.n, 0.f);
   }
+  // on zero inputs every fused kernel writes zeros, so tuning leaves the state as it is
+  if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) rc = autotune_tiles(s);
+  if (!rc && c.pressure_solver == FS_P_MULTIGRID) rc = alloc_rpart(s, std::max(c.pressure_iterations, 1));
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "initial fill failed");
   if (rc) return bail(rc);
   *out = s;
@@ -1443,6 +1496,31 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   if (rc) return rc;
   s->have_maxabs = s->have_maxabs && kernel == FS_K_GRADIENT;
   return sync_stream(s);
+}
+
+int fs_tile_report(void* h, char* buf, size_t len) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  if (!buf || len == 0) return fail(FS_E_INVALID_ARG, "buffer is NULL or empty");
+  fs_solver_t* s = (fs_solver_t*)h;
+  std::string out;
+  char line[256];
+  for (int l = 0; s->fused && l < s->coarse_start; l++) {
+    const Level& L = s->lv[l];
+    for (int dir = 0; dir < 2; dir++) {
+      snprintf(line, sizeof line, "level %d n=%d %s: %s", l, L.n, dir == 0 ? (l == 0 ? "down0" : "down") : "up",
+               kTileName[L.tile[dir]]);
+      out += line;
+      for (int k = 0; k < TILE_KINDS; k++)
+        if (L.tune_us[dir][k] > 0) {
+          snprintf(line, sizeof line, " | %s %.2f us", kTileName[k], L.tune_us[dir][k]);
+          out += line;
+        }
+      out += "\n";
+    }
+  }
+  snprintf(buf, len, "%s", out.c_str());
+  return FS_OK;
 }
 
 int fs_profile_enable(void* h, unsigned mask) {

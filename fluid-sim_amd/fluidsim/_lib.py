@@ -44,6 +44,7 @@ class FsConfig(C.Structure):
         ("solid_mask_width", C.c_double),
         ("device", C.c_int),
         ("mg_fused", C.c_int),
+        ("autotune", C.c_int),
     ]
 
 
@@ -124,6 +125,7 @@ _SIGNATURES = {
     "fs_diagnostics": (C.c_int, [_VP, C.POINTER(FsDiag)]),
     "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),
     "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
+    "fs_tile_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
     "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
 }

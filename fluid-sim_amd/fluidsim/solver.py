@@ -54,7 +54,7 @@ class GPUFluidSolver:
                  confinement=0, dyeFade=0, advection="maccormack", solidMaskWidth=1.5,
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
-                 mgPrecision="fp32", guess="zero", mgFused=True):
+                 mgPrecision="fp32", guess="zero", mgFused=True, autotune=True):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -81,6 +81,7 @@ class GPUFluidSolver:
         cfg.solid_mask_width = solidMaskWidth
         cfg.device = int(device)
         cfg.mg_fused = 1 if mgFused else 0
+        cfg.autotune = 1 if autotune else 0
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))
@@ -272,6 +273,12 @@ class GPUFluidSolver:
         for k in names:
             mask |= 1 << _lib.PROFILE_KINDS[k]
         check(self._L.fs_profile_enable(self.handle, mask))
+
+    def tileReport(self) -> str:
+        """Tile shape per grid-wide level and the autotuner's timings."""
+        buf = C.create_string_buffer(4096)
+        check(self._L.fs_tile_report(self.handle, buf, 4096))
+        return buf.value.decode()
 
     def profileRead(self) -> dict:
         """{kind: (total_ms, launches, algorithmic_bytes)} since profileEnable."""

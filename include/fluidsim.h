@@ -87,6 +87,8 @@ typedef struct fs_config {
   int device;                 /* HIP device ordinal; default 0 */
   int mg_fused;               /* 1: LDS-tiled V-cycle (two launches per level, levels <= 64
                                  in one workgroup); 0: one launch per operator. Same bits. */
+  int autotune;               /* 1 (default): time each tile shape of the V-cycle kernels at
+                                 creation and keep the fastest per level (same bits) */
 } fs_config;
 
 /* per-step statistics (new: the reference reports none) */
@@ -197,6 +199,8 @@ enum {
  * events recorded at the dispatch's own start and end on the solver's stream, and
  * accumulate each launch's algorithmic bytes (inputs read once, outputs written once). */
 int fs_profile_enable(void* solver, unsigned mask);
+/* the V-cycle tile shape chosen for each grid-wide level and the autotuner's timings, as text */
+int fs_tile_report(void* solver, char* buf, size_t len);
 int fs_profile_read(void* solver, int kind, double* total_ms, int64_t* launches, double* bytes);
 
 #ifdef __cplusplus

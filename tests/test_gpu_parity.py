@@ -455,3 +455,23 @@ def test_free_disk_exchanges_momentum_with_the_fluid(fs, O):
     # the stored field is FP32: its momentum change carries the rounding of each face
     assert abs(dpx - imp[0]) < 1e-5 * abs(imp[0]) and abs(dpy - imp[1]) < 1e-5 * abs(imp[1])
     assert abs((m1[0] - m0[0]) + imp[0]) < 1e-12 and abs((m1[1] - m0[1]) + imp[1]) < 1e-12
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_autotuned_tiles_give_the_same_bits(fs, O, n):
+    # the tile shape changes only the schedule: autotuned and fixed-rule solvers agree
+    # bit for bit, and so do both with the per-operator V-cycle
+    u0, v0 = O.js_random_velocity(n, 11)
+    outs = []
+    for kw in (dict(autotune=True), dict(autotune=False), dict(mgFused=False)):
+        with solver(fs, n=n, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
+                    mgMinSize=4, **kw) as g:
+            g.uploadVelocity(u0, v0)
+            for _ in range(2):
+                g.step(2e-3)
+            outs.append(g.readVelocity())
+            if kw.get("autotune"):
+                rep = g.tileReport()
+                assert "level 0" in rep and "down0" in rep and "us" in rep
+    for o in outs[1:]:
+        assert np.array_equal(outs[0]["u"], o["u"]) and np.array_equal(outs[0]["v"], o["v"])
