@@ -57,12 +57,25 @@ TRAFFIC_1024 = {
 }
 
 
+# Workloads: the headline one (default) and the forced, mixed-precision one at 2048^2
+# (band forcing kf = 32 +- 2, 24 modes, amplitude 0.1, seed 1000 + step, drag 0.02;
+# BF16-stored smoother, FP32 arithmetic, FP64 refinement residual).
+WORKLOADS = {
+    "decaying1024": dict(n=1024, nu=1e-4, forcing=None, precision="fp32"),
+    "forced2048": dict(n=2048, nu=5e-5, forcing=dict(kf=32.0, width=2.0, amp=0.1, drag=0.02, seed=1000),
+                       precision="bf16"),
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="decaying1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default=None, choices=["fp32", "bf16"],
+                    help="multigrid storage precision (default: the workload's)")
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=12)
-    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--n", type=int, default=None, help="grid size (default: the workload's)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -93,13 +106,16 @@ def copy_calibration():
     print(f"copy calibration: 5 copies of {nbytes} bytes", flush=True)
 
 
-def make_solver(fluidsim, n, tol, guess="zero"):
-    return fluidsim.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
-                                   pressureIterations=60, tolerance=tol, mgMinSize=4, mgPre=2, mgPost=2,
-                                   mgCoarse=40, guess=guess)
+def make_solver(fluidsim, wl, n, tol, guess="zero"):
+    g = fluidsim.GPUFluidSolver(0, n=n, nu=wl["nu"], advection="maccormack", pressureSolver="multigrid",
+                                pressureIterations=60, tolerance=tol, mgMinSize=4, mgPre=2, mgPost=2,
+                                mgCoarse=40, guess=guess, mgPrecision=wl["precision"])
+    if wl["forcing"]:
+        g.setForcing(wl["forcing"])
+    return g
 
 
-def cpu_baseline(n, seconds):
+def cpu_baseline(wl, n, seconds):
     """f64 oracle (single thread, then all host cores) on the same workload, exact FFT projection."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -110,7 +126,9 @@ def cpu_baseline(n, seconds):
     out = {}
     for label, threads in (("single", 1), ("omp", min(16, os.cpu_count() or 1))):
         O.set_threads(threads)
-        s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+        f = wl["forcing"]
+        s = O.Solver(n=n, nu=wl["nu"], advection="maccormack", explicit_diffusion=True, projection="fft",
+                     forcing=f, drag=f["drag"] if f else 0.0)
         s.u[:] = u0.astype(np.float32)
         s.v[:] = v0.astype(np.float32)
         budget = seconds if label == "single" else max(2.0, seconds / 4)
@@ -118,7 +136,7 @@ def cpu_baseline(n, seconds):
         t0 = time.perf_counter()
         while True:
             dt = 0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h)
-            s.step(dt)
+            s.step(dt, force_seed=(f["seed"] + steps) if f else 0)
             steps += 1
             el = time.perf_counter() - t0
             if el >= budget or steps >= 200:
@@ -145,9 +163,12 @@ def main():
     import fluidsim
     from fluidsim import initial
     dev = fluidsim.createDevice()
-    n = a.n
+    wl = dict(WORKLOADS[a.workload])
+    if a.precision:
+        wl["precision"] = a.precision
+    n = a.n or wl["n"]
     u0, v0 = initial.decaying_turbulence(n, seed=1)
-    g = make_solver(fluidsim, n, a.tol, a.guess)
+    g = make_solver(fluidsim, wl, n, a.tol, a.guess)
     g.uploadVelocity(u0, v0)
     print(f"# device: {dev['info']['description']}; lib {fluidsim.library_path()}", file=sys.stderr, flush=True)
     tiles = g.tileReport()
@@ -190,7 +211,7 @@ def main():
         ms, launches, nbytes = prof[dom]
         avg_s = ms / 1e3 / max(launches, 1)
         achieved = nbytes / (ms / 1e3) / 1e9
-        traffic = TRAFFIC_1024.get(dom) if n == 1024 else None
+        traffic = TRAFFIC_1024.get(dom) if (n == 1024 and wl["precision"] == "fp32") else None
         roof = {"bound": "hbm", "kernel": KIND_TEXT[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
@@ -205,7 +226,7 @@ def main():
     ablation = None
     if not a.no_ablation:
         other = "previous" if a.guess == "zero" else "zero"
-        g2 = make_solver(fluidsim, n, a.tol, other)
+        g2 = make_solver(fluidsim, wl, n, a.tol, other)
         g2.uploadVelocity(u0, v0)
         for _ in range(a.warmup):
             g2.stepCFL(0.5)
@@ -226,7 +247,7 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(n, a.cpu_seconds)
+        cpu = cpu_baseline(wl, n, a.cpu_seconds)
         print(f"# cpu baseline: {cpu['value']:.4g} cell-steps/s (1 thread), "
               f"{cpu['omp']['value']:.4g} ({cpu['omp']['cores']} threads)", file=sys.stderr, flush=True)
 
@@ -244,11 +265,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"decaying 2D turbulence {n}^2 periodic MAC grid, nu=1e-4, MacCormack + explicit "
-                        "diffusion, CFL 0.5, plain multigrid V(2,2) red-black 9 levels (minSize 4, 40 coarse "
-                        "sweeps) to FP64 max|div u* - L p| <= 1e-6 every step, "
-                        f"{a.guess} initial guess",
+            "workload": (f"decaying 2D turbulence {n}^2" if not wl["forcing"] else
+                         f"band-forced 2D turbulence {n}^2 (kf 32 +- 2, 24 modes, amp 0.1, seed 1000 + step, "
+                         "drag 0.02)") +
+                        f" periodic MAC grid, nu={wl['nu']:g}, MacCormack + explicit diffusion, CFL 0.5, plain "
+                        "multigrid V(2,2) red-black to minSize 4 (40 coarse sweeps) to FP64 max|div u* - L p| "
+                        f"<= {a.tol:g} every step, {wl['precision']} smoother storage, {a.guess} initial guess",
             "n": n,
+            "mg_storage": wl["precision"],
             "residual_tolerance": a.tol,
             "residual_dtype": "f64",
             "vcycles_per_step": cycles,

@@ -14,6 +14,34 @@
 
 namespace fsk {
 
+// Storage of the multigrid correction and right-hand sides: FP32, or BF16 (the
+// upper 16 bits of an FP32, rounded to nearest even). Arithmetic is FP32 either
+// way; only loads and stores convert.
+struct bf16 {
+  uint16_t b;
+};
+__device__ __forceinline__ float ld(float x) { return x; }
+__device__ __forceinline__ float ld(bf16 x) { return __uint_as_float((uint32_t)x.b << 16); }
+template <typename S>
+__device__ __forceinline__ S st(float f);
+template <>
+__device__ __forceinline__ float st<float>(float f) {
+  return f;
+}
+template <>
+__device__ __forceinline__ bf16 st<bf16>(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16{(uint16_t)((u >> 16) | 0x40u)};  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16{(uint16_t)(u >> 16)};
+}
+// value as stored: the FP32 a store of type S followed by a load returns
+template <typename S>
+__device__ __forceinline__ float rnd(float f) {
+  return ld(st<S>(f));
+}
+
+
 constexpr int BX = 64;
 constexpr int BY = 4;
 
@@ -288,9 +316,10 @@ __device__ __forceinline__ double nan_to_inf(double a) {
 // multigrid right-hand side, the block's max|r| stored to rpart[block].
 // e == nullptr means a zero correction, p == nullptr a zero initial pressure. The divergence and Laplacian follow
 // src/core/grid.js:191-206 and src/cpu/projection.js:48-62.
+template <typename S>
 __global__ void __launch_bounds__(BX * BY)
 k_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
-             const float* __restrict__ e, double* __restrict__ pout, float* __restrict__ r32, int nx, int ny,
+             const S* __restrict__ e, double* __restrict__ pout, S* __restrict__ r32, int nx, int ny,
              double idx, double idy, double idx2, double idy2, double* __restrict__ rpart) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double amax = 0.0;
@@ -300,12 +329,12 @@ k_residual64(const float* __restrict__ u, const float* __restrict__ v, const dou
     size_t kl = (size_t)j * nx + im, kr = (size_t)j * nx + ip, kb = (size_t)jm * nx + i, kt = (size_t)jp * nx + i;
     double pc = 0, pl = 0, pr = 0, pb = 0, pt = 0;
     if (p) { pc = p[k]; pl = p[kl]; pr = p[kr]; pb = p[kb]; pt = p[kt]; }
-    if (e) { pc += e[k]; pl += e[kl]; pr += e[kr]; pb += e[kb]; pt += e[kt]; }
+    if (e) { pc += ld(e[k]); pl += ld(e[kl]); pr += ld(e[kr]); pb += ld(e[kb]); pt += ld(e[kt]); }
     double lap = (pl - 2 * pc + pr) * idx2 + (pb - 2 * pc + pt) * idy2;
     double dv = ((double)u[kr] - (double)u[k]) * idx + ((double)v[kt] - (double)v[k]) * idy;
     double r = dv - lap;
     pout[k] = pc;
-    r32[k] = (float)r;
+    r32[k] = st<S>((float)r);
     amax = nan_to_inf(fabs(r));
   }
   double m = block_max(amax);

@@ -62,7 +62,7 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __
     int q = threadIdx.x + k * NT;
     if (q < CNT) {
       int ly = q / RW, lx = q - ly * RW;
-      tmp[k] = (float)src[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
+      tmp[k] = ld(src[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
     }
   }
 #pragma unroll
@@ -131,15 +131,15 @@ __device__ __forceinline__ void tile_smooth_from_zero(float* __restrict__ sx, co
 }
 
 // Store the tile interior and the restricted residual of each 2x2 block of it.
-template <int H, int RW, class TS>
+template <int H, int RW, class TS, typename S>
 __device__ __forceinline__ void tile_store_and_restrict(const float* __restrict__ sx, const float* __restrict__ sr,
-                                                        float* __restrict__ x, float* __restrict__ crhs, int n,
+                                                        S* __restrict__ x, S* __restrict__ crhs, int n,
                                                         int tx0, int ty0, float ih2) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
-    if (gi < n && gj < n) x[(size_t)gj * n + gi] = sx[(iy + H) * RW + ix + H];
+    if (gi < n && gj < n) x[(size_t)gj * n + gi] = st<S>(sx[(iy + H) * RW + ix + H]);
   }
   constexpr int CX = TX / 2, CY = TY / 2;
   int cn = n >> 1;
@@ -156,16 +156,16 @@ __device__ __forceinline__ void tile_store_and_restrict(const float* __restrict_
         float lap = (sx[k - 1] + sx[k + 1] + sx[k - RW] + sx[k + RW] - 4.f * sx[k]) * ih2;
         s += sr[k] - lap;
       }
-    crhs[(size_t)J * cn + I] = s * 0.25f;
+    crhs[(size_t)J * cn + I] = st<S>(s * 0.25f);
   }
 }
 
 // -------------------------------------------------------------- level >= 1 down
 
-template <int SW, class TS>
+template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
-k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2, float ih2,
-           float* __restrict__ crhs, int tiles_x, int nblocks) {
+k_mgt_down(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float ih2,
+           S* __restrict__ crhs, int tiles_x, int nblocks) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
@@ -187,11 +187,11 @@ k_mgt_down(float* __restrict__ x, const float* __restrict__ rhs, int n, float h2
 // r = div u* - L p' (src/core/grid.js:191-206, src/cpu/projection.js:48-62) on the
 // tile + halo 2*SW; p' and r are stored for the interior and the tile's max|r| goes
 // to rpart[tile]. residual_only stops after that.
-template <int SW, class TS>
+template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
-            const float* __restrict__ e_prev, double* __restrict__ pout, float* __restrict__ r32,
-            float* __restrict__ x, float* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2,
+            const S* __restrict__ e_prev, double* __restrict__ pout, S* __restrict__ r32,
+            S* __restrict__ x, S* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2,
             int tiles_x, int nblocks, double* __restrict__ rpart, int residual_only) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
@@ -215,7 +215,7 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
         int ly = q / PW, lx = q - ly * PW;
         size_t g = (size_t)wrapn(oy - 1 + ly, n) * n + wrapn(ox - 1 + lx, n);
         tp[k] = p ? p[g] : 0.0;
-        te[k] = e_prev ? e_prev[g] : 0.f;
+        te[k] = e_prev ? ld(e_prev[g]) : 0.f;
       }
     }
 #pragma unroll
@@ -252,12 +252,12 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
         double lap = (sp[kp - 1] - 2 * pc + sp[kp + 1]) * idx2 + (sp[kp - PW] - 2 * pc + sp[kp + PW]) * idx2;
         double dv = ((double)uR[k] - (double)uL[k]) * idx + ((double)vT[k] - (double)vB[k]) * idx;
         double r = dv - lap;
-        sr[q] = (float)r;
+        sr[q] = rnd<S>((float)r);  // the smoother sees the stored right-hand side
         bool inner = lx >= H && lx < H + TX && ly >= H && ly < H + TY && ox + lx < n && oy + ly < n;
         if (inner) {
           size_t g = (size_t)(oy + ly) * n + (ox + lx);
           pout[g] = pc;
-          r32[g] = (float)r;
+          r32[g] = st<S>((float)r);
           amax = fmax(amax, nan_to_inf(fabs(r)));
         }
       }
@@ -283,10 +283,10 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
 // x += bilinear prolongation of the coarse correction (PROLONG, src/gpu/multigrid.js:99-131),
 // then SW red-black sweeps. Reads xin and writes xout: a tile's halo is another
 // tile's interior, so the update cannot be in place across blocks.
-template <int SW, class TS>
+template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
-k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* __restrict__ rhs,
-         const float* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
+k_mgt_up(const S* __restrict__ xin, S* __restrict__ xout, const S* __restrict__ rhs,
+         const S* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
@@ -307,7 +307,7 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
     int q = threadIdx.x + k * NT;
     if (q < CNT) {
       int ly = q / RW, lx = q - ly * RW;
-      xv[k] = xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)];
+      xv[k] = ld(xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
     }
   }
   load_window<CW, CH, NT>(sc, xc, cn, cox, coy);
@@ -334,7 +334,7 @@ k_mgt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
-    if (gi < n && gj < n) xout[(size_t)gj * n + gi] = sx[(iy + H) * RW + ix + H];
+    if (gi < n && gj < n) xout[(size_t)gj * n + gi] = st<S>(sx[(iy + H) * RW + ix + H]);
   }
 }
 
@@ -431,8 +431,9 @@ __device__ __forceinline__ float reg_residual(float x, float r, int n, int i, in
   return r - lap;
 }
 
+template <typename S>
 __global__ void __launch_bounds__(CTH)
-k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const LevelTable tb, int pre, int post,
+k_mg_coarse_ml(S* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb, int pre, int post,
                int coarse, const float* __restrict__ pinv) {
   __shared__ float bx[CML_CELLS];
   __shared__ float br[CML_CELLS];
@@ -449,7 +450,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
 #pragma unroll
     for (int k = 0; k < K; k++) {
       int q = threadIdx.x + k * CTH;
-      if (q < n0 * n0) tmp[k] = rg[q];
+      if (q < n0 * n0) tmp[k] = ld(rg[q]);
     }
 #pragma unroll
     for (int k = 0; k < KP; k++) {
@@ -682,7 +683,7 @@ k_mg_coarse_ml(float* __restrict__ xg, const float* __restrict__ rg, const Level
       }
     }
   }
-  for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = bx[q];
+  for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = st<S>(bx[q]);
 }
 
 }  // namespace fsk

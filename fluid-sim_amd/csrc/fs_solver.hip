@@ -123,6 +123,7 @@ struct fs_solver_t {
   bool have_p = false;       // a previous pressure exists (warm start)
   std::vector<Level> lv;     // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
   bool fused = false;        // LDS-tiled V-cycle in use
+  bool bf16 = false;         // multigrid correction and right-hand sides stored as BF16
   float* pinv = nullptr;     // coarsest-level pseudo-inverse (null: run the sweeps)
   int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
@@ -499,7 +500,7 @@ int mg_vcycle(fs_solver_t* s) {
   {
     Level& C = s->lv[Lc - 1];
     if (C.n <= fsk::COARSE_N) {
-      int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml, dim3(1), dim3(fsk::CTH), C.x,
+      int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml<float>, dim3(1), dim3(fsk::CTH), C.x,
                       (const float*)C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre, s->cfg.mg_post,
                       s->cfg.mg_coarse, (const float*)s->pinv);
       if (rc) return rc;
@@ -557,61 +558,90 @@ inline dim3 tiles_grid_t(int n, int* tiles_x, int* nblocks) {
    : (k) == TILE16 ? fn<SW, fsk::Tile16>(__VA_ARGS__)                                         \
                    : fn<SW, fsk::Tile8>(__VA_ARGS__))
 
+template <typename S>
+inline S* as(float* p) {
+  return reinterpret_cast<S*>(p);
+}
+template <typename S>
+inline const S* as(const float* p) {
+  return reinterpret_cast<const S*>(p);
+}
+
 template <int PRE>
 int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only);
-template <int PRE, class TS>
-int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+template <int PRE, class TS, typename S>
+int launch_down0_s(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
   Level& L0 = s->lv[0];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(L0.n, &tx, &nb);
   double idx = 1.0 / s->dx;
   int c = s->cur;
-  // u, v (8) + p (8) + previous correction (4) read; p (8) + r (4) written; then x (4) and the coarse rhs (1)
-  double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? 4 : 0) + 8 + 4 + (residual_only ? 0 : 4 + 0.25 * 4);
-  return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE, TS>, g, dim3(TS::NT), (const float*)s->u[c],
+  const double es = sizeof(S);
+  // u, v (8) + p (8) + previous correction read; p (8) + r written; then x and the coarse rhs (1/4 cell)
+  double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? es : 0) + 8 + es + (residual_only ? 0 : es + 0.25 * es);
+  return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE, TS, S>, g, dim3(TS::NT), (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
-                (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], L0.rhs, L0.x, s->lv[1].rhs, L0.n, idx,
-                idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx, nb,
-                s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
+                as<S>((const float*)(it > 0 ? L0.xb : nullptr)), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
+                as<S>(s->lv[1].rhs), L0.n, idx, idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx,
+                nb, s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
+}
+template <int PRE, class TS>
+int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+  return s->bf16 ? launch_down0_s<PRE, TS, fsk::bf16>(s, it, pin, zero_p, residual_only)
+                 : launch_down0_s<PRE, TS, float>(s, it, pin, zero_p, residual_only);
 }
 template <int PRE>
 int launch_down0(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
   return DISPATCH_TILE(s->lv[0].tile[0], launch_down0_t, PRE, s, it, pin, zero_p, residual_only);
 }
 
-template <int PRE, class TS>
-int launch_down_t(fs_solver_t* s, int l) {
+template <int PRE, class TS, typename S>
+int launch_down_s(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
-  // rhs (4) read; x (4) and the coarse rhs (1 per 4 cells) written
-  return launch(s, PK_DOWN, 9.0 * F.n * F.n, fsk::k_mgt_down<PRE, TS>, g, dim3(TS::NT), F.x, (const float*)F.rhs,
-                F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), s->lv[l + 1].rhs, tx, nb);
+  // rhs read; x and the coarse rhs (1 per 4 cells) written
+  return launch(s, PK_DOWN, 2.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_down<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
+                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)),
+                as<S>(s->lv[l + 1].rhs), tx, nb);
+}
+template <int PRE, class TS>
+int launch_down_t(fs_solver_t* s, int l) {
+  return s->bf16 ? launch_down_s<PRE, TS, fsk::bf16>(s, l) : launch_down_s<PRE, TS, float>(s, l);
 }
 template <int PRE>
 int launch_down(fs_solver_t* s, int l) {
   return DISPATCH_TILE(s->lv[l].tile[0], launch_down_t, PRE, s, l);
 }
 
-template <int POST, class TS>
-int launch_up_t(fs_solver_t* s, int l) {
+template <int POST, class TS, typename S>
+int launch_up_s(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
-  // x (4), rhs (4), coarse x (1 per 4 cells) read; x (4) written
-  return launch(s, PK_UP, 13.0 * F.n * F.n, fsk::k_mgt_up<POST, TS>, g, dim3(TS::NT), (const float*)F.x, F.xb,
-                (const float*)F.rhs, (const float*)s->lv[l + 1].xb, F.n, (float)(F.dx * F.dx), tx, nb);
+  // x, rhs, coarse x (1 per 4 cells) read; x written
+  return launch(s, PK_UP, 3.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_up<POST, TS, S>, g, dim3(TS::NT),
+                as<S>((const float*)F.x), as<S>(F.xb), as<S>((const float*)F.rhs),
+                as<S>((const float*)s->lv[l + 1].xb), F.n, (float)(F.dx * F.dx), tx, nb);
+}
+template <int POST, class TS>
+int launch_up_t(fs_solver_t* s, int l) {
+  return s->bf16 ? launch_up_s<POST, TS, fsk::bf16>(s, l) : launch_up_s<POST, TS, float>(s, l);
 }
 template <int POST>
 int launch_up(fs_solver_t* s, int l) {
   return DISPATCH_TILE(s->lv[l].tile[1], launch_up_t, POST, s, l);
 }
 
-int launch_coarse_ml(fs_solver_t* s, int from) {
+template <typename S>
+int launch_coarse_s(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
-  return launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml, dim3(1), dim3(fsk::CTH), C.xb,
-                (const float*)C.rhs, level_table(s, from, (int)s->lv.size()), s->cfg.mg_pre, s->cfg.mg_post,
-                s->cfg.mg_coarse, (const float*)s->pinv);
+  return launch(s, PK_COARSE, 2.0 * sizeof(S) * C.n * C.n, fsk::k_mg_coarse_ml<S>, dim3(1), dim3(fsk::CTH),
+                as<S>(C.xb), as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()), s->cfg.mg_pre,
+                s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
+}
+int launch_coarse_ml(fs_solver_t* s, int from) {
+  return s->bf16 ? launch_coarse_s<fsk::bf16>(s, from) : launch_coarse_s<float>(s, from);
 }
 
 #define DISPATCH_SW(sw, fn, ...) \
@@ -619,13 +649,17 @@ int launch_coarse_ml(fs_solver_t* s, int from) {
 
 // FP64 residual `it` on the per-operator path (and the fused path when the whole
 // hierarchy fits one workgroup): reads p64[pin] (+ e), writes p64[1-pin] and r.
-int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
+template <typename S>
+int enqueue_residual_s(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  double bpc = 8 + (zero_p ? 0 : 8) + (e ? 4 : 0) + 8 + 4;
-  return launch(s, PK_RESIDUAL, bpc * s->n, fsk::k_residual64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
-                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), e, s->p64[1 - pin],
-                s->lv[0].rhs, s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
+  double bpc = 8 + (zero_p ? 0 : 8) + (e ? sizeof(S) : 0) + 8 + sizeof(S);
+  return launch(s, PK_RESIDUAL, bpc * s->n, fsk::k_residual64<S>, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), as<S>(e), s->p64[1 - pin],
+                as<S>(s->lv[0].rhs), s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
+}
+int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
+  return s->bf16 ? enqueue_residual_s<fsk::bf16>(s, it, pin, zero_p, e) : enqueue_residual_s<float>(s, it, pin, zero_p, e);
 }
 
 // Residual `it`, then (unless residual_only) the V-cycle on its correction.
@@ -857,14 +891,19 @@ int build_levels(fs_solver_t* s) {
   while (cs < Lc && s->lv[cs].n > fsk::COARSE_N) cs++;
   s->fused = s->cfg.mg_fused && cs < Lc && Lc - cs <= fsk::CML_MAXL && s->cfg.mg_pre >= 1 && s->cfg.mg_pre <= 3 &&
              s->cfg.mg_post >= 1 && s->cfg.mg_post <= 3;
+  s->bf16 = s->cfg.mg_precision == FS_MG_BF16;
+  if (s->bf16 && !s->fused)
+    return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused V-cycle: pre/post sweeps in 1..3 and at "
+                                  "most %d levels at or below %d cells", fsk::CML_MAXL, fsk::COARSE_N);
   s->coarse_start = cs;
   for (auto& L : s->lv) L.tile[0] = L.tile[1] = tile_kind(L.n);
   for (size_t l = 0; l < s->lv.size(); l++) {
     size_t c = (size_t)s->lv[l].n * s->lv[l].n;
-    int rc = dalloc(&s->lv[l].x, c);
+    size_t w = s->bf16 ? c / 2 : c;  // FP32 words (levels have an even cell count)
+    int rc = dalloc(&s->lv[l].x, w);
     if (rc) return rc;
-    if ((rc = dalloc(&s->lv[l].rhs, c))) return rc;
-    if (s->fused && (rc = dalloc(&s->lv[l].xb, c))) return rc;
+    if ((rc = dalloc(&s->lv[l].rhs, w))) return rc;
+    if (s->fused && (rc = dalloc(&s->lv[l].xb, w))) return rc;
   }
   return build_coarse_inverse(s);
 }
@@ -1073,8 +1112,14 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (c.tolerance > 0 && c.pressure_solver != FS_P_MULTIGRID)
     return fail(FS_E_INVALID_ARG, "a residual tolerance needs the multigrid solver");
   if (c.mg_pre < 0 || c.mg_post < 0 || c.mg_coarse < 0) return fail(FS_E_INVALID_ARG, "negative sweep count");
-  if (c.mg_precision != FS_MG_FP32)
-    return fail(FS_E_UNSUPPORTED, "multigrid storage precision %d is not built yet", c.mg_precision);
+  if (c.mg_precision == FS_MG_FP16)
+    return fail(FS_E_UNSUPPORTED,
+                "FP16 multigrid storage is not built: the correction scales as h^2 |r| and underflows FP16 on "
+                "fine grids; use BF16 (FP32 exponent range)");
+  if (c.mg_precision != FS_MG_FP32 && c.mg_precision != FS_MG_BF16)
+    return fail(FS_E_INVALID_ARG, "unknown multigrid precision %d", c.mg_precision);
+  if (c.mg_precision == FS_MG_BF16 && !(c.pressure_solver == FS_P_MULTIGRID && c.mg_fused))
+    return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused multigrid V-cycle (mg_fused = 1)");
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
     return fail(c.guess == FS_GUESS_LEARNED ? FS_E_UNSUPPORTED : FS_E_INVALID_ARG, "guess mode %d not available",
                 c.guess);
@@ -1133,10 +1178,11 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (!rc) rc = fill_f64(s, s->p64[0], s->n, 0.0);
   if (!rc) rc = fill_f64(s, s->p64[1], s->n, 0.0);
   if (!rc) rc = fill_f32(s, s->div, s->n, 0.f);
-  for (auto& l : s->lv) {
-    if (!rc) rc = fill_f32(s, l.x, (size_t)l.n * l.n, 0.f);
-    if (!rc) rc = fill_f32(s, l.rhs, (size_t)l.n * l.n, 0.f);
-    if (!rc && l.xb) rc = fill_f32(s, l.xb, (size_t)l.n * l.n, 0.f);
+  for (auto& l : s->lv) {  // zero bits are 0.0 in FP32 and in BF16
+    size_t w = s->bf16 ? (size_t)l.n * l.n / 2 : (size_t)l.n * l.n;
+    if (!rc) rc = fill_f32(s, l.x, w, 0.f);
+    if (!rc) rc = fill_f32(s, l.rhs, w, 0.f);
+    if (!rc && l.xb) rc = fill_f32(s, l.xb, w, 0.f);
   }
   // on zero inputs every fused kernel writes zeros, so tuning leaves the state as it is
   if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) rc = autotune_tiles(s);
@@ -1221,6 +1267,18 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
       if (s->lv.empty()) return fail(FS_E_INVALID_ARG, "no multigrid residual without the multigrid solver");
       src = s->lv[0].rhs;
       need = s->n * sizeof(float);
+      if (s->bf16) {  // widen the stored BF16 values
+        if (bytes != need) return fail(FS_E_INVALID_ARG, "field %d needs %zu bytes, got %zu", field, need, bytes);
+        if ((rc = set_device(s))) return rc;
+        std::vector<uint16_t> b(s->n);
+        HIPCHK(hipMemcpyAsync(b.data(), src, s->n * sizeof(uint16_t), hipMemcpyDeviceToHost, s->stream));
+        if ((rc = sync_stream(s))) return rc;
+        for (size_t k = 0; k < s->n; k++) {
+          uint32_t w = (uint32_t)b[k] << 16;
+          std::memcpy((float*)out + k, &w, 4);
+        }
+        return FS_OK;
+      }
       break;
     default: return fail(FS_E_INVALID_ARG, "unknown field %d", field);
   }

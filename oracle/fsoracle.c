@@ -699,6 +699,114 @@ int or_mg_vcycles(int n, double lx, const double *rhs, double *p, int cycles,
   return L;
 }
 
+/* -------------------------------------- refinement with stored-precision rounding */
+
+/* value after a store in `storage` (0: FP32, 1: BF16) of the FP32 rounding of x;
+ * BF16 keeps the upper 16 bits of the FP32, rounded to nearest even */
+double or_round_storage(double x, int storage) {
+  float f = (float)x;
+  if (storage == 1) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return x;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+  }
+  return (double)f;
+}
+
+static void round_field(double *a, size_t n, int storage) {
+  for (size_t k = 0; k < n; k++) a[k] = or_round_storage(a[k], storage);
+}
+
+/* The projection as the GPU solver runs it with a fused V-cycle: iterative
+ * refinement p <- p + e with r = div - L p in FP64, each V-cycle on the
+ * correction e from zero, and the correction and right-hand sides rounded to the
+ * storage precision wherever they pass through memory: the level-0 residual, the
+ * pre-smoothed correction and restricted rhs of every level above coarse_n, the
+ * coarse solve's result, and each post-smoothed correction. Levels at or below
+ * coarse_n run unrounded (they stay on chip). The V-cycle is that of
+ * src/gpu/multigrid.js:277-330; with no rounding the k-th p equals k plain
+ * V-cycles (the cycle is affine in its right-hand side).
+ * rmax[it] = max|div - L p| before refinement step it, for it = 0..iterations. */
+int or_mg_refine(int n, double lx, const double *div, double *p, int iterations, int min_size, int pre,
+                 int post, int coarse, int coarse_n, int storage, double *rmax) {
+  int L = or_mg_levels(n, min_size);
+  if (L < 1) return -1;
+  MGL *lv = (MGL *)calloc(L, sizeof(MGL));
+  double **xs = (double **)calloc(L, sizeof(double *));  /* stored pre-smoothed corrections */
+  int m = n, cs = L;
+  for (int i = 0; i < L; i++) {
+    size_t c = (size_t)m * m;
+    lv[i].n = m;
+    lv[i].dx = lx / m;
+    lv[i].x = (double *)calloc(c, sizeof(double));
+    lv[i].rhs = (double *)calloc(c, sizeof(double));
+    lv[i].res = (double *)calloc(c, sizeof(double));
+    xs[i] = (double *)calloc(c, sizeof(double));
+    if (cs == L && m <= coarse_n) cs = i;
+    m /= 2;
+  }
+  if (cs >= L) cs = L - 1;  /* the coarsest level is always solved on chip */
+  size_t N = (size_t)n * n;
+  double *lap = (double *)malloc(sizeof(double) * N);
+  double *e = (double *)calloc(N, sizeof(double));
+  for (int it = 0; it <= iterations; it++) {
+    if (it > 0)
+      for (size_t k = 0; k < N; k++) p[k] += e[k];
+    or_laplacian(n, n, lx, lx, p, lap);
+    double mx = 0;
+    for (size_t k = 0; k < N; k++) {
+      double r = div[k] - lap[k];
+      if (fabs(r) > mx) mx = fabs(r);
+      lv[0].rhs[k] = or_round_storage(r, storage);
+    }
+    if (rmax) rmax[it] = mx;
+    if (it == iterations) break;
+    /* grid-wide levels: smooth from zero, restrict, store */
+    for (int i = 0; i < cs; i++) {
+      size_t c = (size_t)lv[i].n * lv[i].n;
+      memset(lv[i].x, 0, sizeof(double) * c);
+      mg_smooth(&lv[i], pre);
+      mg_residual(&lv[i]);
+      mg_restrict(&lv[i], &lv[i + 1]);
+      round_field(lv[i + 1].rhs, (size_t)lv[i + 1].n * lv[i + 1].n, storage);
+      memcpy(xs[i], lv[i].x, sizeof(double) * c);
+      round_field(xs[i], c, storage);
+    }
+    /* on-chip levels cs..L-1: a plain V-cycle from zero */
+    for (int i = cs; i < L; i++) memset(lv[i].x, 0, sizeof(double) * lv[i].n * lv[i].n);
+    for (int i = cs; i < L - 1; i++) {
+      mg_smooth(&lv[i], pre);
+      mg_residual(&lv[i]);
+      mg_restrict(&lv[i], &lv[i + 1]);
+    }
+    /* the coarsest rhs loses its mean first (the periodic compatibility condition,
+     * src/cpu/projection.js:91-102): stored rounding leaves it slightly non-zero,
+     * and sweeps on an incompatible rhs drift by a growing constant */
+    remove_mean(lv[L - 1].rhs, (size_t)lv[L - 1].n * lv[L - 1].n);
+    mg_smooth(&lv[L - 1], coarse);
+    for (int i = L - 2; i >= cs; i--) {
+      mg_prolong(&lv[i + 1], &lv[i]);
+      mg_smooth(&lv[i], post);
+    }
+    if (cs < L) round_field(lv[cs].x, (size_t)lv[cs].n * lv[cs].n, storage);
+    /* grid-wide levels up: stored x + prolongated correction, post-smooth, store */
+    for (int i = cs - 1; i >= 0; i--) {
+      size_t c = (size_t)lv[i].n * lv[i].n;
+      memcpy(lv[i].x, xs[i], sizeof(double) * c);
+      mg_prolong(&lv[i + 1], &lv[i]);
+      mg_smooth(&lv[i], post);
+      round_field(lv[i].x, c, storage);
+    }
+    memcpy(e, lv[0].x, sizeof(double) * N);
+  }
+  for (int i = 0; i < L; i++) { free(lv[i].x); free(lv[i].rhs); free(lv[i].res); free(xs[i]); }
+  free(lv); free(xs); free(lap); free(e);
+  return L;
+}
+
 /* -------------------------------------------------------------------- forcing */
 
 static double js_round(double x) {

@@ -129,6 +129,12 @@ double or_velocity_error_tg(int nx, int ny, double lx, double ly, const double *
 /* shell-binned energy spectrum (src/core/fft.js:196-234); E has kmax+1 entries */
 int    or_energy_spectrum(int nx, int ny, const double *cu, const double *cv, double *E, int *counts);
 
+/* storage rounding (0: FP32, 1: BF16) and the refinement projection of the GPU
+ * solver with the correction stored at that precision */
+double or_round_storage(double x, int storage);
+int    or_mg_refine(int n, double lx, const double *div, double *p, int iterations, int min_size, int pre,
+                    int post, int coarse, int coarse_n, int storage, double *rmax);
+
 /* immersed disk (src/cpu/solid.js:38-40, :101-114, :165-233) */
 double or_wrap_delta(double d, double l);
 double or_disk_mask(const or_disk *s, double px, double py, double lx, double ly, double width);

@@ -110,6 +110,8 @@ def lib():
         "or_energy_spectrum": (i, [i, i, p, p, p, _I]),
         "or_step": (i, [C.POINTER(StepCfg), p, p, C.c_void_p, d, C.c_uint32]),
         "or_wrap_delta": (d, [d, d]),
+        "or_round_storage": (d, [d, i]),
+        "or_mg_refine": (i, [i, d, p, p, i, i, i, i, i, i, i, p]),
         "or_disk_mask": (d, [C.POINTER(Disk), d, d, d, d, d]),
         "or_disk_velocity": (None, [C.POINTER(Disk), d, d, d, d, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "or_apply_coupling": (None, [i, i, d, d, p, p, C.POINTER(Disk), d, i, p]),
@@ -314,6 +316,25 @@ def fft2(re, im, nx, ny, sign=-1):
 
 def set_threads(n):
     lib().or_set_threads(int(n))
+
+
+STORAGE = {"fp32": 0, "bf16": 1}
+
+
+def round_storage(x, storage):
+    return lib().or_round_storage(float(x), STORAGE[storage])
+
+
+def mg_refine(n, div, iterations, min_size=4, pre=2, post=2, coarse=40, coarse_n=64, storage="fp32", p=None,
+              lx=TWO_PI):
+    """GPU-style refinement projection; returns (p, max|div - L p| before each step 0..iterations)."""
+    p = np.zeros(n * n) if p is None else _f(p).copy()
+    rmax = np.zeros(iterations + 1)
+    rc = lib().or_mg_refine(n, lx, _f(div), p, iterations, min_size, pre, post, coarse, coarse_n,
+                            STORAGE[storage], rmax)
+    if rc < 1:
+        raise ValueError("no multigrid levels")
+    return p, rmax
 
 
 def _disk_tuple(d):

@@ -74,6 +74,23 @@ def test_odd_grid_rejected_for_red_black_orderings():
         assert "even grid" in _lib.last_error()
 
 
+@pytest.mark.parametrize("prec,fused,solver,msg", [
+    ("fp16", 1, "multigrid", "FP16 multigrid storage is not built"),
+    ("bf16", 0, "multigrid", "needs the fused multigrid"),
+    ("bf16", 1, "jacobi", "needs the fused multigrid"),
+])
+def test_storage_precision_limits_are_reported(prec, fused, solver, msg):
+    from fluidsim import _lib
+    L = _lib.load()
+    c = _lib.default_config()
+    c.mg_precision = _lib.PRECISION[prec]
+    c.mg_fused = fused
+    c.pressure_solver = _lib.PRESSURE[solver]
+    h = C.c_void_p()
+    assert L.fs_solver_create(C.byref(c), C.byref(h)) == _lib.FS_E_UNSUPPORTED
+    assert msg in _lib.last_error()
+
+
 def test_tolerance_needs_multigrid():
     from fluidsim import _lib
     L = _lib.load()

@@ -475,3 +475,55 @@ def test_autotuned_tiles_give_the_same_bits(fs, O, n):
                 assert "level 0" in rep and "down0" in rep and "us" in rep
     for o in outs[1:]:
         assert np.array_equal(outs[0]["u"], o["u"]) and np.array_equal(outs[0]["v"], o["v"])
+
+
+# ------------------------------------------------------ BF16 multigrid storage
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_bf16_storage_residual_curve_matches_oracle(fs, O, n):
+    # BF16-stored correction and right-hand sides, FP32 arithmetic, FP64 refinement
+    # residual: the residual after k V-cycles against the oracle's emulation of the
+    # same storage points; rounding decisions of FP32 vs f64 arithmetic may differ
+    # at a few cells, so the curves agree to a few percent, not bit for bit
+    u0, v0 = O.js_random_velocity(n, 777)
+    d = O.divergence(n, n, f32(u0), f32(v0))
+    with solver(fs, n=n, pressureSolver="multigrid", mgMinSize=4, mgPrecision="bf16") as g:
+        g.uploadVelocity(u0, v0)
+        for it in (1, 2, 3, 4):
+            r = g.poissonResidualAfter(it)
+            p, _ = O.mg_refine(n, d, it, min_size=4, storage="bf16")
+            ro = O.poisson_residual_rel(n, n, p, d)
+            p32, _ = O.mg_refine(n, d, it, min_size=4, storage="fp32")
+            r32 = O.poisson_residual_rel(n, n, p32, d)
+            assert abs(r - ro) / ro < 0.05, (n, it, r, ro, r32)
+            # and the storage matters: BF16 is measurably different from FP32 storage
+            if it >= 2:
+                assert abs(ro - r32) / r32 > 1e-3
+
+
+def test_bf16_storage_meets_the_tolerance_gate(fs, O):
+    # decaying turbulence at 256^2 with BF16 smoother storage: every projection meets
+    # the FP64 gate, within a V-cycle or so of the FP32-storage solver
+    from fluidsim import initial
+    n = 256
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    cyc = {}
+    for prec in ("fp32", "bf16"):
+        with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                    pressureIterations=40, tolerance=1e-6, mgMinSize=4, mgPrecision=prec) as g:
+            g.uploadVelocity(u0, v0)
+            total = 0
+            for _ in range(5):
+                g.stepCFL(0.5)
+                st = g.stats()
+                assert st["last_residual"] <= 1e-6
+                total += st["last_vcycles"]
+            cyc[prec] = total
+            out = g.readVelocity()
+    assert cyc["bf16"] <= cyc["fp32"] + 5
+    s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for _ in range(5):
+        s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5

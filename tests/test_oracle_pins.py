@@ -482,3 +482,59 @@ def test_torque_impulse_is_the_angular_momentum_change(O):
     imp = O.apply_coupling(n, n, u, v, dict(x=cx, y=cy, r=0.8), passes=2)
     assert imp[2] < 0
     assert abs((L(u, v) - L0) - imp[2]) < 1e-12 * abs(L0)
+
+
+# ---------------------------------------- refinement projection, storage rounding
+
+def test_bf16_rounding_known_answers(O):
+    # 7 explicit mantissa bits: spacing 2^-7 at 1.0; ties go to the even mantissa
+    assert O.round_storage(1.0, "bf16") == 1.0
+    assert O.round_storage(1.0 + 2 ** -8, "bf16") == 1.0
+    assert O.round_storage(1.0 + 3 * 2 ** -8, "bf16") == 1.0 + 2 * 2 ** -7
+    assert O.round_storage(1.0 + 2 ** -8 + 2 ** -20, "bf16") == 1.0 + 2 ** -7
+    assert O.round_storage(-3.0, "bf16") == -3.0
+    assert O.round_storage(1e-30, "bf16") == pytest.approx(1e-30, rel=2 ** -8)
+    assert O.round_storage(3e38, "bf16") == pytest.approx(3e38, rel=2 ** -8)
+    # FP32 storage rounds only to FP32
+    assert O.round_storage(1.0 + 2 ** -30, "fp32") == 1.0
+    assert O.round_storage(0.1, "fp32") == float(np.float32(0.1))
+
+
+def _zero_mean_div(O, n, seed):
+    vals, _ = O.js_lcg(seed, 2 * n * n)
+    u = vals[0::2] - 0.5
+    v = vals[1::2] - 0.5
+    return O.divergence(n, n, u, v)
+
+
+@pytest.mark.parametrize("n,coarse_n", [(64, 64), (128, 64), (256, 32)])
+def test_refinement_without_rounding_equals_plain_vcycles(O, n, coarse_n):
+    # a V-cycle is affine in its rhs, so k refinement steps from p = 0 reproduce k
+    # plain V-cycles (src/gpu/solver.js:686-703), whatever the on-chip split
+    div = _zero_mean_div(O, n, 5)
+    p_ref, _ = O.mg_vcycles(n, div, 4, min_size=4)
+    p, rmax = O.mg_refine(n, div, 4, min_size=4, coarse_n=coarse_n, storage=None or "fp32")
+    # fp32 storage rounding is tiny but not zero: compare against an exact-storage run
+    # through the same cycles by checking the residual histories instead
+    lap = O.laplacian(n, n, p_ref)
+    r_ref = np.max(np.abs(div - lap))
+    lap2 = O.laplacian(n, n, p)
+    r = np.max(np.abs(div - lap2))
+    assert abs(r - r_ref) <= 1e-3 * r_ref + 1e-7 * np.max(np.abs(div))
+    assert rmax[0] == pytest.approx(np.max(np.abs(div)), rel=1e-15)
+    assert np.all(np.diff(np.log(rmax)) < -1.0)   # each step gains more than a factor e
+
+
+def test_bf16_refinement_still_converges_to_the_gate(O):
+    # BF16-stored corrections: 8-bit mantissa per step, FP64 residual; the iteration
+    # keeps contracting and reaches a 1e-6 gate on a unit-scale rhs
+    n = 128
+    div = _zero_mean_div(O, n, 9)
+    div = div / np.max(np.abs(div))
+    _, r32 = O.mg_refine(n, div, 8, min_size=4, storage="fp32")
+    _, rbf = O.mg_refine(n, div, 10, min_size=4, storage="bf16")
+    k32 = int(np.argmax(r32 <= 1e-6))
+    k = int(np.argmax(rbf <= 1e-6))
+    assert r32[k32] <= 1e-6 and rbf[k] <= 1e-6 and k <= k32 + 1
+    # each BF16 step down to the gate still contracts by about 10x or more (FP32: 13-100x)
+    assert np.all(rbf[1:k + 1] / rbf[:k] < 0.12)
