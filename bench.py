@@ -44,6 +44,7 @@ KIND_TEXT = {
     "gradient": "FP64 pressure-gradient subtraction (k_gradient64), 24 B/cell",
     "residual": "FP64 residual only (k_residual64), 20-32 B/cell",
     "rb": "red-black colour sweep (k_mg_rb), 6 B/cell",
+    "guess": "learned-guess 3x3 convolutions (k_conv3x3, BF16 MFMA), BF16 activations in + out",
 }
 
 # HBM bytes per launch at 1024^2 measured by `rocprofv3 --pmc FETCH_SIZE` and
@@ -80,7 +81,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--guess", default="zero", choices=["zero", "previous"])
+    ap.add_argument("--guess", default="zero", choices=["zero", "previous", "learned"])
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the second timed run with the other pressure guess")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -106,10 +107,17 @@ def copy_calibration():
     print(f"copy calibration: 5 copies of {nbytes} bytes", flush=True)
 
 
+GUESS_WEIGHTS = os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn")
+
+
 def make_solver(fluidsim, wl, n, tol, guess="zero"):
     g = fluidsim.GPUFluidSolver(0, n=n, nu=wl["nu"], advection="maccormack", pressureSolver="multigrid",
                                 pressureIterations=60, tolerance=tol, mgMinSize=4, mgPre=2, mgPost=2,
-                                mgCoarse=40, guess=guess, mgPrecision=wl["precision"])
+                                mgCoarse=40, guess="zero" if guess == "learned" else guess,
+                                mgPrecision=wl["precision"])
+    if guess == "learned":
+        g.loadGuess(GUESS_WEIGHTS)
+        g.setGuess("learned")
     if wl["forcing"]:
         g.setForcing(wl["forcing"])
     return g
@@ -222,28 +230,32 @@ def main():
                                 for k, v in prof.items() if v[1]}}
     g.destroy()
 
-    # ablation: the same workload and timing with the other initial pressure guess
+    # ablation: the same workload and timing with the other initial pressure guesses
     ablation = None
     if not a.no_ablation:
-        other = "previous" if a.guess == "zero" else "zero"
-        g2 = make_solver(fluidsim, wl, n, a.tol, other)
-        g2.uploadVelocity(u0, v0)
-        for _ in range(a.warmup):
-            g2.stepCFL(0.5)
-        g2.sync()
-        s0 = g2.stats()
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            g2.stepCFL(0.5)
-        g2.sync()
-        el2 = time.perf_counter() - t0
-        s1 = g2.stats()
-        c2 = (s1["total_vcycles"] - s0["total_vcycles"]) / max(a.steps, 1)
-        ablation = {"guess": other, "value": n * n * a.steps / el2, "ms_per_step": el2 / a.steps * 1e3,
-                    "vcycles_per_step": c2, "last_residual": s1["last_residual"]}
-        g2.destroy()
-        print(f"# {other} guess: {ablation['value']:.4g} cell-steps/s, {c2:.2f} V-cycles/step",
-              file=sys.stderr, flush=True)
+        ablation = []
+        others = [x for x in ("zero", "previous", "learned") if x != a.guess]
+        for other in others:
+            if other == "learned" and (wl["precision"] != "fp32" or n % 128 or not os.path.exists(GUESS_WEIGHTS)):
+                continue
+            g2 = make_solver(fluidsim, wl, n, a.tol, other)
+            g2.uploadVelocity(u0, v0)
+            for _ in range(a.warmup):
+                g2.stepCFL(0.5)
+            g2.sync()
+            s0 = g2.stats()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                g2.stepCFL(0.5)
+            g2.sync()
+            el2 = time.perf_counter() - t0
+            s1 = g2.stats()
+            c2 = (s1["total_vcycles"] - s0["total_vcycles"]) / max(a.steps, 1)
+            ablation.append({"guess": other, "value": n * n * a.steps / el2, "ms_per_step": el2 / a.steps * 1e3,
+                             "vcycles_per_step": c2, "last_residual": s1["last_residual"]})
+            g2.destroy()
+            print(f"# {other} guess: {ablation[-1]['value']:.4g} cell-steps/s, {c2:.2f} V-cycles/step, "
+                  f"{ablation[-1]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:

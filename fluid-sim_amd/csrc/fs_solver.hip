@@ -35,6 +35,7 @@
 #include "../../include/fluidsim.h"
 #include "fs_kernels.hpp"
 #include "fs_mg_fused.hpp"
+#include "fs_guess.hpp"
 
 namespace {
 
@@ -82,7 +83,7 @@ struct Level {
 // itself (not the gap before it), and carries its algorithmic byte count.
 enum {
   PK_ADVECT_SL = 0, PK_ADVECT_MC, PK_DIFFUSE, PK_DOWN0, PK_DOWN, PK_COARSE, PK_UP, PK_GRADIENT, PK_RESIDUAL,
-  PK_RB, PK_COUNT
+  PK_RB, PK_GUESS, PK_COUNT
 };
 
 struct Profile {
@@ -144,6 +145,18 @@ struct fs_solver_t {
   // forcing
   bool forcing_on = false;
   fs_forcing forcing{};
+  // learned initial guess: weights (BF16 [cout][tap][cin] for the MFMA layers) and activations
+  struct GuessNet {
+    bool loaded = false;
+    float* w0 = nullptr;            // first layer FP32 [16][9][2]
+    float* b[15] = {};              // biases (FP32)
+    uint16_t* w[15] = {};           // 3x3 layers 1..13 (BF16)
+    float* wout = nullptr;          // output 1x1 FP32 [16]
+    uint16_t* act[14] = {};         // activations, NHWC BF16
+    double* scales = nullptr;       // max|r|, max|p_prev|
+    double* ppart = nullptr;        // per-block max|p_prev|
+    int nblk = 0;
+  } guess_net;
   // immersed disk: state for the next step, per-block impulse partials, FP64 total
   bool solid_on = false;
   fs_disk solid{};
@@ -701,13 +714,111 @@ int read_residuals(fs_solver_t* s, int rows, std::vector<double>& rmax) {
   return FS_OK;
 }
 
+// ----------------------------------------------------------- learned guess
+
+// Layer table of the network: input channels, output channels, kernel size, level
+// (0 = n, 1 = n/2, ...), input mode and the channels that come from the coarser
+// tensor when concatenating.
+struct GuessLayer {
+  int cin, cout, k, level, mode, c2;
+};
+static const GuessLayer kGuessLayers[15] = {
+    {2, 16, 3, 0, fsk::CONV_DIRECT, 0},    // enc0a (VALU)
+    {16, 16, 3, 0, fsk::CONV_DIRECT, 0},   // enc0b
+    {16, 32, 3, 1, fsk::CONV_POOL, 0},     // enc1a
+    {32, 32, 3, 1, fsk::CONV_DIRECT, 0},   // enc1b
+    {32, 64, 3, 2, fsk::CONV_POOL, 0},     // enc2a
+    {64, 64, 3, 2, fsk::CONV_DIRECT, 0},   // enc2b
+    {64, 64, 3, 3, fsk::CONV_POOL, 0},     // bot_a
+    {64, 64, 3, 3, fsk::CONV_DIRECT, 0},   // bot_b
+    {128, 64, 3, 2, fsk::CONV_UPCAT, 64},  // dec2a = [up(bot_b), enc2b]
+    {64, 64, 3, 2, fsk::CONV_DIRECT, 0},   // dec2b
+    {96, 32, 3, 1, fsk::CONV_UPCAT, 64},   // dec1a = [up(dec2b), enc1b]
+    {32, 32, 3, 1, fsk::CONV_DIRECT, 0},   // dec1b
+    {48, 16, 3, 0, fsk::CONV_UPCAT, 32},   // dec0a = [up(dec1b), enc0b]
+    {16, 16, 3, 0, fsk::CONV_DIRECT, 0},   // dec0b
+    {16, 1, 1, 0, fsk::CONV_DIRECT, 0},    // out
+};
+// input tensors (x1, x2) of each MFMA layer by activation index (-1: none)
+static const int kGuessIn[15][2] = {{-1, -1}, {0, -1}, {1, -1}, {2, -1}, {3, -1}, {4, -1}, {5, -1}, {6, -1},
+                                    {5, 7},   {8, -1}, {3, 9},  {10, -1}, {1, 11}, {12, -1}, {13, -1}};
+
+template <int CIN, int COUT, int MODE, int C2>
+int launch_conv(fs_solver_t* s, int layer) {
+  auto& G = s->guess_net;
+  const int m = s->nx >> kGuessLayers[layer].level;
+  const uint16_t* x1 = G.act[kGuessIn[layer][0]];
+  const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? G.act[kGuessIn[layer][1]] : nullptr;
+  dim3 g((unsigned)((m / fsk::GT_W) * (m / fsk::GT_H)));
+  // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and out, BF16
+  double in_ch = MODE == fsk::CONV_POOL ? 4.0 * CIN : MODE == fsk::CONV_UPCAT ? C2 / 4.0 + (CIN - C2) : CIN;
+  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2>, g, dim3(fsk::GT_NT),
+                x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m);
+}
+
+// Evaluate the network on r (level-0 residual of p_prev, FP32) and p_prev = p64[pi],
+// leaving p_prev + s_r h^2 out in p64[pi]. Inputs come from the last residual pass
+// (rpart row 0 holds its per-tile maxima).
+int run_guess(fs_solver_t* s, int pi) {
+  auto& G = s->guess_net;
+  const int n = s->nx;
+  int rc;
+  fsk::k_absmax64<<<G.nblk, 256, 0, s->stream>>>(s->p64[pi], s->n, G.ppart);
+  LAUNCHCHK();
+  fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(s->rpart, s->rp_cols, G.ppart, G.nblk, G.scales);
+  LAUNCHCHK();
+  fsk::k_guess_first<<<dim3((unsigned)(n / 64), (unsigned)(n / 4)), 256, 0, s->stream>>>(
+      s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0], n);
+  LAUNCHCHK();
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0>(s, 1))) return rc;
+  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0>(s, 2))) return rc;
+  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0>(s, 3))) return rc;
+  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0>(s, 4))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 5))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0>(s, 6))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 7))) return rc;
+  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64>(s, 8))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 9))) return rc;
+  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64>(s, 10))) return rc;
+  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0>(s, 11))) return rc;
+  if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32>(s, 12))) return rc;
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0>(s, 13))) return rc;
+  unsigned blocks = (unsigned)((s->n + 255) / 256);
+  fsk::k_guess_out<<<blocks, 256, 0, s->stream>>>(G.act[13], G.wout, G.b[14], G.scales, s->p64[pi], n,
+                                                  s->dx * s->dx);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+void release_guess(fs_solver_t* s) {
+  auto& G = s->guess_net;
+  dfree(G.w0);
+  dfree(G.wout);
+  for (int l = 0; l < 15; l++) {
+    dfree(G.b[l]);
+    dfree(G.w[l]);
+  }
+  for (int a = 0; a < 14; a++) dfree(G.act[a]);
+  dfree(G.scales);
+  dfree(G.ppart);
+  G = fs_solver_t::GuessNet{};
+}
+
 // Multigrid projection by iterative refinement; leaves the pressure in p64[s->pidx].
 int solve_multigrid(fs_solver_t* s, int* cycles_out) {
   const double tol = s->cfg.tolerance;
   const int maxc = std::max(0, s->cfg.pressure_iterations);
-  const bool zero_p = !(s->cfg.guess == FS_GUESS_PREVIOUS && s->have_p);
-  const int a0 = s->pidx;
+  bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || s->cfg.guess == FS_GUESS_LEARNED) && s->have_p);
+  int a0 = s->pidx;
   int rc;
+  if (s->cfg.guess == FS_GUESS_LEARNED) {
+    // the residual the previous pressure leaves, the network's increment on top of it,
+    // then refinement from that guess
+    if ((rc = enqueue_iteration(s, 0, a0 & 1, zero_p, true))) return rc;
+    if ((rc = run_guess(s, 1 - (a0 & 1)))) return rc;
+    a0 = 1 - (a0 & 1);
+    zero_p = false;
+  }
   if (!(tol > 0)) {
     // k V-cycles: residual(0), V(0), ..., residual(k-1), V(k-1), residual(k); no host round trip
     for (int it = 0; it <= maxc; it++)
@@ -987,10 +1098,13 @@ int ensure_rpart(fs_solver_t* s, int iterations) {
   return alloc_rpart(s, iterations);
 }
 
+void release_guess(fs_solver_t* s);
+
 void release(fs_solver_t* s) {
   if (!s) return;
   (void)hipSetDevice(s->dev);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  release_guess(s);
   for (int k = 0; k < 2; k++) {
     dfree(s->u[k]);
     dfree(s->v[k]);
@@ -1120,9 +1234,10 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     return fail(FS_E_INVALID_ARG, "unknown multigrid precision %d", c.mg_precision);
   if (c.mg_precision == FS_MG_BF16 && !(c.pressure_solver == FS_P_MULTIGRID && c.mg_fused))
     return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused multigrid V-cycle (mg_fused = 1)");
+  if (c.guess == FS_GUESS_LEARNED)
+    return fail(FS_E_INVALID_ARG, "create with guess zero or previous, then fs_load_guess and fs_set_guess(learned)");
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
-    return fail(c.guess == FS_GUESS_LEARNED ? FS_E_UNSUPPORTED : FS_E_INVALID_ARG, "guess mode %d not available",
-                c.guess);
+    return fail(FS_E_INVALID_ARG, "unknown guess mode %d", c.guess);
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0) return fail(FS_E_DEVICE, "no HIP device available");
@@ -1411,9 +1526,88 @@ int fs_set_nu(void* h, double nu) {
 int fs_set_guess(void* h, int guess) {
   int rc = check_solver(h);
   if (rc) return rc;
-  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS)
-    return fail(guess == FS_GUESS_LEARNED ? FS_E_UNSUPPORTED : FS_E_INVALID_ARG, "guess mode %d not available", guess);
-  ((fs_solver_t*)h)->cfg.guess = guess;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && guess != FS_GUESS_LEARNED)
+    return fail(FS_E_INVALID_ARG, "unknown guess mode %d", guess);
+  if (guess == FS_GUESS_LEARNED && !s->guess_net.loaded)
+    return fail(FS_E_INVALID_ARG, "the learned guess needs its weights: call fs_load_guess first");
+  s->cfg.guess = guess;
+  return FS_OK;
+}
+
+// Weights blob: "FSGN", uint32 version 1, uint32 layer count 15, then per layer
+// uint32 cin, cout, k and FP32 weights [cout][k*k][cin] followed by FP32 bias[cout].
+int fs_load_guess(void* h, const void* blob, size_t bytes) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (!blob) return fail(FS_E_INVALID_ARG, "blob is NULL");
+  if (s->cfg.pressure_solver != FS_P_MULTIGRID || s->bf16)
+    return fail(FS_E_UNSUPPORTED, "the learned guess needs the multigrid solver with FP32 storage");
+  if (s->nx != s->ny || s->nx % 128 != 0)
+    return fail(FS_E_UNSUPPORTED, "the learned guess needs a square grid whose size is a multiple of 128, got %dx%d",
+                s->nx, s->ny);
+  const unsigned char* b = (const unsigned char*)blob;
+  size_t off = 0;
+  auto take = [&](void* dst, size_t nb) -> bool {
+    if (off + nb > bytes) return false;
+    std::memcpy(dst, b + off, nb);
+    off += nb;
+    return true;
+  };
+  char magic[4];
+  uint32_t ver = 0, nl = 0;
+  if (!take(magic, 4) || std::memcmp(magic, "FSGN", 4) != 0 || !take(&ver, 4) || ver != 1 || !take(&nl, 4) || nl != 15)
+    return fail(FS_E_INVALID_ARG, "not a version-1 guess network blob");
+  if ((rc = set_device(s))) return rc;
+  if ((rc = sync_stream(s))) return rc;
+  release_guess(s);
+  auto& G = s->guess_net;
+  auto bail = [&](int code) {
+    release_guess(s);
+    return code;
+  };
+  for (int l = 0; l < 15; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    uint32_t dims[3];
+    if (!take(dims, sizeof dims)) return bail(fail(FS_E_INVALID_ARG, "blob truncated at layer %d", l));
+    if ((int)dims[0] != L.cin || (int)dims[1] != L.cout || (int)dims[2] != L.k)
+      return bail(fail(FS_E_INVALID_ARG, "layer %d is %ux%u k%u, expected %dx%d k%d", l, dims[0], dims[1], dims[2],
+                       L.cin, L.cout, L.k));
+    size_t nw = (size_t)L.cout * L.k * L.k * L.cin;
+    std::vector<float> w(nw), bias(L.cout);
+    if (!take(w.data(), nw * 4) || !take(bias.data(), (size_t)L.cout * 4))
+      return bail(fail(FS_E_INVALID_ARG, "blob truncated at layer %d", l));
+    for (float v : w)
+      if (!std::isfinite(v)) return bail(fail(FS_E_INVALID_ARG, "non-finite weight in layer %d", l));
+    if ((rc = dalloc(&G.b[l], (size_t)L.cout))) return bail(rc);
+    HIPCHK(hipMemcpy(G.b[l], bias.data(), (size_t)L.cout * 4, hipMemcpyHostToDevice));
+    if (l == 0 || l == 14) {
+      float** dst = l == 0 ? &G.w0 : &G.wout;
+      if ((rc = dalloc(dst, nw))) return bail(rc);
+      HIPCHK(hipMemcpy(*dst, w.data(), nw * 4, hipMemcpyHostToDevice));
+    } else {
+      std::vector<uint16_t> wb(nw);
+      for (size_t k = 0; k < nw; k++) {  // FP32 -> BF16, round to nearest even
+        uint32_t u;
+        std::memcpy(&u, &w[k], 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        wb[k] = (uint16_t)(u >> 16);
+      }
+      if ((rc = dalloc(&G.w[l], nw))) return bail(rc);
+      HIPCHK(hipMemcpy(G.w[l], wb.data(), nw * 2, hipMemcpyHostToDevice));
+    }
+  }
+  if (off != bytes) return bail(fail(FS_E_INVALID_ARG, "%zu trailing bytes after the last layer", bytes - off));
+  for (int a = 0; a < 14; a++) {
+    const GuessLayer& L = kGuessLayers[a];
+    size_t m = (size_t)(s->nx >> L.level);
+    if ((rc = dalloc(&G.act[a], m * m * L.cout))) return bail(rc);
+  }
+  G.nblk = 1024;
+  if ((rc = dalloc(&G.scales, 2))) return bail(rc);
+  if ((rc = dalloc(&G.ppart, (size_t)G.nblk))) return bail(rc);
+  G.loaded = true;
   return FS_OK;
 }
 

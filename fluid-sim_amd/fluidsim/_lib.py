@@ -23,7 +23,7 @@ PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GUESS = {"zero": 0, "previous": 1, "learned": 2}
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
-                 "gradient": 7, "residual": 8, "rb": 9}
+                 "gradient": 7, "residual": 8, "rb": 9, "guess": 10}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6}
 
 
@@ -126,6 +126,7 @@ _SIGNATURES = {
     "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),
     "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
     "fs_tile_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
+    "fs_load_guess": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
     "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
 }

@@ -124,6 +124,15 @@ class GPUFluidSolver:
     def setTolerance(self, tol: float):
         check(self._L.fs_set_tolerance(self.handle, float(tol)))
 
+    def loadGuess(self, blob_or_path):
+        """Weights of the learned pressure guess (bytes, or a path to a .fsgn file)."""
+        if isinstance(blob_or_path, (str, bytes)) and not isinstance(blob_or_path, bytes):
+            with open(blob_or_path, "rb") as f:
+                blob = f.read()
+        else:
+            blob = bytes(blob_or_path)
+        check(self._L.fs_load_guess(self.handle, blob, len(blob)))
+
     def setGuess(self, guess: str):
         if guess not in _lib.GUESS:
             raise ValueError(f"unknown guess mode {guess}")

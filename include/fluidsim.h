@@ -170,6 +170,15 @@ int fs_set_tolerance(void* solver, double tolerance);
 int fs_set_nu(void* solver, double nu);
 int fs_set_guess(void* solver, int guess);
 
+/* learned initial pressure guess (new: the reference zeroes p every step,
+ * src/gpu/solver.js:669-672). Loads the weights of the 4-level U-Net that maps the
+ * residual of the previous pressure and the previous pressure to an increment;
+ * blob: "FSGN", uint32 1, uint32 15, then per layer uint32 cin, cout, k, FP32
+ * weights [cout][k*k][cin], FP32 bias[cout]. Needs the multigrid solver with FP32
+ * storage and a square grid whose size is a multiple of 128. Then
+ * fs_set_guess(solver, FS_GUESS_LEARNED). */
+int fs_load_guess(void* solver, const void* blob, size_t bytes);
+
 /* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's
  * seeded band-forcing semantics; NULL disables */
 int fs_set_forcing(void* solver, const fs_forcing* forcing);
@@ -193,7 +202,9 @@ int fs_run_kernel(void* solver, int kernel, double dt);
  * it, src/gpu/solver.js:202-210). Kernel classes: */
 enum {
   FS_PK_ADVECT_SL = 0, FS_PK_ADVECT_MC = 1, FS_PK_DIFFUSE = 2, FS_PK_DOWN0 = 3, FS_PK_DOWN = 4,
-  FS_PK_COARSE = 5, FS_PK_UP = 6, FS_PK_GRADIENT = 7, FS_PK_RESIDUAL = 8, FS_PK_RB = 9, FS_PK_COUNT = 10
+  FS_PK_COARSE = 5, FS_PK_UP = 6, FS_PK_GRADIENT = 7, FS_PK_RESIDUAL = 8, FS_PK_RB = 9,
+  FS_PK_GUESS = 10,  /* 3x3 layers of the learned guess */
+  FS_PK_COUNT = 11
 };
 /* Time every launch of the classes in `mask` (bit k = class k; 0 = off) with HIP
  * events recorded at the dispatch's own start and end on the solver's stream, and

@@ -527,3 +527,59 @@ def test_bf16_storage_meets_the_tolerance_gate(fs, O):
     for _ in range(5):
         s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
     assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+
+
+# ------------------------------------------------------------ learned guess
+
+def _random_guess_blob(seed):
+    import torch
+    from fluidsim.guess_net import build_model, export_blob
+    torch.manual_seed(seed)
+    return export_blob(build_model())
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_learned_guess_network_matches_torch(fs, O, seed):
+    # the HIP U-Net (BF16 MFMA convolutions, FP32 accumulation) against the same
+    # network in float64 torch with the library's BF16 roundings of weights and
+    # activations; zero iterations leave the guess itself in the pressure
+    from fluidsim import initial
+    from fluidsim.guess_net import reference_guess
+    n = 256
+    blob = _random_guess_blob(seed)
+    u0, v0 = initial.decaying_turbulence(n, seed=3)
+    with solver(fs, n=n, pressureSolver="multigrid", pressureIterations=0, mgMinSize=4) as g:
+        g.uploadVelocity(u0, v0)
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.runKernel("project", 0.0)
+        p = g.readField("pressure")
+    r = O.divergence(n, n, f32(u0), f32(v0))
+    ref = reference_guess(blob, r, np.zeros(n * n), n, TWO_PI / n)
+    assert np.max(np.abs(ref)) > 0
+    err = rel(p, ref)
+    assert err < 3e-2, err
+
+
+def test_learned_guess_keeps_the_gate(fs, O):
+    # whatever the weights, the refinement after the guess meets the FP64 gate and the
+    # velocity tracks the exact projection
+    from fluidsim import initial
+    n = 256
+    blob = _random_guess_blob(5)
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-6, mgMinSize=4) as g:
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.uploadVelocity(u0, v0)
+        for _ in range(4):
+            g.stepCFL(0.5)
+            assert g.stats()["last_residual"] <= 1e-6
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for _ in range(4):
+        s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5

@@ -538,3 +538,35 @@ def test_bf16_refinement_still_converges_to_the_gate(O):
     assert r32[k32] <= 1e-6 and rbf[k] <= 1e-6 and k <= k32 + 1
     # each BF16 step down to the gate still contracts by about 10x or more (FP32: 13-100x)
     assert np.all(rbf[1:k + 1] / rbf[:k] < 0.12)
+
+
+# --------------------------------------------------- learned-guess weights format
+
+def test_guess_blob_round_trip_and_reference_forward():
+    import torch
+    from fluidsim.guess_net import build_model, export_blob, load_blob, model_from_blob, LAYERS
+    torch.manual_seed(0)
+    m = build_model()
+    blob = export_blob(m)
+    layers = load_blob(blob)
+    assert len(layers) == 15 and layers[-1][0].shape == (1, 16, 1, 1)
+    for (ci, co), (w, b) in zip(LAYERS, layers):
+        assert w.shape == (co, ci, 3, 3) and b.shape == (co,)
+    m2 = model_from_blob(blob)
+    x = torch.randn(1, 2, 64, 64)
+    with torch.no_grad():
+        assert torch.allclose(m(x), m2(x), atol=1e-6)
+    # the network is periodic: a circular shift of the input shifts the output
+    with torch.no_grad():
+        y = m(x)
+        ys = m(torch.roll(x, (8, 16), (2, 3)))
+    assert torch.allclose(torch.roll(y, (8, 16), (2, 3)), ys, atol=1e-5)
+
+
+def test_bf16_round_matches_the_oracle_rounding(O):
+    from fluidsim.guess_net import bf16_round
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(500) * 10.0 ** rng.integers(-5, 5, 500)).astype(np.float32)
+    ours = bf16_round(x).astype(np.float64)
+    ref = np.array([O.round_storage(float(v), "bf16") for v in x])
+    assert np.array_equal(ours, ref)
