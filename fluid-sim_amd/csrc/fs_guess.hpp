@@ -17,10 +17,9 @@
 //   out 1x1 16->1, p_guess = p_prev + s_r h^2 out
 //
 // Activations are NHWC BF16 (channels of a pixel contiguous). A 3x3 layer is an
-// implicit GEMM per workgroup: 8 x 16 output pixels, all output channels; the
-// input tile with its one-pixel periodic halo is staged in LDS (pooling and
-// upsampling + concatenation happen in that load), weights are read as MFMA B
-// fragments from L2. Each wavefront owns two 16-pixel rows.
+// implicit GEMM per workgroup over a tile of 16-pixel rows and a block of output
+// channels; the input tile with its one-pixel periodic halo (pooling, upsampling
+// and concatenation happen in that load) and the weight slice are staged in LDS.
 #pragma once
 #include "fs_kernels.hpp"
 
@@ -29,8 +28,7 @@ namespace fsk {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT_H = 8, GT_W = 16, GT_NT = 256;  // output tile and threads
-constexpr int GT_LH = GT_H + 2, GT_LW = GT_W + 2;
+constexpr int GT_W = 16, GT_NT = 256;  // tile width (one MFMA M-block) and threads
 
 enum ConvIn { CONV_DIRECT = 0, CONV_POOL = 1, CONV_UPCAT = 2 };
 
@@ -45,98 +43,158 @@ __device__ __forceinline__ int wrapm(int i, int m) { return i < 0 ? i + m : (i >
 //   CONV_UPCAT:  channels [0, C2) are x2 (C2 channels at m/2, nearest-upsampled),
 //                channels [C2, CIN) are x1 (CIN - C2 channels at m).
 // w: BF16 [COUT][9][CIN] (tap = 3 (dy + 1) + dx + 1); bias FP32 [COUT].
-template <int CIN, int COUT, int MODE, int C2>
+// A workgroup computes TH rows x 16 columns of pixels and NBW blocks of 16 output
+// channels (blockIdx.y selects which); each wavefront owns TH/4 rows (one MFMA
+// M-block each). The workgroup's weight slice and its input tile (with the
+// one-pixel halo) are staged in LDS, so every B fragment feeds TH/4 MFMAs from LDS.
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
 __global__ void __launch_bounds__(GT_NT)
 k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, const uint16_t* __restrict__ w,
           const float* __restrict__ bias, uint16_t* __restrict__ y, int m) {
-  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts are multiples of 16");
-  constexpr int PS = CIN + 4;  // LDS pixel stride (bf16): breaks bank conflicts of the A reads
-  constexpr int CH4 = CIN / 4;
-  constexpr int NB = COUT / 16;
-  __shared__ __align__(16) uint16_t tile[GT_LH * GT_LW * PS];
-  const int tiles_x = m / GT_W;
-  const int tx0 = (blockIdx.x % tiles_x) * GT_W, ty0 = (blockIdx.x / tiles_x) * GT_H;
-  // ---- stage the input tile (with halo) in LDS, 4 channels (8 bytes) per item
-  for (int q = threadIdx.x; q < GT_LH * GT_LW * CH4; q += GT_NT) {
-    int pix = q / CH4, cg = q - pix * CH4;
-    int ly = pix / GT_LW, lx = pix - ly * GT_LW;
+  static_assert(CIN % 16 == 0 && COUT % (16 * NBW) == 0 && TH % 4 == 0, "tile shape");
+  constexpr int PS = CIN + 8;          // LDS pixel stride (bf16): 16-byte aligned, breaks A-read bank conflicts
+  constexpr int WS = 9 * CIN + 4;      // LDS weight row stride (bf16): breaks B-read bank conflicts
+  constexpr int CH8 = CIN / 8;
+  constexpr int LH = TH + 2, LW = GT_W + 2;
+  constexpr int MB = TH / 4;           // M-blocks (rows) per wavefront
+  constexpr int OC = NBW * 16, OCP = OC + 8;  // output channels of the workgroup, staged stride
+  __shared__ __align__(16) uint16_t tile[LH * LW * PS];
+  __shared__ __align__(16) uint16_t wl[NBW * 16 * WS];
+  __shared__ __align__(16) uint16_t otile[TH * GT_W * OCP];
+  const int tiles_x = m / GT_W, ntiles = tiles_x * (m / TH);
+  const int co0 = blockIdx.y * NBW * 16;
+  // ---- weights of this workgroup's output channels, once for all its tiles
+  {
+    constexpr int RW8 = 9 * CIN / 8, WI = NBW * 16 * RW8;  // 16-byte items per row, in total
+    auto wsrc = [&](int q) { return (const uint4*)(w + (size_t)(co0 + q / RW8) * 9 * CIN + (q % RW8) * 8); };
+    auto wdst = [&](int q) { return (uint2*)(wl + (q / RW8) * WS + (q % RW8) * 8); };
+    for (int q0 = threadIdx.x; q0 < WI; q0 += 4 * GT_NT) {
+      const int q1 = q0 + GT_NT, q2 = q1 + GT_NT, q3 = q2 + GT_NT;
+      uint4 v0 = *wsrc(q0), v1, v2, v3;
+      if (q1 < WI) v1 = *wsrc(q1);
+      if (q2 < WI) v2 = *wsrc(q2);
+      if (q3 < WI) v3 = *wsrc(q3);
+      // the weight rows are 8-byte aligned in LDS (stride 9 CIN + 4), so two 8-byte stores
+      wdst(q0)[0] = uint2{v0.x, v0.y};
+      wdst(q0)[1] = uint2{v0.z, v0.w};
+      if (q1 < WI) { wdst(q1)[0] = uint2{v1.x, v1.y}; wdst(q1)[1] = uint2{v1.z, v1.w}; }
+      if (q2 < WI) { wdst(q2)[0] = uint2{v2.x, v2.y}; wdst(q2)[1] = uint2{v2.z, v2.w}; }
+      if (q3 < WI) { wdst(q3)[0] = uint2{v3.x, v3.y}; wdst(q3)[1] = uint2{v3.z, v3.w}; }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  float bv[NBW];
+#pragma unroll
+  for (int nb = 0; nb < NBW; nb++) bv[nb] = bias[co0 + nb * 16 + li];
+  // ---- input tiles (with halo), 8 channels (16 bytes) per item; the next tile's
+  // items are loaded into registers while the current one is multiplied
+  constexpr int TI = LH * LW * CH8, TK = (TI + GT_NT - 1) / GT_NT;
+  auto load_item = [&](int q, int tx0, int ty0) -> uint4 {
+    int pix = q / CH8, cg = q - pix * CH8;
+    int ly = pix / LW, lx = pix - ly * LW;
     int gy = wrapm(ty0 - 1 + ly, m), gx = wrapm(tx0 - 1 + lx, m);
-    int c = cg * 4;
-    uint2 v;
-    if (MODE == CONV_DIRECT) {
-      v = *(const uint2*)(x1 + ((size_t)gy * m + gx) * CIN + c);
-    } else if (MODE == CONV_POOL) {
+    int c = cg * 8;
+    if (MODE == CONV_DIRECT) return *(const uint4*)(x1 + ((size_t)gy * m + gx) * CIN + c);
+    if (MODE == CONV_POOL) {
       const int m2 = 2 * m;
-      float a[4] = {0.f, 0.f, 0.f, 0.f};
+      float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f, a6 = 0.f, a7 = 0.f;
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
         for (int aa = 0; aa < 2; aa++) {
-          uint2 t = *(const uint2*)(x1 + ((size_t)(2 * gy + b) * m2 + (2 * gx + aa)) * CIN + c);
-          a[0] += bf_val(t.x & 0xffff);
-          a[1] += bf_val(t.x >> 16);
-          a[2] += bf_val(t.y & 0xffff);
-          a[3] += bf_val(t.y >> 16);
+          uint4 tt = *(const uint4*)(x1 + ((size_t)(2 * gy + b) * m2 + (2 * gx + aa)) * CIN + c);
+          a0 += bf_val(tt.x & 0xffff);
+          a1 += bf_val(tt.x >> 16);
+          a2 += bf_val(tt.y & 0xffff);
+          a3 += bf_val(tt.y >> 16);
+          a4 += bf_val(tt.z & 0xffff);
+          a5 += bf_val(tt.z >> 16);
+          a6 += bf_val(tt.w & 0xffff);
+          a7 += bf_val(tt.w >> 16);
         }
-      v.x = (uint32_t)bf_bits(0.25f * a[0]) | ((uint32_t)bf_bits(0.25f * a[1]) << 16);
-      v.y = (uint32_t)bf_bits(0.25f * a[2]) | ((uint32_t)bf_bits(0.25f * a[3]) << 16);
-    } else {
-      if (c < C2) {
-        const int mh = m >> 1;
-        v = *(const uint2*)(x2 + ((size_t)(gy >> 1) * mh + (gx >> 1)) * C2 + c);
-      } else {
-        constexpr int C1 = CIN - C2;
-        v = *(const uint2*)(x1 + ((size_t)gy * m + gx) * C1 + (c - C2));
+      auto pk = [](float lo, float hi) {
+        return (uint32_t)bf_bits(0.25f * lo) | ((uint32_t)bf_bits(0.25f * hi) << 16);
+      };
+      return uint4{pk(a0, a1), pk(a2, a3), pk(a4, a5), pk(a6, a7)};
+    }
+    if (c < C2) return *(const uint4*)(x2 + ((size_t)(gy >> 1) * (m >> 1) + (gx >> 1)) * C2 + c);
+    return *(const uint4*)(x1 + ((size_t)gy * m + gx) * (CIN - C2) + (c - C2));
+  };
+  uint4 pre[TK];
+  auto prefetch = [&](int t) {
+    const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * TH;
+#pragma unroll
+    for (int k = 0; k < TK; k++) {
+      int q = threadIdx.x + k * GT_NT;
+      if (q < TI) pre[k] = load_item(q, tx0, ty0);
+    }
+  };
+  if (blockIdx.x < ntiles) prefetch(blockIdx.x);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * TH;
+#pragma unroll
+    for (int k = 0; k < TK; k++) {
+      int q = threadIdx.x + k * GT_NT;
+      if (q < TI) {
+        int pix = q / CH8, cg = q - pix * CH8;
+        *(uint4*)(tile + pix * PS + cg * 8) = pre[k];
       }
     }
-    *(uint2*)(tile + pix * PS + c) = v;
-  }
-  __syncthreads();
-  // ---- implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input channels of one tap
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  f32x4 acc[2][NB];
+    __syncthreads();
+    if (t + (int)gridDim.x < ntiles) prefetch(t + gridDim.x);
+    // ---- implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input
+    // channels of one tap. A fragment of LDS row lr (shifted by dx) serves output rows
+    // lr, lr-1, lr-2 through taps dy = 0, 1, 2, so each is read once per (dx, kc).
+    f32x4 acc[MB][NBW];
 #pragma unroll
-  for (int r = 0; r < 2; r++)
+    for (int r = 0; r < MB; r++)
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int tap = 0; tap < 9; tap++) {
-    const int dy = tap / 3, dx = tap % 3;  // LDS offsets (halo included)
+      for (int nb = 0; nb < NBW; nb++) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kc = 0; kc < CIN / 16; kc++) {
-      bf16x4 bfrag[NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; nb++) {
-        const uint16_t* wp = w + ((size_t)(nb * 16 + li) * 9 + tap) * CIN + kc * 16 + lk;
-        bfrag[nb] = *(const bf16x4*)wp;
-      }
+      for (int dx = 0; dx < 3; dx++) {
+        bf16x4 bfrag[3][NBW];
 #pragma unroll
-      for (int r = 0; r < 2; r++) {
-        int row = 2 * wave + r;
-        const uint16_t* ap = tile + ((row + dy) * GT_LW + (li + dx)) * PS + kc * 16 + lk;
-        bf16x4 afrag = *(const bf16x4*)ap;
+        for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-          acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afrag, bfrag[nb], acc[r][nb], 0, 0, 0);
-      }
-    }
-  }
-  // ---- epilogue: lane holds D[pixel 4*(lane/16) + q][channel lane%16] of each block
+          for (int nb = 0; nb < NBW; nb++)
+            bfrag[dy][nb] = *(const bf16x4*)(wl + (nb * 16 + li) * WS + (dy * 3 + dx) * CIN + kc * 16 + lk);
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
-    int py = ty0 + 2 * wave + r;
+        for (int lr = 0; lr < MB + 2; lr++) {
+          bf16x4 afrag = *(const bf16x4*)(tile + ((wave * MB + lr) * LW + (li + dx)) * PS + kc * 16 + lk);
 #pragma unroll
-    for (int nb = 0; nb < NB; nb++) {
-      int co = nb * 16 + li;
-      float b = bias[co];
+          for (int dy = 0; dy < 3; dy++) {
+            const int r = lr - dy;
+            if (r >= 0 && r < MB) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int px = tx0 + lk + q;
-        float val = fmaxf(acc[r][nb][q] + b, 0.f);
-        y[((size_t)py * m + px) * COUT + co] = bf_bits(val);
+              for (int nb = 0; nb < NBW; nb++)
+                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afrag, bfrag[dy][nb], acc[r][nb], 0, 0, 0);
+            }
+          }
+        }
       }
     }
+    // ---- epilogue: lane holds D[pixel 4*(lane/16) + q][channel lane%16] of each block;
+    // staged through LDS so that each pixel's channels leave as 16-byte stores
+#pragma unroll
+    for (int r = 0; r < MB; r++) {
+      int row = wave * MB + r;
+#pragma unroll
+      for (int nb = 0; nb < NBW; nb++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          otile[(row * GT_W + lk + q) * OCP + nb * 16 + li] = bf_bits(fmaxf(acc[r][nb][q] + bv[nb], 0.f));
+    }
+    __syncthreads();
+    constexpr int OI = TH * GT_W * (OC / 8);
+    for (int q = threadIdx.x; q < OI; q += GT_NT) {
+      int pix = q / (OC / 8), cg = q - pix * (OC / 8);
+      int py = ty0 + pix / GT_W, px = tx0 + (pix % GT_W);
+      *(uint4*)(y + ((size_t)py * m + px) * COUT + co0 + cg * 8) = *(const uint4*)(otile + pix * OCP + cg * 8);
+    }
+    __syncthreads();  // the next tile overwrites the staged input
   }
 }
 

@@ -743,17 +743,26 @@ static const GuessLayer kGuessLayers[15] = {
 static const int kGuessIn[15][2] = {{-1, -1}, {0, -1}, {1, -1}, {2, -1}, {3, -1}, {4, -1}, {5, -1}, {6, -1},
                                     {5, 7},   {8, -1}, {3, 9},  {10, -1}, {1, 11}, {12, -1}, {13, -1}};
 
-template <int CIN, int COUT, int MODE, int C2>
+// Tile rows and output-channel blocks per workgroup, by level: large tiles where
+// the level is big, output channels split across workgroups where it is small, so
+// every level launches at least one workgroup per CU and the LDS holds the weights.
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
 int launch_conv(fs_solver_t* s, int layer) {
   auto& G = s->guess_net;
   const int m = s->nx >> kGuessLayers[layer].level;
   const uint16_t* x1 = G.act[kGuessIn[layer][0]];
   const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? G.act[kGuessIn[layer][1]] : nullptr;
-  dim3 g((unsigned)((m / fsk::GT_W) * (m / fsk::GT_H)));
+  // persistent workgroups: each stages its weight slice once and loops over tiles;
+  // as many per CU as the LDS holds (at most 8), over all channel blocks
+  const int ntiles = (m / fsk::GT_W) * (m / TH), nyb = COUT / (16 * NBW);
+  const int lds = 2 * ((TH + 2) * (fsk::GT_W + 2) * (CIN + 8) + NBW * 16 * (9 * CIN + 4) +
+                       TH * fsk::GT_W * (NBW * 16 + 8));
+  const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
+  dim3 g((unsigned)std::max(1, std::min(ntiles, 256 * per_cu / nyb)), (unsigned)nyb);
   // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and out, BF16
   double in_ch = MODE == fsk::CONV_POOL ? 4.0 * CIN : MODE == fsk::CONV_UPCAT ? C2 / 4.0 + (CIN - C2) : CIN;
-  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2>, g, dim3(fsk::GT_NT),
-                x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m);
+  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW>, g,
+                dim3(fsk::GT_NT), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m);
 }
 
 // Evaluate the network on r (level-0 residual of p_prev, FP32) and p_prev = p64[pi],
@@ -770,19 +779,21 @@ int run_guess(fs_solver_t* s, int pi) {
   fsk::k_guess_first<<<dim3((unsigned)(n / 64), (unsigned)(n / 4)), 256, 0, s->stream>>>(
       s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0], n);
   LAUNCHCHK();
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0>(s, 1))) return rc;
-  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0>(s, 2))) return rc;
-  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0>(s, 3))) return rc;
-  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0>(s, 4))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 5))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0>(s, 6))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 7))) return rc;
-  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64>(s, 8))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0>(s, 9))) return rc;
-  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64>(s, 10))) return rc;
-  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0>(s, 11))) return rc;
-  if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32>(s, 12))) return rc;
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0>(s, 13))) return rc;
+  // tile rows and channel blocks per layer: deep tiles where the weights are small,
+  // shallow tiles and split channels where input + weights would crowd the LDS
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 16, 1>(s, 1))) return rc;
+  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0, 16, 2>(s, 2))) return rc;
+  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 3))) return rc;
+  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0, 8, 2>(s, 4))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 2>(s, 5))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0, 8, 1>(s, 6))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 1>(s, 7))) return rc;
+  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64, 8, 2>(s, 8))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 2>(s, 9))) return rc;
+  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64, 16, 2>(s, 10))) return rc;
+  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 11))) return rc;
+  if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32, 16, 1>(s, 12))) return rc;
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 16, 1>(s, 13))) return rc;
   unsigned blocks = (unsigned)((s->n + 255) / 256);
   fsk::k_guess_out<<<blocks, 256, 0, s->stream>>>(G.act[13], G.wout, G.b[14], G.scales, s->p64[pi], n,
                                                   s->dx * s->dx);
