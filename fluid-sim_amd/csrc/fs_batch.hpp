@@ -1,0 +1,500 @@
+// fs_batch.hpp -- batched lid-driven cavities: many independent unit-box
+// simulations of different power-of-two sizes stepped together, one launch per
+// stage over all of them.
+//
+// Walls (no counterpart in the reference, whose domain is periodic): the MAC layout
+// of the periodic solver with u[j n + 0] and v[0 n + i] as wall faces that stay 0
+// and the right/top wall faces implied 0; no slip on every wall, the top wall (lid)
+// moving with velocity U. Beyond a wall, ghosts put the wall value on the wall:
+// u below the bottom -u, above the lid 2U - u; v beyond a side wall -v. Departure
+// points are clamped into the box. Pressure: zero normal derivative at the walls.
+//
+// Layout: the simulations are ordered by size, largest first. The velocity and
+// pressure of all of them live in one arena (simulation k at cell offset off_k);
+// since every size is a multiple of 16 x 16, a 256-thread block never straddles two
+// simulations and finds its own by a binary search over block offsets. The
+// multigrid levels of one size s (all unit boxes, so the same spacing 1/s) form one
+// contiguous class holding the simulations 0..count_s-1 that are at least that
+// large, so a level operator is one flat launch over a class.
+#pragma once
+#include "fs_kernels.hpp"
+
+namespace fsk {
+
+struct CavSim {
+  int n, lg;            // cells per side, log2(n)
+  long off;             // first cell in the field arena
+  long lvl;             // first cell of its finest multigrid level in the level buffers
+  float lid, nu;        // lid velocity, viscosity
+  int disk;             // 1: a pinned disk
+  float dx_, dy_, dr;   // disk centre and radius
+  double dt;            // this step's dt (written on the device)
+};
+
+__device__ __forceinline__ int sim_of_block(const int* __restrict__ blk0, int nsim, int b) {
+  int lo = 0, hi = nsim - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct CavCell {
+  int k, n, i, j;
+  long c;  // cell index in the field arena
+};
+
+__device__ __forceinline__ CavCell cav_cell(const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim) {
+  CavCell r;
+  r.k = sim_of_block(blk0, nsim, blockIdx.x);
+  const CavSim& S = sims[r.k];
+  int local = (blockIdx.x - blk0[r.k]) * 256 + threadIdx.x;
+  r.n = S.n;
+  r.j = local >> S.lg;
+  r.i = local & (S.n - 1);
+  r.c = S.off + local;
+  return r;
+}
+
+// wall-aware face values (ghosts beyond the walls)
+__device__ __forceinline__ float cu_at(const float* __restrict__ u, int n, float U, int i, int j) {
+  if (i <= 0 || i >= n) return 0.f;
+  if (j < 0) return -u[i];
+  if (j >= n) return 2.f * U - u[(size_t)(n - 1) * n + i];
+  return u[(size_t)j * n + i];
+}
+__device__ __forceinline__ float cv_at(const float* __restrict__ v, int n, int i, int j) {
+  if (j <= 0 || j >= n) return 0.f;
+  if (i < 0) return -v[(size_t)j * n];
+  if (i >= n) return -v[(size_t)j * n + n - 1];
+  return v[(size_t)j * n + i];
+}
+
+// bilinear sample at (x, y) in the unit box of component comp (0: u, 1: v); lo/hi get
+// the range of the four source values
+template <int COMP>
+__device__ __forceinline__ float cav_sample(const float* __restrict__ a, int n, float U, float x, float y, float* lo,
+                                            float* hi) {
+  float fn = (float)n;
+  x = fminf(fmaxf(x, 0.f), 1.f);
+  y = fminf(fmaxf(y, 0.f), 1.f);
+  float gx = COMP == 0 ? x * fn : x * fn - 0.5f;
+  float gy = COMP == 0 ? y * fn - 0.5f : y * fn;
+  int i0 = (int)floorf(gx), j0 = (int)floorf(gy);
+  i0 = min(i0, n - 1);
+  j0 = min(j0, n - 1);
+  float fx = gx - (float)i0, fy = gy - (float)j0;
+  float s00, s10, s01, s11;
+  if (COMP == 0) {
+    s00 = cu_at(a, n, U, i0, j0);
+    s10 = cu_at(a, n, U, i0 + 1, j0);
+    s01 = cu_at(a, n, U, i0, j0 + 1);
+    s11 = cu_at(a, n, U, i0 + 1, j0 + 1);
+  } else {
+    s00 = cv_at(a, n, i0, j0);
+    s10 = cv_at(a, n, i0 + 1, j0);
+    s01 = cv_at(a, n, i0, j0 + 1);
+    s11 = cv_at(a, n, i0 + 1, j0 + 1);
+  }
+  if (lo) {
+    *lo = fminf(fminf(s00, s10), fminf(s01, s11));
+    *hi = fmaxf(fmaxf(s00, s10), fmaxf(s01, s11));
+  }
+  float bot = s00 + (s10 - s00) * fx, top = s01 + (s11 - s01) * fx;
+  return bot + (top - bot) * fy;
+}
+
+__device__ __forceinline__ void cav_trace(const float* __restrict__ u, const float* __restrict__ v, int n, float U,
+                                          float x, float y, float dt, float* xo, float* yo) {
+  float u1 = cav_sample<0>(u, n, U, x, y, nullptr, nullptr), v1 = cav_sample<1>(v, n, 0.f, x, y, nullptr, nullptr);
+  float xm = x - 0.5f * dt * u1, ym = y - 0.5f * dt * v1;
+  float u2 = cav_sample<0>(u, n, U, xm, ym, nullptr, nullptr), v2 = cav_sample<1>(v, n, 0.f, xm, ym, nullptr, nullptr);
+  *xo = x - dt * u2;
+  *yo = y - dt * v2;
+}
+
+// Semi-Lagrangian forward pass of the interior faces (the MacCormack predictor).
+__global__ void __launch_bounds__(256)
+k_cav_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
+                float* __restrict__ ov, const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim) {
+  CavCell q = cav_cell(sims, blk0, nsim);
+  const CavSim& S = sims[q.k];
+  const float* us = u + S.off;
+  const float* vs = v + S.off;
+  float h = 1.f / (float)q.n, dt = (float)S.dt, U = S.lid;
+  float xd, yd, a = 0.f, b = 0.f;
+  if (q.i > 0) {
+    cav_trace(us, vs, q.n, U, q.i * h, (q.j + 0.5f) * h, dt, &xd, &yd);
+    a = cav_sample<0>(us, q.n, U, xd, yd, nullptr, nullptr);
+  }
+  if (q.j > 0) {
+    cav_trace(us, vs, q.n, U, (q.i + 0.5f) * h, q.j * h, dt, &xd, &yd);
+    b = cav_sample<1>(vs, q.n, 0.f, xd, yd, nullptr, nullptr);
+  }
+  ou[q.c] = a;
+  ov[q.c] = b;
+}
+
+// MacCormack corrector: back-trace of the forward state through the original velocity,
+// fwd + (u - back) / 2, clamped to the range of the forward trace's four sources.
+__global__ void __launch_bounds__(256)
+k_cav_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ fu,
+                const float* __restrict__ fv, float* __restrict__ ou, float* __restrict__ ov,
+                const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim) {
+  CavCell q = cav_cell(sims, blk0, nsim);
+  const CavSim& S = sims[q.k];
+  const float* us = u + S.off;
+  const float* vs = v + S.off;
+  float h = 1.f / (float)q.n, dt = (float)S.dt, U = S.lid;
+  float xd, yd, lo, hi, a = 0.f, b = 0.f;
+  if (q.i > 0) {
+    float x = q.i * h, y = (q.j + 0.5f) * h;
+    cav_trace(us, vs, q.n, U, x, y, dt, &xd, &yd);
+    cav_sample<0>(us, q.n, U, xd, yd, &lo, &hi);
+    cav_trace(us, vs, q.n, U, x, y, -dt, &xd, &yd);
+    float back = cav_sample<0>(fu + S.off, q.n, U, xd, yd, nullptr, nullptr);
+    float c = fu[q.c] + 0.5f * (us[(size_t)q.j * q.n + q.i] - back);
+    a = fminf(fmaxf(c, lo), hi);
+  }
+  if (q.j > 0) {
+    float x = (q.i + 0.5f) * h, y = q.j * h;
+    cav_trace(us, vs, q.n, U, x, y, dt, &xd, &yd);
+    cav_sample<1>(vs, q.n, 0.f, xd, yd, &lo, &hi);
+    cav_trace(us, vs, q.n, U, x, y, -dt, &xd, &yd);
+    float back = cav_sample<1>(fv + S.off, q.n, 0.f, xd, yd, nullptr, nullptr);
+    float c = fv[q.c] + 0.5f * (vs[(size_t)q.j * q.n + q.i] - back);
+    b = fminf(fmaxf(c, lo), hi);
+  }
+  ou[q.c] = a;
+  ov[q.c] = b;
+}
+
+// Explicit diffusion of the interior faces with the wall ghosts, then the pinned
+// disk's direct forcing u += chi (0 - u), `passes` times (src/cpu/solid.js:165-233
+// with plain separations; the wall faces are never forced).
+__global__ void __launch_bounds__(256)
+k_cav_diffuse_solid(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
+                    float* __restrict__ ov, const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim,
+                    float mask_width, int passes) {
+  CavCell q = cav_cell(sims, blk0, nsim);
+  const CavSim& S = sims[q.k];
+  const float* us = u + S.off;
+  const float* vs = v + S.off;
+  int n = q.n, i = q.i, j = q.j;
+  float h = 1.f / (float)n;
+  float a = S.nu > 0.f ? (float)((double)S.nu * S.dt * (double)n * (double)n) : 0.f;
+  float U = S.lid;
+  float un = 0.f, vn = 0.f;
+  if (i > 0) {
+    float c = cu_at(us, n, U, i, j);
+    un = c + a * (cu_at(us, n, U, i - 1, j) + cu_at(us, n, U, i + 1, j) + cu_at(us, n, U, i, j - 1) +
+                  cu_at(us, n, U, i, j + 1) - 4.f * c);
+  }
+  if (j > 0) {
+    float c = cv_at(vs, n, i, j);
+    vn = c + a * (cv_at(vs, n, i - 1, j) + cv_at(vs, n, i + 1, j) + cv_at(vs, n, i, j - 1) + cv_at(vs, n, i, j + 1) -
+                  4.f * c);
+  }
+  if (S.disk) {
+    float w = mask_width * h;
+    if (i > 0) {
+      float dx = i * h - S.dx_, dy = (j + 0.5f) * h - S.dy_;
+      float t = (sqrtf(dx * dx + dy * dy) - S.dr) / w;
+      float chi = t > 8.f ? 0.f : 0.5f * (1.f - tanhf(t));
+      if (chi >= 1e-6f)
+        for (int p = 0; p < passes; p++) un += chi * (0.f - un);
+    }
+    if (j > 0) {
+      float dx = (i + 0.5f) * h - S.dx_, dy = j * h - S.dy_;
+      float t = (sqrtf(dx * dx + dy * dy) - S.dr) / w;
+      float chi = t > 8.f ? 0.f : 0.5f * (1.f - tanhf(t));
+      if (chi >= 1e-6f)
+        for (int p = 0; p < passes; p++) vn += chi * (0.f - vn);
+    }
+  }
+  ou[q.c] = un;
+  ov[q.c] = vn;
+}
+
+// FP64 refinement residual r = div u - L_N (p + e) of every simulation; p' = p + e is
+// stored, r goes (FP32) to the simulation's finest multigrid rhs, block max |r| to part.
+__global__ void __launch_bounds__(256)
+k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
+                 const float* __restrict__ xe, double* __restrict__ pout, float* __restrict__ rhs,
+                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
+  CavCell q = cav_cell(sims, blk0, nsim);
+  const CavSim& S = sims[q.k];
+  int n = q.n, i = q.i, j = q.j;
+  long c = q.c, cl = S.lvl + ((long)j << S.lg) + i;
+  auto pv = [&](long cc, long lv) -> double {
+    double a = p ? p[cc] : 0.0;
+    if (xe) a += (double)xe[lv];
+    return a;
+  };
+  double pc = pv(c, cl), s = 0.0;
+  if (i > 0) s += pv(c - 1, cl - 1) - pc;
+  if (i < n - 1) s += pv(c + 1, cl + 1) - pc;
+  if (j > 0) s += pv(c - n, cl - n) - pc;
+  if (j < n - 1) s += pv(c + n, cl + n) - pc;
+  double dn = (double)n;
+  double lap = s * dn * dn;
+  const float* us = u + S.off;
+  const float* vs = v + S.off;
+  double div = ((double)cu_at(us, n, 0.f, i + 1, j) - (double)cu_at(us, n, 0.f, i, j)) * dn +
+               ((double)cv_at(vs, n, i, j + 1) - (double)cv_at(vs, n, i, j)) * dn;
+  double r = div - lap;
+  pout[c] = pc;
+  rhs[cl] = (float)r;
+  double m = block_max(nan_to_inf(fabs(r)));
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// u -= grad p on the interior faces; block max |u|, |v| for the next step's dt.
+__global__ void __launch_bounds__(256)
+k_cav_gradient64(float* __restrict__ u, float* __restrict__ v, const double* __restrict__ p,
+                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
+  CavCell q = cav_cell(sims, blk0, nsim);
+  int n = q.n;
+  double dn = (double)n;
+  long c = q.c;
+  float uu = 0.f, vv = 0.f;
+  if (q.i > 0) {
+    uu = (float)((double)u[c] - (p[c] - p[c - 1]) * dn);
+    u[c] = uu;
+  }
+  if (q.j > 0) {
+    vv = (float)((double)v[c] - (p[c] - p[c - n]) * dn);
+    v[c] = vv;
+  }
+  double a = block_max(nan_to_inf(fabs((double)uu)));
+  double b = block_max(nan_to_inf(fabs((double)vv)));
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// dt of every simulation (one workgroup each): cfl h / (max(max|u|, |U|) + max|v|),
+// from the block maxima of the last gradient pass; NaN if a velocity is not finite.
+__global__ void __launch_bounds__(256)
+k_cav_dt(const double* __restrict__ part, CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim,
+         double cfl, double dt_max) {
+  int k = blockIdx.x;
+  int b0 = blk0[k], b1 = k + 1 < nsim ? blk0[k + 1] : b0 + ((sims[k].n * sims[k].n) >> 8);
+  double mu = 0.0, mv = 0.0;
+  for (int b = b0 + threadIdx.x; b < b1; b += 256) {
+    mu = fmax(mu, part[2 * b]);
+    mv = fmax(mv, part[2 * b + 1]);
+  }
+  mu = block_max(mu);
+  mv = block_max(mv);
+  if (threadIdx.x == 0) {
+    double U = fabs((double)sims[k].lid);
+    double rate = (fmax(mu, U) + mv) * sims[k].n;
+    double dt = rate > 0 ? cfl / rate : dt_max;
+    if (isinf(mu) || isinf(mv)) dt = __longlong_as_double(0x7ff8000000000000LL);
+    sims[k].dt = fmin(dt, dt_max);
+  }
+}
+
+// ------------------------------------------------ multigrid, zero-gradient walls
+// One size class: `count` grids of s x s cells (spacing 1/s), one after another.
+
+// Red-black colour pass: x = (sum of the neighbours inside the box - h^2 rhs) / their count.
+__global__ void __launch_bounds__(256)
+k_nm_rb(float* __restrict__ x, const float* __restrict__ rhs, int lg, long count, float h2, int colour, int x_zero) {
+  long q = (long)blockIdx.x * 256 + threadIdx.x;
+  const int s = 1 << lg, hs = s >> 1;
+  if (q >= count * s * hs) return;
+  long g = q >> (2 * lg - 1);
+  int rem = (int)(q - (g << (2 * lg - 1)));
+  int j = rem / hs, m = rem - j * hs;
+  int i = 2 * m + ((j + colour) & 1);
+  float* xg = x + (g << (2 * lg));
+  long c = (long)j * s + i;
+  float sum = 0.f;
+  int k = 0;
+  if (i > 0) { sum += x_zero ? 0.f : xg[c - 1]; k++; }
+  if (i < s - 1) { sum += x_zero ? 0.f : xg[c + 1]; k++; }
+  if (j > 0) { sum += x_zero ? 0.f : xg[c - s]; k++; }
+  if (j < s - 1) { sum += x_zero ? 0.f : xg[c + s]; k++; }
+  xg[c] = (sum - h2 * rhs[(g << (2 * lg)) + c]) / (float)k;
+}
+
+// The black half of a from-zero first sweep reads red values only; when x_zero is set
+// for the red pass, the black cells hold stale values that are never read.
+
+// Residual of the four children of each coarse cell, averaged (restriction).
+__global__ void __launch_bounds__(256)
+k_nm_restrict(const float* __restrict__ x, const float* __restrict__ rhs, int lg, long count, float ih2,
+              float* __restrict__ crhs) {
+  long q = (long)blockIdx.x * 256 + threadIdx.x;
+  const int s = 1 << lg, cs = s >> 1, clg = lg - 1;
+  if (q >= count * cs * cs) return;
+  long g = q >> (2 * clg);
+  int rem = (int)(q - (g << (2 * clg)));
+  int J = rem >> clg, I = rem & (cs - 1);
+  const float* xg = x + (g << (2 * lg));
+  const float* rg = rhs + (g << (2 * lg));
+  float acc = 0.f;
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      int i = 2 * I + a, j = 2 * J + b;
+      long c = (long)j * s + i;
+      float xc = xg[c], sum = 0.f;
+      if (i > 0) sum += xg[c - 1] - xc;
+      if (i < s - 1) sum += xg[c + 1] - xc;
+      if (j > 0) sum += xg[c - s] - xc;
+      if (j < s - 1) sum += xg[c + s] - xc;
+      acc += rg[c] - sum * ih2;
+    }
+  crhs[q] = acc * 0.25f;
+}
+
+// x += bilinear cell-centred prolongation of the coarse correction, the coarse
+// indices clamped into the box (constant extrapolation, zero gradient at the walls).
+__global__ void __launch_bounds__(256)
+k_nm_prolong(const float* __restrict__ xc, float* __restrict__ x, int lg, long count) {
+  long q = (long)blockIdx.x * 256 + threadIdx.x;
+  const int s = 1 << lg, cs = s >> 1, clg = lg - 1;
+  if (q >= count * s * s) return;
+  long g = q >> (2 * lg);
+  int rem = (int)(q - (g << (2 * lg)));
+  int j = rem >> lg, i = rem & (s - 1);
+  int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1, j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+  float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+  int x0 = max(i0, 0), x1 = min(i0 + 1, cs - 1), y0 = max(j0, 0), y1 = min(j0 + 1, cs - 1);
+  const float* cg = xc + (g << (2 * clg));
+  float a = cg[y0 * cs + x0], b = cg[y0 * cs + x1], c = cg[y1 * cs + x0], d = cg[y1 * cs + x1];
+  float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+  x[q] += top * (1.f - fy) + bot * fy;
+}
+
+// The levels from s0 <= 64 down to 4x4 of one simulation per workgroup, in LDS:
+// pre-smoothing from zero, restriction, the coarsest 4x4 solved with the
+// pseudo-inverse of its Neumann Laplacian (after removing the rhs mean), then
+// prolongation and post-smoothing back up. rhs_base/x_base: the simulation's
+// level-s0 grid in the level buffers.
+constexpr int NMC_CELLS = 4096 + 1024 + 256 + 64 + 16;
+__global__ void __launch_bounds__(1024)
+k_nm_coarse(float* __restrict__ xall, const float* __restrict__ rall, const long* __restrict__ base, int s0,
+            int pre, int post, const float* __restrict__ pinv) {
+  __shared__ float bx[NMC_CELLS];
+  __shared__ float br[NMC_CELLS];
+  __shared__ float sp[256];
+  __shared__ double red[16];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const long b = base[k];
+  int lg0 = 0;
+  while ((1 << lg0) < s0) lg0++;
+  const int L = lg0 - 1;  // levels s0 .. 4
+  int off[8];
+  {
+    int o = 0;
+    for (int l = 0; l < L; l++) {
+      off[l] = o;
+      int s = s0 >> l;
+      o += s * s;
+    }
+  }
+  for (int q = t; q < s0 * s0; q += 1024) br[q] = rall[b + q];
+  for (int q = t; q < 256; q += 1024) sp[q] = pinv[q];
+  __syncthreads();
+  auto pass = [&](int l, int colour, bool zero) {
+    int s = s0 >> l, lg = lg0 - l, hs = s >> 1;
+    float h2 = 1.f / (float)(s * s);
+    float* x = bx + off[l];
+    const float* r = br + off[l];
+    for (int q = t; q < s * hs; q += 1024) {
+      int j = q / hs, m = q - j * hs;
+      int i = 2 * m + ((j + colour) & 1);
+      int c = (j << lg) + i;
+      float sum = 0.f;
+      int kk = 0;
+      if (i > 0) { sum += zero ? 0.f : x[c - 1]; kk++; }
+      if (i < s - 1) { sum += zero ? 0.f : x[c + 1]; kk++; }
+      if (j > 0) { sum += zero ? 0.f : x[c - s]; kk++; }
+      if (j < s - 1) { sum += zero ? 0.f : x[c + s]; kk++; }
+      x[c] = (sum - h2 * r[c]) / (float)kk;
+    }
+    __syncthreads();
+  };
+  for (int l = 0; l < L - 1; l++) {
+    int s = s0 >> l, cs = s >> 1;
+    for (int sw = 0; sw < pre; sw++) {
+      pass(l, 0, sw == 0);
+      pass(l, 1, false);
+    }
+    if (pre == 0) {
+      for (int q = t; q < s * s; q += 1024) bx[off[l] + q] = 0.f;
+      __syncthreads();
+    }
+    float ih2 = (float)(s * s);
+    const float* x = bx + off[l];
+    const float* r = br + off[l];
+    for (int q = t; q < cs * cs; q += 1024) {
+      int J = q / cs, I = q - J * cs;
+      float acc = 0.f;
+      for (int bb = 0; bb < 2; bb++)
+        for (int a = 0; a < 2; a++) {
+          int i = 2 * I + a, j = 2 * J + bb, c = j * s + i;
+          float xc = x[c], sum = 0.f;
+          if (i > 0) sum += x[c - 1] - xc;
+          if (i < s - 1) sum += x[c + 1] - xc;
+          if (j > 0) sum += x[c - s] - xc;
+          if (j < s - 1) sum += x[c + s] - xc;
+          acc += r[c] - sum * ih2;
+        }
+      br[off[l + 1] + q] = acc * 0.25f;
+    }
+    __syncthreads();
+  }
+  {
+    // coarsest 4x4: remove the rhs mean, x = pinv r (16 x 16, columns broadcast)
+    const int l = L - 1;
+    float* x = bx + off[l];
+    float* r = br + off[l];
+    double sum = t < 16 ? (double)r[t] : 0.0;
+    sum = wave_sum(sum);
+    if ((t & 63) == 0) red[t >> 6] = sum;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < 16; w++) tot += red[w];
+    float mean = (float)(tot / 16.0);
+    __syncthreads();
+    if (t < 16) r[t] -= mean;
+    __syncthreads();
+    if (t < 16) {
+      float acc = 0.f;
+      for (int c = 0; c < 16; c++) acc = fmaf(sp[c * 16 + t], r[c], acc);
+      x[t] = acc;
+    }
+    __syncthreads();
+  }
+  for (int l = L - 2; l >= 0; l--) {
+    int s = s0 >> l, cs = s >> 1, lg = lg0 - l;
+    float* x = bx + off[l];
+    const float* xc = bx + off[l + 1];
+    for (int q = t; q < s * s; q += 1024) {
+      int j = q >> lg, i = q & (s - 1);
+      int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1, j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+      float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+      int x0 = max(i0, 0), x1 = min(i0 + 1, cs - 1), y0 = max(j0, 0), y1 = min(j0 + 1, cs - 1);
+      float a = xc[y0 * cs + x0], bb = xc[y0 * cs + x1], c = xc[y1 * cs + x0], d = xc[y1 * cs + x1];
+      float top = a * (1.f - fx) + bb * fx, bot = c * (1.f - fx) + d * fx;
+      x[q] += top * (1.f - fy) + bot * fy;
+    }
+    __syncthreads();
+    for (int sw = 0; sw < post; sw++) {
+      pass(l, 0, false);
+      pass(l, 1, false);
+    }
+  }
+  for (int q = t; q < s0 * s0; q += 1024) xall[b + q] = bx[q];
+}
+
+}  // namespace fsk

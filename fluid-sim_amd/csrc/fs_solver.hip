@@ -36,6 +36,7 @@
 #include "fs_kernels.hpp"
 #include "fs_mg_fused.hpp"
 #include "fs_guess.hpp"
+#include "fs_batch.hpp"
 
 namespace {
 
@@ -1168,6 +1169,234 @@ int run_diag(fs_solver_t* s, double out[8]) {
 
 // ======================================================================= ABI
 
+
+namespace {
+
+// ------------------------------------------------------------ cavity batches
+
+struct fs_batch_t {
+  fs_batch_config cfg{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int nsim = 0;
+  std::vector<int> ext;      // internal index -> caller's index
+  std::vector<int> intl;     // caller's index -> internal index
+  std::vector<fsk::CavSim> sims_h;
+  fsk::CavSim* sims = nullptr;
+  int* blk0 = nullptr;
+  int nblocks = 0;
+  long cells = 0;
+  int max_lg = 0;
+  float *u[2] = {}, *v[2] = {}, *fu = nullptr, *fv = nullptr;
+  int cur = 0;
+  double* p64[2] = {};
+  int pidx = 0;
+  bool have_p = false;
+  // multigrid size classes s = 2^lg (lg from max_lg down to 2): grids of the first
+  // count[lg] simulations, at base[lg] in the level buffers
+  long base[13] = {}, count[13] = {};
+  float *lx = nullptr, *lr = nullptr;
+  long* cbase = nullptr;     // per simulation: its 64 x 64 level
+  float* pinv4 = nullptr;    // pseudo-inverse of the 4 x 4 Neumann Laplacian (h = 1/4)
+  double *rpart = nullptr, *rpart_host = nullptr;
+  int rp_rows = 0;
+  double* mpart = nullptr;
+  int last_pred = 2;
+  int64_t steps = 0, total_vcycles = 0;
+  int last_vcycles = 0;
+  double last_residual = 0;
+};
+
+int bsync(fs_batch_t* b) {
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return FS_OK;
+}
+
+template <typename K, typename... Args>
+int blaunch(fs_batch_t* b, K kern, dim3 g, dim3 t, Args... args) {
+  kern<<<g, t, 0, b->stream>>>(args...);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+inline unsigned nblk(long items) { return (unsigned)((items + 255) / 256); }
+
+// refinement iteration `it`: residual of p64[pin] (+ the finest corrections),
+// then (unless residual_only) one V-cycle over every size class
+int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_only) {
+  int rc;
+  const int c = b->cur;
+  if ((rc = blaunch(b, fsk::k_cav_residual64, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+                    (const float*)b->v[c], (const double*)(zero_p ? nullptr : b->p64[pin]),
+                    (const float*)(it > 0 ? b->lx : nullptr), b->p64[1 - pin], b->lr, (const fsk::CavSim*)b->sims,
+                    (const int*)b->blk0, b->nsim, b->rpart + (size_t)it * b->nblocks)))
+    return rc;
+  if (residual_only) return FS_OK;
+  const int pre = b->cfg.mg_pre, post = b->cfg.mg_post;
+  for (int lg = b->max_lg; lg > 6; lg--) {  // grid-wide levels above 64 x 64, finest first
+    const long cnt = b->count[lg];
+    const int s = 1 << lg;
+    float* x = b->lx + b->base[lg];
+    const float* r = b->lr + b->base[lg];
+    const float h2 = 1.f / ((float)s * s);
+    const long half = cnt * s * (s / 2);
+    if (pre == 0) {
+      fsk::k_fill_f32<<<nblk(cnt * s * s), 256, 0, b->stream>>>(x, (size_t)cnt * s * s, 0.f);
+      LAUNCHCHK();
+    }
+    for (int sw = 0; sw < pre; sw++)
+      for (int colour = 0; colour < 2; colour++)
+        if ((rc = blaunch(b, fsk::k_nm_rb, dim3(nblk(half)), dim3(256), x, r, lg, cnt, h2, colour,
+                          (sw == 0 && colour == 0) ? 1 : 0)))
+          return rc;
+    if ((rc = blaunch(b, fsk::k_nm_restrict, dim3(nblk(cnt * (s / 2) * (s / 2))), dim3(256), (const float*)x, r, lg,
+                      cnt, (float)((double)s * s), b->lr + b->base[lg - 1])))
+      return rc;
+  }
+  if ((rc = blaunch(b, fsk::k_nm_coarse, dim3(b->nsim), dim3(1024), b->lx, (const float*)b->lr, (const long*)b->cbase,
+                    64, pre, post, (const float*)b->pinv4)))
+    return rc;
+  for (int lg = 7; lg <= b->max_lg; lg++) {
+    const long cnt = b->count[lg];
+    const int s = 1 << lg;
+    float* x = b->lx + b->base[lg];
+    const float* r = b->lr + b->base[lg];
+    const float h2 = 1.f / ((float)s * s);
+    const long half = cnt * s * (s / 2);
+    if ((rc = blaunch(b, fsk::k_nm_prolong, dim3(nblk(cnt * s * s)), dim3(256), (const float*)(b->lx + b->base[lg - 1]),
+                      x, lg, cnt)))
+      return rc;
+    for (int sw = 0; sw < post; sw++)
+      for (int colour = 0; colour < 2; colour++)
+        if ((rc = blaunch(b, fsk::k_nm_rb, dim3(nblk(half)), dim3(256), x, r, lg, cnt, h2, colour, 0))) return rc;
+  }
+  return FS_OK;
+}
+
+int batch_read_rmax(fs_batch_t* b, int rows, std::vector<double>& rmax) {
+  HIPCHK(hipMemcpyAsync(b->rpart_host, b->rpart, sizeof(double) * (size_t)rows * b->nblocks, hipMemcpyDeviceToHost,
+                        b->stream));
+  int rc = bsync(b);
+  if (rc) return rc;
+  rmax.assign(rows, 0.0);
+  for (int k = 0; k < rows; k++) {
+    const double* row = b->rpart_host + (size_t)k * b->nblocks;
+    double m = 0;
+    for (int q = 0; q < b->nblocks; q++) m = std::max(m, row[q]);
+    rmax[k] = m;
+  }
+  return FS_OK;
+}
+
+// Refinement until every simulation's FP64 residual max is within the tolerance:
+// the predicted number of V-cycles, one read-back, then single cycles if needed.
+int batch_project(fs_batch_t* b, int* cycles_out) {
+  const double tol = b->cfg.tolerance;
+  const int maxc = b->cfg.max_vcycles;
+  const bool zero_p = !(b->cfg.guess == FS_GUESS_PREVIOUS && b->have_p);
+  const int a0 = b->pidx;
+  int rc;
+  int c = std::max(0, std::min(b->last_pred, maxc));
+  for (int it = 0; it < c; it++)
+    if ((rc = batch_iteration(b, it, (a0 + it) & 1, it == 0 && zero_p, false))) return rc;
+  if ((rc = batch_iteration(b, c, (a0 + c) & 1, c == 0 && zero_p, true))) return rc;
+  std::vector<double> rmax;
+  for (;;) {
+    if ((rc = batch_read_rmax(b, c + 1, rmax))) return rc;
+    double last = rmax[c];
+    if (!std::isfinite(last)) {
+      b->pidx = (a0 + c + 1) & 1;
+      b->last_residual = last;
+      *cycles_out = c;
+      return fail(FS_E_NONFINITE, "non-finite residual in a cavity projection (V-cycle %d)", c);
+    }
+    if (last <= tol || c >= maxc) {
+      int need = c;
+      for (int k = 0; k <= c; k++)
+        if (rmax[k] <= tol) {
+          need = k;
+          break;
+        }
+      b->last_pred = need;
+      b->pidx = (a0 + c + 1) & 1;
+      b->last_residual = last;
+      *cycles_out = c;
+      if (last > tol)
+        return fail(FS_E_NOT_CONVERGED, "cavity projection residual %.3g > %.3g after %d V-cycles", last, tol, c);
+      return FS_OK;
+    }
+    if ((rc = batch_iteration(b, c, (a0 + c) & 1, c == 0 && zero_p, false))) return rc;
+    c++;
+    if ((rc = batch_iteration(b, c, (a0 + c) & 1, false, true))) return rc;
+  }
+}
+
+void batch_release(fs_batch_t* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->dev);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int k = 0; k < 2; k++) {
+    dfree(b->u[k]);
+    dfree(b->v[k]);
+    dfree(b->p64[k]);
+  }
+  dfree(b->fu);
+  dfree(b->fv);
+  dfree(b->sims);
+  dfree(b->blk0);
+  dfree(b->lx);
+  dfree(b->lr);
+  dfree(b->cbase);
+  dfree(b->pinv4);
+  dfree(b->rpart);
+  hfree(b->rpart_host);
+  dfree(b->mpart);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// pseudo-inverse of the 4 x 4 Laplacian with zero-gradient walls, h = 1/4, transposed:
+// A+ = (A + 11^T/N)^-1 - 11^T/N (null space: the constants)
+std::vector<float> neumann_pinv4() {
+  const int n = 4, N = 16;
+  const double h2 = 1.0 / 16.0;
+  std::vector<double> M((size_t)N * 2 * N, 0.0);
+  auto at = [&](int r, int c) -> double& { return M[(size_t)r * 2 * N + c]; };
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      int r = j * n + i;
+      int nb[4][2] = {{i - 1, j}, {i + 1, j}, {i, j - 1}, {i, j + 1}};
+      for (auto& q : nb)
+        if (q[0] >= 0 && q[0] < n && q[1] >= 0 && q[1] < n) {
+          at(r, q[1] * n + q[0]) += 1.0 / h2;
+          at(r, r) -= 1.0 / h2;
+        }
+      for (int c = 0; c < N; c++) at(r, c) += 1.0 / N;
+      at(r, N + r) = 1.0;
+    }
+  for (int col = 0; col < N; col++) {
+    int piv = col;
+    for (int r = col + 1; r < N; r++)
+      if (std::fabs(at(r, col)) > std::fabs(at(piv, col))) piv = r;
+    if (piv != col)
+      for (int c = 0; c < 2 * N; c++) std::swap(at(piv, c), at(col, c));
+    double d = at(col, col);
+    for (int c = 0; c < 2 * N; c++) at(col, c) /= d;
+    for (int r = 0; r < N; r++)
+      if (r != col) {
+        double f = at(r, col);
+        if (f != 0.0)
+          for (int c = 0; c < 2 * N; c++) at(r, c) -= f * at(col, c);
+      }
+  }
+  std::vector<float> pt((size_t)N * N);
+  for (int r = 0; r < N; r++)
+    for (int c = 0; c < N; c++) pt[(size_t)c * N + r] = (float)(at(r, N + c) - 1.0 / N);
+  return pt;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fs_abi_version(void) { return FS_ABI_VERSION; }
@@ -1812,6 +2041,232 @@ int fs_profile_read(void* h, int kind, double* total_ms, int64_t* launches, doub
   if (total_ms) *total_ms = s->prof.ms[kind];
   if (launches) *launches = s->prof.launches[kind];
   if (bytes) *bytes = s->prof.bytes[kind];
+  return FS_OK;
+}
+
+
+// ------------------------------------------------------------ cavity batches
+
+void fs_batch_config_default(fs_batch_config* c) {
+  if (!c) return;
+  c->advection = FS_ADV_MACCORMACK;
+  c->tolerance = 1e-6;
+  c->max_vcycles = 60;
+  c->mg_pre = 2;
+  c->mg_post = 2;
+  c->coupling_passes = 2;
+  c->mask_width = 1.5;
+  c->guess = FS_GUESS_ZERO;
+  c->device = 0;
+}
+
+int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count, void** out) {
+  if (!cfg || !sims || !out) return fail(FS_E_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  const fs_batch_config& c = *cfg;
+  if (count < 1) return fail(FS_E_INVALID_ARG, "a batch needs at least one cavity");
+  if (c.advection != FS_ADV_SEMI_LAGRANGIAN && c.advection != FS_ADV_MACCORMACK)
+    return fail(FS_E_INVALID_ARG, "unknown advection scheme %d", c.advection);
+  if (!(c.tolerance > 0)) return fail(FS_E_INVALID_ARG, "tolerance must be > 0");
+  if (c.max_vcycles < 1 || c.max_vcycles > 1000) return fail(FS_E_INVALID_ARG, "max_vcycles must be in 1..1000");
+  if (c.mg_pre < 0 || c.mg_pre > 8 || c.mg_post < 0 || c.mg_post > 8)
+    return fail(FS_E_INVALID_ARG, "mg_pre and mg_post must be in 0..8");
+  if (c.coupling_passes < 1 || c.coupling_passes > 16) return fail(FS_E_INVALID_ARG, "coupling_passes must be in 1..16");
+  if (!(c.mask_width > 0)) return fail(FS_E_INVALID_ARG, "mask_width must be > 0");
+  if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
+    return fail(FS_E_INVALID_ARG, "a batch takes the zero or the previous pressure guess");
+  for (int k = 0; k < count; k++) {
+    const fs_cavity& q = sims[k];
+    if (q.n < 64 || q.n > 4096 || (q.n & (q.n - 1)))
+      return fail(FS_E_INVALID_ARG, "cavity %d: n must be a power of two in 64..4096, got %d", k, q.n);
+    if (!std::isfinite(q.lid) || !(q.nu >= 0) || !std::isfinite(q.nu))
+      return fail(FS_E_INVALID_ARG, "cavity %d: lid velocity must be finite and nu >= 0", k);
+    if (q.has_disk && !(q.disk_r > 0 && std::isfinite(q.disk_x) && std::isfinite(q.disk_y)))
+      return fail(FS_E_INVALID_ARG, "cavity %d: disk radius must be > 0 and its centre finite", k);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FS_E_DEVICE, "no HIP device");
+  if (c.device < 0 || c.device >= ndev) return fail(FS_E_INVALID_ARG, "device %d of %d", c.device, ndev);
+  fs_batch_t* b = new fs_batch_t();
+  b->cfg = c;
+  b->dev = c.device;
+  auto bail = [&](int rc) {
+    batch_release(b);
+    return rc;
+  };
+  HIPCHK(hipSetDevice(b->dev));
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(FS_E_DEVICE, "hipStreamCreate failed"));
+  b->nsim = count;
+  b->ext.resize(count);
+  for (int k = 0; k < count; k++) b->ext[k] = k;
+  std::stable_sort(b->ext.begin(), b->ext.end(), [&](int a, int bb) { return sims[a].n > sims[bb].n; });
+  b->intl.resize(count);
+  for (int k = 0; k < count; k++) b->intl[b->ext[k]] = k;
+  int lg_max = 0;
+  while ((1 << lg_max) < sims[b->ext[0]].n) lg_max++;
+  b->max_lg = lg_max;
+  // size classes, largest first
+  long lvl_cells = 0;
+  for (int lg = lg_max; lg >= 2; lg--) {
+    long cnt = 0;
+    for (int k = 0; k < count; k++)
+      if (sims[b->ext[k]].n >= (1 << lg)) cnt++;
+    b->count[lg] = cnt;
+    b->base[lg] = lvl_cells;
+    lvl_cells += cnt << (2 * lg);
+  }
+  b->sims_h.resize(count);
+  std::vector<int> blk0(count);
+  std::vector<long> cbase(count);
+  long off = 0;
+  for (int k = 0; k < count; k++) {
+    const fs_cavity& q = sims[b->ext[k]];
+    fsk::CavSim& S = b->sims_h[k];
+    S.n = q.n;
+    S.lg = 0;
+    while ((1 << S.lg) < q.n) S.lg++;
+    S.off = off;
+    S.lvl = b->base[S.lg] + ((long)k << (2 * S.lg));
+    S.lid = (float)q.lid;
+    S.nu = (float)q.nu;
+    S.disk = q.has_disk ? 1 : 0;
+    S.dx_ = (float)q.disk_x;
+    S.dy_ = (float)q.disk_y;
+    S.dr = (float)q.disk_r;
+    S.dt = 0;
+    blk0[k] = (int)(off / 256);
+    cbase[k] = b->base[6] + ((long)k << 12);
+    off += (long)q.n * q.n;
+  }
+  b->cells = off;
+  b->nblocks = (int)(off / 256);
+  int rc = 0;
+  for (int k = 0; k < 2 && !rc; k++) {
+    rc = dalloc(&b->u[k], (size_t)off);
+    if (!rc) rc = dalloc(&b->v[k], (size_t)off);
+    if (!rc) rc = dalloc(&b->p64[k], (size_t)off);
+  }
+  if (!rc) rc = dalloc(&b->fu, (size_t)off);
+  if (!rc) rc = dalloc(&b->fv, (size_t)off);
+  if (!rc) rc = dalloc(&b->sims, (size_t)count);
+  if (!rc) rc = dalloc(&b->blk0, (size_t)count);
+  if (!rc) rc = dalloc(&b->lx, (size_t)lvl_cells);
+  if (!rc) rc = dalloc(&b->lr, (size_t)lvl_cells);
+  if (!rc) rc = dalloc(&b->cbase, (size_t)count);
+  if (!rc) rc = dalloc(&b->pinv4, 256);
+  b->rp_rows = c.max_vcycles + 2;
+  if (!rc) rc = dalloc(&b->rpart, (size_t)b->rp_rows * b->nblocks);
+  if (!rc) rc = halloc(&b->rpart_host, (size_t)b->rp_rows * b->nblocks);
+  if (!rc) rc = dalloc(&b->mpart, (size_t)2 * b->nblocks);
+  if (rc) return bail(rc);
+  HIPCHK(hipMemcpy(b->sims, b->sims_h.data(), sizeof(fsk::CavSim) * count, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->blk0, blk0.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->cbase, cbase.data(), sizeof(long) * count, hipMemcpyHostToDevice));
+  std::vector<float> pinv = neumann_pinv4();
+  HIPCHK(hipMemcpy(b->pinv4, pinv.data(), sizeof(float) * 256, hipMemcpyHostToDevice));
+  // state at rest
+  for (float* a : {b->u[0], b->v[0], b->u[1], b->v[1], b->fu, b->fv}) {
+    fsk::k_fill_f32<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.f);
+  }
+  for (double* a : {b->p64[0], b->p64[1]}) fsk::k_fill_f64<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.0);
+  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lx, (size_t)lvl_cells, 0.f);
+  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lr, (size_t)lvl_cells, 0.f);
+  fsk::k_fill_f64<<<nblk(2L * b->nblocks), 256, 0, b->stream>>>(b->mpart, (size_t)2 * b->nblocks, 0.0);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+    return bail(fail(FS_E_DEVICE, "initial fill failed"));
+  *out = b;
+  return FS_OK;
+}
+
+int fs_batch_destroy(void* h) {
+  batch_release((fs_batch_t*)h);
+  return FS_OK;
+}
+
+int fs_batch_step_cfl(void* h, double cfl) {
+  if (!h) return fail(FS_E_INVALID_ARG, "batch handle is NULL");
+  fs_batch_t* b = (fs_batch_t*)h;
+  if (!(cfl > 0) || !std::isfinite(cfl)) return fail(FS_E_INVALID_ARG, "cfl must be finite and > 0");
+  HIPCHK(hipSetDevice(b->dev));
+  int rc;
+  const fsk::CavSim* S = b->sims;
+  if ((rc = blaunch(b, fsk::k_cav_dt, dim3(b->nsim), dim3(256), (const double*)b->mpart, b->sims, (const int*)b->blk0,
+                    b->nsim, cfl, 1.0)))
+    return rc;
+  int c = b->cur;
+  if (b->cfg.advection == FS_ADV_MACCORMACK) {
+    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+                      (const float*)b->v[c], b->fu, b->fv, S, (const int*)b->blk0, b->nsim)))
+      return rc;
+    if ((rc = blaunch(b, fsk::k_cav_advect_mc, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+                      (const float*)b->v[c], (const float*)b->fu, (const float*)b->fv, b->u[1 - c], b->v[1 - c], S,
+                      (const int*)b->blk0, b->nsim)))
+      return rc;
+  } else {
+    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+                      (const float*)b->v[c], b->u[1 - c], b->v[1 - c], S, (const int*)b->blk0, b->nsim)))
+      return rc;
+  }
+  c = 1 - c;
+  if ((rc = blaunch(b, fsk::k_cav_diffuse_solid, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+                    (const float*)b->v[c], b->u[1 - c], b->v[1 - c], S, (const int*)b->blk0, b->nsim,
+                    (float)b->cfg.mask_width, b->cfg.coupling_passes)))
+    return rc;
+  b->cur = 1 - c;
+  int cycles = 0;
+  int prc = batch_project(b, &cycles);
+  if (prc && prc != FS_E_NOT_CONVERGED) return prc;
+  if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3(b->nblocks), dim3(256), b->u[b->cur], b->v[b->cur],
+                    (const double*)b->p64[b->pidx], S, (const int*)b->blk0, b->nsim, b->mpart)))
+    return rc;
+  b->have_p = true;
+  b->last_vcycles = cycles;
+  b->total_vcycles += cycles;
+  b->steps++;
+  return prc;
+}
+
+int fs_batch_sync(void* h) {
+  if (!h) return fail(FS_E_INVALID_ARG, "batch handle is NULL");
+  fs_batch_t* b = (fs_batch_t*)h;
+  HIPCHK(hipSetDevice(b->dev));
+  return bsync(b);
+}
+
+int fs_batch_read_velocity(void* h, int sim, float* u, float* v, size_t count) {
+  if (!h || !u || !v) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_batch_t* b = (fs_batch_t*)h;
+  if (sim < 0 || sim >= b->nsim) return fail(FS_E_INVALID_ARG, "cavity %d of %d", sim, b->nsim);
+  const fsk::CavSim& S = b->sims_h[b->intl[sim]];
+  if (count != (size_t)S.n * S.n) return fail(FS_E_INVALID_ARG, "cavity %d has %d cells, got %zu", sim, S.n * S.n, count);
+  HIPCHK(hipSetDevice(b->dev));
+  HIPCHK(hipMemcpyAsync(u, b->u[b->cur] + S.off, sizeof(float) * count, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(v, b->v[b->cur] + S.off, sizeof(float) * count, hipMemcpyDeviceToHost, b->stream));
+  return bsync(b);
+}
+
+int fs_batch_read_dt(void* h, double* dt, int count) {
+  if (!h || !dt) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_batch_t* b = (fs_batch_t*)h;
+  if (count != b->nsim) return fail(FS_E_INVALID_ARG, "the batch has %d cavities, got %d", b->nsim, count);
+  HIPCHK(hipSetDevice(b->dev));
+  std::vector<fsk::CavSim> t(b->nsim);
+  HIPCHK(hipMemcpyAsync(t.data(), b->sims, sizeof(fsk::CavSim) * b->nsim, hipMemcpyDeviceToHost, b->stream));
+  int rc = bsync(b);
+  if (rc) return rc;
+  for (int k = 0; k < b->nsim; k++) dt[b->ext[k]] = t[k].dt;
+  return FS_OK;
+}
+
+int fs_batch_get_stats(void* h, fs_batch_stats* out) {
+  if (!h || !out) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_batch_t* b = (fs_batch_t*)h;
+  out->steps = b->steps;
+  out->cells = b->cells;
+  out->last_vcycles = b->last_vcycles;
+  out->total_vcycles = b->total_vcycles;
+  out->last_residual = b->last_residual;
   return FS_OK;
 }
 

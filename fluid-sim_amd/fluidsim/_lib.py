@@ -86,6 +86,22 @@ class FsDisk(C.Structure):
     ]
 
 
+class FsCavity(C.Structure):
+    _fields_ = [("n", C.c_int), ("lid", C.c_double), ("nu", C.c_double), ("has_disk", C.c_int),
+                ("disk_x", C.c_double), ("disk_y", C.c_double), ("disk_r", C.c_double)]
+
+
+class FsBatchConfig(C.Structure):
+    _fields_ = [("advection", C.c_int), ("tolerance", C.c_double), ("max_vcycles", C.c_int),
+                ("mg_pre", C.c_int), ("mg_post", C.c_int), ("coupling_passes", C.c_int),
+                ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int)]
+
+
+class FsBatchStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("cells", C.c_int64), ("last_vcycles", C.c_int),
+                ("total_vcycles", C.c_int64), ("last_residual", C.c_double)]
+
+
 class FluidSimError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"{message} (status {code})")
@@ -127,6 +143,14 @@ _SIGNATURES = {
     "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
     "fs_tile_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
     "fs_load_guess": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
+    "fs_batch_config_default": (None, [C.POINTER(FsBatchConfig)]),
+    "fs_batch_create": (C.c_int, [C.POINTER(FsBatchConfig), C.POINTER(FsCavity), C.c_int, C.POINTER(_VP)]),
+    "fs_batch_destroy": (C.c_int, [_VP]),
+    "fs_batch_step_cfl": (C.c_int, [_VP, C.c_double]),
+    "fs_batch_sync": (C.c_int, [_VP]),
+    "fs_batch_read_velocity": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_size_t]),
+    "fs_batch_read_dt": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_int]),
+    "fs_batch_get_stats": (C.c_int, [_VP, C.POINTER(FsBatchStats)]),
     "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
 }

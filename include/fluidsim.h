@@ -128,6 +128,35 @@ typedef struct fs_disk {
   int passes;
 } fs_disk;
 
+/* one lid-driven cavity of a batch (new: the reference's domain is periodic and has
+ * no walls). Unit box, n x n cells (a power of two, 64..4096), no slip on every wall,
+ * the top wall moving with velocity `lid`, viscosity nu, optionally a pinned disk. */
+typedef struct fs_cavity {
+  int n;
+  double lid, nu;
+  int has_disk;
+  double disk_x, disk_y, disk_r;
+} fs_cavity;
+
+typedef struct fs_batch_config {
+  int advection;              /* FS_ADV_*; default MacCormack */
+  double tolerance;           /* FP64 residual gate for every cavity; default 1e-6 */
+  int max_vcycles;            /* cap per step; default 60 */
+  int mg_pre, mg_post;        /* red-black sweeps; default 2, 2 */
+  int coupling_passes;        /* disk forcing passes; default 2 (the CPU default) */
+  double mask_width;          /* disk mask width in cells; default 1.5 */
+  int guess;                  /* FS_GUESS_ZERO (default) or FS_GUESS_PREVIOUS */
+  int device;
+} fs_batch_config;
+
+typedef struct fs_batch_stats {
+  int64_t steps;
+  int64_t cells;              /* cells over all cavities */
+  int last_vcycles;           /* V-cycles of the last projection (every cavity runs them) */
+  int64_t total_vcycles;
+  double last_residual;       /* max over cavities of the FP64 residual */
+} fs_batch_stats;
+
 int         fs_abi_version(void);
 const char* fs_last_error(void);
 
@@ -212,6 +241,18 @@ enum {
 int fs_profile_enable(void* solver, unsigned mask);
 /* the V-cycle tile shape chosen for each grid-wide level and the autotuner's timings, as text */
 int fs_tile_report(void* solver, char* buf, size_t len);
+
+/* batched cavities: every stage of a step is one launch over all of them; each
+ * cavity advances with its own dt = cfl h / (max(max|u|, |lid|) + max|v|). Cavity
+ * indices are the caller's order. */
+void fs_batch_config_default(fs_batch_config* cfg);
+int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* cavities, int count, void** batch);
+int fs_batch_destroy(void* batch);
+int fs_batch_step_cfl(void* batch, double cfl);
+int fs_batch_sync(void* batch);
+int fs_batch_read_velocity(void* batch, int cavity, float* u, float* v, size_t count);
+int fs_batch_read_dt(void* batch, double* dt, int count);   /* dt of the last step, per cavity */
+int fs_batch_get_stats(void* batch, fs_batch_stats* out);
 int fs_profile_read(void* solver, int kind, double* total_ms, int64_t* launches, double* bytes);
 
 #ifdef __cplusplus

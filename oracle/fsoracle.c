@@ -699,6 +699,272 @@ int or_mg_vcycles(int n, double lx, const double *rhs, double *p, int cycles,
   return L;
 }
 
+/* ---------------------------------------------------------- lid-driven cavity */
+/* Unit box [0,1]^2, n x n cells, h = 1/n, the periodic MAC layout: u[j n + i] on the
+ * left face of cell (i, j), v[j n + i] on its bottom face. u[j n + 0] and v[i] (row 0)
+ * are wall faces and stay 0; the right and top wall faces are not stored (0). No slip
+ * on every wall, the top wall (lid) moving with velocity U. Beyond a wall, ghosts put
+ * the wall value on the wall: u below the bottom -u, above the lid 2U - u; v beyond a
+ * side wall -v. Departure points are clamped into the box. (The reference has no
+ * walls; this is the walled counterpart of the periodic operators above.) */
+
+static inline double cav_u(const double *u, int n, double U, int i, int j) {
+  if (i <= 0 || i >= n) return 0.0;
+  if (j < 0) return -u[i];
+  if (j >= n) return 2.0 * U - u[(size_t)(n - 1) * n + i];
+  return u[(size_t)j * n + i];
+}
+static inline double cav_v(const double *v, int n, int i, int j) {
+  if (j <= 0 || j >= n) return 0.0;
+  if (i < 0) return -v[(size_t)j * n];
+  if (i >= n) return -v[(size_t)j * n + n - 1];
+  return v[(size_t)j * n + i];
+}
+
+/* bilinear sample of u (or v) at (x, y); lo/hi receive the four source values' range */
+static double cav_sample(const double *a, int n, double U, int comp, double x, double y, double *lo, double *hi) {
+  double h = 1.0 / n;
+  x = x < 0 ? 0 : (x > 1 ? 1 : x);
+  y = y < 0 ? 0 : (y > 1 ? 1 : y);
+  double gx = comp == 0 ? x / h : x / h - 0.5;
+  double gy = comp == 0 ? y / h - 0.5 : y / h;
+  int i0 = (int)floor(gx), j0 = (int)floor(gy);
+  if (i0 > n - 1) i0 = n - 1;
+  if (j0 > n - 1) j0 = n - 1;
+  double fx = gx - i0, fy = gy - j0;
+  double s00, s10, s01, s11;
+  if (comp == 0) {
+    s00 = cav_u(a, n, U, i0, j0); s10 = cav_u(a, n, U, i0 + 1, j0);
+    s01 = cav_u(a, n, U, i0, j0 + 1); s11 = cav_u(a, n, U, i0 + 1, j0 + 1);
+  } else {
+    s00 = cav_v(a, n, i0, j0); s10 = cav_v(a, n, i0 + 1, j0);
+    s01 = cav_v(a, n, i0, j0 + 1); s11 = cav_v(a, n, i0 + 1, j0 + 1);
+  }
+  if (lo) {
+    *lo = fmin(fmin(s00, s10), fmin(s01, s11));
+    *hi = fmax(fmax(s00, s10), fmax(s01, s11));
+  }
+  double bot = s00 + (s10 - s00) * fx, top = s01 + (s11 - s01) * fx;
+  return bot + (top - bot) * fy;
+}
+
+double or_cav_sample_u(int n, double U, const double *u, double x, double y) {
+  return cav_sample(u, n, U, 0, x, y, NULL, NULL);
+}
+double or_cav_sample_v(int n, const double *v, double x, double y) {
+  return cav_sample(v, n, 0, 1, x, y, NULL, NULL);
+}
+
+/* RK2 midpoint back-trace through (u, v) (src/cpu/advect.js:46-54 with walls) */
+static void cav_trace(const double *u, const double *v, int n, double U, double x, double y, double dt,
+                      double *xo, double *yo) {
+  double u1 = cav_sample(u, n, U, 0, x, y, NULL, NULL), v1 = cav_sample(v, n, 0, 1, x, y, NULL, NULL);
+  double xm = x - 0.5 * dt * u1, ym = y - 0.5 * dt * v1;
+  double u2 = cav_sample(u, n, U, 0, xm, ym, NULL, NULL), v2 = cav_sample(v, n, 0, 1, xm, ym, NULL, NULL);
+  *xo = x - dt * u2;
+  *yo = y - dt * v2;
+}
+
+/* semi-Lagrangian or limited MacCormack self-advection of the interior faces
+ * (src/cpu/advect.js:74-111, :149-206 with walls); wall faces stay 0 */
+void or_cav_advect(int n, double U, int scheme, const double *u, const double *v, double *ou, double *ov,
+                   double dt) {
+  double h = 1.0 / n;
+  size_t N = (size_t)n * n;
+  double *fu = (double *)calloc(N, sizeof(double)), *fv = (double *)calloc(N, sizeof(double));
+  double *lou = (double *)calloc(N, sizeof(double)), *hiu = (double *)calloc(N, sizeof(double));
+  double *lov = (double *)calloc(N, sizeof(double)), *hiv = (double *)calloc(N, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      size_t k = (size_t)j * n + i;
+      double xd, yd;
+      if (i > 0) {
+        cav_trace(u, v, n, U, i * h, (j + 0.5) * h, dt, &xd, &yd);
+        fu[k] = cav_sample(u, n, U, 0, xd, yd, &lou[k], &hiu[k]);
+      }
+      if (j > 0) {
+        cav_trace(u, v, n, U, (i + 0.5) * h, j * h, dt, &xd, &yd);
+        fv[k] = cav_sample(v, n, 0, 1, xd, yd, &lov[k], &hiv[k]);
+      }
+    }
+  if (scheme == OR_ADV_SL) {
+    memcpy(ou, fu, sizeof(double) * N);
+    memcpy(ov, fv, sizeof(double) * N);
+  } else {
+    /* back-trace of the forward state through the original velocity */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        size_t k = (size_t)j * n + i;
+        double xd, yd;
+        ou[k] = 0;
+        ov[k] = 0;
+        if (i > 0) {
+          cav_trace(u, v, n, U, i * h, (j + 0.5) * h, -dt, &xd, &yd);
+          double back = cav_sample(fu, n, U, 0, xd, yd, NULL, NULL);
+          double c = fu[k] + 0.5 * (u[k] - back);
+          ou[k] = c < lou[k] ? lou[k] : (c > hiu[k] ? hiu[k] : c);
+        }
+        if (j > 0) {
+          cav_trace(u, v, n, U, (i + 0.5) * h, j * h, -dt, &xd, &yd);
+          double back = cav_sample(fv, n, 0, 1, xd, yd, NULL, NULL);
+          double c = fv[k] + 0.5 * (v[k] - back);
+          ov[k] = c < lov[k] ? lov[k] : (c > hiv[k] ? hiv[k] : c);
+        }
+      }
+  }
+  free(fu); free(fv); free(lou); free(hiu); free(lov); free(hiv);
+}
+
+/* explicit diffusion of the interior faces with the wall ghosts (src/cpu/solver.js:52-69) */
+void or_cav_diffuse(int n, double U, const double *u, const double *v, double *ou, double *ov, double nu,
+                    double dt) {
+  double h = 1.0 / n, a = nu * dt / (h * h);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      size_t k = (size_t)j * n + i;
+      double uc = cav_u(u, n, U, i, j), vc = cav_v(v, n, i, j);
+      ou[k] = i > 0 ? uc + a * (cav_u(u, n, U, i - 1, j) + cav_u(u, n, U, i + 1, j) + cav_u(u, n, U, i, j - 1) +
+                                cav_u(u, n, U, i, j + 1) - 4.0 * uc)
+                    : 0.0;
+      ov[k] = j > 0 ? vc + a * (cav_v(v, n, i - 1, j) + cav_v(v, n, i + 1, j) + cav_v(v, n, i, j - 1) +
+                                cav_v(v, n, i, j + 1) - 4.0 * vc)
+                    : 0.0;
+    }
+}
+
+void or_cav_divergence(int n, const double *u, const double *v, double *div) {
+  double h = 1.0 / n;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++)
+      div[(size_t)j * n + i] = (cav_u(u, n, 0, i + 1, j) - cav_u(u, n, 0, i, j)) / h +
+                               (cav_v(v, n, i, j + 1) - cav_v(v, n, i, j)) / h;
+}
+
+/* Laplacian with zero normal derivative at the walls (mirror ghosts) */
+void or_cav_laplacian(int n, const double *p, double *out) {
+  double h2 = 1.0 / ((double)n * n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      size_t k = (size_t)j * n + i;
+      double s = 0;
+      if (i > 0) s += p[k - 1] - p[k];
+      if (i < n - 1) s += p[k + 1] - p[k];
+      if (j > 0) s += p[k - n] - p[k];
+      if (j < n - 1) s += p[k + n] - p[k];
+      out[k] = s / h2;
+    }
+}
+
+/* subtract grad p on the interior faces */
+void or_cav_subtract_gradient(int n, double *u, double *v, const double *p) {
+  double h = 1.0 / n;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      size_t k = (size_t)j * n + i;
+      if (i > 0) u[k] -= (p[k] - p[k - 1]) / h;
+      if (j > 0) v[k] -= (p[k] - p[k - n]) / h;
+    }
+}
+
+/* exact Neumann Poisson solve by the cosine eigenbasis phi_k(i) = cos(pi k (i + 1/2) / n)
+ * (eigenvalues -(4/h^2) sin^2(pi k / 2n) per axis); the rhs mean (k = l = 0) is dropped */
+int or_cav_poisson_exact(int n, const double *rhs, double *p) {
+  double h = 1.0 / n;
+  double *C = (double *)malloc(sizeof(double) * n * n);   /* C[k n + i] = phi_k(i) */
+  double *T = (double *)malloc(sizeof(double) * n * n), *A = (double *)malloc(sizeof(double) * n * n);
+  double *lam = (double *)malloc(sizeof(double) * n);
+  for (int k = 0; k < n; k++) {
+    lam[k] = -4.0 / (h * h) * pow(sin(M_PI * k / (2.0 * n)), 2);
+    for (int i = 0; i < n; i++) C[(size_t)k * n + i] = cos(M_PI * k * (i + 0.5) / n);
+  }
+  /* A[l][k] = sum_j sum_i rhs[j][i] C[k][i] C[l][j] / norms */
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int k = 0; k < n; k++) {
+      double s = 0;
+      for (int i = 0; i < n; i++) s += rhs[(size_t)j * n + i] * C[(size_t)k * n + i];
+      T[(size_t)j * n + k] = s;
+    }
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < n; l++)
+    for (int k = 0; k < n; k++) {
+      double s = 0;
+      for (int j = 0; j < n; j++) s += T[(size_t)j * n + k] * C[(size_t)l * n + j];
+      double nk = k == 0 ? n : n / 2.0, nl = l == 0 ? n : n / 2.0;
+      double den = lam[k] + lam[l];
+      A[(size_t)l * n + k] = (k == 0 && l == 0) ? 0.0 : s / (nk * nl) / den;
+    }
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < n; l++)
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int k = 0; k < n; k++) s += A[(size_t)l * n + k] * C[(size_t)k * n + i];
+      T[(size_t)l * n + i] = s;
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int l = 0; l < n; l++) s += T[(size_t)l * n + i] * C[(size_t)l * n + j];
+      p[(size_t)j * n + i] = s;
+    }
+  free(C); free(T); free(A); free(lam);
+  return 0;
+}
+
+/* disk coupling with plain separations, wall faces untouched */
+void or_cav_apply_coupling(int n, double *u, double *v, const or_disk *s, double mask_width, int passes,
+                           double *imp) {
+  double h = 1.0 / n, dA = h * h, width = mask_width * h;
+  double fx = 0, fy = 0, tq = 0;
+  for (int pass = 0; pass < passes; pass++) {
+    for (int j = 0; j < n; j++)
+      for (int i = 1; i < n; i++) {
+        double px = i * h, py = (j + 0.5) * h, dx = px - s->x, dy = py - s->y;
+        double chi = 0.5 * (1.0 - tanh((hypot(dx, dy) - s->r) / width));
+        if (chi < 1e-6) continue;
+        size_t k = (size_t)j * n + i;
+        double du = chi * ((s->vx - s->omega * dy) - u[k]);
+        u[k] += du;
+        fx += du * dA;
+        tq += -dy * du * dA;
+      }
+    for (int j = 1; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        double px = (i + 0.5) * h, py = j * h, dx = px - s->x, dy = py - s->y;
+        double chi = 0.5 * (1.0 - tanh((hypot(dx, dy) - s->r) / width));
+        if (chi < 1e-6) continue;
+        size_t k = (size_t)j * n + i;
+        double dv = chi * ((s->vy + s->omega * dx) - v[k]);
+        v[k] += dv;
+        fy += dv * dA;
+        tq += dx * dv * dA;
+      }
+  }
+  if (imp) { imp[0] = fx; imp[1] = fy; imp[2] = tq; }
+}
+
+/* one cavity step: advect, diffuse, couple the disk, project exactly */
+int or_cav_step(int n, double U, double nu, int scheme, const or_disk *disk, double mask_width, int passes,
+                double dt, double *u, double *v, double *p_out) {
+  size_t N = (size_t)n * n;
+  double *tu = (double *)malloc(sizeof(double) * N), *tv = (double *)malloc(sizeof(double) * N);
+  double *div = (double *)malloc(sizeof(double) * N), *p = (double *)malloc(sizeof(double) * N);
+  or_cav_advect(n, U, scheme, u, v, tu, tv, dt);
+  if (nu > 0) or_cav_diffuse(n, U, tu, tv, u, v, nu, dt);
+  else { memcpy(u, tu, sizeof(double) * N); memcpy(v, tv, sizeof(double) * N); }
+  if (disk) or_cav_apply_coupling(n, u, v, disk, mask_width, passes, NULL);
+  or_cav_divergence(n, u, v, div);
+  or_cav_poisson_exact(n, div, p);
+  or_cav_subtract_gradient(n, u, v, p);
+  if (p_out) memcpy(p_out, p, sizeof(double) * N);
+  free(tu); free(tv); free(div); free(p);
+  return 0;
+}
+
 /* -------------------------------------- refinement with stored-precision rounding */
 
 /* value after a store in `storage` (0: FP32, 1: BF16) of the FP32 rounding of x;

@@ -143,6 +143,22 @@ void   or_disk_velocity(const or_disk *s, double px, double py, double lx, doubl
 void   or_apply_coupling(int nx, int ny, double lx, double ly, double *u, double *v, const or_disk *s,
                          double mask_width, int passes, double *imp);
 
+/* lid-driven cavity on the unit box (walls; no reference counterpart) */
+double or_cav_sample_u(int n, double U, const double *u, double x, double y);
+double or_cav_sample_v(int n, const double *v, double x, double y);
+void   or_cav_advect(int n, double U, int scheme, const double *u, const double *v, double *ou, double *ov,
+                     double dt);
+void   or_cav_diffuse(int n, double U, const double *u, const double *v, double *ou, double *ov, double nu,
+                      double dt);
+void   or_cav_divergence(int n, const double *u, const double *v, double *div);
+void   or_cav_laplacian(int n, const double *p, double *out);
+void   or_cav_subtract_gradient(int n, double *u, double *v, const double *p);
+int    or_cav_poisson_exact(int n, const double *rhs, double *p);
+void   or_cav_apply_coupling(int n, double *u, double *v, const or_disk *s, double mask_width, int passes,
+                             double *imp);
+int    or_cav_step(int n, double U, double nu, int scheme, const or_disk *disk, double mask_width, int passes,
+                   double dt, double *u, double *v, double *p_out);
+
 /* one full step (src/cpu/solver.js:336-398); p_out may be NULL */
 int    or_step(const or_step_cfg *cfg, double *u, double *v, double *p_out, double dt, uint32_t force_seed);
 

@@ -111,6 +111,16 @@ def lib():
         "or_step": (i, [C.POINTER(StepCfg), p, p, C.c_void_p, d, C.c_uint32]),
         "or_wrap_delta": (d, [d, d]),
         "or_round_storage": (d, [d, i]),
+        "or_cav_sample_u": (d, [i, d, p, d, d]),
+        "or_cav_sample_v": (d, [i, p, d, d]),
+        "or_cav_advect": (None, [i, d, i, p, p, p, p, d]),
+        "or_cav_diffuse": (None, [i, d, p, p, p, p, d, d]),
+        "or_cav_divergence": (None, [i, p, p, p]),
+        "or_cav_laplacian": (None, [i, p, p]),
+        "or_cav_subtract_gradient": (None, [i, p, p, p]),
+        "or_cav_poisson_exact": (i, [i, p, p]),
+        "or_cav_apply_coupling": (None, [i, p, p, C.POINTER(Disk), d, i, p]),
+        "or_cav_step": (i, [i, d, d, i, C.c_void_p, d, i, d, p, p, C.c_void_p]),
         "or_mg_refine": (i, [i, d, p, p, i, i, i, i, i, i, i, p]),
         "or_disk_mask": (d, [C.POINTER(Disk), d, d, d, d, d]),
         "or_disk_velocity": (None, [C.POINTER(Disk), d, d, d, d, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -335,6 +345,79 @@ def mg_refine(n, div, iterations, min_size=4, pre=2, post=2, coarse=40, coarse_n
     if rc < 1:
         raise ValueError("no multigrid levels")
     return p, rmax
+
+
+# ------------------------------------------------------ lid-driven cavity (walls)
+
+def cav_sample_u(n, U, u, x, y):
+    return lib().or_cav_sample_u(n, U, _f(u), x, y)
+
+
+def cav_sample_v(n, v, x, y):
+    return lib().or_cav_sample_v(n, _f(v), x, y)
+
+
+def cav_advect(scheme, n, U, u, v, dt):
+    ou, ov = np.zeros(n * n), np.zeros(n * n)
+    lib().or_cav_advect(n, U, ADV[scheme], _f(u), _f(v), ou, ov, dt)
+    return ou, ov
+
+
+def cav_diffuse(n, U, u, v, nu, dt):
+    ou, ov = np.zeros(n * n), np.zeros(n * n)
+    lib().or_cav_diffuse(n, U, _f(u), _f(v), ou, ov, nu, dt)
+    return ou, ov
+
+
+def cav_divergence(n, u, v):
+    out = np.zeros(n * n)
+    lib().or_cav_divergence(n, _f(u), _f(v), out)
+    return out
+
+
+def cav_laplacian(n, p):
+    out = np.zeros(n * n)
+    lib().or_cav_laplacian(n, _f(p), out)
+    return out
+
+
+def cav_poisson_exact(n, rhs):
+    p = np.zeros(n * n)
+    lib().or_cav_poisson_exact(n, _f(rhs), p)
+    return p
+
+
+def cav_subtract_gradient(n, u, v, p):
+    u, v = _f(u).copy(), _f(v).copy()
+    lib().or_cav_subtract_gradient(n, u, v, _f(p))
+    return u, v
+
+
+def cav_apply_coupling(n, u, v, d, mask_width=1.5, passes=1):
+    imp = np.zeros(3)
+    lib().or_cav_apply_coupling(n, u, v, C.byref(make_disk(d)), mask_width, passes, imp)
+    return imp
+
+
+class Cavity:
+    """Oracle lid-driven cavity: advect, diffuse, couple a disk, exact Neumann projection."""
+
+    def __init__(self, n, U=1.0, nu=1e-3, advection="maccormack", disk=None, mask_width=1.5, passes=1):
+        self.n, self.U, self.nu = n, float(U), float(nu)
+        self.scheme = ADV[advection]
+        self.disk = make_disk(disk) if disk else None
+        self.mask_width, self.passes = mask_width, passes
+        self.u = np.zeros(n * n)
+        self.v = np.zeros(n * n)
+        self.p = np.zeros(n * n)
+
+    def step(self, dt):
+        lib().or_cav_step(self.n, self.U, self.nu, self.scheme, C.byref(self.disk) if self.disk else None,
+                          self.mask_width, self.passes, dt, self.u, self.v, self.p.ctypes.data_as(C.c_void_p))
+
+    def dt_cfl(self, cfl=0.5):
+        h = 1.0 / self.n
+        return cfl * h / (max(np.max(np.abs(self.u)), abs(self.U)) + np.max(np.abs(self.v)))
 
 
 def _disk_tuple(d):

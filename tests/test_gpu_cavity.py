@@ -1,0 +1,77 @@
+"""Batched lid-driven cavities on the GPU against the f64 oracle's walled step
+(oracle/fsoracle.c or_cav_*: advection with wall ghosts, explicit diffusion,
+pinned-disk forcing, exact cosine-basis Neumann projection)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return math.sqrt(np.sum((a - b) ** 2) / max(np.sum(b * b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def fs():
+    import fluidsim
+    fluidsim.createDevice()
+    return fluidsim
+
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_single_cavity_matches_oracle(fs, O, scheme):
+    # one 64^2 cavity with a pinned disk, 20 steps at the batch's own dt
+    cav = dict(n=64, lid=1.0, nu=2e-3, disk=dict(x=0.55, y=0.45, r=0.12))
+    o = O.Cavity(64, U=1.0, nu=2e-3, advection=scheme, disk=cav["disk"], passes=2)
+    with fs.CavityBatch([cav], advection=scheme, tolerance=1e-9) as b:
+        for _ in range(20):
+            b.stepCFL(0.5)
+            dt = b.lastDt()[0]
+            assert abs(dt - o.dt_cfl(0.5)) <= 2e-5 * dt
+            o.step(dt)
+            assert b.stats()["last_residual"] <= 1e-9
+        out = b.readVelocity(0)
+    err = max(rel(out["u"], o.u), rel(out["v"], o.v))
+    assert err < 1e-4, err
+    uu = out["u"].reshape(64, 64)
+    assert np.all(uu[:, 0] == 0) and np.all(out["v"].reshape(64, 64)[0, :] == 0)
+    d = O.cav_divergence(64, out["u"].astype(np.float64), out["v"].astype(np.float64))
+    assert np.max(np.abs(d)) < 1e-3     # FP32 storage of a projected field
+
+
+def test_batch_members_match_their_own_runs(fs, O):
+    # four cavities of three sizes in one batch, in the caller's order, against the
+    # same cavities stepped alone (each run meets the same FP64 gate)
+    cavs = [dict(n=64, lid=1.0, nu=1e-2, disk=dict(x=0.5, y=0.5, r=0.1)),
+            dict(n=256, lid=0.7, nu=2e-4, disk=None),
+            dict(n=128, lid=1.5, nu=1e-3, disk=dict(x=0.3, y=0.6, r=0.08)),
+            dict(n=64, lid=2.0, nu=5e-3, disk=None)]
+    steps = 6
+    with fs.CavityBatch(cavs, tolerance=1e-8) as b:
+        for _ in range(steps):
+            b.stepCFL(0.5)
+            assert b.stats()["last_residual"] <= 1e-8
+        together = [b.readVelocity(k) for k in range(len(cavs))]
+        st = b.stats()
+    assert st["cells"] == sum(c["n"] ** 2 for c in cavs) and st["steps"] == steps
+    for k, c in enumerate(cavs):
+        with fs.CavityBatch([c], tolerance=1e-8) as s:
+            for _ in range(steps):
+                s.stepCFL(0.5)
+            alone = s.readVelocity(0)
+        assert max(rel(together[k]["u"], alone["u"]), rel(together[k]["v"], alone["v"])) < 1e-5, k
+
+
+def test_cavity_batch_c5_draws(fs):
+    cavs = fs.cavities_c5(256, seed=1)
+    assert len(cavs) == 256 and {c["n"] for c in cavs} == {64, 128, 256, 512}
+    for c in cavs:
+        h = 1.0 / c["n"]
+        d = c["disk"]
+        assert 0.5 <= c["lid"] <= 2 and 100 <= c["lid"] / c["nu"] <= 5000
+        assert 0.05 <= d["r"] <= 0.2
+        assert d["r"] + 2 * h <= d["x"] <= 1 - d["r"] - 2 * h and d["r"] + 2 * h <= d["y"] <= 1 - d["r"] - 2 * h

@@ -570,3 +570,103 @@ def test_bf16_round_matches_the_oracle_rounding(O):
     ours = bf16_round(x).astype(np.float64)
     ref = np.array([O.round_storage(float(v), "bf16") for v in x])
     assert np.array_equal(ours, ref)
+
+
+# ------------------------------------------------ lid-driven cavity (walled box)
+
+def test_cavity_neumann_poisson_single_mode_closed_form(O):
+    n = 32
+    h = 1.0 / n
+    i = np.arange(n)
+    for k, l in ((1, 0), (2, 3), (5, 7)):
+        rhs = np.outer(np.cos(np.pi * l * (i + 0.5) / n), np.cos(np.pi * k * (i + 0.5) / n)).ravel()
+        lam = -4 / h ** 2 * (np.sin(np.pi * k / (2 * n)) ** 2 + np.sin(np.pi * l / (2 * n)) ** 2)
+        p = O.cav_poisson_exact(n, rhs)
+        assert np.max(np.abs(p - rhs / lam)) < 1e-12 * np.max(np.abs(rhs / lam))
+
+
+def test_cavity_poisson_inverts_the_neumann_laplacian(O):
+    n = 48
+    rhs = np.random.default_rng(2).standard_normal(n * n)
+    p = O.cav_poisson_exact(n, rhs)
+    lap = O.cav_laplacian(n, p)
+    assert np.max(np.abs(lap - (rhs - rhs.mean()))) < 1e-9 * np.max(np.abs(rhs))
+    # the Neumann Laplacian is symmetric and annihilates constants
+    a, b = np.random.default_rng(3).standard_normal((2, n * n))
+    assert abs(a @ O.cav_laplacian(n, b) - b @ O.cav_laplacian(n, a)) < 1e-9 * n * n * n * n
+    assert np.max(np.abs(O.cav_laplacian(n, np.full(n * n, 2.5)))) == 0
+
+
+def test_cavity_projection_is_divergence_free_and_keeps_walls(O):
+    n = 40
+    rng = np.random.default_rng(4)
+    u, v = rng.standard_normal((2, n * n))
+    u.reshape(n, n)[:, 0] = 0
+    v.reshape(n, n)[0, :] = 0
+    d = O.cav_divergence(n, u, v)
+    p = O.cav_poisson_exact(n, d)
+    u2, v2 = O.cav_subtract_gradient(n, u, v, p)
+    assert np.max(np.abs(O.cav_divergence(n, u2, v2))) < 1e-9 * np.max(np.abs(d))
+    assert np.all(u2.reshape(n, n)[:, 0] == 0) and np.all(v2.reshape(n, n)[0, :] == 0)
+    # idempotent
+    p2 = O.cav_poisson_exact(n, O.cav_divergence(n, u2, v2))
+    u3, v3 = O.cav_subtract_gradient(n, u2, v2, p2)
+    assert np.max(np.abs(u3 - u2)) < 1e-10 and np.max(np.abs(v3 - v2)) < 1e-10
+
+
+def test_cavity_samplers_put_the_wall_values_on_the_walls(O):
+    n, U = 16, 1.7
+    rng = np.random.default_rng(5)
+    u, v = rng.standard_normal((2, n * n))
+    u.reshape(n, n)[:, 0] = 0
+    v.reshape(n, n)[0, :] = 0
+    for x in (0.13, 0.5, 0.77):
+        assert abs(O.cav_sample_u(n, U, u, x, 0.0)) < 1e-15           # no slip at the bottom
+        assert abs(O.cav_sample_u(n, U, u, x, 1.0) - U) < 1e-14       # the lid
+        assert abs(O.cav_sample_u(n, U, u, x, -0.3)) < 1e-15          # clamped into the box
+    for y in (0.2, 0.6):
+        assert abs(O.cav_sample_v(n, v, 0.0, y)) < 1e-15 and abs(O.cav_sample_v(n, v, 1.0, y)) < 1e-15
+        assert abs(O.cav_sample_u(n, U, u, 0.0, y)) < 1e-15 and abs(O.cav_sample_u(n, U, u, 1.0, y)) < 1e-15
+    # at a stored face the sample is the stored value
+    assert O.cav_sample_u(n, U, u, 5 / n, 7.5 / n) == pytest.approx(u[7 * n + 5], abs=1e-14)
+
+
+def test_cavity_couette_profile_is_steady_under_diffusion(O):
+    # u = U y between the resting bottom wall and the lid is a steady solution away
+    # from the side walls
+    n, U = 32, 1.3
+    y = (np.arange(n) + 0.5) / n
+    u = np.tile((U * y)[:, None], (1, n)).ravel()
+    u.reshape(n, n)[:, 0] = 0
+    v = np.zeros(n * n)
+    ou, ov = O.cav_diffuse(n, U, u, v, 0.01, 1e-3)
+    inner = np.s_[:, 2:n - 1]
+    assert np.max(np.abs(ou.reshape(n, n)[inner] - u.reshape(n, n)[inner])) < 1e-13
+    assert np.all(ov == 0)
+
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_cavity_advection_at_rest_and_bounded(O, scheme):
+    n = 32
+    z = np.zeros(n * n)
+    ou, ov = O.cav_advect(scheme, n, 1.0, z, z, 0.01)
+    assert np.all(ou == 0) and np.all(ov == 0)
+    rng = np.random.default_rng(6)
+    u, v = rng.uniform(-1, 1, (2, n * n))
+    u.reshape(n, n)[:, 0] = 0
+    v.reshape(n, n)[0, :] = 0
+    ou, ov = O.cav_advect(scheme, n, 1.0, u, v, 0.02)
+    # values come from the field, the lid ghosts (<= 2U + 1) or walls: bounded, walls kept
+    assert np.all(ou.reshape(n, n)[:, 0] == 0) and np.all(ov.reshape(n, n)[0, :] == 0)
+    assert np.max(np.abs(ou)) <= 3.0 + 1e-12 and np.max(np.abs(ov)) <= 1.0 + 1e-12
+
+
+def test_cavity_flow_spins_up_divergence_free(O):
+    s = O.Cavity(32, U=1.0, nu=1e-2, disk=dict(x=0.5, y=0.4, r=0.12), passes=2)
+    for _ in range(20):
+        s.step(s.dt_cfl(0.5))
+    d = O.cav_divergence(32, s.u, s.v)
+    assert np.max(np.abs(d)) < 1e-9
+    uu = s.u.reshape(32, 32)
+    assert uu[-1, 1:-1].mean() > 0.2      # the lid drags the top row
+    assert np.all(uu[:, 0] == 0)
