@@ -87,6 +87,10 @@ def parse():
     ap.add_argument("--guess", default="zero", choices=["zero", "previous", "learned"])
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the second timed run with the other pressure guess")
+    ap.add_argument("--report", action="store_true",
+                    help="forced2048: time-averaged E(k) (device spectrum, Parseval) over a second run and the "
+                         "field error against the f64 oracle over --validate-steps steps")
+    ap.add_argument("--validate-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -160,6 +164,59 @@ def cpu_baseline(wl, n, seconds):
             "sample": f"f64 oracle (oracle/fsoracle.c), same {n}^2 workload with exact FFT projection, "
                       f"{st} steps in {el:.1f} s on 1 thread",
             "omp": {"value": vo, "cores": co, "sample": f"{sto} steps in {elo:.1f} s"}}
+
+
+def forced_report(a, fluidsim, wl, n, u0, v0):
+    """Statistics of the workload: E(k) averaged over samples every 50 steps of a run as
+    long as the timed one (device spectrum, FP64), its Parseval shortfall and the
+    log-log slope over k in [2 kf, n/8]; and the FP32 field error against the f64
+    oracle (exact FFT projection, same forcing seeds and dt) over the first steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    g = make_solver(fluidsim, wl, n, a.tol, "previous")
+    g.uploadVelocity(u0, v0)
+    f = wl["forcing"]
+    # field error over the first steps
+    O.set_threads(min(16, os.cpu_count() or 1))
+    s = O.Solver(n=n, nu=wl["nu"], advection="maccormack", explicit_diffusion=True, projection="fft",
+                 forcing=f, drag=f["drag"] if f else 0.0)
+    s.u[:] = u0.astype(np.float32)
+    s.v[:] = v0.astype(np.float32)
+    errs = []
+    t0 = time.perf_counter()
+    for k in range(a.validate_steps):
+        dt = g.stepCFL(0.5)
+        s.step(dt, force_seed=(f["seed"] + k) if f else 0)
+        if k in (0, a.validate_steps // 2, a.validate_steps - 1):
+            vel = g.readVelocity()
+            den = math.sqrt(np.sum(s.u ** 2) + np.sum(s.v ** 2))
+            errs.append({"step": k + 1, "relL2": math.sqrt(np.sum((vel["u"] - s.u) ** 2) +
+                                                            np.sum((vel["v"] - s.v) ** 2)) / den})
+    O.set_threads(1)
+    print(f"# field error vs f64 oracle: {errs} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    # time-averaged spectrum
+    Es = []
+    for k in range(a.steps):
+        g.stepCFL(0.5)
+        if (k + 1) % 50 == 0:
+            _, E, _ = g.energySpectrum()
+            Es.append(E)
+    d = g.diagnostics()
+    _, Elast, cnt = g.energySpectrum()
+    g.destroy()
+    E = np.mean(Es, axis=0)
+    kf = f["kf"] if f else 8.0
+    k = np.arange(len(E))
+    sel = (k >= 2 * kf) & (k <= n // 8) & (E > 0)
+    slope = float(np.polyfit(np.log(k[sel]), np.log(E[sel]), 1)[0]) if sel.sum() > 3 else None
+    parseval = (d["kinetic_energy"] - float(np.sum(Elast))) / d["kinetic_energy"]
+    shells = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]
+    rep = {"samples": len(Es), "every": 50, "E_mean": {str(q): float(E[q]) for q in shells if q < len(E)},
+           "slope_2kf_to_n8": slope, "parseval_shortfall_last": parseval, "field_error_vs_f64": errs}
+    print(f"# spectrum: {len(Es)} samples, slope {slope}, Parseval shortfall {parseval:.3g}", file=sys.stderr,
+          flush=True)
+    return rep
 
 
 def bench_cavities(a, fluidsim, dev):
@@ -297,6 +354,10 @@ def main():
             print(f"# {other} guess: {ablation[-1]['value']:.4g} cell-steps/s, {c2:.2f} V-cycles/step, "
                   f"{ablation[-1]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
 
+    report = None
+    if a.report:
+        report = forced_report(a, fluidsim, wl, n, u0, v0)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(wl, n, a.cpu_seconds)
@@ -335,6 +396,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "ablation": ablation,
+        "report": report,
     }
     print(json.dumps(line), flush=True)
     return 0

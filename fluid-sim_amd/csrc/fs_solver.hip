@@ -37,6 +37,7 @@
 #include "fs_mg_fused.hpp"
 #include "fs_guess.hpp"
 #include "fs_batch.hpp"
+#include "fs_spectrum.hpp"
 
 namespace {
 
@@ -158,6 +159,10 @@ struct fs_solver_t {
     double* ppart = nullptr;        // per-block max|p_prev|
     int nblk = 0;
   } guess_net;
+  // energy spectrum work space (allocated on first use)
+  fsk::cplx* spec_Z = nullptr;
+  double* spec_part = nullptr;
+  double* spec_E = nullptr;
   // immersed disk: state for the next step, per-block impulse partials, FP64 total
   bool solid_on = false;
   fs_disk solid{};
@@ -1117,6 +1122,9 @@ void release(fs_solver_t* s) {
   (void)hipSetDevice(s->dev);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   release_guess(s);
+  dfree(s->spec_Z);
+  dfree(s->spec_part);
+  dfree(s->spec_E);
   for (int k = 0; k < 2; k++) {
     dfree(s->u[k]);
     dfree(s->v[k]);
@@ -2044,6 +2052,49 @@ int fs_profile_read(void* h, int kind, double* total_ms, int64_t* launches, doub
   return FS_OK;
 }
 
+
+int fs_energy_spectrum(void* h, double* E, int64_t* counts, int len) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  const int n = s->nx;
+  if (s->nx != s->ny || n < 4 || n > 4096 || (n & (n - 1)))
+    return fail(FS_E_UNSUPPORTED, "the spectrum needs a square power-of-two grid up to 4096, got %dx%d", s->nx,
+                s->ny);
+  const int kmax = n / 2;
+  if (len != kmax + 1) return fail(FS_E_INVALID_ARG, "E needs n/2 + 1 = %d entries, got %d", kmax + 1, len);
+  if (!E) return fail(FS_E_INVALID_ARG, "E is NULL");
+  if ((rc = set_device(s))) return rc;
+  if (!s->spec_Z) {
+    if ((rc = dalloc(&s->spec_Z, (size_t)n * n))) return rc;
+    if ((rc = dalloc(&s->spec_part, (size_t)n * (kmax + 1)))) return rc;
+    if ((rc = dalloc(&s->spec_E, (size_t)kmax + 1))) return rc;
+  }
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  const int c = s->cur;
+  size_t sh_rows = sizeof(fsk::cplx) * (size_t)(n + n / 2), sh_cols = sh_rows + sizeof(double) * (size_t)n;
+  fsk::k_spec_rows<<<n, 256, sh_rows, s->stream>>>(s->u[c], s->v[c], s->spec_Z, n, lg);
+  LAUNCHCHK();
+  fsk::k_spec_cols<<<n, 256, sh_cols, s->stream>>>(s->spec_Z, s->spec_part, n, lg);
+  LAUNCHCHK();
+  fsk::k_spec_final<<<(unsigned)((kmax + 1 + 255) / 256), 256, 0, s->stream>>>(s->spec_part, s->spec_E, n);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(E, s->spec_E, sizeof(double) * (kmax + 1), hipMemcpyDeviceToHost, s->stream));
+  if ((rc = sync_stream(s))) return rc;
+  if (counts) {  // modes per shell (src/core/fft.js:214-226), independent of the field
+    for (int k = 0; k <= kmax; k++) counts[k] = 0;
+    for (int j = 0; j < n; j++) {
+      long ky = j <= n / 2 ? j : j - n;
+      for (int i = 0; i < n; i++) {
+        long kx = i <= n / 2 ? i : i - n;
+        int sh = (int)std::floor(std::sqrt((double)(kx * kx + ky * ky)) + 0.5);
+        if (sh <= kmax) counts[sh]++;
+      }
+    }
+  }
+  return FS_OK;
+}
 
 // ------------------------------------------------------------ cavity batches
 

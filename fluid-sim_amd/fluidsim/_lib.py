@@ -143,6 +143,7 @@ _SIGNATURES = {
     "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
     "fs_tile_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
     "fs_load_guess": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
+    "fs_energy_spectrum": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]),
     "fs_batch_config_default": (None, [C.POINTER(FsBatchConfig)]),
     "fs_batch_create": (C.c_int, [C.POINTER(FsBatchConfig), C.POINTER(FsCavity), C.c_int, C.POINTER(_VP)]),
     "fs_batch_destroy": (C.c_int, [_VP]),

@@ -283,6 +283,15 @@ class GPUFluidSolver:
             mask |= 1 << _lib.PROFILE_KINDS[k]
         check(self._L.fs_profile_enable(self.handle, mask))
 
+    def energySpectrum(self):
+        """energySpectrum (src/core/fft.js:196-234) of the current velocity, on the device in FP64:
+        returns (k, E, counts) for shells k = 0..n/2."""
+        kmax = self.nx // 2
+        E = (C.c_double * (kmax + 1))()
+        cnt = (C.c_int64 * (kmax + 1))()
+        check(self._L.fs_energy_spectrum(self.handle, E, cnt, kmax + 1))
+        return np.arange(kmax + 1), np.array(E[:]), np.array(cnt[:])
+
     def tileReport(self) -> str:
         """Tile shape per grid-wide level and the autotuner's timings."""
         buf = C.create_string_buffer(4096)

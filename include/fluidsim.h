@@ -224,6 +224,13 @@ int fs_solid_impulse(void* solver, double* out);
 int fs_get_stats(void* solver, fs_stats* out);
 int fs_diagnostics(void* solver, fs_diag* out);
 
+/* energySpectrum(cu, cv) of src/core/fft.js:196-234 on the device, in FP64: the
+ * centre-interpolated velocity, shells round(|k|) <= n/2, E[k] = sum of
+ * 0.5 (|U_k|^2 + |V_k|^2) / N^2 over the shell (Parseval: sum E = kinetic energy less
+ * the corner modes). Square power-of-two grids up to 4096; len = n/2 + 1; counts
+ * (modes per shell) may be NULL. */
+int fs_energy_spectrum(void* solver, double* E, int64_t* counts, int len);
+
 /* per-kernel hook replacing gputest's use of internals */
 int fs_run_kernel(void* solver, int kernel, double dt);
 

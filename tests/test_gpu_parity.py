@@ -583,3 +583,31 @@ def test_learned_guess_keeps_the_gate(fs, O):
     for _ in range(4):
         s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
     assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+
+
+# ------------------------------------------------------------ energy spectrum
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_energy_spectrum_matches_oracle(fs, O, n):
+    # energySpectrum (src/core/fft.js:196-234) on the device against the oracle's FFT of
+    # the same centre velocities; Parseval against the physical-space energy
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=4)
+    with solver(fs, n=n) as g:
+        g.uploadVelocity(u0, v0)
+        k, E, cnt = g.energySpectrum()
+        vel = g.readVelocity()
+    cu, cv = O.velocity_at_centers(n, n, f32(vel["u"]), f32(vel["v"]))
+    Eo, co = O.energy_spectrum(n, n, cu, cv)
+    assert np.array_equal(cnt, co)
+    tot = np.sum(Eo)
+    assert np.max(np.abs(E - Eo)) <= 1e-12 * tot
+    phys = 0.5 * np.mean(cu * cu + cv * cv)
+    corners = phys - tot                   # modes beyond n/2 are dropped, as in the reference
+    assert -1e-12 * phys <= corners <= 1e-3 * phys
+    # a single Taylor-Green mode sits in shell 1 (sqrt 2 rounds to 1)
+    tu, tv = O.taylor_green(n, n)
+    with solver(fs, n=n) as g:
+        g.uploadVelocity(tu, tv)
+        _, Et, _ = g.energySpectrum()
+    assert Et[1] > 0.99 * np.sum(Et)
