@@ -18,6 +18,7 @@
 // large, so a level operator is one flat launch over a class.
 #pragma once
 #include "fs_kernels.hpp"
+#include "fs_mg_fused.hpp"
 
 namespace fsk {
 
@@ -355,10 +356,10 @@ k_nm_restrict(const float* __restrict__ x, const float* __restrict__ rhs, int lg
   crhs[q] = acc * 0.25f;
 }
 
-// x += bilinear cell-centred prolongation of the coarse correction, the coarse
+// x = xin + bilinear cell-centred prolongation of the coarse correction, the coarse
 // indices clamped into the box (constant extrapolation, zero gradient at the walls).
 __global__ void __launch_bounds__(256)
-k_nm_prolong(const float* __restrict__ xc, float* __restrict__ x, int lg, long count) {
+k_nm_prolong(const float* __restrict__ xc, const float* __restrict__ xin, float* __restrict__ x, int lg, long count) {
   long q = (long)blockIdx.x * 256 + threadIdx.x;
   const int s = 1 << lg, cs = s >> 1, clg = lg - 1;
   if (q >= count * s * s) return;
@@ -371,7 +372,181 @@ k_nm_prolong(const float* __restrict__ xc, float* __restrict__ x, int lg, long c
   const float* cg = xc + (g << (2 * clg));
   float a = cg[y0 * cs + x0], b = cg[y0 * cs + x1], c = cg[y1 * cs + x0], d = cg[y1 * cs + x1];
   float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-  x[q] += top * (1.f - fy) + bot * fy;
+  x[q] = xin[q] + (top * (1.f - fy) + bot * fy);
+}
+
+// ---- fused level kernels (LDS tiles, as on the periodic grid): a tile with a halo of
+// 2 SW cells; halo cells outside the box hold 0 and are never updated, so every
+// in-box cell sees exactly the operands of the one-pass kernels above in the same
+// order (an absent neighbour adds +0), and the results are the same bits.
+
+// LDS window of grid g (s x s) with origin (ox, oy); cells outside the box are 0.
+template <int RW, int RH, int NT>
+__device__ __forceinline__ void nm_load(float* __restrict__ dst, const float* __restrict__ src, int s, int ox, int oy) {
+  constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+  float tmp[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) {
+      int ly = q / RW, lx = q - ly * RW;
+      int gx = ox + lx, gy = oy + ly;
+      tmp[k] = (gx >= 0 && gx < s && gy >= 0 && gy < s) ? src[(size_t)gy * s + gx] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNT) dst[q] = tmp[k];
+  }
+}
+
+// One colour half-sweep on the LDS region [LO, RW-LO) x [LO, RH-LO); cells outside
+// the box are skipped (they stay 0).
+template <int RW, int RH, int LO, int NT>
+__device__ __forceinline__ void nm_half_sweep(float* __restrict__ sx, const float* __restrict__ sr, int colour,
+                                              int par0, float h2, int ox, int oy, int s) {
+  constexpr int w = RW - 2 * LO, hgt = RH - 2 * LO, hw = w / 2;
+  constexpr int CNT = hw * hgt, K = (CNT + NT - 1) / NT;
+  float acc[K];
+  int kk[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = threadIdx.x + k * NT;
+    kk[k] = -1;
+    if (q < CNT) {
+      int ry = q / hw, rx = q - ry * hw;
+      int ly = LO + ry;
+      int lx = LO + 2 * rx + ((colour + par0 + LO + ly) & 1);
+      int gi = ox + lx, gj = oy + ly;
+      if (gi >= 0 && gi < s && gj >= 0 && gj < s) {
+        int c = ly * RW + lx;
+        kk[k] = c;
+        int nk = 4 - (gi == 0) - (gi == s - 1) - (gj == 0) - (gj == s - 1);
+        float sum = sx[c - 1] + sx[c + 1] + sx[c - RW] + sx[c + RW];
+        acc[k] = (sum - h2 * sr[c]) / (float)nk;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (kk[k] >= 0) sx[kk[k]] = acc[k];
+}
+
+template <int RW, int RH, int S, int SEND, int LOOFF, int NT>
+struct NmSweeps {
+  static __device__ __forceinline__ void run(float* sx, const float* sr, int par0, float h2, int ox, int oy, int s) {
+    nm_half_sweep<RW, RH, S + LOOFF, NT>(sx, sr, S & 1, par0, h2, ox, oy, s);
+    __syncthreads();
+    NmSweeps<RW, RH, S + 1, SEND, LOOFF, NT>::run(sx, sr, par0, h2, ox, oy, s);
+  }
+};
+template <int RW, int RH, int SEND, int LOOFF, int NT>
+struct NmSweeps<RW, RH, SEND, SEND, LOOFF, NT> {
+  static __device__ __forceinline__ void run(float*, const float*, int, float, int, int, int) {}
+};
+
+// Down: SW sweeps from zero on a tile of grid g, the pre-smoothed correction stored,
+// and the restricted residual of each 2x2 block of the tile to the coarse rhs.
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
+k_nmt_down(float* __restrict__ x, const float* __restrict__ rhs, int s, int tiles_x, int tpg, float h2, float ih2,
+           float* __restrict__ crhs) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT, H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  __shared__ float sx[RH * RW];
+  __shared__ float sr[RH * RW];
+  const int g = blockIdx.x / tpg, t = blockIdx.x - g * tpg;
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const size_t gb = (size_t)g * s * s;
+  nm_load<RW, RH, NT>(sr, rhs + gb, s, ox, oy);
+  __syncthreads();
+  const int par0 = (ox + oy) & 1;
+  // red from zero on the whole region (black and outside cells 0), then shrinking regions
+  for (int q = threadIdx.x; q < RW * RH; q += NT) {
+    int ly = q / RW, lx = q - ly * RW;
+    int gi = ox + lx, gj = oy + ly;
+    bool in = gi >= 0 && gi < s && gj >= 0 && gj < s;
+    bool red = ((lx + ly + par0) & 1) == 0;
+    int nk = 4 - (gi == 0) - (gi == s - 1) - (gj == 0) - (gj == s - 1);
+    sx[q] = (in && red) ? (0.f - h2 * sr[q]) / (float)nk : 0.f;
+  }
+  __syncthreads();
+  NmSweeps<RW, RH, 1, 2 * SW, 0, NT>::run(sx, sr, par0, h2, ox, oy, s);
+  for (int q = threadIdx.x; q < TX * TY; q += NT) {
+    int iy = q / TX, ix = q - iy * TX;
+    x[gb + (size_t)(ty0 + iy) * s + tx0 + ix] = sx[(iy + H) * RW + ix + H];
+  }
+  constexpr int CX = TX / 2, CY = TY / 2;
+  const int cs = s >> 1;
+  const size_t cb = (size_t)g * cs * cs;
+  for (int q = threadIdx.x; q < CX * CY; q += NT) {
+    int cy = q / CX, cx = q - cy * CX;
+    int I = (tx0 >> 1) + cx, J = (ty0 >> 1) + cy;
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        int i = 2 * I + a, j = 2 * J + b;
+        int k = (2 * cy + b + H) * RW + (2 * cx + a + H);
+        float xc = sx[k], sum = 0.f;
+        if (i > 0) sum += sx[k - 1] - xc;
+        if (i < s - 1) sum += sx[k + 1] - xc;
+        if (j > 0) sum += sx[k - RW] - xc;
+        if (j < s - 1) sum += sx[k + RW] - xc;
+        acc += sr[k] - sum * ih2;
+      }
+    crhs[cb + (size_t)J * cs + I] = acc * 0.25f;
+  }
+}
+
+// Up: x = xin + bilinear prolongation of the coarse correction (coarse indices clamped
+// into the box), then SW sweeps; the tile interior goes to xout.
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
+k_nmt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* __restrict__ rhs,
+         const float* __restrict__ xc, int s, int tiles_x, int tpg, float h2) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT, H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  constexpr int CW = RW / 2 + 3, CH = RH / 2 + 3;
+  __shared__ float sx[RH * RW];
+  __shared__ float sr[RH * RW];
+  __shared__ float sc[CH * CW];
+  const int g = blockIdx.x / tpg, t = blockIdx.x - g * tpg;
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const int cs = s >> 1;
+  const size_t gb = (size_t)g * s * s, cb = (size_t)g * cs * cs;
+  const int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
+  // coarse footprint, indices clamped into the box
+  for (int q = threadIdx.x; q < CW * CH; q += NT) {
+    int ly = q / CW, lx = q - ly * CW;
+    int cx = min(max(cox + lx, 0), cs - 1), cy = min(max(coy + ly, 0), cs - 1);
+    sc[q] = xc[cb + (size_t)cy * cs + cx];
+  }
+  nm_load<RW, RH, NT>(sx, xin + gb, s, ox, oy);
+  nm_load<RW, RH, NT>(sr, rhs + gb, s, ox, oy);
+  __syncthreads();
+  for (int q = threadIdx.x; q < RW * RH; q += NT) {
+    int ly = q / RW, lx = q - ly * RW;
+    int i = ox + lx, j = oy + ly;
+    if (i < 0 || i >= s || j < 0 || j >= s) continue;
+    int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1, j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+    float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+    // clamped coarse indices, as local footprint coordinates
+    int x0 = max(i0, 0) - cox, x1 = min(i0 + 1, cs - 1) - cox, y0 = max(j0, 0) - coy, y1 = min(j0 + 1, cs - 1) - coy;
+    float a = sc[y0 * CW + x0], b = sc[y0 * CW + x1], c = sc[y1 * CW + x0], d = sc[y1 * CW + x1];
+    float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+    sx[q] = sx[q] + (top * (1.f - fy) + bot * fy);
+  }
+  __syncthreads();
+  NmSweeps<RW, RH, 0, 2 * SW, 1, NT>::run(sx, sr, (ox + oy) & 1, h2, ox, oy, s);
+  for (int q = threadIdx.x; q < TX * TY; q += NT) {
+    int iy = q / TX, ix = q - iy * TX;
+    xout[gb + (size_t)(ty0 + iy) * s + tx0 + ix] = sx[(iy + H) * RW + ix + H];
+  }
 }
 
 // The levels from s0 <= 64 down to 4x4 of one simulation per workgroup, in LDS:

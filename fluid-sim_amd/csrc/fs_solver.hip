@@ -1203,7 +1203,9 @@ struct fs_batch_t {
   // multigrid size classes s = 2^lg (lg from max_lg down to 2): grids of the first
   // count[lg] simulations, at base[lg] in the level buffers
   long base[13] = {}, count[13] = {};
-  float *lx = nullptr, *lr = nullptr;
+  float *lx = nullptr, *lr = nullptr;  // pre-smoothed corrections, right-hand sides
+  float* lxb = nullptr;      // post-smoothed corrections (the coarse solve's at 64 x 64)
+  bool fused = false;        // LDS-tiled level kernels
   long* cbase = nullptr;     // per simulation: its 64 x 64 level
   float* pinv4 = nullptr;    // pseudo-inverse of the 4 x 4 Neumann Laplacian (h = 1/4)
   double *rpart = nullptr, *rpart_host = nullptr;
@@ -1236,7 +1238,7 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
   const int c = b->cur;
   if ((rc = blaunch(b, fsk::k_cav_residual64, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
                     (const float*)b->v[c], (const double*)(zero_p ? nullptr : b->p64[pin]),
-                    (const float*)(it > 0 ? b->lx : nullptr), b->p64[1 - pin], b->lr, (const fsk::CavSim*)b->sims,
+                    (const float*)(it > 0 ? b->lxb : nullptr), b->p64[1 - pin], b->lr, (const fsk::CavSim*)b->sims,
                     (const int*)b->blk0, b->nsim, b->rpart + (size_t)it * b->nblocks)))
     return rc;
   if (residual_only) return FS_OK;
@@ -1247,6 +1249,17 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
     float* x = b->lx + b->base[lg];
     const float* r = b->lr + b->base[lg];
     const float h2 = 1.f / ((float)s * s);
+    if (b->fused) {
+      const int tiles_x = s / fsk::Tile32::TX, tpg = tiles_x * (s / fsk::Tile32::TY);
+      dim3 g((unsigned)(cnt * tpg));
+      float ih2 = (float)((double)s * s);
+      float* cr = b->lr + b->base[lg - 1];
+      rc = pre == 1 ? blaunch(b, fsk::k_nmt_down<1, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr)
+         : pre == 2 ? blaunch(b, fsk::k_nmt_down<2, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr)
+                    : blaunch(b, fsk::k_nmt_down<3, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr);
+      if (rc) return rc;
+      continue;
+    }
     const long half = cnt * s * (s / 2);
     if (pre == 0) {
       fsk::k_fill_f32<<<nblk(cnt * s * s), 256, 0, b->stream>>>(x, (size_t)cnt * s * s, 0.f);
@@ -1261,22 +1274,31 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
                       cnt, (float)((double)s * s), b->lr + b->base[lg - 1])))
       return rc;
   }
-  if ((rc = blaunch(b, fsk::k_nm_coarse, dim3(b->nsim), dim3(1024), b->lx, (const float*)b->lr, (const long*)b->cbase,
-                    64, pre, post, (const float*)b->pinv4)))
+  if ((rc = blaunch(b, fsk::k_nm_coarse, dim3(b->nsim), dim3(1024), b->lxb, (const float*)b->lr,
+                    (const long*)b->cbase, 64, pre, post, (const float*)b->pinv4)))
     return rc;
   for (int lg = 7; lg <= b->max_lg; lg++) {
     const long cnt = b->count[lg];
     const int s = 1 << lg;
-    float* x = b->lx + b->base[lg];
+    const float* x = b->lx + b->base[lg];
+    float* xb = b->lxb + b->base[lg];
     const float* r = b->lr + b->base[lg];
+    const float* xc = b->lxb + b->base[lg - 1];
     const float h2 = 1.f / ((float)s * s);
+    if (b->fused) {
+      const int tiles_x = s / fsk::Tile32::TX, tpg = tiles_x * (s / fsk::Tile32::TY);
+      dim3 g((unsigned)(cnt * tpg));
+      rc = post == 1 ? blaunch(b, fsk::k_nmt_up<1, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2)
+         : post == 2 ? blaunch(b, fsk::k_nmt_up<2, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2)
+                     : blaunch(b, fsk::k_nmt_up<3, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2);
+      if (rc) return rc;
+      continue;
+    }
     const long half = cnt * s * (s / 2);
-    if ((rc = blaunch(b, fsk::k_nm_prolong, dim3(nblk(cnt * s * s)), dim3(256), (const float*)(b->lx + b->base[lg - 1]),
-                      x, lg, cnt)))
-      return rc;
+    if ((rc = blaunch(b, fsk::k_nm_prolong, dim3(nblk(cnt * s * s)), dim3(256), xc, x, xb, lg, cnt))) return rc;
     for (int sw = 0; sw < post; sw++)
       for (int colour = 0; colour < 2; colour++)
-        if ((rc = blaunch(b, fsk::k_nm_rb, dim3(nblk(half)), dim3(256), x, r, lg, cnt, h2, colour, 0))) return rc;
+        if ((rc = blaunch(b, fsk::k_nm_rb, dim3(nblk(half)), dim3(256), xb, r, lg, cnt, h2, colour, 0))) return rc;
   }
   return FS_OK;
 }
@@ -1354,6 +1376,7 @@ void batch_release(fs_batch_t* b) {
   dfree(b->blk0);
   dfree(b->lx);
   dfree(b->lr);
+  dfree(b->lxb);
   dfree(b->cbase);
   dfree(b->pinv4);
   dfree(b->rpart);
@@ -2109,6 +2132,7 @@ void fs_batch_config_default(fs_batch_config* c) {
   c->mask_width = 1.5;
   c->guess = FS_GUESS_ZERO;
   c->device = 0;
+  c->mg_fused = 1;
 }
 
 int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count, void** out) {
@@ -2141,6 +2165,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   fs_batch_t* b = new fs_batch_t();
   b->cfg = c;
   b->dev = c.device;
+  b->fused = c.mg_fused && c.mg_pre >= 1 && c.mg_pre <= 3 && c.mg_post >= 1 && c.mg_post <= 3;
   auto bail = [&](int rc) {
     batch_release(b);
     return rc;
@@ -2204,6 +2229,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (!rc) rc = dalloc(&b->blk0, (size_t)count);
   if (!rc) rc = dalloc(&b->lx, (size_t)lvl_cells);
   if (!rc) rc = dalloc(&b->lr, (size_t)lvl_cells);
+  if (!rc) rc = dalloc(&b->lxb, (size_t)lvl_cells);
   if (!rc) rc = dalloc(&b->cbase, (size_t)count);
   if (!rc) rc = dalloc(&b->pinv4, 256);
   b->rp_rows = c.max_vcycles + 2;
@@ -2223,6 +2249,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   for (double* a : {b->p64[0], b->p64[1]}) fsk::k_fill_f64<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.0);
   fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lx, (size_t)lvl_cells, 0.f);
   fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lr, (size_t)lvl_cells, 0.f);
+  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lxb, (size_t)lvl_cells, 0.f);
   fsk::k_fill_f64<<<nblk(2L * b->nblocks), 256, 0, b->stream>>>(b->mpart, (size_t)2 * b->nblocks, 0.0);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(fail(FS_E_DEVICE, "initial fill failed"));

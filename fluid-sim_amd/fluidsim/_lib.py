@@ -94,7 +94,7 @@ class FsCavity(C.Structure):
 class FsBatchConfig(C.Structure):
     _fields_ = [("advection", C.c_int), ("tolerance", C.c_double), ("max_vcycles", C.c_int),
                 ("mg_pre", C.c_int), ("mg_post", C.c_int), ("coupling_passes", C.c_int),
-                ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int)]
+                ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int), ("mg_fused", C.c_int)]
 
 
 class FsBatchStats(C.Structure):

@@ -147,6 +147,8 @@ typedef struct fs_batch_config {
   double mask_width;          /* disk mask width in cells; default 1.5 */
   int guess;                  /* FS_GUESS_ZERO (default) or FS_GUESS_PREVIOUS */
   int device;
+  int mg_fused;               /* 1 (default): LDS-tiled level kernels; 0: one launch per operator.
+                                 Same bits. */
 } fs_batch_config;
 
 typedef struct fs_batch_stats {

@@ -75,3 +75,21 @@ def test_cavity_batch_c5_draws(fs):
         assert 0.5 <= c["lid"] <= 2 and 100 <= c["lid"] / c["nu"] <= 5000
         assert 0.05 <= d["r"] <= 0.2
         assert d["r"] + 2 * h <= d["x"] <= 1 - d["r"] - 2 * h and d["r"] + 2 * h <= d["y"] <= 1 - d["r"] - 2 * h
+
+
+@pytest.mark.parametrize("pre,post", [(2, 2), (1, 3), (3, 1)])
+def test_fused_cavity_multigrid_gives_the_same_bits(fs, pre, post):
+    # the LDS-tiled level kernels against the one-pass-per-operator kernels
+    cavs = [dict(n=256, lid=1.0, nu=1e-3, disk=dict(x=0.4, y=0.5, r=0.1)),
+            dict(n=128, lid=1.5, nu=2e-3, disk=None),
+            dict(n=512, lid=0.8, nu=4e-4, disk=dict(x=0.7, y=0.3, r=0.15))]
+    outs = []
+    for fused in (True, False):
+        with fs.CavityBatch(cavs, tolerance=1e-7, mgPre=pre, mgPost=post, mgFused=fused) as b:
+            for _ in range(4):
+                b.stepCFL(0.5)
+            outs.append([b.readVelocity(k) for k in range(len(cavs))])
+            cyc = b.stats()["total_vcycles"]
+    for a, c in zip(*outs):
+        assert np.array_equal(a["u"], c["u"]) and np.array_equal(a["v"], c["v"])
+    assert cyc > 0
