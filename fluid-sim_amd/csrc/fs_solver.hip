@@ -36,6 +36,7 @@
 #include "fs_kernels.hpp"
 #include "fs_mg_fused.hpp"
 #include "fs_guess.hpp"
+#include "fs_train.hpp"
 #include "fs_batch.hpp"
 #include "fs_spectrum.hpp"
 
@@ -768,7 +769,8 @@ int launch_conv(fs_solver_t* s, int layer) {
   // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and out, BF16
   double in_ch = MODE == fsk::CONV_POOL ? 4.0 * CIN : MODE == fsk::CONV_UPCAT ? C2 / 4.0 + (CIN - C2) : CIN;
   return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW>, g,
-                dim3(fsk::GT_NT), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m);
+                dim3(fsk::GT_NT), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m, 1,
+                fsk::ConvEpiArgs{});
 }
 
 // Evaluate the network on r (level-0 residual of p_prev, FP32) and p_prev = p64[pi],
@@ -1424,6 +1426,231 @@ std::vector<float> neumann_pinv4() {
   for (int r = 0; r < N; r++)
     for (int c = 0; c < N; c++) pt[(size_t)c * N + r] = (float)(at(r, N + c) - 1.0 / N);
   return pt;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ guess trainer
+
+namespace {
+
+// Training state of the learned guess for batches of B images at n x n.
+struct fs_trainer_t {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, B = 0;
+  double h = 0;
+  size_t P = 0;                              // parameters, blob order
+  size_t woff[15] = {}, boff[15] = {};       // offsets of each layer's weights and bias
+  float *par = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+  int64_t adam_steps = 0;
+  bool loaded = false, have_batch = false;
+  uint16_t *wf[15] = {}, *wt[15] = {};       // BF16 weights: forward, transposed and tap-flipped
+  uint16_t *act[14] = {}, *gz[14] = {};      // activations and pre-activation gradients, NHWC BF16
+  float* gcat[3] = {};                       // gradients reaching enc0b, enc1b, enc2b through the skips
+  double* pp = nullptr;                      // p_prev of the batch (FP64)
+  float *r32 = nullptr, *tgt = nullptr;      // residual (FP32) and target
+  float* err = nullptr;                      // output minus target
+  double *scales = nullptr, *mr = nullptr, *mp = nullptr;
+  int pblk = 0;                              // blocks per image of the pair preparation
+  float* part = nullptr;                     // per-workgroup gradient partials
+  size_t part_cap = 0;
+  float* opart = nullptr;
+  double* lpart = nullptr;
+  int oblk = 0;
+  double *loss_d = nullptr, *loss_h = nullptr;
+  int *idx_d = nullptr, *idx_h = nullptr;
+  double* snap = nullptr;                    // pressure snapshots, FP64 [nsnap][n][n]
+  size_t nsnap = 0;
+};
+
+void trainer_release(fs_trainer_t* t) {
+  if (!t) return;
+  if (t->stream) {
+    (void)hipSetDevice(t->dev);
+    (void)hipStreamSynchronize(t->stream);
+  }
+  for (void* q : {(void*)t->par, (void*)t->grad, (void*)t->m1, (void*)t->m2, (void*)t->pp, (void*)t->r32,
+                  (void*)t->tgt, (void*)t->err, (void*)t->scales, (void*)t->mr, (void*)t->mp, (void*)t->part, (void*)t->opart,
+                  (void*)t->lpart, (void*)t->loss_d, (void*)t->idx_d, (void*)t->snap})
+    dfree(q);
+  for (int l = 0; l < 15; l++) {
+    dfree(t->wf[l]);
+    dfree(t->wt[l]);
+  }
+  for (int a = 0; a < 14; a++) {
+    dfree(t->act[a]);
+    dfree(t->gz[a]);
+  }
+  for (int a = 0; a < 3; a++) dfree(t->gcat[a]);
+  hfree(t->loss_h);
+  hfree(t->idx_h);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+template <typename K, typename... Args>
+int tlaunch(fs_trainer_t* t, K kern, dim3 g, dim3 b, Args... args) {
+  kern<<<g, b, 0, t->stream>>>(args...);
+  LAUNCHCHK();
+  return FS_OK;
+}
+
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
+int train_fwd(fs_trainer_t* t, int layer) {
+  const int m = t->n >> kGuessLayers[layer].level;
+  const uint16_t* x1 = t->act[kGuessIn[layer][0]];
+  const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? t->act[kGuessIn[layer][1]] : nullptr;
+  const int ntiles = (m / fsk::GT_W) * (m / TH) * t->B, nyb = COUT / (16 * NBW);
+  dim3 g((unsigned)std::max(1, std::min(ntiles, 2048 / nyb)), (unsigned)nyb);
+  return tlaunch(t, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW>, g, dim3(fsk::GT_NT), x1, x2,
+                 (const uint16_t*)t->wf[layer], (const float*)(t->par + t->boff[layer]), t->act[layer], m, t->B,
+                 fsk::ConvEpiArgs{});
+}
+
+// Data gradient of 3x3 layer `layer` (CIN, COUT its own channels): k_conv3x3 over
+// gz[layer] with the transposed weights, COUT -> CIN channels, epilogue EPI.
+template <int CIN, int COUT, int TH, int NBW, int EPI, int OS>
+int train_bwd_data(fs_trainer_t* t, int layer, uint16_t* y, fsk::ConvEpiArgs e) {
+  const int m = t->n >> kGuessLayers[layer].level;
+  const int ntiles = (m / fsk::GT_W) * (m / TH) * t->B, nyb = CIN / (16 * NBW);
+  dim3 g((unsigned)std::max(1, std::min(ntiles, 2048 / nyb)), (unsigned)nyb);
+  return tlaunch(t, fsk::k_conv3x3<COUT, CIN, fsk::CONV_DIRECT, 0, TH, NBW, EPI, OS>, g, dim3(fsk::GT_NT),
+                 (const uint16_t*)t->gz[layer], (const uint16_t*)nullptr, (const uint16_t*)t->wt[layer],
+                 (const float*)nullptr, y, m, t->B, e);
+}
+
+// Weight and bias gradient of 3x3 layer `layer` into grad (partials, then a fixed-order sum).
+template <int CIN, int COUT, int MODE, int C2, int TH>
+int train_bwd_weights(fs_trainer_t* t, int layer, bool size_only = false, size_t* need = nullptr) {
+  const int m = t->n >> kGuessLayers[layer].level;
+  const int ntiles = (m / fsk::GT_W) * (m / TH) * t->B, nyb = COUT / 16;
+  const int gx = std::max(1, std::min(std::max(1, ntiles / 4), 2048 / nyb));
+  const size_t PL = (size_t)COUT * 9 * CIN + COUT;
+  if (size_only) {
+    *need = std::max(*need, PL * (size_t)gx);
+    return FS_OK;
+  }
+  const uint16_t* x1 = t->act[kGuessIn[layer][0]];
+  const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? t->act[kGuessIn[layer][1]] : nullptr;
+  int rc;
+  if ((rc = tlaunch(t, fsk::k_conv3x3_wgrad<CIN, COUT, MODE, C2, TH>, dim3((unsigned)gx, (unsigned)nyb),
+                    dim3(fsk::GT_NT), x1, x2, (const uint16_t*)t->gz[layer], t->part, m, t->B)))
+    return rc;
+  return tlaunch(t, fsk::k_tr_reduce, dim3((unsigned)((PL + 255) / 256)), dim3(256), (const float*)t->part, gx, PL,
+                 t->grad + t->woff[layer]);
+}
+
+// every weight-gradient launch (or only the partial-buffer size it needs)
+int train_all_weights(fs_trainer_t* t, bool size_only, size_t* need) {
+  using namespace fsk;
+  int rc;
+  if ((rc = train_bwd_weights<16, 16, CONV_DIRECT, 0, 16>(t, 13, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<48, 16, CONV_UPCAT, 32, 16>(t, 12, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<32, 32, CONV_DIRECT, 0, 16>(t, 11, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<96, 32, CONV_UPCAT, 64, 16>(t, 10, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<64, 64, CONV_DIRECT, 0, 8>(t, 9, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<128, 64, CONV_UPCAT, 64, 8>(t, 8, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<64, 64, CONV_DIRECT, 0, 8>(t, 7, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<64, 64, CONV_POOL, 0, 8>(t, 6, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<64, 64, CONV_DIRECT, 0, 8>(t, 5, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<32, 64, CONV_POOL, 0, 8>(t, 4, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<32, 32, CONV_DIRECT, 0, 16>(t, 3, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<16, 32, CONV_POOL, 0, 16>(t, 2, size_only, need))) return rc;
+  if ((rc = train_bwd_weights<16, 16, CONV_DIRECT, 0, 16>(t, 1, size_only, need))) return rc;
+  return FS_OK;
+}
+
+int train_first_grid(const fs_trainer_t* t) {
+  const int ntiles = (t->n / 16) * (t->n / 16) * t->B;
+  return std::max(1, std::min(ntiles / 4, 1024));
+}
+
+// Forward pass, loss and every gradient of the batch whose indices are in idx_d.
+int train_forward_backward(fs_trainer_t* t) {
+  using namespace fsk;
+  const int n = t->n, B = t->B;
+  const size_t nn = (size_t)n * n;
+  const double h2 = t->h * t->h;
+  int rc;
+  // ---- pairs, scales, targets
+  if ((rc = tlaunch(t, k_tr_prep, dim3((unsigned)t->pblk, (unsigned)B), dim3(256), (const double*)t->snap,
+                    (const int*)t->idx_d, n, 1.0 / h2, t->pp, t->r32, t->mr, t->mp)))
+    return rc;
+  if ((rc = tlaunch(t, k_tr_scales, dim3((unsigned)B), dim3(256), (const double*)t->mr, (const double*)t->mp, t->pblk,
+                    t->scales)))
+    return rc;
+  if ((rc = tlaunch(t, k_tr_target, dim3((unsigned)t->pblk, (unsigned)B), dim3(256), (const double*)t->snap,
+                    (const int*)t->idx_d, n, h2, (const double*)t->scales, t->tgt)))
+    return rc;
+  // ---- forward (the inference kernels over the batch)
+  if ((rc = tlaunch(t, k_guess_first, dim3((unsigned)(n / 64), (unsigned)(n / 4), (unsigned)B), dim3(256),
+                    (const float*)t->r32, (const double*)t->pp, (const double*)t->scales,
+                    (const float*)(t->par + t->woff[0]), (const float*)(t->par + t->boff[0]), t->act[0], n)))
+    return rc;
+  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 16, 1>(t, 1))) return rc;
+  if ((rc = train_fwd<16, 32, CONV_POOL, 0, 16, 2>(t, 2))) return rc;
+  if ((rc = train_fwd<32, 32, CONV_DIRECT, 0, 16, 2>(t, 3))) return rc;
+  if ((rc = train_fwd<32, 64, CONV_POOL, 0, 8, 2>(t, 4))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 2>(t, 5))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_POOL, 0, 8, 1>(t, 6))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 1>(t, 7))) return rc;
+  if ((rc = train_fwd<128, 64, CONV_UPCAT, 64, 8, 2>(t, 8))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 2>(t, 9))) return rc;
+  if ((rc = train_fwd<96, 32, CONV_UPCAT, 64, 16, 2>(t, 10))) return rc;
+  if ((rc = train_fwd<32, 32, CONV_DIRECT, 0, 16, 2>(t, 11))) return rc;
+  if ((rc = train_fwd<48, 16, CONV_UPCAT, 32, 16, 1>(t, 12))) return rc;
+  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 16, 1>(t, 13))) return rc;
+  // ---- output layer: loss, its gradients, gz of dec0b
+  const size_t count = nn * B;
+  if ((rc = tlaunch(t, k_tr_out, dim3((unsigned)t->oblk), dim3(256), (const uint16_t*)t->act[13],
+                    (const float*)(t->par + t->woff[14]), (const float*)(t->par + t->boff[14]), (const float*)t->tgt,
+                    count, t->gz[13], t->lpart, t->opart, t->err)))
+    return rc;
+  if ((rc = tlaunch(t, k_tr_loss, dim3(1), dim3(256), (const double*)t->lpart, t->oblk, (double)count, t->loss_d)))
+    return rc;
+  if ((rc = tlaunch(t, k_tr_reduce, dim3(1), dim3(256), (const float*)t->opart, t->oblk, (size_t)17,
+                    t->grad + t->woff[14])))
+    return rc;
+  // ---- data gradients, deepest first (each skip gradient is written before it is added)
+  auto E = [](const uint16_t* act, const float* gin, float* gout) { return ConvEpiArgs{act, gin, gout}; };
+  if ((rc = train_bwd_data<16, 16, 16, 1, EPI_MASK, 0>(t, 13, t->gz[12], E(t->act[12], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<48, 16, 16, 1, EPI_UPCAT, 32>(t, 12, t->gz[11], E(t->act[11], nullptr, t->gcat[0]))))
+    return rc;
+  if ((rc = train_bwd_data<32, 32, 16, 2, EPI_MASK, 0>(t, 11, t->gz[10], E(t->act[10], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<96, 32, 16, 2, EPI_UPCAT, 64>(t, 10, t->gz[9], E(t->act[9], nullptr, t->gcat[1]))))
+    return rc;
+  if ((rc = train_bwd_data<64, 64, 8, 2, EPI_MASK, 0>(t, 9, t->gz[8], E(t->act[8], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<128, 64, 8, 2, EPI_UPCAT, 64>(t, 8, t->gz[7], E(t->act[7], nullptr, t->gcat[2]))))
+    return rc;
+  if ((rc = train_bwd_data<64, 64, 8, 1, EPI_MASK, 0>(t, 7, t->gz[6], E(t->act[6], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<64, 64, 8, 1, EPI_UNPOOL, 0>(t, 6, t->gz[5], E(t->act[5], t->gcat[2], nullptr))))
+    return rc;
+  if ((rc = train_bwd_data<64, 64, 8, 2, EPI_MASK, 0>(t, 5, t->gz[4], E(t->act[4], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<32, 64, 8, 2, EPI_UNPOOL, 0>(t, 4, t->gz[3], E(t->act[3], t->gcat[1], nullptr))))
+    return rc;
+  if ((rc = train_bwd_data<32, 32, 16, 2, EPI_MASK, 0>(t, 3, t->gz[2], E(t->act[2], nullptr, nullptr)))) return rc;
+  if ((rc = train_bwd_data<16, 32, 16, 1, EPI_UNPOOL, 0>(t, 2, t->gz[1], E(t->act[1], t->gcat[0], nullptr))))
+    return rc;
+  if ((rc = train_bwd_data<16, 16, 16, 1, EPI_MASK, 0>(t, 1, t->gz[0], E(t->act[0], nullptr, nullptr)))) return rc;
+  // ---- weight gradients
+  if ((rc = train_all_weights(t, false, nullptr))) return rc;
+  const int g0 = train_first_grid(t);
+  if ((rc = tlaunch(t, k_tr_first_wgrad, dim3((unsigned)g0), dim3(320), (const uint16_t*)t->gz[0],
+                    (const float*)t->r32, (const double*)t->pp, (const double*)t->scales, n, B, t->part)))
+    return rc;
+  return tlaunch(t, k_tr_reduce, dim3(2), dim3(256), (const float*)t->part, g0, (size_t)304, t->grad + t->woff[0]);
+}
+
+int train_pack(fs_trainer_t* t) {
+  for (int l = 1; l < 14; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    int nw = L.cout * 9 * L.cin;
+    int rc = tlaunch(t, fsk::k_tr_pack, dim3((unsigned)((nw + 255) / 256)), dim3(256),
+                     (const float*)(t->par + t->woff[l]), L.cin, L.cout, t->wf[l], t->wt[l]);
+    if (rc) return rc;
+  }
+  return FS_OK;
 }
 
 }  // namespace
@@ -2345,6 +2572,306 @@ int fs_batch_get_stats(void* h, fs_batch_stats* out) {
   out->last_vcycles = b->last_vcycles;
   out->total_vcycles = b->total_vcycles;
   out->last_residual = b->last_residual;
+  return FS_OK;
+}
+
+// ------------------------------------------------------------ guess trainer
+
+int fs_trainer_create(int device, int n, int batch, double h, void** out) {
+  if (!out) return fail(FS_E_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  if (n < 128 || n > 4096 || n % 128 != 0)
+    return fail(FS_E_UNSUPPORTED, "the trainer needs n a multiple of 128 in 128..4096, got %d", n);
+  if (batch < 1 || batch > 256) return fail(FS_E_INVALID_ARG, "batch must be in 1..256, got %d", batch);
+  if (!(h > 0) || !std::isfinite(h)) return fail(FS_E_INVALID_ARG, "h must be finite and > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FS_E_DEVICE, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(FS_E_INVALID_ARG, "device %d of %d", device, ndev);
+  fs_trainer_t* t = new fs_trainer_t();
+  t->dev = device;
+  t->n = n;
+  t->B = batch;
+  t->h = h;
+  auto bail = [&](int rc) {
+    trainer_release(t);
+    return rc;
+  };
+  HIPCHK(hipSetDevice(device));
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(FS_E_DEVICE, "hipStreamCreate failed"));
+  size_t off = 0;
+  for (int l = 0; l < 15; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    t->woff[l] = off;
+    off += (size_t)L.cout * L.k * L.k * L.cin;
+    t->boff[l] = off;
+    off += (size_t)L.cout;
+  }
+  t->P = off;
+  int rc;
+  for (float** q : {&t->par, &t->grad, &t->m1, &t->m2})
+    if ((rc = dalloc(q, t->P))) return bail(rc);
+  for (int l = 1; l < 14; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    if ((rc = dalloc(&t->wf[l], (size_t)L.cout * 9 * L.cin)) || (rc = dalloc(&t->wt[l], (size_t)L.cout * 9 * L.cin)))
+      return bail(rc);
+  }
+  const size_t nn = (size_t)n * n;
+  for (int a = 0; a < 14; a++) {
+    const GuessLayer& L = kGuessLayers[a];
+    size_t m = (size_t)(n >> L.level);
+    if ((rc = dalloc(&t->act[a], m * m * L.cout * batch)) || (rc = dalloc(&t->gz[a], m * m * L.cout * batch)))
+      return bail(rc);
+  }
+  // skip gradients of enc0b (16 at n), enc1b (32 at n/2), enc2b (64 at n/4)
+  if ((rc = dalloc(&t->gcat[0], nn * 16 * batch)) || (rc = dalloc(&t->gcat[1], nn / 4 * 32 * batch)) ||
+      (rc = dalloc(&t->gcat[2], nn / 16 * 64 * batch)))
+    return bail(rc);
+  t->pblk = (int)std::min<size_t>(256, (nn + 255) / 256);
+  t->oblk = (int)((nn * batch + 255) / 256);
+  if ((rc = dalloc(&t->pp, nn * batch)) || (rc = dalloc(&t->r32, nn * batch)) || (rc = dalloc(&t->tgt, nn * batch)) ||
+      (rc = dalloc(&t->err, nn * batch)) ||
+      (rc = dalloc(&t->scales, (size_t)2 * batch)) || (rc = dalloc(&t->mr, (size_t)t->pblk * batch)) ||
+      (rc = dalloc(&t->mp, (size_t)t->pblk * batch)) || (rc = dalloc(&t->opart, (size_t)t->oblk * 17)) ||
+      (rc = dalloc(&t->lpart, (size_t)t->oblk)) || (rc = dalloc(&t->loss_d, 1)) || (rc = dalloc(&t->idx_d, batch)) ||
+      (rc = halloc(&t->loss_h, 1)) || (rc = halloc(&t->idx_h, batch)))
+    return bail(rc);
+  size_t need = (size_t)304 * train_first_grid(t);
+  train_all_weights(t, true, &need);
+  t->part_cap = need;
+  if ((rc = dalloc(&t->part, need))) return bail(rc);
+  *out = t;
+  return FS_OK;
+}
+
+int fs_trainer_destroy(void* h) {
+  trainer_release((fs_trainer_t*)h);
+  return FS_OK;
+}
+
+int fs_trainer_param_count(void* h, size_t* count) {
+  if (!h || !count) return fail(FS_E_INVALID_ARG, "NULL argument");
+  *count = ((fs_trainer_t*)h)->P;
+  return FS_OK;
+}
+
+// Parameters from a weights blob (the fs_load_guess format); clears the Adam state.
+int fs_trainer_load(void* h, const void* blob, size_t bytes) {
+  if (!h || !blob) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  const unsigned char* b = (const unsigned char*)blob;
+  size_t off = 0;
+  auto take = [&](void* dst, size_t nb) -> bool {
+    if (off + nb > bytes) return false;
+    std::memcpy(dst, b + off, nb);
+    off += nb;
+    return true;
+  };
+  char magic[4];
+  uint32_t ver = 0, nl = 0;
+  if (!take(magic, 4) || std::memcmp(magic, "FSGN", 4) != 0 || !take(&ver, 4) || ver != 1 || !take(&nl, 4) || nl != 15)
+    return fail(FS_E_INVALID_ARG, "not a version-1 guess network blob");
+  std::vector<float> flat(t->P);
+  for (int l = 0; l < 15; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    uint32_t dims[3];
+    if (!take(dims, sizeof dims)) return fail(FS_E_INVALID_ARG, "blob truncated at layer %d", l);
+    if ((int)dims[0] != L.cin || (int)dims[1] != L.cout || (int)dims[2] != L.k)
+      return fail(FS_E_INVALID_ARG, "layer %d is %ux%u k%u, expected %dx%d k%d", l, dims[0], dims[1], dims[2], L.cin,
+                  L.cout, L.k);
+    size_t nw = t->boff[l] - t->woff[l];
+    if (!take(flat.data() + t->woff[l], (nw + L.cout) * 4)) return fail(FS_E_INVALID_ARG, "blob truncated at layer %d", l);
+  }
+  if (off != bytes) return fail(FS_E_INVALID_ARG, "%zu trailing bytes after the last layer", bytes - off);
+  for (float v : flat)
+    if (!std::isfinite(v)) return fail(FS_E_INVALID_ARG, "non-finite parameter in the blob");
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(t->par, flat.data(), t->P * 4, hipMemcpyHostToDevice, t->stream));
+  HIPCHK(hipMemsetAsync(t->m1, 0, t->P * 4, t->stream));
+  HIPCHK(hipMemsetAsync(t->m2, 0, t->P * 4, t->stream));
+  int rc = train_pack(t);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(t->stream));
+  t->adam_steps = 0;
+  t->loaded = true;
+  t->have_batch = false;
+  return FS_OK;
+}
+
+// The current parameters as a weights blob; *bytes is set to the size it needs.
+int fs_trainer_export(void* h, void* blob, size_t capacity, size_t* bytes) {
+  if (!h || !bytes) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  const size_t need = 12 + 15 * 12 + t->P * 4;
+  *bytes = need;
+  if (!blob) return FS_OK;
+  if (capacity < need) return fail(FS_E_INVALID_ARG, "the blob needs %zu bytes, got %zu", need, capacity);
+  if (!t->loaded) return fail(FS_E_INVALID_ARG, "no parameters loaded");
+  std::vector<float> flat(t->P);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(flat.data(), t->par, t->P * 4, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  unsigned char* o = (unsigned char*)blob;
+  uint32_t head[2] = {1, 15};
+  std::memcpy(o, "FSGN", 4);
+  std::memcpy(o + 4, head, 8);
+  size_t off = 12;
+  for (int l = 0; l < 15; l++) {
+    const GuessLayer& L = kGuessLayers[l];
+    uint32_t dims[3] = {(uint32_t)L.cin, (uint32_t)L.cout, (uint32_t)L.k};
+    std::memcpy(o + off, dims, 12);
+    off += 12;
+    size_t nb = (t->boff[l] + L.cout - t->woff[l]) * 4;
+    std::memcpy(o + off, flat.data() + t->woff[l], nb);
+    off += nb;
+  }
+  return FS_OK;
+}
+
+// Room for `count` pressure snapshots (FP64 n x n each, zeroed); replaces any held.
+int fs_trainer_reserve(void* h, size_t count) {
+  if (!h) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (count < 2) return fail(FS_E_INVALID_ARG, "a pair needs at least two snapshots");
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  dfree(t->snap);
+  t->snap = nullptr;
+  t->nsnap = 0;
+  const size_t cells = (size_t)t->n * t->n;
+  int rc = dalloc(&t->snap, count * cells);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(t->snap, 0, count * cells * sizeof(double), t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  t->nsnap = count;
+  t->have_batch = false;
+  return FS_OK;
+}
+
+// Snapshot `slot` from host memory (n x n FP64, row-major).
+int fs_trainer_store(void* h, size_t slot, const double* p, size_t cells) {
+  if (!h || !p) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (slot >= t->nsnap) return fail(FS_E_INVALID_ARG, "slot %zu of %zu reserved", slot, t->nsnap);
+  if (cells != (size_t)t->n * t->n) return fail(FS_E_INVALID_ARG, "a snapshot has %d^2 cells, got %zu", t->n, cells);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  HIPCHK(hipMemcpyAsync(t->snap + slot * cells, p, cells * sizeof(double), hipMemcpyHostToDevice, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FS_OK;
+}
+
+// Snapshot `slot` from a solver's current pressure, device to device (the solver is
+// n x n on the trainer's device; its queued work finishes first).
+int fs_trainer_store_pressure(void* h, size_t slot, void* solver) {
+  if (!h || !solver) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  fs_solver_t* s = (fs_solver_t*)solver;
+  if (slot >= t->nsnap) return fail(FS_E_INVALID_ARG, "slot %zu of %zu reserved", slot, t->nsnap);
+  if (s->nx != t->n || s->ny != t->n) return fail(FS_E_INVALID_ARG, "the solver is %dx%d, the trainer %dx%d", s->nx, s->ny, t->n, t->n);
+  if (s->dev != t->dev) return fail(FS_E_INVALID_ARG, "the solver is on device %d, the trainer on %d", s->dev, t->dev);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  const size_t cells = (size_t)t->n * t->n;
+  HIPCHK(hipMemcpyAsync(t->snap + slot * cells, s->p64[s->pidx], cells * sizeof(double), hipMemcpyDeviceToDevice,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+// Forward and backward pass over a batch of pairs: image b is
+// (p_prev, p) = (snapshot idx[b], snapshot idx[b] + 1). Leaves the loss and the
+// gradients on the device.
+int fs_trainer_batch(void* h, const int* idx, int count) {
+  if (!h || !idx) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (!t->loaded) return fail(FS_E_INVALID_ARG, "load parameters first (fs_trainer_load)");
+  if (count != t->B) return fail(FS_E_INVALID_ARG, "the trainer takes batches of %d pairs, got %d", t->B, count);
+  for (int b = 0; b < count; b++)
+    if (idx[b] < 0 || (size_t)idx[b] + 1 >= t->nsnap)
+      return fail(FS_E_INVALID_ARG, "pair %d starts at snapshot %d of %zu", b, idx[b], t->nsnap);
+  HIPCHK(hipSetDevice(t->dev));
+  // the previous batch's kernels may still read the index buffer
+  HIPCHK(hipStreamSynchronize(t->stream));
+  std::memcpy(t->idx_h, idx, sizeof(int) * count);
+  HIPCHK(hipMemcpyAsync(t->idx_d, t->idx_h, sizeof(int) * count, hipMemcpyHostToDevice, t->stream));
+  int rc = train_forward_backward(t);
+  if (rc) return rc;
+  t->have_batch = true;
+  return FS_OK;
+}
+
+// The last batch's loss (waits for it).
+int fs_trainer_loss(void* h, double* loss) {
+  if (!h || !loss) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (!t->have_batch) return fail(FS_E_INVALID_ARG, "no batch has been run");
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(t->loss_h, t->loss_d, sizeof(double), hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  *loss = *t->loss_h;
+  return FS_OK;
+}
+
+// The last batch's gradient, flat in blob order (count = fs_trainer_param_count).
+int fs_trainer_read_gradients(void* h, float* grad, size_t count) {
+  if (!h || !grad) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (!t->have_batch) return fail(FS_E_INVALID_ARG, "no batch has been run");
+  if (count != t->P) return fail(FS_E_INVALID_ARG, "the network has %zu parameters, got %zu", t->P, count);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(grad, t->grad, count * 4, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FS_OK;
+}
+
+// The last batch's output minus target, [batch][n][n].
+int fs_trainer_read_error(void* h, float* err, size_t count) {
+  if (!h || !err) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (!t->have_batch) return fail(FS_E_INVALID_ARG, "no batch has been run");
+  const size_t need = (size_t)t->n * t->n * t->B;
+  if (count != need) return fail(FS_E_INVALID_ARG, "the batch has %zu pixels, got %zu", need, count);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(err, t->err, count * 4, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FS_OK;
+}
+
+// The parameters, flat in blob order.
+int fs_trainer_read_params(void* h, float* params, size_t count) {
+  if (!h || !params) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (count != t->P) return fail(FS_E_INVALID_ARG, "the network has %zu parameters, got %zu", t->P, count);
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipMemcpyAsync(params, t->par, count * 4, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return FS_OK;
+}
+
+// One Adam update from the last batch's gradient, then the BF16 weight copies.
+int fs_trainer_adam(void* h, double lr, double beta1, double beta2, double eps) {
+  if (!h) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  if (!t->have_batch) return fail(FS_E_INVALID_ARG, "no batch has been run");
+  if (!(lr >= 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
+    return fail(FS_E_INVALID_ARG, "Adam needs lr >= 0, 0 <= beta < 1, eps > 0");
+  HIPCHK(hipSetDevice(t->dev));
+  t->adam_steps++;
+  const double bc1 = 1.0 - std::pow(beta1, (double)t->adam_steps), bc2 = 1.0 - std::pow(beta2, (double)t->adam_steps);
+  int rc = tlaunch(t, fsk::k_tr_adam, dim3((unsigned)((t->P + 255) / 256)), dim3(256), t->par, (const float*)t->grad,
+                   t->m1, t->m2, t->P, (float)(lr / bc1), (float)(1.0 / std::sqrt(bc2)), (float)beta1, (float)beta2,
+                   (float)eps);
+  if (rc) return rc;
+  return train_pack(t);
+}
+
+// Waits for the trainer's queued work.
+int fs_trainer_sync(void* h) {
+  if (!h) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_trainer_t* t = (fs_trainer_t*)h;
+  HIPCHK(hipSetDevice(t->dev));
+  HIPCHK(hipStreamSynchronize(t->stream));
   return FS_OK;
 }
 

@@ -154,6 +154,21 @@ _SIGNATURES = {
     "fs_batch_get_stats": (C.c_int, [_VP, C.POINTER(FsBatchStats)]),
     "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
+    "fs_trainer_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(_VP)]),
+    "fs_trainer_destroy": (C.c_int, [_VP]),
+    "fs_trainer_param_count": (C.c_int, [_VP, C.POINTER(C.c_size_t)]),
+    "fs_trainer_load": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
+    "fs_trainer_export": (C.c_int, [_VP, _VP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "fs_trainer_reserve": (C.c_int, [_VP, C.c_size_t]),
+    "fs_trainer_store": (C.c_int, [_VP, C.c_size_t, _VP, C.c_size_t]),
+    "fs_trainer_store_pressure": (C.c_int, [_VP, C.c_size_t, _VP]),
+    "fs_trainer_batch": (C.c_int, [_VP, C.POINTER(C.c_int), C.c_int]),
+    "fs_trainer_loss": (C.c_int, [_VP, C.POINTER(C.c_double)]),
+    "fs_trainer_read_gradients": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "fs_trainer_read_params": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "fs_trainer_read_error": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "fs_trainer_adam": (C.c_int, [_VP, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "fs_trainer_sync": (C.c_int, [_VP]),
 }
 
 _lib = None

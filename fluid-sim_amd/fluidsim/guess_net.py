@@ -1,9 +1,9 @@
 """The learned pressure guess: the network definition used for training, and the
 weights-blob format the HIP library loads (fs_load_guess).
 
-The library evaluates the network itself (fs_guess.hpp: BF16 matrix-core
-convolutions); torch is used only here, to train the weights offline and to
-restate the forward pass for the parity tests.
+The library evaluates and trains the network itself (fs_guess.hpp: BF16
+matrix-core convolutions; fs_train.hpp: loss, backward pass, Adam); torch is used
+only here, to restate the network for the tests.
 
 Network (channels 16/32/64/64, periodic 3x3 convolutions + ReLU, 2x2 average
 pooling down, nearest-neighbour upsampling and [upsampled, skip] concatenation
@@ -78,6 +78,18 @@ def export_blob(model) -> bytes:
         out.append(struct.pack("<III", cin, cout, k))
         out.append(np.ascontiguousarray(w.transpose(0, 2, 3, 1), np.float32).tobytes())
         out.append(np.ascontiguousarray(b, np.float32).tobytes())
+    return b"".join(out)
+
+
+def init_blob(seed: int) -> bytes:
+    """Initial weights blob: every weight and bias uniform in +-1/sqrt(cin k^2), seeded."""
+    rng = np.random.default_rng(seed)
+    out = [MAGIC, struct.pack("<II", 1, 15)]
+    for (ci, co), k in list(zip(LAYERS, [3] * len(LAYERS))) + [((16, 1), 1)]:
+        bound = 1.0 / np.sqrt(ci * k * k)
+        out.append(struct.pack("<III", ci, co, k))
+        out.append(rng.uniform(-bound, bound, co * k * k * ci).astype(np.float32).tobytes())
+        out.append(rng.uniform(-bound, bound, co).astype(np.float32).tobytes())
     return b"".join(out)
 
 

@@ -264,6 +264,35 @@ int fs_batch_read_dt(void* batch, double* dt, int count);   /* dt of the last st
 int fs_batch_get_stats(void* batch, fs_batch_stats* out);
 int fs_profile_read(void* solver, int kind, double* total_ms, int64_t* launches, double* bytes);
 
+/* trainer of the learned guess (new: the reference has no learned component). The
+ * network's forward pass is the guess's own kernels over a batch; the backward pass
+ * and Adam are the library's kernels. Parameters are FP32 in blob order (per layer
+ * w[cout][k*k][cin], b[cout]); the loss is the mean over the batch and the pixels
+ * of (o - t)^2 with t = (p - p_prev) / (max|r| h^2), r = L p - L p_prev. n is a
+ * multiple of 128, h the grid spacing of the pressure snapshots. */
+int fs_trainer_create(int device, int n, int batch, double h, void** trainer);
+int fs_trainer_destroy(void* trainer);
+int fs_trainer_param_count(void* trainer, size_t* count);
+/* parameters from a weights blob (fs_load_guess's format); clears the Adam moments */
+int fs_trainer_load(void* trainer, const void* blob, size_t bytes);
+/* the parameters as a weights blob; *bytes = the size needed (blob may be NULL) */
+int fs_trainer_export(void* trainer, void* blob, size_t capacity, size_t* bytes);
+/* the trainer's store of pressure snapshots (FP64 n x n each, device memory):
+ * room for `count` (zeroed), one from host memory, one from a solver's current
+ * pressure (device to device; same n and device) */
+int fs_trainer_reserve(void* trainer, size_t count);
+int fs_trainer_store(void* trainer, size_t slot, const double* p, size_t cells);
+int fs_trainer_store_pressure(void* trainer, size_t slot, void* solver);
+/* forward and backward over `count` (= batch) pairs (snapshot idx[b], snapshot idx[b] + 1) */
+int fs_trainer_batch(void* trainer, const int* idx, int count);
+int fs_trainer_loss(void* trainer, double* loss);                      /* of the last batch */
+int fs_trainer_read_gradients(void* trainer, float* grad, size_t count);
+int fs_trainer_read_params(void* trainer, float* params, size_t count);
+int fs_trainer_read_error(void* trainer, float* err, size_t count);   /* o - t, [batch][n][n] */
+/* one Adam step (PyTorch's formulation) from the last batch's gradient */
+int fs_trainer_adam(void* trainer, double lr, double beta1, double beta2, double eps);
+int fs_trainer_sync(void* trainer);
+
 #ifdef __cplusplus
 }
 #endif
