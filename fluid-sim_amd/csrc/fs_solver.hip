@@ -1217,6 +1217,16 @@ struct fs_batch_t {
   int64_t steps = 0, total_vcycles = 0;
   int last_vcycles = 0;
   double last_residual = 0;
+  // replayed step heads (graph mode): one executable graph per (predicted V-cycles,
+  // pressure parity, zero start, cfl); the launches depend on nothing else
+  struct Head {
+    int c, a0;
+    bool zero_p;
+    double cfl;
+    hipGraphExec_t exec;
+  };
+  std::vector<Head> heads;
+  int64_t graph_captures = 0, graph_replays = 0;
 };
 
 int bsync(fs_batch_t* b) {
@@ -1320,18 +1330,44 @@ int batch_read_rmax(fs_batch_t* b, int rows, std::vector<double>& rmax) {
   return FS_OK;
 }
 
-// Refinement until every simulation's FP64 residual max is within the tolerance:
-// the predicted number of V-cycles, one read-back, then single cycles if needed.
-int batch_project(fs_batch_t* b, int* cycles_out) {
-  const double tol = b->cfg.tolerance;
-  const int maxc = b->cfg.max_vcycles;
-  const bool zero_p = !(b->cfg.guess == FS_GUESS_PREVIOUS && b->have_p);
-  const int a0 = b->pidx;
+// The step up to the projection's first read-back: per-cavity dt, advection and
+// diffusion (u* lands back in u[cur]), then refinement iterations 0 .. c-1 and the
+// residual of iteration c. Launches only, so it can be captured as a graph.
+int batch_enqueue_head(fs_batch_t* b, double cfl, int c, int a0, bool zero_p) {
   int rc;
-  int c = std::max(0, std::min(b->last_pred, maxc));
+  const fsk::CavSim* S = b->sims;
+  if ((rc = blaunch(b, fsk::k_cav_dt, dim3(b->nsim), dim3(256), (const double*)b->mpart, b->sims, (const int*)b->blk0,
+                    b->nsim, cfl, 1.0)))
+    return rc;
+  const int c0 = b->cur;
+  if (b->cfg.advection == FS_ADV_MACCORMACK) {
+    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c0],
+                      (const float*)b->v[c0], b->fu, b->fv, S, (const int*)b->blk0, b->nsim)))
+      return rc;
+    if ((rc = blaunch(b, fsk::k_cav_advect_mc, dim3(b->nblocks), dim3(256), (const float*)b->u[c0],
+                      (const float*)b->v[c0], (const float*)b->fu, (const float*)b->fv, b->u[1 - c0], b->v[1 - c0], S,
+                      (const int*)b->blk0, b->nsim)))
+      return rc;
+  } else {
+    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c0],
+                      (const float*)b->v[c0], b->u[1 - c0], b->v[1 - c0], S, (const int*)b->blk0, b->nsim)))
+      return rc;
+  }
+  if ((rc = blaunch(b, fsk::k_cav_diffuse_solid, dim3(b->nblocks), dim3(256), (const float*)b->u[1 - c0],
+                    (const float*)b->v[1 - c0], b->u[c0], b->v[c0], S, (const int*)b->blk0, b->nsim,
+                    (float)b->cfg.mask_width, b->cfg.coupling_passes)))
+    return rc;
   for (int it = 0; it < c; it++)
     if ((rc = batch_iteration(b, it, (a0 + it) & 1, it == 0 && zero_p, false))) return rc;
-  if ((rc = batch_iteration(b, c, (a0 + c) & 1, c == 0 && zero_p, true))) return rc;
+  return batch_iteration(b, c, (a0 + c) & 1, c == 0 && zero_p, true);
+}
+
+// After the head: read the residual rows back; stop when every simulation's FP64
+// residual max is within the tolerance, else single further V-cycles.
+int batch_finish_projection(fs_batch_t* b, int c, int a0, bool zero_p, int* cycles_out) {
+  const double tol = b->cfg.tolerance;
+  const int maxc = b->cfg.max_vcycles;
+  int rc;
   std::vector<double> rmax;
   for (;;) {
     if ((rc = batch_read_rmax(b, c + 1, rmax))) return rc;
@@ -1363,10 +1399,44 @@ int batch_project(fs_batch_t* b, int* cycles_out) {
   }
 }
 
+// The head of a step as a replayed graph: captured on first use of its key.
+int batch_launch_head_graph(fs_batch_t* b, double cfl, int c, int a0, bool zero_p) {
+  for (const auto& h : b->heads)
+    if (h.c == c && h.a0 == a0 && h.zero_p == zero_p && h.cfl == cfl) {
+      HIPCHK(hipGraphLaunch(h.exec, b->stream));
+      b->graph_replays++;
+      return FS_OK;
+    }
+  if (b->heads.size() >= 64) {  // a changing cfl or prediction: start the cache afresh
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto& h : b->heads) (void)hipGraphExecDestroy(h.exec);
+    b->heads.clear();
+  }
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue_head(b, cfl, c, a0, zero_p);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? rc : fail(FS_E_DEVICE, "capturing the step failed: %s", hipGetErrorString(e));
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(FS_E_DEVICE, "instantiating the step graph failed: %s", hipGetErrorString(e));
+  b->heads.push_back({c, a0, zero_p, cfl, exec});
+  b->graph_captures++;
+  HIPCHK(hipGraphLaunch(exec, b->stream));
+  b->graph_replays++;
+  return FS_OK;
+}
+
 void batch_release(fs_batch_t* b) {
   if (!b) return;
   (void)hipSetDevice(b->dev);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& h : b->heads) (void)hipGraphExecDestroy(h.exec);
+  b->heads.clear();
   for (int k = 0; k < 2; k++) {
     dfree(b->u[k]);
     dfree(b->v[k]);
@@ -2360,6 +2430,7 @@ void fs_batch_config_default(fs_batch_config* c) {
   c->guess = FS_GUESS_ZERO;
   c->device = 0;
   c->mg_fused = 1;
+  c->graph = 1;
 }
 
 int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count, void** out) {
@@ -2495,35 +2566,17 @@ int fs_batch_step_cfl(void* h, double cfl) {
   if (!(cfl > 0) || !std::isfinite(cfl)) return fail(FS_E_INVALID_ARG, "cfl must be finite and > 0");
   HIPCHK(hipSetDevice(b->dev));
   int rc;
-  const fsk::CavSim* S = b->sims;
-  if ((rc = blaunch(b, fsk::k_cav_dt, dim3(b->nsim), dim3(256), (const double*)b->mpart, b->sims, (const int*)b->blk0,
-                    b->nsim, cfl, 1.0)))
-    return rc;
-  int c = b->cur;
-  if (b->cfg.advection == FS_ADV_MACCORMACK) {
-    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
-                      (const float*)b->v[c], b->fu, b->fv, S, (const int*)b->blk0, b->nsim)))
-      return rc;
-    if ((rc = blaunch(b, fsk::k_cav_advect_mc, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
-                      (const float*)b->v[c], (const float*)b->fu, (const float*)b->fv, b->u[1 - c], b->v[1 - c], S,
-                      (const int*)b->blk0, b->nsim)))
-      return rc;
-  } else {
-    if ((rc = blaunch(b, fsk::k_cav_advect_sl, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
-                      (const float*)b->v[c], b->u[1 - c], b->v[1 - c], S, (const int*)b->blk0, b->nsim)))
-      return rc;
-  }
-  c = 1 - c;
-  if ((rc = blaunch(b, fsk::k_cav_diffuse_solid, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
-                    (const float*)b->v[c], b->u[1 - c], b->v[1 - c], S, (const int*)b->blk0, b->nsim,
-                    (float)b->cfg.mask_width, b->cfg.coupling_passes)))
-    return rc;
-  b->cur = 1 - c;
+  const bool zero_p = !(b->cfg.guess == FS_GUESS_PREVIOUS && b->have_p);
+  const int a0 = b->pidx;
+  const int c = std::max(0, std::min(b->last_pred, b->cfg.max_vcycles));
+  rc = b->cfg.graph ? batch_launch_head_graph(b, cfl, c, a0, zero_p) : batch_enqueue_head(b, cfl, c, a0, zero_p);
+  if (rc) return rc;
   int cycles = 0;
-  int prc = batch_project(b, &cycles);
+  int prc = batch_finish_projection(b, c, a0, zero_p, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
   if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3(b->nblocks), dim3(256), b->u[b->cur], b->v[b->cur],
-                    (const double*)b->p64[b->pidx], S, (const int*)b->blk0, b->nsim, b->mpart)))
+                    (const double*)b->p64[b->pidx], (const fsk::CavSim*)b->sims, (const int*)b->blk0, b->nsim,
+                    b->mpart)))
     return rc;
   b->have_p = true;
   b->last_vcycles = cycles;
@@ -2572,6 +2625,8 @@ int fs_batch_get_stats(void* h, fs_batch_stats* out) {
   out->last_vcycles = b->last_vcycles;
   out->total_vcycles = b->total_vcycles;
   out->last_residual = b->last_residual;
+  out->graph_captures = b->graph_captures;
+  out->graph_replays = b->graph_replays;
   return FS_OK;
 }
 

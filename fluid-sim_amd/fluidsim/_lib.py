@@ -94,12 +94,14 @@ class FsCavity(C.Structure):
 class FsBatchConfig(C.Structure):
     _fields_ = [("advection", C.c_int), ("tolerance", C.c_double), ("max_vcycles", C.c_int),
                 ("mg_pre", C.c_int), ("mg_post", C.c_int), ("coupling_passes", C.c_int),
-                ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int), ("mg_fused", C.c_int)]
+                ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int), ("mg_fused", C.c_int),
+                ("graph", C.c_int)]
 
 
 class FsBatchStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("cells", C.c_int64), ("last_vcycles", C.c_int),
-                ("total_vcycles", C.c_int64), ("last_residual", C.c_double)]
+                ("total_vcycles", C.c_int64), ("last_residual", C.c_double), ("graph_captures", C.c_int64),
+                ("graph_replays", C.c_int64)]
 
 
 class FluidSimError(RuntimeError):

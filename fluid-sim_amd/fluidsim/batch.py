@@ -38,7 +38,7 @@ class CavityBatch:
     """A batch of cavities: list of dict(n, lid, nu, disk=None | dict(x, y, r))."""
 
     def __init__(self, cavities, device=0, advection="maccormack", tolerance=1e-6, maxVcycles=60, mgPre=2,
-                 mgPost=2, couplingPasses=2, maskWidth=1.5, guess="zero", mgFused=True):
+                 mgPost=2, couplingPasses=2, maskWidth=1.5, guess="zero", mgFused=True, graph=True):
         if advection not in _lib.ADVECTION:
             raise ValueError(f"unknown advection scheme {advection}")
         if guess not in ("zero", "previous"):
@@ -53,6 +53,7 @@ class CavityBatch:
         cfg.guess = _lib.GUESS[guess]
         cfg.device = int(device)
         cfg.mg_fused = 1 if mgFused else 0
+        cfg.graph = 1 if graph else 0
         arr = (FsCavity * len(cavities))()
         for a, c in zip(arr, cavities):
             a.n, a.lid, a.nu = int(c["n"]), float(c["lid"]), float(c["nu"])

@@ -149,6 +149,9 @@ typedef struct fs_batch_config {
   int device;
   int mg_fused;               /* 1 (default): LDS-tiled level kernels; 0: one launch per operator.
                                  Same bits. */
+  int graph;                  /* 1 (default): each step's launches up to the projection's read-back
+                                 replay as a captured hipGraph (one per predicted V-cycle count,
+                                 pressure parity and cfl); 0: launched one by one. Same bits. */
 } fs_batch_config;
 
 typedef struct fs_batch_stats {
@@ -157,6 +160,8 @@ typedef struct fs_batch_stats {
   int last_vcycles;           /* V-cycles of the last projection (every cavity runs them) */
   int64_t total_vcycles;
   double last_residual;       /* max over cavities of the FP64 residual */
+  int64_t graph_captures;     /* step graphs captured so far */
+  int64_t graph_replays;      /* steps whose head ran as a graph */
 } fs_batch_stats;
 
 int         fs_abi_version(void);

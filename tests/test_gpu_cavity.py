@@ -93,3 +93,26 @@ def test_fused_cavity_multigrid_gives_the_same_bits(fs, pre, post):
     for a, c in zip(*outs):
         assert np.array_equal(a["u"], c["u"]) and np.array_equal(a["v"], c["v"])
     assert cyc > 0
+
+
+@pytest.mark.parametrize("guess", ["zero", "previous"])
+def test_graph_replay_gives_the_same_bits(fs, guess):
+    # each step's head replayed as a captured hipGraph against the same launches one
+    # by one: every step compared, with reads (synchronisations) between the replays
+    cavs = [dict(n=256, lid=1.0, nu=1e-3, disk=dict(x=0.4, y=0.5, r=0.1)),
+            dict(n=64, lid=2.0, nu=4e-4, disk=dict(x=0.5, y=0.5, r=0.2)),
+            dict(n=512, lid=0.8, nu=4e-4, disk=None),
+            dict(n=128, lid=1.5, nu=2e-3, disk=None)]
+    steps = 8
+    with fs.CavityBatch(cavs, tolerance=1e-7, guess=guess, graph=True) as g, \
+            fs.CavityBatch(cavs, tolerance=1e-7, guess=guess, graph=False) as e:
+        for s in range(steps):
+            g.stepCFL(0.5)
+            e.stepCFL(0.5)
+            for k in range(len(cavs)):
+                a, c = g.readVelocity(k), e.readVelocity(k)
+                assert np.array_equal(a["u"], c["u"]) and np.array_equal(a["v"], c["v"]), (s, k)
+            assert g.stats()["last_vcycles"] == e.stats()["last_vcycles"]
+        sg, se = g.stats(), e.stats()
+    assert sg["graph_replays"] == steps and se["graph_replays"] == 0
+    assert 1 <= sg["graph_captures"] < steps          # heads are reused across steps
