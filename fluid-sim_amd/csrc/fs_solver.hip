@@ -23,6 +23,7 @@
 //                       over-prediction costs time, never accuracy.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+#include <chrono>
 
 #include <algorithm>
 #include <cmath>
@@ -1227,6 +1228,17 @@ struct fs_batch_t {
   };
   std::vector<Head> heads;
   int64_t graph_captures = 0, graph_replays = 0;
+  // level-kernel tile shape (TileKind) per size class 2^lg and direction (0 down, 1 up),
+  // the autotuner's timings per shape (us, 0: not timed) and its sweep trials
+  int tile[13][2] = {};
+  float tune_us[13][2][TILE_KINDS] = {};
+  struct SweepTrial {
+    int pre, post;
+    double ms_per_step;
+    int vcycles;
+  };
+  std::vector<SweepTrial> sweep_trials;
+  long off = 0, lvl_cells = 0;  // cells of the velocity and level buffers
 };
 
 int bsync(fs_batch_t* b) {
@@ -1242,6 +1254,39 @@ int blaunch(fs_batch_t* b, K kern, dim3 g, dim3 t, Args... args) {
 }
 
 inline unsigned nblk(long items) { return (unsigned)((items + 255) / 256); }
+
+// LDS-tiled Neumann level kernels of size class s over its cnt grids, tile shape TS
+template <class TS>
+int nmt_down_t(fs_batch_t* b, int sw, float* x, const float* r, int s, long cnt, float* cr) {
+  const int tiles_x = s / TS::TX, tpg = tiles_x * (s / TS::TY);
+  const dim3 g((unsigned)(cnt * tpg)), t(TS::NT);
+  const float h2 = 1.f / ((float)s * s), ih2 = (float)((double)s * s);
+  if (sw == 1) return blaunch(b, fsk::k_nmt_down<1, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
+  if (sw == 2) return blaunch(b, fsk::k_nmt_down<2, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
+  return blaunch(b, fsk::k_nmt_down<3, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
+}
+template <class TS>
+int nmt_up_t(fs_batch_t* b, int sw, const float* x, float* xb, const float* r, const float* xc, int s, long cnt) {
+  const int tiles_x = s / TS::TX, tpg = tiles_x * (s / TS::TY);
+  const dim3 g((unsigned)(cnt * tpg)), t(TS::NT);
+  const float h2 = 1.f / ((float)s * s);
+  if (sw == 1) return blaunch(b, fsk::k_nmt_up<1, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
+  if (sw == 2) return blaunch(b, fsk::k_nmt_up<2, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
+  return blaunch(b, fsk::k_nmt_up<3, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
+}
+int nmt_down(fs_batch_t* b, int kind, int sw, float* x, const float* r, int s, long cnt, float* cr) {
+  return kind == TILE64 ? nmt_down_t<fsk::Tile64>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE32 ? nmt_down_t<fsk::Tile32>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE16 ? nmt_down_t<fsk::Tile16>(b, sw, x, r, s, cnt, cr)
+                        : nmt_down_t<fsk::Tile8>(b, sw, x, r, s, cnt, cr);
+}
+int nmt_up(fs_batch_t* b, int kind, int sw, const float* x, float* xb, const float* r, const float* xc, int s,
+           long cnt) {
+  return kind == TILE64 ? nmt_up_t<fsk::Tile64>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE32 ? nmt_up_t<fsk::Tile32>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE16 ? nmt_up_t<fsk::Tile16>(b, sw, x, xb, r, xc, s, cnt)
+                        : nmt_up_t<fsk::Tile8>(b, sw, x, xb, r, xc, s, cnt);
+}
 
 // refinement iteration `it`: residual of p64[pin] (+ the finest corrections),
 // then (unless residual_only) one V-cycle over every size class
@@ -1262,14 +1307,7 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
     const float* r = b->lr + b->base[lg];
     const float h2 = 1.f / ((float)s * s);
     if (b->fused) {
-      const int tiles_x = s / fsk::Tile32::TX, tpg = tiles_x * (s / fsk::Tile32::TY);
-      dim3 g((unsigned)(cnt * tpg));
-      float ih2 = (float)((double)s * s);
-      float* cr = b->lr + b->base[lg - 1];
-      rc = pre == 1 ? blaunch(b, fsk::k_nmt_down<1, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr)
-         : pre == 2 ? blaunch(b, fsk::k_nmt_down<2, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr)
-                    : blaunch(b, fsk::k_nmt_down<3, fsk::Tile32>, g, dim3(256), x, r, s, tiles_x, tpg, h2, ih2, cr);
-      if (rc) return rc;
+      if ((rc = nmt_down(b, b->tile[lg][0], pre, x, r, s, cnt, b->lr + b->base[lg - 1]))) return rc;
       continue;
     }
     const long half = cnt * s * (s / 2);
@@ -1298,12 +1336,7 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
     const float* xc = b->lxb + b->base[lg - 1];
     const float h2 = 1.f / ((float)s * s);
     if (b->fused) {
-      const int tiles_x = s / fsk::Tile32::TX, tpg = tiles_x * (s / fsk::Tile32::TY);
-      dim3 g((unsigned)(cnt * tpg));
-      rc = post == 1 ? blaunch(b, fsk::k_nmt_up<1, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2)
-         : post == 2 ? blaunch(b, fsk::k_nmt_up<2, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2)
-                     : blaunch(b, fsk::k_nmt_up<3, fsk::Tile32>, g, dim3(256), x, xb, r, xc, s, tiles_x, tpg, h2);
-      if (rc) return rc;
+      if ((rc = nmt_up(b, b->tile[lg][1], post, x, xb, r, xc, s, cnt))) return rc;
       continue;
     }
     const long half = cnt * s * (s / 2);
@@ -1429,6 +1462,105 @@ int batch_launch_head_graph(fs_batch_t* b, double cfl, int c, int a0, bool zero_
   HIPCHK(hipGraphLaunch(exec, b->stream));
   b->graph_replays++;
   return FS_OK;
+}
+
+// The batch at rest: zero velocities, pressures and level buffers, step counters reset.
+int batch_reset(fs_batch_t* b) {
+  const long off = b->off, lvl = b->lvl_cells;
+  for (float* a : {b->u[0], b->v[0], b->u[1], b->v[1], b->fu, b->fv})
+    fsk::k_fill_f32<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.f);
+  for (double* a : {b->p64[0], b->p64[1]}) fsk::k_fill_f64<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.0);
+  for (float* a : {b->lx, b->lr, b->lxb}) fsk::k_fill_f32<<<nblk(lvl), 256, 0, b->stream>>>(a, (size_t)lvl, 0.f);
+  fsk::k_fill_f64<<<nblk(2L * b->nblocks), 256, 0, b->stream>>>(b->mpart, (size_t)2 * b->nblocks, 0.0);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+    return fail(FS_E_DEVICE, "filling the batch state failed");
+  b->cur = 0;
+  b->pidx = 0;
+  b->have_p = false;
+  b->last_pred = 2;
+  b->steps = b->total_vcycles = 0;
+  b->last_vcycles = 0;
+  b->last_residual = 0;
+  return FS_OK;
+}
+
+// Tile shape of each size class's down and up kernels: every shape computes the
+// same bits, so each is timed (five launches after one untimed) on the level buffers
+// and the fastest kept.
+int batch_autotune_tiles(fs_batch_t* b) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FS_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = fail(FS_E_DEVICE, "hipEventCreate failed");
+  for (int lg = b->max_lg; lg > 6 && !rc; lg--) {
+    const long cnt = b->count[lg];
+    const int s = 1 << lg;
+    float* x = b->lx + b->base[lg];
+    float* xb = b->lxb + b->base[lg];
+    const float* r = b->lr + b->base[lg];
+    float* cr = b->lr + b->base[lg - 1];
+    const float* xc = b->lxb + b->base[lg - 1];
+    for (int dir = 0; dir < 2 && !rc; dir++) {
+      auto run = [&](int kind) {
+        return dir == 0 ? nmt_down(b, kind, b->cfg.mg_pre, x, r, s, cnt, cr)
+                        : nmt_up(b, kind, b->cfg.mg_post, x, xb, r, xc, s, cnt);
+      };
+      float best = 0;
+      for (int k = TILE64; k < TILE_KINDS && !rc; k++) {
+        if ((rc = run(k))) break;
+        if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
+        for (int q = 0; q < 5 && !rc; q++) rc = run(k);
+        if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
+        float ms = 0;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = fail(FS_E_DEVICE, "autotune timing failed");
+        if (rc) break;
+        b->tune_us[lg][dir][k] = ms * 1000.f / 5.f;
+        if (best == 0 || b->tune_us[lg][dir][k] < best) {
+          best = b->tune_us[lg][dir][k];
+          b->tile[lg][dir] = k;
+        }
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = batch_reset(b);
+  return rc;
+}
+
+// Sweeps per V-cycle: each (pre, post) in {1, 2, 3}^2 runs the batch from rest for
+// four steps at CFL 0.5 (launched one by one); the wall time of steps 2-4 decides.
+// The choice changes the bits (every choice meets the same tolerance).
+int batch_autotune_sweeps(fs_batch_t* b) {
+  const int graph = b->cfg.graph;
+  b->cfg.graph = 0;
+  int rc = FS_OK;
+  double best = 0;
+  int bp = b->cfg.mg_pre, bq = b->cfg.mg_post;
+  for (int pre = 1; pre <= 3 && !rc; pre++)
+    for (int post = 1; post <= 3 && !rc; post++) {
+      b->cfg.mg_pre = pre;
+      b->cfg.mg_post = post;
+      if ((rc = batch_reset(b)) || (rc = ::fs_batch_step_cfl(b, 0.5))) break;
+      auto t0 = std::chrono::steady_clock::now();
+      int64_t v0 = b->total_vcycles;
+      for (int k = 0; k < 3 && !rc; k++) rc = ::fs_batch_step_cfl(b, 0.5);
+      if (!rc) rc = bsync(b);
+      if (rc) break;
+      double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 3.0;
+      b->sweep_trials.push_back({pre, post, ms, (int)(b->total_vcycles - v0)});
+      if (best == 0 || ms < best) {
+        best = ms;
+        bp = pre;
+        bq = post;
+      }
+    }
+  b->cfg.mg_pre = bp;
+  b->cfg.mg_post = bq;
+  b->cfg.graph = graph;
+  if (!rc) rc = batch_reset(b);
+  return rc;
 }
 
 void batch_release(fs_batch_t* b) {
@@ -2431,6 +2563,8 @@ void fs_batch_config_default(fs_batch_config* c) {
   c->device = 0;
   c->mg_fused = 1;
   c->graph = 1;
+  c->autotune = 1;
+  c->tile = 0;
 }
 
 int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count, void** out) {
@@ -2448,6 +2582,8 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (!(c.mask_width > 0)) return fail(FS_E_INVALID_ARG, "mask_width must be > 0");
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
     return fail(FS_E_INVALID_ARG, "a batch takes the zero or the previous pressure guess");
+  if (c.autotune < 0 || c.autotune > 2) return fail(FS_E_INVALID_ARG, "autotune must be 0, 1 or 2");
+  if (c.tile < 0 || c.tile > TILE_KINDS) return fail(FS_E_INVALID_ARG, "tile must be in 0..%d", TILE_KINDS);
   for (int k = 0; k < count; k++) {
     const fs_cavity& q = sims[k];
     if (q.n < 64 || q.n > 4096 || (q.n & (q.n - 1)))
@@ -2540,17 +2676,13 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   HIPCHK(hipMemcpy(b->cbase, cbase.data(), sizeof(long) * count, hipMemcpyHostToDevice));
   std::vector<float> pinv = neumann_pinv4();
   HIPCHK(hipMemcpy(b->pinv4, pinv.data(), sizeof(float) * 256, hipMemcpyHostToDevice));
-  // state at rest
-  for (float* a : {b->u[0], b->v[0], b->u[1], b->v[1], b->fu, b->fv}) {
-    fsk::k_fill_f32<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.f);
-  }
-  for (double* a : {b->p64[0], b->p64[1]}) fsk::k_fill_f64<<<nblk(off), 256, 0, b->stream>>>(a, (size_t)off, 0.0);
-  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lx, (size_t)lvl_cells, 0.f);
-  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lr, (size_t)lvl_cells, 0.f);
-  fsk::k_fill_f32<<<nblk(lvl_cells), 256, 0, b->stream>>>(b->lxb, (size_t)lvl_cells, 0.f);
-  fsk::k_fill_f64<<<nblk(2L * b->nblocks), 256, 0, b->stream>>>(b->mpart, (size_t)2 * b->nblocks, 0.0);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
-    return bail(fail(FS_E_DEVICE, "initial fill failed"));
+  b->off = off;
+  b->lvl_cells = lvl_cells;
+  const int forced = c.tile >= 1 ? c.tile - 1 : -1;
+  for (int lg = 0; lg < 13; lg++) b->tile[lg][0] = b->tile[lg][1] = forced >= 0 ? forced : TILE32;
+  if ((rc = batch_reset(b))) return bail(rc);
+  if (c.autotune >= 1 && forced < 0 && b->fused && (rc = batch_autotune_tiles(b))) return bail(rc);
+  if (c.autotune >= 2 && b->fused && (rc = batch_autotune_sweeps(b))) return bail(rc);
   *out = b;
   return FS_OK;
 }
@@ -2628,6 +2760,37 @@ int fs_batch_get_stats(void* h, fs_batch_stats* out) {
   out->graph_captures = b->graph_captures;
   out->graph_replays = b->graph_replays;
   return FS_OK;
+}
+
+int fs_batch_tune_report(void* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return fail(FS_E_INVALID_ARG, "NULL argument");
+  fs_batch_t* b = (fs_batch_t*)h;
+  std::string r;
+  char line[256];
+  for (int lg = b->max_lg; lg > 6; lg--)
+    for (int dir = 0; dir < 2; dir++) {
+      snprintf(line, sizeof line, "class %d^2 (%ld grids) %s: %s", 1 << lg, b->count[lg], dir ? "up" : "down",
+               kTileName[b->tile[lg][dir]]);
+      r += line;
+      if (b->tune_us[lg][dir][0] > 0) {
+        r += " [";
+        for (int k = 0; k < TILE_KINDS; k++) {
+          snprintf(line, sizeof line, "%s%s %.1f us", k ? ", " : "", kTileName[k], b->tune_us[lg][dir][k]);
+          r += line;
+        }
+        r += "]";
+      }
+      r += "\n";
+    }
+  snprintf(line, sizeof line, "sweeps: pre %d, post %d\n", b->cfg.mg_pre, b->cfg.mg_post);
+  r += line;
+  for (const auto& t : b->sweep_trials) {
+    snprintf(line, sizeof line, "  trial pre %d post %d: %.3f ms/step, %d V-cycles in 3 steps\n", t.pre, t.post,
+             t.ms_per_step, t.vcycles);
+    r += line;
+  }
+  snprintf(buf, len, "%s", r.c_str());
+  return r.size() < len ? FS_OK : fail(FS_E_INVALID_ARG, "report truncated: needs %zu bytes", r.size() + 1);
 }
 
 // ------------------------------------------------------------ guess trainer

@@ -95,7 +95,7 @@ class FsBatchConfig(C.Structure):
     _fields_ = [("advection", C.c_int), ("tolerance", C.c_double), ("max_vcycles", C.c_int),
                 ("mg_pre", C.c_int), ("mg_post", C.c_int), ("coupling_passes", C.c_int),
                 ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int), ("mg_fused", C.c_int),
-                ("graph", C.c_int)]
+                ("graph", C.c_int), ("autotune", C.c_int), ("tile", C.c_int)]
 
 
 class FsBatchStats(C.Structure):
@@ -154,6 +154,7 @@ _SIGNATURES = {
     "fs_batch_read_velocity": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_size_t]),
     "fs_batch_read_dt": (C.c_int, [_VP, C.POINTER(C.c_double), C.c_int]),
     "fs_batch_get_stats": (C.c_int, [_VP, C.POINTER(FsBatchStats)]),
+    "fs_batch_tune_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),
     "fs_profile_read": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_double)]),
     "fs_trainer_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(_VP)]),

@@ -38,7 +38,8 @@ class CavityBatch:
     """A batch of cavities: list of dict(n, lid, nu, disk=None | dict(x, y, r))."""
 
     def __init__(self, cavities, device=0, advection="maccormack", tolerance=1e-6, maxVcycles=60, mgPre=2,
-                 mgPost=2, couplingPasses=2, maskWidth=1.5, guess="zero", mgFused=True, graph=True):
+                 mgPost=2, couplingPasses=2, maskWidth=1.5, guess="zero", mgFused=True, graph=True,
+                 autotune=1, tile=None):
         if advection not in _lib.ADVECTION:
             raise ValueError(f"unknown advection scheme {advection}")
         if guess not in ("zero", "previous"):
@@ -54,6 +55,8 @@ class CavityBatch:
         cfg.device = int(device)
         cfg.mg_fused = 1 if mgFused else 0
         cfg.graph = 1 if graph else 0
+        cfg.autotune = int(autotune)
+        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4}[tile]
         arr = (FsCavity * len(cavities))()
         for a, c in zip(arr, cavities):
             a.n, a.lid, a.nu = int(c["n"]), float(c["lid"]), float(c["nu"])
@@ -90,6 +93,11 @@ class CavityBatch:
         dt = (C.c_double * len(self.sizes))()
         check(self._L.fs_batch_read_dt(self.handle, dt, len(self.sizes)))
         return np.array(dt[:])
+
+    def tuneReport(self) -> str:
+        buf = C.create_string_buffer(8192)
+        check(self._L.fs_batch_tune_report(self.handle, buf, len(buf)))
+        return buf.value.decode()
 
     def stats(self):
         st = FsBatchStats()

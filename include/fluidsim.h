@@ -152,6 +152,12 @@ typedef struct fs_batch_config {
   int graph;                  /* 1 (default): each step's launches up to the projection's read-back
                                  replay as a captured hipGraph (one per predicted V-cycle count,
                                  pressure parity and cfl); 0: launched one by one. Same bits. */
+  int autotune;               /* 0: 32x32 tiles, the configured sweeps; 1 (default): the tile shape of
+                                 each size class's level kernels timed at create (same bits);
+                                 2: also (pre, post) in {1,2,3}^2 timed over steps from rest at
+                                 create (the choice changes the bits) */
+  int tile;                   /* 0 (default): as autotune says; 1..4: every size class uses
+                                 64x32, 32x32, 16x16 or 8x8 tiles (no tile timing). Same bits. */
 } fs_batch_config;
 
 typedef struct fs_batch_stats {
@@ -267,6 +273,8 @@ int fs_batch_sync(void* batch);
 int fs_batch_read_velocity(void* batch, int cavity, float* u, float* v, size_t count);
 int fs_batch_read_dt(void* batch, double* dt, int count);   /* dt of the last step, per cavity */
 int fs_batch_get_stats(void* batch, fs_batch_stats* out);
+/* the tile shape per size class and the sweeps chosen, with the autotuner's timings, as text */
+int fs_batch_tune_report(void* batch, char* buf, size_t len);
 int fs_profile_read(void* solver, int kind, double* total_ms, int64_t* launches, double* bytes);
 
 /* trainer of the learned guess (new: the reference has no learned component). The

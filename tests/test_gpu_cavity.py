@@ -116,3 +116,38 @@ def test_graph_replay_gives_the_same_bits(fs, guess):
         sg, se = g.stats(), e.stats()
     assert sg["graph_replays"] == steps and se["graph_replays"] == 0
     assert 1 <= sg["graph_captures"] < steps          # heads are reused across steps
+
+
+@pytest.mark.parametrize("tile", ["64x32", "32x32", "16x16", "8x8"])
+def test_every_cavity_tile_shape_gives_the_same_bits(fs, tile):
+    # what makes the tile autotuner safe: each shape against the one-pass kernels
+    cavs = [dict(n=512, lid=1.0, nu=1e-3, disk=dict(x=0.4, y=0.5, r=0.1)),
+            dict(n=128, lid=1.5, nu=2e-3, disk=None),
+            dict(n=256, lid=0.8, nu=4e-4, disk=dict(x=0.7, y=0.3, r=0.15))]
+    outs = []
+    for fused in (True, False):
+        with fs.CavityBatch(cavs, tolerance=1e-7, mgFused=fused, tile=tile if fused else None) as b:
+            for _ in range(3):
+                b.stepCFL(0.5)
+            outs.append([b.readVelocity(k) for k in range(len(cavs))])
+            if fused:
+                assert b.tuneReport().count(tile) >= 2 * 3  # down and up of classes 512, 256, 128
+    for a, c in zip(*outs):
+        assert np.array_equal(a["u"], c["u"]) and np.array_equal(a["v"], c["v"])
+
+
+def test_cavity_autotuner_picks_tiles_and_sweeps(fs):
+    cavs = fs.cavities_c5(32, seed=2)
+    with fs.CavityBatch(cavs, tolerance=1e-6, autotune=2) as b:
+        rep = b.tuneReport()
+        st = b.stats()
+        assert st["steps"] == 0 and st["total_vcycles"] == 0          # handed back at rest
+        u = b.readVelocity(0)
+        assert not np.any(u["u"]) and not np.any(u["v"])
+        for _ in range(3):
+            b.stepCFL(0.5)
+            assert b.stats()["last_residual"] <= 1e-6
+    print(rep)
+    assert rep.count("trial pre") == 9
+    for lg in (512, 256, 128):
+        assert f"class {lg}^2" in rep and "us," in rep
