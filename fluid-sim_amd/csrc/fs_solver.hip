@@ -1043,6 +1043,8 @@ int build_levels(fs_solver_t* s) {
 // warm-up and five timed launches on the zeroed buffers of a new solver) and keep
 // the fastest. Level 0's down kernel is the fused residual + down-sweep; its
 // per-tile partials go to a scratch row sized for the finest shape.
+constexpr int kTuneRounds = 3, kTuneLaunches = 10;
+
 int autotune_tiles(fs_solver_t* s) {
   const int cs = s->coarse_start;
   if (!s->fused || cs == 0) return FS_OK;
@@ -1062,23 +1064,26 @@ int autotune_tiles(fs_solver_t* s) {
         if (l == 0) return DISPATCH_SW(pre, launch_down0, s, 0, 0, true, false);
         return DISPATCH_SW(pre, launch_down, s, l);
       };
-      int best = L.tile[dir];
-      float best_us = 0;
-      for (int k = TILE64; k < TILE_KINDS && !rc; k++) {
-        L.tile[dir] = k;
-        if ((rc = run())) break;
-        if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
-        for (int r = 0; r < 5 && !rc; r++) rc = run();
-        if (!rc && hipEventRecord(b, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
-        float ms = 0;
-        if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess))
-          rc = fail(FS_E_DEVICE, "autotune timing failed");
-        L.tune_us[dir][k] = ms * 1000.f / 5.f;
-        if (!rc && (best_us == 0 || L.tune_us[dir][k] < best_us)) {
-          best_us = L.tune_us[dir][k];
-          best = k;
+      // three rounds over the shapes (order rotated each round), ten launches each
+      // after an untimed one; a shape's time is its best round
+      for (int k = 0; k < TILE_KINDS; k++) L.tune_us[dir][k] = 0;
+      for (int round = 0; round < kTuneRounds && !rc; round++)
+        for (int j = 0; j < TILE_KINDS && !rc; j++) {
+          const int k = (j + round) % TILE_KINDS;
+          L.tile[dir] = k;
+          if ((rc = run())) break;
+          if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
+          for (int r = 0; r < kTuneLaunches && !rc; r++) rc = run();
+          if (!rc && hipEventRecord(b, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
+          float ms = 0;
+          if (!rc && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = fail(FS_E_DEVICE, "autotune timing failed");
+          const float us = ms * 1000.f / kTuneLaunches;
+          if (!rc && (L.tune_us[dir][k] == 0 || us < L.tune_us[dir][k])) L.tune_us[dir][k] = us;
         }
-      }
+      int best = TILE32;
+      for (int k = 0; k < TILE_KINDS; k++)
+        if (L.tune_us[dir][k] > 0 && L.tune_us[dir][k] < L.tune_us[dir][best]) best = k;
       L.tile[dir] = best;
     }
   }
@@ -1485,8 +1490,8 @@ int batch_reset(fs_batch_t* b) {
 }
 
 // Tile shape of each size class's down and up kernels: every shape computes the
-// same bits, so each is timed (five launches after one untimed) on the level buffers
-// and the fastest kept.
+// same bits, so each is timed on the level buffers (best of three rounds of ten
+// launches, the order rotated each round) and the fastest kept.
 int batch_autotune_tiles(fs_batch_t* b) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = FS_OK;
@@ -1505,22 +1510,23 @@ int batch_autotune_tiles(fs_batch_t* b) {
         return dir == 0 ? nmt_down(b, kind, b->cfg.mg_pre, x, r, s, cnt, cr)
                         : nmt_up(b, kind, b->cfg.mg_post, x, xb, r, xc, s, cnt);
       };
-      float best = 0;
-      for (int k = TILE64; k < TILE_KINDS && !rc; k++) {
-        if ((rc = run(k))) break;
-        if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
-        for (int q = 0; q < 5 && !rc; q++) rc = run(k);
-        if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
-        float ms = 0;
-        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-          rc = fail(FS_E_DEVICE, "autotune timing failed");
-        if (rc) break;
-        b->tune_us[lg][dir][k] = ms * 1000.f / 5.f;
-        if (best == 0 || b->tune_us[lg][dir][k] < best) {
-          best = b->tune_us[lg][dir][k];
-          b->tile[lg][dir] = k;
+      for (int round = 0; round < kTuneRounds && !rc; round++)
+        for (int j = 0; j < TILE_KINDS && !rc; j++) {
+          const int k = (j + round) % TILE_KINDS;
+          if ((rc = run(k))) break;
+          if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
+          for (int q = 0; q < kTuneLaunches && !rc; q++) rc = run(k);
+          if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
+          float ms = 0;
+          if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(FS_E_DEVICE, "autotune timing failed");
+          if (rc) break;
+          const float us = ms * 1000.f / kTuneLaunches;
+          if (b->tune_us[lg][dir][k] == 0 || us < b->tune_us[lg][dir][k]) b->tune_us[lg][dir][k] = us;
         }
-      }
+      for (int k = 0; k < TILE_KINDS; k++)
+        if (b->tune_us[lg][dir][k] > 0 && b->tune_us[lg][dir][k] < b->tune_us[lg][dir][b->tile[lg][dir]])
+          b->tile[lg][dir] = k;
     }
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -1656,6 +1662,7 @@ struct fs_trainer_t {
   double *scales = nullptr, *mr = nullptr, *mp = nullptr;
   int pblk = 0;                              // blocks per image of the pair preparation
   float* part = nullptr;                     // per-workgroup gradient partials
+  float* part2 = nullptr;                    // their sums over groups of TR_RC rows
   size_t part_cap = 0;
   float* opart = nullptr;
   double* lpart = nullptr;
@@ -1674,7 +1681,8 @@ void trainer_release(fs_trainer_t* t) {
   }
   for (void* q : {(void*)t->par, (void*)t->grad, (void*)t->m1, (void*)t->m2, (void*)t->pp, (void*)t->r32,
                   (void*)t->tgt, (void*)t->err, (void*)t->scales, (void*)t->mr, (void*)t->mp, (void*)t->part, (void*)t->opart,
-                  (void*)t->lpart, (void*)t->loss_d, (void*)t->idx_d, (void*)t->snap})
+                  (void*)t->lpart, (void*)t->loss_d, (void*)t->idx_d, (void*)t->snap,
+                  (void*)t->part2})
     dfree(q);
   for (int l = 0; l < 15; l++) {
     dfree(t->wf[l]);
@@ -1696,6 +1704,17 @@ int tlaunch(fs_trainer_t* t, K kern, dim3 g, dim3 b, Args... args) {
   kern<<<g, b, 0, t->stream>>>(args...);
   LAUNCHCHK();
   return FS_OK;
+}
+
+// dst[i] = sum over r < rows of part[r][i] (len columns): in groups of TR_RC rows, then
+// over the groups; the same fixed order every time.
+int train_reduce(fs_trainer_t* t, const float* part, int rows, size_t len, float* dst) {
+  const unsigned gx = (unsigned)((len + 255) / 256);
+  if (rows <= fsk::TR_RC) return tlaunch(t, fsk::k_tr_reduce, dim3(gx), dim3(256), part, rows, len, dst);
+  const int groups = (rows + fsk::TR_RC - 1) / fsk::TR_RC;
+  int rc = tlaunch(t, fsk::k_tr_reduce_rows, dim3(gx, (unsigned)groups), dim3(256), part, rows, len, t->part2);
+  if (rc) return rc;
+  return tlaunch(t, fsk::k_tr_reduce, dim3(gx), dim3(256), (const float*)t->part2, groups, len, dst);
 }
 
 template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
@@ -1739,8 +1758,7 @@ int train_bwd_weights(fs_trainer_t* t, int layer, bool size_only = false, size_t
   if ((rc = tlaunch(t, fsk::k_conv3x3_wgrad<CIN, COUT, MODE, C2, TH>, dim3((unsigned)gx, (unsigned)nyb),
                     dim3(fsk::GT_NT), x1, x2, (const uint16_t*)t->gz[layer], t->part, m, t->B)))
     return rc;
-  return tlaunch(t, fsk::k_tr_reduce, dim3((unsigned)((PL + 255) / 256)), dim3(256), (const float*)t->part, gx, PL,
-                 t->grad + t->woff[layer]);
+  return train_reduce(t, t->part, gx, PL, t->grad + t->woff[layer]);
 }
 
 // every weight-gradient launch (or only the partial-buffer size it needs)
@@ -1811,9 +1829,7 @@ int train_forward_backward(fs_trainer_t* t) {
     return rc;
   if ((rc = tlaunch(t, k_tr_loss, dim3(1), dim3(256), (const double*)t->lpart, t->oblk, (double)count, t->loss_d)))
     return rc;
-  if ((rc = tlaunch(t, k_tr_reduce, dim3(1), dim3(256), (const float*)t->opart, t->oblk, (size_t)17,
-                    t->grad + t->woff[14])))
-    return rc;
+  if ((rc = train_reduce(t, t->opart, t->oblk, 17, t->grad + t->woff[14]))) return rc;
   // ---- data gradients, deepest first (each skip gradient is written before it is added)
   auto E = [](const uint16_t* act, const float* gin, float* gout) { return ConvEpiArgs{act, gin, gout}; };
   if ((rc = train_bwd_data<16, 16, 16, 1, EPI_MASK, 0>(t, 13, t->gz[12], E(t->act[12], nullptr, nullptr)))) return rc;
@@ -1841,7 +1857,7 @@ int train_forward_backward(fs_trainer_t* t) {
   if ((rc = tlaunch(t, k_tr_first_wgrad, dim3((unsigned)g0), dim3(320), (const uint16_t*)t->gz[0],
                     (const float*)t->r32, (const double*)t->pp, (const double*)t->scales, n, B, t->part)))
     return rc;
-  return tlaunch(t, k_tr_reduce, dim3(2), dim3(256), (const float*)t->part, g0, (size_t)304, t->grad + t->woff[0]);
+  return train_reduce(t, t->part, g0, 304, t->grad + t->woff[0]);
 }
 
 int train_pack(fs_trainer_t* t) {
@@ -2857,7 +2873,10 @@ int fs_trainer_create(int device, int n, int batch, double h, void** out) {
   size_t need = (size_t)304 * train_first_grid(t);
   train_all_weights(t, true, &need);
   t->part_cap = need;
-  if ((rc = dalloc(&t->part, need))) return bail(rc);
+  // group sums: rows/TR_RC + 1 groups of at most the longest row (73792: dec2a's
+  // weights and bias), or of the output layer's 17 over its oblk rows
+  const size_t need2 = need / fsk::TR_RC + 304 + 73792 + (size_t)((t->oblk + fsk::TR_RC - 1) / fsk::TR_RC) * 17;
+  if ((rc = dalloc(&t->part, need)) || (rc = dalloc(&t->part2, need2))) return bail(rc);
   *out = t;
   return FS_OK;
 }

@@ -119,6 +119,19 @@ k_tr_reduce(const float* __restrict__ part, int rows, size_t len, float* __restr
   dst[i] = a;
 }
 
+// First level of a two-level fixed-order sum: rows [y TR_RC, (y + 1) TR_RC) of part
+// summed in order into out[y] (y = blockIdx.y); k_tr_reduce then sums the out rows.
+constexpr int TR_RC = 32;
+__global__ void __launch_bounds__(256)
+k_tr_reduce_rows(const float* __restrict__ part, int rows, size_t len, float* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= len) return;
+  const int r0 = blockIdx.y * TR_RC, r1 = min(rows, r0 + TR_RC);
+  float a = 0.f;
+  for (int r = r0; r < r1; r++) a += part[(size_t)r * len + i];
+  out[(size_t)blockIdx.y * len + i] = a;
+}
+
 // Training pairs: image b is (p_prev, p) = (snap[idx[b]], snap[idx[b] + 1]), FP64 n x n.
 // Writes p_prev to pp[b], r = L p - L p_prev (5-point periodic Laplacian) to r32[b] in
 // FP32, and per-block maxima of |r| and |p_prev| (FP64) to mr[b][blk], mp[b][blk].
