@@ -45,6 +45,17 @@ __device__ __forceinline__ float rnd(float f) {
 constexpr int BX = 64;
 constexpr int BY = 4;
 
+// Block coordinates of a 2D grid with the dispatch order remapped so that each XCD
+// takes a contiguous run of row-major blocks: workgroups are handed round-robin to
+// the 8 XCDs in linear order, which would put x-adjacent blocks on different XCDs,
+// each of whose L2s then fetches the lines at both ends of every row segment.
+__device__ __forceinline__ int2 xcd_block2() {
+  const int gx = (int)gridDim.x, nb = gx * (int)gridDim.y;
+  const int b = (int)blockIdx.x + (int)blockIdx.y * gx;
+  const int t = (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+  return int2{t % gx, t / gx};
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) {
   if ((unsigned)i < (unsigned)n) return i;
   int r = i % n;
@@ -115,7 +126,8 @@ struct StepCoef {
 __global__ void __launch_bounds__(BX * BY)
 k_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
             float* __restrict__ ov, int nx, int ny, const StepCoef* __restrict__ cf, float sign) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   float cx = sign * cf->cx, cy = sign * cf->cy;
   float Xb, Yb;
@@ -133,7 +145,8 @@ __global__ void __launch_bounds__(BX * BY)
 k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ fu,
             const float* __restrict__ fv, float* __restrict__ ou, float* __restrict__ ov, int nx, int ny,
             const StepCoef* __restrict__ cf) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   float cx = cf->cx, cy = cf->cy;
   size_t k = (size_t)j * nx + i;
@@ -163,7 +176,8 @@ k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const floa
 __global__ void __launch_bounds__(BX * BY)
 k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
           float* __restrict__ ov, int nx, int ny, const StepCoef* __restrict__ cf) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
   if (i >= nx || j >= ny) return;
   float a = cf->da, b = cf->db;
   int im = prv(i, nx), ip = nxt(i, nx), jm = prv(j, ny), jp = nxt(j, ny);
@@ -412,7 +426,8 @@ __global__ void __launch_bounds__(BX * BY)
 k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
              float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, double idx, double idy,
              double* __restrict__ part) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
   double mu = 0.0, mv = 0.0;
   if (i < nx && j < ny) {
     size_t k = (size_t)j * nx + i;
