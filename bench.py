@@ -85,6 +85,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--guess", default="zero", choices=["zero", "previous", "learned"])
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the forced-1024^2 comparison of learned-guess BF16 multigrid with plain FP32 multigrid")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the second timed run with the other pressure guess")
     ap.add_argument("--report", action="store_true",
@@ -254,6 +256,54 @@ def bench_cavities(a, fluidsim, dev):
     return 0
 
 
+# The learned-guess target: band-forced turbulence at 1024^2 (the 2048^2 forcing band
+# kf = 32 +- 2 scaled to the grid: kf = 16 +- 1; 24 modes, amplitude 0.1, drag 0.02,
+# seed 1000 + step), nu = 1e-4, the same multigrid and FP64 gate as the headline line.
+TARGET_WORKLOAD = dict(n=1024, nu=1e-4, forcing=dict(kf=16.0, width=1.0, amp=0.1, drag=0.02, seed=1000),
+                       precision="fp32")
+TARGET_RUNS = [("zero", "fp32"), ("learned", "bf16"), ("learned", "fp32"), ("previous", "fp32"),
+               ("previous", "bf16"), ("zero", "bf16")]
+
+
+def target_comparison(a, fluidsim, initial):
+    """Learned-guess mixed-precision multigrid against plain FP32 multigrid (zero guess)
+    on forced 1024^2 turbulence, same steps and timing as the headline line."""
+    n = TARGET_WORKLOAD["n"]
+    if not os.path.exists(GUESS_WEIGHTS):
+        return None
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    runs = []
+    for guess, prec in TARGET_RUNS:
+        wl = dict(TARGET_WORKLOAD, precision=prec)
+        g = make_solver(fluidsim, wl, n, a.tol, guess)
+        g.uploadVelocity(u0, v0)
+        for _ in range(a.warmup):
+            g.stepCFL(0.5)
+        g.sync()
+        s0 = g.stats()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            g.stepCFL(0.5)
+        g.sync()
+        el = time.perf_counter() - t0
+        s1 = g.stats()
+        runs.append({"guess": guess, "mg_storage": prec, "value": n * n * a.steps / el,
+                     "ms_per_step": el / a.steps * 1e3,
+                     "vcycles_per_step": (s1["total_vcycles"] - s0["total_vcycles"]) / max(a.steps, 1),
+                     "last_residual": s1["last_residual"]})
+        g.destroy()
+        r = runs[-1]
+        print(f"# target {guess:8s} {prec}: {r['value']:.4g} cell-steps/s, {r['vcycles_per_step']:.2f} V-cycles/step, "
+              f"{r['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    plain = runs[0]["value"]
+    learned_bf16 = runs[1]["value"]
+    return {"workload": "band-forced 2D turbulence 1024^2 (kf 16 +- 1, 24 modes, amp 0.1, drag 0.02), nu=1e-4, "
+                        "MacCormack, multigrid V(2,2) to FP64 max|div u* - L p| <= tol",
+            "target": "learned-guess BF16-storage multigrid >= 2x the cell-steps/s of plain FP32 multigrid",
+            "ratio_learned_bf16_to_plain_fp32": learned_bf16 / plain,
+            "met": learned_bf16 / plain >= 2.0, "runs": runs}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -333,7 +383,7 @@ def main():
         ablation = []
         others = [x for x in ("zero", "previous", "learned") if x != a.guess]
         for other in others:
-            if other == "learned" and (wl["precision"] != "fp32" or n % 128 or not os.path.exists(GUESS_WEIGHTS)):
+            if other == "learned" and (n % 128 or not os.path.exists(GUESS_WEIGHTS)):
                 continue
             g2 = make_solver(fluidsim, wl, n, a.tol, other)
             g2.uploadVelocity(u0, v0)
@@ -353,6 +403,10 @@ def main():
             g2.destroy()
             print(f"# {other} guess: {ablation[-1]['value']:.4g} cell-steps/s, {c2:.2f} V-cycles/step, "
                   f"{ablation[-1]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+
+    target = None
+    if not a.no_target and a.workload == "decaying1024" and not a.n:
+        target = target_comparison(a, fluidsim, initial)
 
     report = None
     if a.report:
@@ -396,6 +450,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "ablation": ablation,
+        "target": target,
         "report": report,
     }
     print(json.dumps(line), flush=True)

@@ -44,14 +44,18 @@ __device__ __forceinline__ int sim_of_block(const int* __restrict__ blk0, int ns
 
 struct CavCell {
   int k, n, i, j;
-  long c;  // cell index in the field arena
+  long c;   // cell index in the field arena
+  int blk;  // the 256-cell block this workgroup covers
 };
 
+// A workgroup covers the 256-cell block xcd_tile(gridDim.x): consecutive blocks (and
+// so the rows above and below a block of a large cavity) go to the same XCD's L2.
 __device__ __forceinline__ CavCell cav_cell(const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim) {
   CavCell r;
-  r.k = sim_of_block(blk0, nsim, blockIdx.x);
+  r.blk = xcd_tile((int)gridDim.x);
+  r.k = sim_of_block(blk0, nsim, r.blk);
   const CavSim& S = sims[r.k];
-  int local = (blockIdx.x - blk0[r.k]) * 256 + threadIdx.x;
+  int local = (r.blk - blk0[r.k]) * 256 + threadIdx.x;
   r.n = S.n;
   r.j = local >> S.lg;
   r.i = local & (S.n - 1);
@@ -249,7 +253,7 @@ k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const
   pout[c] = pc;
   rhs[cl] = (float)r;
   double m = block_max(nan_to_inf(fabs(r)));
-  if (threadIdx.x == 0) part[blockIdx.x] = m;
+  if (threadIdx.x == 0) part[q.blk] = m;
 }
 
 // u -= grad p on the interior faces; block max |u|, |v| for the next step's dt.
@@ -272,8 +276,8 @@ k_cav_gradient64(float* __restrict__ u, float* __restrict__ v, const double* __r
   double a = block_max(nan_to_inf(fabs((double)uu)));
   double b = block_max(nan_to_inf(fabs((double)vv)));
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = a;
-    part[2 * blockIdx.x + 1] = b;
+    part[2 * q.blk] = a;
+    part[2 * q.blk + 1] = b;
   }
 }
 
@@ -456,7 +460,7 @@ k_nmt_down(float* __restrict__ x, const float* __restrict__ rhs, int s, int tile
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
   __shared__ float sx[RH * RW];
   __shared__ float sr[RH * RW];
-  const int g = blockIdx.x / tpg, t = blockIdx.x - g * tpg;
+  const int bid = xcd_tile((int)gridDim.x), g = bid / tpg, t = bid - g * tpg;
   const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   const int ox = tx0 - H, oy = ty0 - H;
   const size_t gb = (size_t)g * s * s;
@@ -514,7 +518,7 @@ k_nmt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
   __shared__ float sx[RH * RW];
   __shared__ float sr[RH * RW];
   __shared__ float sc[CH * CW];
-  const int g = blockIdx.x / tpg, t = blockIdx.x - g * tpg;
+  const int bid = xcd_tile((int)gridDim.x), g = bid / tpg, t = bid - g * tpg;
   const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   const int ox = tx0 - H, oy = ty0 - H;
   const int cs = s >> 1;

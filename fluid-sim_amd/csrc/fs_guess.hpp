@@ -330,11 +330,13 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
 }
 
 // First layer, 2 -> 16 channels at n, FP32 on the vector ALUs: the inputs are
-// formed on the fly from the FP32 residual r and the FP64 previous pressure.
+// formed on the fly from the residual r (stored as S: FP32, or BF16 under BF16
+// multigrid storage) and the FP64 previous pressure.
 // w: FP32 [16][9][2]; scales[0] = max|r|, scales[1] = max|p_prev|. Image blockIdx.z of
 // a batch reads r, pprev at offset z n^2, its scales at 2z and writes y at z n^2 16.
+template <typename S>
 __global__ void __launch_bounds__(256)
-k_guess_first(const float* __restrict__ r, const double* __restrict__ pprev, const double* __restrict__ scales,
+k_guess_first(const S* __restrict__ r, const double* __restrict__ pprev, const double* __restrict__ scales,
               const float* __restrict__ w, const float* __restrict__ bias, uint16_t* __restrict__ y, int n) {
   int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= n || j >= n) return;
@@ -352,7 +354,7 @@ k_guess_first(const float* __restrict__ r, const double* __restrict__ pprev, con
   for (int t = 0; t < 9; t++) {
     int yy = wrapm(j + t / 3 - 1, n), xx = wrapm(i + t % 3 - 1, n);
     size_t k = (size_t)yy * n + xx;
-    in0[t] = r[k] * ir;
+    in0[t] = ld(r[k]) * ir;
     in1[t] = (float)pprev[k] * ip;
   }
   uint32_t packed[8];

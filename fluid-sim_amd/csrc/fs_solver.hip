@@ -785,8 +785,13 @@ int run_guess(fs_solver_t* s, int pi) {
   LAUNCHCHK();
   fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(s->rpart, s->rp_cols, G.ppart, G.nblk, G.scales);
   LAUNCHCHK();
-  fsk::k_guess_first<<<dim3((unsigned)(n / 64), (unsigned)(n / 4)), 256, 0, s->stream>>>(
-      s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0], n);
+  const dim3 g0((unsigned)(n / 64), (unsigned)(n / 4));
+  if (s->bf16)
+    fsk::k_guess_first<fsk::bf16><<<g0, 256, 0, s->stream>>>(as<fsk::bf16>(s->lv[0].rhs), s->p64[pi], G.scales,
+                                                              G.w0, G.b[0], G.act[0], n);
+  else
+    fsk::k_guess_first<float><<<g0, 256, 0, s->stream>>>(s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0],
+                                                          n);
   LAUNCHCHK();
   // tile rows and channel blocks per layer: deep tiles where the weights are small,
   // shallow tiles and split channels where input + weights would crowd the LDS
@@ -1804,7 +1809,7 @@ int train_forward_backward(fs_trainer_t* t) {
                     (const int*)t->idx_d, n, h2, (const double*)t->scales, t->tgt)))
     return rc;
   // ---- forward (the inference kernels over the batch)
-  if ((rc = tlaunch(t, k_guess_first, dim3((unsigned)(n / 64), (unsigned)(n / 4), (unsigned)B), dim3(256),
+  if ((rc = tlaunch(t, k_guess_first<float>, dim3((unsigned)(n / 64), (unsigned)(n / 4), (unsigned)B), dim3(256),
                     (const float*)t->r32, (const double*)t->pp, (const double*)t->scales,
                     (const float*)(t->par + t->woff[0]), (const float*)(t->par + t->boff[0]), t->act[0], n)))
     return rc;
@@ -2258,8 +2263,8 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if (!blob) return fail(FS_E_INVALID_ARG, "blob is NULL");
-  if (s->cfg.pressure_solver != FS_P_MULTIGRID || s->bf16)
-    return fail(FS_E_UNSUPPORTED, "the learned guess needs the multigrid solver with FP32 storage");
+  if (s->cfg.pressure_solver != FS_P_MULTIGRID)
+    return fail(FS_E_UNSUPPORTED, "the learned guess needs the multigrid solver");
   if (s->nx != s->ny || s->nx % 128 != 0)
     return fail(FS_E_UNSUPPORTED, "the learned guess needs a square grid whose size is a multiple of 128, got %dx%d",
                 s->nx, s->ny);

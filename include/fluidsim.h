@@ -216,8 +216,8 @@ int fs_set_guess(void* solver, int guess);
  * src/gpu/solver.js:669-672). Loads the weights of the 4-level U-Net that maps the
  * residual of the previous pressure and the previous pressure to an increment;
  * blob: "FSGN", uint32 1, uint32 15, then per layer uint32 cin, cout, k, FP32
- * weights [cout][k*k][cin], FP32 bias[cout]. Needs the multigrid solver with FP32
- * storage and a square grid whose size is a multiple of 128. Then
+ * weights [cout][k*k][cin], FP32 bias[cout]. Needs the multigrid solver (FP32 or
+ * BF16 storage) and a square grid whose size is a multiple of 128. Then
  * fs_set_guess(solver, FS_GUESS_LEARNED). */
 int fs_load_guess(void* solver, const void* blob, size_t bytes);
 

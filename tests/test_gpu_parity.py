@@ -585,6 +585,48 @@ def test_learned_guess_keeps_the_gate(fs, O):
     assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
 
 
+def test_learned_guess_reads_the_bf16_stored_residual(fs, O):
+    # under BF16 multigrid storage the residual the network reads is stored in BF16:
+    # the same restatement, fed that rounded residual
+    from fluidsim import initial
+    from fluidsim.guess_net import bf16_round, reference_guess
+    n = 256
+    blob = _random_guess_blob(2)
+    u0, v0 = initial.decaying_turbulence(n, seed=5)
+    with solver(fs, n=n, pressureSolver="multigrid", pressureIterations=0, mgMinSize=4, mgPrecision="bf16") as g:
+        g.uploadVelocity(u0, v0)
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.runKernel("project", 0.0)
+        p = g.readField("pressure")
+    r = O.divergence(n, n, f32(u0), f32(v0))
+    ref = reference_guess(blob, bf16_round(np.asarray(r, np.float32)).astype(np.float64), np.zeros(n * n), n,
+                          TWO_PI / n)
+    assert rel(p, ref) < 3e-2
+
+
+def test_learned_guess_with_bf16_storage_keeps_the_gate(fs, O):
+    from fluidsim import initial
+    n = 256
+    blob = _random_guess_blob(6)
+    u0, v0 = initial.decaying_turbulence(n, seed=2)
+    with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid", pressureIterations=40,
+                tolerance=1e-6, mgMinSize=4, mgPrecision="bf16") as g:
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.uploadVelocity(u0, v0)
+        for _ in range(4):
+            g.stepCFL(0.5)
+            assert g.stats()["last_residual"] <= 1e-6
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for _ in range(4):
+        s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+
+
 # ------------------------------------------------------------ energy spectrum
 
 @pytest.mark.parametrize("n", [64, 256])
