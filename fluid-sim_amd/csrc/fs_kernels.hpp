@@ -189,6 +189,259 @@ k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __res
   ov[k] = c + a * (v[kl] - 2.f * c + v[kr]) + b * (v[kb] - 2.f * c + v[kt]);
 }
 
+// ------------------------------------------- fused MacCormack + explicit diffusion
+//
+// One launch for k_advect_sl (forward) + k_advect_mc + k_diffuse, with the same bits.
+// A workgroup owns an AT x AT tile. It stages u, v on the tile with an AU-cell ring in
+// LDS, computes the forward semi-Lagrangian result on an AF-cell ring, the corrected
+// and limited result on a 1-cell ring, and the diffusion of that on the tile. Every
+// sample uses the same floor, fraction and FMA sequence as the global samplers; a
+// sample whose four nodes are not all inside its LDS window reads them from global
+// memory, and a forward value needed outside its window is recomputed exactly as the
+// forward pass computes it, so the result is independent of dt (at CFL <= 0.5 every
+// sample stays in LDS).
+constexpr int AT = 32, AF = 3, AU = AF + 2;
+constexpr int AUW = AT + 2 * AU, AFW = AT + 2 * AF, AMW = AT + 2;
+
+struct AdvWin {
+  const float* w;  // LDS window of side ws, origin (ox, oy) in unwrapped cell indices
+  int ox, oy, ws;
+  const float* g;  // the field in global memory
+};
+
+// The four nodes (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1) at wrapped indices from
+// global memory: the rare path, kept out of line so the common one stays small.
+__device__ __noinline__ float4 nodes4_global(const float* __restrict__ g, int nx, int ny, int fi, int fj) {
+  int i0 = wrapi(fi, nx), j0 = wrapi(fj, ny);
+  int i1 = nxt(i0, nx), j1 = nxt(j0, ny);
+  const float* r0 = g + (size_t)j0 * nx;
+  const float* r1 = g + (size_t)j1 * nx;
+  return float4{r0[i0], r0[i1], r1[i0], r1[i1]};
+}
+
+// Position of wrapped-or-not cell index fi in a window staged from the wrapped cells
+// origin + l: (fi - origin) mod n, with one correction (enough for |fi| < 2n; other
+// indices land outside the window and take the global path).
+__device__ __forceinline__ int win_index(int fi, int origin, int n) {
+  int l = fi - origin;
+  l = l < 0 ? l + n : (l >= n ? l - n : l);
+  return l;
+}
+
+// The four nodes of a bilinear sample at cell (fi, fj): from the window, or (FAST
+// false and not all inside it) from global memory. WRAP false: the tile's windows
+// do not cross the periodic seam, so window positions are plain differences.
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ float4 nodes4_w(const AdvWin& a, int nx, int ny, int fi, int fj) {
+  int li = WRAP ? win_index(fi, a.ox, nx) : fi - a.ox, lj = WRAP ? win_index(fj, a.oy, ny) : fj - a.oy;
+  if (FAST || ((unsigned)li < (unsigned)(a.ws - 1) && (unsigned)lj < (unsigned)(a.ws - 1))) {
+    const float* r0 = a.w + lj * a.ws + li;
+    return float4{r0[0], r0[1], r0[a.ws], r0[a.ws + 1]};
+  }
+  return nodes4_global(a.g, nx, ny, fi, fj);
+}
+
+// bilerp() with the four nodes from the window when they are all inside it
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ float bilerp_w(const AdvWin& a, int nx, int ny, float gx, float gy) {
+  float fi = floorf(gx), fj = floorf(gy);
+  float fx = gx - fi, fy = gy - fj;
+  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)fi, (int)fj);
+  float top = fmaf(q.y - q.x, fx, q.x);
+  float bot = fmaf(q.w - q.z, fx, q.z);
+  return fmaf(bot - top, fy, top);
+}
+
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ void bounds4_w(const AdvWin& a, int nx, int ny, float gx, float gy, float& lo, float& hi) {
+  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)floorf(gx), (int)floorf(gy));
+  lo = fminf(fminf(q.x, q.y), fminf(q.z, q.w));
+  hi = fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
+}
+
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ void trace_w(const AdvWin& u, const AdvWin& v, int nx, int ny, float X, float Y, float cx,
+                                        float cy, float& Xb, float& Yb) {
+  float u1 = bilerp_w<FAST, WRAP>(u, nx, ny, X, Y - 0.5f), v1 = bilerp_w<FAST, WRAP>(v, nx, ny, X - 0.5f, Y);
+  float Xm = fmaf(-0.5f * cx, u1, X), Ym = fmaf(-0.5f * cy, v1, Y);
+  float u2 = bilerp_w<FAST, WRAP>(u, nx, ny, Xm, Ym - 0.5f), v2 = bilerp_w<FAST, WRAP>(v, nx, ny, Xm - 0.5f, Ym);
+  Xb = fmaf(-cx, u2, X);
+  Yb = fmaf(-cy, v2, Y);
+}
+
+// forward semi-Lagrangian value of component COMP at node (i, j) (wrapped indices),
+// as k_advect_sl computes it
+template <int COMP, bool FAST, bool WRAP = true>
+__device__ __forceinline__ float fwd_node(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float cx,
+                                          float cy) {
+  float Xb, Yb;
+  if (COMP == 0) {
+    trace_w<FAST, WRAP>(u, v, nx, ny, (float)i, j + 0.5f, cx, cy, Xb, Yb);
+    return bilerp_w<FAST, WRAP>(u, nx, ny, Xb, Yb - 0.5f);
+  }
+  trace_w<FAST, WRAP>(u, v, nx, ny, i + 0.5f, (float)j, cx, cy, Xb, Yb);
+  return bilerp_w<FAST, WRAP>(v, nx, ny, Xb - 0.5f, Yb);
+}
+
+// forward value at one node, out of line (the rare path)
+template <int COMP>
+__device__ __noinline__ float fwd_node_far(const AdvWin u, const AdvWin v, int nx, int ny, int i, int j, float cx,
+                                           float cy) {
+  return fwd_node<COMP, false>(u, v, nx, ny, i, j, cx, cy);
+}
+
+// the four forward nodes around (i0, j0) at wrapped indices, as the global forward
+// field holds them, recomputed (out of line)
+template <int COMP>
+__device__ __noinline__ float4 fwd_nodes4_far(const AdvWin u, const AdvWin v, int nx, int ny, int i0, int j0,
+                                              float cx, float cy) {
+  int wi0 = wrapi(i0, nx), wj0 = wrapi(j0, ny), wi1 = nxt(wi0, nx), wj1 = nxt(wj0, ny);
+  return float4{fwd_node<COMP, false>(u, v, nx, ny, wi0, wj0, cx, cy),
+                fwd_node<COMP, false>(u, v, nx, ny, wi1, wj0, cx, cy),
+                fwd_node<COMP, false>(u, v, nx, ny, wi0, wj1, cx, cy),
+                fwd_node<COMP, false>(u, v, nx, ny, wi1, wj1, cx, cy)};
+}
+
+// MacCormack back-sample of the forward field: bilerp of fwd, nodes from the forward
+// window when inside it, else recomputed
+template <int COMP, bool FAST, bool WRAP = true>
+__device__ __forceinline__ float back_w(const float* __restrict__ sf, int fox, int foy, const AdvWin& u,
+                                        const AdvWin& v, int nx, int ny, float gx, float gy, float cx, float cy) {
+  float fi = floorf(gx), fj = floorf(gy);
+  float fx = gx - fi, fy = gy - fj;
+  int i0 = (int)fi, j0 = (int)fj;
+  int li = WRAP ? win_index(i0, fox, nx) : i0 - fox, lj = WRAP ? win_index(j0, foy, ny) : j0 - foy;
+  float4 q;
+  if (FAST || ((unsigned)li < (unsigned)(AFW - 1) && (unsigned)lj < (unsigned)(AFW - 1))) {
+    const float* r0 = sf + lj * AFW + li;
+    q = float4{r0[0], r0[1], r0[AFW], r0[AFW + 1]};
+  } else {
+    q = fwd_nodes4_far<COMP>(u, v, nx, ny, i0, j0, cx, cy);
+  }
+  float top = fmaf(q.y - q.x, fx, q.x);
+  float bot = fmaf(q.w - q.z, fx, q.z);
+  return fmaf(bot - top, fy, top);
+}
+
+// Forward pass and corrected, limited pass of one tile (the diffusion follows in the
+// kernel). FAST: every sample is known to lie in its window.
+template <bool FAST, bool WRAP>
+__device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, float* __restrict__ sfu,
+                                            float* __restrict__ sfv, float* __restrict__ smu, float* __restrict__ smv,
+                                            const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
+                                            int tx0, int ty0, float cx, float cy) {
+  const int uox = wu.ox, uoy = wu.oy;
+  // ---- forward pass on the tile and an AF ring; nodes at wrapped indices, as the
+  // forward kernel evaluates them (its nodes are the wrapped cells of the grid)
+  const int fox = tx0 - AF, foy = ty0 - AF;
+  for (int q = threadIdx.x; q < AFW * AFW; q += 256) {
+    int ly = q / AFW, lx = q - ly * AFW;
+    int i = WRAP ? wrapi(fox + lx, nx) : fox + lx, j = WRAP ? wrapi(foy + ly, ny) : foy + ly;
+    sfu[q] = fwd_node<0, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy);
+    sfv[q] = fwd_node<1, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy);
+  }
+  __syncthreads();
+  // ---- correction + limiter on the tile and a 1-cell ring
+  for (int q = threadIdx.x; q < AMW * AMW; q += 256) {
+    int ly = q / AMW, lx = q - ly * AMW;
+    int i = WRAP ? wrapi(tx0 - 1 + lx, nx) : tx0 - 1 + lx, j = WRAP ? wrapi(ty0 - 1 + ly, ny) : ty0 - 1 + ly;
+    // this node's place in the u, v window and in the forward window
+    const int wiu = WRAP ? win_index(i, uox, nx) : i - uox, wju = WRAP ? win_index(j, uoy, ny) : j - uoy;
+    const int wif = WRAP ? win_index(i, fox, nx) : i - fox, wjf = WRAP ? win_index(j, foy, ny) : j - foy;
+    const bool inu = FAST || ((unsigned)wiu < (unsigned)AUW && (unsigned)wju < (unsigned)AUW);
+    const bool inf = FAST || ((unsigned)wif < (unsigned)AFW && (unsigned)wjf < (unsigned)AFW);
+    float Xb, Yb, lo, hi;
+    {  // u face at (i, j+1/2)
+      float X = (float)i, Y = j + 0.5f;
+      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+      float back = back_w<0, FAST, WRAP>(sfu, fox, foy, wu, wv, nx, ny, Xb, Yb - 0.5f, cx, cy);
+      float uo = inu ? wu.w[wju * AUW + wiu] : u[(size_t)j * nx + i];
+      float fo = inf ? sfu[wjf * AFW + wif] : fwd_node_far<0>(wu, wv, nx, ny, i, j, cx, cy);
+      float c = fmaf(0.5f, uo - back, fo);
+      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, cx, cy, Xb, Yb);
+      bounds4_w<FAST, WRAP>(wu, nx, ny, Xb, Yb - 0.5f, lo, hi);
+      smu[q] = c < lo ? lo : (c > hi ? hi : c);
+    }
+    {  // v face at (i+1/2, j)
+      float X = i + 0.5f, Y = (float)j;
+      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+      float back = back_w<1, FAST, WRAP>(sfv, fox, foy, wu, wv, nx, ny, Xb - 0.5f, Yb, cx, cy);
+      float vo = inu ? wv.w[wju * AUW + wiu] : v[(size_t)j * nx + i];
+      float fo = inf ? sfv[wjf * AFW + wif] : fwd_node_far<1>(wu, wv, nx, ny, i, j, cx, cy);
+      float c = fmaf(0.5f, vo - back, fo);
+      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, cx, cy, Xb, Yb);
+      bounds4_w<FAST, WRAP>(wv, nx, ny, Xb - 0.5f, Yb, lo, hi);
+      smv[q] = c < lo ? lo : (c > hi ? hi : c);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
+                    float* __restrict__ ov, int nx, int ny, int tiles_x, int nblocks,
+                    const StepCoef* __restrict__ cf, int diffuse) {
+  __shared__ float su[AUW * AUW], sv[AUW * AUW];
+  __shared__ float sfu[AFW * AFW], sfv[AFW * AFW];
+  __shared__ float smu[AMW * AMW], smv[AMW * AMW];
+  __shared__ float wmax[2][4];
+  int b = (int)blockIdx.x;
+  if ((nblocks & 7) == 0) b = (b & 7) * (nblocks >> 3) + (b >> 3);
+  const int tx0 = (b % tiles_x) * AT, ty0 = (b / tiles_x) * AT;
+  const float cx = cf->cx, cy = cf->cy;
+  // ---- u, v on the tile and an AU ring (unwrapped origin, periodic data), with
+  // the largest |u|, |v| of the window
+  const int uox = tx0 - AU, uoy = ty0 - AU;
+  float mu = 0.f, mv = 0.f;
+  for (int q = threadIdx.x; q < AUW * AUW; q += 256) {
+    int ly = q / AUW, lx = q - ly * AUW;
+    size_t g = (size_t)wrapi(uoy + ly, ny) * nx + wrapi(uox + lx, nx);
+    float a = u[g], c = v[g];
+    su[q] = a;
+    sv[q] = c;
+    mu = fmaxf(mu, fabsf(a));
+    mv = fmaxf(mv, fabsf(c));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mu = fmaxf(mu, __shfl_xor(mu, o, 64));
+    mv = fmaxf(mv, __shfl_xor(mv, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wmax[0][threadIdx.x >> 6] = mu;
+    wmax[1][threadIdx.x >> 6] = mv;
+  }
+  __syncthreads();
+  mu = fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3]));
+  mv = fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3]));
+  const AdvWin wu{su, uox, uoy, AUW, u}, wv{sv, uox, uoy, AUW, v};
+  // every velocity a trace reads comes from the window, so a trace moves at most
+  // cx max|u| by cy max|v| cells; the rings hold every sample for moves up to one
+  // cell (NaN fails the test and takes the general path)
+  const bool fast = cx * mu <= 0.99f && cy * mv <= 0.99f && nx >= AUW && ny >= AUW;
+  // a window that does not cross the periodic seam needs no wrapped indices
+  const bool inner = uox >= 0 && uoy >= 0 && uox + AUW <= nx && uoy + AUW <= ny;
+  if (fast && inner)
+    advect_tile<true, false>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+  else if (fast)
+    advect_tile<true, true>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+  else
+    advect_tile<false, true>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+  __syncthreads();
+  // ---- diffusion (or the corrected field itself) on the tile
+  const float a = cf->da, bb = cf->db;
+  for (int q = threadIdx.x; q < AT * AT; q += 256) {
+    int iy = q / AT, ix = q - iy * AT;
+    int i = tx0 + ix, j = ty0 + iy;
+    if (i >= nx || j >= ny) continue;
+    int m = (iy + 1) * AMW + ix + 1;
+    size_t k = (size_t)j * nx + i;
+    float c = smu[m];
+    ou[k] = diffuse ? c + a * (smu[m - 1] - 2.f * c + smu[m + 1]) + bb * (smu[m - AMW] - 2.f * c + smu[m + AMW]) : c;
+    c = smv[m];
+    ov[k] = diffuse ? c + a * (smv[m - 1] - 2.f * c + smv[m + 1]) + bb * (smv[m - AMW] - 2.f * c + smv[m + AMW]) : c;
+  }
+}
+
 // Band forcing then linear drag, in place (src/cpu/solver.js:200-237 semantics).
 // modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi), coef) in turns per cell;
 // cu/cv carry amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device). The argument is reduced to

@@ -87,7 +87,7 @@ struct Level {
 // itself (not the gap before it), and carries its algorithmic byte count.
 enum {
   PK_ADVECT_SL = 0, PK_ADVECT_MC, PK_DIFFUSE, PK_DOWN0, PK_DOWN, PK_COARSE, PK_UP, PK_GRADIENT, PK_RESIDUAL,
-  PK_RB, PK_GUESS, PK_COUNT
+  PK_RB, PK_GUESS, PK_ADVECT_FUSED, PK_COUNT
 };
 
 struct Profile {
@@ -345,6 +345,19 @@ int enqueue_advection(fs_solver_t* s) {
                      1.0f)))
       return rc;
   }
+  s->cur = 1 - c;
+  return FS_OK;
+}
+
+// MacCormack advection and explicit diffusion in one tiled launch, the same bits as
+// k_advect_sl + k_advect_mc + k_diffuse; 16 B per cell compulsory (u, v in and out).
+int enqueue_advect_diffuse_fused(fs_solver_t* s) {
+  const int c = s->cur;
+  const int tx = (s->nx + fsk::AT - 1) / fsk::AT, ty = (s->ny + fsk::AT - 1) / fsk::AT, nb = tx * ty;
+  int rc = launch(s, PK_ADVECT_FUSED, 16.0 * s->n, fsk::k_advect_mc_diffuse, dim3((unsigned)nb), dim3(256),
+                  (const float*)s->u[c], (const float*)s->v[c], s->u[1 - c], s->v[1 - c], s->nx, s->ny, tx, nb,
+                  (const fsk::StepCoef*)s->coef, s->cfg.nu > 0 ? 1 : 0);
+  if (rc) return rc;
   s->cur = 1 - c;
   return FS_OK;
 }
@@ -937,8 +950,12 @@ int do_step(fs_solver_t* s) {
   if (s->dt_pending >= DT_RING && (rc = sync_stream(s))) return rc;
   HIPCHK(hipMemcpyAsync(s->dt_ring + s->dt_pending, &s->coef->dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   s->dt_pending++;
-  if ((rc = enqueue_advection(s))) return rc;
-  if ((rc = enqueue_diffusion(s))) return rc;
+  if (s->cfg.advection == FS_ADV_MACCORMACK && s->cfg.adv_fused) {
+    if ((rc = enqueue_advect_diffuse_fused(s))) return rc;
+  } else {
+    if ((rc = enqueue_advection(s))) return rc;
+    if ((rc = enqueue_diffusion(s))) return rc;
+  }
   if ((rc = enqueue_forcing(s))) return rc;
   if ((rc = enqueue_solid(s))) return rc;
   int cycles = 0;
@@ -1922,6 +1939,7 @@ void fs_config_default(fs_config* c) {
   c->guess = FS_GUESS_ZERO;
   c->solid_mask_width = 1.5;
   c->autotune = 1;
+  c->adv_fused = 1;
   c->device = 0;
   c->mg_fused = 1;
 }

@@ -23,7 +23,8 @@ PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GUESS = {"zero": 0, "previous": 1, "learned": 2}
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
-                 "gradient": 7, "residual": 8, "rb": 9, "guess": 10}
+                 "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
+                 "advect_fused": 11}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6}
 
 
@@ -45,6 +46,7 @@ class FsConfig(C.Structure):
         ("device", C.c_int),
         ("mg_fused", C.c_int),
         ("autotune", C.c_int),
+        ("adv_fused", C.c_int),
     ]
 
 

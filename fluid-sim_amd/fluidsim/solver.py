@@ -54,7 +54,8 @@ class GPUFluidSolver:
                  confinement=0, dyeFade=0, advection="maccormack", solidMaskWidth=1.5,
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
-                 mgPrecision="fp32", guess="zero", mgFused=True, autotune=True):
+                 mgPrecision="fp32", guess="zero", mgFused=True, autotune=True,
+                 advFused=True):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -82,6 +83,7 @@ class GPUFluidSolver:
         cfg.device = int(device)
         cfg.mg_fused = 1 if mgFused else 0
         cfg.autotune = 1 if autotune else 0
+        cfg.adv_fused = 1 if advFused else 0
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))

@@ -89,6 +89,9 @@ typedef struct fs_config {
                                  in one workgroup); 0: one launch per operator. Same bits. */
   int autotune;               /* 1 (default): time each tile shape of the V-cycle kernels at
                                  creation and keep the fastest per level (same bits) */
+  int adv_fused;              /* 1 (default): MacCormack advection and diffusion in one tiled
+                                 launch; 0: three launches (forward, correction, diffusion).
+                                 Same bits. */
 } fs_config;
 
 /* per-step statistics (new: the reference reports none) */
@@ -253,7 +256,8 @@ enum {
   FS_PK_ADVECT_SL = 0, FS_PK_ADVECT_MC = 1, FS_PK_DIFFUSE = 2, FS_PK_DOWN0 = 3, FS_PK_DOWN = 4,
   FS_PK_COARSE = 5, FS_PK_UP = 6, FS_PK_GRADIENT = 7, FS_PK_RESIDUAL = 8, FS_PK_RB = 9,
   FS_PK_GUESS = 10,  /* 3x3 layers of the learned guess */
-  FS_PK_COUNT = 11
+  FS_PK_ADVECT_FUSED = 11,  /* MacCormack + diffusion in one launch */
+  FS_PK_COUNT = 12
 };
 /* Time every launch of the classes in `mask` (bit k = class k; 0 = off) with HIP
  * events recorded at the dispatch's own start and end on the solver's stream, and

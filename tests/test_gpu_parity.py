@@ -653,3 +653,30 @@ def test_energy_spectrum_matches_oracle(fs, O, n):
         g.uploadVelocity(tu, tv)
         _, Et, _ = g.energySpectrum()
     assert Et[1] > 0.99 * np.sum(Et)
+
+
+@pytest.mark.parametrize("nx,ny,nu,dt_fixed", [(256, 256, 1e-3, None), (80, 48, 5e-4, None), (64, 64, 0.0, None),
+                                               (128, 128, 1e-3, 0.08)])
+def test_fused_advection_diffusion_gives_the_same_bits(fs, nx, ny, nu, dt_fixed):
+    # the one-launch MacCormack + diffusion tile kernel against the three launches;
+    # a fixed dt of 0.08 moves samples several cells, outside the LDS windows
+    from fluidsim import initial
+    n = max(nx, ny)
+    u0, v0 = initial.decaying_turbulence(n, seed=11)
+    u0 = np.asarray(u0).reshape(n, n)[:ny, :nx].reshape(-1)
+    v0 = np.asarray(v0).reshape(n, n)[:ny, :nx].reshape(-1)
+    outs = []
+    for fused in (True, False):
+        # multigrid is square-only: the non-square grid projects with red-black sweeps
+        proj = dict(pressureSolver="multigrid", pressureIterations=30, tolerance=1e-7, mgMinSize=4) \
+            if nx == ny else dict(pressureSolver="red-black", pressureIterations=60)
+        with solver(fs, nx=nx, ny=ny, nu=nu, advection="maccormack", advFused=fused, **proj) as g:
+            g.uploadVelocity(u0, v0)
+            for _ in range(3):
+                if dt_fixed:
+                    g.step(dt_fixed)
+                else:
+                    g.stepCFL(0.5)
+            outs.append(g.readVelocity())
+    assert np.array_equal(outs[0]["u"], outs[1]["u"]) and np.array_equal(outs[0]["v"], outs[1]["v"])
+    assert np.max(np.abs(outs[0]["u"])) > 0
