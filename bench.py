@@ -224,7 +224,7 @@ def forced_report(a, fluidsim, wl, n, u0, v0):
 
 def bench_cavities(a, fluidsim, dev):
     """256 batched cavities: cell-steps/s over the whole batch (sum of n^2 x steps / s)."""
-    cavs = fluidsim.cavities_c5(256, seed=1)
+    cavs = fluidsim.random_cavities(256, seed=1)
     guess = a.guess if a.guess in ("zero", "previous") else "zero"
     b = fluidsim.CavityBatch(cavs, tolerance=a.tol, guess=guess)
     cells = b.stats()["cells"]

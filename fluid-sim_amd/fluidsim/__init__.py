@@ -6,7 +6,7 @@ is the host-side mirror of the reference's src/gpu solver interface.
 from ._lib import FluidSimError, NotConverged, library_path, load
 from .solver import GPUFluidSolver, createDevice
 from . import initial
-from .batch import CavityBatch, cavities_c5
+from .batch import CavityBatch, random_cavities
 
 __all__ = ["GPUFluidSolver", "createDevice", "FluidSimError", "NotConverged", "initial", "load",
            "library_path"]

@@ -16,7 +16,7 @@ from ._lib import FsBatchConfig, FsBatchStats, FsCavity, check, load
 from .initial import uniform01
 
 
-def cavities_c5(count=256, seed=1, sizes=(64, 128, 256, 512)):
+def random_cavities(count=256, seed=1, sizes=(64, 128, 256, 512)):
     """The batch workload: sizes uniform over `sizes`, lid U ~ U[0.5, 2], Re ~ U[100, 5000]
     (nu = U / Re on the unit box), a pinned disk of radius r ~ U[0.05, 0.2] centred
     uniformly in [r + 2h, 1 - r - 2h]^2; one splitmix64 stream, seven draws per cavity."""

@@ -42,7 +42,7 @@ def taylor_green(nx, ny, t=0.0, nu=0.0, lx=TWO_PI, ly=TWO_PI):
 
 
 def decaying_turbulence(n, seed=1, k0=8.0, lx=TWO_PI):
-    """Config C2 initial state: returns (u, v) float64 of n*n at the MAC faces."""
+    """Decaying-turbulence initial state: returns (u, v) float64 of n*n at the MAC faces."""
     h = lx / n
     kx = np.fft.fftfreq(n, d=1.0 / n)[None, :]
     ky = np.fft.fftfreq(n, d=1.0 / n)[:, None]

@@ -66,8 +66,8 @@ def test_batch_members_match_their_own_runs(fs, O):
         assert max(rel(together[k]["u"], alone["u"]), rel(together[k]["v"], alone["v"])) < 1e-5, k
 
 
-def test_cavity_batch_c5_draws(fs):
-    cavs = fs.cavities_c5(256, seed=1)
+def test_random_cavity_draws(fs):
+    cavs = fs.random_cavities(256, seed=1)
     assert len(cavs) == 256 and {c["n"] for c in cavs} == {64, 128, 256, 512}
     for c in cavs:
         h = 1.0 / c["n"]
@@ -137,7 +137,7 @@ def test_every_cavity_tile_shape_gives_the_same_bits(fs, tile):
 
 
 def test_cavity_autotuner_picks_tiles_and_sweeps(fs):
-    cavs = fs.cavities_c5(32, seed=2)
+    cavs = fs.random_cavities(32, seed=2)
     with fs.CavityBatch(cavs, tolerance=1e-6, autotune=2) as b:
         rep = b.tuneReport()
         st = b.stats()

@@ -182,8 +182,8 @@ def test_scheme_energy_after_200_steps(fs, O, golden, scheme):
     assert abs(ke - golden["schemeEnergy"][scheme]) / ke < 2e-5
 
 
-def test_taylor_green_200_steps_c1(fs, O):
-    # TG 64^2, nu = 0.01, MacCormack + explicit diffusion, projection to
+def test_taylor_green_200_steps_fp32_against_f64(fs, O):
+    # Taylor-Green 64^2, nu = 0.01, MacCormack + explicit diffusion, projection to
     # an FP64 residual of 1e-9, dt at CFL 0.5; FP32 GPU against the f64 oracle (exact FFT)
     n, nu = 64, 0.01
     u0, v0 = O.taylor_green(n, n)
@@ -213,7 +213,7 @@ def test_taylor_green_200_steps_c1(fs, O):
 
 
 def test_tolerance_gate_on_turbulence(fs, O):
-    # C2 at 256^2: every projection meets max|div u* - L p| <= 1e-6 in FP64, and the
+    # decaying turbulence at 256^2: every projection meets max|div u* - L p| <= 1e-6 in FP64, and the
     # velocity tracks the oracle's exact projection
     from fluidsim import initial
     n, nu = 256, 1e-4
@@ -352,13 +352,14 @@ def test_vcycle_residual_curve_matches_oracle(fs, O, n, min_size):
             assert abs(r - ro) / ro < 2e-3, (n, it, r, ro)
 
 
-def test_profile_counts_every_launch_and_its_bytes(fs, O):
+@pytest.mark.parametrize("adv_fused", [True, False])
+def test_profile_counts_every_launch_and_its_bytes(fs, O, adv_fused):
     # one fused V(2,2) step at 256^2, minSize 4: levels 256..4, tile kernels on 256
     # and 128, the one-workgroup kernel from 64 down; a fixed 3 V-cycles
     n = 256
     u0, v0 = O.js_random_velocity(n, 7)
     with solver(fs, n=n, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
-                mgMinSize=4) as g:
+                mgMinSize=4, advFused=adv_fused) as g:
         g.uploadVelocity(u0, v0)
         g.profileEnable()
         g.step(1e-3)
@@ -366,10 +367,16 @@ def test_profile_counts_every_launch_and_its_bytes(fs, O):
         p = g.profileRead()
         g.profileEnable([])
         g.step(1e-3)
-        assert g.profileRead()["advect_sl"][1] == 0
-    ms, launches, nbytes = p["advect_sl"]
-    assert launches == 1 and ms > 0 and nbytes == 16 * n * n
-    assert p["advect_mc"][1] == 1 and p["advect_mc"][2] == 24 * n * n
+        assert g.profileRead()["advect_sl"][1] == 0 and g.profileRead()["advect_fused"][1] == 0
+    if adv_fused:
+        ms, launches, nbytes = p["advect_fused"]
+        assert launches == 1 and ms > 0 and nbytes == 16 * n * n
+        assert p["advect_sl"][1] == 0 and p["advect_mc"][1] == 0
+    else:
+        ms, launches, nbytes = p["advect_sl"]
+        assert launches == 1 and ms > 0 and nbytes == 16 * n * n
+        assert p["advect_mc"][1] == 1 and p["advect_mc"][2] == 24 * n * n
+        assert p["advect_fused"][1] == 0
     assert p["gradient"][1] == 1 and p["gradient"][2] == 24 * n * n
     # three V-cycles: one level-0 down-sweep each, plus the final residual-only pass
     assert p["down0"][1] + p["residual"][1] == 4

@@ -325,8 +325,8 @@ def test_multigrid_levels_follow_reference_rule(O):
 
 # ------------------------------------------------------------------- full step
 
-def test_taylor_green_decay_c1(O):
-    # TG 64^2, nu=0.01, MacCormack, explicit diffusion, exact projection,
+def test_taylor_green_decay_f64(O):
+    # Taylor-Green 64^2, nu=0.01, MacCormack, explicit diffusion, exact projection,
     # dt at CFL 0.5 (src/core/grid.js:322-327), 200 steps
     n, nu = 64, 0.01
     s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
