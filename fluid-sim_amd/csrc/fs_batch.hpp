@@ -32,15 +32,6 @@ struct CavSim {
   double dt;            // this step's dt (written on the device)
 };
 
-__device__ __forceinline__ int sim_of_block(const int* __restrict__ blk0, int nsim, int b) {
-  int lo = 0, hi = nsim - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (blk0[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
 
 struct CavCell {
   int k, n, i, j;
@@ -50,10 +41,11 @@ struct CavCell {
 
 // A workgroup covers the 256-cell block xcd_tile(gridDim.x): consecutive blocks (and
 // so the rows above and below a block of a large cavity) go to the same XCD's L2.
+// blk0 holds each cavity's first block, then (from index nsim) each block's cavity.
 __device__ __forceinline__ CavCell cav_cell(const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim) {
   CavCell r;
   r.blk = xcd_tile((int)gridDim.x);
-  r.k = sim_of_block(blk0, nsim, r.blk);
+  r.k = blk0[nsim + r.blk];
   const CavSim& S = sims[r.k];
   int local = (r.blk - blk0[r.k]) * 256 + threadIdx.x;
   r.n = S.n;

@@ -1221,7 +1221,7 @@ struct fs_batch_t {
   std::vector<int> intl;     // caller's index -> internal index
   std::vector<fsk::CavSim> sims_h;
   fsk::CavSim* sims = nullptr;
-  int* blk0 = nullptr;
+  int* blk0 = nullptr;       // first block of each cavity, then the cavity of each block
   int nblocks = 0;
   long cells = 0;
   int max_lg = 0;
@@ -2690,6 +2690,12 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   }
   b->cells = off;
   b->nblocks = (int)(off / 256);
+  // after the first block of each cavity: the cavity of each block
+  blk0.resize((size_t)count + b->nblocks);
+  for (int k = 0; k < count; k++) {
+    const int e = k + 1 < count ? blk0[k + 1] : b->nblocks;
+    for (int q = blk0[k]; q < e; q++) blk0[(size_t)count + q] = k;
+  }
   int rc = 0;
   for (int k = 0; k < 2 && !rc; k++) {
     rc = dalloc(&b->u[k], (size_t)off);
@@ -2699,7 +2705,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (!rc) rc = dalloc(&b->fu, (size_t)off);
   if (!rc) rc = dalloc(&b->fv, (size_t)off);
   if (!rc) rc = dalloc(&b->sims, (size_t)count);
-  if (!rc) rc = dalloc(&b->blk0, (size_t)count);
+  if (!rc) rc = dalloc(&b->blk0, blk0.size());
   if (!rc) rc = dalloc(&b->lx, (size_t)lvl_cells);
   if (!rc) rc = dalloc(&b->lr, (size_t)lvl_cells);
   if (!rc) rc = dalloc(&b->lxb, (size_t)lvl_cells);
@@ -2711,7 +2717,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (!rc) rc = dalloc(&b->mpart, (size_t)2 * b->nblocks);
   if (rc) return bail(rc);
   HIPCHK(hipMemcpy(b->sims, b->sims_h.data(), sizeof(fsk::CavSim) * count, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->blk0, blk0.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->blk0, blk0.data(), sizeof(int) * blk0.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->cbase, cbase.data(), sizeof(long) * count, hipMemcpyHostToDevice));
   std::vector<float> pinv = neumann_pinv4();
   HIPCHK(hipMemcpy(b->pinv4, pinv.data(), sizeof(float) * 256, hipMemcpyHostToDevice));
