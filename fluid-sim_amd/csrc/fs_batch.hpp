@@ -32,7 +32,6 @@ struct CavSim {
   double dt;            // this step's dt (written on the device)
 };
 
-
 struct CavCell {
   int k, n, i, j;
   long c;   // cell index in the field arena
