@@ -46,6 +46,7 @@ KIND_TEXT = {
     "rb": "red-black colour sweep (k_mg_rb), 6 B/cell",
     "guess": "learned-guess 3x3 convolutions (k_conv3x3, BF16 MFMA), BF16 activations in + out",
     "advect_fused": "fused MacCormack + diffusion (k_advect_mc_diffuse), 16 B/cell: u, v in + out",
+    "forcing": "band forcing + drag (k_forcing_sep), 16 B/cell: u, v in + out",
 }
 
 # HBM bytes per launch at 1024^2 measured by `rocprofv3 --pmc FETCH_SIZE` and

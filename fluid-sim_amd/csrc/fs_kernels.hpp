@@ -444,31 +444,88 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
 
 // Band forcing then linear drag, in place (src/cpu/solver.js:200-237 semantics).
 // modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi), coef) in turns per cell;
-// cu/cv carry amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device). The argument is reduced to
-// [0,1) turns in FP64 before cospi, so large wavenumbers keep full FP32 accuracy.
+// cu/cv carry amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device).
 struct Mode { double tx, ty, tph; float cu, cv; };
 constexpr int MAX_MODES = 24;
 struct ModeSet { Mode m[MAX_MODES]; int n; };
 
-__global__ void __launch_bounds__(BX * BY)
-k_forcing(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms,
-          const StepCoef* __restrict__ cf) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-  if (i >= nx || j >= ny) return;
-  size_t k = (size_t)j * nx + i;
-  float fu = 0.f, fv = 0.f;
-  for (int m = 0; m < ms.n; m++) {
-    const Mode& md = ms.m[m];
-    double tu = md.tx * i + md.ty * (j + 0.5) + md.tph;
-    double tv = md.tx * (i + 0.5) + md.ty * j + md.tph;
-    tu -= floor(tu);
-    tv -= floor(tv);
-    fu = fmaf(md.cu, cospif(2.f * (float)tu), fu);
-    fv = fmaf(md.cv, cospif(2.f * (float)tv), fv);
+// Evaluated separably: with A = frac(tx x) and B = frac(ty y + phase) reduced to
+// [0, 1) turns in FP64 (so large wavenumbers keep full FP32 accuracy),
+// cos 2pi(A + B) = cos 2piA cos 2piB - sin 2piA sin 2piB: 4 (nx + ny) sincospi per
+// mode and step instead of 2 nx ny cospi. A block of FX x FY cells stages its
+// columns' and rows' entries of the tables in LDS.
+constexpr int FX = 64, FY = 16;
+
+// Per-step tables of the separable forcing: col[m][i] = (cos 2piA, sin 2piA) at the u
+// face (x = i) and at the v face (x = i + 1/2); row[m][j] = (cos 2piB, sin 2piB) at the
+// u face (y = j + 1/2) and at the v face (y = j).
+__global__ void __launch_bounds__(256)
+k_forcing_tables(const ModeSet ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row) {
+  const int q = blockIdx.x * 256 + threadIdx.x, per = nx + ny;
+  if (q >= ms.n * per) return;
+  const int m = q / per, r = q - m * per;
+  const Mode& md = ms.m[m];
+  double a, b;
+  if (r < nx) {
+    a = md.tx * r;
+    b = md.tx * (r + 0.5);
+  } else {
+    const int j = r - nx;
+    a = md.ty * (j + 0.5) + md.tph;
+    b = md.ty * j + md.tph;
   }
-  float fdt = cf->fdt, drag = cf->drag;
-  u[k] = (u[k] + fu * fdt) * drag;
-  v[k] = (v[k] + fv * fdt) * drag;
+  a -= floor(a);
+  b -= floor(b);
+  float ca, sa, cb, sb;
+  sincospif(2.f * (float)a, &sa, &ca);
+  sincospif(2.f * (float)b, &sb, &cb);
+  if (r < nx) col[(size_t)m * nx + r] = float4{ca, sa, cb, sb};
+  else row[(size_t)m * ny + (r - nx)] = float4{ca, sa, cb, sb};
+}
+
+__global__ void __launch_bounds__(256)
+k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms,
+              const float4* __restrict__ col, const float4* __restrict__ row, const StepCoef* __restrict__ cf) {
+  __shared__ float4 sc[MAX_MODES][FX], sr[MAX_MODES][FY];
+  const int i0 = blockIdx.x * FX, j0 = blockIdx.y * FY;
+  const int nm = ms.n;
+  for (int q = threadIdx.x; q < nm * (FX + FY); q += 256) {
+    const int m = q / (FX + FY), r = q - m * (FX + FY);
+    if (r < FX) {
+      if (i0 + r < nx) sc[m][r] = col[(size_t)m * nx + i0 + r];
+    } else if (j0 + r - FX < ny) {
+      sr[m][r - FX] = row[(size_t)m * ny + j0 + r - FX];
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & (FX - 1), ty = threadIdx.x / FX;  // 4 rows of threads
+  const int i = i0 + tx;
+  if (i >= nx) return;
+  const float fdt = cf->fdt, drag = cf->drag;
+  constexpr int R = FY / 4;  // rows per thread: ty, ty + 4, ...
+  float fu[R], fv[R];
+#pragma unroll
+  for (int rr = 0; rr < R; rr++) fu[rr] = fv[rr] = 0.f;
+  for (int m = 0; m < nm; m++) {
+    const Mode& md = ms.m[m];
+    const float4 c = sc[m][tx];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+      const float4 w = sr[m][ty + 4 * rr];
+      float cu = c.x * w.x - c.y * w.y;
+      float cv = c.z * w.z - c.w * w.w;
+      fu[rr] = fmaf(md.cu, cu, fu[rr]);
+      fv[rr] = fmaf(md.cv, cv, fv[rr]);
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < R; rr++) {
+    const int j = j0 + ty + 4 * rr;
+    if (j >= ny) break;
+    const size_t k = (size_t)j * nx + i;
+    u[k] = (u[k] + fu[rr] * fdt) * drag;
+    v[k] = (v[k] + fv[rr] * fdt) * drag;
+  }
 }
 
 // --------------------------------------------------- projection, FP32 paths

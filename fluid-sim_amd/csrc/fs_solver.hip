@@ -87,7 +87,7 @@ struct Level {
 // itself (not the gap before it), and carries its algorithmic byte count.
 enum {
   PK_ADVECT_SL = 0, PK_ADVECT_MC, PK_DIFFUSE, PK_DOWN0, PK_DOWN, PK_COARSE, PK_UP, PK_GRADIENT, PK_RESIDUAL,
-  PK_RB, PK_GUESS, PK_ADVECT_FUSED, PK_COUNT
+  PK_RB, PK_GUESS, PK_ADVECT_FUSED, PK_FORCING, PK_COUNT
 };
 
 struct Profile {
@@ -165,6 +165,8 @@ struct fs_solver_t {
   fsk::cplx* spec_Z = nullptr;
   double* spec_part = nullptr;
   double* spec_E = nullptr;
+  // separable forcing tables: MAX_MODES x nx and MAX_MODES x ny (allocated on first use)
+  float4 *ftab_col = nullptr, *ftab_row = nullptr;
   // immersed disk: state for the next step, per-block impulse partials, FP64 total
   bool solid_on = false;
   fs_disk solid{};
@@ -408,10 +410,21 @@ int enqueue_forcing(fs_solver_t* s) {
     md.cv = (float)(scale * (-kx / kk));
   }
   ms.n = k;
-  int c = s->cur;
-  fsk::k_forcing<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, ms, s->coef);
-  LAUNCHCHK();
-  return FS_OK;
+  int c = s->cur, rc;
+  if (!s->ftab_col) {
+    if ((rc = dalloc(&s->ftab_col, (size_t)fsk::MAX_MODES * s->nx)) ||
+        (rc = dalloc(&s->ftab_row, (size_t)fsk::MAX_MODES * s->ny)))
+      return rc;
+  }
+  if (ms.n > 0) {  // with no mode in the band the pass below still applies the drag
+    const unsigned tb = (unsigned)((ms.n * (s->nx + s->ny) + 255) / 256);
+    if ((rc = launch(s, PK_FORCING, 16.0 * ms.n * (s->nx + s->ny), fsk::k_forcing_tables, dim3(tb), dim3(256), ms,
+                     s->nx, s->ny, s->ftab_col, s->ftab_row)))
+      return rc;
+  }
+  const dim3 g((unsigned)((s->nx + fsk::FX - 1) / fsk::FX), (unsigned)((s->ny + fsk::FY - 1) / fsk::FY));
+  return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny, ms,
+                (const float4*)s->ftab_col, (const float4*)s->ftab_row, (const fsk::StepCoef*)s->coef);
 }
 
 // Direct-forcing coupling of the disk, after forcing and before the projection
@@ -1155,6 +1168,8 @@ void release(fs_solver_t* s) {
   dfree(s->spec_Z);
   dfree(s->spec_part);
   dfree(s->spec_E);
+  dfree(s->ftab_col);
+  dfree(s->ftab_row);
   for (int k = 0; k < 2; k++) {
     dfree(s->u[k]);
     dfree(s->v[k]);

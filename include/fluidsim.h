@@ -257,7 +257,8 @@ enum {
   FS_PK_COARSE = 5, FS_PK_UP = 6, FS_PK_GRADIENT = 7, FS_PK_RESIDUAL = 8, FS_PK_RB = 9,
   FS_PK_GUESS = 10,  /* 3x3 layers of the learned guess */
   FS_PK_ADVECT_FUSED = 11,  /* MacCormack + diffusion in one launch */
-  FS_PK_COUNT = 12
+  FS_PK_FORCING = 12,       /* band forcing + drag */
+  FS_PK_COUNT = 13
 };
 /* Time every launch of the classes in `mask` (bit k = class k; 0 = off) with HIP
  * events recorded at the dispatch's own start and end on the solver's stream, and
