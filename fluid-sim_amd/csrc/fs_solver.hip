@@ -21,11 +21,15 @@
 //                       added only if the last residual is still above the gate.
 //                       Extra V-cycles only lower the residual, so an
 //                       over-prediction costs time, never accuracy.
+//
+// Also here: the learned pressure guess (fs_guess.hpp) and its trainer
+// (fs_train.hpp, fs_trainer_*), batched walled cavities (fs_batch.hpp,
+// fs_batch_*) and the on-device energy spectrum (fs_spectrum.hpp).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
-#include <chrono>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
