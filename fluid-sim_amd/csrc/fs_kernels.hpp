@@ -643,7 +643,7 @@ __device__ __forceinline__ double nan_to_inf(double a) {
 template <typename S>
 __global__ void __launch_bounds__(BX * BY)
 k_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
-             const S* __restrict__ e, double* __restrict__ pout, S* __restrict__ r32, int nx, int ny,
+             const float* __restrict__ e, double* __restrict__ pout, S* __restrict__ r32, int nx, int ny,
              double idx, double idy, double idx2, double idy2, double* __restrict__ rpart) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double amax = 0.0;
@@ -653,7 +653,7 @@ k_residual64(const float* __restrict__ u, const float* __restrict__ v, const dou
     size_t kl = (size_t)j * nx + im, kr = (size_t)j * nx + ip, kb = (size_t)jm * nx + i, kt = (size_t)jp * nx + i;
     double pc = 0, pl = 0, pr = 0, pb = 0, pt = 0;
     if (p) { pc = p[k]; pl = p[kl]; pr = p[kr]; pb = p[kb]; pt = p[kt]; }
-    if (e) { pc += ld(e[k]); pl += ld(e[kl]); pr += ld(e[kr]); pb += ld(e[kb]); pt += ld(e[kt]); }
+    if (e) { pc += e[k]; pl += e[kl]; pr += e[kr]; pb += e[kb]; pt += e[kt]; }
     double lap = (pl - 2 * pc + pr) * idx2 + (pb - 2 * pc + pt) * idy2;
     double dv = ((double)u[kr] - (double)u[k]) * idx + ((double)v[kt] - (double)v[k]) * idy;
     double r = dv - lap;

@@ -190,7 +190,7 @@ k_mgt_down(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float 
 template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
-            const S* __restrict__ e_prev, double* __restrict__ pout, S* __restrict__ r32,
+            const float* __restrict__ e_prev, double* __restrict__ pout, S* __restrict__ r32,
             S* __restrict__ x, S* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2,
             int tiles_x, int nblocks, double* __restrict__ rpart, int residual_only) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
@@ -215,7 +215,7 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
         int ly = q / PW, lx = q - ly * PW;
         size_t g = (size_t)wrapn(oy - 1 + ly, n) * n + wrapn(ox - 1 + lx, n);
         tp[k] = p ? p[g] : 0.0;
-        te[k] = e_prev ? ld(e_prev[g]) : 0.f;
+        te[k] = e_prev ? e_prev[g] : 0.f;
       }
     }
 #pragma unroll
@@ -283,9 +283,9 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
 // x += bilinear prolongation of the coarse correction (PROLONG, src/gpu/multigrid.js:99-131),
 // then SW red-black sweeps. Reads xin and writes xout: a tile's halo is another
 // tile's interior, so the update cannot be in place across blocks.
-template <int SW, class TS, typename S>
+template <int SW, class TS, typename S, typename SO = S>
 __global__ void __launch_bounds__(TS::NT)
-k_mgt_up(const S* __restrict__ xin, S* __restrict__ xout, const S* __restrict__ rhs,
+k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__ rhs,
          const S* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
@@ -334,7 +334,7 @@ k_mgt_up(const S* __restrict__ xin, S* __restrict__ xout, const S* __restrict__ 
   for (int q = threadIdx.x; q < TX * TY; q += NT) {
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
-    if (gi < n && gj < n) xout[(size_t)gj * n + gi] = st<S>(sx[(iy + H) * RW + ix + H]);
+    if (gi < n && gj < n) xout[(size_t)gj * n + gi] = st<SO>(sx[(iy + H) * RW + ix + H]);
   }
 }
 
@@ -431,9 +431,9 @@ __device__ __forceinline__ float reg_residual(float x, float r, int n, int i, in
   return r - lap;
 }
 
-template <typename S>
+template <typename S, typename SO = S>
 __global__ void __launch_bounds__(CTH)
-k_mg_coarse_ml(S* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb, int pre, int post,
+k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb, int pre, int post,
                int coarse, const float* __restrict__ pinv) {
   __shared__ float bx[CML_CELLS];
   __shared__ float br[CML_CELLS];
@@ -683,7 +683,7 @@ k_mg_coarse_ml(S* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb
       }
     }
   }
-  for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = st<S>(bx[q]);
+  for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = st<SO>(bx[q]);
 }
 
 }  // namespace fsk

@@ -81,7 +81,7 @@ struct Level {
   double dx = 0;
   float* x = nullptr;    // correction (level 0: e); fused path: after pre-smoothing
   float* rhs = nullptr;  // right-hand side (level 0: r)
-  float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e)
+  float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e, FP32 under every storage)
   int tile[2] = {0, 0};  // fused path: tile shape (TileKind) of the down and up kernels
   float tune_us[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
 };
@@ -628,11 +628,11 @@ int launch_down0_s(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_o
   double idx = 1.0 / s->dx;
   int c = s->cur;
   const double es = sizeof(S);
-  // u, v (8) + p (8) + previous correction read; p (8) + r written; then x and the coarse rhs (1/4 cell)
-  double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? es : 0) + 8 + es + (residual_only ? 0 : es + 0.25 * es);
+  // u, v (8) + p (8) + previous correction (FP32) read; p (8) + r written; then x and the coarse rhs (1/4 cell)
+  double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? 4 : 0) + 8 + es + (residual_only ? 0 : es + 0.25 * es);
   return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE, TS, S>, g, dim3(TS::NT), (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
-                as<S>((const float*)(it > 0 ? L0.xb : nullptr)), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
+                (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
                 as<S>(s->lv[1].rhs), L0.n, idx, idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx,
                 nb, s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
 }
@@ -670,7 +670,11 @@ int launch_up_s(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
-  // x, rhs, coarse x (1 per 4 cells) read; x written
+  // x, rhs, coarse x (1 per 4 cells) read; x written (level 0: in FP32, see lv[0].xb)
+  if (l == 0)
+    return launch(s, PK_UP, (2.25 * sizeof(S) + 4.0) * F.n * F.n, fsk::k_mgt_up<POST, TS, S, float>, g,
+                  dim3(TS::NT), as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs),
+                  as<S>((const float*)s->lv[l + 1].xb), F.n, (float)(F.dx * F.dx), tx, nb);
   return launch(s, PK_UP, 3.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_up<POST, TS, S>, g, dim3(TS::NT),
                 as<S>((const float*)F.x), as<S>(F.xb), as<S>((const float*)F.rhs),
                 as<S>((const float*)s->lv[l + 1].xb), F.n, (float)(F.dx * F.dx), tx, nb);
@@ -687,6 +691,10 @@ int launch_up(fs_solver_t* s, int l) {
 template <typename S>
 int launch_coarse_s(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
+  if (from == 0)  // the whole hierarchy in one workgroup: its result is lv[0].xb, FP32
+    return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
+                  dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
+                  s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
   return launch(s, PK_COARSE, 2.0 * sizeof(S) * C.n * C.n, fsk::k_mg_coarse_ml<S>, dim3(1), dim3(fsk::CTH),
                 as<S>(C.xb), as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()), s->cfg.mg_pre,
                 s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
@@ -704,9 +712,9 @@ template <typename S>
 int enqueue_residual_s(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  double bpc = 8 + (zero_p ? 0 : 8) + (e ? sizeof(S) : 0) + 8 + sizeof(S);
+  double bpc = 8 + (zero_p ? 0 : 8) + (e ? 4 : 0) + 8 + sizeof(S);
   return launch(s, PK_RESIDUAL, bpc * s->n, fsk::k_residual64<S>, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
-                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), as<S>(e), s->p64[1 - pin],
+                (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), e, s->p64[1 - pin],
                 as<S>(s->lv[0].rhs), s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
 }
 int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
@@ -1073,7 +1081,10 @@ int build_levels(fs_solver_t* s) {
     int rc = dalloc(&s->lv[l].x, w);
     if (rc) return rc;
     if ((rc = dalloc(&s->lv[l].rhs, w))) return rc;
-    if (s->fused && (rc = dalloc(&s->lv[l].xb, w))) return rc;
+    // the level-0 correction is FP32 under every storage: it goes into the FP64
+    // pressure, and rounding it to BF16 puts back a residual of the size of the one
+    // the V-cycle removed
+    if (s->fused && (rc = dalloc(&s->lv[l].xb, l == 0 ? c : w))) return rc;
   }
   return build_coarse_inverse(s);
 }
@@ -2055,7 +2066,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     size_t w = s->bf16 ? (size_t)l.n * l.n / 2 : (size_t)l.n * l.n;
     if (!rc) rc = fill_f32(s, l.x, w, 0.f);
     if (!rc) rc = fill_f32(s, l.rhs, w, 0.f);
-    if (!rc && l.xb) rc = fill_f32(s, l.xb, w, 0.f);
+    if (!rc && l.xb) rc = fill_f32(s, l.xb, &l == &s->lv[0] ? (size_t)l.n * l.n : w, 0.f);
   }
   // on zero inputs every fused kernel writes zeros, so tuning leaves the state as it is
   if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) rc = autotune_tiles(s);
