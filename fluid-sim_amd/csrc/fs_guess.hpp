@@ -1,5 +1,5 @@
 // fs_guess.hpp -- learned initial pressure guess: a 4-level U-Net evaluated with
-// BF16 matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation).
+// BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
 //
 // The network maps (r / s_r, p_prev / s_p), where r = div u* - L p_prev is the
 // divergence the previous pressure leaves and s_r, s_p are their maxima, to an
@@ -26,7 +26,11 @@
 namespace fsk {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int GT_W = 16, GT_NT = 256;  // tile width (one MFMA M-block) and threads
 
@@ -107,142 +111,108 @@ __device__ __forceinline__ uint32_t bf_pair_masked(float lo, float hi, uint32_t 
   uint32_t b = (act2 & 0x80000000u) || !(act2 & 0x7fff0000u) ? 0u : (uint32_t)bf_bits(hi);
   return a | (b << 16);
 }
-__device__ __forceinline__ uint4 bf8_masked(const float* v, uint4 act) {
-  return uint4{bf_pair_masked(v[0], v[1], act.x), bf_pair_masked(v[2], v[3], act.y),
-               bf_pair_masked(v[4], v[5], act.z), bf_pair_masked(v[6], v[7], act.w)};
+
+// Two FP32 values to a packed BF16 pair, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t bf_pack(float lo, float hi) {
+  bf16x2v b = __builtin_convertvector(f32x2v{lo, hi}, bf16x2v);
+  return __builtin_bit_cast(uint32_t, b);
 }
 
-// Backward epilogues of k_conv3x3 (EPI_MASK / EPI_UNPOOL / EPI_UPCAT) over one staged
-// tile: ofl[pixel][channel] holds the FP32 accumulators of TH x 16 pixels and the OC
-// channels [co0, co0 + OC) of image z; 8 channels per item.
-template <int COUT, int TH, int OC, int OFP, int EPI, int OS>
-__device__ __forceinline__ void conv_bwd_epilogue(const float* ofl, uint16_t* __restrict__ y, const ConvEpiArgs& e,
-                                                  int m, int z, int tx0, int ty0, int co0) {
-  constexpr int C8 = OC / 8;
-  if (EPI == EPI_MASK) {
-    const size_t base = (size_t)z * m * m * COUT;
-    for (int q = threadIdx.x; q < TH * GT_W * C8; q += GT_NT) {
-      int pix = q / C8, cg = q - pix * C8;
-      size_t k = base + ((size_t)(ty0 + pix / GT_W) * m + tx0 + pix % GT_W) * COUT + co0 + cg * 8;
-      uint4 act = *(const uint4*)(e.act + k);
-      *(uint4*)(y + k) = bf8_masked(ofl + pix * OFP + cg * 8, act);
-    }
-  } else if (EPI == EPI_UNPOOL) {
-    const int m2 = 2 * m;
-    const size_t base = (size_t)z * m2 * m2 * COUT;
-    for (int q = threadIdx.x; q < 4 * TH * GT_W * C8; q += GT_NT) {
-      int f = q / C8, cg = q - f * C8;                  // fine pixel of the 2TH x 32 block
-      int fy = f / (2 * GT_W), fx = f - fy * (2 * GT_W);
-      int pix = (fy >> 1) * GT_W + (fx >> 1);
-      size_t k = base + ((size_t)(2 * ty0 + fy) * m2 + 2 * tx0 + fx) * COUT + co0 + cg * 8;
-      float g[8];
-      const float* a = ofl + pix * OFP + cg * 8;
-      if (e.gin) {
-        float4 g0 = *(const float4*)(e.gin + k), g1 = *(const float4*)(e.gin + k + 4);
-        g[0] = 0.25f * a[0] + g0.x; g[1] = 0.25f * a[1] + g0.y; g[2] = 0.25f * a[2] + g0.z; g[3] = 0.25f * a[3] + g0.w;
-        g[4] = 0.25f * a[4] + g1.x; g[5] = 0.25f * a[5] + g1.y; g[6] = 0.25f * a[6] + g1.z; g[7] = 0.25f * a[7] + g1.w;
-      } else {
-#pragma unroll
-        for (int c = 0; c < 8; c++) g[c] = 0.25f * a[c];
-      }
-      uint4 act = *(const uint4*)(e.act + k);
-      *(uint4*)(y + k) = bf8_masked(g, act);
-    }
-  } else if (EPI == EPI_UPCAT) {
-    if (co0 < OS) {
-      const int mc = m >> 1;
-      const size_t base = (size_t)z * mc * mc * OS;
-      for (int q = threadIdx.x; q < (TH / 2) * (GT_W / 2) * C8; q += GT_NT) {
-        int cp = q / C8, cg = q - cp * C8;
-        int cy = cp / (GT_W / 2), cx = cp - cy * (GT_W / 2);
-        const float* a00 = ofl + ((2 * cy) * GT_W + 2 * cx) * OFP + cg * 8;
-        float g[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) g[c] = (a00[c] + a00[OFP + c]) + (a00[GT_W * OFP + c] + a00[(GT_W + 1) * OFP + c]);
-        size_t k = base + ((size_t)((ty0 >> 1) + cy) * mc + (tx0 >> 1) + cx) * OS + co0 + cg * 8;
-        uint4 act = *(const uint4*)(e.act + k);
-        *(uint4*)(y + k) = bf8_masked(g, act);
-      }
-    } else {
-      constexpr int CS = COUT - OS;
-      const size_t base = (size_t)z * m * m * CS;
-      for (int q = threadIdx.x; q < TH * GT_W * C8; q += GT_NT) {
-        int pix = q / C8, cg = q - pix * C8;
-        size_t k = base + ((size_t)(ty0 + pix / GT_W) * m + tx0 + pix % GT_W) * CS + (co0 - OS) + cg * 8;
-        const float* a = ofl + pix * OFP + cg * 8;
-        *(float4*)(e.gout + k) = float4{a[0], a[1], a[2], a[3]};
-        *(float4*)(e.gout + k + 4) = float4{a[4], a[5], a[6], a[7]};
-      }
-    }
-  }
+// LDS bytes of k_conv3x3: the input tile with its halo and the weight slice.
+template <int CIN, int TH, int NBW>
+constexpr int conv_lds_bytes() {
+  return 2 * ((TH + 2) * (GT_W + 2) * (CIN + 8) + NBW * 16 * (9 * CIN + 8));
 }
 
 // 3x3 periodic convolution, CIN -> COUT channels at an m x m level, over nimg images
 // (tensors of consecutive images follow each other). Input modes as in conv_in_item.
 // w: BF16 [COUT][9][CIN] (tap = 3 (dy + 1) + dx + 1); bias FP32 [COUT] (EPI_RELU).
-// A workgroup computes TH rows x 16 columns of pixels and NBW blocks of 16 output
-// channels (blockIdx.y selects which); each wavefront owns TH/4 rows (one MFMA
-// M-block each). The workgroup's weight slice and its input tile (with the
-// one-pixel halo) are staged in LDS, so every B fragment feeds TH/4 MFMAs from LDS.
-template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int EPI = EPI_RELU, int OS = 0>
-__global__ void __launch_bounds__(GT_NT)
+//
+// A workgroup of 4 WC wavefronts computes TH rows x 16 columns of pixels and NBW
+// blocks of 16 output channels (blockIdx.y selects which); wavefront (wr, wc) owns
+// rows [wr TH/4, (wr + 1) TH/4) and channel blocks [wc NBW/WC, (wc + 1) NBW/WC). The
+// weight slice stays in LDS for all of the workgroup's tiles; each tile is staged in
+// LDS with its one-pixel periodic halo (the next tile's loads are issued while the
+// current one is multiplied).
+//
+// Implicit GEMM per tap: D[channel][pixel] += W[channel][k] X[k][pixel], with the
+// weights as the MFMA A operand and 16 pixels of a row as B, on
+// v_mfma_f32_16x16x32_bf16 (32 input channels per step; a 16-channel remainder on
+// v_mfma_f32_16x16x16_bf16). Lane l holds channels 4 (l / 16) + q of pixel l % 16, so
+// the epilogue writes 4 channels of a pixel per lane straight to memory. An LDS row
+// of pixels (shifted by dx) serves output rows lr, lr - 1, lr - 2 through taps
+// dy = 0, 1, 2, so it is read once per (dx, channel step).
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int EPI = EPI_RELU, int OS = 0, int WC = 1>
+__global__ void __launch_bounds__(GT_NT * WC)
 k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, const uint16_t* __restrict__ w,
           const float* __restrict__ bias, uint16_t* __restrict__ y, int m, int nimg, ConvEpiArgs e) {
-  static_assert(CIN % 16 == 0 && COUT % (16 * NBW) == 0 && TH % 4 == 0, "tile shape");
-  static_assert(EPI != EPI_UPCAT || (OS % (16 * NBW) == 0 && OS < COUT), "split on a channel block");
-  constexpr int PS = CIN + 8;          // LDS pixel stride (bf16): 16-byte aligned, breaks A-read bank conflicts
-  constexpr int WS = 9 * CIN + 4;      // LDS weight row stride (bf16): breaks B-read bank conflicts
-  constexpr int CH8 = CIN / 8;
+  constexpr int NT = GT_NT * WC, R = TH / 4, N = NBW / WC;
+  static_assert(CIN % 16 == 0 && COUT % (16 * NBW) == 0 && TH % 4 == 0 && NBW % WC == 0, "tile shape");
+  static_assert(EPI != EPI_UPCAT || (OS % (16 * NBW) == 0 && OS < COUT && R % 2 == 0), "split on a channel block");
+  constexpr int PS = CIN + 8;      // LDS pixel stride (bf16): 16-byte rows, 16 pixels hit distinct banks
+  constexpr int WS = 9 * CIN + 8;  // LDS weight row stride (bf16): the same for 16 channel rows
   constexpr int LH = TH + 2, LW = GT_W + 2;
-  constexpr int MB = TH / 4;           // M-blocks (rows) per wavefront
-  constexpr int OC = NBW * 16, OCP = OC + 8;  // output channels of the workgroup, staged stride
-  constexpr int OFP = OC + 4;                 // FP32 staged stride (backward epilogues)
+  constexpr int OC = NBW * 16;
+  constexpr int K32 = CIN / 32, K16 = (CIN % 32) / 16;
   __shared__ __align__(16) uint16_t tile[LH * LW * PS];
-  __shared__ __align__(16) uint16_t wl[NBW * 16 * WS];
-  __shared__ __align__(16) uint16_t otile[EPI == EPI_RELU ? TH * GT_W * OCP : 8];
-  __shared__ __align__(16) float ofl[EPI == EPI_RELU ? 4 : TH * GT_W * OFP];
+  __shared__ __align__(16) uint16_t wl[OC * WS];
   const int tiles_x = m / GT_W, tpi = tiles_x * (m / TH), ntiles = tpi * nimg;
-  const int co0 = blockIdx.y * NBW * 16;
+  const int co0 = blockIdx.y * OC;
   // ---- weights of this workgroup's output channels, once for all its tiles
   {
-    constexpr int RW8 = 9 * CIN / 8, WI = NBW * 16 * RW8;  // 16-byte items per row, in total
-    auto wsrc = [&](int q) { return (const uint4*)(w + (size_t)(co0 + q / RW8) * 9 * CIN + (q % RW8) * 8); };
-    auto wdst = [&](int q) { return (uint2*)(wl + (q / RW8) * WS + (q % RW8) * 8); };
-    for (int q0 = threadIdx.x; q0 < WI; q0 += 4 * GT_NT) {
-      const int q1 = q0 + GT_NT, q2 = q1 + GT_NT, q3 = q2 + GT_NT;
-      uint4 v0 = *wsrc(q0), v1, v2, v3;
-      if (q1 < WI) v1 = *wsrc(q1);
-      if (q2 < WI) v2 = *wsrc(q2);
-      if (q3 < WI) v3 = *wsrc(q3);
-      // the weight rows are 8-byte aligned in LDS (stride 9 CIN + 4), so two 8-byte stores
-      wdst(q0)[0] = uint2{v0.x, v0.y};
-      wdst(q0)[1] = uint2{v0.z, v0.w};
-      if (q1 < WI) { wdst(q1)[0] = uint2{v1.x, v1.y}; wdst(q1)[1] = uint2{v1.z, v1.w}; }
-      if (q2 < WI) { wdst(q2)[0] = uint2{v2.x, v2.y}; wdst(q2)[1] = uint2{v2.z, v2.w}; }
-      if (q3 < WI) { wdst(q3)[0] = uint2{v3.x, v3.y}; wdst(q3)[1] = uint2{v3.z, v3.w}; }
+    constexpr int RW8 = 9 * CIN / 8, WI = OC * RW8;  // 16-byte items per row, in total
+#pragma unroll 4
+    for (int q = threadIdx.x; q < WI; q += NT) {
+      const int o = q / RW8, k8 = q - o * RW8;
+      *(uint4*)(wl + o * WS + k8 * 8) = *(const uint4*)(w + (size_t)(co0 + o) * 9 * CIN + k8 * 8);
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  float bv[NBW];
+  const int wr = wave & 3, wc = wave >> 2;
+  const int li = lane & 15, lg = lane >> 4;
+  const int cb0 = wc * N;  // first channel block of this wavefront
+  float bv[N][4];
 #pragma unroll
-  for (int nb = 0; nb < NBW; nb++) bv[nb] = EPI == EPI_RELU ? bias[co0 + nb * 16 + li] : 0.f;
-  // ---- input tiles (with halo), 8 channels (16 bytes) per item; the next tile's
-  // items are loaded into registers while the current one is multiplied
-  constexpr int TI = LH * LW * CH8, TK = (TI + GT_NT - 1) / GT_NT;
-  uint4 pre[TK];
+  for (int nb = 0; nb < N; nb++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) bv[nb][q] = EPI == EPI_RELU ? bias[co0 + (cb0 + nb) * 16 + 4 * lg + q] : 0.f;
+  // ---- input tile items (8 channels, 16 bytes). An upsample-concatenation input is
+  // staged in two passes, the C2 upsampled channels (part A) and the rest (part B), so
+  // each item loop reads one tensor; the other modes have part A only.
+  constexpr int CA = MODE == CONV_UPCAT ? C2 : CIN, CB = CIN - CA;
+  constexpr int IA = LH * LW * (CA / 8), IB = LH * LW * (CB / 8);
+  constexpr int KA = (IA + NT - 1) / NT, KB = CB > 0 ? (IB + NT - 1) / NT : 0;
+  constexpr int CB8 = CB > 0 ? CB / 8 : 1;  // items per pixel of part B
+  // staged in native vectors (the HIP uint4 struct made the compiler keep them in scratch)
+  u32x4 preA[KA], preB[KB > 0 ? KB : 1];
   auto prefetch = [&](int t) {
     const int z = t / tpi, tt = t - z * tpi;
     const int tx0 = (tt % tiles_x) * GT_W, ty0 = (tt / tiles_x) * TH;
     const uint16_t* x1z = x1 + z * conv_x1_image<CIN, MODE, C2>(m);
-    const uint16_t* x2z = MODE == CONV_UPCAT ? x2 + z * conv_x2_image<C2>(m) : x2;
 #pragma unroll
-    for (int k = 0; k < TK; k++) {
-      int q = threadIdx.x + k * GT_NT;
-      if (q < TI) {
-        int pix = q / CH8, cg = q - pix * CH8;
-        int ly = pix / LW, lx = pix - ly * LW;
-        pre[k] = conv_in_item<CIN, MODE, C2>(x1z, x2z, m, wrapm(ty0 - 1 + ly, m), wrapm(tx0 - 1 + lx, m), cg * 8);
+    for (int k = 0; k < KA; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < IA) {
+        const int pix = q / (CA / 8), cg = q - pix * (CA / 8);
+        const int ly = pix / LW, lx = pix - ly * LW;
+        const int gy = wrapm(ty0 - 1 + ly, m), gx = wrapm(tx0 - 1 + lx, m);
+        if (MODE == CONV_UPCAT)
+          preA[k] = *(const u32x4*)(x2 + z * conv_x2_image<C2>(m) + ((size_t)(gy >> 1) * (m >> 1) + (gx >> 1)) * C2 +
+                                    cg * 8);
+        else {
+          const uint4 v = conv_in_item<CIN, MODE, C2>(x1z, nullptr, m, gy, gx, cg * 8);
+          preA[k] = u32x4{v.x, v.y, v.z, v.w};
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < IB) {
+        const int pix = q / CB8, cg = q - pix * CB8;
+        const int ly = pix / LW, lx = pix - ly * LW;
+        const int gy = wrapm(ty0 - 1 + ly, m), gx = wrapm(tx0 - 1 + lx, m);
+        preB[k] = *(const u32x4*)(x1z + ((size_t)gy * m + gx) * CB + cg * 8);
       }
     }
   };
@@ -251,81 +221,174 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
     const int z = t / tpi, tt = t - z * tpi;
     const int tx0 = (tt % tiles_x) * GT_W, ty0 = (tt / tiles_x) * TH;
 #pragma unroll
-    for (int k = 0; k < TK; k++) {
-      int q = threadIdx.x + k * GT_NT;
-      if (q < TI) {
-        int pix = q / CH8, cg = q - pix * CH8;
-        *(uint4*)(tile + pix * PS + cg * 8) = pre[k];
+    for (int k = 0; k < KA; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < IA) {
+        const int pix = q / (CA / 8), cg = q - pix * (CA / 8);
+        *(u32x4*)(tile + pix * PS + cg * 8) = preA[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < IB) {
+        const int pix = q / CB8, cg = q - pix * CB8;
+        *(u32x4*)(tile + pix * PS + CA + cg * 8) = preB[k];
       }
     }
     __syncthreads();
     if (t + (int)gridDim.x < ntiles) prefetch(t + gridDim.x);
-    // ---- implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input
-    // channels of one tap. A fragment of LDS row lr (shifted by dx) serves output rows
-    // lr, lr-1, lr-2 through taps dy = 0, 1, 2, so each is read once per (dx, kc).
-    f32x4 acc[MB][NBW];
+    f32x4 acc[R][N];
 #pragma unroll
-    for (int r = 0; r < MB; r++)
+    for (int r = 0; r < R; r++)
 #pragma unroll
-      for (int nb = 0; nb < NBW; nb++) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int nb = 0; nb < N; nb++) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wrow = wl + (cb0 * 16 + li) * WS + lg * 8;
+    const uint16_t* trow = tile + (wr * R * LW + li) * PS + lg * 8;
 #pragma unroll
-    for (int kc = 0; kc < CIN / 16; kc++) {
+    for (int kc = 0; kc < K32; kc++) {
 #pragma unroll
       for (int dx = 0; dx < 3; dx++) {
-        bf16x4 bfrag[3][NBW];
+        bf16x8v af[3][N];
 #pragma unroll
         for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-          for (int nb = 0; nb < NBW; nb++)
-            bfrag[dy][nb] = *(const bf16x4*)(wl + (nb * 16 + li) * WS + (dy * 3 + dx) * CIN + kc * 16 + lk);
+          for (int nb = 0; nb < N; nb++)
+            af[dy][nb] = *(const bf16x8v*)(wrow + nb * 16 * WS + (dy * 3 + dx) * CIN + kc * 32);
 #pragma unroll
-        for (int lr = 0; lr < MB + 2; lr++) {
-          bf16x4 afrag = *(const bf16x4*)(tile + ((wave * MB + lr) * LW + (li + dx)) * PS + kc * 16 + lk);
+        for (int lr = 0; lr < R + 2; lr++) {
+          const bf16x8v bx = *(const bf16x8v*)(trow + (lr * LW + dx) * PS + kc * 32);
 #pragma unroll
           for (int dy = 0; dy < 3; dy++) {
             const int r = lr - dy;
-            if (r >= 0 && r < MB) {
+            if (r >= 0 && r < R) {
 #pragma unroll
-              for (int nb = 0; nb < NBW; nb++)
-                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afrag, bfrag[dy][nb], acc[r][nb], 0, 0, 0);
+              for (int nb = 0; nb < N; nb++)
+                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[dy][nb], bx, acc[r][nb], 0, 0, 0);
             }
           }
         }
       }
     }
-    // ---- epilogue: lane holds D[pixel 4*(lane/16) + q][channel lane%16] of each block;
-    // staged through LDS so that each pixel's channels leave as 16-byte stores
-    if (EPI == EPI_RELU) {
+    if constexpr (K16 > 0) {
+      // the last 16 channels: lane l holds k = 4 (l / 16) + j
+      const uint16_t* wrow16 = wl + (cb0 * 16 + li) * WS + K32 * 32 + lg * 4;
+      const uint16_t* trow16 = tile + (wr * R * LW + li) * PS + K32 * 32 + lg * 4;
 #pragma unroll
-      for (int r = 0; r < MB; r++) {
-        int row = wave * MB + r;
+      for (int dx = 0; dx < 3; dx++) {
+        bf16x4 af[3][N];
 #pragma unroll
-        for (int nb = 0; nb < NBW; nb++)
+        for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            otile[(row * GT_W + lk + q) * OCP + nb * 16 + li] = bf_bits(fmaxf(acc[r][nb][q] + bv[nb], 0.f));
+          for (int nb = 0; nb < N; nb++) af[dy][nb] = *(const bf16x4*)(wrow16 + nb * 16 * WS + (dy * 3 + dx) * CIN);
+#pragma unroll
+        for (int lr = 0; lr < R + 2; lr++) {
+          const bf16x4 bx = *(const bf16x4*)(trow16 + (lr * LW + dx) * PS);
+#pragma unroll
+          for (int dy = 0; dy < 3; dy++) {
+            const int r = lr - dy;
+            if (r >= 0 && r < R) {
+#pragma unroll
+              for (int nb = 0; nb < N; nb++)
+                acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[dy][nb], bx, acc[r][nb], 0, 0, 0);
+            }
+          }
+        }
       }
-      __syncthreads();
-      uint16_t* yz = y + (size_t)z * m * m * COUT;
-      constexpr int OI = TH * GT_W * (OC / 8);
-      for (int q = threadIdx.x; q < OI; q += GT_NT) {
-        int pix = q / (OC / 8), cg = q - pix * (OC / 8);
-        int py = ty0 + pix / GT_W, px = tx0 + (pix % GT_W);
-        *(uint4*)(yz + ((size_t)py * m + px) * COUT + co0 + cg * 8) = *(const uint4*)(otile + pix * OCP + cg * 8);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < MB; r++) {
-        int row = wave * MB + r;
-#pragma unroll
-        for (int nb = 0; nb < NBW; nb++)
-#pragma unroll
-          for (int q = 0; q < 4; q++) ofl[(row * GT_W + lk + q) * OFP + nb * 16 + li] = acc[r][nb][q];
-      }
-      __syncthreads();
-      conv_bwd_epilogue<COUT, TH, OC, OFP, EPI, OS>(ofl, y, e, m, z, tx0, ty0, co0);
     }
-    __syncthreads();  // the next tile overwrites the staged input and outputs
+    __syncthreads();  // the next tile overwrites the staged input
+    // ---- epilogue: lane holds channels c .. c + 3 of pixel (py, px) of each (row, block)
+    const int px = tx0 + li;
+    if (EPI == EPI_RELU || EPI == EPI_MASK) {
+      const size_t base = (size_t)z * m * m * COUT;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int py = ty0 + wr * R + r;
+#pragma unroll
+        for (int nb = 0; nb < N; nb++) {
+          const size_t k = base + ((size_t)py * m + px) * COUT + co0 + (cb0 + nb) * 16 + 4 * lg;
+          const f32x4 a = acc[r][nb];
+          uint2 o;
+          if (EPI == EPI_RELU) {
+            o = uint2{bf_pack(fmaxf(a[0] + bv[nb][0], 0.f), fmaxf(a[1] + bv[nb][1], 0.f)),
+                      bf_pack(fmaxf(a[2] + bv[nb][2], 0.f), fmaxf(a[3] + bv[nb][3], 0.f))};
+          } else {
+            const uint2 act = *(const uint2*)(e.act + k);
+            o = uint2{bf_pair_masked(a[0], a[1], act.x), bf_pair_masked(a[2], a[3], act.y)};
+          }
+          *(uint2*)(y + k) = o;
+        }
+      }
+    } else if (EPI == EPI_UNPOOL) {
+      // the gradient of a 2x2-averaged input: a quarter to each fine pixel, plus the skip
+      const int m2 = 2 * m;
+      const size_t base = (size_t)z * m2 * m2 * COUT;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int py = ty0 + wr * R + r;
+#pragma unroll
+        for (int nb = 0; nb < N; nb++) {
+          const f32x4 a = acc[r][nb];
+#pragma unroll
+          for (int f = 0; f < 4; f++) {
+            const size_t k =
+                base + ((size_t)(2 * py + (f >> 1)) * m2 + 2 * px + (f & 1)) * COUT + co0 + (cb0 + nb) * 16 + 4 * lg;
+            float g[4];
+            if (e.gin) {
+              const float4 g0 = *(const float4*)(e.gin + k);
+              g[0] = 0.25f * a[0] + g0.x;
+              g[1] = 0.25f * a[1] + g0.y;
+              g[2] = 0.25f * a[2] + g0.z;
+              g[3] = 0.25f * a[3] + g0.w;
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; q++) g[q] = 0.25f * a[q];
+            }
+            const uint2 act = *(const uint2*)(e.act + k);
+            *(uint2*)(y + k) = uint2{bf_pair_masked(g[0], g[1], act.x), bf_pair_masked(g[2], g[3], act.y)};
+          }
+        }
+      }
+    } else if (EPI == EPI_UPCAT) {
+      if (co0 < OS) {
+        // the upsampled channels: sum over each 2x2 block, (a00 + a01) + (a10 + a11); the
+        // column pair sits in lanes l, l ^ 1 and the row pair in rows r, r + 1
+        const int mc = m >> 1;
+        const size_t base = (size_t)z * mc * mc * OS;
+#pragma unroll
+        for (int r = 0; r < R; r += 2) {
+          const int cy = (ty0 + wr * R + r) >> 1, cx = px >> 1;
+#pragma unroll
+          for (int nb = 0; nb < N; nb++) {
+            float g[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const float a0 = acc[r][nb][q], a1 = acc[r + 1][nb][q];
+              g[q] = (a0 + __shfl_xor(a0, 1)) + (a1 + __shfl_xor(a1, 1));
+            }
+            if ((li & 1) == 0) {
+              const size_t k = base + ((size_t)cy * mc + cx) * OS + co0 + (cb0 + nb) * 16 + 4 * lg;
+              const uint2 act = *(const uint2*)(e.act + k);
+              *(uint2*)(y + k) = uint2{bf_pair_masked(g[0], g[1], act.x), bf_pair_masked(g[2], g[3], act.y)};
+            }
+          }
+        }
+      } else {
+        // the skip channels: FP32, unrounded
+        constexpr int CS = COUT - OS;
+        const size_t base = (size_t)z * m * m * CS;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int py = ty0 + wr * R + r;
+#pragma unroll
+          for (int nb = 0; nb < N; nb++) {
+            const size_t k = base + ((size_t)py * m + px) * CS + (co0 - OS) + (cb0 + nb) * 16 + 4 * lg;
+            const f32x4 a = acc[r][nb];
+            *(float4*)(e.gout + k) = float4{a[0], a[1], a[2], a[3]};
+          }
+        }
+      }
+    }
   }
 }
 

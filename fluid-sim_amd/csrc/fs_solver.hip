@@ -792,7 +792,7 @@ static const int kGuessIn[15][2] = {{-1, -1}, {0, -1}, {1, -1}, {2, -1}, {3, -1}
 // Tile rows and output-channel blocks per workgroup, by level: large tiles where
 // the level is big, output channels split across workgroups where it is small, so
 // every level launches at least one workgroup per CU and the LDS holds the weights.
-template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int WC = 1>
 int launch_conv(fs_solver_t* s, int layer) {
   auto& G = s->guess_net;
   const int m = s->nx >> kGuessLayers[layer].level;
@@ -801,14 +801,13 @@ int launch_conv(fs_solver_t* s, int layer) {
   // persistent workgroups: each stages its weight slice once and loops over tiles;
   // as many per CU as the LDS holds (at most 8), over all channel blocks
   const int ntiles = (m / fsk::GT_W) * (m / TH), nyb = COUT / (16 * NBW);
-  const int lds = 2 * ((TH + 2) * (fsk::GT_W + 2) * (CIN + 8) + NBW * 16 * (9 * CIN + 4) +
-                       TH * fsk::GT_W * (NBW * 16 + 8));
+  const int lds = fsk::conv_lds_bytes<CIN, TH, NBW>();
   const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
   dim3 g((unsigned)std::max(1, std::min(ntiles, 256 * per_cu / nyb)), (unsigned)nyb);
   // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and out, BF16
   double in_ch = MODE == fsk::CONV_POOL ? 4.0 * CIN : MODE == fsk::CONV_UPCAT ? C2 / 4.0 + (CIN - C2) : CIN;
-  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW>, g,
-                dim3(fsk::GT_NT), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m, 1,
+  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW, fsk::EPI_RELU, 0, WC>,
+                g, dim3(fsk::GT_NT * WC), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m, 1,
                 fsk::ConvEpiArgs{});
 }
 
@@ -831,21 +830,22 @@ int run_guess(fs_solver_t* s, int pi) {
     fsk::k_guess_first<float><<<g0, 256, 0, s->stream>>>(s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0],
                                                           n);
   LAUNCHCHK();
-  // tile rows and channel blocks per layer: deep tiles where the weights are small,
-  // shallow tiles and split channels where input + weights would crowd the LDS
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 16, 1>(s, 1))) return rc;
-  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0, 16, 2>(s, 2))) return rc;
+  // tile rows, channel blocks and wavefront columns per layer (timed at 1024^2): deep
+  // tiles where the weights are small, channels split over workgroups or wavefronts
+  // where input + weights would crowd the LDS
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1>(s, 1))) return rc;
+  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0, 8, 2, 2>(s, 2))) return rc;
   if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 3))) return rc;
-  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0, 8, 2>(s, 4))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 2>(s, 5))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0, 8, 1>(s, 6))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 1>(s, 7))) return rc;
-  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64, 8, 2>(s, 8))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 8, 2>(s, 9))) return rc;
-  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64, 16, 2>(s, 10))) return rc;
+  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0, 8, 4, 2>(s, 4))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 4, 2>(s, 5))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0, 8, 2, 2>(s, 6))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 1>(s, 7))) return rc;
+  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64, 16, 2, 2>(s, 8))) return rc;
+  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 4, 2>(s, 9))) return rc;
+  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64, 16, 2, 2>(s, 10))) return rc;
   if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 11))) return rc;
   if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32, 16, 1>(s, 12))) return rc;
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 16, 1>(s, 13))) return rc;
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1>(s, 13))) return rc;
   unsigned blocks = (unsigned)((s->n + 255) / 256);
   fsk::k_guess_out<<<blocks, 256, 0, s->stream>>>(G.act[13], G.wout, G.b[14], G.scales, s->p64[pi], n,
                                                   s->dx * s->dx);
@@ -1769,14 +1769,14 @@ int train_reduce(fs_trainer_t* t, const float* part, int rows, size_t len, float
   return tlaunch(t, fsk::k_tr_reduce, dim3(gx), dim3(256), (const float*)t->part2, groups, len, dst);
 }
 
-template <int CIN, int COUT, int MODE, int C2, int TH, int NBW>
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int WC = 1>
 int train_fwd(fs_trainer_t* t, int layer) {
   const int m = t->n >> kGuessLayers[layer].level;
   const uint16_t* x1 = t->act[kGuessIn[layer][0]];
   const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? t->act[kGuessIn[layer][1]] : nullptr;
   const int ntiles = (m / fsk::GT_W) * (m / TH) * t->B, nyb = COUT / (16 * NBW);
   dim3 g((unsigned)std::max(1, std::min(ntiles, 2048 / nyb)), (unsigned)nyb);
-  return tlaunch(t, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW>, g, dim3(fsk::GT_NT), x1, x2,
+  return tlaunch(t, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW, fsk::EPI_RELU, 0, WC>, g, dim3(fsk::GT_NT * WC), x1, x2,
                  (const uint16_t*)t->wf[layer], (const float*)(t->par + t->boff[layer]), t->act[layer], m, t->B,
                  fsk::ConvEpiArgs{});
 }
@@ -1860,19 +1860,19 @@ int train_forward_backward(fs_trainer_t* t) {
                     (const float*)t->r32, (const double*)t->pp, (const double*)t->scales,
                     (const float*)(t->par + t->woff[0]), (const float*)(t->par + t->boff[0]), t->act[0], n)))
     return rc;
-  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 16, 1>(t, 1))) return rc;
-  if ((rc = train_fwd<16, 32, CONV_POOL, 0, 16, 2>(t, 2))) return rc;
+  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 8, 1>(t, 1))) return rc;
+  if ((rc = train_fwd<16, 32, CONV_POOL, 0, 8, 2, 2>(t, 2))) return rc;
   if ((rc = train_fwd<32, 32, CONV_DIRECT, 0, 16, 2>(t, 3))) return rc;
-  if ((rc = train_fwd<32, 64, CONV_POOL, 0, 8, 2>(t, 4))) return rc;
-  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 2>(t, 5))) return rc;
-  if ((rc = train_fwd<64, 64, CONV_POOL, 0, 8, 1>(t, 6))) return rc;
-  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 1>(t, 7))) return rc;
-  if ((rc = train_fwd<128, 64, CONV_UPCAT, 64, 8, 2>(t, 8))) return rc;
-  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 8, 2>(t, 9))) return rc;
-  if ((rc = train_fwd<96, 32, CONV_UPCAT, 64, 16, 2>(t, 10))) return rc;
+  if ((rc = train_fwd<32, 64, CONV_POOL, 0, 8, 4, 2>(t, 4))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 16, 4, 2>(t, 5))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_POOL, 0, 8, 2, 2>(t, 6))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 16, 1>(t, 7))) return rc;
+  if ((rc = train_fwd<128, 64, CONV_UPCAT, 64, 16, 2, 2>(t, 8))) return rc;
+  if ((rc = train_fwd<64, 64, CONV_DIRECT, 0, 16, 4, 2>(t, 9))) return rc;
+  if ((rc = train_fwd<96, 32, CONV_UPCAT, 64, 16, 2, 2>(t, 10))) return rc;
   if ((rc = train_fwd<32, 32, CONV_DIRECT, 0, 16, 2>(t, 11))) return rc;
   if ((rc = train_fwd<48, 16, CONV_UPCAT, 32, 16, 1>(t, 12))) return rc;
-  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 16, 1>(t, 13))) return rc;
+  if ((rc = train_fwd<16, 16, CONV_DIRECT, 0, 8, 1>(t, 13))) return rc;
   // ---- output layer: loss, its gradients, gz of dec0b
   const size_t count = nn * B;
   if ((rc = tlaunch(t, k_tr_out, dim3((unsigned)t->oblk), dim3(256), (const uint16_t*)t->act[13],
