@@ -99,11 +99,20 @@ __device__ __forceinline__ size_t conv_x2_image(int m) {
 //               [0, OS) are summed over each 2x2 block into the coarse pixel and
 //               y = BF16(sum) where e.act > 0 (y, e.act: OS channels at m/2); channels
 //               [OS, COUT) go unrounded to e.gout (FP32, COUT - OS channels at m).
-enum ConvEpi { EPI_RELU = 0, EPI_MASK = 1, EPI_UNPOOL = 2, EPI_UPCAT = 3 };
+//   EPI_OUT:    the last 3x3 layer of inference (COUT = 16) with the 1x1 output layer
+//               after it: the BF16 activations relu(acc + bias) are not stored; each
+//               pixel's o = e.bout + sum_c e.wout[c] a_c (channels in order) updates the
+//               pressure, e.p += e.scales[0] e.h2 o (FP64, one image).
+enum ConvEpi { EPI_RELU = 0, EPI_MASK = 1, EPI_UNPOOL = 2, EPI_UPCAT = 3, EPI_OUT = 4 };
 struct ConvEpiArgs {
   const uint16_t* act;
   const float* gin;
   float* gout;
+  const float* wout;  // EPI_OUT: 1x1 weights [16] and bias [1]
+  const float* bout;
+  const double* scales;
+  double* p;
+  double h2;
 };
 
 __device__ __forceinline__ uint32_t bf_pair_masked(float lo, float hi, uint32_t act2) {
@@ -149,6 +158,7 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
   constexpr int NT = GT_NT * WC, R = TH / 4, N = NBW / WC;
   static_assert(CIN % 16 == 0 && COUT % (16 * NBW) == 0 && TH % 4 == 0 && NBW % WC == 0, "tile shape");
   static_assert(EPI != EPI_UPCAT || (OS % (16 * NBW) == 0 && OS < COUT && R % 2 == 0), "split on a channel block");
+  static_assert(EPI != EPI_OUT || (COUT == 16 && NBW == 1), "the output layer reads 16 channels");
   constexpr int PS = CIN + 8;      // LDS pixel stride (bf16): 16-byte rows, 16 pixels hit distinct banks
   constexpr int WS = 9 * CIN + 8;  // LDS weight row stride (bf16): the same for 16 channel rows
   constexpr int LH = TH + 2, LW = GT_W + 2;
@@ -175,7 +185,8 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
 #pragma unroll
   for (int nb = 0; nb < N; nb++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) bv[nb][q] = EPI == EPI_RELU ? bias[co0 + (cb0 + nb) * 16 + 4 * lg + q] : 0.f;
+    for (int q = 0; q < 4; q++)
+      bv[nb][q] = EPI == EPI_RELU || EPI == EPI_OUT ? bias[co0 + (cb0 + nb) * 16 + 4 * lg + q] : 0.f;
   // ---- input tile items (8 channels, 16 bytes). An upsample-concatenation input is
   // staged in two passes, the C2 upsampled channels (part A) and the rest (part B), so
   // each item loop reads one tensor; the other modes have part A only.
@@ -319,6 +330,27 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
           *(uint2*)(y + k) = o;
         }
       }
+    } else if (EPI == EPI_OUT) {
+      // lane l % 16 gathers its pixel's 16 channels from lanes l % 16 + 16 j, in order
+      const double sc = e.scales[0] * e.h2;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int py = ty0 + wr * R + r;
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = bf_val(bf_bits(fmaxf(acc[r][0][q] + bv[0][q], 0.f)));
+        float a[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) a[4 * j + q] = __shfl(v[q], li + 16 * j);
+        if (lg == 0) {
+          float o = e.bout[0];
+#pragma unroll
+          for (int c = 0; c < 16; c++) o = fmaf(e.wout[c], a[c], o);
+          e.p[(size_t)py * m + px] += sc * (double)o;
+        }
+      }
     } else if (EPI == EPI_UNPOOL) {
       // the gradient of a 2x2-averaged input: a quarter to each fine pixel, plus the skip
       const int m2 = 2 * m;
@@ -438,22 +470,119 @@ k_guess_first(const S* __restrict__ r, const double* __restrict__ pprev, const d
   dst[1] = uint4{packed[4], packed[5], packed[6], packed[7]};
 }
 
-// Output layer (1x1, 16 -> 1) and the guess: p = p_prev + s_r h^2 (b + w . x).
-__global__ void __launch_bounds__(256)
-k_guess_out(const uint16_t* __restrict__ x, const float* __restrict__ w, const float* __restrict__ bias,
-            const double* __restrict__ scales, double* __restrict__ p, int n, double h2) {
-  size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (k >= (size_t)n * n) return;
-  const uint4* src = (const uint4*)(x + k * 16);
-  uint4 a = src[0], b = src[1];
-  uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  float o = bias[0];
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    o = fmaf(w[2 * c], bf_val(v[c] & 0xffff), o);
-    o = fmaf(w[2 * c + 1], bf_val(v[c] >> 16), o);
+// The first two layers of inference in one kernel: enc0a (2 -> 16, FP32 on the vector
+// ALUs, exactly as k_guess_first) evaluated into LDS on the tile and its one-pixel
+// halo, then enc0b (16 -> 16, BF16 MFMA, as k_conv3x3) from there; enc0a's output never
+// reaches memory. Per tile of TH x 16 pixels: the scaled inputs (r / s_r, p_prev / s_p)
+// on a (TH + 4) x 20 window, enc0a on the (TH + 2) x 18 pixels around the tile, the
+// 3x3 convolution, y = BF16(relu(acc + b1)) (16 channels at n, one image).
+// w0: FP32 [16][9][2], b0 [16]; w1: BF16 [16][9][16], b1 [16].
+template <typename S, int TH>
+__global__ void __launch_bounds__(GT_NT)
+k_guess_enc0(const S* __restrict__ r, const double* __restrict__ pprev, const double* __restrict__ scales,
+             const float* __restrict__ w0, const float* __restrict__ b0, const uint16_t* __restrict__ w1,
+             const float* __restrict__ b1, uint16_t* __restrict__ y, int n) {
+  constexpr int R = TH / 4, PS = 24, WS = 9 * 16 + 8;
+  constexpr int LH = TH + 2, LW = GT_W + 2, IH = TH + 4, IW = GT_W + 4;
+  static_assert(TH % 4 == 0 && LH * LW <= GT_NT, "one enc0a pixel per thread");
+  __shared__ __align__(16) float2 in2[IH * IW];
+  __shared__ __align__(16) uint16_t tile[LH * LW * PS];
+  __shared__ __align__(16) uint16_t wl[16 * WS];
+  for (int q = threadIdx.x; q < 16 * 18; q += GT_NT) {
+    const int o = q / 18, k8 = q - o * 18;
+    *(uint4*)(wl + o * WS + k8 * 8) = *(const uint4*)(w1 + (size_t)o * 144 + k8 * 8);
   }
-  p[k] += scales[0] * h2 * (double)o;
+  const double sr = scales[0], sp = scales[1];
+  const float ir = sr > 0 ? (float)(1.0 / sr) : 0.f, ip = sp > 0 ? (float)(1.0 / sp) : 0.f;
+  const int lane = threadIdx.x & 63, wr = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  float bv[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) bv[q] = b1[4 * lg + q];
+  const int tiles_x = n / GT_W, ntiles = tiles_x * (n / TH);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * TH;
+    // ---- scaled inputs on the window (origin 2 cells up and left of the tile)
+    {
+      constexpr int CNT = IH * IW, K = (CNT + GT_NT - 1) / GT_NT;
+      float a[K], b[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * GT_NT;
+        if (q < CNT) {
+          const int ly = q / IW, lx = q - ly * IW;
+          const size_t g = (size_t)wrapm(ty0 - 2 + ly, n) * n + wrapm(tx0 - 2 + lx, n);
+          a[k] = ld(r[g]) * ir;
+          b[k] = (float)pprev[g] * ip;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * GT_NT;
+        if (q < CNT) in2[q] = make_float2(a[k], b[k]);
+      }
+    }
+    __syncthreads();
+    // ---- enc0a on the tile and its halo: one pixel per thread, 16 channels
+    if (threadIdx.x < LH * LW) {
+      const int ly = threadIdx.x / LW, lx = threadIdx.x - ly * LW;
+      float in0[9], in1[9];
+#pragma unroll
+      for (int tp = 0; tp < 9; tp++) {
+        const float2 v = in2[(ly + tp / 3) * IW + lx + tp % 3];
+        in0[tp] = v.x;
+        in1[tp] = v.y;
+      }
+      uint32_t packed[8];
+#pragma unroll
+      for (int co = 0; co < 16; co += 2) {
+        float a0 = b0[co], a1 = b0[co + 1];
+#pragma unroll
+        for (int tp = 0; tp < 9; tp++) {
+          a0 = fmaf(w0[(co * 9 + tp) * 2 + 0], in0[tp], a0);
+          a0 = fmaf(w0[(co * 9 + tp) * 2 + 1], in1[tp], a0);
+          a1 = fmaf(w0[((co + 1) * 9 + tp) * 2 + 0], in0[tp], a1);
+          a1 = fmaf(w0[((co + 1) * 9 + tp) * 2 + 1], in1[tp], a1);
+        }
+        packed[co / 2] = (uint32_t)bf_bits(fmaxf(a0, 0.f)) | ((uint32_t)bf_bits(fmaxf(a1, 0.f)) << 16);
+      }
+      uint4* dst = (uint4*)(tile + threadIdx.x * PS);
+      dst[0] = uint4{packed[0], packed[1], packed[2], packed[3]};
+      dst[1] = uint4{packed[4], packed[5], packed[6], packed[7]};
+    }
+    __syncthreads();
+    // ---- enc0b: 16 -> 16 on v_mfma_f32_16x16x16_bf16, weights as A, pixels as B
+    f32x4 acc[R];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) acc[rr] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wrow = wl + li * WS + lg * 4;
+    const uint16_t* trow = tile + (wr * R * LW + li) * PS + lg * 4;
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++) {
+      bf16x4 af[3];
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) af[dy] = *(const bf16x4*)(wrow + (dy * 3 + dx) * 16);
+#pragma unroll
+      for (int lr = 0; lr < R + 2; lr++) {
+        const bf16x4 bx = *(const bf16x4*)(trow + (lr * LW + dx) * PS);
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+          const int rr = lr - dy;
+          if (rr >= 0 && rr < R) acc[rr] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[dy], bx, acc[rr], 0, 0, 0);
+        }
+      }
+    }
+    // (the next tile's window goes to in2, which every wavefront has finished reading;
+    // its enc0a waits at the barrier after that load until every wavefront is done here)
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+      const int py = ty0 + wr * R + rr, px = tx0 + li;
+      const f32x4 a = acc[rr];
+      *(uint2*)(y + ((size_t)py * n + px) * 16 + 4 * lg) =
+          uint2{bf_pack(fmaxf(a[0] + bv[0], 0.f), fmaxf(a[1] + bv[1], 0.f)),
+                bf_pack(fmaxf(a[2] + bv[2], 0.f), fmaxf(a[3] + bv[3], 0.f))};
+    }
+  }
 }
 
 // Per-block max |p| (FP64) for the input scale.

@@ -792,8 +792,8 @@ static const int kGuessIn[15][2] = {{-1, -1}, {0, -1}, {1, -1}, {2, -1}, {3, -1}
 // Tile rows and output-channel blocks per workgroup, by level: large tiles where
 // the level is big, output channels split across workgroups where it is small, so
 // every level launches at least one workgroup per CU and the LDS holds the weights.
-template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int WC = 1>
-int launch_conv(fs_solver_t* s, int layer) {
+template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int WC = 1, int EPI = fsk::EPI_RELU>
+int launch_conv(fs_solver_t* s, int layer, fsk::ConvEpiArgs e = fsk::ConvEpiArgs{}) {
   auto& G = s->guess_net;
   const int m = s->nx >> kGuessLayers[layer].level;
   const uint16_t* x1 = G.act[kGuessIn[layer][0]];
@@ -804,11 +804,13 @@ int launch_conv(fs_solver_t* s, int layer) {
   const int lds = fsk::conv_lds_bytes<CIN, TH, NBW>();
   const int per_cu = std::max(1, std::min(8, (160 * 1024) / lds));
   dim3 g((unsigned)std::max(1, std::min(ntiles, 256 * per_cu / nyb)), (unsigned)nyb);
-  // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and out, BF16
   double in_ch = MODE == fsk::CONV_POOL ? 4.0 * CIN : MODE == fsk::CONV_UPCAT ? C2 / 4.0 + (CIN - C2) : CIN;
-  return launch(s, PK_GUESS, 2.0 * (in_ch + COUT) * m * m, fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW, fsk::EPI_RELU, 0, WC>,
-                g, dim3(fsk::GT_NT * WC), x1, x2, (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m, 1,
-                fsk::ConvEpiArgs{});
+  // activations in (pooled: 4 fine pixels per pixel; upsampled: 1 coarse pixel per 4) and
+  // out, BF16; with the output layer fused, the FP64 pressure read and written instead
+  const double out_bytes = EPI == fsk::EPI_OUT ? 16.0 : 2.0 * COUT;
+  return launch(s, PK_GUESS, (2.0 * in_ch + out_bytes) * m * m,
+                fsk::k_conv3x3<CIN, COUT, MODE, C2, TH, NBW, EPI, 0, WC>, g, dim3(fsk::GT_NT * WC), x1, x2,
+                (const uint16_t*)G.w[layer], (const float*)G.b[layer], G.act[layer], m, 1, e);
 }
 
 // Evaluate the network on r (level-0 residual of p_prev, FP32) and p_prev = p64[pi],
@@ -822,18 +824,26 @@ int run_guess(fs_solver_t* s, int pi) {
   LAUNCHCHK();
   fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(s->rpart, s->rp_cols, G.ppart, G.nblk, G.scales);
   LAUNCHCHK();
-  const dim3 g0((unsigned)(n / 64), (unsigned)(n / 4));
-  if (s->bf16)
-    fsk::k_guess_first<fsk::bf16><<<g0, 256, 0, s->stream>>>(as<fsk::bf16>(s->lv[0].rhs), s->p64[pi], G.scales,
-                                                              G.w0, G.b[0], G.act[0], n);
-  else
-    fsk::k_guess_first<float><<<g0, 256, 0, s->stream>>>(s->lv[0].rhs, s->p64[pi], G.scales, G.w0, G.b[0], G.act[0],
-                                                          n);
-  LAUNCHCHK();
+  // enc0a and enc0b in one kernel (enc0a's output stays in LDS); the residual r is read
+  // at its storage precision
+  {
+    constexpr int TH0 = 8;
+    const int ntiles = (n / fsk::GT_W) * (n / TH0);
+    const dim3 g((unsigned)std::min(ntiles, 256 * 8));
+    const double bytes = (s->bf16 ? 2.0 : 4.0) * n * n + 8.0 * n * n + 32.0 * n * n;
+    if (s->bf16)
+      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_enc0<fsk::bf16, TH0>, g, dim3(fsk::GT_NT),
+                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), (const double*)s->p64[pi], (const double*)G.scales,
+                  (const float*)G.w0, (const float*)G.b[0], (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1], n);
+    else
+      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_enc0<float, TH0>, g, dim3(fsk::GT_NT), (const float*)s->lv[0].rhs,
+                  (const double*)s->p64[pi], (const double*)G.scales, (const float*)G.w0, (const float*)G.b[0],
+                  (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1], n);
+    if (rc) return rc;
+  }
   // tile rows, channel blocks and wavefront columns per layer (timed at 1024^2): deep
   // tiles where the weights are small, channels split over workgroups or wavefronts
   // where input + weights would crowd the LDS
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1>(s, 1))) return rc;
   if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0, 8, 2, 2>(s, 2))) return rc;
   if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 3))) return rc;
   if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0, 8, 4, 2>(s, 4))) return rc;
@@ -845,12 +855,14 @@ int run_guess(fs_solver_t* s, int pi) {
   if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64, 16, 2, 2>(s, 10))) return rc;
   if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 11))) return rc;
   if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32, 16, 1>(s, 12))) return rc;
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1>(s, 13))) return rc;
-  unsigned blocks = (unsigned)((s->n + 255) / 256);
-  fsk::k_guess_out<<<blocks, 256, 0, s->stream>>>(G.act[13], G.wout, G.b[14], G.scales, s->p64[pi], n,
-                                                  s->dx * s->dx);
-  LAUNCHCHK();
-  return FS_OK;
+  // the last 3x3 layer with the 1x1 output layer and the pressure update in its epilogue
+  fsk::ConvEpiArgs out{};
+  out.wout = G.wout;
+  out.bout = G.b[14];
+  out.scales = G.scales;
+  out.p = s->p64[pi];
+  out.h2 = s->dx * s->dx;
+  return launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1, 1, fsk::EPI_OUT>(s, 13, out);
 }
 
 void release_guess(fs_solver_t* s) {
@@ -2368,7 +2380,9 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
     }
   }
   if (off != bytes) return bail(fail(FS_E_INVALID_ARG, "%zu trailing bytes after the last layer", bytes - off));
-  for (int a = 0; a < 14; a++) {
+  // enc0a's and dec0b's outputs never leave their kernels (k_guess_enc0, the output
+  // epilogue), so only layers 1..12 have activation tensors
+  for (int a = 1; a < 13; a++) {
     const GuessLayer& L = kGuessLayers[a];
     size_t m = (size_t)(s->nx >> L.level);
     if ((rc = dalloc(&G.act[a], m * m * L.cout))) return bail(rc);
