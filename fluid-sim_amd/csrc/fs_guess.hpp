@@ -249,6 +249,12 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
     }
     __syncthreads();
     if (t + (int)gridDim.x < ntiles) prefetch(t + gridDim.x);
+    // EPI_OUT: the pressure this tile updates, loaded before the multiply
+    double pold[EPI == EPI_OUT ? R : 1];
+    if (EPI == EPI_OUT) {
+#pragma unroll
+      for (int r = 0; r < R; r++) pold[r] = e.p[(size_t)(ty0 + wr * R + r) * m + tx0 + li];
+    }
     f32x4 acc[R][N];
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -348,7 +354,7 @@ k_conv3x3(const uint16_t* __restrict__ x1, const uint16_t* __restrict__ x2, cons
           float o = e.bout[0];
 #pragma unroll
           for (int c = 0; c < 16; c++) o = fmaf(e.wout[c], a[c], o);
-          e.p[(size_t)py * m + px] += sc * (double)o;
+          e.p[(size_t)py * m + px] = pold[r] + sc * (double)o;
         }
       }
     } else if (EPI == EPI_UNPOOL) {
@@ -500,27 +506,30 @@ k_guess_enc0(const S* __restrict__ r, const double* __restrict__ pprev, const do
 #pragma unroll
   for (int q = 0; q < 4; q++) bv[q] = b1[4 * lg + q];
   const int tiles_x = n / GT_W, ntiles = tiles_x * (n / TH);
+  // ---- scaled inputs on the window (origin 2 cells up and left of the tile); the next
+  // tile's window is loaded into registers while the current tile is multiplied
+  constexpr int CNT = IH * IW, K = (CNT + GT_NT - 1) / GT_NT;
+  float wa[K], wb[K];
+  auto window = [&](int t) {
+    const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * TH;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * GT_NT;
+      if (q < CNT) {
+        const int ly = q / IW, lx = q - ly * IW;
+        const size_t g = (size_t)wrapm(ty0 - 2 + ly, n) * n + wrapm(tx0 - 2 + lx, n);
+        wa[k] = ld(r[g]) * ir;
+        wb[k] = (float)pprev[g] * ip;
+      }
+    }
+  };
+  if ((int)blockIdx.x < ntiles) window(blockIdx.x);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * TH;
-    // ---- scaled inputs on the window (origin 2 cells up and left of the tile)
-    {
-      constexpr int CNT = IH * IW, K = (CNT + GT_NT - 1) / GT_NT;
-      float a[K], b[K];
 #pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int q = threadIdx.x + k * GT_NT;
-        if (q < CNT) {
-          const int ly = q / IW, lx = q - ly * IW;
-          const size_t g = (size_t)wrapm(ty0 - 2 + ly, n) * n + wrapm(tx0 - 2 + lx, n);
-          a[k] = ld(r[g]) * ir;
-          b[k] = (float)pprev[g] * ip;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int q = threadIdx.x + k * GT_NT;
-        if (q < CNT) in2[q] = make_float2(a[k], b[k]);
-      }
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * GT_NT;
+      if (q < CNT) in2[q] = make_float2(wa[k], wb[k]);
     }
     __syncthreads();
     // ---- enc0a on the tile and its halo: one pixel per thread, 16 channels
@@ -551,6 +560,7 @@ k_guess_enc0(const S* __restrict__ r, const double* __restrict__ pprev, const do
       dst[1] = uint4{packed[4], packed[5], packed[6], packed[7]};
     }
     __syncthreads();
+    if (t + (int)gridDim.x < ntiles) window(t + gridDim.x);
     // ---- enc0b: 16 -> 16 on v_mfma_f32_16x16x16_bf16, weights as A, pixels as B
     f32x4 acc[R];
 #pragma unroll
@@ -573,7 +583,7 @@ k_guess_enc0(const S* __restrict__ r, const double* __restrict__ pprev, const do
       }
     }
     // (the next tile's window goes to in2, which every wavefront has finished reading;
-    // its enc0a waits at the barrier after that load until every wavefront is done here)
+    // its enc0a waits at the barrier after that store until every wavefront is done here)
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const int py = ty0 + wr * R + rr, px = tx0 + li;
