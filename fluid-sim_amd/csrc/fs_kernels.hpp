@@ -30,10 +30,8 @@ __device__ __forceinline__ float st<float>(float f) {
 }
 template <>
 __device__ __forceinline__ bf16 st<bf16>(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return bf16{(uint16_t)((u >> 16) | 0x40u)};  // NaN stays NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return bf16{(uint16_t)(u >> 16)};
+  // v_cvt_pk_bf16_f32: round to nearest, ties to even; NaN stays NaN
+  return bf16{__builtin_bit_cast(uint16_t, (__bf16)f)};
 }
 // value as stored: the FP32 a store of type S followed by a load returns
 template <typename S>

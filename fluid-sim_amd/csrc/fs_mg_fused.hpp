@@ -286,7 +286,7 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
 template <int SW, class TS, typename S, typename SO = S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__ rhs,
-         const S* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
+         const SO* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;

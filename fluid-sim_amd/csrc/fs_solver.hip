@@ -81,7 +81,7 @@ struct Level {
   double dx = 0;
   float* x = nullptr;    // correction (level 0: e); fused path: after pre-smoothing
   float* rhs = nullptr;  // right-hand side (level 0: r)
-  float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e, FP32 under every storage)
+  float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e), FP32 under every storage
   int tile[2] = {0, 0};  // fused path: tile shape (TileKind) of the down and up kernels
   float tune_us[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
 };
@@ -670,14 +670,11 @@ int launch_up_s(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
-  // x, rhs, coarse x (1 per 4 cells) read; x written (level 0: in FP32, see lv[0].xb)
-  if (l == 0)
-    return launch(s, PK_UP, (2.25 * sizeof(S) + 4.0) * F.n * F.n, fsk::k_mgt_up<POST, TS, S, float>, g,
-                  dim3(TS::NT), as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs),
-                  as<S>((const float*)s->lv[l + 1].xb), F.n, (float)(F.dx * F.dx), tx, nb);
-  return launch(s, PK_UP, 3.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_up<POST, TS, S>, g, dim3(TS::NT),
-                as<S>((const float*)F.x), as<S>(F.xb), as<S>((const float*)F.rhs),
-                as<S>((const float*)s->lv[l + 1].xb), F.n, (float)(F.dx * F.dx), tx, nb);
+  // x, rhs (storage precision) and the coarse correction (FP32, 1 per 4 cells) read;
+  // the correction written in FP32 (see Level::xb)
+  return launch(s, PK_UP, (2.0 * sizeof(S) + 5.0) * F.n * F.n, fsk::k_mgt_up<POST, TS, S, float>, g, dim3(TS::NT),
+                as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
+                (float)(F.dx * F.dx), tx, nb);
 }
 template <int POST, class TS>
 int launch_up_t(fs_solver_t* s, int l) {
@@ -691,13 +688,10 @@ int launch_up(fs_solver_t* s, int l) {
 template <typename S>
 int launch_coarse_s(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
-  if (from == 0)  // the whole hierarchy in one workgroup: its result is lv[0].xb, FP32
-    return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
-                  dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
-                  s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
-  return launch(s, PK_COARSE, 2.0 * sizeof(S) * C.n * C.n, fsk::k_mg_coarse_ml<S>, dim3(1), dim3(fsk::CTH),
-                as<S>(C.xb), as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()), s->cfg.mg_pre,
-                s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
+  // rhs at the storage precision in, the result (the correction passed up) out in FP32
+  return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
+                dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
+                s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
 }
 int launch_coarse_ml(fs_solver_t* s, int from) {
   return s->bf16 ? launch_coarse_s<fsk::bf16>(s, from) : launch_coarse_s<float>(s, from);
@@ -1093,10 +1087,11 @@ int build_levels(fs_solver_t* s) {
     int rc = dalloc(&s->lv[l].x, w);
     if (rc) return rc;
     if ((rc = dalloc(&s->lv[l].rhs, w))) return rc;
-    // the level-0 correction is FP32 under every storage: it goes into the FP64
-    // pressure, and rounding it to BF16 puts back a residual of the size of the one
-    // the V-cycle removed
-    if (s->fused && (rc = dalloc(&s->lv[l].xb, l == 0 ? c : w))) return rc;
+    // the corrections passed up are FP32 under every storage: rounding a smooth
+    // correction to BF16 adds grid-scale noise whose Laplacian is as large as the
+    // residual the V-cycle removed (at level 0 it goes straight into the FP64
+    // pressure; at the levels below, prolongation carries it up)
+    if (s->fused && (rc = dalloc(&s->lv[l].xb, c))) return rc;
   }
   return build_coarse_inverse(s);
 }
@@ -2078,7 +2073,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     size_t w = s->bf16 ? (size_t)l.n * l.n / 2 : (size_t)l.n * l.n;
     if (!rc) rc = fill_f32(s, l.x, w, 0.f);
     if (!rc) rc = fill_f32(s, l.rhs, w, 0.f);
-    if (!rc && l.xb) rc = fill_f32(s, l.xb, &l == &s->lv[0] ? (size_t)l.n * l.n : w, 0.f);
+    if (!rc && l.xb) rc = fill_f32(s, l.xb, (size_t)l.n * l.n, 0.f);
   }
   // on zero inputs every fused kernel writes zeros, so tuning leaves the state as it is
   if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) rc = autotune_tiles(s);

@@ -988,11 +988,11 @@ static void round_field(double *a, size_t n, int storage) {
 
 /* The projection as the GPU solver runs it with a fused V-cycle: iterative
  * refinement p <- p + e with r = div - L p in FP64, each V-cycle on the
- * correction e from zero, and the correction and right-hand sides rounded to the
- * storage precision wherever they pass through memory: the level-0 residual, the
- * pre-smoothed correction and restricted rhs of every level above coarse_n, the
- * coarse solve's result, and each post-smoothed correction -- except the level-0
- * correction e itself, which is stored in FP32 under every storage precision.
+ * correction e from zero, and the right-hand sides and pre-smoothed corrections
+ * rounded to the storage precision wherever they pass through memory: the level-0
+ * residual, and the pre-smoothed correction and restricted rhs of every level above
+ * coarse_n. The corrections passed up (the coarse solve's result and each
+ * post-smoothed correction) are stored in FP32 under every storage precision.
  * Levels at or below coarse_n run unrounded (they stay on chip). The V-cycle is that of
  * src/gpu/multigrid.js:277-330; with no rounding the k-th p equals k plain
  * V-cycles (the cycle is affine in its right-hand side).
@@ -1058,14 +1058,12 @@ int or_mg_refine(int n, double lx, const double *div, double *p, int iterations,
       mg_prolong(&lv[i + 1], &lv[i]);
       mg_smooth(&lv[i], post);
     }
-    if (cs < L && cs > 0) round_field(lv[cs].x, (size_t)lv[cs].n * lv[cs].n, storage);
     /* grid-wide levels up: stored x + prolongated correction, post-smooth, store */
     for (int i = cs - 1; i >= 0; i--) {
       size_t c = (size_t)lv[i].n * lv[i].n;
       memcpy(lv[i].x, xs[i], sizeof(double) * c);
       mg_prolong(&lv[i + 1], &lv[i]);
       mg_smooth(&lv[i], post);
-      if (i > 0) round_field(lv[i].x, c, storage);
     }
     memcpy(e, lv[0].x, sizeof(double) * N);
   }

@@ -504,7 +504,7 @@ def test_bf16_storage_residual_curve_matches_oracle(fs, O, n):
             r32 = O.poisson_residual_rel(n, n, p32, d)
             assert abs(r - ro) / ro < 0.05, (n, it, r, ro, r32)
             # and the storage matters: after the first V-cycle the BF16 curve is a few
-            # percent off the FP32 one (the level-0 correction itself stays FP32)
+            # percent off the FP32 one (the corrections passed up stay FP32)
             if it == 1:
                 assert abs(ro - r32) / r32 > 1e-2
 
