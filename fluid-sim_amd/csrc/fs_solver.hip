@@ -138,6 +138,8 @@ struct fs_solver_t {
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
   double* rpart = nullptr;
   double* rpart_host = nullptr;  // pinned mirror
+  hipEvent_t rb_event = nullptr; // the residual rows of a tolerance-driven projection are on the host
+  bool grad_pending = false;     // the gradient of the final pressure is already enqueued
   int rp_rows = 0, rp_cols = 0;
   int resid_row_pending = -1;    // fixed-count projection whose residual is not yet read
   // |u|, |v| block maxima of the last gradient pass, for the CFL step
@@ -260,7 +262,14 @@ int launch(fs_solver_t* s, int kind, double bytes, K kern, dim3 g, dim3 b, Args.
 
 // Everything the host learns lazily after a stream synchronisation.
 void after_sync(fs_solver_t* s) {
+  // launches still running (after a wait on an event rather than the stream) stay
+  // pending until a later synchronisation
+  size_t keep = 0;
   for (auto& pr : s->prof.pending) {
+    if (hipEventQuery(pr.b) != hipSuccess) {
+      s->prof.pending[keep++] = pr;
+      continue;
+    }
     float ms = 0;
     if (hipEventElapsedTime(&ms, pr.a, pr.b) == hipSuccess) {
       s->prof.ms[pr.kind] += ms;
@@ -270,7 +279,7 @@ void after_sync(fs_solver_t* s) {
     s->prof.pool.push_back(pr.a);
     s->prof.pool.push_back(pr.b);
   }
-  s->prof.pending.clear();
+  s->prof.pending.resize(keep);
   for (int k = 0; k < s->dt_pending; k++) {
     s->t += s->dt_ring[k];
     s->last_dt = s->dt_ring[k];
@@ -739,11 +748,18 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
 }
 
 // Read back residual rows [0, rows) and return the row maxima.
-int read_residuals(fs_solver_t* s, int rows, std::vector<double>& rmax) {
+int launch_gradient64(fs_solver_t* s);
+
+// The residual rows 0 .. rows-1 to the host; the gradient of p64[s->pidx] is
+// enqueued behind the copy and runs while the host waits for the copy alone.
+int read_residuals_overlapped(fs_solver_t* s, int rows, std::vector<double>& rmax) {
   HIPCHK(hipMemcpyAsync(s->rpart_host, s->rpart, sizeof(double) * (size_t)rows * s->rp_cols,
                         hipMemcpyDeviceToHost, s->stream));
-  int rc = sync_stream(s);
+  HIPCHK(hipEventRecord(s->rb_event, s->stream));
+  int rc = launch_gradient64(s);
   if (rc) return rc;
+  HIPCHK(hipEventSynchronize(s->rb_event));
+  after_sync(s);
   rmax.assign(rows, 0.0);
   for (int k = 0; k < rows; k++) {
     const double* row = s->rpart_host + (size_t)k * s->rp_cols;
@@ -899,22 +915,28 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
     *cycles_out = maxc;
     return FS_OK;
   }
-  // predicted V-cycles, then one read-back; add V-cycles one at a time if needed
+  // predicted V-cycles, then one read-back; add V-cycles one at a time if needed.
+  // While the host waits for the residuals, the device already subtracts the
+  // gradient of the pressure they belong to (into the other velocity buffer); the
+  // projection keeps it if the gate is met and enqueues it again otherwise.
   int c = std::max(0, std::min(s->last_pred, maxc));
   for (int it = 0; it < c; it++)
     if ((rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, false))) return rc;
   if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, c == 0 && zero_p, true))) return rc;
   std::vector<double> rmax;
   for (;;) {
-    if ((rc = read_residuals(s, c + 1, rmax))) return rc;
+    s->pidx = (a0 + c + 1) & 1;
+    if ((rc = read_residuals_overlapped(s, c + 1, rmax))) return rc;
     double last = rmax[c];
     if (std::isnan(last) || std::isinf(last)) {
       s->pidx = (a0 + c + 1) & 1;
       *cycles_out = c;
       s->last_residual = last;
+      s->have_maxabs = false;  // the gradient pass that ran rewrote the |u|, |v| maxima
       return fail(FS_E_NONFINITE, "non-finite residual in the projection (V-cycle %d)", c);
     }
     if (last <= tol || c >= maxc) {
+      s->grad_pending = true;
       int need = c;
       for (int k = 0; k <= c; k++)
         if (rmax[k] <= tol) {
@@ -937,24 +959,40 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
   }
 }
 
-int enqueue_gradient64(fs_solver_t* s) {
+// u - grad p64[pidx] into the other velocity buffer, with the |u|, |v| block maxima;
+// the solver's state is not changed (see commit_gradient64).
+int launch_gradient64(fs_solver_t* s) {
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  int rc = launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
-                  (const float*)s->v[c], (const double*)s->p64[s->pidx], s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
-                  idy, s->mpart);
-  if (rc) return rc;
+  return launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+                (const float*)s->v[c], (const double*)s->p64[s->pidx], s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
+                idy, s->mpart);
+}
+
+void commit_gradient64(fs_solver_t* s) {
   s->have_maxabs = true;
-  s->cur = 1 - c;
+  s->cur = 1 - s->cur;
+}
+
+int enqueue_gradient64(fs_solver_t* s) {
+  int rc = launch_gradient64(s);
+  if (rc) return rc;
+  commit_gradient64(s);
   return FS_OK;
 }
 
 int enqueue_projection(fs_solver_t* s, int* cycles) {
   *cycles = 0;
   if (s->cfg.pressure_solver == FS_P_MULTIGRID) {
+    s->grad_pending = false;
     int rc = solve_multigrid(s, cycles);
     if (rc && rc != FS_E_NOT_CONVERGED) return rc;
     s->have_p = true;
+    if (s->grad_pending) {  // enqueued while the residuals were read
+      s->grad_pending = false;
+      commit_gradient64(s);
+      return rc;
+    }
     int rc2 = enqueue_gradient64(s);
     return rc2 ? rc2 : rc;
   }
@@ -1222,6 +1260,7 @@ void release(fs_solver_t* s) {
     (void)hipEventDestroy(pr.b);
   }
   for (auto e : s->prof.pool) (void)hipEventDestroy(e);
+  if (s->rb_event) (void)hipEventDestroy(s->rb_event);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -2036,6 +2075,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (hipSetDevice(s->dev) != hipSuccess) return bail(fail(FS_E_DEVICE, "hipSetDevice(%d) failed", s->dev));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(FS_E_DEVICE, "hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&s->rb_event, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(FS_E_DEVICE, "hipEventCreate failed"));
   int rc = FS_OK;
   for (int k = 0; k < 2 && !rc; k++) {
     rc = dalloc(&s->u[k], s->n);
