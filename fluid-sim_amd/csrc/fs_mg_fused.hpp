@@ -36,6 +36,7 @@ using Tile64 = TileShape<64, 32, 256>;
 using Tile32 = TileShape<32, 32, 256>;
 using Tile16 = TileShape<16, 16, 128>;
 using Tile8 = TileShape<8, 8, 64>;
+using Tile8x256 = TileShape<8, 8, 256>;  // k_mgt_down2: the level-l region is ~25 times the tile
 
 // Blocks are issued round-robin over the 8 XCDs; give each XCD a contiguous band of
 // tiles so neighbouring tiles (which share halos) share an L2.
@@ -178,6 +179,60 @@ k_mgt_down(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float 
   __syncthreads();
   tile_smooth_from_zero<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
   tile_store_and_restrict<H, RW, TS>(sx, sr, x, crhs, n, tx0, ty0, ih2);
+}
+
+// Two grid-wide levels down in one launch, level l (n0) and level l+1 (n0 / 2),
+// for the small levels whose single launches are latency-bound. A workgroup owns a
+// TX x TY tile of level l+1 and the 2TX x 2TY cells of level l under it. It smooths
+// level l from zero on the region whose residual covers the level-(l+1) tile and its
+// smoothing halo (neighbouring workgroups recompute the overlap), restricts that
+// residual into LDS at the stored precision, then runs the level-(l+1) down-sweep on
+// it exactly as k_mgt_down does. Every stored value (x at both levels, the rhs of
+// level l+1 and of level l+2) equals the two separate launches' bit for bit.
+template <int SW, class TS, typename S>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_down2(S* __restrict__ x0, const S* __restrict__ rhs0, int n0, float h20, float ih20, S* __restrict__ x1,
+            S* __restrict__ rhs1, float h21, float ih21, S* __restrict__ crhs, int tiles_x, int nblocks) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW1 = TX + 2 * H, RH1 = TY + 2 * H;            // level l+1 region
+  constexpr int RW0 = 2 * RW1 + 2 * H, RH0 = 2 * RH1 + 2 * H;  // level l region: residual on 2 RW1 x 2 RH1
+  __shared__ float sx0[RH0 * RW0];
+  __shared__ float sr0[RH0 * RW0];
+  __shared__ float sx1[RH1 * RW1];
+  __shared__ float sr1[RH1 * RW1];
+  const int n1 = n0 >> 1;
+  const int t = xcd_tile(nblocks);
+  const int tx1 = (t % tiles_x) * TX, ty1 = (t / tiles_x) * TY;
+  const int ox1 = tx1 - H, oy1 = ty1 - H;          // level-(l+1) coordinate of sx1 (0, 0)
+  const int ox0 = 2 * ox1 - H, oy0 = 2 * oy1 - H;  // level-l coordinate of sx0 (0, 0)
+  load_window<RW0, RH0, NT>(sr0, rhs0, n0, ox0, oy0);
+  __syncthreads();
+  // after 2 SW half-sweeps from zero, x is exact on [H - 1, RW0 - H + 1) and the
+  // residual on [H, RW0 - H), the 2 RW1 x 2 RH1 block under the level-(l+1) region
+  tile_smooth_from_zero<SW, RW0, RH0, NT>(sx0, sr0, (ox0 + oy0) & 1, h20);
+  for (int q = threadIdx.x; q < 4 * TX * TY; q += NT) {
+    const int iy = q / (2 * TX), ix = q - iy * (2 * TX);
+    x0[(size_t)(2 * ty1 + iy) * n0 + 2 * tx1 + ix] = st<S>(sx0[(3 * H + iy) * RW0 + 3 * H + ix]);
+  }
+  for (int q = threadIdx.x; q < RW1 * RH1; q += NT) {
+    const int cy = q / RW1, cx = q - cy * RW1;
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int k = (2 * cy + b + H) * RW0 + (2 * cx + a + H);
+        const float lap = (sx0[k - 1] + sx0[k + 1] + sx0[k - RW0] + sx0[k + RW0] - 4.f * sx0[k]) * ih20;
+        s += sr0[k] - lap;
+      }
+    sr1[q] = rnd<S>(s * 0.25f);  // as the level-(l+1) sweep would load it
+    if (cx >= H && cx < H + TX && cy >= H && cy < H + TY)
+      rhs1[(size_t)(oy1 + cy) * n1 + ox1 + cx] = st<S>(s * 0.25f);
+  }
+  __syncthreads();
+  tile_smooth_from_zero<SW, RW1, RH1, NT>(sx1, sr1, (ox1 + oy1) & 1, h21);
+  tile_store_and_restrict<H, RW1, TS>(sx1, sr1, x1, crhs, n1, tx1, ty1, ih21);
 }
 
 // -------------------------------------------------------------- level 0 down

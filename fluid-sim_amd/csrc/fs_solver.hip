@@ -674,6 +674,25 @@ int launch_down(fs_solver_t* s, int l) {
   return DISPATCH_TILE(s->lv[l].tile[0], launch_down_t, PRE, s, l);
 }
 
+// Levels l and l + 1 down in one launch (k_mgt_down2), tiles of 8 x 8 cells of level l + 1.
+template <int PRE, typename S>
+int launch_down2_s(fs_solver_t* s, int l) {
+  using TS = fsk::Tile8x256;
+  Level& F = s->lv[l];
+  Level& C = s->lv[l + 1];
+  int tx, nb;
+  dim3 g = tiles_grid_t<TS>(C.n, &tx, &nb);
+  // level l: rhs read, x written; level l + 1: x and rhs written; level l + 2 rhs written
+  const double bytes = 2.0 * sizeof(S) * F.n * F.n + 2.25 * sizeof(S) * C.n * C.n;
+  return launch(s, PK_DOWN, bytes, fsk::k_mgt_down2<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
+                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), as<S>(C.x),
+                as<S>(C.rhs), (float)(C.dx * C.dx), (float)(1.0 / (C.dx * C.dx)), as<S>(s->lv[l + 2].rhs), tx, nb);
+}
+template <int PRE>
+int launch_down2(fs_solver_t* s, int l) {
+  return s->bf16 ? launch_down2_s<PRE, fsk::bf16>(s, l) : launch_down2_s<PRE, float>(s, l);
+}
+
 template <int POST, class TS, typename S>
 int launch_up_s(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
@@ -739,19 +758,23 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
   int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
   if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only))) return rc;
   if (residual_only) return FS_OK;
-  for (int l = 1; l < cs; l++)
+  // the last two grid-wide levels above the coarse solve go down in one launch
+  // (level l + 1 tiled by 8 x 8 exactly)
+  const int pair = cs - 2 >= 1 && s->lv[cs - 1].n % 8 == 0 ? cs - 2 : cs;
+  for (int l = 1; l < pair; l++)
     if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
+  if (pair < cs && (rc = DISPATCH_SW(pre, launch_down2, s, pair))) return rc;
   if ((rc = launch_coarse_ml(s, cs))) return rc;
   for (int l = cs - 1; l >= 0; l--)
     if ((rc = DISPATCH_SW(post, launch_up, s, l))) return rc;
   return FS_OK;
 }
 
-// Read back residual rows [0, rows) and return the row maxima.
 int launch_gradient64(fs_solver_t* s);
 
-// The residual rows 0 .. rows-1 to the host; the gradient of p64[s->pidx] is
-// enqueued behind the copy and runs while the host waits for the copy alone.
+// The residual rows 0 .. rows-1 to the host, and their maxima; the gradient of
+// p64[s->pidx] is enqueued behind the copy and runs while the host waits for the
+// copy alone.
 int read_residuals_overlapped(fs_solver_t* s, int rows, std::vector<double>& rmax) {
   HIPCHK(hipMemcpyAsync(s->rpart_host, s->rpart, sizeof(double) * (size_t)rows * s->rp_cols,
                         hipMemcpyDeviceToHost, s->stream));
