@@ -36,7 +36,8 @@ using Tile64 = TileShape<64, 32, 256>;
 using Tile32 = TileShape<32, 32, 256>;
 using Tile16 = TileShape<16, 16, 128>;
 using Tile8 = TileShape<8, 8, 64>;
-using Tile8x256 = TileShape<8, 8, 256>;  // k_mgt_down2: the level-l region is ~25 times the tile
+using Tile8x256 = TileShape<8, 8, 256>;    // k_mgt_down2: the level-l region is ~25 times the tile
+using Tile16x256 = TileShape<16, 16, 256>;  // k_mgt_up2
 
 // Blocks are issued round-robin over the 8 XCDs; give each XCD a contiguous band of
 // tiles so neighbouring tiles (which share halos) share an L2.
@@ -390,6 +391,112 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
     int iy = q / TX, ix = q - iy * TX;
     int gi = tx0 + ix, gj = ty0 + iy;
     if (gi < n && gj < n) xout[(size_t)gj * n + gi] = st<SO>(sx[(iy + H) * RW + ix + H]);
+  }
+}
+
+// Two grid-wide levels up in one launch, level l + 1 (n0 / 2) then level l (n0), the
+// mirror of k_mgt_down2. A workgroup owns a TX x TY tile of level l. It runs the
+// level-(l+1) up-sweep (stored x + prolonged correction of level l + 2, SW sweeps) on
+// a region whose valid interior covers the level-(l+1) nodes the tile's level-l region
+// interpolates from, then the level-l up-sweep with those values read from LDS. The
+// level-(l+1) correction is used as the separate launch would store it (FP32) and is
+// not stored; the level-l correction equals the separate launches' bit for bit.
+template <int SW, class TS, typename S, typename SO>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __restrict__ xc2,
+          const S* __restrict__ x0in, SO* __restrict__ x0out, const S* __restrict__ rhs0, int n0, float h20,
+          float h21, int tiles_x, int nblocks) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW0 = TX + 2 * H, RH0 = TY + 2 * H;         // level-l region
+  constexpr int CW0 = RW0 / 2 + 3, CH0 = RH0 / 2 + 3;       // its level-(l+1) footprint
+  constexpr int D = H / 2 + 1;                              // footprint origin = tile origin / 2 - D
+  constexpr int E = (D + H + 1) & ~1;                       // level-(l+1) region origin = tile origin / 2 - E (even)
+  constexpr int RW1 = (E - D + CW0 + H + 1) & ~1, RH1 = (E - D + CH0 + H + 1) & ~1;
+  constexpr int CW1 = RW1 / 2 + 3, CH1 = RH1 / 2 + 3;       // level-(l+2) footprint of the level-(l+1) region
+  static_assert(TX % 8 == 0 && TY % 8 == 0, "level-(l+1) region origin must be even");
+  __shared__ float sx1[RH1 * RW1];
+  __shared__ float sr1[RH1 * RW1];
+  __shared__ float sc1[CH1 * CW1];
+  __shared__ float sx0[RH0 * RW0];
+  __shared__ float sr0[RH0 * RW0];
+  const int n1 = n0 >> 1, n2 = n1 >> 1;
+  const int t = xcd_tile(nblocks);
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox0 = tx0 - H, oy0 = ty0 - H;
+  const int ox1 = (tx0 >> 1) - E, oy1 = (ty0 >> 1) - E;
+  const int cox1 = (ox1 >> 1) - 1, coy1 = (oy1 >> 1) - 1;
+  // ---- level l + 1: x + prolong(correction of level l + 2), then the post-sweeps
+  {
+    constexpr int CNT = RW1 * RH1, K = (CNT + NT - 1) / NT;
+    float xv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        const int ly = q / RW1, lx = q - ly * RW1;
+        xv[k] = ld(x1in[(size_t)wrapn(oy1 + ly, n1) * n1 + wrapn(ox1 + lx, n1)]);
+      }
+    }
+    load_window<CW1, CH1, NT>(sc1, xc2, n2, cox1, coy1);
+    load_window<RW1, RH1, NT>(sr1, rhs1, n1, ox1, oy1);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        const int ly = q / RW1, lx = q - ly * RW1;
+        const int i = ox1 + lx, j = oy1 + ly;
+        const int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+        const int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+        const float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+        const int cx = i0 - cox1, cy = j0 - coy1;
+        const float a = sc1[cy * CW1 + cx], b = sc1[cy * CW1 + cx + 1];
+        const float c = sc1[(cy + 1) * CW1 + cx], d = sc1[(cy + 1) * CW1 + cx + 1];
+        const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+        sx1[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+      }
+    }
+    __syncthreads();
+    HalfSweeps<RW1, RH1, 0, 2 * SW, 1, NT>::run(sx1, sr1, (ox1 + oy1) & 1, h21);
+  }
+  // ---- level l: x + prolong(the level-(l+1) correction, from LDS), then the post-sweeps
+  {
+    constexpr int CNT = RW0 * RH0, K = (CNT + NT - 1) / NT;
+    float xv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        const int ly = q / RW0, lx = q - ly * RW0;
+        xv[k] = ld(x0in[(size_t)wrapn(oy0 + ly, n0) * n0 + wrapn(ox0 + lx, n0)]);
+      }
+    }
+    load_window<RW0, RH0, NT>(sr0, rhs0, n0, ox0, oy0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        const int ly = q / RW0, lx = q - ly * RW0;
+        const int i = ox0 + lx, j = oy0 + ly;
+        const int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+        const int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+        const float fx = (i & 1) ? 0.25f : 0.75f, fy = (j & 1) ? 0.25f : 0.75f;
+        // the level-(l+1) node i0 sits at sx1 column i0 - ox1, inside its valid interior
+        const int cx = i0 - ox1, cy = j0 - oy1;
+        const float a = rnd<SO>(sx1[cy * RW1 + cx]), b = rnd<SO>(sx1[cy * RW1 + cx + 1]);
+        const float c = rnd<SO>(sx1[(cy + 1) * RW1 + cx]), d = rnd<SO>(sx1[(cy + 1) * RW1 + cx + 1]);
+        const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+        sx0[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+      }
+    }
+    __syncthreads();
+    HalfSweeps<RW0, RH0, 0, 2 * SW, 1, NT>::run(sx0, sr0, (ox0 + oy0) & 1, h20);
+    for (int q = threadIdx.x; q < TX * TY; q += NT) {
+      const int iy = q / TX, ix = q - iy * TX;
+      x0out[(size_t)(ty0 + iy) * n0 + tx0 + ix] = st<SO>(sx0[(iy + H) * RW0 + ix + H]);
+    }
   }
 }
 

@@ -704,6 +704,25 @@ int launch_up_s(fs_solver_t* s, int l) {
                 as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
                 (float)(F.dx * F.dx), tx, nb);
 }
+// Levels l + 1 and l up in one launch (k_mgt_up2), tiles of 16 x 16 cells of level l.
+template <int POST, typename S>
+int launch_up2_s(fs_solver_t* s, int l) {
+  using TS = fsk::Tile16x256;
+  Level& F = s->lv[l];
+  Level& C = s->lv[l + 1];
+  int tx, nb;
+  dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
+  // level l + 1: x, rhs, the level-(l + 2) correction read; level l: x, rhs read, correction written
+  const double bytes = (2.0 * sizeof(S) + 1.0) * C.n * C.n + (2.0 * sizeof(S) + 4.0) * F.n * F.n;
+  return launch(s, PK_UP, bytes, fsk::k_mgt_up2<POST, TS, S, float>, g, dim3(TS::NT), as<S>((const float*)C.x),
+                as<S>((const float*)C.rhs), (const float*)s->lv[l + 2].xb, as<S>((const float*)F.x), F.xb,
+                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb);
+}
+template <int POST>
+int launch_up2(fs_solver_t* s, int l) {
+  return s->bf16 ? launch_up2_s<POST, fsk::bf16>(s, l) : launch_up2_s<POST, float>(s, l);
+}
+
 template <int POST, class TS>
 int launch_up_t(fs_solver_t* s, int l) {
   return s->bf16 ? launch_up_s<POST, TS, fsk::bf16>(s, l) : launch_up_s<POST, TS, float>(s, l);
@@ -765,7 +784,13 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
     if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
   if (pair < cs && (rc = DISPATCH_SW(pre, launch_down2, s, pair))) return rc;
   if ((rc = launch_coarse_ml(s, cs))) return rc;
-  for (int l = cs - 1; l >= 0; l--)
+  // and come back up in one launch (level l tiled by 16 x 16 exactly)
+  int l = cs - 1;
+  if (pair < cs && s->lv[pair].n % 16 == 0) {
+    if ((rc = DISPATCH_SW(post, launch_up2, s, pair))) return rc;
+    l = pair - 1;
+  }
+  for (; l >= 0; l--)
     if ((rc = DISPATCH_SW(post, launch_up, s, l))) return rc;
   return FS_OK;
 }
