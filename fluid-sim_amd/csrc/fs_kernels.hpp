@@ -730,27 +730,53 @@ k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero) {
 
 // u = u* - grad p with FP64 pressure, stored FP32; the block maxima of |u|, |v|
 // of the result go to part[2*block + {0,1}] for the CFL time step of the next step.
-__global__ void __launch_bounds__(BX * BY)
+// One wavefront per BX x BY block: lane l takes 4 consecutive cells of row l / 16
+// (16-byte loads and stores when nx % 4 == 0), so the maxima are wavefront
+// reductions with no barrier. Launched with 64 threads on grid2(nx, ny).
+__global__ void __launch_bounds__(64)
 k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
              float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, double idx, double idy,
              double* __restrict__ part) {
+  static_assert(BX == 64 && BY == 4, "16 lanes x 4 cells per row, 4 rows");
   const int2 bk = xcd_block2();
-  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+  const int i0 = bk.x * BX + 4 * lx, j = bk.y * BY + ly;
   double mu = 0.0, mv = 0.0;
-  if (i < nx && j < ny) {
-    size_t k = (size_t)j * nx + i;
-    double pc = p[k];
-    float nu_ = (float)((double)u[k] - (pc - p[(size_t)j * nx + prv(i, nx)]) * idx);
-    float nv_ = (float)((double)v[k] - (pc - p[(size_t)prv(j, ny) * nx + i]) * idy);
-    ou[k] = nu_;
-    ov[k] = nv_;
-    mu = nan_to_inf(fabs((double)nu_));
-    mv = nan_to_inf(fabs((double)nv_));
+  if (j < ny && i0 < nx) {
+    const size_t row = (size_t)j * nx, rowb = (size_t)prv(j, ny) * nx;
+    if ((nx & 3) == 0) {
+      const float4 uu = *(const float4*)(u + row + i0), vv = *(const float4*)(v + row + i0);
+      const double2 c01 = *(const double2*)(p + row + i0), c23 = *(const double2*)(p + row + i0 + 2);
+      const double2 b01 = *(const double2*)(p + rowb + i0), b23 = *(const double2*)(p + rowb + i0 + 2);
+      const double pl = p[row + prv(i0, nx)];
+      const float4 nu4{(float)((double)uu.x - (c01.x - pl) * idx), (float)((double)uu.y - (c01.y - c01.x) * idx),
+                       (float)((double)uu.z - (c23.x - c01.y) * idx), (float)((double)uu.w - (c23.y - c23.x) * idx)};
+      const float4 nv4{(float)((double)vv.x - (c01.x - b01.x) * idy), (float)((double)vv.y - (c01.y - b01.y) * idy),
+                       (float)((double)vv.z - (c23.x - b23.x) * idy), (float)((double)vv.w - (c23.y - b23.y) * idy)};
+      *(float4*)(ou + row + i0) = nu4;
+      *(float4*)(ov + row + i0) = nv4;
+      mu = fmax(fmax(nan_to_inf(fabs((double)nu4.x)), nan_to_inf(fabs((double)nu4.y))),
+                fmax(nan_to_inf(fabs((double)nu4.z)), nan_to_inf(fabs((double)nu4.w))));
+      mv = fmax(fmax(nan_to_inf(fabs((double)nv4.x)), nan_to_inf(fabs((double)nv4.y))),
+                fmax(nan_to_inf(fabs((double)nv4.z)), nan_to_inf(fabs((double)nv4.w))));
+    } else {
+      for (int c = 0; c < 4 && i0 + c < nx; c++) {
+        const size_t k = row + i0 + c;
+        const double pc = p[k];
+        const float nu_ = (float)((double)u[k] - (pc - p[row + prv(i0 + c, nx)]) * idx);
+        const float nv_ = (float)((double)v[k] - (pc - p[rowb + i0 + c]) * idy);
+        ou[k] = nu_;
+        ov[k] = nv_;
+        mu = fmax(mu, nan_to_inf(fabs((double)nu_)));
+        mv = fmax(mv, nan_to_inf(fabs((double)nv_)));
+      }
+    }
   }
-  double a = block_max(mu), b = block_max(mv);
-  if (thread0()) {
-    part[2 * flat_block()] = a;
-    part[2 * flat_block() + 1] = b;
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  if (threadIdx.x == 0) {
+    part[2 * flat_block()] = mu;
+    part[2 * flat_block() + 1] = mv;
   }
 }
 

@@ -1012,7 +1012,7 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
 int launch_gradient64(fs_solver_t* s) {
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
-  return launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
+  return launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), dim3(64), (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)s->p64[s->pidx], s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
                 idy, s->mpart);
 }
