@@ -55,7 +55,7 @@ KIND_TEXT = {
 # (`bench.py --copy-calibration`: read = 2.0 x FETCH_SIZE KiB, write = 1.0 x
 # WRITE_SIZE KiB). For a class that runs on several levels, the mean over them.
 TRAFFIC_1024 = {
-    "up": 4.95e6, "down": 1.21e6, "advect_fused": 17.64e6, "gradient": 25.40e6, "down0": 35.90e6, "coarse": 0.07e6,
+    "up": 6.53e6, "down": 1.77e6, "advect_fused": 17.64e6, "gradient": 25.42e6, "down0": 35.97e6, "coarse": 0.07e6,
 }
 
 
@@ -94,6 +94,8 @@ def parse():
                     help="forced2048: time-averaged E(k) (device spectrum, Parseval) over a second run and the "
                          "field error against the f64 oracle over --validate-steps steps")
     ap.add_argument("--validate-steps", type=int, default=20)
+    ap.add_argument("--no-copy-ceiling", action="store_true",
+                    help="skip timing plain 1 GiB device copies for the roofline's copy_ceiling_gbs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (counter calibration) and exit")
     return ap.parse_args()
@@ -115,6 +117,37 @@ def copy_calibration():
     hip.hipFree(a)
     hip.hipFree(b)
     print(f"copy calibration: 5 copies of {nbytes} bytes", flush=True)
+
+
+def copy_ceiling():
+    """GB/s (read + write) of plain device-to-device copies of 1 GiB through the HIP
+    runtime, timed with HIP events: what a streaming kernel can reach on this GPU."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hip.hipMemset(a, 0, C.c_size_t(nbytes))
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+        reps = 10
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 GUESS_WEIGHTS = os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn")
@@ -373,6 +406,8 @@ def main():
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
                 "share_of_step": ms / psteps / step_ms,
+                # plain 1 GiB device copies on the same GPU: the streaming ceiling in practice
+                "copy_ceiling_gbs": None if a.no_copy_ceiling else copy_ceiling(),
                 "all_kernels": {k: {"us_per_launch": v[0] / max(v[1], 1) * 1e3, "launches_per_step": v[1] / psteps,
                                     "gbs": v[2] / (v[0] / 1e3) / 1e9 if v[0] else 0.0}
                                 for k, v in prof.items() if v[1]}}
