@@ -20,6 +20,9 @@
 // implicit GEMM per workgroup over a tile of 16-pixel rows and a block of output
 // channels; the input tile with its one-pixel periodic halo (pooling, upsampling
 // and concatenation happen in that load) and the weight slice are staged in LDS.
+// Inference runs enc0a and enc0b as one kernel (k_guess_enc0) and the 1x1 output
+// layer in dec0b's epilogue (EPI_OUT), so neither enc0a's nor dec0b's activations
+// are stored; the trainer, which needs them, runs k_guess_first and EPI_RELU.
 #pragma once
 #include "fs_kernels.hpp"
 
@@ -32,7 +35,7 @@ typedef float f32x2v __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT_W = 16, GT_NT = 256;  // tile width (one MFMA M-block) and threads
+constexpr int GT_W = 16, GT_NT = 256;  // tile width (the 16 pixels of one MFMA block) and threads
 
 enum ConvIn { CONV_DIRECT = 0, CONV_POOL = 1, CONV_UPCAT = 2 };
 
