@@ -704,10 +704,9 @@ int launch_up_s(fs_solver_t* s, int l) {
                 as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
                 (float)(F.dx * F.dx), tx, nb);
 }
-// Levels l + 1 and l up in one launch (k_mgt_up2), tiles of 16 x 16 cells of level l.
-template <int POST, typename S>
+// Levels l + 1 and l up in one launch (k_mgt_up2), tiles TS of level l.
+template <int POST, class TS, typename S>
 int launch_up2_s(fs_solver_t* s, int l) {
-  using TS = fsk::Tile16x256;
   Level& F = s->lv[l];
   Level& C = s->lv[l + 1];
   int tx, nb;
@@ -718,9 +717,13 @@ int launch_up2_s(fs_solver_t* s, int l) {
                 as<S>((const float*)C.rhs), (const float*)s->lv[l + 2].xb, as<S>((const float*)F.x), F.xb,
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb);
 }
+// 16 x 16 tiles for the pair above the coarse solve, 32 x 32 for levels 1 and 0
 template <int POST>
 int launch_up2(fs_solver_t* s, int l) {
-  return s->bf16 ? launch_up2_s<POST, fsk::bf16>(s, l) : launch_up2_s<POST, float>(s, l);
+  if (l == 0)
+    return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
+  return s->bf16 ? launch_up2_s<POST, fsk::Tile16x256, fsk::bf16>(s, l)
+                 : launch_up2_s<POST, fsk::Tile16x256, float>(s, l);
 }
 
 template <int POST, class TS>
@@ -784,14 +787,20 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
     if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
   if (pair < cs && (rc = DISPATCH_SW(pre, launch_down2, s, pair))) return rc;
   if ((rc = launch_coarse_ml(s, cs))) return rc;
-  // and come back up in one launch (level l tiled by 16 x 16 exactly)
+  // and come back up in one launch (level l tiled by 16 x 16 exactly); levels 1 and 0,
+  // when neither is in that pair, also go up in one launch (32 x 32 tiles of level 0)
   int l = cs - 1;
   if (pair < cs && s->lv[pair].n % 16 == 0) {
     if ((rc = DISPATCH_SW(post, launch_up2, s, pair))) return rc;
     l = pair - 1;
   }
-  for (; l >= 0; l--)
+  for (; l >= 0; l--) {
+    if (l == 1 && s->lv[0].n % 32 == 0) {
+      if ((rc = DISPATCH_SW(post, launch_up2, s, 0))) return rc;
+      break;
+    }
     if ((rc = DISPATCH_SW(post, launch_up, s, l))) return rc;
+  }
   return FS_OK;
 }
 

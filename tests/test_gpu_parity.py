@@ -378,9 +378,10 @@ def test_profile_counts_every_launch_and_its_bytes(fs, O, adv_fused):
         assert p["advect_mc"][1] == 1 and p["advect_mc"][2] == 24 * n * n
         assert p["advect_fused"][1] == 0
     assert p["gradient"][1] == 1 and p["gradient"][2] == 24 * n * n
-    # three V-cycles: one level-0 down-sweep each, plus the final residual-only pass
+    # three V-cycles: one level-0 down-sweep each, plus the final residual-only pass;
+    # levels 1 and 0 come back up in one launch
     assert p["down0"][1] + p["residual"][1] == 4
-    assert p["up"][1] == 3 * 2 and p["down"][1] == 3 and p["coarse"][1] == 3
+    assert p["up"][1] == 3 and p["down"][1] == 3 and p["coarse"][1] == 3
     for k, (ms_k, l_k, b_k) in p.items():
         assert (ms_k > 0) == (l_k > 0) and (b_k > 0) == (l_k > 0), k
 
