@@ -216,35 +216,82 @@ k_cav_diffuse_solid(const float* __restrict__ u, const float* __restrict__ v, fl
 
 // FP64 refinement residual r = div u - L_N (p + e) of every simulation; p' = p + e is
 // stored, r goes (FP32) to the simulation's finest multigrid rhs, block max |r| to part.
-__global__ void __launch_bounds__(256)
+// One wavefront per 256-cell block: lane l takes the 4 consecutive cells 4l .. 4l + 3 of
+// it (one row; sizes are multiples of 64, arena offsets multiples of 4096 cells), with
+// 16-byte loads and stores; each cell's arithmetic is the one-cell formula's.
+__global__ void __launch_bounds__(64)
 k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
                  const float* __restrict__ xe, double* __restrict__ pout, float* __restrict__ rhs,
                  const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
-  CavCell q = cav_cell(sims, blk0, nsim);
-  const CavSim& S = sims[q.k];
-  int n = q.n, i = q.i, j = q.j;
-  long c = q.c, cl = S.lvl + ((long)j << S.lg) + i;
-  auto pv = [&](long cc, long lv) -> double {
+  const int blk = xcd_tile((int)gridDim.x);
+  const int k = blk0[nsim + blk];
+  const CavSim& S = sims[k];
+  const int n = S.n;
+  const int local = (blk - blk0[k]) * 256 + 4 * (int)threadIdx.x;
+  const int j = local >> S.lg, i = local & (n - 1);
+  const long c = S.off + local, cl = S.lvl + local;
+  // p + e at 4 cells of row jj (0.0 + e with no p, as the one-cell formula)
+  auto pv4 = [&](long cc, long lv, double out[4]) {
+    double2 a01{0.0, 0.0}, a23{0.0, 0.0};
+    if (p) {
+      a01 = *(const double2*)(p + cc);
+      a23 = *(const double2*)(p + cc + 2);
+    }
+    out[0] = a01.x;
+    out[1] = a01.y;
+    out[2] = a23.x;
+    out[3] = a23.y;
+    if (xe) {
+      const float4 e = *(const float4*)(xe + lv);
+      out[0] += (double)e.x;
+      out[1] += (double)e.y;
+      out[2] += (double)e.z;
+      out[3] += (double)e.w;
+    }
+  };
+  auto pv1 = [&](long cc, long lv) -> double {
     double a = p ? p[cc] : 0.0;
     if (xe) a += (double)xe[lv];
     return a;
   };
-  double pc = pv(c, cl), s = 0.0;
-  if (i > 0) s += pv(c - 1, cl - 1) - pc;
-  if (i < n - 1) s += pv(c + 1, cl + 1) - pc;
-  if (j > 0) s += pv(c - n, cl - n) - pc;
-  if (j < n - 1) s += pv(c + n, cl + n) - pc;
-  double dn = (double)n;
-  double lap = s * dn * dn;
-  const float* us = u + S.off;
+  double pc[4], pb[4], pt[4];
+  pv4(c, cl, pc);
+  if (j > 0) pv4(c - n, cl - n, pb);
+  if (j < n - 1) pv4(c + n, cl + n, pt);
+  const double pl = i > 0 ? pv1(c - 1, cl - 1) : 0.0;
+  const double pr = i + 4 < n ? pv1(c + 4, cl + 4) : 0.0;
+  const float* us = u + S.off + (long)j * n;
   const float* vs = v + S.off;
-  double div = ((double)cu_at(us, n, 0.f, i + 1, j) - (double)cu_at(us, n, 0.f, i, j)) * dn +
-               ((double)cv_at(vs, n, i, j + 1) - (double)cv_at(vs, n, i, j)) * dn;
-  double r = div - lap;
-  pout[c] = pc;
-  rhs[cl] = (float)r;
-  double m = block_max(nan_to_inf(fabs(r)));
-  if (threadIdx.x == 0) part[q.blk] = m;
+  const float4 u4 = *(const float4*)(us + i);
+  const float un = i + 4 < n ? us[i + 4] : 0.f;
+  const float4 vb4 = j > 0 ? *(const float4*)(vs + (long)j * n + i) : float4{0.f, 0.f, 0.f, 0.f};
+  const float4 vt4 = j + 1 < n ? *(const float4*)(vs + (long)(j + 1) * n + i) : float4{0.f, 0.f, 0.f, 0.f};
+  const float uf[5] = {i > 0 ? u4.x : 0.f, u4.y, u4.z, u4.w, i + 4 < n ? un : 0.f};  // faces i .. i + 4
+  const float vb[4] = {vb4.x, vb4.y, vb4.z, vb4.w}, vt[4] = {vt4.x, vt4.y, vt4.z, vt4.w};
+  const double dn = (double)n;
+  double out[4];
+  double amax = 0.0;
+  float r4[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int im = i + m;
+    double s = 0.0;
+    if (im > 0) s += (m > 0 ? pc[m - 1] : pl) - pc[m];
+    if (im < n - 1) s += (m < 3 ? pc[m + 1] : pr) - pc[m];
+    if (j > 0) s += pb[m] - pc[m];
+    if (j < n - 1) s += pt[m] - pc[m];
+    const double lap = s * dn * dn;
+    const double div = ((double)uf[m + 1] - (double)uf[m]) * dn + ((double)vt[m] - (double)vb[m]) * dn;
+    const double r = div - lap;
+    out[m] = pc[m];
+    r4[m] = (float)r;
+    amax = fmax(amax, nan_to_inf(fabs(r)));
+  }
+  *(double2*)(pout + c) = double2{out[0], out[1]};
+  *(double2*)(pout + c + 2) = double2{out[2], out[3]};
+  *(float4*)(rhs + cl) = float4{r4[0], r4[1], r4[2], r4[3]};
+  amax = wave_max(amax);
+  if (threadIdx.x == 0) part[blk] = amax;
 }
 
 // u -= grad p on the interior faces; block max |u|, |v| for the next step's dt.

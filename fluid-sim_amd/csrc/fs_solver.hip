@@ -1453,7 +1453,7 @@ int nmt_up(fs_batch_t* b, int kind, int sw, const float* x, float* xb, const flo
 int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_only) {
   int rc;
   const int c = b->cur;
-  if ((rc = blaunch(b, fsk::k_cav_residual64, dim3(b->nblocks), dim3(256), (const float*)b->u[c],
+  if ((rc = blaunch(b, fsk::k_cav_residual64, dim3(b->nblocks), dim3(64), (const float*)b->u[c],
                     (const float*)b->v[c], (const double*)(zero_p ? nullptr : b->p64[pin]),
                     (const float*)(it > 0 ? b->lxb : nullptr), b->p64[1 - pin], b->lr, (const fsk::CavSim*)b->sims,
                     (const int*)b->blk0, b->nsim, b->rpart + (size_t)it * b->nblocks)))
