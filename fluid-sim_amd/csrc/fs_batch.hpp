@@ -295,27 +295,51 @@ k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const
 }
 
 // u -= grad p on the interior faces; block max |u|, |v| for the next step's dt.
-__global__ void __launch_bounds__(256)
+// One wavefront per 256-cell block, 4 cells of a row per lane (as k_cav_residual64).
+__global__ void __launch_bounds__(64)
 k_cav_gradient64(float* __restrict__ u, float* __restrict__ v, const double* __restrict__ p,
                  const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
-  CavCell q = cav_cell(sims, blk0, nsim);
-  int n = q.n;
-  double dn = (double)n;
-  long c = q.c;
-  float uu = 0.f, vv = 0.f;
-  if (q.i > 0) {
-    uu = (float)((double)u[c] - (p[c] - p[c - 1]) * dn);
-    u[c] = uu;
+  const int blk = xcd_tile((int)gridDim.x);
+  const int k = blk0[nsim + blk];
+  const CavSim& S = sims[k];
+  const int n = S.n;
+  const int local = (blk - blk0[k]) * 256 + 4 * (int)threadIdx.x;
+  const int j = local >> S.lg, i = local & (n - 1);
+  const long c = S.off + local;
+  const double dn = (double)n;
+  const double2 c01 = *(const double2*)(p + c), c23 = *(const double2*)(p + c + 2);
+  const double pc[4] = {c01.x, c01.y, c23.x, c23.y};
+  const double pl[4] = {i > 0 ? p[c - 1] : 0.0, c01.x, c01.y, c23.x};  // the cell to the left of each
+  float4 u4 = *(const float4*)(u + c);
+  float uu[4] = {u4.x, u4.y, u4.z, u4.w}, un[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+    if (i + m > 0) {
+      un[m] = (float)((double)uu[m] - (pc[m] - pl[m]) * dn);
+      uu[m] = un[m];
+    }
+  *(float4*)(u + c) = float4{uu[0], uu[1], uu[2], uu[3]};
+  float vn[4] = {0.f, 0.f, 0.f, 0.f};
+  if (j > 0) {
+    const double2 b01 = *(const double2*)(p + c - n), b23 = *(const double2*)(p + c - n + 2);
+    const double pb[4] = {b01.x, b01.y, b23.x, b23.y};
+    const float4 v4 = *(const float4*)(v + c);
+    const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+    for (int m = 0; m < 4; m++) vn[m] = (float)((double)vv[m] - (pc[m] - pb[m]) * dn);
+    *(float4*)(v + c) = float4{vn[0], vn[1], vn[2], vn[3]};
   }
-  if (q.j > 0) {
-    vv = (float)((double)v[c] - (p[c] - p[c - n]) * dn);
-    v[c] = vv;
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    a = fmax(a, nan_to_inf(fabs((double)un[m])));
+    b = fmax(b, nan_to_inf(fabs((double)vn[m])));
   }
-  double a = block_max(nan_to_inf(fabs((double)uu)));
-  double b = block_max(nan_to_inf(fabs((double)vv)));
+  a = wave_max(a);
+  b = wave_max(b);
   if (threadIdx.x == 0) {
-    part[2 * q.blk] = a;
-    part[2 * q.blk + 1] = b;
+    part[2 * blk] = a;
+    part[2 * blk + 1] = b;
   }
 }
 

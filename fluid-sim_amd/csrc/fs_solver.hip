@@ -2888,7 +2888,7 @@ int fs_batch_step_cfl(void* h, double cfl) {
   int cycles = 0;
   int prc = batch_finish_projection(b, c, a0, zero_p, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
-  if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3(b->nblocks), dim3(256), b->u[b->cur], b->v[b->cur],
+  if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3(b->nblocks), dim3(64), b->u[b->cur], b->v[b->cur],
                     (const double*)b->p64[b->pidx], (const fsk::CavSim*)b->sims, (const int*)b->blk0, b->nsim,
                     b->mpart)))
     return rc;
