@@ -55,8 +55,33 @@ __device__ __forceinline__ int wrapn(int i, int n) {
 // Load a periodic RW x RH window with origin (ox, oy) of an n x n field into LDS.
 // Every load a thread owns is issued before any of them is stored, so the
 // window costs one memory latency instead of one per row of threads.
+// BF16 windows with an even origin and width are read two cells per 4-byte load (a
+// pair starts at an even index, so it never straddles the periodic seam).
 template <int RW, int RH, int NT, typename T>
 __device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __restrict__ src, int n, int ox, int oy) {
+  if constexpr (sizeof(T) == 2 && RW % 2 == 0) {
+    if ((ox & 1) == 0) {
+      constexpr int CNT2 = RW * RH / 2, K2 = (CNT2 + NT - 1) / NT, RW2 = RW / 2;
+      uint32_t tw[K2];
+#pragma unroll
+      for (int k = 0; k < K2; k++) {
+        int q = threadIdx.x + k * NT;
+        if (q < CNT2) {
+          int ly = q / RW2, lx = 2 * (q - ly * RW2);
+          tw[k] = *(const uint32_t*)(src + (size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K2; k++) {
+        int q = threadIdx.x + k * NT;
+        if (q < CNT2) {
+          dst[2 * q] = __uint_as_float(tw[k] << 16);
+          dst[2 * q + 1] = __uint_as_float(tw[k] & 0xffff0000u);
+        }
+      }
+      return;
+    }
+  }
   constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
   float tmp[K];
 #pragma unroll
@@ -357,13 +382,18 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
   // lowest coarse node any region cell interpolates from (ox, oy are even)
   int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
   constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
-  float xv[K];
+  constexpr bool PAIRS = sizeof(S) == 2;  // BF16 x: staged through LDS two cells per load
+  float xv[PAIRS ? 1 : K];
+  if constexpr (PAIRS) {
+    load_window<RW, RH, NT>(sx, xin, n, ox, oy);
+  } else {
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    int q = threadIdx.x + k * NT;
-    if (q < CNT) {
-      int ly = q / RW, lx = q - ly * RW;
-      xv[k] = ld(xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        int ly = q / RW, lx = q - ly * RW;
+        xv[k] = ld(xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
+      }
     }
   }
   load_window<CW, CH, NT>(sc, xc, cn, cox, coy);
@@ -382,7 +412,7 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
       float a = sc[cy * CW + cx], b = sc[cy * CW + cx + 1];
       float c = sc[(cy + 1) * CW + cx], d = sc[(cy + 1) * CW + cx + 1];
       float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-      sx[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+      sx[q] = (PAIRS ? sx[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
     }
   }
   __syncthreads();
@@ -429,13 +459,18 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   // ---- level l + 1: x + prolong(correction of level l + 2), then the post-sweeps
   {
     constexpr int CNT = RW1 * RH1, K = (CNT + NT - 1) / NT;
-    float xv[K];
+    constexpr bool PAIRS = sizeof(S) == 2;  // BF16 x: staged through LDS two cells per load
+    float xv[PAIRS ? 1 : K];
+    if constexpr (PAIRS) {
+      load_window<RW1, RH1, NT>(sx1, x1in, n1, ox1, oy1);
+    } else {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int q = threadIdx.x + k * NT;
-      if (q < CNT) {
-        const int ly = q / RW1, lx = q - ly * RW1;
-        xv[k] = ld(x1in[(size_t)wrapn(oy1 + ly, n1) * n1 + wrapn(ox1 + lx, n1)]);
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (q < CNT) {
+          const int ly = q / RW1, lx = q - ly * RW1;
+          xv[k] = ld(x1in[(size_t)wrapn(oy1 + ly, n1) * n1 + wrapn(ox1 + lx, n1)]);
+        }
       }
     }
     load_window<CW1, CH1, NT>(sc1, xc2, n2, cox1, coy1);
@@ -454,7 +489,7 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
         const float a = sc1[cy * CW1 + cx], b = sc1[cy * CW1 + cx + 1];
         const float c = sc1[(cy + 1) * CW1 + cx], d = sc1[(cy + 1) * CW1 + cx + 1];
         const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-        sx1[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+        sx1[q] = (PAIRS ? sx1[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
       }
     }
     __syncthreads();
@@ -463,13 +498,18 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   // ---- level l: x + prolong(the level-(l+1) correction, from LDS), then the post-sweeps
   {
     constexpr int CNT = RW0 * RH0, K = (CNT + NT - 1) / NT;
-    float xv[K];
+    constexpr bool PAIRS = sizeof(S) == 2;
+    float xv[PAIRS ? 1 : K];
+    if constexpr (PAIRS) {
+      load_window<RW0, RH0, NT>(sx0, x0in, n0, ox0, oy0);
+    } else {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-      const int q = threadIdx.x + k * NT;
-      if (q < CNT) {
-        const int ly = q / RW0, lx = q - ly * RW0;
-        xv[k] = ld(x0in[(size_t)wrapn(oy0 + ly, n0) * n0 + wrapn(ox0 + lx, n0)]);
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (q < CNT) {
+          const int ly = q / RW0, lx = q - ly * RW0;
+          xv[k] = ld(x0in[(size_t)wrapn(oy0 + ly, n0) * n0 + wrapn(ox0 + lx, n0)]);
+        }
       }
     }
     load_window<RW0, RH0, NT>(sr0, rhs0, n0, ox0, oy0);
@@ -488,7 +528,7 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
         const float a = rnd<SO>(sx1[cy * RW1 + cx]), b = rnd<SO>(sx1[cy * RW1 + cx + 1]);
         const float c = rnd<SO>(sx1[(cy + 1) * RW1 + cx]), d = rnd<SO>(sx1[(cy + 1) * RW1 + cx + 1]);
         const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-        sx0[q] = xv[k] + (top * (1.f - fy) + bot * fy);
+        sx0[q] = (PAIRS ? sx0[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
       }
     }
     __syncthreads();
