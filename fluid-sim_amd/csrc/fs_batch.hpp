@@ -68,8 +68,16 @@ __device__ __forceinline__ float cv_at(const float* __restrict__ v, int n, int i
   return v[(size_t)j * n + i];
 }
 
+// a[idx] with a 32-bit byte offset (idx < 2^30), so a block-uniform base stays in
+// scalar registers and the load takes a scalar base plus a 32-bit lane offset
+__device__ __forceinline__ float ld_u32(const float* __restrict__ a, int idx) {
+  return *(const float*)((const char*)a + (unsigned)((unsigned)idx * 4u));
+}
+
 // bilinear sample at (x, y) in the unit box of component comp (0: u, 1: v); lo/hi get
-// the range of the four source values
+// the range of the four source values. Branch-free: the four loads use indices
+// clamped into the simulation, then selects apply what cu_at / cv_at return there
+// (the ghost beyond a wall from the clamped row or column, 0 on a wall face).
 template <int COMP>
 __device__ __forceinline__ float cav_sample(const float* __restrict__ a, int n, float U, float x, float y, float* lo,
                                             float* hi) {
@@ -82,17 +90,23 @@ __device__ __forceinline__ float cav_sample(const float* __restrict__ a, int n, 
   i0 = min(i0, n - 1);
   j0 = min(j0, n - 1);
   float fx = gx - (float)i0, fy = gy - (float)j0;
-  float s00, s10, s01, s11;
+  // u: i0 in [0, n-1], j0 in [-1, n-1]; v: i0 in [-1, n-1], j0 in [0, n-1]
+  const int i1 = i0 + 1, j1 = j0 + 1;
+  const int r0 = max(j0, 0) * n, r1 = min(j1, n - 1) * n;
+  const int c0 = max(i0, 0), c1 = min(i1, n - 1);
+  float s00 = ld_u32(a, r0 + c0), s10 = ld_u32(a, r0 + c1), s01 = ld_u32(a, r1 + c0), s11 = ld_u32(a, r1 + c1);
   if (COMP == 0) {
-    s00 = cu_at(a, n, U, i0, j0);
-    s10 = cu_at(a, n, U, i0 + 1, j0);
-    s01 = cu_at(a, n, U, i0, j0 + 1);
-    s11 = cu_at(a, n, U, i0 + 1, j0 + 1);
+    // below the bottom wall -u, above the lid 2U - u; the side wall faces are 0
+    if (j0 < 0) { s00 = -s00; s10 = -s10; }
+    if (j1 >= n) { s01 = 2.f * U - s01; s11 = 2.f * U - s11; }
+    if (i0 <= 0) { s00 = 0.f; s01 = 0.f; }
+    if (i1 >= n) { s10 = 0.f; s11 = 0.f; }
   } else {
-    s00 = cv_at(a, n, i0, j0);
-    s10 = cv_at(a, n, i0 + 1, j0);
-    s01 = cv_at(a, n, i0, j0 + 1);
-    s11 = cv_at(a, n, i0 + 1, j0 + 1);
+    // beyond a side wall -v; the bottom and top wall faces are 0
+    if (i0 < 0) { s00 = -s00; s01 = -s01; }
+    if (i1 >= n) { s10 = -s10; s11 = -s11; }
+    if (j0 <= 0) { s00 = 0.f; s10 = 0.f; }
+    if (j1 >= n) { s01 = 0.f; s11 = 0.f; }
   }
   if (lo) {
     *lo = fminf(fminf(s00, s10), fminf(s01, s11));
@@ -102,13 +116,27 @@ __device__ __forceinline__ float cav_sample(const float* __restrict__ a, int n, 
   return bot + (top - bot) * fy;
 }
 
+// RK2 departure point from (x, y) over dt, given the velocity (u1, v1) at (x, y).
 __device__ __forceinline__ void cav_trace(const float* __restrict__ u, const float* __restrict__ v, int n, float U,
-                                          float x, float y, float dt, float* xo, float* yo) {
-  float u1 = cav_sample<0>(u, n, U, x, y, nullptr, nullptr), v1 = cav_sample<1>(v, n, 0.f, x, y, nullptr, nullptr);
+                                          float x, float y, float u1, float v1, float dt, float* xo, float* yo) {
   float xm = x - 0.5f * dt * u1, ym = y - 0.5f * dt * v1;
   float u2 = cav_sample<0>(u, n, U, xm, ym, nullptr, nullptr), v2 = cav_sample<1>(v, n, 0.f, xm, ym, nullptr, nullptr);
   *xo = x - dt * u2;
   *yo = y - dt * v2;
+}
+
+// The velocity at a face's own position: the face's value itself (the bilinear
+// weights there are exactly 0, and x, y > 0 make the sign of a zero product
+// irrelevant to the trace) and the other component interpolated.
+__device__ __forceinline__ void cav_vel_u_face(const float* __restrict__ u, const float* __restrict__ v, int n, int i,
+                                               int j, float x, float y, float* u1, float* v1) {
+  *u1 = ld_u32(u, j * n + i);
+  *v1 = cav_sample<1>(v, n, 0.f, x, y, nullptr, nullptr);
+}
+__device__ __forceinline__ void cav_vel_v_face(const float* __restrict__ u, const float* __restrict__ v, int n, float U,
+                                               int i, int j, float x, float y, float* u1, float* v1) {
+  *u1 = cav_sample<0>(u, n, U, x, y, nullptr, nullptr);
+  *v1 = ld_u32(v, j * n + i);
 }
 
 // Semi-Lagrangian forward pass of the interior faces (the MacCormack predictor).
@@ -119,15 +147,20 @@ k_cav_advect_sl(const float* __restrict__ u, const float* __restrict__ v, float*
   const CavSim& S = sims[q.k];
   const float* us = u + S.off;
   const float* vs = v + S.off;
-  float h = 1.f / (float)q.n, dt = (float)S.dt, U = S.lid;
-  float xd, yd, a = 0.f, b = 0.f;
+  const int n = q.n;
+  float h = 1.f / (float)n, dt = (float)S.dt, U = S.lid;
+  float xd, yd, u1, v1, a = 0.f, b = 0.f;
   if (q.i > 0) {
-    cav_trace(us, vs, q.n, U, q.i * h, (q.j + 0.5f) * h, dt, &xd, &yd);
-    a = cav_sample<0>(us, q.n, U, xd, yd, nullptr, nullptr);
+    float x = q.i * h, y = (q.j + 0.5f) * h;
+    cav_vel_u_face(us, vs, n, q.i, q.j, x, y, &u1, &v1);
+    cav_trace(us, vs, n, U, x, y, u1, v1, dt, &xd, &yd);
+    a = cav_sample<0>(us, n, U, xd, yd, nullptr, nullptr);
   }
   if (q.j > 0) {
-    cav_trace(us, vs, q.n, U, (q.i + 0.5f) * h, q.j * h, dt, &xd, &yd);
-    b = cav_sample<1>(vs, q.n, 0.f, xd, yd, nullptr, nullptr);
+    float x = (q.i + 0.5f) * h, y = q.j * h;
+    cav_vel_v_face(us, vs, n, U, q.i, q.j, x, y, &u1, &v1);
+    cav_trace(us, vs, n, U, x, y, u1, v1, dt, &xd, &yd);
+    b = cav_sample<1>(vs, n, 0.f, xd, yd, nullptr, nullptr);
   }
   ou[q.c] = a;
   ov[q.c] = b;
@@ -143,24 +176,28 @@ k_cav_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const 
   const CavSim& S = sims[q.k];
   const float* us = u + S.off;
   const float* vs = v + S.off;
-  float h = 1.f / (float)q.n, dt = (float)S.dt, U = S.lid;
-  float xd, yd, lo, hi, a = 0.f, b = 0.f;
+  const int n = q.n, own = q.j * n + q.i;
+  float h = 1.f / (float)n, dt = (float)S.dt, U = S.lid;
+  float xd, yd, u1, v1, lo, hi, a = 0.f, b = 0.f;
+  // the forward and the backward trace start from the same velocity at the face
   if (q.i > 0) {
     float x = q.i * h, y = (q.j + 0.5f) * h;
-    cav_trace(us, vs, q.n, U, x, y, dt, &xd, &yd);
-    cav_sample<0>(us, q.n, U, xd, yd, &lo, &hi);
-    cav_trace(us, vs, q.n, U, x, y, -dt, &xd, &yd);
-    float back = cav_sample<0>(fu + S.off, q.n, U, xd, yd, nullptr, nullptr);
-    float c = fu[q.c] + 0.5f * (us[(size_t)q.j * q.n + q.i] - back);
+    cav_vel_u_face(us, vs, n, q.i, q.j, x, y, &u1, &v1);
+    cav_trace(us, vs, n, U, x, y, u1, v1, dt, &xd, &yd);
+    cav_sample<0>(us, n, U, xd, yd, &lo, &hi);
+    cav_trace(us, vs, n, U, x, y, u1, v1, -dt, &xd, &yd);
+    float back = cav_sample<0>(fu + S.off, n, U, xd, yd, nullptr, nullptr);
+    float c = fu[q.c] + 0.5f * (ld_u32(us, own) - back);
     a = fminf(fmaxf(c, lo), hi);
   }
   if (q.j > 0) {
     float x = (q.i + 0.5f) * h, y = q.j * h;
-    cav_trace(us, vs, q.n, U, x, y, dt, &xd, &yd);
-    cav_sample<1>(vs, q.n, 0.f, xd, yd, &lo, &hi);
-    cav_trace(us, vs, q.n, U, x, y, -dt, &xd, &yd);
-    float back = cav_sample<1>(fv + S.off, q.n, 0.f, xd, yd, nullptr, nullptr);
-    float c = fv[q.c] + 0.5f * (vs[(size_t)q.j * q.n + q.i] - back);
+    cav_vel_v_face(us, vs, n, U, q.i, q.j, x, y, &u1, &v1);
+    cav_trace(us, vs, n, U, x, y, u1, v1, dt, &xd, &yd);
+    cav_sample<1>(vs, n, 0.f, xd, yd, &lo, &hi);
+    cav_trace(us, vs, n, U, x, y, u1, v1, -dt, &xd, &yd);
+    float back = cav_sample<1>(fv + S.off, n, 0.f, xd, yd, nullptr, nullptr);
+    float c = fv[q.c] + 0.5f * (ld_u32(vs, own) - back);
     b = fminf(fmaxf(c, lo), hi);
   }
   ou[q.c] = a;
