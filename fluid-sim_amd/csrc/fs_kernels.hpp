@@ -257,14 +257,40 @@ __device__ __forceinline__ void bounds4_w(const AdvWin& a, int nx, int ny, float
   hi = fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
 }
 
+// The value of node (i, j) itself: from the window when inside it, else from global
+// memory at the wrapped indices.
 template <bool FAST, bool WRAP = true>
-__device__ __forceinline__ void trace_w(const AdvWin& u, const AdvWin& v, int nx, int ny, float X, float Y, float cx,
-                                        float cy, float& Xb, float& Yb) {
-  float u1 = bilerp_w<FAST, WRAP>(u, nx, ny, X, Y - 0.5f), v1 = bilerp_w<FAST, WRAP>(v, nx, ny, X - 0.5f, Y);
+__device__ __forceinline__ float node_w(const AdvWin& a, int nx, int ny, int i, int j) {
+  int li = WRAP ? win_index(i, a.ox, nx) : i - a.ox, lj = WRAP ? win_index(j, a.oy, ny) : j - a.oy;
+  if (FAST || ((unsigned)li < (unsigned)a.ws && (unsigned)lj < (unsigned)a.ws)) return a.w[lj * a.ws + li];
+  return a.g[(size_t)wrapi(j, ny) * nx + wrapi(i, nx)];
+}
+
+// RK2 departure point from (X, Y) given the velocity (u1, v1) there
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ void trace2_w(const AdvWin& u, const AdvWin& v, int nx, int ny, float X, float Y, float u1,
+                                         float v1, float cx, float cy, float& Xb, float& Yb) {
   float Xm = fmaf(-0.5f * cx, u1, X), Ym = fmaf(-0.5f * cy, v1, Y);
   float u2 = bilerp_w<FAST, WRAP>(u, nx, ny, Xm, Ym - 0.5f), v2 = bilerp_w<FAST, WRAP>(v, nx, ny, Xm - 0.5f, Ym);
   Xb = fmaf(-cx, u2, X);
   Yb = fmaf(-cy, v2, Y);
+}
+
+// The velocity at the u face (i, j + 1/2) and at the v face (i + 1/2, j): the face's
+// own component is its node value (the bilinear weights there are exactly 0; a zero
+// of either sign then moves the trace's midpoint identically, as a sum with X or
+// Y >= 0 that is exact), the other one is interpolated.
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ void vel_u_face(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float& u1,
+                                           float& v1) {
+  u1 = node_w<FAST, WRAP>(u, nx, ny, i, j);
+  v1 = bilerp_w<FAST, WRAP>(v, nx, ny, i - 0.5f, j + 0.5f);
+}
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ void vel_v_face(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float& u1,
+                                           float& v1) {
+  u1 = bilerp_w<FAST, WRAP>(u, nx, ny, i + 0.5f, j - 0.5f);
+  v1 = node_w<FAST, WRAP>(v, nx, ny, i, j);
 }
 
 // forward semi-Lagrangian value of component COMP at node (i, j) (wrapped indices),
@@ -272,12 +298,14 @@ __device__ __forceinline__ void trace_w(const AdvWin& u, const AdvWin& v, int nx
 template <int COMP, bool FAST, bool WRAP = true>
 __device__ __forceinline__ float fwd_node(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float cx,
                                           float cy) {
-  float Xb, Yb;
+  float Xb, Yb, u1, v1;
   if (COMP == 0) {
-    trace_w<FAST, WRAP>(u, v, nx, ny, (float)i, j + 0.5f, cx, cy, Xb, Yb);
+    vel_u_face<FAST, WRAP>(u, v, nx, ny, i, j, u1, v1);
+    trace2_w<FAST, WRAP>(u, v, nx, ny, (float)i, j + 0.5f, u1, v1, cx, cy, Xb, Yb);
     return bilerp_w<FAST, WRAP>(u, nx, ny, Xb, Yb - 0.5f);
   }
-  trace_w<FAST, WRAP>(u, v, nx, ny, i + 0.5f, (float)j, cx, cy, Xb, Yb);
+  vel_v_face<FAST, WRAP>(u, v, nx, ny, i, j, u1, v1);
+  trace2_w<FAST, WRAP>(u, v, nx, ny, i + 0.5f, (float)j, u1, v1, cx, cy, Xb, Yb);
   return bilerp_w<FAST, WRAP>(v, nx, ny, Xb - 0.5f, Yb);
 }
 
@@ -348,26 +376,30 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
     const int wif = WRAP ? win_index(i, fox, nx) : i - fox, wjf = WRAP ? win_index(j, foy, ny) : j - foy;
     const bool inu = FAST || ((unsigned)wiu < (unsigned)AUW && (unsigned)wju < (unsigned)AUW);
     const bool inf = FAST || ((unsigned)wif < (unsigned)AFW && (unsigned)wjf < (unsigned)AFW);
-    float Xb, Yb, lo, hi;
+    float Xb, Yb, lo, hi, u1, v1;
+    // the backward and the forward trace start from the same velocity at the face,
+    // whose own component is the face's value
     {  // u face at (i, j+1/2)
       float X = (float)i, Y = j + 0.5f;
-      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+      u1 = inu ? wu.w[wju * AUW + wiu] : u[(size_t)j * nx + i];
+      v1 = bilerp_w<FAST, WRAP>(wv, nx, ny, X - 0.5f, Y);
+      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<0, FAST, WRAP>(sfu, fox, foy, wu, wv, nx, ny, Xb, Yb - 0.5f, cx, cy);
-      float uo = inu ? wu.w[wju * AUW + wiu] : u[(size_t)j * nx + i];
       float fo = inf ? sfu[wjf * AFW + wif] : fwd_node_far<0>(wu, wv, nx, ny, i, j, cx, cy);
-      float c = fmaf(0.5f, uo - back, fo);
-      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, cx, cy, Xb, Yb);
+      float c = fmaf(0.5f, u1 - back, fo);
+      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
       bounds4_w<FAST, WRAP>(wu, nx, ny, Xb, Yb - 0.5f, lo, hi);
       smu[q] = c < lo ? lo : (c > hi ? hi : c);
     }
     {  // v face at (i+1/2, j)
       float X = i + 0.5f, Y = (float)j;
-      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, -cx, -cy, Xb, Yb);
+      u1 = bilerp_w<FAST, WRAP>(wu, nx, ny, X, Y - 0.5f);
+      v1 = inu ? wv.w[wju * AUW + wiu] : v[(size_t)j * nx + i];
+      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<1, FAST, WRAP>(sfv, fox, foy, wu, wv, nx, ny, Xb - 0.5f, Yb, cx, cy);
-      float vo = inu ? wv.w[wju * AUW + wiu] : v[(size_t)j * nx + i];
       float fo = inf ? sfv[wjf * AFW + wif] : fwd_node_far<1>(wu, wv, nx, ny, i, j, cx, cy);
-      float c = fmaf(0.5f, vo - back, fo);
-      trace_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, cx, cy, Xb, Yb);
+      float c = fmaf(0.5f, v1 - back, fo);
+      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
       bounds4_w<FAST, WRAP>(wv, nx, ny, Xb - 0.5f, Yb, lo, hi);
       smv[q] = c < lo ? lo : (c > hi ? hi : c);
     }
