@@ -196,9 +196,12 @@ k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __res
 // sample uses the same floor, fraction and FMA sequence as the global samplers; a
 // sample whose four nodes are not all inside its LDS window reads them from global
 // memory, and a forward value needed outside its window is recomputed exactly as the
-// forward pass computes it, so the result is independent of dt (at CFL <= 0.5 every
-// sample stays in LDS).
-constexpr int AT = 32, AF = 3, AU = AF + 2;
+// forward pass computes it, so the result is independent of dt.
+// Ring widths: when every trace moves less than one cell (cx max|u| and cy max|v| of
+// the window at most 0.99), a sample's four nodes lie within one cell of the node it
+// starts from, so the corrected nodes (tile + 1) back-sample forward nodes on tile + 2
+// (AF), whose traces read u, v on tile + 3 (AU).
+constexpr int AT = 32, AF = 2, AU = AF + 1;
 constexpr int AUW = AT + 2 * AU, AFW = AT + 2 * AF, AMW = AT + 2;
 
 struct AdvWin {

@@ -664,11 +664,14 @@ def test_energy_spectrum_matches_oracle(fs, O, n):
     assert Et[1] > 0.99 * np.sum(Et)
 
 
-@pytest.mark.parametrize("nx,ny,nu,dt_fixed", [(256, 256, 1e-3, None), (80, 48, 5e-4, None), (64, 64, 0.0, None),
-                                               (128, 128, 1e-3, 0.08)])
-def test_fused_advection_diffusion_gives_the_same_bits(fs, nx, ny, nu, dt_fixed):
+@pytest.mark.parametrize("nx,ny,nu,dt_fixed,cfl", [(256, 256, 1e-3, None, 0.5), (80, 48, 5e-4, None, 0.5),
+                                                   (64, 64, 0.0, None, 0.5), (128, 128, 1e-3, 0.08, None),
+                                                   (256, 256, 1e-3, None, 1.9)])
+def test_fused_advection_diffusion_gives_the_same_bits(fs, nx, ny, nu, dt_fixed, cfl):
     # the one-launch MacCormack + diffusion tile kernel against the three launches;
-    # a fixed dt of 0.08 moves samples several cells, outside the LDS windows
+    # a fixed dt of 0.08 moves samples several cells, outside the LDS windows; CFL 1.9
+    # (the sum of both components' moves) puts the largest moves just under one cell,
+    # the edge of the windows' rings
     from fluidsim import initial
     n = max(nx, ny)
     u0, v0 = initial.decaying_turbulence(n, seed=11)
@@ -685,7 +688,7 @@ def test_fused_advection_diffusion_gives_the_same_bits(fs, nx, ny, nu, dt_fixed)
                 if dt_fixed:
                     g.step(dt_fixed)
                 else:
-                    g.stepCFL(0.5)
+                    g.stepCFL(cfl)
             outs.append(g.readVelocity())
     assert np.array_equal(outs[0]["u"], outs[1]["u"]) and np.array_equal(outs[0]["v"], outs[1]["v"])
     assert np.max(np.abs(outs[0]["u"])) > 0
