@@ -632,4 +632,53 @@ k_guess_scales(const double* __restrict__ rrow, int nr, const double* __restrict
   }
 }
 
+// ---- the network at half resolution (m = n / 2)
+// Inputs averaged over 2 x 2 cells: r (at its storage precision) to FP32 r_c, p_prev to
+// FP64 p_c; the increment d_c cleared (the output epilogue adds to it); per-block
+// maxima of |r_c| and |p_c| to part[block] and part[nblocks + block] for the input
+// scales. One thread per coarse cell; m * m is a multiple of 256.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_guess_restrict(const S* __restrict__ r, const double* __restrict__ p, int n, float* __restrict__ rc,
+                 double* __restrict__ pc, double* __restrict__ dc, double* __restrict__ part) {
+  const int m = n >> 1;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int J = q / m, I = q - J * m;
+  const size_t f = (size_t)(2 * J) * n + 2 * I;
+  const float a = 0.25f * ((ld(r[f]) + ld(r[f + 1])) + (ld(r[f + n]) + ld(r[f + n + 1])));
+  const double b = 0.25 * ((p[f] + p[f + 1]) + (p[f + n] + p[f + n + 1]));
+  rc[q] = a;
+  pc[q] = b;
+  dc[q] = 0.0;
+  double ma = wave_max(nan_to_inf(fabs((double)a))), mb = wave_max(nan_to_inf(fabs(b)));
+  __shared__ double sa[4], sb[4];
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = ma;
+    sb[threadIdx.x >> 6] = mb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = fmax(fmax(sa[0], sa[1]), fmax(sa[2], sa[3]));
+    part[gridDim.x + blockIdx.x] = fmax(fmax(sb[0], sb[1]), fmax(sb[2], sb[3]));
+  }
+}
+
+// p += the bilinear prolongation of the coarse increment d_c (cell-centred, periodic;
+// the geometry of the multigrid's prolongation): fine centre (i + 1/2) sits at coarse
+// coordinate (i + 1/2) / 2 - 1/2, weights 3/4 and 1/4 per direction. One thread per
+// fine cell; n * n is a multiple of 256.
+__global__ void __launch_bounds__(256)
+k_guess_prolong_add(const double* __restrict__ dc, double* __restrict__ p, int n) {
+  const int m = n >> 1;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int j = q / n, i = q - j * n;
+  const int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1, j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+  const double fx = (i & 1) ? 0.25 : 0.75, fy = (j & 1) ? 0.25 : 0.75;
+  const int x0 = wrapi(i0, m), x1 = nxt(x0, m), y0 = wrapi(j0, m), y1 = nxt(y0, m);
+  const double a = dc[(size_t)y0 * m + x0], b = dc[(size_t)y0 * m + x1];
+  const double c = dc[(size_t)y1 * m + x0], d = dc[(size_t)y1 * m + x1];
+  const double top = a * (1.0 - fx) + b * fx, bot = c * (1.0 - fx) + d * fx;
+  p[q] += top * (1.0 - fy) + bot * fy;
+}
+
 }  // namespace fsk

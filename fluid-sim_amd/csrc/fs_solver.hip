@@ -166,6 +166,12 @@ struct fs_solver_t {
     double* scales = nullptr;       // max|r|, max|p_prev|
     double* ppart = nullptr;        // per-block max|p_prev|
     int nblk = 0;
+    int m = 0;                      // resolution the network runs at (n, or n / 2)
+    // at half resolution: the averaged inputs, the coarse increment, per-block maxima
+    float* rc = nullptr;
+    double* pc = nullptr;
+    double* dc = nullptr;
+    double* cpart = nullptr;
   } guess_net;
   // energy spectrum work space (allocated on first use)
   fsk::cplx* spec_Z = nullptr;
@@ -862,7 +868,7 @@ static const int kGuessIn[15][2] = {{-1, -1}, {0, -1}, {1, -1}, {2, -1}, {3, -1}
 template <int CIN, int COUT, int MODE, int C2, int TH, int NBW, int WC = 1, int EPI = fsk::EPI_RELU>
 int launch_conv(fs_solver_t* s, int layer, fsk::ConvEpiArgs e = fsk::ConvEpiArgs{}) {
   auto& G = s->guess_net;
-  const int m = s->nx >> kGuessLayers[layer].level;
+  const int m = G.m >> kGuessLayers[layer].level;
   const uint16_t* x1 = G.act[kGuessIn[layer][0]];
   const uint16_t* x2 = kGuessIn[layer][1] >= 0 ? G.act[kGuessIn[layer][1]] : nullptr;
   // persistent workgroups: each stages its weight slice once and loops over tiles;
@@ -886,26 +892,49 @@ int launch_conv(fs_solver_t* s, int layer, fsk::ConvEpiArgs e = fsk::ConvEpiArgs
 int run_guess(fs_solver_t* s, int pi) {
   auto& G = s->guess_net;
   const int n = s->nx;
+  // FS_GUESS_LEARNED_COARSE: the network on the 2 x 2 averages of r and p_prev at n / 2,
+  // its increment prolonged bilinearly and added to p_prev at n
+  const bool coarse = s->cfg.guess == FS_GUESS_LEARNED_COARSE;
+  const int m = coarse ? n / 2 : n;
+  G.m = m;
   int rc;
-  fsk::k_absmax64<<<G.nblk, 256, 0, s->stream>>>(s->p64[pi], s->n, G.ppart);
-  LAUNCHCHK();
-  fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(s->rpart, s->rp_cols, G.ppart, G.nblk, G.scales);
-  LAUNCHCHK();
+  if (coarse) {
+    const int nb = m * m / 256;
+    const double bytes = (s->bf16 ? 2.0 : 4.0) * n * n + 8.0 * n * n + 20.0 * m * m;
+    if (s->bf16)
+      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_restrict<fsk::bf16>, dim3(nb), dim3(256),
+                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), (const double*)s->p64[pi], n, G.rc, G.pc, G.dc,
+                  G.cpart);
+    else
+      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_restrict<float>, dim3(nb), dim3(256), (const float*)s->lv[0].rhs,
+                  (const double*)s->p64[pi], n, G.rc, G.pc, G.dc, G.cpart);
+    if (rc) return rc;
+    fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(G.cpart, nb, G.cpart + nb, nb, G.scales);
+    LAUNCHCHK();
+  } else {
+    fsk::k_absmax64<<<G.nblk, 256, 0, s->stream>>>(s->p64[pi], s->n, G.ppart);
+    LAUNCHCHK();
+    fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(s->rpart, s->rp_cols, G.ppart, G.nblk, G.scales);
+    LAUNCHCHK();
+  }
   // enc0a and enc0b in one kernel (enc0a's output stays in LDS); the residual r is read
-  // at its storage precision
+  // at its storage precision (at half resolution: the FP32 averages)
   {
     constexpr int TH0 = 8;
-    const int ntiles = (n / fsk::GT_W) * (n / TH0);
+    const int ntiles = (m / fsk::GT_W) * (m / TH0);
     const dim3 g((unsigned)std::min(ntiles, 256 * 8));
-    const double bytes = (s->bf16 ? 2.0 : 4.0) * n * n + 8.0 * n * n + 32.0 * n * n;
-    if (s->bf16)
+    const bool rb = s->bf16 && !coarse;
+    const double bytes = (rb ? 2.0 : 4.0) * m * m + 8.0 * m * m + 32.0 * m * m;
+    const double* pin = coarse ? G.pc : s->p64[pi];
+    if (rb)
       rc = launch(s, PK_GUESS, bytes, fsk::k_guess_enc0<fsk::bf16, TH0>, g, dim3(fsk::GT_NT),
-                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), (const double*)s->p64[pi], (const double*)G.scales,
-                  (const float*)G.w0, (const float*)G.b[0], (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1], n);
+                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), pin, (const double*)G.scales, (const float*)G.w0,
+                  (const float*)G.b[0], (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1], m);
     else
-      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_enc0<float, TH0>, g, dim3(fsk::GT_NT), (const float*)s->lv[0].rhs,
-                  (const double*)s->p64[pi], (const double*)G.scales, (const float*)G.w0, (const float*)G.b[0],
-                  (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1], n);
+      rc = launch(s, PK_GUESS, bytes, fsk::k_guess_enc0<float, TH0>, g, dim3(fsk::GT_NT),
+                  coarse ? (const float*)G.rc : (const float*)s->lv[0].rhs, pin, (const double*)G.scales,
+                  (const float*)G.w0, (const float*)G.b[0], (const uint16_t*)G.w[1], (const float*)G.b[1], G.act[1],
+                  m);
     if (rc) return rc;
   }
   // tile rows, channel blocks and wavefront columns per layer (timed at 1024^2): deep
@@ -927,9 +956,13 @@ int run_guess(fs_solver_t* s, int pi) {
   out.wout = G.wout;
   out.bout = G.b[14];
   out.scales = G.scales;
-  out.p = s->p64[pi];
-  out.h2 = s->dx * s->dx;
-  return launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1, 1, fsk::EPI_OUT>(s, 13, out);
+  out.p = coarse ? G.dc : s->p64[pi];
+  out.h2 = coarse ? 4.0 * s->dx * s->dx : s->dx * s->dx;
+  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1, 1, fsk::EPI_OUT>(s, 13, out))) return rc;
+  if (coarse)
+    rc = launch(s, PK_GUESS, 8.0 * m * m + 16.0 * n * n, fsk::k_guess_prolong_add, dim3(n * n / 256), dim3(256),
+                (const double*)G.dc, s->p64[pi], n);
+  return rc;
 }
 
 void release_guess(fs_solver_t* s) {
@@ -943,6 +976,10 @@ void release_guess(fs_solver_t* s) {
   for (int a = 0; a < 14; a++) dfree(G.act[a]);
   dfree(G.scales);
   dfree(G.ppart);
+  dfree(G.rc);
+  dfree(G.pc);
+  dfree(G.dc);
+  dfree(G.cpart);
   G = fs_solver_t::GuessNet{};
 }
 
@@ -950,10 +987,11 @@ void release_guess(fs_solver_t* s) {
 int solve_multigrid(fs_solver_t* s, int* cycles_out) {
   const double tol = s->cfg.tolerance;
   const int maxc = std::max(0, s->cfg.pressure_iterations);
-  bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || s->cfg.guess == FS_GUESS_LEARNED) && s->have_p);
+  const bool learned = s->cfg.guess == FS_GUESS_LEARNED || s->cfg.guess == FS_GUESS_LEARNED_COARSE;
+  bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
   int a0 = s->pidx;
   int rc;
-  if (s->cfg.guess == FS_GUESS_LEARNED) {
+  if (learned) {
     // the residual the previous pressure leaves, the network's increment on top of it,
     // then refinement from that guess
     if ((rc = enqueue_iteration(s, 0, a0 & 1, zero_p, true))) return rc;
@@ -2106,7 +2144,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     return fail(FS_E_INVALID_ARG, "unknown multigrid precision %d", c.mg_precision);
   if (c.mg_precision == FS_MG_BF16 && !(c.pressure_solver == FS_P_MULTIGRID && c.mg_fused))
     return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused multigrid V-cycle (mg_fused = 1)");
-  if (c.guess == FS_GUESS_LEARNED)
+  if (c.guess == FS_GUESS_LEARNED || c.guess == FS_GUESS_LEARNED_COARSE)
     return fail(FS_E_INVALID_ARG, "create with guess zero or previous, then fs_load_guess and fs_set_guess(learned)");
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", c.guess);
@@ -2401,10 +2439,14 @@ int fs_set_guess(void* h, int guess) {
   int rc = check_solver(h);
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
-  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && guess != FS_GUESS_LEARNED)
+  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && guess != FS_GUESS_LEARNED &&
+      guess != FS_GUESS_LEARNED_COARSE)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", guess);
-  if (guess == FS_GUESS_LEARNED && !s->guess_net.loaded)
+  if ((guess == FS_GUESS_LEARNED || guess == FS_GUESS_LEARNED_COARSE) && !s->guess_net.loaded)
     return fail(FS_E_INVALID_ARG, "the learned guess needs its weights: call fs_load_guess first");
+  if (guess == FS_GUESS_LEARNED_COARSE && s->nx % 256 != 0)
+    return fail(FS_E_UNSUPPORTED, "the half-resolution learned guess needs a grid size that is a multiple of 256, got %d",
+                s->nx);
   s->cfg.guess = guess;
   return FS_OK;
 }
@@ -2483,6 +2525,12 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
   G.nblk = 1024;
   if ((rc = dalloc(&G.scales, 2))) return bail(rc);
   if ((rc = dalloc(&G.ppart, (size_t)G.nblk))) return bail(rc);
+  if (s->nx % 256 == 0) {  // the half-resolution variant's buffers
+    const size_t mm = (size_t)(s->nx / 2) * (s->nx / 2);
+    if ((rc = dalloc(&G.rc, mm)) || (rc = dalloc(&G.pc, mm)) || (rc = dalloc(&G.dc, mm)) ||
+        (rc = dalloc(&G.cpart, 2 * mm / 256)))
+      return bail(rc);
+  }
   G.loaded = true;
   return FS_OK;
 }

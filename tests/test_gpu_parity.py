@@ -570,6 +570,89 @@ def test_learned_guess_network_matches_torch(fs, O, seed):
     assert err < 3e-2, err
 
 
+def _restrict2(a, n):
+    a = np.asarray(a, np.float64).reshape(n // 2, 2, n // 2, 2)
+    return (0.25 * ((a[:, 0, :, 0] + a[:, 0, :, 1]) + (a[:, 1, :, 0] + a[:, 1, :, 1]))).reshape(-1)
+
+
+def _prolong2(dc, n):
+    # cell-centred bilinear, periodic: fine centre i + 1/2 at coarse coordinate
+    # (i + 1/2) / 2 - 1/2
+    m = n // 2
+    c = np.asarray(dc, np.float64).reshape(m, m)
+    i = np.arange(n)
+    i0 = np.where(i & 1, i >> 1, (i >> 1) - 1) % m
+    f = np.where(i & 1, 0.25, 0.75)
+    i1 = (i0 + 1) % m
+    rows = c[i0, :] * (1 - f)[:, None] + c[i1, :] * f[:, None]
+    return (rows[:, i0] * (1 - f)[None, :] + rows[:, i1] * f[None, :]).reshape(-1)
+
+
+def test_prolong2_restatement_is_bilinear():
+    # pins the test's own prolongation: a linear ramp in the interior is reproduced
+    # exactly and a constant stays constant
+    n = 16
+    m = n // 2
+    assert np.allclose(_prolong2(np.full(m * m, 3.0), n), 3.0)
+    x = np.arange(m, dtype=np.float64)
+    ramp = np.tile(x, (m, 1)).reshape(-1)
+    fine = _prolong2(ramp, n).reshape(n, n)
+    xf = (np.arange(n) + 0.5) / 2 - 0.5
+    assert np.allclose(fine[:, 2:n - 2], np.tile(xf[2:n - 2], (n, 1)))
+
+
+@pytest.mark.parametrize("n,prec", [(256, "fp32"), (512, "bf16")])
+def test_learned_coarse_guess_matches_torch(fs, O, n, prec):
+    # the network at half resolution: the 2x2 averages of r and p_prev in, the increment
+    # prolonged bilinearly onto the grid; against the float64 torch restatement on the
+    # same averages (the residual as stored: BF16 under BF16 storage)
+    from fluidsim import initial
+    from fluidsim.guess_net import bf16_round, reference_guess
+    blob = _random_guess_blob(3)
+    u0, v0 = initial.decaying_turbulence(n, seed=7)
+    with solver(fs, n=n, pressureSolver="multigrid", pressureIterations=0, mgMinSize=4, mgPrecision=prec) as g:
+        g.uploadVelocity(u0, v0)
+        g.loadGuess(blob)
+        g.setGuess("learned-coarse")
+        g.runKernel("project", 0.0)
+        p = g.readField("pressure")
+    r = np.asarray(O.divergence(n, n, f32(u0), f32(v0)), np.float32)
+    if prec == "bf16":
+        r = bf16_round(r)
+    rc = _restrict2(r.astype(np.float64), n).astype(np.float32).astype(np.float64)
+    dc = reference_guess(blob, rc, np.zeros((n // 2) ** 2), n // 2, 2 * TWO_PI / n)
+    ref = _prolong2(dc, n)
+    assert np.max(np.abs(ref)) > 0
+    err = rel(p, ref)
+    assert err < 3e-2, err
+
+
+def test_learned_coarse_guess_keeps_the_gate_and_rejects_odd_sizes(fs, O):
+    from fluidsim import initial
+    n = 256
+    blob = _random_guess_blob(4)
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-6, mgMinSize=4) as g:
+        g.loadGuess(blob)
+        g.setGuess("learned-coarse")
+        g.uploadVelocity(u0, v0)
+        for _ in range(4):
+            g.stepCFL(0.5)
+            assert g.stats()["last_residual"] <= 1e-6
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=1e-4, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for _ in range(4):
+        s.step(0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h))
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+    with solver(fs, n=128, pressureSolver="multigrid", mgMinSize=4) as g:
+        g.loadGuess(blob)
+        with pytest.raises(Exception, match="multiple of 256"):
+            g.setGuess("learned-coarse")
+
+
 def test_learned_guess_keeps_the_gate(fs, O):
     # whatever the weights, the refinement after the guess meets the FP64 gate and the
     # velocity tracks the exact projection
