@@ -632,49 +632,65 @@ k_guess_scales(const double* __restrict__ rrow, int nr, const double* __restrict
   }
 }
 
-// ---- the network at half resolution (m = n / 2)
-// Inputs averaged over 2 x 2 cells: r (at its storage precision) to FP32 r_c, p_prev to
-// FP64 p_c; the increment d_c cleared (the output epilogue adds to it); per-block
-// maxima of |r_c| and |p_c| to part[block] and part[nblocks + block] for the input
-// scales. One thread per coarse cell; m * m is a multiple of 256.
+// ---- the network at a coarser resolution m = n / 2^k (k = 1, 2)
+// Inputs averaged over 2^k x 2^k cells: r (at its storage precision) to FP32 r_c,
+// p_prev to FP64 p_c; the increment d_c cleared (the output epilogue adds to it);
+// per-block maxima of |r_c| and |p_c| to part[block] and part[nblocks + block] for the
+// input scales. One thread per coarse cell; m * m is a multiple of 256.
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_guess_restrict(const S* __restrict__ r, const double* __restrict__ p, int n, float* __restrict__ rc,
+k_guess_restrict(const S* __restrict__ r, const double* __restrict__ p, int n, int k, float* __restrict__ rc,
                  double* __restrict__ pc, double* __restrict__ dc, double* __restrict__ part) {
-  const int m = n >> 1;
+  const int m = n >> k, f = 1 << k;
   const int q = blockIdx.x * 256 + threadIdx.x;
   const int J = q / m, I = q - J * m;
-  const size_t f = (size_t)(2 * J) * n + 2 * I;
-  const float a = 0.25f * ((ld(r[f]) + ld(r[f + 1])) + (ld(r[f + n]) + ld(r[f + n + 1])));
-  const double b = 0.25 * ((p[f] + p[f + 1]) + (p[f + n] + p[f + n + 1]));
+  const size_t base = (size_t)(f * J) * n + f * I;
+  float a;
+  double b;
+  if (k == 1) {
+    a = 0.25f * ((ld(r[base]) + ld(r[base + 1])) + (ld(r[base + n]) + ld(r[base + n + 1])));
+    b = 0.25 * ((p[base] + p[base + 1]) + (p[base + n] + p[base + n + 1]));
+  } else {
+    float sa = 0.f;
+    double sb = 0.0;
+    for (int y = 0; y < f; y++)
+      for (int x = 0; x < f; x++) {
+        sa += ld(r[base + (size_t)y * n + x]);
+        sb += p[base + (size_t)y * n + x];
+      }
+    a = sa / (float)(f * f);
+    b = sb / (double)(f * f);
+  }
   rc[q] = a;
   pc[q] = b;
   dc[q] = 0.0;
   double ma = wave_max(nan_to_inf(fabs((double)a))), mb = wave_max(nan_to_inf(fabs(b)));
-  __shared__ double sa[4], sb[4];
+  __shared__ double sa4[4], sb4[4];
   if ((threadIdx.x & 63) == 0) {
-    sa[threadIdx.x >> 6] = ma;
-    sb[threadIdx.x >> 6] = mb;
+    sa4[threadIdx.x >> 6] = ma;
+    sb4[threadIdx.x >> 6] = mb;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = fmax(fmax(sa[0], sa[1]), fmax(sa[2], sa[3]));
-    part[gridDim.x + blockIdx.x] = fmax(fmax(sb[0], sb[1]), fmax(sb[2], sb[3]));
+    part[blockIdx.x] = fmax(fmax(sa4[0], sa4[1]), fmax(sa4[2], sa4[3]));
+    part[gridDim.x + blockIdx.x] = fmax(fmax(sb4[0], sb4[1]), fmax(sb4[2], sb4[3]));
   }
 }
 
-// p += the bilinear prolongation of the coarse increment d_c (cell-centred, periodic;
-// the geometry of the multigrid's prolongation): fine centre (i + 1/2) sits at coarse
-// coordinate (i + 1/2) / 2 - 1/2, weights 3/4 and 1/4 per direction. One thread per
-// fine cell; n * n is a multiple of 256.
+// p += the bilinear interpolation of the coarse increment d_c (cell-centred,
+// periodic; for k = 1 the geometry of the multigrid's prolongation): fine centre
+// (i + 1/2) sits at coarse coordinate (i + 1/2) / 2^k - 1/2. One thread per fine cell;
+// n * n is a multiple of 256.
 __global__ void __launch_bounds__(256)
-k_guess_prolong_add(const double* __restrict__ dc, double* __restrict__ p, int n) {
-  const int m = n >> 1;
+k_guess_prolong_add(const double* __restrict__ dc, double* __restrict__ p, int n, int k) {
+  const int m = n >> k;
   const int q = blockIdx.x * 256 + threadIdx.x;
   const int j = q / n, i = q - j * n;
-  const int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1, j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
-  const double fx = (i & 1) ? 0.25 : 0.75, fy = (j & 1) ? 0.25 : 0.75;
-  const int x0 = wrapi(i0, m), x1 = nxt(x0, m), y0 = wrapi(j0, m), y1 = nxt(y0, m);
+  const double sc = 1.0 / (double)(1 << k);
+  const double gx = (i + 0.5) * sc - 0.5, gy = (j + 0.5) * sc - 0.5;  // exact
+  const double fi = floor(gx), fj = floor(gy);
+  const double fx = gx - fi, fy = gy - fj;
+  const int x0 = wrapi((int)fi, m), x1 = nxt(x0, m), y0 = wrapi((int)fj, m), y1 = nxt(y0, m);
   const double a = dc[(size_t)y0 * m + x0], b = dc[(size_t)y0 * m + x1];
   const double c = dc[(size_t)y1 * m + x0], d = dc[(size_t)y1 * m + x1];
   const double top = a * (1.0 - fx) + b * fx, bot = c * (1.0 - fx) + d * fx;

@@ -892,10 +892,12 @@ int launch_conv(fs_solver_t* s, int layer, fsk::ConvEpiArgs e = fsk::ConvEpiArgs
 int run_guess(fs_solver_t* s, int pi) {
   auto& G = s->guess_net;
   const int n = s->nx;
-  // FS_GUESS_LEARNED_COARSE: the network on the 2 x 2 averages of r and p_prev at n / 2,
-  // its increment prolonged bilinearly and added to p_prev at n
-  const bool coarse = s->cfg.guess == FS_GUESS_LEARNED_COARSE;
-  const int m = coarse ? n / 2 : n;
+  // FS_GUESS_LEARNED_COARSE / _QUARTER: the network on the 2^k x 2^k averages of r and
+  // p_prev at n / 2^k (k = 1, 2), its increment interpolated bilinearly and added to
+  // p_prev at n
+  const int k = s->cfg.guess == FS_GUESS_LEARNED_COARSE ? 1 : s->cfg.guess == FS_GUESS_LEARNED_QUARTER ? 2 : 0;
+  const bool coarse = k > 0;
+  const int m = n >> k;
   G.m = m;
   int rc;
   if (coarse) {
@@ -903,11 +905,11 @@ int run_guess(fs_solver_t* s, int pi) {
     const double bytes = (s->bf16 ? 2.0 : 4.0) * n * n + 8.0 * n * n + 20.0 * m * m;
     if (s->bf16)
       rc = launch(s, PK_GUESS, bytes, fsk::k_guess_restrict<fsk::bf16>, dim3(nb), dim3(256),
-                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), (const double*)s->p64[pi], n, G.rc, G.pc, G.dc,
+                  (const fsk::bf16*)as<fsk::bf16>(s->lv[0].rhs), (const double*)s->p64[pi], n, k, G.rc, G.pc, G.dc,
                   G.cpart);
     else
       rc = launch(s, PK_GUESS, bytes, fsk::k_guess_restrict<float>, dim3(nb), dim3(256), (const float*)s->lv[0].rhs,
-                  (const double*)s->p64[pi], n, G.rc, G.pc, G.dc, G.cpart);
+                  (const double*)s->p64[pi], n, k, G.rc, G.pc, G.dc, G.cpart);
     if (rc) return rc;
     fsk::k_guess_scales<<<1, 256, 0, s->stream>>>(G.cpart, nb, G.cpart + nb, nb, G.scales);
     LAUNCHCHK();
@@ -957,11 +959,11 @@ int run_guess(fs_solver_t* s, int pi) {
   out.bout = G.b[14];
   out.scales = G.scales;
   out.p = coarse ? G.dc : s->p64[pi];
-  out.h2 = coarse ? 4.0 * s->dx * s->dx : s->dx * s->dx;
+  out.h2 = (double)(1 << (2 * k)) * s->dx * s->dx;
   if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1, 1, fsk::EPI_OUT>(s, 13, out))) return rc;
   if (coarse)
     rc = launch(s, PK_GUESS, 8.0 * m * m + 16.0 * n * n, fsk::k_guess_prolong_add, dim3(n * n / 256), dim3(256),
-                (const double*)G.dc, s->p64[pi], n);
+                (const double*)G.dc, s->p64[pi], n, k);
   return rc;
 }
 
@@ -987,7 +989,8 @@ void release_guess(fs_solver_t* s) {
 int solve_multigrid(fs_solver_t* s, int* cycles_out) {
   const double tol = s->cfg.tolerance;
   const int maxc = std::max(0, s->cfg.pressure_iterations);
-  const bool learned = s->cfg.guess == FS_GUESS_LEARNED || s->cfg.guess == FS_GUESS_LEARNED_COARSE;
+  const bool learned = s->cfg.guess == FS_GUESS_LEARNED || s->cfg.guess == FS_GUESS_LEARNED_COARSE ||
+                       s->cfg.guess == FS_GUESS_LEARNED_QUARTER;
   bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
   int a0 = s->pidx;
   int rc;
@@ -2144,7 +2147,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     return fail(FS_E_INVALID_ARG, "unknown multigrid precision %d", c.mg_precision);
   if (c.mg_precision == FS_MG_BF16 && !(c.pressure_solver == FS_P_MULTIGRID && c.mg_fused))
     return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused multigrid V-cycle (mg_fused = 1)");
-  if (c.guess == FS_GUESS_LEARNED || c.guess == FS_GUESS_LEARNED_COARSE)
+  if (c.guess == FS_GUESS_LEARNED || c.guess == FS_GUESS_LEARNED_COARSE || c.guess == FS_GUESS_LEARNED_QUARTER)
     return fail(FS_E_INVALID_ARG, "create with guess zero or previous, then fs_load_guess and fs_set_guess(learned)");
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", c.guess);
@@ -2439,14 +2442,18 @@ int fs_set_guess(void* h, int guess) {
   int rc = check_solver(h);
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
-  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && guess != FS_GUESS_LEARNED &&
-      guess != FS_GUESS_LEARNED_COARSE)
+  const bool learned = guess == FS_GUESS_LEARNED || guess == FS_GUESS_LEARNED_COARSE ||
+                       guess == FS_GUESS_LEARNED_QUARTER;
+  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && !learned)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", guess);
-  if ((guess == FS_GUESS_LEARNED || guess == FS_GUESS_LEARNED_COARSE) && !s->guess_net.loaded)
+  if (learned && !s->guess_net.loaded)
     return fail(FS_E_INVALID_ARG, "the learned guess needs its weights: call fs_load_guess first");
   if (guess == FS_GUESS_LEARNED_COARSE && s->nx % 256 != 0)
     return fail(FS_E_UNSUPPORTED, "the half-resolution learned guess needs a grid size that is a multiple of 256, got %d",
                 s->nx);
+  if (guess == FS_GUESS_LEARNED_QUARTER && s->nx % 512 != 0)
+    return fail(FS_E_UNSUPPORTED,
+                "the quarter-resolution learned guess needs a grid size that is a multiple of 512, got %d", s->nx);
   s->cfg.guess = guess;
   return FS_OK;
 }
@@ -2525,7 +2532,7 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
   G.nblk = 1024;
   if ((rc = dalloc(&G.scales, 2))) return bail(rc);
   if ((rc = dalloc(&G.ppart, (size_t)G.nblk))) return bail(rc);
-  if (s->nx % 256 == 0) {  // the half-resolution variant's buffers
+  if (s->nx % 256 == 0) {  // the coarse variants' buffers (sized for half resolution)
     const size_t mm = (size_t)(s->nx / 2) * (s->nx / 2);
     if ((rc = dalloc(&G.rc, mm)) || (rc = dalloc(&G.pc, mm)) || (rc = dalloc(&G.dc, mm)) ||
         (rc = dalloc(&G.cpart, 2 * mm / 256)))

@@ -136,8 +136,9 @@ class GPUFluidSolver:
         check(self._L.fs_load_guess(self.handle, blob, len(blob)))
 
     def setGuess(self, guess: str):
-        """"zero", "previous", "learned" (the network at the grid's resolution) or
-        "learned-coarse" (the network at half resolution, its increment prolonged)."""
+        """"zero", "previous", "learned" (the network at the grid's resolution),
+        "learned-coarse" or "learned-quarter" (the network at half or quarter
+        resolution, its increment interpolated back)."""
         if guess not in _lib.GUESS:
             raise ValueError(f"unknown guess mode {guess}")
         check(self._L.fs_set_guess(self.handle, _lib.GUESS[guess]))

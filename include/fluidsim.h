@@ -45,10 +45,17 @@ enum { FS_P_JACOBI = 0, FS_P_RED_BLACK = 1, FS_P_MULTIGRID = 2 };
 enum { FS_MG_FP32 = 0, FS_MG_FP16 = 1, FS_MG_BF16 = 2 };
 
 /* initial pressure guess for the multigrid projection */
-/* FS_GUESS_LEARNED_COARSE: the learned guess evaluated at half resolution on the 2x2
- * averages of its inputs, its increment prolonged bilinearly (grid size a multiple of
- * 256); no counterpart in the reference, which has no learned guess. */
-enum { FS_GUESS_ZERO = 0, FS_GUESS_PREVIOUS = 1, FS_GUESS_LEARNED = 2, FS_GUESS_LEARNED_COARSE = 3 };
+/* FS_GUESS_LEARNED_COARSE / FS_GUESS_LEARNED_QUARTER: the learned guess evaluated at
+ * half / quarter resolution on the 2x2 / 4x4 averages of its inputs, its increment
+ * interpolated bilinearly (grid size a multiple of 256 / 512); no counterpart in the
+ * reference, which has no learned guess. */
+enum {
+  FS_GUESS_ZERO = 0,
+  FS_GUESS_PREVIOUS = 1,
+  FS_GUESS_LEARNED = 2,
+  FS_GUESS_LEARNED_COARSE = 3,
+  FS_GUESS_LEARNED_QUARTER = 4
+};
 
 /* readable fields for fs_read_field */
 enum {
@@ -224,8 +231,8 @@ int fs_set_guess(void* solver, int guess);
  * blob: "FSGN", uint32 1, uint32 15, then per layer uint32 cin, cout, k, FP32
  * weights [cout][k*k][cin], FP32 bias[cout]. Needs the multigrid solver (FP32 or
  * BF16 storage) and a square grid whose size is a multiple of 128. Then
- * fs_set_guess(solver, FS_GUESS_LEARNED), or FS_GUESS_LEARNED_COARSE for a size that
- * is a multiple of 256. */
+ * fs_set_guess(solver, FS_GUESS_LEARNED), or FS_GUESS_LEARNED_COARSE /
+ * FS_GUESS_LEARNED_QUARTER for a size that is a multiple of 256 / 512. */
 int fs_load_guess(void* solver, const void* blob, size_t bytes);
 
 /* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's

@@ -570,58 +570,71 @@ def test_learned_guess_network_matches_torch(fs, O, seed):
     assert err < 3e-2, err
 
 
-def _restrict2(a, n):
-    a = np.asarray(a, np.float64).reshape(n // 2, 2, n // 2, 2)
-    return (0.25 * ((a[:, 0, :, 0] + a[:, 0, :, 1]) + (a[:, 1, :, 0] + a[:, 1, :, 1]))).reshape(-1)
+def _restrict(a, n, k):
+    f = 1 << k
+    a = np.asarray(a, np.float64).reshape(n // f, f, n // f, f)
+    return a.mean(axis=(1, 3)).reshape(-1)
 
 
-def _prolong2(dc, n):
+def _prolong(dc, n, k):
     # cell-centred bilinear, periodic: fine centre i + 1/2 at coarse coordinate
-    # (i + 1/2) / 2 - 1/2
-    m = n // 2
+    # (i + 1/2) / 2^k - 1/2
+    m = n >> k
     c = np.asarray(dc, np.float64).reshape(m, m)
-    i = np.arange(n)
-    i0 = np.where(i & 1, i >> 1, (i >> 1) - 1) % m
-    f = np.where(i & 1, 0.25, 0.75)
+    g = (np.arange(n) + 0.5) / (1 << k) - 0.5
+    i0 = np.floor(g).astype(int)
+    f = g - i0
+    i0 %= m
     i1 = (i0 + 1) % m
     rows = c[i0, :] * (1 - f)[:, None] + c[i1, :] * f[:, None]
     return (rows[:, i0] * (1 - f)[None, :] + rows[:, i1] * f[None, :]).reshape(-1)
 
 
-def test_prolong2_restatement_is_bilinear():
-    # pins the test's own prolongation: a linear ramp in the interior is reproduced
-    # exactly and a constant stays constant
-    n = 16
-    m = n // 2
-    assert np.allclose(_prolong2(np.full(m * m, 3.0), n), 3.0)
+@pytest.mark.parametrize("k", [1, 2])
+def test_prolong_restatement_is_bilinear(k):
+    # pins the test's own interpolation: a constant stays constant, a linear ramp is
+    # reproduced exactly away from the periodic seam, and for k = 1 the weights are
+    # the multigrid's 3/4, 1/4
+    n = 32
+    m = n >> k
+    assert np.allclose(_prolong(np.full(m * m, 3.0), n, k), 3.0)
     x = np.arange(m, dtype=np.float64)
-    ramp = np.tile(x, (m, 1)).reshape(-1)
-    fine = _prolong2(ramp, n).reshape(n, n)
-    xf = (np.arange(n) + 0.5) / 2 - 0.5
-    assert np.allclose(fine[:, 2:n - 2], np.tile(xf[2:n - 2], (n, 1)))
+    fine = _prolong(np.tile(x, (m, 1)).reshape(-1), n, k).reshape(n, n)
+    xf = (np.arange(n) + 0.5) / (1 << k) - 0.5
+    inner = (xf >= 0) & (xf <= m - 1)
+    assert np.allclose(fine[:, inner], np.tile(xf[inner], (n, 1)))
+    if k == 1:
+        e = np.zeros(m * m)
+        e[0] = 1.0
+        assert np.isclose(_prolong(e, n, 1).reshape(n, n)[0, 0], 0.75 * 0.75)
+        assert np.isclose(_prolong(e, n, 1).reshape(n, n)[0, 2], 0.75 * 0.25)
 
 
-@pytest.mark.parametrize("n,prec", [(256, "fp32"), (512, "bf16")])
-def test_learned_coarse_guess_matches_torch(fs, O, n, prec):
-    # the network at half resolution: the 2x2 averages of r and p_prev in, the increment
-    # prolonged bilinearly onto the grid; against the float64 torch restatement on the
-    # same averages (the residual as stored: BF16 under BF16 storage)
+@pytest.mark.parametrize("n,prec,mode", [(256, "fp32", "learned-coarse"), (512, "bf16", "learned-coarse"),
+                                         (512, "fp32", "learned-quarter")])
+def test_learned_coarse_guess_matches_torch(fs, O, n, prec, mode):
+    # the network at half (quarter) resolution: the 2x2 (4x4) averages of r and p_prev
+    # in, the increment interpolated bilinearly onto the grid; against the float64
+    # torch restatement on the same averages (the residual as stored: BF16 under BF16
+    # storage)
     from fluidsim import initial
     from fluidsim.guess_net import bf16_round, reference_guess
+    k = 1 if mode == "learned-coarse" else 2
     blob = _random_guess_blob(3)
     u0, v0 = initial.decaying_turbulence(n, seed=7)
     with solver(fs, n=n, pressureSolver="multigrid", pressureIterations=0, mgMinSize=4, mgPrecision=prec) as g:
         g.uploadVelocity(u0, v0)
         g.loadGuess(blob)
-        g.setGuess("learned-coarse")
+        g.setGuess(mode)
         g.runKernel("project", 0.0)
         p = g.readField("pressure")
     r = np.asarray(O.divergence(n, n, f32(u0), f32(v0)), np.float32)
     if prec == "bf16":
         r = bf16_round(r)
-    rc = _restrict2(r.astype(np.float64), n).astype(np.float32).astype(np.float64)
-    dc = reference_guess(blob, rc, np.zeros((n // 2) ** 2), n // 2, 2 * TWO_PI / n)
-    ref = _prolong2(dc, n)
+    m = n >> k
+    rc = _restrict(r.astype(np.float64), n, k).astype(np.float32).astype(np.float64)
+    dc = reference_guess(blob, rc, np.zeros(m * m), m, (1 << k) * TWO_PI / n)
+    ref = _prolong(dc, n, k)
     assert np.max(np.abs(ref)) > 0
     err = rel(p, ref)
     assert err < 3e-2, err
@@ -651,6 +664,10 @@ def test_learned_coarse_guess_keeps_the_gate_and_rejects_odd_sizes(fs, O):
         g.loadGuess(blob)
         with pytest.raises(Exception, match="multiple of 256"):
             g.setGuess("learned-coarse")
+    with solver(fs, n=256, pressureSolver="multigrid", mgMinSize=4) as g:
+        g.loadGuess(blob)
+        with pytest.raises(Exception, match="multiple of 512"):
+            g.setGuess("learned-quarter")
 
 
 def test_learned_guess_keeps_the_gate(fs, O):
