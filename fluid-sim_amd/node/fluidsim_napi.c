@@ -166,6 +166,25 @@ static napi_value js_upload(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* loadGuess(h, bytes): the learned guess's weights blob (Uint8Array / Buffer) */
+static napi_value js_load_guess(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  napi_typedarray_type t;
+  size_t len, off;
+  void* data;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off) != napi_ok || t != napi_uint8_array) {
+    napi_throw_type_error(env, NULL, "the weights must be a Uint8Array or Buffer");
+    return NULL;
+  }
+  int rc = fs_load_guess(h->solver, data, len);
+  if (rc) return throw_fs(env, rc);
+  return NULL;
+}
+
 static napi_value new_f32(napi_env env, size_t n, float** data) {
   napi_value ab, arr;
   void* p;
@@ -410,7 +429,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"sync", js_sync},       {"maxDivergence", js_max_div},   {"poissonResidualAfter", js_residual_after},
       {"setParam", js_set},    {"setForcing", js_forcing},      {"stats", js_stats},
       {"diagnostics", js_diag}, {"device", js_device},         {"setSolid", js_solid},
-      {"solidImpulse", js_solid_impulse},
+      {"solidImpulse", js_solid_impulse}, {"loadGuess", js_load_guess},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

@@ -16,7 +16,7 @@ const addon = require("./fluidsim.node");
 
 const ADVECTION = { "semi-lagrangian": 0, maccormack: 1 };
 const PRESSURE = { jacobi: 0, "red-black": 1, multigrid: 2 };
-const GUESS = { zero: 0, previous: 1 };
+const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-quarter": 4 };
 
 export async function createDevice({ requireTimestamps = false } = {}) {
   const d = addon.device();
@@ -88,6 +88,9 @@ export class GPUFluidSolver {
   get steps() { return addon.stats(this._h).steps; }
 
   setTolerance(tol) { addon.setParam(this._h, "tolerance", tol); }
+  // the learned guess (no counterpart in the reference): loadGuess(weights blob), then
+  // setGuess("learned" | "learned-coarse" | "learned-quarter")
+  loadGuess(bytes) { addon.loadGuess(this._h, bytes); }
   setGuess(g) {
     if (!(g in GUESS)) throw new Error(`unknown guess mode ${g}`);
     addon.setParam(this._h, "guess", GUESS[g]);

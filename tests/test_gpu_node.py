@@ -60,3 +60,37 @@ def test_node_shim_matches_python_binding_bitwise():
     assert abs(js["disk"]["checksumU"] - du) <= 1e-9 * max(1.0, abs(du))
     for key in ("x", "y", "vx", "vy", "omega"):
         assert abs(js["disk"][key] - getattr(disk, key)) <= 1e-12 * max(1.0, abs(getattr(disk, key))), key
+
+
+@pytest.mark.parametrize("mode", ["learned-quarter", "learned"])
+def test_node_shim_runs_the_learned_guess_like_the_python_binding(mode):
+    # loadGuess / setGuess through the addon: the same weights, field and steps as the
+    # Python binding, the same bits
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed on this machine")
+    out = subprocess.run([node, os.path.join(ROOT, "fluid-sim_amd", "node", "guess_smoke.mjs"), mode],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    js = json.loads(out.stdout.strip().splitlines()[-1])
+    import fluidsim
+    n = 512
+    i = np.arange(n)[None, :]
+    j = np.arange(n)[:, None]
+    u = (((7 * i + 13 * j) % 31) / 31 - 0.5).reshape(-1)
+    v = (((11 * i + 5 * j) % 29) / 29 - 0.5).reshape(-1)
+    with fluidsim.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                                 pressureIterations=60, tolerance=1e-6, mgMinSize=4) as g:
+        g.loadGuess(os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn"))
+        g.setGuess(mode)
+        g.uploadVelocity(u, v)
+        for _ in range(5):
+            g.stepCFL(0.5)
+        vel = g.readVelocity()
+        st = g.stats()
+    k = np.arange(n * n)
+    su = float(np.sum(vel["u"].astype(np.float64) * (k % 97 + 1)))
+    sv = float(np.sum(vel["v"].astype(np.float64) * (k % 89 + 1)))
+    assert st["last_residual"] <= 1e-6 and js["lastResidual"] <= 1e-6
+    assert js["totalVCycles"] == st["total_vcycles"]
+    assert math.isclose(js["checksumU"], su, rel_tol=1e-10) and math.isclose(js["checksumV"], sv, rel_tol=1e-10)
