@@ -939,20 +939,50 @@ int run_guess(fs_solver_t* s, int pi) {
                   m);
     if (rc) return rc;
   }
-  // tile rows, channel blocks and wavefront columns per layer (timed at 1024^2): deep
-  // tiles where the weights are small, channels split over workgroups or wavefronts
-  // where input + weights would crowd the LDS
-  if ((rc = launch_conv<16, 32, fsk::CONV_POOL, 0, 8, 2, 2>(s, 2))) return rc;
-  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 3))) return rc;
-  if ((rc = launch_conv<32, 64, fsk::CONV_POOL, 0, 8, 4, 2>(s, 4))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 4, 2>(s, 5))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_POOL, 0, 8, 2, 2>(s, 6))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 1>(s, 7))) return rc;
-  if ((rc = launch_conv<128, 64, fsk::CONV_UPCAT, 64, 16, 2, 2>(s, 8))) return rc;
-  if ((rc = launch_conv<64, 64, fsk::CONV_DIRECT, 0, 16, 4, 2>(s, 9))) return rc;
-  if ((rc = launch_conv<96, 32, fsk::CONV_UPCAT, 64, 16, 2, 2>(s, 10))) return rc;
-  if ((rc = launch_conv<32, 32, fsk::CONV_DIRECT, 0, 16, 2>(s, 11))) return rc;
-  if ((rc = launch_conv<48, 16, fsk::CONV_UPCAT, 32, 16, 1>(s, 12))) return rc;
+  // Tile rows, channel blocks and wavefront columns per layer: deep tiles where the
+  // weights are small, channels split over workgroups or wavefronts where input +
+  // weights would crowd the LDS. Per layer, by its output size (a network at n / 2 or n / 4 makes every level
+  // smaller; timed at network inputs of 1024, 512 and 256, all shapes giving the same
+  // bits): short tiles and fewer channel blocks per workgroup on the small levels,
+  // where more workgroups in flight is what pays.
+  const int m1 = m >> 1, m2 = m >> 2, m3 = m >> 3;
+  using namespace fsk;
+  if ((rc = launch_conv<16, 32, CONV_POOL, 0, 8, 2, 2>(s, 2))) return rc;
+  if ((rc = m1 <= 128   ? launch_conv<32, 32, CONV_DIRECT, 0, 4, 2>(s, 3)
+            : m1 <= 256 ? launch_conv<32, 32, CONV_DIRECT, 0, 8, 2>(s, 3)
+                        : launch_conv<32, 32, CONV_DIRECT, 0, 16, 2>(s, 3)))
+    return rc;
+  if ((rc = m2 <= 64    ? launch_conv<32, 64, CONV_POOL, 0, 4, 2, 2>(s, 4)
+            : m2 <= 128 ? launch_conv<32, 64, CONV_POOL, 0, 8, 2, 2>(s, 4)
+                        : launch_conv<32, 64, CONV_POOL, 0, 8, 4, 2>(s, 4)))
+    return rc;
+  if ((rc = m2 <= 64    ? launch_conv<64, 64, CONV_DIRECT, 0, 4, 1>(s, 5)
+            : m2 <= 128 ? launch_conv<64, 64, CONV_DIRECT, 0, 8, 2, 2>(s, 5)
+                        : launch_conv<64, 64, CONV_DIRECT, 0, 16, 4, 2>(s, 5)))
+    return rc;
+  if ((rc = m3 <= 32    ? launch_conv<64, 64, CONV_POOL, 0, 4, 1>(s, 6)
+            : m3 <= 64  ? launch_conv<64, 64, CONV_POOL, 0, 4, 2, 2>(s, 6)
+                        : launch_conv<64, 64, CONV_POOL, 0, 8, 2, 2>(s, 6)))
+    return rc;
+  if ((rc = m3 <= 64 ? launch_conv<64, 64, CONV_DIRECT, 0, 4, 1>(s, 7)
+                     : launch_conv<64, 64, CONV_DIRECT, 0, 16, 1>(s, 7)))
+    return rc;
+  if ((rc = m2 <= 64    ? launch_conv<128, 64, CONV_UPCAT, 64, 8, 1>(s, 8)
+            : m2 <= 128 ? launch_conv<128, 64, CONV_UPCAT, 64, 8, 2, 2>(s, 8)
+                        : launch_conv<128, 64, CONV_UPCAT, 64, 16, 2, 2>(s, 8)))
+    return rc;
+  if ((rc = m2 <= 64    ? launch_conv<64, 64, CONV_DIRECT, 0, 4, 1>(s, 9)
+            : m2 <= 128 ? launch_conv<64, 64, CONV_DIRECT, 0, 8, 2, 2>(s, 9)
+                        : launch_conv<64, 64, CONV_DIRECT, 0, 16, 4, 2>(s, 9)))
+    return rc;
+  if ((rc = m1 <= 128 ? launch_conv<96, 32, CONV_UPCAT, 64, 8, 1>(s, 10)
+                      : launch_conv<96, 32, CONV_UPCAT, 64, 16, 2, 2>(s, 10)))
+    return rc;
+  if ((rc = m1 <= 128   ? launch_conv<32, 32, CONV_DIRECT, 0, 4, 1>(s, 11)
+            : m1 <= 256 ? launch_conv<32, 32, CONV_DIRECT, 0, 8, 2>(s, 11)
+                        : launch_conv<32, 32, CONV_DIRECT, 0, 16, 2>(s, 11)))
+    return rc;
+  if ((rc = launch_conv<48, 16, CONV_UPCAT, 32, 16, 1>(s, 12))) return rc;
   // the last 3x3 layer with the 1x1 output layer and the pressure update in its epilogue
   fsk::ConvEpiArgs out{};
   out.wout = G.wout;
@@ -960,7 +990,9 @@ int run_guess(fs_solver_t* s, int pi) {
   out.scales = G.scales;
   out.p = coarse ? G.dc : s->p64[pi];
   out.h2 = (double)(1 << (2 * k)) * s->dx * s->dx;
-  if ((rc = launch_conv<16, 16, fsk::CONV_DIRECT, 0, 8, 1, 1, fsk::EPI_OUT>(s, 13, out))) return rc;
+  if ((rc = m == 512 ? launch_conv<16, 16, CONV_DIRECT, 0, 16, 1, 1, EPI_OUT>(s, 13, out)
+                     : launch_conv<16, 16, CONV_DIRECT, 0, 8, 1, 1, EPI_OUT>(s, 13, out)))
+    return rc;
   if (coarse)
     rc = launch(s, PK_GUESS, 8.0 * m * m + 16.0 * n * n, fsk::k_guess_prolong_add, dim3(n * n / 256), dim3(256),
                 (const double*)G.dc, s->p64[pi], n, k);
