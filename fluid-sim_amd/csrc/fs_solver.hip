@@ -1026,6 +1026,7 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
   bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
   int a0 = s->pidx;
   int rc;
+  if (learned && !s->guess_net.loaded) return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
   if (learned) {
     // the residual the previous pressure leaves, the network's increment on top of it,
     // then refinement from that guess
@@ -2516,6 +2517,12 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
     return fail(FS_E_INVALID_ARG, "not a version-1 guess network blob");
   if ((rc = set_device(s))) return rc;
   if ((rc = sync_stream(s))) return rc;
+  // a learned mode stays selected only if the new weights load: until then (and after
+  // a failed load) the solver falls back to the previous pressure
+  const int mode = s->cfg.guess;
+  const bool learned_mode =
+      mode == FS_GUESS_LEARNED || mode == FS_GUESS_LEARNED_COARSE || mode == FS_GUESS_LEARNED_QUARTER;
+  if (learned_mode) s->cfg.guess = FS_GUESS_PREVIOUS;
   release_guess(s);
   auto& G = s->guess_net;
   auto bail = [&](int code) {
@@ -2571,6 +2578,7 @@ int fs_load_guess(void* h, const void* blob, size_t bytes) {
       return bail(rc);
   }
   G.loaded = true;
+  s->cfg.guess = mode;
   return FS_OK;
 }
 

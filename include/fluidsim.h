@@ -232,7 +232,8 @@ int fs_set_guess(void* solver, int guess);
  * weights [cout][k*k][cin], FP32 bias[cout]. Needs the multigrid solver (FP32 or
  * BF16 storage) and a square grid whose size is a multiple of 128. Then
  * fs_set_guess(solver, FS_GUESS_LEARNED), or FS_GUESS_LEARNED_COARSE /
- * FS_GUESS_LEARNED_QUARTER for a size that is a multiple of 256 / 512. */
+ * FS_GUESS_LEARNED_QUARTER for a size that is a multiple of 256 / 512. A failed load
+ * releases the previous weights and leaves the solver on FS_GUESS_PREVIOUS. */
 int fs_load_guess(void* solver, const void* blob, size_t bytes);
 
 /* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's

@@ -694,6 +694,32 @@ def test_learned_guess_keeps_the_gate(fs, O):
     assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
 
 
+def test_failed_weights_load_falls_back_to_the_previous_pressure(fs):
+    # a bad blob after a good one: the load fails, the solver keeps stepping with the
+    # previous-pressure guess instead of a network without weights; after a good
+    # reload the learned mode can be selected again
+    from fluidsim import initial
+    n = 256
+    blob = _random_guess_blob(8)
+    u0, v0 = initial.decaying_turbulence(n, seed=2)
+    with solver(fs, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                pressureIterations=40, tolerance=1e-6, mgMinSize=4) as g:
+        g.loadGuess(blob)
+        g.setGuess("learned-coarse")
+        g.uploadVelocity(u0, v0)
+        g.stepCFL(0.5)
+        bad = blob[:12] + blob[12:40] + b"\x00" * 8 + blob[48:]
+        with pytest.raises(Exception):
+            g.loadGuess(bad[:len(bad) - 100])
+        for _ in range(2):
+            g.stepCFL(0.5)
+            assert g.stats()["last_residual"] <= 1e-6
+        g.loadGuess(blob)
+        g.setGuess("learned-coarse")
+        g.stepCFL(0.5)
+        assert g.stats()["last_residual"] <= 1e-6
+
+
 def test_learned_guess_reads_the_bf16_stored_residual(fs, O):
     # under BF16 multigrid storage the residual the network reads is stored in BF16:
     # the same restatement, fed that rounded residual
