@@ -220,15 +220,27 @@ k_cav_diffuse_solid(const float* __restrict__ u, const float* __restrict__ v, fl
   float a = S.nu > 0.f ? (float)((double)S.nu * S.dt * (double)n * (double)n) : 0.f;
   float U = S.lid;
   float un = 0.f, vn = 0.f;
+  // the five values cu_at / cv_at return, from loads at indices clamped into the
+  // simulation (32-bit offsets) and selects for the wall faces and ghosts
+  const int own = j * n + i;
+  const int il = max(i - 1, 0), ir = min(i + 1, n - 1), jb = max(j - 1, 0), jt = min(j + 1, n - 1);
   if (i > 0) {
-    float c = cu_at(us, n, U, i, j);
-    un = c + a * (cu_at(us, n, U, i - 1, j) + cu_at(us, n, U, i + 1, j) + cu_at(us, n, U, i, j - 1) +
-                  cu_at(us, n, U, i, j + 1) - 4.f * c);
+    const float c = ld_u32(us, own);
+    float l = ld_u32(us, j * n + il), r = ld_u32(us, j * n + ir), b = ld_u32(us, jb * n + i), t = ld_u32(us, jt * n + i);
+    if (i - 1 == 0) l = 0.f;                // the left wall face
+    if (i + 1 == n) r = 0.f;                // the right wall face
+    if (j == 0) b = -c;                     // below the bottom wall: -u
+    if (j + 1 == n) t = 2.f * U - c;        // above the lid: 2U - u
+    un = c + a * (l + r + b + t - 4.f * c);
   }
   if (j > 0) {
-    float c = cv_at(vs, n, i, j);
-    vn = c + a * (cv_at(vs, n, i - 1, j) + cv_at(vs, n, i + 1, j) + cv_at(vs, n, i, j - 1) + cv_at(vs, n, i, j + 1) -
-                  4.f * c);
+    const float c = ld_u32(vs, own);
+    float l = ld_u32(vs, j * n + il), r = ld_u32(vs, j * n + ir), b = ld_u32(vs, jb * n + i), t = ld_u32(vs, jt * n + i);
+    if (i == 0) l = -c;                     // beyond the left wall: -v
+    if (i + 1 == n) r = -c;                 // beyond the right wall: -v
+    if (j - 1 == 0) b = 0.f;                // the bottom wall face
+    if (j + 1 == n) t = 0.f;                // the top wall face
+    vn = c + a * (l + r + b + t - 4.f * c);
   }
   if (S.disk) {
     float w = mask_width * h;
