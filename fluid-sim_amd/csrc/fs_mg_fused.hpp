@@ -36,8 +36,8 @@ using Tile64 = TileShape<64, 32, 256>;
 using Tile32 = TileShape<32, 32, 256>;
 using Tile16 = TileShape<16, 16, 128>;
 using Tile8 = TileShape<8, 8, 64>;
-using Tile8x256 = TileShape<8, 8, 256>;    // k_mgt_down2: the level-l region is ~25 times the tile
-using Tile16x256 = TileShape<16, 16, 256>;  // k_mgt_up2
+using Tile8x256 = TileShape<8, 8, 256>;  // k_mgt_down2 and k_mgt_up2 above the coarse solve: the
+                                         // level-l region is ~25 times the tile
 
 // Blocks are issued round-robin over the 8 XCDs; give each XCD a contiguous band of
 // tiles so neighbouring tiles (which share halos) share an L2.

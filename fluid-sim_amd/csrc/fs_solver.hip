@@ -723,13 +723,15 @@ int launch_up2_s(fs_solver_t* s, int l) {
                 as<S>((const float*)C.rhs), (const float*)s->lv[l + 2].xb, as<S>((const float*)F.x), F.xb,
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb);
 }
-// 16 x 16 tiles for the pair above the coarse solve, 32 x 32 for levels 1 and 0
+// 8 x 8 tiles for the pair above the coarse solve (levels 256 and 128 over a 64
+// coarse level: more workgroups in flight; 16 x 16 took 0.4 us more per launch),
+// 32 x 32 for levels 1 and 0
 template <int POST>
 int launch_up2(fs_solver_t* s, int l) {
   if (l == 0)
     return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
-  return s->bf16 ? launch_up2_s<POST, fsk::Tile16x256, fsk::bf16>(s, l)
-                 : launch_up2_s<POST, fsk::Tile16x256, float>(s, l);
+  return s->bf16 ? launch_up2_s<POST, fsk::Tile8x256, fsk::bf16>(s, l)
+                 : launch_up2_s<POST, fsk::Tile8x256, float>(s, l);
 }
 
 template <int POST, class TS>
