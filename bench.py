@@ -55,7 +55,7 @@ KIND_TEXT = {
 # (`bench.py --copy-calibration`: read = 2.0 x FETCH_SIZE KiB, write = 1.0 x
 # WRITE_SIZE KiB). For a class that runs on several levels, the mean over them.
 TRAFFIC_1024 = {
-    "up": 8.80e6, "down": 1.76e6, "advect_fused": 17.64e6, "gradient": 25.42e6, "down0": 36.70e6, "coarse": 0.07e6,
+    "up": 8.78e6, "down": 1.76e6, "advect_fused": 17.4e6, "gradient": 25.4e6, "down0": 36.7e6, "coarse": 0.0705e6,
 }
 
 
