@@ -112,17 +112,19 @@ __device__ __forceinline__ float cav_sample(const float* __restrict__ a, int n, 
     *lo = fminf(fminf(s00, s10), fminf(s01, s11));
     *hi = fmaxf(fmaxf(s00, s10), fmaxf(s01, s11));
   }
-  float bot = s00 + (s10 - s00) * fx, top = s01 + (s11 - s01) * fx;
-  return bot + (top - bot) * fy;
+  // fused multiply-adds: the walled batch has no bit-for-bit counterpart to match, only
+  // the f64 oracle within its tolerance
+  float bot = fmaf(s10 - s00, fx, s00), top = fmaf(s11 - s01, fx, s01);
+  return fmaf(top - bot, fy, bot);
 }
 
 // RK2 departure point from (x, y) over dt, given the velocity (u1, v1) at (x, y).
 __device__ __forceinline__ void cav_trace(const float* __restrict__ u, const float* __restrict__ v, int n, float U,
                                           float x, float y, float u1, float v1, float dt, float* xo, float* yo) {
-  float xm = x - 0.5f * dt * u1, ym = y - 0.5f * dt * v1;
+  float xm = fmaf(-0.5f * dt, u1, x), ym = fmaf(-0.5f * dt, v1, y);
   float u2 = cav_sample<0>(u, n, U, xm, ym, nullptr, nullptr), v2 = cav_sample<1>(v, n, 0.f, xm, ym, nullptr, nullptr);
-  *xo = x - dt * u2;
-  *yo = y - dt * v2;
+  *xo = fmaf(-dt, u2, x);
+  *yo = fmaf(-dt, v2, y);
 }
 
 // The velocity at a face's own position: the face's value itself (the bilinear
