@@ -679,22 +679,33 @@ k_guess_restrict(const S* __restrict__ r, const double* __restrict__ p, int n, i
 
 // p += the bilinear interpolation of the coarse increment d_c (cell-centred,
 // periodic; for k = 1 the geometry of the multigrid's prolongation): fine centre
-// (i + 1/2) sits at coarse coordinate (i + 1/2) / 2^k - 1/2. One thread per fine cell;
-// n * n is a multiple of 256.
+// (i + 1/2) sits at coarse coordinate (i + 1/2) / 2^k - 1/2. One thread per pair of
+// fine cells of a row (a 16-byte read-modify-write of p); n * n / 2 is a multiple of 256.
 __global__ void __launch_bounds__(256)
 k_guess_prolong_add(const double* __restrict__ dc, double* __restrict__ p, int n, int k) {
   const int m = n >> k;
   const int q = blockIdx.x * 256 + threadIdx.x;
-  const int j = q / n, i = q - j * n;
+  const int hn = n >> 1;
+  const int j = q / hn, i2 = 2 * (q - j * hn);
   const double sc = 1.0 / (double)(1 << k);
-  const double gx = (i + 0.5) * sc - 0.5, gy = (j + 0.5) * sc - 0.5;  // exact
-  const double fi = floor(gx), fj = floor(gy);
-  const double fx = gx - fi, fy = gy - fj;
-  const int x0 = wrapi((int)fi, m), x1 = nxt(x0, m), y0 = wrapi((int)fj, m), y1 = nxt(y0, m);
-  const double a = dc[(size_t)y0 * m + x0], b = dc[(size_t)y0 * m + x1];
-  const double c = dc[(size_t)y1 * m + x0], d = dc[(size_t)y1 * m + x1];
-  const double top = a * (1.0 - fx) + b * fx, bot = c * (1.0 - fx) + d * fx;
-  p[q] += top * (1.0 - fy) + bot * fy;
+  const double gy = (j + 0.5) * sc - 0.5, fj = floor(gy), fy = gy - fj;  // exact
+  const int y0 = wrapi((int)fj, m), y1 = nxt(y0, m);
+  double add[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int i = i2 + h;
+    const double gx = (i + 0.5) * sc - 0.5, fi = floor(gx), fx = gx - fi;
+    const int x0 = wrapi((int)fi, m), x1 = nxt(x0, m);
+    const double a = dc[(size_t)y0 * m + x0], b = dc[(size_t)y0 * m + x1];
+    const double c = dc[(size_t)y1 * m + x0], d = dc[(size_t)y1 * m + x1];
+    const double top = a * (1.0 - fx) + b * fx, bot = c * (1.0 - fx) + d * fx;
+    add[h] = top * (1.0 - fy) + bot * fy;
+  }
+  double2* pp = (double2*)(p + (size_t)j * n + i2);
+  double2 v = *pp;
+  v.x += add[0];
+  v.y += add[1];
+  *pp = v;
 }
 
 }  // namespace fsk

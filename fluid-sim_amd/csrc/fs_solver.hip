@@ -996,7 +996,7 @@ int run_guess(fs_solver_t* s, int pi) {
                      : launch_conv<16, 16, CONV_DIRECT, 0, 8, 1, 1, EPI_OUT>(s, 13, out)))
     return rc;
   if (coarse)
-    rc = launch(s, PK_GUESS, 8.0 * m * m + 16.0 * n * n, fsk::k_guess_prolong_add, dim3(n * n / 256), dim3(256),
+    rc = launch(s, PK_GUESS, 8.0 * m * m + 16.0 * n * n, fsk::k_guess_prolong_add, dim3(n * n / 512), dim3(256),
                 (const double*)G.dc, s->p64[pi], n, k);
   return rc;
 }
