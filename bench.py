@@ -222,7 +222,8 @@ def forced_report(a, fluidsim, wl, n, u0, v0):
     errs = []
     t0 = time.perf_counter()
     for k in range(a.validate_steps):
-        dt = g.stepCFL(0.5)
+        g.stepCFL(0.5)
+        dt = g.stats()["last_dt"]
         s.step(dt, force_seed=(f["seed"] + k) if f else 0)
         if k in (0, a.validate_steps // 2, a.validate_steps - 1):
             vel = g.readVelocity()

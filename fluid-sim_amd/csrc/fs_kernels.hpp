@@ -115,6 +115,54 @@ __device__ __forceinline__ void trace(const float* __restrict__ u, const float* 
 struct StepCoef {
   double dt;
   float cx, cy, da, db, fdt, drag;
+  int64_t step;  // index of the step these coefficients belong to
+};
+
+// Device-side convergence gate of a tolerance-driven projection. flag[k] != 0 means
+// "refinement residual k still has a cell above the tolerance"; every block of the
+// residual pass that finds one stores 1 there (the same value from any number of
+// blocks, so plain stores suffice), and the flags start the step at zero.
+//   skip >= 0: the launch does nothing unless flag[skip] is set (a V-cycle on
+//              residual k checks k; the residual pass k checks k - 1, whose V-cycle
+//              produced the correction it adds)
+//   rec >= 0:  a residual pass records into flag[rec]
+// Skipped iterations therefore leave their flags at zero, and the first zero flag
+// is the iteration at which the gate was met. skip = rec = -1: ungated.
+struct Gate {
+  int* flag;
+  double tol;
+  int skip, rec;
+};
+__device__ __forceinline__ bool gate_closed(const Gate& g) { return g.skip >= 0 && g.flag[g.skip] == 0; }
+__device__ __forceinline__ void gate_record(const Gate& g, double block_max) {
+  if (g.rec >= 0 && block_max > g.tol) g.flag[g.rec] = 1;
+}
+__host__ __device__ inline Gate gate_open() { return Gate{nullptr, 0.0, -1, -1}; }
+
+// Where the gradient pass of a device-gated projection finds the pressure, and the
+// per-step record it leaves for the host.
+//   k >= 0: the projection ran gated iterations 0 .. k; the pressure after the first
+//           iteration j whose flag is zero (j = k when none is) is pbuf[(a0 + j + 1) & 1].
+//           The pass subtracts its gradient and, when that buffer is not pbuf[keep],
+//           also copies it there, so the host knows where the pressure is without
+//           reading anything back.
+//   k < 0:  the pressure is pbuf[0]; no record.
+struct StepRecord {
+  double dt;        // the step's dt
+  double residual;  // FP64 max|div u* - L p| after the last iteration run
+  int vcycles;      // V-cycles run (j)
+  int missed;       // 1: no iteration met the gate
+  int64_t step;     // step index (0-based), so the host can check the slot
+};
+struct FinalP {
+  const int* flag;
+  const double* rpart;  // per-block residual maxima, one row of `cols` per iteration
+  int cols;
+  int k, a0, keep;
+  double* pbuf[2];
+  StepRecord* ring;     // host-visible records, slot = step % ring_len (null: no record)
+  int ring_len;
+  const StepCoef* coef; // dt and the step index
 };
 
 // ------------------------------------------------------------------ advection
@@ -493,11 +541,11 @@ constexpr int FX = 64, FY = 16;
 // face (x = i) and at the v face (x = i + 1/2); row[m][j] = (cos 2piB, sin 2piB) at the
 // u face (y = j + 1/2) and at the v face (y = j).
 __global__ void __launch_bounds__(256)
-k_forcing_tables(const ModeSet ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row) {
+k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row) {
   const int q = blockIdx.x * 256 + threadIdx.x, per = nx + ny;
-  if (q >= ms.n * per) return;
+  if (q >= ms->n * per) return;
   const int m = q / per, r = q - m * per;
-  const Mode& md = ms.m[m];
+  const Mode& md = ms->m[m];
   double a, b;
   if (r < nx) {
     a = md.tx * r;
@@ -517,11 +565,11 @@ k_forcing_tables(const ModeSet ms, int nx, int ny, float4* __restrict__ col, flo
 }
 
 __global__ void __launch_bounds__(256)
-k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet ms,
+k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet* __restrict__ ms,
               const float4* __restrict__ col, const float4* __restrict__ row, const StepCoef* __restrict__ cf) {
   __shared__ float4 sc[MAX_MODES][FX], sr[MAX_MODES][FY];
   const int i0 = blockIdx.x * FX, j0 = blockIdx.y * FY;
-  const int nm = ms.n;
+  const int nm = ms->n;
   for (int q = threadIdx.x; q < nm * (FX + FY); q += 256) {
     const int m = q / (FX + FY), r = q - m * (FX + FY);
     if (r < FX) {
@@ -540,7 +588,7 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
 #pragma unroll
   for (int rr = 0; rr < R; rr++) fu[rr] = fv[rr] = 0.f;
   for (int m = 0; m < nm; m++) {
-    const Mode& md = ms.m[m];
+    const Mode& md = ms->m[m];
     const float4 c = sc[m][tx];
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
@@ -677,7 +725,8 @@ template <typename S>
 __global__ void __launch_bounds__(BX * BY)
 k_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
              const float* __restrict__ e, double* __restrict__ pout, S* __restrict__ r32, int nx, int ny,
-             double idx, double idy, double idx2, double idy2, double* __restrict__ rpart) {
+             double idx, double idy, double idx2, double idy2, double* __restrict__ rpart, Gate gate) {
+  if (gate_closed(gate)) return;
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   double amax = 0.0;
   if (i < nx && j < ny) {
@@ -695,7 +744,10 @@ k_residual64(const float* __restrict__ u, const float* __restrict__ v, const dou
     amax = nan_to_inf(fabs(r));
   }
   double m = block_max(amax);
-  if (thread0()) rpart[flat_block()] = m;
+  if (thread0()) {
+    rpart[flat_block()] = m;
+    gate_record(gate, m);
+  }
 }
 
 // ------------------------------------------------------------- multigrid
@@ -768,11 +820,23 @@ k_mg_prolong(const T* __restrict__ xc, T* __restrict__ x, int n, int x_zero) {
 // One wavefront per BX x BY block: lane l takes 4 consecutive cells of row l / 16
 // (16-byte loads and stores when nx % 4 == 0), so the maxima are wavefront
 // reductions with no barrier. Launched with 64 threads on grid2(nx, ny).
+// The pressure is chosen as FinalP says; block 0 also writes the step record.
 __global__ void __launch_bounds__(64)
-k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
-             float* __restrict__ ou, float* __restrict__ ov, int nx, int ny, double idx, double idy,
-             double* __restrict__ part) {
+k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const FinalP fp, float* __restrict__ ou,
+             float* __restrict__ ov, int nx, int ny, double idx, double idy, double* __restrict__ part) {
   static_assert(BX == 64 && BY == 4, "16 lanes x 4 cells per row, 4 rows");
+  int jc = 0, fin = 0;
+  if (fp.k >= 0) {
+    jc = fp.k;
+    for (int q = 0; q < fp.k; q++)
+      if (fp.flag[q] == 0) {
+        jc = q;
+        break;
+      }
+    fin = (fp.a0 + jc + 1) & 1;
+  }
+  const double* __restrict__ p = fp.pbuf[fin];
+  double* __restrict__ pk = (fp.k >= 0 && fin != fp.keep) ? fp.pbuf[fp.keep] : nullptr;
   const int2 bk = xcd_block2();
   const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
   const int i0 = bk.x * BX + 4 * lx, j = bk.y * BY + ly;
@@ -790,6 +854,10 @@ k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const dou
                        (float)((double)vv.z - (c23.x - b23.x) * idy), (float)((double)vv.w - (c23.y - b23.y) * idy)};
       *(float4*)(ou + row + i0) = nu4;
       *(float4*)(ov + row + i0) = nv4;
+      if (pk) {
+        *(double2*)(pk + row + i0) = c01;
+        *(double2*)(pk + row + i0 + 2) = c23;
+      }
       mu = fmax(fmax(nan_to_inf(fabs((double)nu4.x)), nan_to_inf(fabs((double)nu4.y))),
                 fmax(nan_to_inf(fabs((double)nu4.z)), nan_to_inf(fabs((double)nu4.w))));
       mv = fmax(fmax(nan_to_inf(fabs((double)nv4.x)), nan_to_inf(fabs((double)nv4.y))),
@@ -802,6 +870,7 @@ k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const dou
         const float nv_ = (float)((double)v[k] - (pc - p[rowb + i0 + c]) * idy);
         ou[k] = nu_;
         ov[k] = nv_;
+        if (pk) pk[k] = pc;
         mu = fmax(mu, nan_to_inf(fabs((double)nu_)));
         mv = fmax(mv, nan_to_inf(fabs((double)nv_)));
       }
@@ -812,6 +881,22 @@ k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const dou
   if (threadIdx.x == 0) {
     part[2 * flat_block()] = mu;
     part[2 * flat_block() + 1] = mv;
+  }
+  if (fp.ring && fp.k >= 0 && flat_block() == 0) {
+    // the residual the last iteration run left: the row of iteration jc
+    const double* rowp = fp.rpart + (size_t)jc * fp.cols;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < fp.cols; q += 64) m = fmax(m, rowp[q]);
+    m = wave_max(m);
+    if (threadIdx.x == 0) {
+      const int64_t st = fp.coef->step;
+      StepRecord* r = fp.ring + (size_t)(st % fp.ring_len);
+      r->dt = fp.coef->dt;
+      r->residual = m;
+      r->vcycles = jc;
+      r->missed = (jc == fp.k && fp.flag[fp.k] != 0) ? 1 : 0;
+      r->step = st;
+    }
   }
 }
 
@@ -842,41 +927,53 @@ __device__ __forceinline__ void fill_coef(StepCoef* cf, double dt, double dx, do
   cf->drag = drag > 0 ? (float)exp(-drag * dt) : 1.0f;
 }
 
-// Coefficients of a step with a dt chosen on the host.
-__global__ void k_set_coef(StepCoef* cf, double dt, double dx, double dy, double nu, double drag) {
-  if (threadIdx.x == 0) fill_coef(cf, dt, dx, dy, nu, drag);
-}
-
-// dt = cfl / (max|u|/dx + max|v|/dy) (src/core/grid.js:322-327) from the block
-// maxima part[2*b + {0,1}] of the last gradient (or maxabs) pass; one workgroup.
+// What the first launch of a step sets up on the device (one workgroup of 1024):
+//   dt: `value` itself (use_cfl = 0), or value / (max|u|/dx + max|v|/dy)
+//       (src/core/grid.js:322-327) from the block maxima part[2*b + {0,1}] of the
+//       last gradient (or maxabs) pass; then the step's coefficients and index
+//   the step's forcing modes, copied from the argument into ms_dev (copy_modes)
+//   the convergence flags 0 .. nflags-1 of the gated projection, cleared
 __global__ void __launch_bounds__(1024)
-k_cfl_dt(const double* __restrict__ part, int nparts, double cfl, double dx, double dy, double nu, double drag,
-         double dt_max, StepCoef* __restrict__ cf) {
+k_step_head(const double* __restrict__ part, int nparts, int use_cfl, double value, double dx, double dy,
+            double nu, double drag, double dt_max, int64_t step, StepCoef* __restrict__ cf, int* __restrict__ flags,
+            int nflags, const ModeSet ms, ModeSet* __restrict__ ms_dev, int copy_modes) {
   __shared__ double su[16], sv[16];
-  double mu = 0.0, mv = 0.0;
-  for (int b = threadIdx.x; b < nparts; b += 1024) {
-    mu = fmax(mu, part[2 * b]);
-    mv = fmax(mv, part[2 * b + 1]);
+  for (int q = threadIdx.x; q < nflags; q += 1024) flags[q] = 0;
+  if (copy_modes) {
+    const int* src = reinterpret_cast<const int*>(&ms);
+    int* dst = reinterpret_cast<int*>(ms_dev);
+    constexpr int W = (int)(sizeof(ModeSet) / sizeof(int));
+    for (int q = threadIdx.x; q < W; q += 1024) dst[q] = src[q];
   }
-  mu = wave_max(mu);
-  mv = wave_max(mv);
-  if ((threadIdx.x & 63) == 0) {
-    su[threadIdx.x >> 6] = mu;
-    sv[threadIdx.x >> 6] = mv;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 16; w++) {
-      mu = fmax(mu, su[w]);
-      mv = fmax(mv, sv[w]);
+  double dt = value;
+  if (use_cfl) {
+    double mu = 0.0, mv = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 1024) {
+      mu = fmax(mu, part[2 * b]);
+      mv = fmax(mv, part[2 * b + 1]);
     }
-    mu = fmax(mu, su[0]);
-    mv = fmax(mv, sv[0]);
-    double rate = mu / dx + mv / dy;
-    double dt;
-    if (!(rate <= 1.7976931348623157e308)) dt = __longlong_as_double(0x7ff8000000000000LL);  // non-finite field
-    else dt = rate > 0 ? fmin(cfl / rate, dt_max) : dt_max;
+    mu = wave_max(mu);
+    mv = wave_max(mv);
+    if ((threadIdx.x & 63) == 0) {
+      su[threadIdx.x >> 6] = mu;
+      sv[threadIdx.x >> 6] = mv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 16; w++) {
+        mu = fmax(mu, su[w]);
+        mv = fmax(mv, sv[w]);
+      }
+      mu = fmax(mu, su[0]);
+      mv = fmax(mv, sv[0]);
+      const double rate = mu / dx + mv / dy;
+      if (!(rate <= 1.7976931348623157e308)) dt = __longlong_as_double(0x7ff8000000000000LL);  // non-finite field
+      else dt = rate > 0 ? fmin(value / rate, dt_max) : dt_max;
+    }
+  }
+  if (threadIdx.x == 0) {
     fill_coef(cf, dt, dx, dy, nu, drag);
+    cf->step = step;
   }
 }
 

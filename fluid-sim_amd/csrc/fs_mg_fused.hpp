@@ -192,7 +192,8 @@ __device__ __forceinline__ void tile_store_and_restrict(const float* __restrict_
 template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_down(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float ih2,
-           S* __restrict__ crhs, int tiles_x, int nblocks) {
+           S* __restrict__ crhs, int tiles_x, int nblocks, Gate gate) {
+  if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
@@ -218,7 +219,9 @@ k_mgt_down(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float 
 template <int SW, class TS, typename S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_down2(S* __restrict__ x0, const S* __restrict__ rhs0, int n0, float h20, float ih20, S* __restrict__ x1,
-            S* __restrict__ rhs1, float h21, float ih21, S* __restrict__ crhs, int tiles_x, int nblocks) {
+            S* __restrict__ rhs1, float h21, float ih21, S* __restrict__ crhs, int tiles_x, int nblocks,
+            Gate gate) {
+  if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW1 = TX + 2 * H, RH1 = TY + 2 * H;            // level l+1 region
@@ -273,7 +276,8 @@ __global__ void __launch_bounds__(TS::NT)
 k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
             const float* __restrict__ e_prev, double* __restrict__ pout, S* __restrict__ r32,
             S* __restrict__ x, S* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2,
-            int tiles_x, int nblocks, double* __restrict__ rpart, int residual_only) {
+            int tiles_x, int nblocks, double* __restrict__ rpart, int residual_only, Gate gate) {
+  if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
@@ -352,6 +356,7 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
       double b = wm[0];
       for (int w = 1; w < NT / 64; w++) b = fmax(b, wm[w]);
       rpart[t] = b;
+      gate_record(gate, b);
     }
   }
   if (residual_only) return;
@@ -367,7 +372,8 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
 template <int SW, class TS, typename S, typename SO = S>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__ rhs,
-         const SO* __restrict__ xc, int n, float h2, int tiles_x, int nblocks) {
+         const SO* __restrict__ xc, int n, float h2, int tiles_x, int nblocks, Gate gate) {
+  if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
@@ -435,7 +441,8 @@ template <int SW, class TS, typename S, typename SO>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __restrict__ xc2,
           const S* __restrict__ x0in, SO* __restrict__ x0out, const S* __restrict__ rhs0, int n0, float h20,
-          float h21, int tiles_x, int nblocks) {
+          float h21, int tiles_x, int nblocks, Gate gate) {
+  if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
   constexpr int RW0 = TX + 2 * H, RH0 = TY + 2 * H;         // level-l region
@@ -636,7 +643,8 @@ __device__ __forceinline__ float reg_residual(float x, float r, int n, int i, in
 template <typename S, typename SO = S>
 __global__ void __launch_bounds__(CTH)
 k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb, int pre, int post,
-               int coarse, const float* __restrict__ pinv) {
+               int coarse, const float* __restrict__ pinv, Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ float bx[CML_CELLS];
   __shared__ float br[CML_CELLS];
   __shared__ float spv[REG_N * REG_N * REG_N * REG_N];  // pseudo-inverse of the coarsest register level

@@ -109,6 +109,9 @@ struct Profile {
 };
 
 constexpr int DT_RING = 4096;  // steps whose dt may be pending a host read
+constexpr int REC_RING = 64;   // host-visible step records of gated steps
+constexpr int MAX_AHEAD = 4;   // gated steps in flight before the host waits for the oldest
+constexpr int GATE_SPARE = 1;  // V-cycles enqueued beyond the prediction
 
 }  // namespace
 
@@ -193,6 +196,35 @@ struct fs_solver_t {
   double last_dt = 0;
   int last_pred = 2;  // predicted V-cycles for the next projection
   Profile prof;
+  // Device-gated projection (tolerance-driven multigrid on the fused V-cycle): each
+  // step enqueues the predicted V-cycles plus a spare, every launch gated by the
+  // convergence flags its residual pass leaves, so the host never waits inside a
+  // step; the gradient pass records what happened in a host-visible ring, read when
+  // the step's event has completed.
+  bool gated = true;          // use it (after the first projection has set the prediction)
+  bool pred_known = false;    // last_pred comes from a projection of the current flow
+  int* gate_flag = nullptr;   // device: one flag per refinement iteration
+  int gate_len = 0;
+  fsk::Gate gate_r = fsk::gate_open(), gate_v = fsk::gate_open();  // gates of the launches being enqueued
+  fsk::ModeSet* modes_dev = nullptr;  // the step's forcing modes (device), written by the step head
+  fsk::StepRecord* ring = nullptr;    // host-visible step records, slot = step % RING
+  struct Inflight {
+    hipEvent_t ev;
+    int64_t step;
+  };
+  std::vector<Inflight> inflight;     // gated steps whose record has not been read
+  std::vector<hipEvent_t> ev_pool;
+  int64_t gate_misses = 0;            // gated steps whose last residual was above the tolerance
+  int64_t gated_steps = 0;
+  // captured step bodies, keyed by what changes the launches they hold
+  struct StepGraph {
+    int k, a0, zero_p, cur, keep;
+    int cur_after;  // velocity buffer after the body
+    hipGraphExec_t exec;
+  };
+  bool use_graph = true;
+  std::vector<StepGraph> graphs;
+  int64_t graph_captures = 0, graph_launches = 0;
 };
 
 namespace {
@@ -229,6 +261,19 @@ int halloc(T** p, size_t count) {
     *p = nullptr;
     return fail(FS_E_OOM, "hipHostMalloc(%zu bytes) failed", bytes);
   }
+  return FS_OK;
+}
+
+// Host memory the device writes during a kernel and the host reads once the kernel's
+// completion has been observed (step records): coherent, so no cache holds the writes.
+template <typename T>
+int halloc_coherent(T** p, size_t count) {
+  size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 1;
+  if (hipHostMalloc((void**)p, bytes, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+    *p = nullptr;
+    return fail(FS_E_OOM, "hipHostMalloc(%zu bytes, coherent) failed", bytes);
+  }
+  std::memset((void*)*p, 0xff, bytes);  // no record claims a step yet
   return FS_OK;
 }
 
@@ -301,10 +346,13 @@ void after_sync(fs_solver_t* s) {
   }
 }
 
+int consume_records(fs_solver_t* s, bool wait);
+
+// Wait for the stream; then every timing, dt and step record is on the host.
 int sync_stream(fs_solver_t* s) {
   HIPCHK(hipStreamSynchronize(s->stream));
   after_sync(s);
-  return FS_OK;
+  return consume_records(s, true);
 }
 
 int check_solver(void* h) {
@@ -332,19 +380,57 @@ double drag_alpha(const fs_solver_t* s) { return s->forcing_on ? s->forcing.drag
 
 // ------------------------------------------------------------- step pieces
 
-int enqueue_coef_fixed(fs_solver_t* s, double dt) {
-  fsk::k_set_coef<<<1, 64, 0, s->stream>>>(s->coef, dt, s->dx, s->dy, s->cfg.nu, drag_alpha(s));
-  LAUNCHCHK();
-  return FS_OK;
+double js_round(double x) {
+  double r = std::floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
 }
 
-int enqueue_coef_cfl(fs_solver_t* s, double cfl) {
-  if (!s->have_maxabs) {
+// The band-forcing modes of step `step` (src/cpu/solver.js:181-199) from the exact
+// uint32 LCG; the amplitude factor dt and the drag factor come from the device
+// coefficients.
+void make_modes(const fs_solver_t* s, int64_t step, fsk::ModeSet& ms) {
+  const fs_forcing& f = s->forcing;
+  uint32_t st = f.seed + (uint32_t)step;
+  auto rnd = [&]() {
+    st = st * 1664525u + 1013904223u;
+    return (double)st / 4294967296.0;
+  };
+  ms = fsk::ModeSet{};
+  const int M = 24;
+  double scale = f.amp / std::sqrt((double)M);
+  int k = 0;
+  for (int m = 0; m < M; m++) {
+    double theta = 2 * M_PI * rnd();
+    double kmag = f.kf + (rnd() - 0.5) * 2 * f.width;
+    int kx = (int)js_round(kmag * std::cos(theta));
+    int ky = (int)js_round(kmag * std::sin(theta));
+    double ph = 2 * M_PI * rnd();
+    if (kx == 0 && ky == 0) continue;
+    double kk = std::sqrt((double)kx * kx + (double)ky * ky);
+    fsk::Mode& md = ms.m[k++];
+    md.tx = kx * s->dx / (2 * M_PI);
+    md.ty = ky * s->dy / (2 * M_PI);
+    md.tph = ph / (2 * M_PI);
+    md.cu = (float)(scale * (ky / kk));
+    md.cv = (float)(scale * (-kx / kk));
+  }
+  ms.n = k;
+}
+
+// The first launch of a step: dt (fixed, or from the CFL condition on the device),
+// the coefficients and step index, the step's forcing modes, cleared gate flags.
+int enqueue_head(fs_solver_t* s, bool use_cfl, double value, int nflags) {
+  if (use_cfl && !s->have_maxabs) {
     fsk::k_maxabs<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[s->cur], s->v[s->cur], s->nx, s->ny, s->mpart);
     LAUNCHCHK();
   }
-  fsk::k_cfl_dt<<<1, 1024, 0, s->stream>>>(s->mpart, s->mp_count, cfl, s->dx, s->dy, s->cfg.nu, drag_alpha(s), 1e30,
-                                           s->coef);
+  fsk::ModeSet ms{};
+  if (s->forcing_on) make_modes(s, s->steps, ms);
+  fsk::k_step_head<<<1, 1024, 0, s->stream>>>(s->mpart, s->mp_count, use_cfl ? 1 : 0, value, s->dx, s->dy,
+                                              s->cfg.nu, drag_alpha(s), 1e30, s->steps, s->coef, s->gate_flag,
+                                              std::min(nflags, s->gate_len), ms, s->modes_dev,
+                                              s->forcing_on ? 1 : 0);
   LAUNCHCHK();
   return FS_OK;
 }
@@ -393,57 +479,21 @@ int enqueue_diffusion(fs_solver_t* s) {
   return FS_OK;
 }
 
-double js_round(double x) {
-  double r = std::floor(x);
-  if (x - r >= 0.5) r += 1.0;
-  return r;
-}
-
-// 24 modes of src/cpu/solver.js:181-199 from the exact uint32 LCG; the amplitude
-// factor dt and the drag factor come from the device coefficients.
+// Band forcing then drag of src/cpu/solver.js:200-237 with the modes the step head
+// copied to the device: per-step cos/sin tables, then the separable update.
 int enqueue_forcing(fs_solver_t* s) {
   if (!s->forcing_on) return FS_OK;
-  const fs_forcing& f = s->forcing;
-  uint32_t st = f.seed + (uint32_t)s->steps;
-  auto rnd = [&]() {
-    st = st * 1664525u + 1013904223u;
-    return (double)st / 4294967296.0;
-  };
-  fsk::ModeSet ms{};
-  const int M = 24;
-  double scale = f.amp / std::sqrt((double)M);
-  int k = 0;
-  for (int m = 0; m < M; m++) {
-    double theta = 2 * M_PI * rnd();
-    double kmag = f.kf + (rnd() - 0.5) * 2 * f.width;
-    int kx = (int)js_round(kmag * std::cos(theta));
-    int ky = (int)js_round(kmag * std::sin(theta));
-    double ph = 2 * M_PI * rnd();
-    if (kx == 0 && ky == 0) continue;
-    double kk = std::sqrt((double)kx * kx + (double)ky * ky);
-    fsk::Mode& md = ms.m[k++];
-    md.tx = kx * s->dx / (2 * M_PI);
-    md.ty = ky * s->dy / (2 * M_PI);
-    md.tph = ph / (2 * M_PI);
-    md.cu = (float)(scale * (ky / kk));
-    md.cv = (float)(scale * (-kx / kk));
-  }
-  ms.n = k;
   int c = s->cur, rc;
-  if (!s->ftab_col) {
-    if ((rc = dalloc(&s->ftab_col, (size_t)fsk::MAX_MODES * s->nx)) ||
-        (rc = dalloc(&s->ftab_row, (size_t)fsk::MAX_MODES * s->ny)))
-      return rc;
-  }
-  if (ms.n > 0) {  // with no mode in the band the pass below still applies the drag
-    const unsigned tb = (unsigned)((ms.n * (s->nx + s->ny) + 255) / 256);
-    if ((rc = launch(s, PK_FORCING, 16.0 * ms.n * (s->nx + s->ny), fsk::k_forcing_tables, dim3(tb), dim3(256), ms,
-                     s->nx, s->ny, s->ftab_col, s->ftab_row)))
-      return rc;
-  }
+  // sized for every mode: the count of the step is on the device (a mode whose rounded
+  // wavenumber is zero is dropped); with no mode in the band the pass below still drags
+  const unsigned tb = (unsigned)((fsk::MAX_MODES * (s->nx + s->ny) + 255) / 256);
+  if ((rc = launch(s, PK_FORCING, 16.0 * fsk::MAX_MODES * (s->nx + s->ny), fsk::k_forcing_tables, dim3(tb), dim3(256),
+                   (const fsk::ModeSet*)s->modes_dev, s->nx, s->ny, s->ftab_col, s->ftab_row)))
+    return rc;
   const dim3 g((unsigned)((s->nx + fsk::FX - 1) / fsk::FX), (unsigned)((s->ny + fsk::FY - 1) / fsk::FY));
-  return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny, ms,
-                (const float4*)s->ftab_col, (const float4*)s->ftab_row, (const fsk::StepCoef*)s->coef);
+  return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny,
+                (const fsk::ModeSet*)s->modes_dev, (const float4*)s->ftab_col, (const float4*)s->ftab_row,
+                (const fsk::StepCoef*)s->coef);
 }
 
 // Direct-forcing coupling of the disk, after forcing and before the projection
@@ -568,7 +618,7 @@ int mg_vcycle(fs_solver_t* s) {
     if (C.n <= fsk::COARSE_N) {
       int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml<float>, dim3(1), dim3(fsk::CTH), C.x,
                       (const float*)C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre, s->cfg.mg_post,
-                      s->cfg.mg_coarse, (const float*)s->pinv);
+                      s->cfg.mg_coarse, (const float*)s->pinv, fsk::gate_open());
       if (rc) return rc;
     } else {
       bool z = true;
@@ -649,7 +699,7 @@ int launch_down0_s(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_o
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
                 (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
                 as<S>(s->lv[1].rhs), L0.n, idx, idx * idx, (float)(L0.dx * L0.dx), (float)(1.0 / (L0.dx * L0.dx)), tx,
-                nb, s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0);
+                nb, s->rpart + (size_t)it * s->rp_cols, residual_only ? 1 : 0, s->gate_r);
 }
 template <int PRE, class TS>
 int launch_down0_t(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
@@ -669,7 +719,7 @@ int launch_down_s(fs_solver_t* s, int l) {
   // rhs read; x and the coarse rhs (1 per 4 cells) written
   return launch(s, PK_DOWN, 2.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_down<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)),
-                as<S>(s->lv[l + 1].rhs), tx, nb);
+                as<S>(s->lv[l + 1].rhs), tx, nb, s->gate_v);
 }
 template <int PRE, class TS>
 int launch_down_t(fs_solver_t* s, int l) {
@@ -692,7 +742,8 @@ int launch_down2_s(fs_solver_t* s, int l) {
   const double bytes = 2.0 * sizeof(S) * F.n * F.n + 2.25 * sizeof(S) * C.n * C.n;
   return launch(s, PK_DOWN, bytes, fsk::k_mgt_down2<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), as<S>(C.x),
-                as<S>(C.rhs), (float)(C.dx * C.dx), (float)(1.0 / (C.dx * C.dx)), as<S>(s->lv[l + 2].rhs), tx, nb);
+                as<S>(C.rhs), (float)(C.dx * C.dx), (float)(1.0 / (C.dx * C.dx)), as<S>(s->lv[l + 2].rhs), tx, nb,
+                s->gate_v);
 }
 template <int PRE>
 int launch_down2(fs_solver_t* s, int l) {
@@ -708,7 +759,7 @@ int launch_up_s(fs_solver_t* s, int l) {
   // the correction written in FP32 (see Level::xb)
   return launch(s, PK_UP, (2.0 * sizeof(S) + 5.0) * F.n * F.n, fsk::k_mgt_up<POST, TS, S, float>, g, dim3(TS::NT),
                 as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
-                (float)(F.dx * F.dx), tx, nb);
+                (float)(F.dx * F.dx), tx, nb, s->gate_v);
 }
 // Levels l + 1 and l up in one launch (k_mgt_up2), tiles TS of level l.
 template <int POST, class TS, typename S>
@@ -721,7 +772,7 @@ int launch_up2_s(fs_solver_t* s, int l) {
   const double bytes = (2.0 * sizeof(S) + 1.0) * C.n * C.n + (2.0 * sizeof(S) + 4.0) * F.n * F.n;
   return launch(s, PK_UP, bytes, fsk::k_mgt_up2<POST, TS, S, float>, g, dim3(TS::NT), as<S>((const float*)C.x),
                 as<S>((const float*)C.rhs), (const float*)s->lv[l + 2].xb, as<S>((const float*)F.x), F.xb,
-                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb);
+                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v);
 }
 // 8 x 8 tiles for the pair above the coarse solve (levels 256 and 128 over a 64
 // coarse level: more workgroups in flight; 16 x 16 took 0.4 us more per launch),
@@ -749,7 +800,7 @@ int launch_coarse_s(fs_solver_t* s, int from) {
   // rhs at the storage precision in, the result (the correction passed up) out in FP32
   return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
                 dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
-                s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv);
+                s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv, s->gate_v);
 }
 int launch_coarse_ml(fs_solver_t* s, int from) {
   return s->bf16 ? launch_coarse_s<fsk::bf16>(s, from) : launch_coarse_s<float>(s, from);
@@ -767,7 +818,8 @@ int enqueue_residual_s(fs_solver_t* s, int it, int pin, bool zero_p, const float
   double bpc = 8 + (zero_p ? 0 : 8) + (e ? 4 : 0) + 8 + sizeof(S);
   return launch(s, PK_RESIDUAL, bpc * s->n, fsk::k_residual64<S>, grid2(s->nx, s->ny), BLK, (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]), e, s->p64[1 - pin],
-                as<S>(s->lv[0].rhs), s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols);
+                as<S>(s->lv[0].rhs), s->nx, s->ny, idx, idy, idx * idx, idy * idy, s->rpart + (size_t)it * s->rp_cols,
+                s->gate_r);
 }
 int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* e) {
   return s->bf16 ? enqueue_residual_s<fsk::bf16>(s, it, pin, zero_p, e) : enqueue_residual_s<float>(s, it, pin, zero_p, e);
@@ -812,16 +864,16 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
   return FS_OK;
 }
 
-int launch_gradient64(fs_solver_t* s);
+int launch_gradient64(fs_solver_t* s, const fsk::FinalP* fp);
 
-// The residual rows 0 .. rows-1 to the host, and their maxima; the gradient of
-// p64[s->pidx] is enqueued behind the copy and runs while the host waits for the
-// copy alone.
-int read_residuals_overlapped(fs_solver_t* s, int rows, std::vector<double>& rmax) {
+// The residual rows 0 .. rows-1 to the host, and their maxima; the gradient of the
+// pressure `fp` selects is enqueued behind the copy and runs while the host waits for
+// the copy alone.
+int read_residuals_overlapped(fs_solver_t* s, int rows, std::vector<double>& rmax, const fsk::FinalP* fp) {
   HIPCHK(hipMemcpyAsync(s->rpart_host, s->rpart, sizeof(double) * (size_t)rows * s->rp_cols,
                         hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipEventRecord(s->rb_event, s->stream));
-  int rc = launch_gradient64(s);
+  int rc = launch_gradient64(s, fp);
   if (rc) return rc;
   HIPCHK(hipEventSynchronize(s->rb_event));
   after_sync(s);
@@ -1048,58 +1100,83 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
     *cycles_out = maxc;
     return FS_OK;
   }
-  // predicted V-cycles, then one read-back; add V-cycles one at a time if needed.
-  // While the host waits for the residuals, the device already subtracts the
-  // gradient of the pressure they belong to (into the other velocity buffer); the
-  // projection keeps it if the gate is met and enqueues it again otherwise.
-  int c = std::max(0, std::min(s->last_pred, maxc));
+  // Predicted V-cycles, then one read-back; add V-cycles one at a time if needed.
+  // On the fused path every launch is gated on the device as in a gated step, so the
+  // projection stops at the first iteration that meets the gate even when the
+  // prediction was larger; rows after that iteration are stale and never consulted.
+  // The per-operator path has no gated launches and searches from zero instead.
+  // While the host waits for the residuals, the device already subtracts the gradient
+  // of the pressure they belong to (into the other velocity buffer); the projection
+  // keeps it if the gate is met and enqueues it again otherwise.
+  int c = s->fused ? std::max(0, std::min(s->last_pred, maxc)) : 0;
+  if (s->fused && (rc = fill_f32(s, (float*)s->gate_flag, (size_t)s->gate_len, 0.f))) return rc;
+  auto iteration = [&](int it, bool residual_only) {
+    if (s->fused) {
+      s->gate_r = fsk::Gate{s->gate_flag, tol, it > 0 ? it - 1 : -1, it};
+      s->gate_v = fsk::Gate{s->gate_flag, tol, it, -1};
+    }
+    int r = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, residual_only);
+    s->gate_r = s->gate_v = fsk::gate_open();
+    return r;
+  };
   for (int it = 0; it < c; it++)
-    if ((rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, false))) return rc;
-  if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, c == 0 && zero_p, true))) return rc;
+    if ((rc = iteration(it, false))) return rc;
+  if ((rc = iteration(c, true))) return rc;
   std::vector<double> rmax;
   for (;;) {
     s->pidx = (a0 + c + 1) & 1;
-    if ((rc = read_residuals_overlapped(s, c + 1, rmax))) return rc;
-    double last = rmax[c];
-    if (std::isnan(last) || std::isinf(last)) {
-      s->pidx = (a0 + c + 1) & 1;
-      *cycles_out = c;
-      s->last_residual = last;
-      s->have_maxabs = false;  // the gradient pass that ran rewrote the |u|, |v| maxima
-      return fail(FS_E_NONFINITE, "non-finite residual in the projection (V-cycle %d)", c);
+    fsk::FinalP f{};
+    f.flag = s->gate_flag;
+    f.k = s->fused ? c : -1;
+    f.a0 = a0;
+    f.keep = s->pidx;
+    f.pbuf[0] = s->fused ? s->p64[0] : s->p64[s->pidx];
+    f.pbuf[1] = s->fused ? s->p64[1] : s->p64[s->pidx];
+    if ((rc = read_residuals_overlapped(s, c + 1, rmax, &f))) return rc;
+    int need = -1;
+    for (int k = 0; k <= c && need < 0; k++) {
+      if (std::isnan(rmax[k]) || std::isinf(rmax[k])) {
+        *cycles_out = k;
+        s->last_residual = rmax[k];
+        s->have_maxabs = false;  // the gradient pass that ran rewrote the |u|, |v| maxima
+        return fail(FS_E_NONFINITE, "non-finite residual in the projection (V-cycle %d)", k);
+      }
+      if (rmax[k] <= tol) need = k;
     }
-    if (last <= tol || c >= maxc) {
+    if (need >= 0 || c >= maxc) {
+      const int used = need >= 0 ? need : c;
       s->grad_pending = true;
-      int need = c;
-      for (int k = 0; k <= c; k++)
-        if (rmax[k] <= tol) {
-          need = k;
-          break;
-        }
-      s->last_pred = need;
-      s->pidx = (a0 + c + 1) & 1;
-      s->last_residual = last;
-      *cycles_out = c;
-      if (last > tol)
+      s->last_pred = used;
+      s->pred_known = true;
+      s->last_residual = rmax[used];
+      *cycles_out = used;
+      if (need < 0)
         return fail(FS_E_NOT_CONVERGED, "multigrid did not reach tolerance %.3g in %d V-cycles (residual %.3g)",
-                    tol, maxc, last);
+                    tol, maxc, rmax[c]);
       return FS_OK;
     }
     // residual c was enqueued residual-only: rerun it with its V-cycle, then probe c+1
-    if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, c == 0 && zero_p, false))) return rc;
+    if ((rc = iteration(c, false))) return rc;
     c++;
-    if ((rc = enqueue_iteration(s, c, (a0 + c) & 1, false, true))) return rc;
+    if ((rc = iteration(c, true))) return rc;
   }
 }
 
-// u - grad p64[pidx] into the other velocity buffer, with the |u|, |v| block maxima;
-// the solver's state is not changed (see commit_gradient64).
-int launch_gradient64(fs_solver_t* s) {
+// u - grad p into the other velocity buffer, with the |u|, |v| block maxima; the
+// solver's state is not changed (see commit_gradient64). Without a FinalP the
+// pressure is p64[pidx].
+int launch_gradient64(fs_solver_t* s, const fsk::FinalP* fp) {
   int c = s->cur;
   double idx = 1.0 / s->dx, idy = 1.0 / s->dy;
+  fsk::FinalP f{};
+  if (fp) {
+    f = *fp;
+  } else {
+    f.k = -1;
+    f.pbuf[0] = f.pbuf[1] = s->p64[s->pidx];
+  }
   return launch(s, PK_GRADIENT, 24.0 * s->n, fsk::k_gradient64, grid2(s->nx, s->ny), dim3(64), (const float*)s->u[c],
-                (const float*)s->v[c], (const double*)s->p64[s->pidx], s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx,
-                idy, s->mpart);
+                (const float*)s->v[c], f, s->u[1 - c], s->v[1 - c], s->nx, s->ny, idx, idy, s->mpart);
 }
 
 void commit_gradient64(fs_solver_t* s) {
@@ -1108,7 +1185,7 @@ void commit_gradient64(fs_solver_t* s) {
 }
 
 int enqueue_gradient64(fs_solver_t* s) {
-  int rc = launch_gradient64(s);
+  int rc = launch_gradient64(s, nullptr);
   if (rc) return rc;
   commit_gradient64(s);
   return FS_OK;
@@ -1167,6 +1244,198 @@ int do_step(fs_solver_t* s) {
   s->total_vcycles += cycles;
   s->steps++;
   return prc;
+}
+
+
+// ------------------------------------------------------- device-gated step
+
+// Read the records of gated steps whose event has completed, oldest first; with
+// `wait`, wait for all of them. The prediction for the next steps is the V-cycle
+// count of the latest record (one more than it ran when it missed the gate).
+int consume_records(fs_solver_t* s, bool wait) {
+  size_t done = 0;
+  int rc = FS_OK;
+  for (; done < s->inflight.size(); done++) {
+    const auto& f = s->inflight[done];
+    if (wait) {
+      hipError_t e = hipEventSynchronize(f.ev);
+      if (e != hipSuccess) {
+        rc = fail(FS_E_DEVICE, "hipEventSynchronize failed: %s", hipGetErrorString(e));
+        break;
+      }
+    } else {
+      hipError_t e = hipEventQuery(f.ev);
+      if (e == hipErrorNotReady) break;
+      if (e != hipSuccess) {
+        rc = fail(FS_E_DEVICE, "hipEventQuery failed: %s", hipGetErrorString(e));
+        break;
+      }
+    }
+    const fsk::StepRecord r = s->ring[f.step % REC_RING];
+    s->ev_pool.push_back(f.ev);
+    if (r.step != f.step) {
+      done++;
+      rc = fail(FS_E_DEVICE, "the record of step %lld is missing", (long long)f.step);
+      break;
+    }
+    s->t += r.dt;
+    s->last_dt = r.dt;
+    s->last_vcycles = r.vcycles;
+    s->total_vcycles += r.vcycles;
+    s->last_residual = r.residual;
+    s->last_pred = r.missed ? r.vcycles + 1 : r.vcycles;
+    if (r.missed) s->gate_misses++;
+    if (!std::isfinite(r.residual) && rc == FS_OK)
+      rc = fail(FS_E_NONFINITE, "non-finite residual in the projection of step %lld", (long long)f.step);
+  }
+  s->inflight.erase(s->inflight.begin(), s->inflight.begin() + done);
+  return rc;
+}
+
+void destroy_graphs(fs_solver_t* s) {
+  if (s->graphs.empty()) return;
+  (void)hipStreamSynchronize(s->stream);
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
+bool learned_guess(const fs_solver_t* s) {
+  return s->cfg.guess == FS_GUESS_LEARNED || s->cfg.guess == FS_GUESS_LEARNED_COARSE ||
+         s->cfg.guess == FS_GUESS_LEARNED_QUARTER;
+}
+
+// A tolerance-driven multigrid projection on the fused V-cycle, with a prediction
+// from the current flow and no launch timing (a profiled pass times every launch it
+// enqueues, so it takes the host-checked path).
+bool gated_eligible(const fs_solver_t* s) {
+  return s->gated && s->pred_known && s->cfg.pressure_solver == FS_P_MULTIGRID && s->cfg.tolerance > 0 &&
+         s->fused && s->gate_flag && s->prof.mask == 0;
+}
+
+// What the launches of a gated step body depend on besides the configuration:
+// iterations k, the pressure buffer the step starts from, a zero initial pressure,
+// the velocity buffer, and the buffer the pressure is left in.
+struct BodyKey {
+  int k, a0, zero_p, cur, keep;
+};
+
+// The step after its head: advection, diffusion, forcing, disk; the learned guess;
+// refinement iterations 0 .. k-1, each launch gated on the flags of its residual
+// pass; the residual pass k; the gradient of the pressure of the first iteration
+// that met the gate, left in p64[keep]. Launches only, so it can be captured.
+int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
+  int rc;
+  if (s->cfg.advection == FS_ADV_MACCORMACK && s->cfg.adv_fused) {
+    if ((rc = enqueue_advect_diffuse_fused(s))) return rc;
+  } else {
+    if ((rc = enqueue_advection(s))) return rc;
+    if ((rc = enqueue_diffusion(s))) return rc;
+  }
+  if ((rc = enqueue_forcing(s))) return rc;
+  if ((rc = enqueue_solid(s))) return rc;
+  int a0 = key.a0;
+  bool zero_p = key.zero_p != 0;
+  if (learned_guess(s)) {
+    // the residual the previous pressure leaves, the network's increment on top of it
+    if ((rc = enqueue_iteration(s, 0, a0 & 1, zero_p, true))) return rc;
+    if ((rc = run_guess(s, 1 - (a0 & 1)))) return rc;
+    a0 = 1 - (a0 & 1);
+    zero_p = false;
+  }
+  const double tol = s->cfg.tolerance;
+  for (int it = 0; it <= key.k && !rc; it++) {
+    s->gate_r = fsk::Gate{s->gate_flag, tol, it > 0 ? it - 1 : -1, it};
+    s->gate_v = fsk::Gate{s->gate_flag, tol, it, -1};
+    rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, it == key.k);
+  }
+  s->gate_r = s->gate_v = fsk::gate_open();
+  if (rc) return rc;
+  fsk::FinalP f{};
+  f.flag = s->gate_flag;
+  f.rpart = s->rpart;
+  f.cols = s->rp_cols;
+  f.k = key.k;
+  f.a0 = a0;
+  f.keep = key.keep;
+  f.pbuf[0] = s->p64[0];
+  f.pbuf[1] = s->p64[1];
+  f.ring = s->ring;
+  f.ring_len = REC_RING;
+  f.coef = s->coef;
+  if ((rc = launch_gradient64(s, &f))) return rc;
+  commit_gradient64(s);
+  return FS_OK;
+}
+
+// The body as a graph, captured on first use of its key and replayed after.
+int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
+  for (const auto& g : s->graphs)
+    if (g.k == key.k && g.a0 == key.a0 && g.zero_p == key.zero_p && g.cur == key.cur && g.keep == key.keep) {
+      HIPCHK(hipGraphLaunch(g.exec, s->stream));
+      s->graph_launches++;
+      s->cur = g.cur_after;
+      s->have_maxabs = true;
+      return FS_OK;
+    }
+  if (s->graphs.size() >= 32) destroy_graphs(s);  // a drifting prediction: start afresh
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_gated_body(s, key);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc ? rc : fail(FS_E_DEVICE, "capturing the step failed: %s", hipGetErrorString(e));
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(FS_E_DEVICE, "instantiating the step graph failed: %s", hipGetErrorString(e));
+  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, s->cur, exec});
+  s->graph_captures++;
+  HIPCHK(hipGraphLaunch(exec, s->stream));
+  s->graph_launches++;
+  return FS_OK;
+}
+
+// One step whose projection is checked on the device: the predicted V-cycles plus a
+// spare are enqueued, the host reads the step's record once its event has completed.
+int step_gated(fs_solver_t* s, bool use_cfl, double value) {
+  int rc = consume_records(s, false);
+  while (rc == FS_OK && (int)s->inflight.size() >= MAX_AHEAD) {
+    // bound how far the host runs ahead of the device (and how stale the prediction gets)
+    HIPCHK(hipEventSynchronize(s->inflight.front().ev));
+    rc = consume_records(s, false);
+  }
+  if (rc) return rc;
+  const int maxc = std::max(0, s->cfg.pressure_iterations);
+  const bool learned = learned_guess(s);
+  if (learned && !s->guess_net.loaded) return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
+  const int pred = std::max(0, std::min(s->last_pred, maxc));
+  BodyKey key{};
+  key.k = std::min(pred + GATE_SPARE, maxc);
+  key.a0 = s->pidx;
+  key.zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
+  key.cur = s->cur;
+  const int a0e = learned ? 1 - (s->pidx & 1) : s->pidx;
+  key.keep = (a0e + pred + 1) & 1;
+  if ((rc = enqueue_head(s, use_cfl, value, key.k + 1))) return rc;
+  if (s->use_graph && !s->solid_on) rc = launch_gated_body_graph(s, key);
+  else rc = enqueue_gated_body(s, key);
+  if (rc) return rc;
+  hipEvent_t ev = nullptr;
+  if (!s->ev_pool.empty()) {
+    ev = s->ev_pool.back();
+    s->ev_pool.pop_back();
+  } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    return fail(FS_E_DEVICE, "hipEventCreate failed");
+  }
+  HIPCHK(hipEventRecord(ev, s->stream));
+  s->inflight.push_back({ev, s->steps});
+  s->pidx = key.keep;
+  s->have_p = true;
+  s->steps++;
+  s->gated_steps++;
+  return FS_OK;
 }
 
 
@@ -1334,20 +1603,32 @@ int alloc_rpart(fs_solver_t* s, int iterations) {
     dfree(d);
     return rc;
   }
+  int* flags = nullptr;
+  if ((rc = dalloc(&flags, (size_t)rows))) {
+    dfree(d);
+    hfree(h);
+    return rc;
+  }
   dfree(s->rpart);
   hfree(s->rpart_host);
+  dfree(s->gate_flag);
   s->rpart = d;
   s->rpart_host = h;
   s->rp_rows = rows;
   s->rp_cols = cols;
+  s->gate_flag = flags;
+  s->gate_len = rows;
   return FS_OK;
 }
+
+void destroy_graphs(fs_solver_t* s);
 
 int ensure_rpart(fs_solver_t* s, int iterations) {
   if (s->cfg.pressure_solver != FS_P_MULTIGRID || iterations + 2 <= s->rp_rows) return FS_OK;
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = sync_stream(s))) return rc;
+  destroy_graphs(s);  // captured bodies hold the old row and flag pointers
   return alloc_rpart(s, iterations);
 }
 
@@ -1357,6 +1638,12 @@ void release(fs_solver_t* s) {
   if (!s) return;
   (void)hipSetDevice(s->dev);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  destroy_graphs(s);
+  for (auto& f : s->inflight) (void)hipEventDestroy(f.ev);
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  dfree(s->gate_flag);
+  dfree(s->modes_dev);
+  hfree(s->ring);
   release_guess(s);
   dfree(s->spec_Z);
   dfree(s->spec_part);
@@ -2151,6 +2438,8 @@ void fs_config_default(fs_config* c) {
   c->adv_fused = 1;
   c->device = 0;
   c->mg_fused = 1;
+  c->gated = 1;
+  c->graph = 1;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -2199,6 +2488,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   s->n = (size_t)c.nx * c.ny;
   s->dx = c.lx / c.nx;
   s->dy = c.ly / c.ny;
+  s->gated = c.gated != 0;
+  s->use_graph = c.graph != 0;
   auto bail = [&](int code) {
     std::string msg = g_err;
     release(s);
@@ -2230,6 +2521,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   s->mp_count = grid_blocks(s->nx, s->ny);
   if (!rc) rc = dalloc(&s->mpart, (size_t)2 * s->mp_count);
   if (!rc) rc = dalloc(&s->coef, 1);
+  if (!rc) rc = dalloc(&s->modes_dev, 1);
+  if (!rc) rc = halloc_coherent(&s->ring, REC_RING);
   if (!rc) rc = halloc(&s->dt_ring, DT_RING);
   if (!rc) rc = dalloc(&s->diag_part, (size_t)grid_blocks(s->nx, s->ny) * fsk::DIAG_W);
   if (!rc) rc = dalloc(&s->diag_out, 8);
@@ -2275,6 +2568,7 @@ int fs_upload_velocity_f32(void* h, const float* u, const float* v, size_t n) {
   if ((rc = sync_stream(s))) return rc;
   s->have_maxabs = false;
   s->have_p = false;
+  s->pred_known = false;
   return FS_OK;
 }
 
@@ -2359,7 +2653,9 @@ int fs_step(void* h, double dt) {
   fs_solver_t* s = (fs_solver_t*)h;
   if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0, got %g", dt);
   if ((rc = set_device(s))) return rc;
-  if ((rc = enqueue_coef_fixed(s, dt))) return rc;
+  if (gated_eligible(s)) return step_gated(s, false, dt);
+  if ((rc = consume_records(s, true))) return rc;
+  if ((rc = enqueue_head(s, false, dt, 0))) return rc;
   return do_step(s);
 }
 
@@ -2369,7 +2665,9 @@ int fs_step_cfl(void* h, double cfl) {
   fs_solver_t* s = (fs_solver_t*)h;
   if (!(cfl > 0) || !std::isfinite(cfl)) return fail(FS_E_INVALID_ARG, "cfl must be > 0");
   if ((rc = set_device(s))) return rc;
-  if ((rc = enqueue_coef_cfl(s, cfl))) return rc;
+  if (gated_eligible(s)) return step_gated(s, true, cfl);
+  if ((rc = consume_records(s, true))) return rc;
+  if ((rc = enqueue_head(s, true, cfl, 0))) return rc;
   return do_step(s);
 }
 
@@ -2461,7 +2759,10 @@ int fs_set_tolerance(void* h, double tol) {
   if (tol < 0 || !std::isfinite(tol)) return fail(FS_E_INVALID_ARG, "tolerance must be >= 0");
   if (tol > 0 && s->cfg.pressure_solver != FS_P_MULTIGRID)
     return fail(FS_E_INVALID_ARG, "a residual tolerance needs the multigrid solver");
+  if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+  destroy_graphs(s);
   s->cfg.tolerance = tol;
+  s->pred_known = false;
   return FS_OK;
 }
 
@@ -2469,7 +2770,12 @@ int fs_set_nu(void* h, double nu) {
   int rc = check_solver(h);
   if (rc) return rc;
   if (!(nu >= 0) || !std::isfinite(nu)) return fail(FS_E_INVALID_ARG, "nu must be >= 0");
-  ((fs_solver_t*)h)->cfg.nu = nu;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if ((nu > 0) != (s->cfg.nu > 0)) {  // captured bodies hold the diffusion on/off switch
+    if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+    destroy_graphs(s);
+  }
+  s->cfg.nu = nu;
   return FS_OK;
 }
 
@@ -2489,6 +2795,11 @@ int fs_set_guess(void* h, int guess) {
   if (guess == FS_GUESS_LEARNED_QUARTER && s->nx % 512 != 0)
     return fail(FS_E_UNSUPPORTED,
                 "the quarter-resolution learned guess needs a grid size that is a multiple of 512, got %d", s->nx);
+  if (guess != s->cfg.guess) {
+    if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+    destroy_graphs(s);
+    s->pred_known = false;
+  }
   s->cfg.guess = guess;
   return FS_OK;
 }
@@ -2589,12 +2900,23 @@ int fs_set_forcing(void* h, const fs_forcing* f) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if (!f) {
+    if (s->forcing_on) {  // captured bodies hold the forcing launches
+      if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+      destroy_graphs(s);
+    }
     s->forcing_on = false;
     return FS_OK;
   }
   if (!std::isfinite(f->kf) || !std::isfinite(f->width) || !std::isfinite(f->amp) || !(f->drag >= 0) ||
       !std::isfinite(f->drag))
     return fail(FS_E_INVALID_ARG, "forcing parameters must be finite, drag >= 0");
+  if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+  if (!s->ftab_col) {
+    if ((rc = dalloc(&s->ftab_col, (size_t)fsk::MAX_MODES * s->nx)) ||
+        (rc = dalloc(&s->ftab_row, (size_t)fsk::MAX_MODES * s->ny)))
+      return rc;
+  }
+  if (!s->forcing_on) destroy_graphs(s);
   s->forcing = *f;
   s->forcing_on = true;
   return FS_OK;
@@ -2643,7 +2965,7 @@ int fs_get_stats(void* h, fs_stats* out) {
   if (rc) return rc;
   if (!out) return fail(FS_E_INVALID_ARG, "out is NULL");
   fs_solver_t* s = (fs_solver_t*)h;
-  if (s->dt_pending > 0 || s->resid_row_pending >= 0) {
+  if (s->dt_pending > 0 || s->resid_row_pending >= 0 || !s->inflight.empty()) {
     if ((rc = set_device(s))) return rc;
     if ((rc = sync_stream(s))) return rc;
   }
@@ -2653,6 +2975,11 @@ int fs_get_stats(void* h, fs_stats* out) {
   out->total_vcycles = s->total_vcycles;
   out->last_residual = s->last_residual;
   out->last_dt = s->last_dt;
+  out->gated_steps = s->gated_steps;
+  out->gate_misses = s->gate_misses;
+  out->graph_captures = s->graph_captures;
+  out->graph_launches = s->graph_launches;
+  out->predicted_vcycles = s->pred_known ? s->last_pred : -1;
   return FS_OK;
 }
 
@@ -2684,7 +3011,7 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   if ((rc = set_device(s))) return rc;
   if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE) {
     if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0");
-    if ((rc = enqueue_coef_fixed(s, dt))) return rc;
+    if ((rc = enqueue_head(s, false, dt, 0))) return rc;
   }
   switch (kernel) {
     case FS_K_ADVECT_SL: {
@@ -2755,6 +3082,7 @@ int fs_profile_enable(void* h, unsigned mask) {
   if (mask >> PK_COUNT) return fail(FS_E_INVALID_ARG, "unknown profile kinds in mask 0x%x", mask);
   if ((rc = set_device(s))) return rc;
   if ((rc = sync_stream(s))) return rc;
+  destroy_graphs(s);  // a profiled pass launches one by one; later captures start afresh
   s->prof.mask = mask;
   for (int k = 0; k < PK_COUNT; k++) {
     s->prof.ms[k] = 0;

@@ -9,6 +9,8 @@ import os
 
 from . import build as _build
 
+ABI_VERSION = 2  # FS_ABI_VERSION of include/fluidsim.h these structures mirror
+
 FS_OK = 0
 FS_E_INVALID_ARG = -1
 FS_E_DEVICE = -2
@@ -47,6 +49,8 @@ class FsConfig(C.Structure):
         ("mg_fused", C.c_int),
         ("autotune", C.c_int),
         ("adv_fused", C.c_int),
+        ("gated", C.c_int),
+        ("graph", C.c_int),
     ]
 
 
@@ -58,6 +62,11 @@ class FsStats(C.Structure):
         ("total_vcycles", C.c_int64),
         ("last_residual", C.c_double),
         ("last_dt", C.c_double),
+        ("gated_steps", C.c_int64),
+        ("gate_misses", C.c_int64),
+        ("graph_captures", C.c_int64),
+        ("graph_launches", C.c_int64),
+        ("predicted_vcycles", C.c_int),
     ]
 
 
@@ -198,6 +207,9 @@ def load():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
+    if L.fs_abi_version() != ABI_VERSION:
+        raise ImportError(f"{path} has ABI version {L.fs_abi_version()}, this binding expects {ABI_VERSION}; "
+                          "rebuild it")
     _lib = L
     return L
 

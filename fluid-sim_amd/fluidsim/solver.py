@@ -47,7 +47,9 @@ class GPUFluidSolver:
     advection="maccormack", pressureSolver="jacobi", pressureIterations=40,
     multigrid V(2,2) with 40 coarse sweeps and minSize 8. Extra options:
     tolerance (FP64 residual gate for multigrid; 0 = fixed V-cycle count),
-    mgMinSize, mgPrecision, guess ("zero" | "previous").
+    mgMinSize, mgPrecision, guess ("zero" | "previous"), gated (the tolerance
+    checked on the device, no host wait inside a step) and graph (gated steps
+    replayed as captured hipGraphs).
     """
 
     def __init__(self, device=0, *, n=256, nx=None, ny=None, lx=TWO_PI, ly=TWO_PI, nu=0.0,
@@ -55,7 +57,7 @@ class GPUFluidSolver:
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
                  mgPrecision="fp32", guess="zero", mgFused=True, autotune=True,
-                 advFused=True):
+                 advFused=True, gated=True, graph=True):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -84,6 +86,8 @@ class GPUFluidSolver:
         cfg.mg_fused = 1 if mgFused else 0
         cfg.autotune = 1 if autotune else 0
         cfg.adv_fused = 1 if advFused else 0
+        cfg.gated = 1 if gated else 0
+        cfg.graph = 1 if graph else 0
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))
@@ -190,18 +194,21 @@ class GPUFluidSolver:
 
     # ------------------------------------------------------------------ step
     def step(self, dt, splats=()):
-        """step(dt) (src/gpu/solver.js:518-591). Raises NotConverged if a
-        tolerance was set and not met within pressureIterations V-cycles."""
+        """step(dt) (src/gpu/solver.js:518-591): enqueued, returns before the device
+        finishes, as the reference's does. With a tolerance the projection is checked
+        on the device (stats()["gate_misses"] counts steps that ended above it); a
+        step checked on the host (the first after an upload) raises NotConverged if
+        the tolerance was not met within pressureIterations V-cycles."""
         if splats:
             raise ValueError("splats are a display-only interaction and are not built here")
         self._push_solid()
         check(self._L.fs_step(self.handle, float(dt)))
 
     def stepCFL(self, cfl=0.5):
-        """One step with dt = cfl / (max|u|/dx + max|v|/dy), computed on the device."""
+        """One step with dt = cfl / (max|u|/dx + max|v|/dy), computed on the device.
+        Enqueued like step(); the dt it took is stats()["last_dt"] afterwards."""
         self._push_solid()
         check(self._L.fs_step_cfl(self.handle, float(cfl)))
-        return self.stats()["last_dt"]
 
     # ----------------------------------------------------------- immersed disk
     def setSolid(self, disk, passes=1):

@@ -383,6 +383,9 @@ static napi_value js_stats(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "totalVCycles", make_number(env, (double)st.total_vcycles));
   napi_set_named_property(env, o, "lastResidual", make_number(env, st.last_residual));
   napi_set_named_property(env, o, "lastDt", make_number(env, st.last_dt));
+  napi_set_named_property(env, o, "gatedSteps", make_number(env, (double)st.gated_steps));
+  napi_set_named_property(env, o, "gateMisses", make_number(env, (double)st.gate_misses));
+  napi_set_named_property(env, o, "graphLaunches", make_number(env, (double)st.graph_launches));
   return o;
 }
 

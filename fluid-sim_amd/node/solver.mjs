@@ -106,7 +106,7 @@ export class GPUFluidSolver {
     addon.step(this._h, dt);
   }
 
-  stepCFL(cfl = 0.5) { this._pushSolid(); addon.stepCFL(this._h, cfl); return addon.stats(this._h).lastDt; }
+  stepCFL(cfl = 0.5) { this._pushSolid(); addon.stepCFL(this._h, cfl); }
 
   // setSolid(disk | null) (src/gpu/solver.js:425-440): a RigidDisk from
   // src/cpu/solid.js or any {x, y, r, vx, vy, omega}; its state is sent before

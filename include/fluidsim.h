@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 1
+#define FS_ABI_VERSION 2
 
 /* status codes */
 enum {
@@ -102,6 +102,14 @@ typedef struct fs_config {
   int adv_fused;              /* 1 (default): MacCormack advection and diffusion in one tiled
                                  launch; 0: three launches (forward, correction, diffusion).
                                  Same bits. */
+  int gated;                  /* 1 (default): with a tolerance on the fused multigrid, the
+                                 convergence check runs on the device and steps never wait for
+                                 the host (see fs_stats); 0: every projection reads its residuals
+                                 back and adds V-cycles until the gate is met. Same bits when
+                                 both run the same V-cycles. */
+  int graph;                  /* 1 (default): a gated step's launches after its first replay as
+                                 a captured hipGraph (one per V-cycle count, buffer parity and
+                                 initial-pressure kind); 0: launched one by one. Same bits. */
 } fs_config;
 
 /* per-step statistics (new: the reference reports none) */
@@ -112,6 +120,17 @@ typedef struct fs_stats {
   int64_t total_vcycles;      /* V-cycles since creation */
   double last_residual;       /* FP64 max|div u* - L p| at the end of the last projection */
   double last_dt;             /* dt of the last step */
+  /* Device-gated projection: with a tolerance on the fused multigrid, a step enqueues
+   * the V-cycle count of the last completed step plus one spare, each V-cycle skipped
+   * on the device once the FP64 residual gate is met, so no step waits for the host.
+   * A step that needed more than it enqueued ends above the tolerance and is counted in
+   * gate_misses (the next steps enqueue more); the first step after an upload or a
+   * change of guess or tolerance is checked on the host and sets the prediction. */
+  int64_t gated_steps;        /* steps projected that way */
+  int64_t gate_misses;        /* of them, steps whose final residual is above the tolerance */
+  int64_t graph_captures;     /* step bodies captured as hipGraphs */
+  int64_t graph_launches;     /* steps whose body was launched as a graph */
+  int predicted_vcycles;      /* V-cycles the next gated step expects (-1: none yet) */
 } fs_stats;
 
 /* FP64 diagnostics of the current velocity (src/core/grid.js:290-319) */

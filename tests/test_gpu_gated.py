@@ -1,0 +1,145 @@
+"""The device-gated projection and the captured step graph against the host-checked
+projection: the same V-cycles, the same bits, graphs replayed with eager work and
+synchronisations between the replays, and the gate misses counted."""
+import os
+
+import numpy as np
+import pytest
+
+import fluidsim
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn")
+
+
+def make(n, gated, graph, guess="zero", forcing=None, prec="fp32", iters=40, tol=1e-6):
+    learned = guess.startswith("learned")
+    g = fluidsim.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                                pressureIterations=iters, tolerance=tol, mgMinSize=4,
+                                guess="previous" if learned else guess, mgPrecision=prec, gated=gated, graph=graph)
+    if learned:
+        g.loadGuess(WEIGHTS)
+        g.setGuess(guess)
+    if forcing:
+        g.setForcing(forcing)
+    return g
+
+
+FORCED = dict(kf=8.0, width=1.0, amp=0.1, drag=0.02, seed=1000)
+
+
+@pytest.mark.parametrize("n,guess,forcing,prec", [
+    (256, "zero", None, "fp32"),
+    (256, "previous", FORCED, "fp32"),
+    (256, "previous", FORCED, "bf16"),
+    (256, "learned", None, "fp32"),
+    (512, "learned-coarse", FORCED, "bf16"),
+    (1024, "zero", None, "fp32"),
+])
+def test_gated_steps_equal_host_checked_steps(n, guess, forcing, prec):
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=2)
+    steps, checks = 10, (1, 2, 4, 7, 10)
+    runs = []
+    for gated, graph in ((False, False), (True, False), (True, True)):
+        print(f"gated n={n} guess={guess} forcing={bool(forcing)} {prec} gated={gated} graph={graph}", flush=True)
+        g = make(n, gated, graph, guess, forcing, prec)
+        g.uploadVelocity(u0, v0)
+        seen = []
+        for k in range(1, steps + 1):
+            g.stepCFL(0.5)
+            if k in checks:  # eager work and a synchronisation between graph replays
+                st = g.stats()
+                vel = g.readVelocity()
+                seen.append((vel["u"], vel["v"], g.readField("pressure"), st["last_vcycles"], st["last_residual"],
+                             st["last_dt"]))
+                assert st["last_residual"] <= 1e-6
+        st = g.stats()
+        runs.append((seen, st))
+        g.destroy()
+    (ref, sr), (eager, se), (graph, sg) = runs
+    assert sr["gated_steps"] == 0
+    assert se["gated_steps"] == steps - 1 and se["graph_launches"] == 0 and se["gate_misses"] == 0
+    assert sg["gated_steps"] == steps - 1 and sg["graph_launches"] == steps - 1 and sg["gate_misses"] == 0
+    assert 1 <= sg["graph_captures"] <= steps - 1
+    for other in (eager, graph):
+        for a, b in zip(ref, other):
+            assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+            assert (a[2] == b[2]).all()
+            assert a[3:] == b[3:]
+    assert sr["total_vcycles"] == se["total_vcycles"] == sg["total_vcycles"]
+
+
+def test_gate_miss_is_counted_and_the_prediction_catches_up():
+    from fluidsim import initial
+    n = 256
+    u0, v0 = initial.decaying_turbulence(n, seed=4)
+    g = make(n, True, True, "previous")
+    g.uploadVelocity(u0, v0)
+    for _ in range(4):
+        g.stepCFL(0.5)
+    st = g.stats()
+    assert st["gate_misses"] == 0 and st["last_residual"] <= 1e-6
+    # a time step 16 times longer: the divergence advection leaves grows with dt and the
+    # previous pressure is far from the new one, so the step needs more V-cycles than the
+    # prediction plus the spare; every step either meets the gate or is counted as a miss
+    # that ran exactly what it enqueued, and the prediction then catches up
+    misses, met_after_miss = 0, False
+    for _ in range(10):
+        pred = g.stats()["predicted_vcycles"]
+        g.stepCFL(8.0)
+        st = g.stats()
+        if st["gate_misses"] > misses:
+            misses = st["gate_misses"]
+            assert st["gate_misses"] == misses
+            assert st["last_vcycles"] == pred + 1 and st["last_residual"] > 1e-6
+            assert st["predicted_vcycles"] == pred + 2
+        else:
+            assert st["last_residual"] <= 1e-6 and st["last_vcycles"] <= pred + 1
+            met_after_miss = met_after_miss or misses > 0
+    assert misses >= 1 and met_after_miss
+    g.destroy()
+
+
+def test_vcycle_cap_raises_on_the_host_checked_step_and_counts_gated_misses():
+    from fluidsim import initial
+    n = 256
+    u0, v0 = initial.decaying_turbulence(n, seed=5)
+    g = make(n, True, True, "zero", iters=2, tol=1e-9)
+    g.uploadVelocity(u0, v0)
+    with pytest.raises(fluidsim.NotConverged):
+        g.stepCFL(0.5)
+    for _ in range(3):
+        g.stepCFL(0.5)
+    st = g.stats()
+    assert st["gated_steps"] == 3 and st["gate_misses"] == 3 and st["last_vcycles"] == 2
+    assert st["last_residual"] > 1e-9
+    g.destroy()
+
+
+def test_setters_invalidate_captured_bodies():
+    from fluidsim import initial
+    n = 256
+    u0, v0 = initial.decaying_turbulence(n, seed=6)
+    outs = []
+    for graph in (False, True):
+        g = make(n, True, graph, "previous")
+        g.uploadVelocity(u0, v0)
+        for k in range(12):
+            if k == 3:
+                g.setForcing(FORCED)
+            if k == 6:
+                g.nu = 0.0
+            if k == 8:
+                g.setTolerance(1e-7)
+            if k == 10:
+                g.setForcing(None)
+            g.stepCFL(0.5)
+        outs.append((g.readVelocity(), g.readField("pressure"), g.stats()))
+        g.destroy()
+    (a, pa, sa), (b, pb, sb) = outs
+    assert (a["u"] == b["u"]).all() and (a["v"] == b["v"]).all() and (pa == pb).all()
+    assert sa["total_vcycles"] == sb["total_vcycles"] and sb["graph_launches"] > 0
+    assert sa["gate_misses"] == sb["gate_misses"] == 0
