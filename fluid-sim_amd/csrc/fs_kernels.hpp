@@ -978,17 +978,21 @@ k_step_head(const double* __restrict__ part, int nparts, int use_cfl, double val
 }
 
 // Per-block FP64 diagnostics of the stored FP32 field (src/core/grid.js:290-319):
-// part[b*8 + {0..7}] = {sum KE_c, max|div|, sum div^2, sum u, sum v, max|u|, max|v|, nonfinite}.
-constexpr int DIAG_W = 8;
+// part[b*DIAG_W + q] = {sum KE_c, max|div|, sum div^2, sum u, sum v, max|u|, max|v|,
+// nonfinite, sum omega_c^2}, omega_c the centre average of the four corner vorticities
+// (vorticityCorner / vorticityCenter, src/core/grid.js:228-257).
+constexpr int DIAG_W = 9;
+__device__ __forceinline__ bool diag_is_max(int q) { return q == 1 || q == 5 || q == 6; }
 __global__ void __launch_bounds__(BX * BY)
 k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double idx, double idy,
        double* __restrict__ part) {
   int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-  double ke = 0, md = 0, d2 = 0, su = 0, sv = 0, mu = 0, mv = 0, nf = 0;
+  double ke = 0, md = 0, d2 = 0, su = 0, sv = 0, mu = 0, mv = 0, nf = 0, en = 0;
   if (i < nx && j < ny) {
     size_t k = (size_t)j * nx + i;
+    const int ip = nxt(i, nx), im = prv(i, nx), jp = nxt(j, ny), jm = prv(j, ny);
     double uc = u[k], vc = v[k];
-    double ur = u[(size_t)j * nx + nxt(i, nx)], vt = v[(size_t)nxt(j, ny) * nx + i];
+    double ur = u[(size_t)j * nx + ip], vt = v[(size_t)jp * nx + i];
     double cu = 0.5 * (uc + ur), cv = 0.5 * (vc + vt);
     ke = cu * cu + cv * cv;
     double dv = (ur - uc) * idx + (vt - vc) * idy;
@@ -1000,22 +1004,30 @@ k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
     mv = nan_to_inf(fabs(vc));
     nf = (isfinite(uc) ? 0 : 1) + (isfinite(vc) ? 0 : 1);
     if (md != md) md = __longlong_as_double(0x7ff0000000000000LL);
+    // corner vorticity w(a, b) = (v(a,b) - v(a-1,b)) / dx - (u(a,b) - u(a,b-1)) / dy at the
+    // corners (i,j), (i+1,j), (i,j+1), (i+1,j+1)
+    auto U = [&](int a, int b) { return (double)u[(size_t)b * nx + a]; };
+    auto V = [&](int a, int b) { return (double)v[(size_t)b * nx + a]; };
+    const double w00 = (V(i, j) - V(im, j)) * idx - (U(i, j) - U(i, jm)) * idy;
+    const double w10 = (V(ip, j) - V(i, j)) * idx - (U(ip, j) - U(ip, jm)) * idy;
+    const double w01 = (V(i, jp) - V(im, jp)) * idx - (U(i, jp) - U(i, j)) * idy;
+    const double w11 = (V(ip, jp) - V(i, jp)) * idx - (U(ip, jp) - U(ip, j)) * idy;
+    const double wc = 0.25 * (w00 + w10 + w01 + w11);
+    en = wc * wc;
   }
-  __shared__ double sm[8][4];
-  double vals[8] = {ke, md, d2, su, sv, mu, mv, nf};
+  __shared__ double sm[DIAG_W][4];
+  double vals[DIAG_W] = {ke, md, d2, su, sv, mu, mv, nf, en};
   int lane = threadIdx.x + threadIdx.y * BX;
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    bool isMax = (q == 1 || q == 5 || q == 6);
-    double x = isMax ? wave_max(vals[q]) : wave_sum(vals[q]);
+  for (int q = 0; q < DIAG_W; q++) {
+    double x = diag_is_max(q) ? wave_max(vals[q]) : wave_sum(vals[q]);
     if ((lane & 63) == 0) sm[q][lane >> 6] = x;
   }
   __syncthreads();
-  if (lane < 8) {
+  if (lane < DIAG_W) {
     int q = lane;
-    bool isMax = (q == 1 || q == 5 || q == 6);
     double x = sm[q][0];
-    for (int w = 1; w < 4; w++) x = isMax ? fmax(x, sm[q][w]) : x + sm[q][w];
+    for (int w = 1; w < 4; w++) x = diag_is_max(q) ? fmax(x, sm[q][w]) : x + sm[q][w];
     part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * DIAG_W + q] = x;
   }
 }
@@ -1023,22 +1035,20 @@ k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
 // Deterministic fold of the per-block partials (one workgroup, fixed order).
 __global__ void __launch_bounds__(256)
 k_diag_final(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
-  __shared__ double sm[8][256];
+  __shared__ double sm[DIAG_W][256];
   int t = threadIdx.x;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[DIAG_W] = {};
   for (int b = t; b < nblocks; b += 256)
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < DIAG_W; q++) {
       double x = part[(size_t)b * DIAG_W + q];
-      bool isMax = (q == 1 || q == 5 || q == 6);
-      acc[q] = isMax ? fmax(acc[q], x) : acc[q] + x;
+      acc[q] = diag_is_max(q) ? fmax(acc[q], x) : acc[q] + x;
     }
-  for (int q = 0; q < 8; q++) sm[q][t] = acc[q];
+  for (int q = 0; q < DIAG_W; q++) sm[q][t] = acc[q];
   __syncthreads();
-  if (t < 8) {
+  if (t < DIAG_W) {
     int q = t;
-    bool isMax = (q == 1 || q == 5 || q == 6);
     double x = sm[q][0];
-    for (int k = 1; k < 256; k++) x = isMax ? fmax(x, sm[q][k]) : x + sm[q][k];
+    for (int k = 1; k < 256; k++) x = diag_is_max(q) ? fmax(x, sm[q][k]) : x + sm[q][k];
     out[q] = x;
   }
 }

@@ -1685,17 +1685,17 @@ void release(fs_solver_t* s) {
   delete s;
 }
 
-int run_diag(fs_solver_t* s, double out[8]) {
+int run_diag(fs_solver_t* s, double out[fsk::DIAG_W]) {
   int c = s->cur;
   dim3 g = grid2(s->nx, s->ny);
   fsk::k_diag<<<g, BLK, 0, s->stream>>>(s->u[c], s->v[c], s->nx, s->ny, 1.0 / s->dx, 1.0 / s->dy, s->diag_part);
   LAUNCHCHK();
   fsk::k_diag_final<<<1, 256, 0, s->stream>>>(s->diag_part, (int)(g.x * g.y), s->diag_out);
   LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(s->host, s->diag_out, sizeof(double) * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(s->host, s->diag_out, sizeof(double) * fsk::DIAG_W, hipMemcpyDeviceToHost, s->stream));
   int rc = sync_stream(s);
   if (rc) return rc;
-  for (int k = 0; k < 8; k++) out[k] = s->host[k];
+  for (int k = 0; k < fsk::DIAG_W; k++) out[k] = s->host[k];
   return FS_OK;
 }
 
@@ -2525,7 +2525,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (!rc) rc = halloc_coherent(&s->ring, REC_RING);
   if (!rc) rc = halloc(&s->dt_ring, DT_RING);
   if (!rc) rc = dalloc(&s->diag_part, (size_t)grid_blocks(s->nx, s->ny) * fsk::DIAG_W);
-  if (!rc) rc = dalloc(&s->diag_out, 8);
+  if (!rc) rc = dalloc(&s->diag_out, fsk::DIAG_W);
   if (!rc) rc = dalloc(&s->solid_part, (size_t)grid_blocks(s->nx, s->ny) * 3);
   if (!rc) rc = dalloc(&s->solid_imp, 3);
   if (!rc) rc = halloc(&s->host, 64);
@@ -2686,7 +2686,7 @@ int fs_max_divergence(void* h, double* out) {
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
   if ((rc = enqueue_divergence(s))) return rc;  // the reference also refreshes its div texture
-  double d[8];
+  double d[fsk::DIAG_W];
   if ((rc = run_diag(s, d))) return rc;
   *out = d[1];
   return FS_OK;
@@ -2989,7 +2989,7 @@ int fs_diagnostics(void* h, fs_diag* out) {
   if (!out) return fail(FS_E_INVALID_ARG, "out is NULL");
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
-  double d[8];
+  double d[fsk::DIAG_W];
   if ((rc = run_diag(s, d))) return rc;
   double N = (double)s->n, dA = s->dx * s->dy;
   out->kinetic_energy = 0.5 * d[0] / N;
@@ -3000,6 +3000,7 @@ int fs_diagnostics(void* h, fs_diag* out) {
   out->max_abs_u = d[5];
   out->max_abs_v = d[6];
   out->nonfinite = (int)d[7];
+  out->enstrophy = 0.5 * d[8] / N;
   if (out->nonfinite) return fail(FS_E_NONFINITE, "%d non-finite velocity faces", out->nonfinite);
   return FS_OK;
 }

@@ -78,6 +78,7 @@ class FsDiag(C.Structure):
         ("momentum_x", C.c_double), ("momentum_y", C.c_double),
         ("max_abs_u", C.c_double), ("max_abs_v", C.c_double),
         ("nonfinite", C.c_int),
+        ("enstrophy", C.c_double),
     ]
 
 

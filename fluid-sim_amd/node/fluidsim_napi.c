@@ -404,6 +404,7 @@ static napi_value js_diag(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "l2Divergence", make_number(env, d.l2_divergence));
   napi_set_named_property(env, o, "momentumX", make_number(env, d.momentum_x));
   napi_set_named_property(env, o, "momentumY", make_number(env, d.momentum_y));
+  napi_set_named_property(env, o, "enstrophy", make_number(env, d.enstrophy));
   return o;
 }
 

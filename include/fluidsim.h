@@ -141,6 +141,8 @@ typedef struct fs_diag {
   double momentum_x, momentum_y; /* sum u dA, sum v dA (src/cpu/solid.js:264-270) */
   double max_abs_u, max_abs_v;
   int nonfinite;              /* count of non-finite faces */
+  double enstrophy;           /* 1/2 <omega_c^2>, omega_c the centre average of the corner
+                                 vorticities (grid.js:228-257, :297-303) */
 } fs_diag;
 
 /* band forcing (src/cpu/solver.js:179-237 semantics: force, then drag) */

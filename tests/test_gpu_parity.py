@@ -95,6 +95,21 @@ def test_max_divergence_reduction(fs, O, golden):
     assert abs(md - golden["projection"]["red-black"]["before"]) / md < 2e-7
     assert abs(d["kinetic_energy"] - O.kinetic_energy(N, N, f32(u0), f32(v0))) < 1e-12
     assert abs(d["momentum_x"] - f32(u0).sum() * (TWO_PI / N) ** 2) < 1e-9
+    ens = O.enstrophy(N, N, f32(u0), f32(v0))
+    assert abs(d["enstrophy"] - ens) <= 1e-12 * ens
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_enstrophy_reduction_matches_oracle(fs, O, n):
+    # src/core/grid.js:297-303 on the device in FP64: Taylor-Green (its closed form is
+    # pinned on the oracle) and decaying turbulence
+    from fluidsim import initial
+    for u0, v0 in (O.taylor_green(n, n), initial.decaying_turbulence(n, seed=1)):
+        with solver(fs, n=n) as g:
+            g.uploadVelocity(u0, v0)
+            d = g.diagnostics()
+        ens = O.enstrophy(n, n, f32(u0), f32(v0))
+        assert abs(d["enstrophy"] - ens) <= 1e-11 * ens
 
 
 # ---------------------------------------------------- projection and solvers

@@ -359,6 +359,17 @@ def test_taylor_green_reference_gate(O):
     assert ke < 0.25 * math.exp(-4 * nu * 0.5) * 1.02
 
 
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_enstrophy_of_taylor_green_is_its_discrete_closed_form(O, n):
+    # src/core/grid.js:297-303. For u = sin x cos y, v = -cos x sin y at the faces the
+    # corner vorticity is 2 s sin x sin y (s = sin(h/2)/(h/2)), its centre average
+    # 2 s cos^2(h/2) sin x_c sin y_c, and 1/2 <omega_c^2> = s^2 cos^4(h/2) / 2, which
+    # tends to the continuum 1/2 <(2 sin x sin y)^2> = 1/2
+    h = TWO_PI / n
+    u, v = O.taylor_green(n, n)
+    s = math.sin(h / 2) / (h / 2)
+    assert abs(O.enstrophy(n, n, u, v) - 0.5 * s ** 2 * math.cos(h / 2) ** 4) < 1e-14
+
 def test_band_forcing_modes_deterministic_and_in_shell(O):
     kx, ky, ph = O.band_modes(1000, 32.0, 2.0)
     kx2, ky2, ph2 = O.band_modes(1000, 32.0, 2.0)
@@ -369,6 +380,68 @@ def test_band_forcing_modes_deterministic_and_in_shell(O):
     kx3, _, _ = O.band_modes(1001, 32.0, 2.0)
     assert not (kx3 == kx).all()
 
+
+
+def _face_coefficients(F, n, comp):
+    """Complex Fourier coefficient C(k) of a real face field sampled at its MAC positions,
+    normalised so that c cos(k.x + phi) has C(k) = c e^{i phi}: u faces sit at
+    (i h, (j + 1/2) h), v faces at ((i + 1/2) h, j h) on [0, 2pi]^2."""
+    h = TWO_PI / n
+    Z = np.fft.fft2(F.reshape(n, n)) * (2.0 / (n * n))   # Z[ky, kx] = 2/N sum F e^{-i k.(i h, j h)}
+    k = np.fft.fftfreq(n, 1.0 / n)
+    KX, KY = np.meshgrid(k, k)
+    shift = np.exp(-0.5j * h * (KY if comp == "u" else KX))  # the staggered half cell
+    return Z * shift, KX, KY
+
+
+def _forcing_pin_error(O, Fu, Fv, n, seed, kf, width, amp, dt):
+    """Largest deviation of the faces' Fourier coefficients from the band forcing's
+    closed form: c_u = amp dt / sqrt(24) ky/|k|, c_v = -amp dt / sqrt(24) kx/|k| at each
+    mode's wavevector with its phase (modes sharing a wavevector, or opposite ones, add),
+    and nothing at any other wavevector. Relative to amp dt."""
+    kx, ky, ph = O.band_modes(seed, kf, width)
+    scale = amp * dt / math.sqrt(24)
+    err = 0.0
+    for comp, F in (("u", Fu), ("v", Fv)):
+        C, KX, KY = _face_coefficients(F, n, comp)
+        want = np.zeros_like(C)
+        for a, b, p in zip(kx, ky, ph):
+            if a == 0 and b == 0:
+                continue
+            c = scale * (b if comp == "u" else -a) / math.hypot(a, b)
+            want[(KX == a) & (KY == b)] += c * np.exp(1j * p)
+            want[(KX == -a) & (KY == -b)] += c * np.exp(-1j * p)
+        err = max(err, float(np.max(np.abs(C - want))) / (amp * dt))
+    return err
+
+
+def test_band_forcing_field_is_the_closed_form_at_every_mode(O):
+    # src/cpu/solver.js:200-222: each mode adds the curl of a streamfunction, sampled at
+    # the faces; its Fourier coefficients on the grid are fixed by the mode list alone
+    n, kf, width, amp, dt = 64, 8.0, 2.0, 0.4, 0.01
+    z = np.zeros(n * n)
+    for seed in (1, 1000, 4242):
+        Fu, Fv = O.band_forcing(n, n, z, z, kf, width, amp, seed, dt)
+        assert _forcing_pin_error(O, Fu, Fv, n, seed, kf, width, amp, dt) < 1e-12
+
+
+def test_band_forcing_pin_rejects_planted_mistakes(O):
+    # the pin above fails for a flipped v sign, a half-cell face offset and 1/M scaling
+    n, kf, width, amp, dt, seed = 64, 8.0, 2.0, 0.4, 0.01, 1000
+    z = np.zeros(n * n)
+    Fu, Fv = O.band_forcing(n, n, z, z, kf, width, amp, seed, dt)
+    assert _forcing_pin_error(O, Fu, -Fv, n, seed, kf, width, amp, dt) > 1e-2
+    assert _forcing_pin_error(O, Fu * math.sqrt(24) / 24, Fv * math.sqrt(24) / 24, n, seed, kf, width, amp,
+                              dt) > 1e-2
+    # u sampled at the cell centre row y = j h instead of the face row (j + 1/2) h
+    h = TWO_PI / n
+    kx, ky, ph = O.band_modes(seed, kf, width)
+    I, J = np.meshgrid(np.arange(n), np.arange(n))
+    Gu = np.zeros((n, n))
+    for a, b, p in zip(kx, ky, ph):
+        if a or b:
+            Gu += amp * dt / math.sqrt(24) * b / math.hypot(a, b) * np.cos(a * I * h + b * J * h + p)
+    assert _forcing_pin_error(O, Gu.reshape(-1), Fv, n, seed, kf, width, amp, dt) > 1e-2
 
 # ------------------------------------------------- immersed disk (src/cpu/solid.js)
 # restating test/solid.test.js:29-160 on the oracle's coupling
@@ -660,6 +733,48 @@ def test_cavity_advection_at_rest_and_bounded(O, scheme):
     assert np.all(ou.reshape(n, n)[:, 0] == 0) and np.all(ov.reshape(n, n)[0, :] == 0)
     assert np.max(np.abs(ou)) <= 3.0 + 1e-12 and np.max(np.abs(ov)) <= 1.0 + 1e-12
 
+
+
+@pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
+def test_cavity_advection_away_from_walls_equals_the_periodic_operator(O, scheme):
+    # far from every wall a walled trace, sample and limiter read only interior faces,
+    # so the cavity operator must give the periodic one's numbers on the unit box
+    n, dt = 64, 0.01
+    h = 1.0 / n
+    i = np.arange(n)
+    X, Y = np.meshgrid(i * h, (i + 0.5) * h)       # u faces
+    Xv, Yv = np.meshgrid((i + 0.5) * h, i * h)     # v faces
+    blob = lambda x, y: np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2) / 0.01)
+    u = (0.6 + 0.3 * blob(X, Y) + 0.05 * np.sin(9 * X) * np.cos(7 * Y)).reshape(-1)
+    v = (-0.4 + 0.2 * blob(Xv, Yv) + 0.05 * np.cos(5 * Xv) * np.sin(11 * Yv)).reshape(-1)
+    u.reshape(n, n)[:, 0] = 0
+    v.reshape(n, n)[0, :] = 0
+    cu, cv = O.cav_advect(scheme, n, 1.0, u, v, dt)
+    pu, pv = O.advect_velocity(scheme, n, n, u, v, dt, lx=1.0, ly=1.0)
+    inner = np.s_[8:n - 8, 8:n - 8]   # traces move < 0.02 cells; stencils reach 2 cells
+    assert np.max(np.abs(cu.reshape(n, n)[inner] - pu.reshape(n, n)[inner])) < 1e-13
+    assert np.max(np.abs(cv.reshape(n, n)[inner] - pv.reshape(n, n)[inner])) < 1e-13
+
+
+def test_cavity_samples_take_the_wall_values_on_the_walls(O):
+    # no slip: a sample on the lid is the lid speed, on the bottom wall 0, and v on a
+    # side wall 0, whatever the interior holds (ghost 2U - u above the lid, -u below
+    # the bottom, -v beyond a side wall)
+    n, U = 16, 1.7
+    rng = np.random.default_rng(3)
+    u, v = rng.uniform(-1, 1, (2, n * n))
+    u.reshape(n, n)[:, 0] = 0
+    v.reshape(n, n)[0, :] = 0
+    for i in range(1, n):
+        x = i / n                                  # on a u face column
+        assert abs(O.cav_sample_u(n, U, u, x, 1.0) - U) < 1e-15
+        assert abs(O.cav_sample_u(n, U, u, x, 0.0)) < 1e-15
+        y = i / n                                  # on a v face row
+        assert abs(O.cav_sample_v(n, v, 0.0, y)) < 1e-15
+        assert abs(O.cav_sample_v(n, v, 1.0, y)) < 1e-15
+    # and inside, a sample on a face node is that face's value
+    assert O.cav_sample_u(n, U, u, 5 / n, 7.5 / n) == u[7 * n + 5]
+    assert O.cav_sample_v(n, v, 5.5 / n, 7 / n) == v[7 * n + 5]
 
 def test_cavity_flow_spins_up_divergence_free(O):
     s = O.Cavity(32, U=1.0, nu=1e-2, disk=dict(x=0.5, y=0.4, r=0.12), passes=2)
