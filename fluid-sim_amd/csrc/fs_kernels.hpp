@@ -983,40 +983,41 @@ k_step_head(const double* __restrict__ part, int nparts, int use_cfl, double val
 // (vorticityCorner / vorticityCenter, src/core/grid.js:228-257).
 constexpr int DIAG_W = 9;
 __device__ __forceinline__ bool diag_is_max(int q) { return q == 1 || q == 5 || q == 6; }
-__global__ void __launch_bounds__(BX * BY)
-k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double idx, double idy,
-       double* __restrict__ part) {
-  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-  double ke = 0, md = 0, d2 = 0, su = 0, sv = 0, mu = 0, mv = 0, nf = 0, en = 0;
-  if (i < nx && j < ny) {
-    size_t k = (size_t)j * nx + i;
-    const int ip = nxt(i, nx), im = prv(i, nx), jp = nxt(j, ny), jm = prv(j, ny);
-    double uc = u[k], vc = v[k];
-    double ur = u[(size_t)j * nx + ip], vt = v[(size_t)jp * nx + i];
-    double cu = 0.5 * (uc + ur), cv = 0.5 * (vc + vt);
-    ke = cu * cu + cv * cv;
-    double dv = (ur - uc) * idx + (vt - vc) * idy;
-    md = fabs(dv);
-    d2 = dv * dv;
-    su = uc;
-    sv = vc;
-    mu = nan_to_inf(fabs(uc));
-    mv = nan_to_inf(fabs(vc));
-    nf = (isfinite(uc) ? 0 : 1) + (isfinite(vc) ? 0 : 1);
-    if (md != md) md = __longlong_as_double(0x7ff0000000000000LL);
-    // corner vorticity w(a, b) = (v(a,b) - v(a-1,b)) / dx - (u(a,b) - u(a,b-1)) / dy at the
-    // corners (i,j), (i+1,j), (i,j+1), (i+1,j+1)
-    auto U = [&](int a, int b) { return (double)u[(size_t)b * nx + a]; };
-    auto V = [&](int a, int b) { return (double)v[(size_t)b * nx + a]; };
-    const double w00 = (V(i, j) - V(im, j)) * idx - (U(i, j) - U(i, jm)) * idy;
-    const double w10 = (V(ip, j) - V(i, j)) * idx - (U(ip, j) - U(ip, jm)) * idy;
-    const double w01 = (V(i, jp) - V(im, jp)) * idx - (U(i, jp) - U(i, j)) * idy;
-    const double w11 = (V(ip, jp) - V(i, jp)) * idx - (U(ip, jp) - U(ip, j)) * idy;
-    const double wc = 0.25 * (w00 + w10 + w01 + w11);
-    en = wc * wc;
-  }
+
+// The diagnostics' terms of cell (i, j).
+__device__ __forceinline__ void diag_cell(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
+                                          int i, int j, double idx, double idy, double vals[DIAG_W]) {
+  size_t k = (size_t)j * nx + i;
+  const int ip = nxt(i, nx), im = prv(i, nx), jp = nxt(j, ny), jm = prv(j, ny);
+  double uc = u[k], vc = v[k];
+  double ur = u[(size_t)j * nx + ip], vt = v[(size_t)jp * nx + i];
+  double cu = 0.5 * (uc + ur), cv = 0.5 * (vc + vt);
+  double dv = (ur - uc) * idx + (vt - vc) * idy;
+  double md = fabs(dv);
+  if (md != md) md = __longlong_as_double(0x7ff0000000000000LL);
+  // corner vorticity w(a, b) = (v(a,b) - v(a-1,b)) / dx - (u(a,b) - u(a,b-1)) / dy at the
+  // corners (i,j), (i+1,j), (i,j+1), (i+1,j+1)
+  auto U = [&](int a, int b) { return (double)u[(size_t)b * nx + a]; };
+  auto V = [&](int a, int b) { return (double)v[(size_t)b * nx + a]; };
+  const double w00 = (V(i, j) - V(im, j)) * idx - (U(i, j) - U(i, jm)) * idy;
+  const double w10 = (V(ip, j) - V(i, j)) * idx - (U(ip, j) - U(ip, jm)) * idy;
+  const double w01 = (V(i, jp) - V(im, jp)) * idx - (U(i, jp) - U(i, j)) * idy;
+  const double w11 = (V(ip, jp) - V(i, jp)) * idx - (U(ip, jp) - U(ip, j)) * idy;
+  const double wc = 0.25 * (w00 + w10 + w01 + w11);
+  vals[0] = cu * cu + cv * cv;
+  vals[1] = md;
+  vals[2] = dv * dv;
+  vals[3] = uc;
+  vals[4] = vc;
+  vals[5] = nan_to_inf(fabs(uc));
+  vals[6] = nan_to_inf(fabs(vc));
+  vals[7] = (isfinite(uc) ? 0 : 1) + (isfinite(vc) ? 0 : 1);
+  vals[8] = wc * wc;
+}
+
+// The block's fold of its threads' terms (BX x BY threads) to part[blockslot * DIAG_W + q].
+__device__ __forceinline__ void diag_block_store(double vals[DIAG_W], double* __restrict__ part, size_t blockslot) {
   __shared__ double sm[DIAG_W][4];
-  double vals[DIAG_W] = {ke, md, d2, su, sv, mu, mv, nf, en};
   int lane = threadIdx.x + threadIdx.y * BX;
 #pragma unroll
   for (int q = 0; q < DIAG_W; q++) {
@@ -1028,8 +1029,52 @@ k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
     int q = lane;
     double x = sm[q][0];
     for (int w = 1; w < 4; w++) x = diag_is_max(q) ? fmax(x, sm[q][w]) : x + sm[q][w];
-    part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * DIAG_W + q] = x;
+    part[blockslot * DIAG_W + q] = x;
   }
+}
+
+__global__ void __launch_bounds__(BX * BY)
+k_diag(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double idx, double idy,
+       double* __restrict__ part) {
+  int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  double vals[DIAG_W] = {};
+  if (i < nx && j < ny) diag_cell(u, v, nx, ny, i, j, idx, idy, vals);
+  diag_block_store(vals, part, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// The same terms with a fixed number of workgroups (gridDim.x, 1D, BX x BY threads)
+// striding over the BX x BY blocks of the grid, so the partials have a fixed size
+// whatever the grid: every workgroup writes its slot, also when it found no block.
+__global__ void __launch_bounds__(BX * BY)
+k_diag_strided(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double idx, double idy,
+               double* __restrict__ part) {
+  const int bx = (nx + BX - 1) / BX, nb = bx * ((ny + BY - 1) / BY);
+  double acc[DIAG_W] = {};
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    int i = (b % bx) * BX + threadIdx.x, j = (b / bx) * BY + threadIdx.y;
+    if (i < nx && j < ny) {
+      double vals[DIAG_W];
+      diag_cell(u, v, nx, ny, i, j, idx, idy, vals);
+#pragma unroll
+      for (int q = 0; q < DIAG_W; q++) acc[q] = diag_is_max(q) ? fmax(acc[q], vals[q]) : acc[q] + vals[q];
+    }
+  }
+  diag_block_store(acc, part, blockIdx.x);
+}
+
+// The folded sums and maxima as the diagnostics report them (one thread):
+// {KE, max|div|, l2 div, momentum x, momentum y, max|u|, max|v|, non-finite faces, enstrophy}.
+__global__ void k_diag_finish(const double* __restrict__ raw, double ncells, double dA, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  out[0] = 0.5 * raw[0] / ncells;
+  out[1] = raw[1];
+  out[2] = sqrt(raw[2] / ncells);
+  out[3] = raw[3] * dA;
+  out[4] = raw[4] * dA;
+  out[5] = raw[5];
+  out[6] = raw[6];
+  out[7] = raw[7];
+  out[8] = 0.5 * raw[8] / ncells;
 }
 
 // Deterministic fold of the per-block partials (one workgroup, fixed order).

@@ -34,6 +34,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3002,6 +3003,59 @@ int fs_diagnostics(void* h, fs_diag* out) {
   out->nonfinite = (int)d[7];
   out->enstrophy = 0.5 * d[8] / N;
   if (out->nonfinite) return fail(FS_E_NONFINITE, "%d non-finite velocity faces", out->nonfinite);
+  return FS_OK;
+}
+
+}  // extern "C"
+
+// Scratch of fs_diagnostics_noalloc: the per-workgroup partials of a fixed number of
+// striding workgroups, then their fold. Each GPU the library runs on holds its own
+// copy until the process ends; every call on that GPU shares it.
+namespace {
+constexpr int NA_GROUPS = 1024;
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+double* g_na_scratch[NA_MAX_DEVICES] = {};  // device address of `scratch` per device
+}  // namespace
+static __device__ __align__(256) double scratch[(NA_GROUPS + 1) * fsk::DIAG_W];
+
+extern "C" {
+
+int fs_diagnostics_noalloc(const float* u, const float* v, int nx, int ny, double lx, double ly, double* out,
+                           void* stream) {
+  if (!u || !v || !out) return fail(FS_E_INVALID_ARG, "u, v or out is NULL");
+  if (nx < 2 || ny < 2) return fail(FS_E_INVALID_ARG, "grid must be >= 2x2, got %dx%d", nx, ny);
+  if ((int64_t)nx * ny > (int64_t)1 << 28)
+    return fail(FS_E_INVALID_ARG, "grid %dx%d exceeds the maximum of 2^28 cells", nx, ny);
+  if (!(lx > 0) || !(ly > 0) || !std::isfinite(lx) || !std::isfinite(ly))
+    return fail(FS_E_INVALID_ARG, "domain size must be positive and finite");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(FS_E_DEVICE, "device %d beyond %d", dev, NA_MAX_DEVICES);
+  double* base = nullptr;
+  {
+    // the array's address on this device, looked up once with the device current; the
+    // array's host-side name is not a device address
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_scratch[dev]) {
+      void* p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(FS_E_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+      g_na_scratch[dev] = (double*)p;
+    }
+    base = g_na_scratch[dev];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  double* part = base;
+  double* raw = base + (size_t)NA_GROUPS * fsk::DIAG_W;
+  const double dx = lx / nx, dy = ly / ny;
+  fsk::k_diag_strided<<<NA_GROUPS, BLK, 0, st>>>(u, v, nx, ny, 1.0 / dx, 1.0 / dy, part);
+  LAUNCHCHK();
+  fsk::k_diag_final<<<1, 256, 0, st>>>(part, NA_GROUPS, raw);
+  LAUNCHCHK();
+  fsk::k_diag_finish<<<1, 64, 0, st>>>(raw, (double)nx * ny, dx * dy, out);
+  LAUNCHCHK();
   return FS_OK;
 }
 

@@ -153,6 +153,7 @@ _SIGNATURES = {
     "fs_solid_impulse": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "fs_get_stats": (C.c_int, [_VP, C.POINTER(FsStats)]),
     "fs_diagnostics": (C.c_int, [_VP, C.POINTER(FsDiag)]),
+    "fs_diagnostics_noalloc": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_double, C.c_double, _VP, _VP]),
     "fs_run_kernel": (C.c_int, [_VP, C.c_int, C.c_double]),
     "fs_profile_enable": (C.c_int, [_VP, C.c_uint]),
     "fs_tile_report": (C.c_int, [_VP, C.c_char_p, C.c_size_t]),

@@ -280,6 +280,18 @@ int fs_diagnostics(void* solver, fs_diag* out);
  * (modes per shell) may be NULL. */
 int fs_energy_spectrum(void* solver, double* E, int64_t* counts, int len);
 
+/* No counterpart in the reference: a variant of fs_diagnostics for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It reads caller-owned device fields u, v (nx*ny FP32 each, MAC order, on
+ * the current HIP device) and writes 9 doubles to the caller-owned device buffer `out`:
+ * {kinetic energy, max|div|, l2 div, momentum x, momentum y, max|u|, max|v|, non-finite
+ * faces, enstrophy}, the definitions of fs_diag. Enqueued on `stream` (a hipStream_t;
+ * NULL = the default stream), nothing is waited for. It allocates nothing: its partials
+ * live in a fixed 74 KB device array per GPU, held until the process ends and shared by
+ * every call on that GPU, so calls that overlap on one GPU race. At most 2^28 cells. */
+int fs_diagnostics_noalloc(const float* u, const float* v, int nx, int ny, double lx, double ly, double* out,
+                           void* stream);
+
 /* per-kernel hook replacing gputest's use of internals */
 int fs_run_kernel(void* solver, int kernel, double dt);
 
