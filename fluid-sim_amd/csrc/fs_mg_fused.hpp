@@ -574,6 +574,8 @@ struct LevelTable {
   int mh[CML_MAXL];     // qdiv magic for division by n / 2
   int levels;           // levels in this kernel
   int tail;             // first level run in registers (== levels: none)
+  int wave;             // first level run as register blocks in wavefront 0 (== tail: none); the
+                        // levels wave .. tail-1 are 32 x 32 then 16 x 16, or 16 x 16 alone
 };
 
 // q / d for 0 <= q < 4096 and 1 <= d <= 64 as one multiply and shift, with
@@ -640,6 +642,136 @@ __device__ __forceinline__ float reg_residual(float x, float r, int n, int i, in
   return r - lap;
 }
 
+// ---- register blocks: a level of n = 8 B cells (B = 4 or 2) in wavefront 0, lane
+// (bx, by) = (lane & 7, lane >> 3) holding the B x B cells (B bx + a, B by + b) at
+// x[b B + a]. Block origins are even, so the red-black colour of a slot is the same in
+// every lane and each neighbour a block edge needs is one uniform cross-lane shuffle.
+// The arithmetic and its order are those of the LDS passes (same bits).
+__device__ __forceinline__ int wb_lane(int bx, int by) { return ((by & 7) << 3) | (bx & 7); }
+
+template <int B>
+struct WbNbr {  // the lanes of the four neighbouring blocks (periodic over the 8 x 8 blocks)
+  int l, r, d, u;
+  __device__ __forceinline__ WbNbr(int bx, int by)
+      : l(wb_lane(bx + 7, by)), r(wb_lane(bx + 1, by)), d(wb_lane(bx, by + 7)), u(wb_lane(bx, by + 1)) {}
+};
+
+// the four neighbours of slot (a, b): left, right, below (j - 1), above (j + 1)
+template <int B>
+__device__ __forceinline__ void wb_nbrs(const float (&x)[B * B], const WbNbr<B>& nb, int a, int b, float& xl,
+                                        float& xr, float& xb, float& xt) {
+  xl = a > 0 ? x[b * B + a - 1] : __shfl(x[b * B + B - 1], nb.l, 64);
+  xr = a < B - 1 ? x[b * B + a + 1] : __shfl(x[b * B], nb.r, 64);
+  xb = b > 0 ? x[(b - 1) * B + a] : __shfl(x[(B - 1) * B + a], nb.d, 64);
+  xt = b < B - 1 ? x[(b + 1) * B + a] : __shfl(x[a], nb.u, 64);
+}
+
+// one colour pass (cml_pass / reg_colour): cells of colour C read only the other colour
+template <int B, int C, bool FZ>
+__device__ __forceinline__ void wb_pass(float (&x)[B * B], const float (&r)[B * B], const WbNbr<B>& nb, float h2) {
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int a = 0; a < B; a++) {
+      if (((a + b) & 1) != C) continue;
+      float sum = 0.f;
+      if (!FZ) {
+        float xl, xr, xb, xt;
+        wb_nbrs<B>(x, nb, a, b, xl, xr, xb, xt);
+        sum = xl + xr + xb + xt;
+      }
+      x[b * B + a] = (sum - h2 * r[b * B + a]) * 0.25f;
+    }
+}
+
+// pre (or post) red-black sweeps; from_zero: x is a zero start (its first red pass is
+// neighbour-free)
+template <int B>
+__device__ __forceinline__ void wb_sweeps(float (&x)[B * B], const float (&r)[B * B], const WbNbr<B>& nb, float h2,
+                                          int sweeps, bool from_zero) {
+  for (int s = 0; s < sweeps; s++) {
+    if (s == 0 && from_zero) wb_pass<B, 0, true>(x, r, nb, h2);
+    else wb_pass<B, 0, false>(x, r, nb, h2);
+    wb_pass<B, 1, false>(x, r, nb, h2);
+  }
+}
+
+// residual of the four children of each coarse cell, summed in order and quartered
+// (the LDS restriction); the coarse block of B/2 x B/2 cells is this lane's own
+template <int B>
+__device__ __forceinline__ void wb_restrict(const float (&x)[B * B], const float (&r)[B * B], const WbNbr<B>& nb,
+                                            float ih2, float (&rc)[(B / 2) * (B / 2)]) {
+  constexpr int BC = B / 2;
+  float res[B * B];
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int a = 0; a < B; a++) {
+      float xl, xr, xb, xt;
+      wb_nbrs<B>(x, nb, a, b, xl, xr, xb, xt);
+      float lap = (xl + xr + xb + xt - 4.f * x[b * B + a]) * ih2;
+      res[b * B + a] = r[b * B + a] - lap;
+    }
+#pragma unroll
+  for (int cb = 0; cb < BC; cb++)
+#pragma unroll
+    for (int ca = 0; ca < BC; ca++) {
+      float sum = 0.f;
+      sum += res[(2 * cb) * B + 2 * ca];
+      sum += res[(2 * cb) * B + 2 * ca + 1];
+      sum += res[(2 * cb + 1) * B + 2 * ca];
+      sum += res[(2 * cb + 1) * B + 2 * ca + 1];
+      rc[cb * BC + ca] = sum * 0.25f;
+    }
+}
+
+// x += bilinear cell-centred prolongation of the coarse blocks xc (B/2 x B/2 per lane;
+// PROLONG, src/gpu/multigrid.js:99-131): fine slot a reads coarse offsets a/2 - 1 + (a & 1)
+// and one more, relative to this lane's coarse origin, so from the lane and its eight
+// neighbours
+template <int B>
+__device__ __forceinline__ void wb_prolong(float (&x)[B * B], const float (&xc)[(B / 2) * (B / 2)], int bx, int by) {
+  constexpr int BC = B / 2, W = BC + 2;
+  float ring[W * W];  // coarse values at offsets -1 .. BC, row-major
+#pragma unroll
+  for (int oy = -1; oy <= BC; oy++)
+#pragma unroll
+    for (int ox = -1; ox <= BC; ox++) {
+      const int dx = ox < 0 ? -1 : (ox >= BC ? 1 : 0), dy = oy < 0 ? -1 : (oy >= BC ? 1 : 0);
+      const int sx = ox - dx * BC, sy = oy - dy * BC;
+      const float v = xc[sy * BC + sx];
+      ring[(oy + 1) * W + ox + 1] = (dx == 0 && dy == 0) ? v : __shfl(v, wb_lane(bx + dx + 8, by + dy + 8), 64);
+    }
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int a = 0; a < B; a++) {
+      const int ox0 = (a & 1) ? (a >> 1) : (a >> 1) - 1, oy0 = (b & 1) ? (b >> 1) : (b >> 1) - 1;
+      const float fx = (a & 1) ? 0.25f : 0.75f, fy = (b & 1) ? 0.25f : 0.75f;
+      const float va = ring[(oy0 + 1) * W + ox0 + 1], vb = ring[(oy0 + 1) * W + ox0 + 2];
+      const float vc = ring[(oy0 + 2) * W + ox0 + 1], vd = ring[(oy0 + 2) * W + ox0 + 2];
+      const float top = va * (1.f - fx) + vb * fx, bot = vc * (1.f - fx) + vd * fx;
+      x[b * B + a] = x[b * B + a] + (top * (1.f - fy) + bot * fy);
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void wb_load(float (&v)[B * B], const float* __restrict__ lds, int bx, int by) {
+  constexpr int n = 8 * B;
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int a = 0; a < B; a++) v[b * B + a] = lds[(B * by + b) * n + B * bx + a];
+}
+template <int B>
+__device__ __forceinline__ void wb_store(const float (&v)[B * B], float* __restrict__ lds, int bx, int by) {
+  constexpr int n = 8 * B;
+#pragma unroll
+  for (int b = 0; b < B; b++)
+#pragma unroll
+    for (int a = 0; a < B; a++) lds[(B * by + b) * n + B * bx + a] = v[b * B + a];
+}
+
 template <typename S, typename SO = S>
 __global__ void __launch_bounds__(CTH)
 k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable tb, int pre, int post,
@@ -679,8 +811,9 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
     }
   }
   __syncthreads();
-  // ---- LDS levels above the register tail (or above the LDS coarsest): down
-  const int ldsDown = tail < levels ? tail : last;
+  // ---- LDS levels above the register levels (or above the LDS coarsest): down
+  const int wave = tb.wave;
+  const int ldsDown = tail < levels ? wave : last;
   for (int l = 0; l < ldsDown; l++) {
     int n = tb.n[l];
     float h2 = tb.h2[l], ih2 = tb.ih2[l];
@@ -762,7 +895,30 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
     float xr[REG_MAXL], rr[REG_MAXL];
 #pragma unroll
     for (int k = 0; k < REG_MAXL; k++) xr[k] = rr[k] = 0.f;
-    {
+    // register blocks above the tail: 32 x 32 (4 x 4 per lane) and 16 x 16 (2 x 2 per lane)
+    const int bxl = lane & 7, byl = lane >> 3;
+    const int nblk = tail - wave;  // 0, 1 (16) or 2 (32, 16)
+    float x32[16], r32[16], x16[4], r16[4];
+    if (nblk == 2) {
+      const WbNbr<4> nb4(bxl, byl);
+      wb_load<4>(r32, br + tb.off[wave], bxl, byl);
+#pragma unroll
+      for (int q = 0; q < 16; q++) x32[q] = 0.f;
+      wb_sweeps<4>(x32, r32, nb4, tb.h2[wave], pre, true);
+      wb_restrict<4>(x32, r32, nb4, tb.ih2[wave], r16);
+    } else if (nblk == 1) {
+      wb_load<2>(r16, br + tb.off[wave], bxl, byl);
+    }
+    if (nblk >= 1) {
+      const WbNbr<2> nb2(bxl, byl);
+      const int l16 = tail - 1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) x16[q] = 0.f;
+      wb_sweeps<2>(x16, r16, nb2, tb.h2[l16], pre, true);
+      float rc1[1];
+      wb_restrict<2>(x16, r16, nb2, tb.ih2[l16], rc1);
+      rr[0] = rc1[0];  // lane = J * 8 + I of the 8 x 8 tail level
+    } else {
       int n = tb.n[tail];
       rr[0] = lane < n * n ? br[tb.off[tail] + lane] : 0.f;
     }
@@ -849,8 +1005,23 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
         xr[k] = x;
       }
     }
-    int n = tb.n[tail];
-    if (lane < n * n) bx[tb.off[tail] + lane] = xr[0];
+    if (nblk >= 1) {
+      const WbNbr<2> nb2(bxl, byl);
+      const float xc1[1] = {xr[0]};
+      wb_prolong<2>(x16, xc1, bxl, byl);
+      wb_sweeps<2>(x16, r16, nb2, tb.h2[tail - 1], post, false);
+      if (nblk == 2) {
+        const WbNbr<4> nb4(bxl, byl);
+        wb_prolong<4>(x32, x16, bxl, byl);
+        wb_sweeps<4>(x32, r32, nb4, tb.h2[wave], post, false);
+        wb_store<4>(x32, bx + tb.off[wave], bxl, byl);
+      } else {
+        wb_store<2>(x16, bx + tb.off[wave], bxl, byl);
+      }
+    } else {
+      int n = tb.n[tail];
+      if (lane < n * n) bx[tb.off[tail] + lane] = xr[0];
+    }
   }
   __syncthreads();
   // ---- LDS levels: up

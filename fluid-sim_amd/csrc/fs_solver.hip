@@ -579,6 +579,12 @@ fsk::LevelTable level_table(const fs_solver_t* s, int from, int end) {
     o += s->lv[l].n * s->lv[l].n;
     if (tb.tail == tb.levels && s->lv[l].n <= fsk::REG_N) tb.tail = k;
   }
+  // 16 x 16 and 32 x 32 levels just above an 8 x 8 register tail run as register blocks
+  tb.wave = tb.tail;
+  if (tb.tail < tb.levels && tb.n[tb.tail] == 8 && tb.tail >= 1 && tb.n[tb.tail - 1] == 16) {
+    tb.wave = tb.tail - 1;
+    if (tb.tail >= 2 && tb.n[tb.tail - 2] == 32) tb.wave = tb.tail - 2;
+  }
   return tb;
 }
 
