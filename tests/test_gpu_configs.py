@@ -1,0 +1,184 @@
+"""The workloads at their stated sizes against the f64 oracle: decaying turbulence at
+1024^2 (FP32) and band-forced turbulence with BF16 multigrid storage (256^2 over ten
+steps, 2048^2 over three) through the FP64 gate, the shipped learned-guess weights at
+full resolution on 1024^2, and batched walled cavities (a mixed-size batch member by
+member, and the 256-cavity workload)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * math.pi
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return math.sqrt(np.sum((a - b) ** 2) / max(np.sum(b * b), 1e-300))
+
+
+def f32(a):
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def fs():
+    import fluidsim
+    fluidsim.createDevice()
+    return fluidsim
+
+
+@pytest.fixture(scope="module")
+def threads(O):
+    # the oracle on the host's cores for the large grids
+    O.set_threads(min(16, os.cpu_count() or 1))
+    yield
+    O.set_threads(1)
+
+
+def run_pair(fs, O, n, nu, steps, forcing=None, prec="fp32", guess="zero", tol=1e-6):
+    """GPU steps at CFL 0.5 through the FP64 gate and the oracle (exact FFT projection)
+    on the GPU's dt sequence; returns the relative L2 velocity error and the GPU stats."""
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    print(f"configs n={n} forcing={bool(forcing)} {prec} {guess} steps={steps}", flush=True)
+    g = fs.GPUFluidSolver(0, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid", pressureIterations=60,
+                          tolerance=tol, mgMinSize=4, mgPrecision=prec, guess=guess)
+    if forcing:
+        g.setForcing(forcing)
+    g.uploadVelocity(u0, v0)
+    dts = []
+    for _ in range(steps):
+        g.stepCFL(0.5)
+        st = g.stats()
+        assert st["last_residual"] <= tol, st
+        dts.append(st["last_dt"])
+    out = g.readVelocity()
+    st = g.stats()
+    g.destroy()
+    f = forcing or {}
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft",
+                 forcing=dict(kf=f["kf"], width=f["width"], amp=f["amp"]) if forcing else None,
+                 drag=f.get("drag", 0.0))
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    h = TWO_PI / n
+    for k in range(steps):
+        dt = 0.5 / (np.max(np.abs(s.u)) / h + np.max(np.abs(s.v)) / h)
+        assert abs(dt - dts[k]) / dt < 1e-5
+        s.step(dts[k], force_seed=(f["seed"] + k) if forcing else 0)
+    return max(rel(out["u"], s.u), rel(out["v"], s.v)), st
+
+
+def test_decaying_turbulence_1024_meets_the_gate_and_tracks_the_oracle(fs, O, threads):
+    err, st = run_pair(fs, O, 1024, 1e-4, 3)
+    assert st["gate_misses"] == 0 and st["last_vcycles"] >= 1
+    assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("guess", ["zero", "previous"])
+def test_forced_bf16_storage_tracks_the_oracle_over_ten_steps(fs, O, threads, guess):
+    # the forced workload's band scaled to 256^2 (kf 8 +- 1 for 32 +- 2 at 2048^2)
+    forcing = dict(kf=8.0, width=1.0, amp=0.1, drag=0.02, seed=1000)
+    err, st = run_pair(fs, O, 256, 5e-5, 10, forcing=forcing, prec="bf16", guess=guess)
+    assert st["gate_misses"] == 0
+    assert err < 1e-5, err
+
+
+def test_forced_2048_bf16_storage_tracks_the_oracle(fs, O, threads):
+    # the forced workload as specified: 2048^2, kf 32 +- 2, amp 0.1, drag 0.02,
+    # nu 5e-5, BF16 storage, seed 1000 + step
+    forcing = dict(kf=32.0, width=2.0, amp=0.1, drag=0.02, seed=1000)
+    err, st = run_pair(fs, O, 2048, 5e-5, 3, forcing=forcing, prec="bf16", guess="previous")
+    assert st["gate_misses"] == 0
+    assert err < 1e-5, err
+
+
+def test_shipped_weights_at_1024_match_torch_and_keep_the_gate(fs, O, threads):
+    # the trained network at the grid's own resolution on the decaying 1024^2 flow: its
+    # output against the float64 torch restatement of the library's BF16 inference, then
+    # steps whose projections start from it meet the gate
+    from fluidsim import initial
+    from guess_torch import reference_guess
+    n = 1024
+    with open(WEIGHTS, "rb") as fh:
+        blob = fh.read()
+    u0, v0 = initial.decaying_turbulence(n, seed=1)
+    with fs.GPUFluidSolver(0, n=n, pressureSolver="multigrid", pressureIterations=0, mgMinSize=4) as g:
+        g.uploadVelocity(u0, v0)
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.runKernel("project", 0.0)
+        p = g.readField("pressure")
+    r = O.divergence(n, n, f32(u0), f32(v0))
+    ref = reference_guess(blob, r, np.zeros(n * n), n, TWO_PI / n)
+    assert np.max(np.abs(ref)) > 0
+    err = rel(p, ref)
+    assert err < 3e-2, err
+    with fs.GPUFluidSolver(0, n=n, nu=1e-4, advection="maccormack", pressureSolver="multigrid",
+                           pressureIterations=60, tolerance=1e-6, mgMinSize=4) as g:
+        g.loadGuess(blob)
+        g.setGuess("learned")
+        g.uploadVelocity(u0, v0)
+        for _ in range(4):
+            g.stepCFL(0.5)
+            assert g.stats()["last_residual"] <= 1e-6
+        assert g.stats()["gate_misses"] == 0
+
+
+def test_mixed_size_batch_members_track_the_oracle(fs, O, threads):
+    # one cavity of each size in one batch; every member against the oracle's walled
+    # step (exact Neumann projection) on the batch's own per-cavity dt
+    cavs = [dict(n=512, lid=0.8, nu=4e-4, disk=dict(x=0.6, y=0.4, r=0.15)),
+            dict(n=64, lid=2.0, nu=5e-3, disk=dict(x=0.5, y=0.5, r=0.1)),
+            dict(n=256, lid=1.2, nu=6e-4, disk=None),
+            dict(n=128, lid=1.5, nu=1e-3, disk=dict(x=0.3, y=0.6, r=0.08))]
+    oracles = [O.Cavity(c["n"], U=c["lid"], nu=c["nu"], disk=c["disk"], passes=2) for c in cavs]
+    steps = 6
+    with fs.CavityBatch(cavs, tolerance=1e-8) as b:
+        for _ in range(steps):
+            b.stepCFL(0.5)
+            assert b.stats()["last_residual"] <= 1e-8
+            dts = b.lastDt()
+            for o, dt in zip(oracles, dts):
+                assert abs(dt - o.dt_cfl(0.5)) <= 2e-5 * dt
+                o.step(dt)
+        outs = [b.readVelocity(k) for k in range(len(cavs))]
+    for c, o, out in zip(cavs, oracles, outs):
+        n = c["n"]
+        err = max(rel(out["u"], o.u), rel(out["v"], o.v))
+        assert err < 1e-4, (n, err)
+        assert np.all(out["u"].reshape(n, n)[:, 0] == 0) and np.all(out["v"].reshape(n, n)[0, :] == 0)
+
+
+def test_256_cavity_workload_meets_the_gate_with_walls_and_no_divergence(fs, O):
+    # the batched workload: 256 cavities of 64^2 .. 512^2, each with its lid, viscosity
+    # and pinned disk; three steps through the FP64 gate over every cavity
+    cavs = fs.random_cavities(256, seed=1)
+    with fs.CavityBatch(cavs, tolerance=1e-6) as b:
+        for _ in range(3):
+            b.stepCFL(0.5)
+            st = b.stats()
+            assert st["last_residual"] <= 1e-6
+            assert st["last_vcycles"] >= 1
+        dts = b.lastDt()
+        worst = 0.0
+        for k, c in enumerate(cavs):
+            n = c["n"]
+            out = b.readVelocity(k)
+            uu, vv = out["u"].reshape(n, n), out["v"].reshape(n, n)
+            assert np.all(uu[:, 0] == 0) and np.all(vv[0, :] == 0)
+            assert np.all(np.isfinite(uu)) and np.all(np.isfinite(vv))
+            d = O.cav_divergence(n, out["u"].astype(np.float64), out["v"].astype(np.float64))
+            # FP32 storage of a field projected to 1e-6 in FP64: the stored field's own
+            # divergence sits at the FP32 rounding floor, ~ eps |u| / h
+            worst = max(worst, float(np.max(np.abs(d))) * (1.0 / n) / max(c["lid"], 1e-30))
+            # the lid drags the fluid under it the right way after three steps
+            assert uu[-1, 1:n].mean() * c["lid"] > 0
+        assert worst < 1e-5, worst
+        h = np.array([1.0 / c["n"] for c in cavs])
+        assert np.all(dts > 0) and np.all(dts <= 0.5 * h / np.array([c["lid"] for c in cavs]) * (1 + 1e-6))

@@ -558,7 +558,7 @@ def test_bf16_storage_meets_the_tolerance_gate(fs, O):
 
 def _random_guess_blob(seed):
     import torch
-    from fluidsim.guess_net import build_model, export_blob
+    from guess_torch import build_model, export_blob
     torch.manual_seed(seed)
     return export_blob(build_model())
 
@@ -569,7 +569,7 @@ def test_learned_guess_network_matches_torch(fs, O, seed):
     # network in float64 torch with the library's BF16 roundings of weights and
     # activations; zero iterations leave the guess itself in the pressure
     from fluidsim import initial
-    from fluidsim.guess_net import reference_guess
+    from guess_torch import reference_guess
     n = 256
     blob = _random_guess_blob(seed)
     u0, v0 = initial.decaying_turbulence(n, seed=3)
@@ -634,7 +634,7 @@ def test_learned_coarse_guess_matches_torch(fs, O, n, prec, mode):
     # torch restatement on the same averages (the residual as stored: BF16 under BF16
     # storage)
     from fluidsim import initial
-    from fluidsim.guess_net import bf16_round, reference_guess
+    from guess_torch import bf16_round, reference_guess
     k = 1 if mode == "learned-coarse" else 2
     blob = _random_guess_blob(3)
     u0, v0 = initial.decaying_turbulence(n, seed=7)
@@ -740,7 +740,7 @@ def test_learned_guess_reads_the_bf16_stored_residual(fs, O):
     # under BF16 multigrid storage the residual the network reads is stored in BF16:
     # the same restatement, fed that rounded residual
     from fluidsim import initial
-    from fluidsim.guess_net import bf16_round, reference_guess
+    from guess_torch import bf16_round, reference_guess
     n = 256
     blob = _random_guess_blob(2)
     u0, v0 = initial.decaying_turbulence(n, seed=5)

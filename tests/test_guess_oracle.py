@@ -1,7 +1,7 @@
 """Pins of the learned guess's CPU oracle (oracle/guess_oracle.py), the checker of the
 library's training kernels: the analytic gradient against central finite
 differences of the loss, and against torch autograd of an independent
-implementation of the same network (fluidsim.guess_net.build_model); the
+implementation of the same network (oracle/guess_torch.py, build_model); the
 BF16-storage forward against the torch restatement of the library's inference;
 Adam against torch.optim.Adam. CPU only.
 """
@@ -80,7 +80,7 @@ def test_gradient_matches_central_finite_differences():
 def test_gradient_matches_torch_autograd():
     # an independent implementation (torch conv2d / avg_pool2d / interpolate, NCHW)
     torch = pytest.importorskip("torch")
-    from fluidsim.guess_net import model_from_blob
+    from guess_torch import model_from_blob
     n, B, h = 16, 2, 2 * math.pi / 16
     p, pp = smooth_pair(n, B, 2)
     x, t, _ = G.make_inputs(p, pp, h)
@@ -104,10 +104,10 @@ def test_gradient_matches_torch_autograd():
 
 
 def test_bf16_forward_matches_the_inference_restatement():
-    # the storage-point emulation against fluidsim.guess_net.reference_guess (torch,
+    # the storage-point emulation against guess_torch.reference_guess (torch,
     # float64, the inference's BF16 roundings)
     pytest.importorskip("torch")
-    from fluidsim.guess_net import reference_guess
+    from guess_torch import reference_guess
     n, h = 16, 2 * math.pi / 16
     p, pp = smooth_pair(n, 1, 4)
     x, t, sc = G.make_inputs(p, pp, h)

@@ -617,7 +617,8 @@ def test_bf16_refinement_still_converges_to_the_gate(O):
 
 def test_guess_blob_round_trip_and_reference_forward():
     import torch
-    from fluidsim.guess_net import build_model, export_blob, load_blob, model_from_blob, LAYERS
+    from fluidsim.guess_net import LAYERS, load_blob
+    from guess_torch import build_model, export_blob, model_from_blob
     torch.manual_seed(0)
     m = build_model()
     blob = export_blob(m)
@@ -637,7 +638,7 @@ def test_guess_blob_round_trip_and_reference_forward():
 
 
 def test_bf16_round_matches_the_oracle_rounding(O):
-    from fluidsim.guess_net import bf16_round
+    from guess_torch import bf16_round
     rng = np.random.default_rng(3)
     x = (rng.standard_normal(500) * 10.0 ** rng.integers(-5, 5, 500)).astype(np.float32)
     ours = bf16_round(x).astype(np.float64)
