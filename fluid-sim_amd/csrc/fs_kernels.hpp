@@ -116,6 +116,8 @@ struct StepCoef {
   double dt;
   float cx, cy, da, db, fdt, drag;
   int64_t step;  // index of the step these coefficients belong to
+  double t;      // simulated time at the start of the step
+  double t_end;  // and at its end (the next step's start)
 };
 
 // Device-side convergence gate of a tolerance-driven projection. flag[k] != 0 means
@@ -523,12 +525,20 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
   }
 }
 
-// Band forcing then linear drag, in place (src/cpu/solver.js:200-237 semantics).
-// modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi), coef) in turns per cell;
-// cu/cv carry amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device).
+// Forcing and linear drag, in place, for two sources of modes:
+//   band forcing (src/cpu/solver.js:200-237): (u + f dt) exp(-alpha dt), 24 seeded modes
+//   a mode list (setForcing, src/gpu/solver.js:370-414; FORCING src/gpu/shaders.js:532-566):
+//     u exp(-alpha dt) + f dt, the phases drifting by drift * t (after_drag = 1)
+// modes[m] = (kx*dx/(2pi), ky*dy/(2pi), phase/(2pi)) in turns per cell; cu/cv carry
+// amp/sqrt(M)*ky/|k| and -amp/sqrt(M)*kx/|k| (times dt on the device).
 struct Mode { double tx, ty, tph; float cu, cv; };
-constexpr int MAX_MODES = 24;
-struct ModeSet { Mode m[MAX_MODES]; int n; };
+constexpr int MAX_MODES = 32;
+struct ModeSet {
+  Mode m[MAX_MODES];
+  int n;
+  int after_drag;      // 1: the mode list's order
+  double drift_turns;  // phase drift per unit time, in turns
+};
 
 // Evaluated separably: with A = frac(tx x) and B = frac(ty y + phase) reduced to
 // [0, 1) turns in FP64 (so large wavenumbers keep full FP32 accuracy),
@@ -541,7 +551,8 @@ constexpr int FX = 64, FY = 16;
 // face (x = i) and at the v face (x = i + 1/2); row[m][j] = (cos 2piB, sin 2piB) at the
 // u face (y = j + 1/2) and at the v face (y = j).
 __global__ void __launch_bounds__(256)
-k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row) {
+k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row,
+                 const StepCoef* __restrict__ cf) {
   const int q = blockIdx.x * 256 + threadIdx.x, per = nx + ny;
   if (q >= ms->n * per) return;
   const int m = q / per, r = q - m * per;
@@ -552,8 +563,10 @@ k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restr
     b = md.tx * (r + 0.5);
   } else {
     const int j = r - nx;
-    a = md.ty * (j + 0.5) + md.tph;
-    b = md.ty * j + md.tph;
+    double ph = md.tph + ms->drift_turns * cf->t;
+    ph -= floor(ph);
+    a = md.ty * (j + 0.5) + ph;
+    b = md.ty * j + ph;
   }
   a -= floor(a);
   b -= floor(b);
@@ -583,6 +596,7 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
   const int i = i0 + tx;
   if (i >= nx) return;
   const float fdt = cf->fdt, drag = cf->drag;
+  const bool after_drag = ms->after_drag != 0;
   constexpr int R = FY / 4;  // rows per thread: ty, ty + 4, ...
   float fu[R], fv[R];
 #pragma unroll
@@ -604,8 +618,13 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
     const int j = j0 + ty + 4 * rr;
     if (j >= ny) break;
     const size_t k = (size_t)j * nx + i;
-    u[k] = (u[k] + fu[rr] * fdt) * drag;
-    v[k] = (v[k] + fv[rr] * fdt) * drag;
+    if (after_drag) {
+      u[k] = u[k] * drag + fu[rr] * fdt;
+      v[k] = v[k] * drag + fv[rr] * fdt;
+    } else {
+      u[k] = (u[k] + fu[rr] * fdt) * drag;
+      v[k] = (v[k] + fv[rr] * fdt) * drag;
+    }
   }
 }
 
@@ -931,12 +950,13 @@ __device__ __forceinline__ void fill_coef(StepCoef* cf, double dt, double dx, do
 //   dt: `value` itself (use_cfl = 0), or value / (max|u|/dx + max|v|/dy)
 //       (src/core/grid.js:322-327) from the block maxima part[2*b + {0,1}] of the
 //       last gradient (or maxabs) pass; then the step's coefficients and index
+//   the step's start time t (and, for a step, its end time t + dt)
 //   the step's forcing modes, copied from the argument into ms_dev (copy_modes)
 //   the convergence flags 0 .. nflags-1 of the gated projection, cleared
 __global__ void __launch_bounds__(1024)
 k_step_head(const double* __restrict__ part, int nparts, int use_cfl, double value, double dx, double dy,
-            double nu, double drag, double dt_max, int64_t step, StepCoef* __restrict__ cf, int* __restrict__ flags,
-            int nflags, const ModeSet ms, ModeSet* __restrict__ ms_dev, int copy_modes) {
+            double nu, double drag, double dt_max, int64_t step, int advance_t, StepCoef* __restrict__ cf,
+            int* __restrict__ flags, int nflags, const ModeSet ms, ModeSet* __restrict__ ms_dev, int copy_modes) {
   __shared__ double su[16], sv[16];
   for (int q = threadIdx.x; q < nflags; q += 1024) flags[q] = 0;
   if (copy_modes) {
@@ -972,8 +992,11 @@ k_step_head(const double* __restrict__ part, int nparts, int use_cfl, double val
     }
   }
   if (threadIdx.x == 0) {
+    const double t0 = cf->t_end;
     fill_coef(cf, dt, dx, dy, nu, drag);
     cf->step = step;
+    cf->t = t0;
+    if (advance_t) cf->t_end = t0 + dt;
   }
 }
 

@@ -159,6 +159,11 @@ struct fs_solver_t {
   // forcing
   bool forcing_on = false;
   fs_forcing forcing{};
+  // the mode list of setForcing(modes, {amplitude, drag, phaseDrift}) (src/gpu/solver.js:
+  // 370-414) when forcing_on and mode_list; otherwise the seeded band forcing
+  bool mode_list = false;
+  std::vector<double> mkx, mky, mph;
+  double m_amp = 0, m_drag = 0, m_drift = 0;
   // learned initial guess: weights (BF16 [cout][tap][cin] for the MFMA layers) and activations
   struct GuessNet {
     bool loaded = false;
@@ -377,7 +382,9 @@ int fill_f64(fs_solver_t* s, double* a, size_t n, double val) {
   return FS_OK;
 }
 
-double drag_alpha(const fs_solver_t* s) { return s->forcing_on ? s->forcing.drag : 0.0; }
+double drag_alpha(const fs_solver_t* s) {
+  return !s->forcing_on ? 0.0 : (s->mode_list ? s->m_drag : s->forcing.drag);
+}
 
 // ------------------------------------------------------------- step pieces
 
@@ -391,6 +398,28 @@ double js_round(double x) {
 // uint32 LCG; the amplitude factor dt and the drag factor come from the device
 // coefficients.
 void make_modes(const fs_solver_t* s, int64_t step, fsk::ModeSet& ms) {
+  if (s->mode_list) {
+    // FORCING (src/gpu/shaders.js:532-566): every mode in the list counts in sqrt(count);
+    // a zero wavevector adds nothing
+    ms = fsk::ModeSet{};
+    const int M = (int)s->mkx.size();
+    const double scale = s->m_amp / std::sqrt((double)M);
+    int k = 0;
+    for (int m = 0; m < M; m++) {
+      const double kx = s->mkx[m], ky = s->mky[m], kk = std::sqrt(kx * kx + ky * ky);
+      if (kk < 1e-6) continue;
+      fsk::Mode& md = ms.m[k++];
+      md.tx = kx * s->dx / (2 * M_PI);
+      md.ty = ky * s->dy / (2 * M_PI);
+      md.tph = s->mph[m] / (2 * M_PI);
+      md.cu = (float)(scale * (ky / kk));
+      md.cv = (float)(scale * (-kx / kk));
+    }
+    ms.n = k;
+    ms.after_drag = 1;
+    ms.drift_turns = s->m_drift / (2 * M_PI);
+    return;
+  }
   const fs_forcing& f = s->forcing;
   uint32_t st = f.seed + (uint32_t)step;
   auto rnd = [&]() {
@@ -421,7 +450,7 @@ void make_modes(const fs_solver_t* s, int64_t step, fsk::ModeSet& ms) {
 
 // The first launch of a step: dt (fixed, or from the CFL condition on the device),
 // the coefficients and step index, the step's forcing modes, cleared gate flags.
-int enqueue_head(fs_solver_t* s, bool use_cfl, double value, int nflags) {
+int enqueue_head(fs_solver_t* s, bool use_cfl, double value, int nflags, bool advance_t = true) {
   if (use_cfl && !s->have_maxabs) {
     fsk::k_maxabs<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[s->cur], s->v[s->cur], s->nx, s->ny, s->mpart);
     LAUNCHCHK();
@@ -429,7 +458,8 @@ int enqueue_head(fs_solver_t* s, bool use_cfl, double value, int nflags) {
   fsk::ModeSet ms{};
   if (s->forcing_on) make_modes(s, s->steps, ms);
   fsk::k_step_head<<<1, 1024, 0, s->stream>>>(s->mpart, s->mp_count, use_cfl ? 1 : 0, value, s->dx, s->dy,
-                                              s->cfg.nu, drag_alpha(s), 1e30, s->steps, s->coef, s->gate_flag,
+                                              s->cfg.nu, drag_alpha(s), 1e30, s->steps, advance_t ? 1 : 0, s->coef,
+                                              s->gate_flag,
                                               std::min(nflags, s->gate_len), ms, s->modes_dev,
                                               s->forcing_on ? 1 : 0);
   LAUNCHCHK();
@@ -489,7 +519,8 @@ int enqueue_forcing(fs_solver_t* s) {
   // wavenumber is zero is dropped); with no mode in the band the pass below still drags
   const unsigned tb = (unsigned)((fsk::MAX_MODES * (s->nx + s->ny) + 255) / 256);
   if ((rc = launch(s, PK_FORCING, 16.0 * fsk::MAX_MODES * (s->nx + s->ny), fsk::k_forcing_tables, dim3(tb), dim3(256),
-                   (const fsk::ModeSet*)s->modes_dev, s->nx, s->ny, s->ftab_col, s->ftab_row)))
+                   (const fsk::ModeSet*)s->modes_dev, s->nx, s->ny, s->ftab_col, s->ftab_row,
+                   (const fsk::StepCoef*)s->coef)))
     return rc;
   const dim3 g((unsigned)((s->nx + fsk::FX - 1) / fsk::FX), (unsigned)((s->ny + fsk::FY - 1) / fsk::FY));
   return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny,
@@ -2528,6 +2559,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   s->mp_count = grid_blocks(s->nx, s->ny);
   if (!rc) rc = dalloc(&s->mpart, (size_t)2 * s->mp_count);
   if (!rc) rc = dalloc(&s->coef, 1);
+  if (!rc && hipMemsetAsync(s->coef, 0, sizeof(fsk::StepCoef), s->stream) != hipSuccess)
+    rc = fail(FS_E_DEVICE, "clearing the step coefficients failed");
   if (!rc) rc = dalloc(&s->modes_dev, 1);
   if (!rc) rc = halloc_coherent(&s->ring, REC_RING);
   if (!rc) rc = halloc(&s->dt_ring, DT_RING);
@@ -2912,6 +2945,7 @@ int fs_set_forcing(void* h, const fs_forcing* f) {
       destroy_graphs(s);
     }
     s->forcing_on = false;
+    s->mode_list = false;
     return FS_OK;
   }
   if (!std::isfinite(f->kf) || !std::isfinite(f->width) || !std::isfinite(f->amp) || !(f->drag >= 0) ||
@@ -2923,8 +2957,46 @@ int fs_set_forcing(void* h, const fs_forcing* f) {
         (rc = dalloc(&s->ftab_row, (size_t)fsk::MAX_MODES * s->ny)))
       return rc;
   }
-  if (!s->forcing_on) destroy_graphs(s);
+  if (!s->forcing_on || s->mode_list) destroy_graphs(s);
   s->forcing = *f;
+  s->forcing_on = true;
+  s->mode_list = false;
+  return FS_OK;
+}
+
+int fs_set_forcing_modes(void* h, const double* kx, const double* ky, const double* phase, int count,
+                         double amplitude, double drag, double phase_drift) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (count < 0) return fail(FS_E_INVALID_ARG, "count must be >= 0, got %d", count);
+  if (count > 0 && (!kx || !ky || !phase)) return fail(FS_E_INVALID_ARG, "kx, ky or phase is NULL");
+  if (count > fsk::MAX_MODES)
+    return fail(FS_E_UNSUPPORTED, "at most %d forcing modes, got %d", fsk::MAX_MODES, count);
+  for (int m = 0; m < count; m++)
+    if (!std::isfinite(kx[m]) || !std::isfinite(ky[m]) || !std::isfinite(phase[m]))
+      return fail(FS_E_INVALID_ARG, "mode %d is not finite", m);
+  if (!std::isfinite(amplitude) || !(drag >= 0) || !std::isfinite(drag) || !std::isfinite(phase_drift))
+    return fail(FS_E_INVALID_ARG, "amplitude and phase drift must be finite, drag >= 0");
+  if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+  destroy_graphs(s);  // captured bodies hold the forcing launches (or their absence)
+  if (count == 0) {   // setForcing(null) or an empty list disables it (src/gpu/solver.js:372)
+    s->forcing_on = false;
+    s->mode_list = false;
+    return FS_OK;
+  }
+  if (!s->ftab_col) {
+    if ((rc = dalloc(&s->ftab_col, (size_t)fsk::MAX_MODES * s->nx)) ||
+        (rc = dalloc(&s->ftab_row, (size_t)fsk::MAX_MODES * s->ny)))
+      return rc;
+  }
+  s->mkx.assign(kx, kx + count);
+  s->mky.assign(ky, ky + count);
+  s->mph.assign(phase, phase + count);
+  s->m_amp = amplitude;
+  s->m_drag = drag;
+  s->m_drift = phase_drift;
+  s->mode_list = true;
   s->forcing_on = true;
   return FS_OK;
 }
@@ -3070,9 +3142,9 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
-  if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE) {
+  if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE || kernel == FS_K_FORCING) {
     if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0");
-    if ((rc = enqueue_head(s, false, dt, 0))) return rc;
+    if ((rc = enqueue_head(s, false, dt, 0, false))) return rc;
   }
   switch (kernel) {
     case FS_K_ADVECT_SL: {
@@ -3098,6 +3170,10 @@ int fs_run_kernel(void* h, int kernel, double dt) {
     case FS_K_SOLID:
       if (!s->solid_on) return fail(FS_E_INVALID_ARG, "no disk is set");
       rc = enqueue_solid(s);
+      break;
+    case FS_K_FORCING:
+      if (!s->forcing_on) return fail(FS_E_INVALID_ARG, "no forcing is set");
+      rc = enqueue_forcing(s);
       break;
     case FS_K_GRADIENT:
       if (s->cfg.pressure_solver != FS_P_MULTIGRID)

@@ -27,7 +27,8 @@ FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
                  "advect_fused": 11, "forcing": 12}
-KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6}
+KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6,
+          "forcing": 7}
 
 
 class FsConfig(C.Structure):
@@ -149,6 +150,7 @@ _SIGNATURES = {
     "fs_set_nu": (C.c_int, [_VP, C.c_double]),
     "fs_set_guess": (C.c_int, [_VP, C.c_int]),
     "fs_set_forcing": (C.c_int, [_VP, C.POINTER(FsForcing)]),
+    "fs_set_forcing_modes": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_double, C.c_double, C.c_double]),
     "fs_set_solid": (C.c_int, [_VP, C.POINTER(FsDisk)]),
     "fs_solid_impulse": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "fs_get_stats": (C.c_int, [_VP, C.POINTER(FsStats)]),

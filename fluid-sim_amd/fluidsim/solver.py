@@ -253,11 +253,23 @@ class GPUFluidSolver:
         """device.queue.onSubmittedWorkDone()."""
         check(self._L.fs_sync(self.handle))
 
-    def setForcing(self, forcing):
-        """Band forcing with the CPU's seeded semantics (src/cpu/solver.js:179-237):
-        dict(kf, width, amp, drag, seed) or None to disable; step k uses seed + k."""
-        if forcing is None:
+    def setForcing(self, forcing, amplitude=1.0, drag=0.0, phaseDrift=0.3):
+        """setForcing (src/gpu/solver.js:370-414).
+
+        A list of modes [{kx, ky, phase}] is the reference's own call: its GPU kernel's
+        semantics (u exp(-drag dt) + amplitude dt / sqrt(count) * curl modes, phases
+        drifting by phaseDrift * t). A dict(kf, width, amp, drag, seed) is the seeded band
+        forcing of the CPU solver (src/cpu/solver.js:179-237; step k uses seed + k).
+        None or an empty list disables."""
+        if forcing is None or (isinstance(forcing, (list, tuple)) and len(forcing) == 0):
             check(self._L.fs_set_forcing(self.handle, None))
+            return
+        if isinstance(forcing, (list, tuple)):
+            kx = np.ascontiguousarray([float(m["kx"]) for m in forcing], np.float64)
+            ky = np.ascontiguousarray([float(m["ky"]) for m in forcing], np.float64)
+            ph = np.ascontiguousarray([float(m.get("phase", 0.0)) for m in forcing], np.float64)
+            check(self._L.fs_set_forcing_modes(self.handle, _ptr(kx), _ptr(ky), _ptr(ph), len(forcing),
+                                               float(amplitude), float(drag), float(phaseDrift)))
             return
         f = FsForcing(float(forcing.get("kf", 28)), float(forcing.get("width", 2)),
                       float(forcing.get("amp", 0.4)), float(forcing.get("drag", 0.0)),

@@ -74,7 +74,8 @@ enum {
   FS_K_DIFFUSE = 3,     /* velocity <- explicit diffusion(velocity), dt, nu */
   FS_K_PROJECT = 4,     /* one projection with the configured solver */
   FS_K_GRADIENT = 5,    /* velocity <- velocity - grad(pressure) */
-  FS_K_SOLID = 6        /* velocity <- direct forcing toward the disk's rigid motion */
+  FS_K_SOLID = 6,       /* velocity <- direct forcing toward the disk's rigid motion */
+  FS_K_FORCING = 7      /* velocity <- forcing and drag of the current step index and time, dt */
 };
 
 /* GPUFluidSolver constructor options, src/gpu/solver.js:64-98 */
@@ -257,9 +258,16 @@ int fs_set_guess(void* solver, int guess);
  * releases the previous weights and leaves the solver on FS_GUESS_PREVIOUS. */
 int fs_load_guess(void* solver, const void* blob, size_t bytes);
 
-/* setForcing(modes, {amplitude, drag}) src/gpu/solver.js:370-385, with the CPU's
- * seeded band-forcing semantics; NULL disables */
+/* seeded band forcing with the CPU's semantics (bandForcing + linearDrag,
+ * src/cpu/solver.js:179-237: 24 modes from seed + step, force then drag); NULL disables */
 int fs_set_forcing(void* solver, const fs_forcing* forcing);
+/* setForcing(modes, {amplitude, drag, phaseDrift}) src/gpu/solver.js:370-414 with the GPU
+ * kernel's semantics (FORCING src/gpu/shaders.js:532-566): u <- u exp(-drag dt) + amplitude
+ * dt / sqrt(count) * sum (ky/|k|, -kx/|k|) cos(kx x + ky y + phase + phaseDrift t), t the
+ * simulated time at the start of the step. count = 0 disables (setForcing(null)); at most
+ * 32 modes. Replaces a band forcing and is replaced by one. */
+int fs_set_forcing_modes(void* solver, const double* kx, const double* ky, const double* phase, int count,
+                         double amplitude, double drag, double phase_drift);
 
 /* setSolid(disk) src/gpu/solver.js:425-440 with the state _encodeSolid uploads each
  * step (:442-452): the next steps couple this disk; NULL removes it. Callers that

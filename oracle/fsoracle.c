@@ -1119,6 +1119,32 @@ void or_band_forcing(int nx, int ny, double lx, double ly, double *u, double *v,
   }
 }
 
+/* FORCING, src/gpu/shaders.js:532-566, with the parameters _encodeForcing writes
+ * (src/gpu/solver.js:387-396): drag on the old velocity, then the force */
+void or_mode_forcing(int nx, int ny, double lx, double ly, double *u, double *v, int count, const double *kx,
+                     const double *ky, const double *phase, double amp, double drag, double drift, double t,
+                     double dt) {
+  double dx = lx / nx, dy = ly / ny;
+  double scale = amp * dt / sqrt((double)count);
+  double damp = exp(-drag * dt);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++) {
+    for (int i = 0; i < nx; i++) {
+      double xu = i * dx, yu = (j + 0.5) * dy, xv = (i + 0.5) * dx, yv = j * dy;
+      double fu = 0, fv = 0;
+      for (int m = 0; m < count; m++) {
+        double kk = sqrt(kx[m] * kx[m] + ky[m] * ky[m]);
+        if (kk < 1e-6) continue;
+        double ph = phase[m] + drift * t;
+        fu += (ky[m] / kk) * cos(kx[m] * xu + ky[m] * yu + ph);
+        fv -= (kx[m] / kk) * cos(kx[m] * xv + ky[m] * yv + ph);
+      }
+      u[j * nx + i] = u[j * nx + i] * damp + scale * fu;
+      v[j * nx + i] = v[j * nx + i] * damp + scale * fv;
+    }
+  }
+}
+
 /* src/cpu/solver.js:233-237 */
 void or_linear_drag(int n, double *u, double *v, double alpha, double dt) {
   if (alpha <= 0) return;
@@ -1266,9 +1292,14 @@ int or_step(const or_step_cfg *c, double *u, double *v, double *p_out, double dt
     memcpy(u, tu, sizeof(double) * n);
     memcpy(v, tv, sizeof(double) * n);
   }
-  if (c->forcing)
-    or_band_forcing(nx, ny, c->lx, c->ly, u, v, c->kf, c->width, c->amp, force_seed, dt);
-  if (c->drag > 0) or_linear_drag((int)n, u, v, c->drag, dt);
+  if (c->modes > 0) {
+    or_mode_forcing(nx, ny, c->lx, c->ly, u, v, c->modes, c->mkx, c->mky, c->mph, c->mamp, c->mdrag, c->mdrift,
+                    c->mt, dt);
+  } else {
+    if (c->forcing)
+      or_band_forcing(nx, ny, c->lx, c->ly, u, v, c->kf, c->width, c->amp, force_seed, dt);
+    if (c->drag > 0) or_linear_drag((int)n, u, v, c->drag, dt);
+  }
   if (c->solid) {
     or_disk d = {c->sx, c->sy, c->sr, c->svx, c->svy, c->somega};
     or_apply_coupling(nx, ny, c->lx, c->ly, u, v, &d, c->mask_width, c->passes, NULL);

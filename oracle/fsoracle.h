@@ -55,6 +55,12 @@ typedef struct {
   int solid;
   double sx, sy, sr, svx, svy, somega, mask_width;
   int passes;
+  /* the GPU's mode-list forcing (src/gpu/solver.js:370-414) instead of the band forcing
+   * when modes > 0: kx, ky, phase of each mode, amplitude, drag, phase drift, and the
+   * simulated time t at the start of the step */
+  int modes;
+  const double *mkx, *mky, *mph;
+  double mamp, mdrag, mdrift, mt;
 } or_step_cfg;
 
 /* rigid disk for the direct-forcing coupling (src/cpu/solid.js:52-114) */
@@ -158,6 +164,14 @@ void   or_cav_apply_coupling(int n, double *u, double *v, const or_disk *s, doub
                              double *imp);
 int    or_cav_step(int n, double U, double nu, int scheme, const or_disk *disk, double mask_width, int passes,
                    double dt, double *u, double *v, double *p_out);
+
+/* mode-list forcing of the GPU path (FORCING src/gpu/shaders.js:532-566, parameters
+ * src/gpu/solver.js:387-396): u <- u exp(-drag dt) + amp dt / sqrt(count) *
+ * sum (ky/|k|) cos(kx x + ky y + phase + drift t) at the u faces, -(kx/|k|) at the v
+ * faces; modes with |k| < 1e-6 add nothing but count */
+void   or_mode_forcing(int nx, int ny, double lx, double ly, double *u, double *v, int count, const double *kx,
+                       const double *ky, const double *phase, double amp, double drag, double drift, double t,
+                       double dt);
 
 /* one full step (src/cpu/solver.js:336-398); p_out may be NULL */
 int    or_step(const or_step_cfg *cfg, double *u, double *v, double *p_out, double dt, uint32_t force_seed);

@@ -37,6 +37,9 @@ class StepCfg(C.Structure):
         ("sx", C.c_double), ("sy", C.c_double), ("sr", C.c_double),
         ("svx", C.c_double), ("svy", C.c_double), ("somega", C.c_double), ("mask_width", C.c_double),
         ("passes", C.c_int),
+        ("modes", C.c_int),
+        ("mkx", C.c_void_p), ("mky", C.c_void_p), ("mph", C.c_void_p),
+        ("mamp", C.c_double), ("mdrag", C.c_double), ("mdrift", C.c_double), ("mt", C.c_double),
     ]
 
 
@@ -104,6 +107,7 @@ def lib():
         "or_mg_vcycles": (i, [i, d, p, p, i, i, i, i, i, C.c_void_p]),
         "or_band_modes": (i, [C.c_uint32, d, d, _I, _I, p]),
         "or_band_forcing": (None, [i, i, d, d, p, p, d, d, d, C.c_uint32, d]),
+        "or_mode_forcing": (None, [i, i, d, d, p, p, i, p, p, p, d, d, d, d, d]),
         "or_linear_drag": (None, [i, p, p, d, d]),
         "or_taylor_green": (None, [i, i, d, d, d, d, p, p]),
         "or_velocity_error_tg": (d, [i, i, d, d, p, p, d, d]),
@@ -271,6 +275,16 @@ def band_forcing(nx, ny, u, v, kf, width, amp, seed, dt, lx=TWO_PI, ly=TWO_PI):
     u = _f(u).copy()
     v = _f(v).copy()
     lib().or_band_forcing(nx, ny, lx, ly, u, v, kf, width, amp, seed & 0xFFFFFFFF, dt)
+    return u, v
+
+
+def mode_forcing(nx, ny, u, v, modes, amplitude, drag, drift, t, dt, lx=TWO_PI, ly=TWO_PI):
+    u = _f(u).copy()
+    v = _f(v).copy()
+    kx = np.array([float(m["kx"]) for m in modes])
+    ky = np.array([float(m["ky"]) for m in modes])
+    ph = np.array([float(m["phase"]) for m in modes])
+    lib().or_mode_forcing(nx, ny, lx, ly, u, v, len(modes), kx, ky, ph, amplitude, drag, drift, t, dt)
     return u, v
 
 
@@ -476,7 +490,21 @@ class Solver:
         self.t = 0.0
         self.steps = 0
 
+    def set_modes(self, modes, amplitude=1.0, drag=0.0, phaseDrift=0.3):
+        """The GPU path's mode-list forcing (setForcing(modes, {...}), src/gpu/solver.js:370-385)
+        instead of the band forcing; modes: [{kx, ky, phase}], None or [] to disable."""
+        if not modes:
+            self.cfg.modes = 0
+            return
+        self._mk = np.array([float(m["kx"]) for m in modes])
+        self._my = np.array([float(m["ky"]) for m in modes])
+        self._mp = np.array([float(m["phase"]) for m in modes])
+        self.cfg.modes = len(modes)
+        self.cfg.mkx, self.cfg.mky, self.cfg.mph = (a.ctypes.data for a in (self._mk, self._my, self._mp))
+        self.cfg.mamp, self.cfg.mdrag, self.cfg.mdrift = float(amplitude), float(drag), float(phaseDrift)
+
     def step(self, dt, force_seed=0):
+        self.cfg.mt = self.t
         rc = lib().or_step(C.byref(self.cfg), self.u, self.v, self.p.ctypes.data_as(C.c_void_p), dt,
                            force_seed & 0xFFFFFFFF)
         if rc:

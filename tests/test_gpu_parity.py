@@ -257,6 +257,49 @@ def test_tolerance_gate_on_turbulence(fs, O):
     assert d["max_divergence"] < 5e-4
 
 
+MODES = [dict(kx=3, ky=-2, phase=0.4), dict(kx=0, ky=5, phase=2.0), dict(kx=0, ky=0, phase=1.0),
+         dict(kx=-4, ky=1, phase=5.5), dict(kx=6, ky=4, phase=3.3)]
+
+
+def test_mode_list_forcing_kernel_matches_oracle(fs, O):
+    # setForcing(modes, {amplitude, drag, phaseDrift}) (src/gpu/solver.js:370-414) applied
+    # once at t = 0 against the oracle's FORCING (src/gpu/shaders.js:532-566)
+    n, dt = 128, 0.02
+    from fluidsim import initial
+    u0, v0 = initial.decaying_turbulence(n, seed=2)
+    with solver(fs, n=n) as g:
+        g.uploadVelocity(u0, v0)
+        g.setForcing(MODES, amplitude=1.9, drag=0.42, phaseDrift=0.7)
+        g.runKernel("forcing", dt)
+        out = g.readVelocity()
+    ou, ov = O.mode_forcing(n, n, f32(u0), f32(v0), MODES, 1.9, 0.42, 0.7, 0.0, dt)
+    assert max(rel(out["u"], ou), rel(out["v"], ov)) < 1e-6
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_mode_list_forced_steps_match_oracle(fs, O, graph):
+    # twenty steps whose forcing phases drift with the simulated time, through the gated
+    # projection (and its captured graphs), against the oracle with the exact projection
+    n, nu, dt, steps = 128, 0.01, 0.01, 20
+    u0, v0 = O.taylor_green(n, n)
+    with solver(fs, n=n, nu=nu, advection="maccormack", pressureSolver="multigrid", pressureIterations=40,
+                tolerance=1e-9, mgMinSize=4, graph=graph) as g:
+        g.setForcing(MODES, amplitude=1.9, drag=0.42, phaseDrift=0.7)
+        g.uploadVelocity(u0, v0)
+        for _ in range(steps):
+            g.step(dt)
+        st = g.stats()
+        assert st["last_residual"] <= 1e-9 and st["gate_misses"] == 0
+        assert abs(st["t"] - steps * dt) < 1e-12
+        out = g.readVelocity()
+    s = O.Solver(n=n, nu=nu, advection="maccormack", explicit_diffusion=True, projection="fft")
+    s.set_modes(MODES, amplitude=1.9, drag=0.42, phaseDrift=0.7)
+    s.u[:], s.v[:] = f32(u0), f32(v0)
+    for _ in range(steps):
+        s.step(dt)
+    assert max(rel(out["u"], s.u), rel(out["v"], s.v)) < 1e-5
+
+
 @pytest.mark.parametrize("n,kf,steps", [(64, 8, 20), (256, 16, 5)])
 def test_forced_steps_match_oracle(fs, O, n, kf, steps):
     # band forcing + linear drag (src/cpu/solver.js:179-237, order :354-358): step k

@@ -443,6 +443,38 @@ def test_band_forcing_pin_rejects_planted_mistakes(O):
             Gu += amp * dt / math.sqrt(24) * b / math.hypot(a, b) * np.cos(a * I * h + b * J * h + p)
     assert _forcing_pin_error(O, Gu.reshape(-1), Fv, n, seed, kf, width, amp, dt) > 1e-2
 
+
+def test_mode_list_forcing_is_its_closed_form(O):
+    # FORCING (src/gpu/shaders.js:532-566): drag on the old velocity, then
+    # amp dt / sqrt(count) (ky/|k|, -kx/|k|) cos(k.x + phase + drift t) at the faces; a
+    # zero wavevector counts in sqrt(count) but adds nothing
+    n, amp, drag, drift, t, dt = 64, 1.9, 0.42, 0.7, 1.3, 0.01
+    modes = [dict(kx=3, ky=-2, phase=0.4), dict(kx=0, ky=5, phase=2.0), dict(kx=0, ky=0, phase=1.0),
+             dict(kx=-4, ky=1, phase=5.5)]
+    z = np.zeros(n * n)
+    Fu, Fv = O.mode_forcing(n, n, z, z, modes, amp, drag, drift, t, dt)
+    scale = amp * dt / math.sqrt(len(modes))
+    for comp, F in (("u", Fu), ("v", Fv)):
+        C, KX, KY = _face_coefficients(F, n, comp)
+        want = np.zeros_like(C)
+        for m in modes:
+            a, b = m["kx"], m["ky"]
+            if a == 0 and b == 0:
+                continue
+            c = scale * (b if comp == "u" else -a) / math.hypot(a, b)
+            ph = m["phase"] + drift * t
+            want[(KX == a) & (KY == b)] += c * np.exp(1j * ph)
+            want[(KX == -a) & (KY == -b)] += c * np.exp(-1j * ph)
+        assert np.max(np.abs(C - want)) < 1e-12 * amp * dt
+    # linear in the velocity with the drag factor on the old velocity alone: the CPU
+    # order (u + f) exp(-drag dt) would scale the force as well
+    rng = np.random.default_rng(1)
+    u0, v0 = rng.standard_normal((2, n * n))
+    Gu, Gv = O.mode_forcing(n, n, u0, v0, modes, amp, drag, drift, t, dt)
+    damp = math.exp(-drag * dt)
+    assert np.max(np.abs(Gu - (damp * u0 + Fu))) < 1e-14 and np.max(np.abs(Gv - (damp * v0 + Fv))) < 1e-14
+    assert np.max(np.abs(Gu - damp * (u0 + Fu))) > 1e-6
+
 # ------------------------------------------------- immersed disk (src/cpu/solid.js)
 # restating test/solid.test.js:29-160 on the oracle's coupling
 
