@@ -2,12 +2,15 @@
 // MI355X library through the Node-API addon (fluidsim.node -> libfluidsim.so).
 //
 // Exports the same two names (createDevice, GPUFluidSolver) with the same
-// constructor options, defaults, methods and Promise-returning readbacks.
+// constructor options, defaults, methods and Promise-returning readbacks (the
+// copies run on a worker thread; calls on one solver keep their call order).
 // Deliberate differences: unknown selectors throw (the reference silently runs
-// Jacobi / semi-Lagrangian, src/gpu/solver.js:606, :725); splats, vorticity
-// confinement, dye and rendering are display features that are not built;
-// forcing is the seeded band forcing of src/cpu/solver.js:179-237 via
-// setBandForcing(), and an explicit mode list passed to setForcing() throws.
+// Jacobi / semi-Lagrangian, src/gpu/solver.js:606, :725); vorticity confinement
+// and rendering are display features that are not built. setForcing(modes, opts)
+// is the reference's mode-list forcing; setBandForcing() adds the CPU solver's
+// seeded band forcing (src/cpu/solver.js:179-237). The reference's internal
+// textures are named objects here (velCur, velNext, presCur, div): readTexture()
+// returns their rgba32float texels, runKernel() replaces gputest's dispatches.
 // Written in Node 12 syntax (no optional chaining, no nullish coalescing).
 import { createRequire } from "module";
 
@@ -17,6 +20,11 @@ const addon = require("./fluidsim.node");
 const ADVECTION = { "semi-lagrangian": 0, maccormack: 1 };
 const PRESSURE = { jacobi: 0, "red-black": 1, multigrid: 2 };
 const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-quarter": 4 };
+// fs_run_kernel ids (include/fluidsim.h)
+const KERNEL = { "advect-sl": 0, "advect-mc": 1, divergence: 2, diffuse: 3, project: 4, gradient: 5, solid: 6,
+  forcing: 7 };
+// fs_read_field ids
+const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4 };
 
 export async function createDevice({ requireTimestamps = false } = {}) {
   const d = addon.device();
@@ -27,7 +35,7 @@ export async function createDevice({ requireTimestamps = false } = {}) {
     queue: {
       // the reference awaits this after step() to time the GPU
       onSubmittedWorkDone: async () => {
-        for (const s of solvers) s.sync();
+        for (const s of solvers) await addon.syncAsync(s._h);
       },
     },
   };
@@ -75,7 +83,15 @@ export class GPUFluidSolver {
       guess: GUESS[guess], solidMaskWidth,
     });
     if (device && device._solvers) device._solvers.add(this);
+    // the reference's textures by name (src/gpu/solver.js:111-125, :287-292), for readTexture
+    this.div = { format: "rgba32float", _field: "div" };
+    this.output = { format: "rgba16float", _field: null };
   }
+
+  get velCur() { return { format: "rgba32float", _field: "velocity" }; }
+  get velNext() { return { format: "rgba32float", _field: "velocity" }; }
+  get presCur() { return { format: "rgba32float", _field: "pressure" }; }
+  get presNext() { return { format: "rgba32float", _field: "pressure" }; }
 
   get nu() { return this._nu; }
   set nu(v) { addon.setParam(this._h, "nu", v); this._nu = v; }
@@ -98,7 +114,49 @@ export class GPUFluidSolver {
 
   uploadVelocity(u, v) { addon.uploadVelocity(this._h, u, v); }
 
-  async readVelocity() { return addon.readVelocity(this._h, this.nx * this.ny); }
+  readVelocity() { return addon.readVelocity(this._h, this.nx * this.ny); }
+
+  // readTexture(tex) (src/gpu/solver.js:336-353): Float32Array(nx*ny*4) of rgba texels,
+  // the velocity in (r, g), a scalar field in r; throws on a texture that is not
+  // rgba32float, as the reference does
+  async readTexture(tex) {
+    if (!tex || tex.format !== "rgba32float") {
+      throw new Error(`readTexture expects rgba32float, got ${tex ? tex.format : tex}; ` +
+        "the 16-byte stride would misread it");
+    }
+    const n = this.nx * this.ny;
+    const out = new Float32Array(n * 4);
+    if (tex._field === "velocity") {
+      const { u, v } = await addon.readVelocity(this._h, n);
+      for (let i = 0; i < n; i++) { out[i * 4] = u[i]; out[i * 4 + 1] = v[i]; }
+    } else {
+      const f = await addon.readField(this._h, FIELD[tex._field], n);
+      for (let i = 0; i < n; i++) out[i * 4] = f[i];
+    }
+    return out;
+  }
+
+  // a field by name: "u", "v", "div", "pressure" (Float64Array), "residual"
+  readField(name) {
+    if (!(name in FIELD)) throw new Error(`unknown field ${name}; have ${Object.keys(FIELD).join(", ")}`);
+    return addon.readField(this._h, FIELD[name], this.nx * this.ny);
+  }
+
+  // one kernel by name (replaces gputest's _dispatch of internals, tools/gputest.html:90-129)
+  runKernel(name, dt = 0) {
+    if (!(name in KERNEL)) throw new Error(`unknown kernel ${name}; have ${Object.keys(KERNEL).join(", ")}`);
+    addon.runKernel(this._h, KERNEL[name], dt);
+  }
+
+  // energySpectrum (src/core/fft.js:196-234) of the current velocity on the device:
+  // Promise<{k, E, counts}> for shells k = 0 .. n/2
+  async energySpectrum() {
+    const len = Math.floor(this.nx / 2) + 1;
+    const { E, counts } = await addon.energySpectrum(this._h, len);
+    const k = new Float64Array(len);
+    for (let i = 0; i < len; i++) k[i] = i;
+    return { k, E, counts };
+  }
 
   step(dt, { splats = [] } = {}) {
     if (splats.length) throw new Error("splats are display-only and not built here");
@@ -140,16 +198,18 @@ export class GPUFluidSolver {
 
   sync() { addon.sync(this._h); }
 
-  setForcing(modes) {
-    if (modes === null || modes === undefined || modes.length === 0) { addon.setForcing(this._h, null); return; }
-    throw new Error("explicit forcing mode lists are not supported; use setBandForcing({kf, width, amp, drag, seed})");
+  // setForcing(modes, {amplitude, drag, phaseDrift}) (src/gpu/solver.js:370-414)
+  setForcing(modes, { amplitude = 1, drag = 0, phaseDrift = 0.3 } = {}) {
+    if (!modes || modes.length === 0) { addon.setForcing(this._h, null); return; }
+    addon.setForcingModes(this._h, modes.map(m => m.kx), modes.map(m => m.ky), modes.map(m => m.phase || 0),
+      amplitude, drag, phaseDrift);
   }
 
   setBandForcing(f) { addon.setForcing(this._h, f || null); }
 
-  async maxDivergence() { return addon.maxDivergence(this._h); }
+  maxDivergence() { return addon.maxDivergence(this._h); }
 
-  async poissonResidualAfter(iterations, dt = 0.01) { return addon.poissonResidualAfter(this._h, iterations); }
+  poissonResidualAfter(iterations, dt = 0.01) { return addon.poissonResidualAfter(this._h, iterations); }
 
   stats() { return addon.stats(this._h); }
 

@@ -141,6 +141,7 @@ def test_node_addon_loads_and_exports_the_shim_surface():
     assert out.returncode == 0, out.stderr
     names = set(out.stdout.strip().split(","))
     assert {"create", "destroy", "uploadVelocity", "readVelocity", "step", "stepCFL", "sync", "maxDivergence",
-            "poissonResidualAfter", "setParam", "setForcing", "stats", "setSolid", "solidImpulse"} <= names
+            "poissonResidualAfter", "setParam", "setForcing", "stats", "setSolid", "solidImpulse", "readField",
+            "energySpectrum", "syncAsync", "runKernel", "setForcingModes"} <= names
     shim = open(os.path.join(ROOT, "fluid-sim_amd", "node", "solver.mjs")).read()
     assert "export async function createDevice" in shim and "export class GPUFluidSolver" in shim

@@ -94,3 +94,88 @@ def test_node_shim_runs_the_learned_guess_like_the_python_binding(mode):
     assert st["last_residual"] <= 1e-6 and js["lastResidual"] <= 1e-6
     assert js["totalVCycles"] == st["total_vcycles"]
     assert math.isclose(js["checksumU"], su, rel_tol=1e-10) and math.isclose(js["checksumV"], sv, rel_tol=1e-10)
+
+
+def _load(d, name, dtype):
+    return np.fromfile(os.path.join(d, name), dtype=dtype)
+
+
+def test_node_gputest_checks_equal_the_python_binding(tmp_path):
+    # tools/gputest.html checks 1-7 run through solver.mjs alone (async readbacks,
+    # readTexture of the named textures, runKernel, the Poisson curves, the spectrum);
+    # the same calls through the Python binding on the inputs the JS engine generated
+    # give the same bits
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed on this machine")
+    d = str(tmp_path)
+    out = subprocess.run([node, os.path.join(ROOT, "tests", "node", "gputest.mjs"), d], capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    with open(os.path.join(d, "results.json")) as f:
+        R = json.load(f)
+    import fluidsim
+    N, dt = 64, 0.01
+    tu, tv = _load(d, "ic_tg_u.f64", np.float64), _load(d, "ic_tg_v.f64", np.float64)
+
+    def same(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        return a.shape == b.shape and np.array_equal(a, b)
+
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureIterations=0, advection="semi-lagrangian") as g:
+        g.uploadVelocity(tu, tv)
+        g.runKernel("advect-sl", dt)
+        vel = g.readVelocity()
+    assert same(vel["u"], _load(d, "adv_u.f32", np.float32)) and same(vel["v"], _load(d, "adv_v.f32", np.float32))
+    u0, v0 = _load(d, "ic_991_u.f64", np.float64), _load(d, "ic_991_v.f64", np.float64)
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0) as g:
+        g.uploadVelocity(u0, v0)
+        g.runKernel("divergence")
+        assert same(g.readField("div"), _load(d, "div.f32", np.float32))
+    assert R["readTextureRejectsHalfFloat"] is True
+    u0, v0 = _load(d, "ic_4242_u.f64", np.float64), _load(d, "ic_4242_v.f64", np.float64)
+    for name in ("jacobi", "red-black"):
+        with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver=name, pressureIterations=200) as g:
+            g.uploadVelocity(u0, v0)
+            before = g.maxDivergence()
+            g.step(0.01)
+            after = g.maxDivergence()
+        assert R["projection"][name] == {"before": before, "after": after}
+    u0, v0 = _load(d, "ic_777_u.f64", np.float64), _load(d, "ic_777_v.f64", np.float64)
+    for name in ("jacobi", "red-black", "multigrid"):
+        budget = [0, 1, 2, 3, 4, 5, 6, 8, 10, 12] if name == "multigrid" else [0, 1, 2, 5, 10, 20, 40, 80, 160, 320]
+        with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver=name) as g:
+            g.uploadVelocity(u0, v0)
+            assert R["curves"][name] == [g.poissonResidualAfter(it) for it in budget]
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver="red-black", pressureIterations=400,
+                                 advection="semi-lagrangian") as g:
+        g.uploadVelocity(tu, tv)
+        for _ in range(10):
+            g.step(dt)
+        vel = g.readVelocity()
+        assert R["fullStepMaxDiv"] == g.maxDivergence()
+    assert same(vel["u"], _load(d, "full_u.f32", np.float32)) and same(vel["v"], _load(d, "full_v.f32", np.float32))
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver="multigrid", pressureIterations=4,
+                                 advection="maccormack") as g:
+        g.uploadVelocity(tu, tv)
+        for _ in range(10):
+            g.step(dt)
+        vel = g.readVelocity()
+    assert same(vel["u"], _load(d, "mc_u.f32", np.float32)) and same(vel["v"], _load(d, "mc_v.f32", np.float32))
+    for scheme in ("semi-lagrangian", "maccormack"):
+        with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver="multigrid", pressureIterations=3,
+                                     advection=scheme) as g:
+            g.uploadVelocity(tu, tv)
+            for _ in range(200):
+                g.step(0.01)
+            vel = g.readVelocity()
+        e = 0.5 * float(np.mean(vel["u"].astype(np.float64) ** 2 + vel["v"].astype(np.float64) ** 2))
+        assert abs(R["schemeEnergy"][scheme] - e) <= 1e-12 * e
+    assert R["schemeEnergy"]["maccormack"] > R["schemeEnergy"]["semi-lagrangian"]
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.01, pressureSolver="multigrid", pressureIterations=20,
+                                 tolerance=1e-6) as g:
+        g.uploadVelocity(tu, tv)
+        g.step(dt)
+        assert same(g.readField("pressure"), _load(d, "tg_p.f64", np.float64))
+        _, E, counts = g.energySpectrum()
+    assert same(E, _load(d, "tg_E.f64", np.float64)) and R["spectrumModes"] == int(np.sum(counts))

@@ -370,6 +370,26 @@ def test_enstrophy_of_taylor_green_is_its_discrete_closed_form(O, n):
     s = math.sin(h / 2) / (h / 2)
     assert abs(O.enstrophy(n, n, u, v) - 0.5 * s ** 2 * math.cos(h / 2) ** 4) < 1e-14
 
+def test_js_lcg_emulation_equals_the_js_engine(O, tmp_path):
+    # the oracle's IEEE-double emulation of gputest's seeded generator against Node
+    # evaluating the reference's own expression (tools/gputest.html:113-114)
+    import os
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed on this machine")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    f = str(tmp_path / "lcg.f64")
+    seeds = [991, 4242, 777, 1]
+    out = subprocess.run([node, os.path.join(root, "tests", "node", "lcg_fields.mjs"), f, str(N),
+                          *map(str, seeds)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    js = np.fromfile(f, np.float64).reshape(len(seeds), 2, N * N)
+    for k, seed in enumerate(seeds):
+        u, v = O.js_random_velocity(N, seed)
+        assert np.array_equal(js[k, 0], u) and np.array_equal(js[k, 1], v), seed
+
 def test_band_forcing_modes_deterministic_and_in_shell(O):
     kx, ky, ph = O.band_modes(1000, 32.0, 2.0)
     kx2, ky2, ph2 = O.band_modes(1000, 32.0, 2.0)
