@@ -919,6 +919,24 @@ k_gradient64(const float* __restrict__ u, const float* __restrict__ v, const Fin
   }
 }
 
+// Pressure extrapolated in time, p_guess = 2 p_n - p_{n-1}, into `out`, and the
+// history advanced (hist <- p_n); with no history yet (have_hist = 0) p_guess = p_n.
+// Two cells per thread with 16-byte accesses (n even).
+__global__ void __launch_bounds__(256)
+k_p_extrapolate(const double* __restrict__ pn, double* __restrict__ hist, double* __restrict__ out, size_t n,
+                int have_hist) {
+  const size_t q = ((size_t)blockIdx.x * 256 + threadIdx.x) * 2;
+  if (q >= n) return;
+  const double2 a = *(const double2*)(pn + q);
+  double2 g = a;
+  if (have_hist) {
+    const double2 b = *(const double2*)(hist + q);
+    g = double2{2.0 * a.x - b.x, 2.0 * a.y - b.y};
+  }
+  *(double2*)(out + q) = g;
+  *(double2*)(hist + q) = a;
+}
+
 // block maxima of |u|, |v| of a field (for a CFL step before any projection ran).
 __global__ void __launch_bounds__(BX * BY)
 k_maxabs(const float* __restrict__ u, const float* __restrict__ v, int nx, int ny, double* __restrict__ part) {

@@ -134,6 +134,8 @@ struct fs_solver_t {
   double* p64[2] = {nullptr, nullptr};
   int pidx = 0;              // p64 buffer holding the last projection's pressure
   bool have_p = false;       // a previous pressure exists (warm start)
+  double* p_hist = nullptr;  // FS_GUESS_EXTRAPOLATED: the pressure of the step before the last
+  bool have_hist = false;    // p_hist holds it
   std::vector<Level> lv;     // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
   bool fused = false;        // LDS-tiled V-cycle in use
   bool bf16 = false;         // multigrid correction and right-hand sides stored as BF16
@@ -224,7 +226,7 @@ struct fs_solver_t {
   int64_t gated_steps = 0;
   // captured step bodies, keyed by what changes the launches they hold
   struct StepGraph {
-    int k, a0, zero_p, cur, keep;
+    int k, a0, zero_p, cur, keep, hist;
     int cur_after;  // velocity buffer after the body
     hipGraphExec_t exec;
   };
@@ -1109,17 +1111,29 @@ void release_guess(fs_solver_t* s) {
   G = fs_solver_t::GuessNet{};
 }
 
-// Multigrid projection by iterative refinement; leaves the pressure in p64[s->pidx].
-int solve_multigrid(fs_solver_t* s, int* cycles_out) {
-  const double tol = s->cfg.tolerance;
-  const int maxc = std::max(0, s->cfg.pressure_iterations);
-  const bool learned = s->cfg.guess == FS_GUESS_LEARNED || s->cfg.guess == FS_GUESS_LEARNED_COARSE ||
-                       s->cfg.guess == FS_GUESS_LEARNED_QUARTER;
-  bool zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
-  int a0 = s->pidx;
+bool learned_guess(const fs_solver_t* s);
+
+// Whether the projection starts from a zero pressure (no warm start configured or none yet).
+bool zero_start(const fs_solver_t* s) {
+  const bool warm = s->cfg.guess == FS_GUESS_PREVIOUS || s->cfg.guess == FS_GUESS_EXTRAPOLATED || learned_guess(s);
+  return !(warm && s->have_p);
+}
+
+// The initial pressure before the refinement iterations: the extrapolation in time
+// (FS_GUESS_EXTRAPOLATED), then the network's increment (the learned guesses) on top of
+// the previous pressure. Moves a0 to the buffer the guess is in.
+int enqueue_guess(fs_solver_t* s, int& a0, bool& zero_p) {
   int rc;
-  if (learned && !s->guess_net.loaded) return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
-  if (learned) {
+  if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !zero_p) {
+    const int out = 1 - (a0 & 1);
+    fsk::k_p_extrapolate<<<(unsigned)((s->n / 2 + 255) / 256), 256, 0, s->stream>>>(
+        s->p64[a0 & 1], s->p_hist, s->p64[out], s->n, s->have_hist ? 1 : 0);
+    LAUNCHCHK();
+    s->have_hist = true;
+    a0 = out;
+  }
+  if (learned_guess(s)) {
+    if (!s->guess_net.loaded) return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
     // the residual the previous pressure leaves, the network's increment on top of it,
     // then refinement from that guess
     if ((rc = enqueue_iteration(s, 0, a0 & 1, zero_p, true))) return rc;
@@ -1127,6 +1141,17 @@ int solve_multigrid(fs_solver_t* s, int* cycles_out) {
     a0 = 1 - (a0 & 1);
     zero_p = false;
   }
+  return FS_OK;
+}
+
+// Multigrid projection by iterative refinement; leaves the pressure in p64[s->pidx].
+int solve_multigrid(fs_solver_t* s, int* cycles_out) {
+  const double tol = s->cfg.tolerance;
+  const int maxc = std::max(0, s->cfg.pressure_iterations);
+  bool zero_p = zero_start(s);
+  int a0 = s->pidx;
+  int rc;
+  if ((rc = enqueue_guess(s, a0, zero_p))) return rc;
   if (!(tol > 0)) {
     // k V-cycles: residual(0), V(0), ..., residual(k-1), V(k-1), residual(k); no host round trip
     for (int it = 0; it <= maxc; it++)
@@ -1354,7 +1379,7 @@ bool gated_eligible(const fs_solver_t* s) {
 // iterations k, the pressure buffer the step starts from, a zero initial pressure,
 // the velocity buffer, and the buffer the pressure is left in.
 struct BodyKey {
-  int k, a0, zero_p, cur, keep;
+  int k, a0, zero_p, cur, keep, hist;
 };
 
 // The step after its head: advection, diffusion, forcing, disk; the learned guess;
@@ -1373,13 +1398,9 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
   if ((rc = enqueue_solid(s))) return rc;
   int a0 = key.a0;
   bool zero_p = key.zero_p != 0;
-  if (learned_guess(s)) {
-    // the residual the previous pressure leaves, the network's increment on top of it
-    if ((rc = enqueue_iteration(s, 0, a0 & 1, zero_p, true))) return rc;
-    if ((rc = run_guess(s, 1 - (a0 & 1)))) return rc;
-    a0 = 1 - (a0 & 1);
-    zero_p = false;
-  }
+  const bool hist0 = s->have_hist;
+  if ((rc = enqueue_guess(s, a0, zero_p))) return rc;
+  if (key.hist != (hist0 ? 1 : 0)) return fail(FS_E_DEVICE, "step body key out of date");
   const double tol = s->cfg.tolerance;
   for (int it = 0; it <= key.k && !rc; it++) {
     s->gate_r = fsk::Gate{s->gate_flag, tol, it > 0 ? it - 1 : -1, it};
@@ -1408,11 +1429,13 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
 // The body as a graph, captured on first use of its key and replayed after.
 int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
   for (const auto& g : s->graphs)
-    if (g.k == key.k && g.a0 == key.a0 && g.zero_p == key.zero_p && g.cur == key.cur && g.keep == key.keep) {
+    if (g.k == key.k && g.a0 == key.a0 && g.zero_p == key.zero_p && g.cur == key.cur && g.keep == key.keep &&
+        g.hist == key.hist) {
       HIPCHK(hipGraphLaunch(g.exec, s->stream));
       s->graph_launches++;
       s->cur = g.cur_after;
       s->have_maxabs = true;
+      if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) s->have_hist = true;
       return FS_OK;
     }
   if (s->graphs.size() >= 32) destroy_graphs(s);  // a drifting prediction: start afresh
@@ -1428,7 +1451,7 @@ int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(FS_E_DEVICE, "instantiating the step graph failed: %s", hipGetErrorString(e));
-  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, s->cur, exec});
+  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, key.hist, s->cur, exec});
   s->graph_captures++;
   HIPCHK(hipGraphLaunch(exec, s->stream));
   s->graph_launches++;
@@ -1452,9 +1475,12 @@ int step_gated(fs_solver_t* s, bool use_cfl, double value) {
   BodyKey key{};
   key.k = std::min(pred + GATE_SPARE, maxc);
   key.a0 = s->pidx;
-  key.zero_p = !((s->cfg.guess == FS_GUESS_PREVIOUS || learned) && s->have_p);
+  key.zero_p = zero_start(s) ? 1 : 0;
   key.cur = s->cur;
-  const int a0e = learned ? 1 - (s->pidx & 1) : s->pidx;
+  key.hist = s->have_hist ? 1 : 0;
+  int a0e = s->pidx;
+  if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) a0e = 1 - (a0e & 1);
+  if (learned) a0e = 1 - (a0e & 1);
   key.keep = (a0e + pred + 1) & 1;
   if ((rc = enqueue_head(s, use_cfl, value, key.k + 1))) return rc;
   if (s->use_graph && !s->solid_on) rc = launch_gated_body_graph(s, key);
@@ -1681,6 +1707,7 @@ void release(fs_solver_t* s) {
   for (auto e : s->ev_pool) (void)hipEventDestroy(e);
   dfree(s->gate_flag);
   dfree(s->modes_dev);
+  dfree(s->p_hist);
   hfree(s->ring);
   release_guess(s);
   dfree(s->spec_Z);
@@ -2511,8 +2538,10 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     return fail(FS_E_UNSUPPORTED, "BF16 multigrid storage needs the fused multigrid V-cycle (mg_fused = 1)");
   if (c.guess == FS_GUESS_LEARNED || c.guess == FS_GUESS_LEARNED_COARSE || c.guess == FS_GUESS_LEARNED_QUARTER)
     return fail(FS_E_INVALID_ARG, "create with guess zero or previous, then fs_load_guess and fs_set_guess(learned)");
-  if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
+  if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS && c.guess != FS_GUESS_EXTRAPOLATED)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", c.guess);
+  if (c.guess == FS_GUESS_EXTRAPOLATED && c.pressure_solver != FS_P_MULTIGRID)
+    return fail(FS_E_UNSUPPORTED, "the extrapolated guess needs the multigrid solver");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0) return fail(FS_E_DEVICE, "no HIP device available");
@@ -2555,6 +2584,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   }
   if (!rc) rc = dalloc(&s->p64[0], s->n);
   if (!rc) rc = dalloc(&s->p64[1], s->n);
+  if (!rc && c.guess == FS_GUESS_EXTRAPOLATED) rc = dalloc(&s->p_hist, s->n);
   if (!rc) rc = build_levels(s);
   s->mp_count = grid_blocks(s->nx, s->ny);
   if (!rc) rc = dalloc(&s->mpart, (size_t)2 * s->mp_count);
@@ -2608,6 +2638,7 @@ int fs_upload_velocity_f32(void* h, const float* u, const float* v, size_t n) {
   if ((rc = sync_stream(s))) return rc;
   s->have_maxabs = false;
   s->have_p = false;
+  s->have_hist = false;
   s->pred_known = false;
   return FS_OK;
 }
@@ -2825,8 +2856,15 @@ int fs_set_guess(void* h, int guess) {
   fs_solver_t* s = (fs_solver_t*)h;
   const bool learned = guess == FS_GUESS_LEARNED || guess == FS_GUESS_LEARNED_COARSE ||
                        guess == FS_GUESS_LEARNED_QUARTER;
-  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && !learned)
+  if (guess != FS_GUESS_ZERO && guess != FS_GUESS_PREVIOUS && guess != FS_GUESS_EXTRAPOLATED && !learned)
     return fail(FS_E_INVALID_ARG, "unknown guess mode %d", guess);
+  if (guess == FS_GUESS_EXTRAPOLATED && s->cfg.pressure_solver != FS_P_MULTIGRID)
+    return fail(FS_E_UNSUPPORTED, "the extrapolated guess needs the multigrid solver");
+  if (guess == FS_GUESS_EXTRAPOLATED && !s->p_hist) {
+    if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+    if ((rc = dalloc(&s->p_hist, s->n))) return rc;
+    s->have_hist = false;
+  }
   if (learned && !s->guess_net.loaded)
     return fail(FS_E_INVALID_ARG, "the learned guess needs its weights: call fs_load_guess first");
   if (guess == FS_GUESS_LEARNED_COARSE && s->nx % 256 != 0)

@@ -22,7 +22,8 @@ FS_E_UNSUPPORTED = -6
 ADVECTION = {"semi-lagrangian": 0, "maccormack": 1}
 PRESSURE = {"jacobi": 0, "red-black": 1, "multigrid": 2}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
-GUESS = {"zero": 0, "previous": 1, "learned": 2, "learned-coarse": 3, "learned-quarter": 4}
+GUESS = {"zero": 0, "previous": 1, "learned": 2, "learned-coarse": 3, "learned-quarter": 4,
+         "extrapolated": 5}
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,

@@ -19,7 +19,8 @@ const addon = require("./fluidsim.node");
 
 const ADVECTION = { "semi-lagrangian": 0, maccormack: 1 };
 const PRESSURE = { jacobi: 0, "red-black": 1, multigrid: 2 };
-const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-quarter": 4 };
+const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-quarter": 4,
+  extrapolated: 5 };
 // fs_run_kernel ids (include/fluidsim.h)
 const KERNEL = { "advect-sl": 0, "advect-mc": 1, divergence: 2, diffuse: 3, project: 4, gradient: 5, solid: 6,
   forcing: 7 };

@@ -49,12 +49,16 @@ enum { FS_MG_FP32 = 0, FS_MG_FP16 = 1, FS_MG_BF16 = 2 };
  * half / quarter resolution on the 2x2 / 4x4 averages of its inputs, its increment
  * interpolated bilinearly (grid size a multiple of 256 / 512); no counterpart in the
  * reference, which has no learned guess. */
+/* FS_GUESS_EXTRAPOLATED: the last two pressures extrapolated linearly in time,
+ * 2 p_n - p_{n-1} (the previous pressure until two exist); no counterpart in the
+ * reference either. */
 enum {
   FS_GUESS_ZERO = 0,
   FS_GUESS_PREVIOUS = 1,
   FS_GUESS_LEARNED = 2,
   FS_GUESS_LEARNED_COARSE = 3,
-  FS_GUESS_LEARNED_QUARTER = 4
+  FS_GUESS_LEARNED_QUARTER = 4,
+  FS_GUESS_EXTRAPOLATED = 5
 };
 
 /* readable fields for fs_read_field */

@@ -34,6 +34,8 @@ FORCED = dict(kf=8.0, width=1.0, amp=0.1, drag=0.02, seed=1000)
     (256, "zero", None, "fp32"),
     (256, "previous", FORCED, "fp32"),
     (256, "previous", FORCED, "bf16"),
+    (256, "extrapolated", FORCED, "fp32"),
+    (1024, "extrapolated", None, "fp32"),
     (256, "learned", None, "fp32"),
     (512, "learned-coarse", FORCED, "bf16"),
     (1024, "zero", None, "fp32"),
