@@ -252,6 +252,7 @@ k_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __res
 // starts from, so the corrected nodes (tile + 1) back-sample forward nodes on tile + 2
 // (AF), whose traces read u, v on tile + 3 (AU).
 constexpr int AT = 32, AF = 2, AU = AF + 1;
+constexpr int ANT = 512;  // threads per tile
 constexpr int AUW = AT + 2 * AU, AFW = AT + 2 * AF, AMW = AT + 2;
 
 struct AdvWin {
@@ -362,6 +363,35 @@ __device__ __forceinline__ float fwd_node(const AdvWin& u, const AdvWin& v, int 
   return bilerp_w<FAST, WRAP>(v, nx, ny, Xb - 0.5f, Yb);
 }
 
+// fwd_node, and the range of the four nodes its final sample reads: the bounds the
+// MacCormack limiter of this node takes (interpBounds at the same departure point,
+// src/cpu/advect.js:113-121, :166-183)
+template <int COMP, bool FAST, bool WRAP = true>
+__device__ __forceinline__ float fwd_node_bounds(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j,
+                                                 float cx, float cy, float& lo, float& hi) {
+  float Xb, Yb, u1, v1, gx, gy;
+  if (COMP == 0) {
+    vel_u_face<FAST, WRAP>(u, v, nx, ny, i, j, u1, v1);
+    trace2_w<FAST, WRAP>(u, v, nx, ny, (float)i, j + 0.5f, u1, v1, cx, cy, Xb, Yb);
+    gx = Xb;
+    gy = Yb - 0.5f;
+  } else {
+    vel_v_face<FAST, WRAP>(u, v, nx, ny, i, j, u1, v1);
+    trace2_w<FAST, WRAP>(u, v, nx, ny, i + 0.5f, (float)j, u1, v1, cx, cy, Xb, Yb);
+    gx = Xb - 0.5f;
+    gy = Yb;
+  }
+  const AdvWin& a = COMP == 0 ? u : v;
+  float fi = floorf(gx), fj = floorf(gy);
+  float fx = gx - fi, fy = gy - fj;
+  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)fi, (int)fj);
+  lo = fminf(fminf(q.x, q.y), fminf(q.z, q.w));
+  hi = fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
+  float top = fmaf(q.y - q.x, fx, q.x);
+  float bot = fmaf(q.w - q.z, fx, q.z);
+  return fmaf(bot - top, fy, top);
+}
+
 // forward value at one node, out of line (the rare path)
 template <int COMP>
 __device__ __noinline__ float fwd_node_far(const AdvWin u, const AdvWin v, int nx, int ny, int i, int j, float cx,
@@ -406,22 +436,27 @@ __device__ __forceinline__ float back_w(const float* __restrict__ sf, int fox, i
 // kernel). FAST: every sample is known to lie in its window.
 template <bool FAST, bool WRAP>
 __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, float* __restrict__ sfu,
-                                            float* __restrict__ sfv, float* __restrict__ smu, float* __restrict__ smv,
+                                            float* __restrict__ sfv, float4* __restrict__ sbd,
+                                            float* __restrict__ smu, float* __restrict__ smv,
                                             const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
                                             int tx0, int ty0, float cx, float cy) {
   const int uox = wu.ox, uoy = wu.oy;
   // ---- forward pass on the tile and an AF ring; nodes at wrapped indices, as the
   // forward kernel evaluates them (its nodes are the wrapped cells of the grid)
   const int fox = tx0 - AF, foy = ty0 - AF;
-  for (int q = threadIdx.x; q < AFW * AFW; q += 256) {
+  for (int q = threadIdx.x; q < AFW * AFW; q += ANT) {
     int ly = q / AFW, lx = q - ly * AFW;
     int i = WRAP ? wrapi(fox + lx, nx) : fox + lx, j = WRAP ? wrapi(foy + ly, ny) : foy + ly;
-    sfu[q] = fwd_node<0, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy);
-    sfv[q] = fwd_node<1, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy);
+    // the forward trace from a node is also the limiter's trace at that node: keep the
+    // bounds of its departure point (lo u, hi u, lo v, hi v)
+    float4 bd;
+    sfu[q] = fwd_node_bounds<0, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy, bd.x, bd.y);
+    sfv[q] = fwd_node_bounds<1, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy, bd.z, bd.w);
+    sbd[q] = bd;
   }
   __syncthreads();
   // ---- correction + limiter on the tile and a 1-cell ring
-  for (int q = threadIdx.x; q < AMW * AMW; q += 256) {
+  for (int q = threadIdx.x; q < AMW * AMW; q += ANT) {
     int ly = q / AMW, lx = q - ly * AMW;
     int i = WRAP ? wrapi(tx0 - 1 + lx, nx) : tx0 - 1 + lx, j = WRAP ? wrapi(ty0 - 1 + ly, ny) : ty0 - 1 + ly;
     // this node's place in the u, v window and in the forward window
@@ -440,8 +475,14 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
       float back = back_w<0, FAST, WRAP>(sfu, fox, foy, wu, wv, nx, ny, Xb, Yb - 0.5f, cx, cy);
       float fo = inf ? sfu[wjf * AFW + wif] : fwd_node_far<0>(wu, wv, nx, ny, i, j, cx, cy);
       float c = fmaf(0.5f, u1 - back, fo);
-      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
-      bounds4_w<FAST, WRAP>(wu, nx, ny, Xb, Yb - 0.5f, lo, hi);
+      if (inf) {
+        const float4 bd = sbd[wjf * AFW + wif];
+        lo = bd.x;
+        hi = bd.y;
+      } else {
+        trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
+        bounds4_w<FAST, WRAP>(wu, nx, ny, Xb, Yb - 0.5f, lo, hi);
+      }
       smu[q] = c < lo ? lo : (c > hi ? hi : c);
     }
     {  // v face at (i+1/2, j)
@@ -452,21 +493,28 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
       float back = back_w<1, FAST, WRAP>(sfv, fox, foy, wu, wv, nx, ny, Xb - 0.5f, Yb, cx, cy);
       float fo = inf ? sfv[wjf * AFW + wif] : fwd_node_far<1>(wu, wv, nx, ny, i, j, cx, cy);
       float c = fmaf(0.5f, v1 - back, fo);
-      trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
-      bounds4_w<FAST, WRAP>(wv, nx, ny, Xb - 0.5f, Yb, lo, hi);
+      if (inf) {
+        const float4 bd = sbd[wjf * AFW + wif];
+        lo = bd.z;
+        hi = bd.w;
+      } else {
+        trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
+        bounds4_w<FAST, WRAP>(wv, nx, ny, Xb - 0.5f, Yb, lo, hi);
+      }
       smv[q] = c < lo ? lo : (c > hi ? hi : c);
     }
   }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(ANT)
 k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
                     float* __restrict__ ov, int nx, int ny, int tiles_x, int nblocks,
                     const StepCoef* __restrict__ cf, int diffuse) {
   __shared__ float su[AUW * AUW], sv[AUW * AUW];
   __shared__ float sfu[AFW * AFW], sfv[AFW * AFW];
+  __shared__ float4 sbd[AFW * AFW];  // limiter bounds at each forward node's departure point
   __shared__ float smu[AMW * AMW], smv[AMW * AMW];
-  __shared__ float wmax[2][4];
+  __shared__ float wmax[2][ANT / 64];
   int b = (int)blockIdx.x;
   if ((nblocks & 7) == 0) b = (b & 7) * (nblocks >> 3) + (b >> 3);
   const int tx0 = (b % tiles_x) * AT, ty0 = (b / tiles_x) * AT;
@@ -475,7 +523,7 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
   // the largest |u|, |v| of the window
   const int uox = tx0 - AU, uoy = ty0 - AU;
   float mu = 0.f, mv = 0.f;
-  for (int q = threadIdx.x; q < AUW * AUW; q += 256) {
+  for (int q = threadIdx.x; q < AUW * AUW; q += ANT) {
     int ly = q / AUW, lx = q - ly * AUW;
     size_t g = (size_t)wrapi(uoy + ly, ny) * nx + wrapi(uox + lx, nx);
     float a = u[g], c = v[g];
@@ -494,8 +542,13 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
     wmax[1][threadIdx.x >> 6] = mv;
   }
   __syncthreads();
-  mu = fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3]));
-  mv = fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3]));
+  mu = wmax[0][0];
+  mv = wmax[1][0];
+#pragma unroll
+  for (int w = 1; w < ANT / 64; w++) {
+    mu = fmaxf(mu, wmax[0][w]);
+    mv = fmaxf(mv, wmax[1][w]);
+  }
   const AdvWin wu{su, uox, uoy, AUW, u}, wv{sv, uox, uoy, AUW, v};
   // every velocity a trace reads comes from the window, so a trace moves at most
   // cx max|u| by cy max|v| cells; the rings hold every sample for moves up to one
@@ -504,15 +557,15 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
   // a window that does not cross the periodic seam needs no wrapped indices
   const bool inner = uox >= 0 && uoy >= 0 && uox + AUW <= nx && uoy + AUW <= ny;
   if (fast && inner)
-    advect_tile<true, false>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<true, false>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
   else if (fast)
-    advect_tile<true, true>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<true, true>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
   else
-    advect_tile<false, true>(wu, wv, sfu, sfv, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<false, true>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
   __syncthreads();
   // ---- diffusion (or the corrected field itself) on the tile
   const float a = cf->da, bb = cf->db;
-  for (int q = threadIdx.x; q < AT * AT; q += 256) {
+  for (int q = threadIdx.x; q < AT * AT; q += ANT) {
     int iy = q / AT, ix = q - iy * AT;
     int i = tx0 + ix, j = ty0 + iy;
     if (i >= nx || j >= ny) continue;

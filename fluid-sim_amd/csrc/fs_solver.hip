@@ -494,7 +494,7 @@ int enqueue_advection(fs_solver_t* s) {
 int enqueue_advect_diffuse_fused(fs_solver_t* s) {
   const int c = s->cur;
   const int tx = (s->nx + fsk::AT - 1) / fsk::AT, ty = (s->ny + fsk::AT - 1) / fsk::AT, nb = tx * ty;
-  int rc = launch(s, PK_ADVECT_FUSED, 16.0 * s->n, fsk::k_advect_mc_diffuse, dim3((unsigned)nb), dim3(256),
+  int rc = launch(s, PK_ADVECT_FUSED, 16.0 * s->n, fsk::k_advect_mc_diffuse, dim3((unsigned)nb), dim3(fsk::ANT),
                   (const float*)s->u[c], (const float*)s->v[c], s->u[1 - c], s->v[1 - c], s->nx, s->ny, tx, nb,
                   (const fsk::StepCoef*)s->coef, s->cfg.nu > 0 ? 1 : 0);
   if (rc) return rc;
