@@ -219,6 +219,80 @@ k_advect_mc(const float* __restrict__ u, const float* __restrict__ v, const floa
   }
 }
 
+// ----------------------------------------------------------- dye and splats
+
+// Semi-Lagrangian advection of the cell-centred rgba dye (ADVECT_SCALAR,
+// src/gpu/shaders.js:234-266): the RK2 trace from the centre, a bilinear sample of the
+// four channels, the fade, clamped to [0, 1].
+__global__ void __launch_bounds__(BX * BY)
+k_advect_dye(const float* __restrict__ u, const float* __restrict__ v, const float4* __restrict__ q,
+             float4* __restrict__ out, int nx, int ny, const StepCoef* __restrict__ cf, float fade) {
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  float Xb, Yb;
+  trace(u, v, nx, ny, i + 0.5f, j + 0.5f, cf->cx, cf->cy, Xb, Yb);
+  const float gx = Xb - 0.5f, gy = Yb - 0.5f;
+  const float fi = floorf(gx), fj = floorf(gy);
+  const int i0 = wrapi((int)fi, nx), j0 = wrapi((int)fj, ny), i1 = nxt(i0, nx), j1 = nxt(j0, ny);
+  const float fx = gx - fi, fy = gy - fj;
+  const float4 A = q[(size_t)j0 * nx + i0], B = q[(size_t)j0 * nx + i1];
+  const float4 Cc = q[(size_t)j1 * nx + i0], D = q[(size_t)j1 * nx + i1];
+  auto lerp2 = [&](float a, float b, float c, float d) {
+    float top = fmaf(b - a, fx, a), bot = fmaf(d - c, fx, c);
+    float r = fmaf(bot - top, fy, top) * (1.f - fade);
+    return r < 0.f ? 0.f : (r > 1.f ? 1.f : r);
+  };
+  out[(size_t)j * nx + i] = float4{lerp2(A.x, B.x, Cc.x, D.x), lerp2(A.y, B.y, Cc.y, D.y),
+                                   lerp2(A.z, B.z, Cc.z, D.z), lerp2(A.w, B.w, Cc.w, D.w)};
+}
+
+// Splats (SPLAT, src/gpu/shaders.js:474-511), applied in order to each cell: the weight
+// g = exp(-|d|^2 / max(radius^2, 1e-12)) of the periodic shortest separation d of the
+// cell centre from the splat (round half to even, as WGSL's round); a force splat adds
+// (dx g, dy g) to the cell's u and v faces, a dye splat (r g, g g, b g, 0) to its dye.
+// Every splat carries its own parameters (the reference binds the first one's for all,
+// src/gpu/solver.js:546-553 against :745-768).
+struct Splat {
+  float x, y, dx, dy, r, g, b, radius;
+  int is_dye;
+};
+constexpr int MAX_SPLATS = 16;
+struct SplatSet {
+  Splat s[MAX_SPLATS];
+  int n;
+};
+__global__ void __launch_bounds__(BX * BY)
+k_splat(float* __restrict__ u, float* __restrict__ v, float4* __restrict__ dye, int nx, int ny, float dx, float dy,
+        const SplatSet ss) {
+  const int2 bk = xcd_block2();
+  int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const float px = ((float)i + 0.5f) * dx, py = ((float)j + 0.5f) * dy;
+  const float Lx = (float)nx * dx, Ly = (float)ny * dy;
+  const size_t k = (size_t)j * nx + i;
+  float uu = u[k], vv = v[k];
+  float4 c = dye ? dye[k] : float4{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < ss.n; m++) {
+    const Splat& sp = ss.s[m];
+    float ddx = px - sp.x, ddy = py - sp.y;
+    ddx = ddx - Lx * rintf(ddx / Lx);
+    ddy = ddy - Ly * rintf(ddy / Ly);
+    const float w = expf(-(ddx * ddx + ddy * ddy) / fmaxf(sp.radius * sp.radius, 1e-12f));
+    if (sp.is_dye) {
+      c.x += sp.r * w;
+      c.y += sp.g * w;
+      c.z += sp.b * w;
+    } else {
+      uu += sp.dx * w;
+      vv += sp.dy * w;
+    }
+  }
+  u[k] = uu;
+  v[k] = vv;
+  if (dye) dye[k] = c;
+}
+
 // Explicit 5-point diffusion of both components (src/cpu/solver.js:56-69;
 // DIFFUSE src/gpu/shaders.js:401-425); a = nu dt/dx^2, b = nu dt/dy^2.
 __global__ void __launch_bounds__(BX * BY)

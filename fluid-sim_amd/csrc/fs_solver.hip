@@ -134,6 +134,9 @@ struct fs_solver_t {
   double* p64[2] = {nullptr, nullptr};
   int pidx = 0;              // p64 buffer holding the last projection's pressure
   bool have_p = false;       // a previous pressure exists (warm start)
+  float4* dye[2] = {nullptr, nullptr};  // rgba dye at the cell centres (allocated on first use)
+  int dye_cur = 0;
+  std::vector<fsk::Splat> splats;         // splats of the step being enqueued
   double* p_hist = nullptr;  // FS_GUESS_EXTRAPOLATED: the pressure of the step before the last
   bool have_hist = false;    // p_hist holds it
   std::vector<Level> lv;     // multigrid hierarchy; lv[0].x = e, lv[0].rhs = r
@@ -226,8 +229,9 @@ struct fs_solver_t {
   int64_t gated_steps = 0;
   // captured step bodies, keyed by what changes the launches they hold
   struct StepGraph {
-    int k, a0, zero_p, cur, keep, hist;
+    int k, a0, zero_p, cur, keep, hist, dye;
     int cur_after;  // velocity buffer after the body
+    int dye_after;
     hipGraphExec_t exec;
   };
   bool use_graph = true;
@@ -528,6 +532,32 @@ int enqueue_forcing(fs_solver_t* s) {
   return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny,
                 (const fsk::ModeSet*)s->modes_dev, (const float4*)s->ftab_col, (const float4*)s->ftab_row,
                 (const fsk::StepCoef*)s->coef);
+}
+
+// The step's splats (after the forcing, src/gpu/solver.js:546-554), up to MAX_SPLATS per launch.
+int enqueue_splats(fs_solver_t* s) {
+  for (size_t at = 0; at < s->splats.size(); at += fsk::MAX_SPLATS) {
+    fsk::SplatSet ss{};
+    ss.n = (int)std::min((size_t)fsk::MAX_SPLATS, s->splats.size() - at);
+    for (int m = 0; m < ss.n; m++) ss.s[m] = s->splats[at + m];
+    fsk::k_splat<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[s->cur], s->v[s->cur],
+                                                             s->dye[0] ? s->dye[s->dye_cur] : nullptr, s->nx, s->ny,
+                                                             (float)s->dx, (float)s->dy, ss);
+    LAUNCHCHK();
+  }
+  return FS_OK;
+}
+
+// The dye through the projected velocity (the step's last pass, src/gpu/solver.js:575-584).
+int enqueue_dye(fs_solver_t* s) {
+  if (!s->dye[0]) return FS_OK;
+  const int c = s->dye_cur;
+  fsk::k_advect_dye<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
+      s->u[s->cur], s->v[s->cur], s->dye[c], s->dye[1 - c], s->nx, s->ny, (const fsk::StepCoef*)s->coef,
+      (float)s->cfg.dye_fade);
+  LAUNCHCHK();
+  s->dye_cur = 1 - c;
+  return FS_OK;
 }
 
 // Direct-forcing coupling of the disk, after forcing and before the projection
@@ -1299,10 +1329,12 @@ int do_step(fs_solver_t* s) {
     if ((rc = enqueue_diffusion(s))) return rc;
   }
   if ((rc = enqueue_forcing(s))) return rc;
+  if ((rc = enqueue_splats(s))) return rc;
   if ((rc = enqueue_solid(s))) return rc;
   int cycles = 0;
   int prc = enqueue_projection(s, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
+  if ((rc = enqueue_dye(s))) return rc;
   s->last_vcycles = cycles;
   s->total_vcycles += cycles;
   s->steps++;
@@ -1379,7 +1411,7 @@ bool gated_eligible(const fs_solver_t* s) {
 // iterations k, the pressure buffer the step starts from, a zero initial pressure,
 // the velocity buffer, and the buffer the pressure is left in.
 struct BodyKey {
-  int k, a0, zero_p, cur, keep, hist;
+  int k, a0, zero_p, cur, keep, hist, dye;
 };
 
 // The step after its head: advection, diffusion, forcing, disk; the learned guess;
@@ -1395,6 +1427,7 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
     if ((rc = enqueue_diffusion(s))) return rc;
   }
   if ((rc = enqueue_forcing(s))) return rc;
+  if ((rc = enqueue_splats(s))) return rc;
   if ((rc = enqueue_solid(s))) return rc;
   int a0 = key.a0;
   bool zero_p = key.zero_p != 0;
@@ -1423,17 +1456,18 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
   f.coef = s->coef;
   if ((rc = launch_gradient64(s, &f))) return rc;
   commit_gradient64(s);
-  return FS_OK;
+  return enqueue_dye(s);
 }
 
 // The body as a graph, captured on first use of its key and replayed after.
 int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
   for (const auto& g : s->graphs)
     if (g.k == key.k && g.a0 == key.a0 && g.zero_p == key.zero_p && g.cur == key.cur && g.keep == key.keep &&
-        g.hist == key.hist) {
+        g.hist == key.hist && g.dye == key.dye) {
       HIPCHK(hipGraphLaunch(g.exec, s->stream));
       s->graph_launches++;
       s->cur = g.cur_after;
+      s->dye_cur = g.dye_after;
       s->have_maxabs = true;
       if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) s->have_hist = true;
       return FS_OK;
@@ -1451,7 +1485,7 @@ int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(FS_E_DEVICE, "instantiating the step graph failed: %s", hipGetErrorString(e));
-  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, key.hist, s->cur, exec});
+  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, key.hist, key.dye, s->cur, s->dye_cur, exec});
   s->graph_captures++;
   HIPCHK(hipGraphLaunch(exec, s->stream));
   s->graph_launches++;
@@ -1478,12 +1512,13 @@ int step_gated(fs_solver_t* s, bool use_cfl, double value) {
   key.zero_p = zero_start(s) ? 1 : 0;
   key.cur = s->cur;
   key.hist = s->have_hist ? 1 : 0;
+  key.dye = s->dye[0] ? s->dye_cur : -1;
   int a0e = s->pidx;
   if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) a0e = 1 - (a0e & 1);
   if (learned) a0e = 1 - (a0e & 1);
   key.keep = (a0e + pred + 1) & 1;
   if ((rc = enqueue_head(s, use_cfl, value, key.k + 1))) return rc;
-  if (s->use_graph && !s->solid_on) rc = launch_gated_body_graph(s, key);
+  if (s->use_graph && !s->solid_on && s->splats.empty()) rc = launch_gated_body_graph(s, key);
   else rc = enqueue_gated_body(s, key);
   if (rc) return rc;
   hipEvent_t ev = nullptr;
@@ -1708,6 +1743,8 @@ void release(fs_solver_t* s) {
   dfree(s->gate_flag);
   dfree(s->modes_dev);
   dfree(s->p_hist);
+  dfree(s->dye[0]);
+  dfree(s->dye[1]);
   hfree(s->ring);
   release_guess(s);
   dfree(s->spec_Z);
@@ -2505,6 +2542,7 @@ void fs_config_default(fs_config* c) {
   c->mg_fused = 1;
   c->gated = 1;
   c->graph = 1;
+  c->dye_fade = 0.0;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -2680,6 +2718,11 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
     case FS_F_U: src = s->u[s->cur]; need = s->n * sizeof(float); break;
     case FS_F_V: src = s->v[s->cur]; need = s->n * sizeof(float); break;
     case FS_F_DIV: src = s->div; need = s->n * sizeof(float); break;
+    case FS_F_DYE:
+      if (!s->dye[0]) return fail(FS_E_INVALID_ARG, "no dye: upload one or splat some first");
+      src = s->dye[s->dye_cur];
+      need = s->n * 4 * sizeof(float);
+      break;
     case FS_F_PRESSURE:
       need = s->n * sizeof(double);
       if (s->cfg.pressure_solver != FS_P_MULTIGRID) {
@@ -2718,16 +2761,66 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
   return sync_stream(s);
 }
 
-int fs_step(void* h, double dt) {
+int fs_step(void* h, double dt) { return fs_step_splats(h, dt, nullptr, 0); }
+
+}  // extern "C"
+
+namespace {
+// Allocates the dye (zero) on first use; captured step bodies then need it too.
+int ensure_dye(fs_solver_t* s) {
+  if (s->dye[0]) return FS_OK;
+  int rc;
+  if ((rc = sync_stream(s))) return rc;
+  destroy_graphs(s);
+  if ((rc = dalloc(&s->dye[0], s->n)) || (rc = dalloc(&s->dye[1], s->n))) return rc;
+  if ((rc = fill_f32(s, (float*)s->dye[0], 4 * s->n, 0.f))) return rc;
+  s->dye_cur = 0;
+  return FS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fs_step_splats(void* h, double dt, const fs_splat* splats, int count) {
   int rc = check_solver(h);
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0, got %g", dt);
+  if (count < 0 || (count > 0 && !splats)) return fail(FS_E_INVALID_ARG, "splats is NULL or count < 0");
   if ((rc = set_device(s))) return rc;
-  if (gated_eligible(s)) return step_gated(s, false, dt);
-  if ((rc = consume_records(s, true))) return rc;
-  if ((rc = enqueue_head(s, false, dt, 0))) return rc;
-  return do_step(s);
+  bool dye_splat = false;
+  for (int m = 0; m < count; m++) {
+    const fs_splat& p = splats[m];
+    if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.dx) || !std::isfinite(p.dy) ||
+        !std::isfinite(p.r) || !std::isfinite(p.g) || !std::isfinite(p.b) || !std::isfinite(p.radius))
+      return fail(FS_E_INVALID_ARG, "splat %d is not finite", m);
+    dye_splat = dye_splat || p.is_dye;
+  }
+  if (dye_splat && (rc = ensure_dye(s))) return rc;
+  s->splats.clear();
+  for (int m = 0; m < count; m++) {
+    const fs_splat& p = splats[m];
+    s->splats.push_back(fsk::Splat{(float)p.x, (float)p.y, (float)p.dx, (float)p.dy, (float)p.r, (float)p.g,
+                                   (float)p.b, (float)p.radius, p.is_dye ? 1 : 0});
+  }
+  if (gated_eligible(s)) {
+    rc = step_gated(s, false, dt);
+  } else if (!(rc = consume_records(s, true)) && !(rc = enqueue_head(s, false, dt, 0))) {
+    rc = do_step(s);
+  }
+  s->splats.clear();
+  return rc;
+}
+
+int fs_upload_dye(void* h, const float* rgba, size_t n) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (!rgba) return fail(FS_E_INVALID_ARG, "rgba is NULL");
+  if (n != 4 * s->n) return fail(FS_E_INVALID_ARG, "expected %zu values (rgba per cell), got %zu", 4 * s->n, n);
+  if ((rc = set_device(s)) || (rc = ensure_dye(s))) return rc;
+  HIPCHK(hipMemcpyAsync(s->dye[s->dye_cur], rgba, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  return sync_stream(s);
 }
 
 int fs_step_cfl(void* h, double cfl) {
@@ -3180,7 +3273,8 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   if (rc) return rc;
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
-  if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE || kernel == FS_K_FORCING) {
+  if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE || kernel == FS_K_FORCING ||
+      kernel == FS_K_ADVECT_DYE) {
     if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0");
     if ((rc = enqueue_head(s, false, dt, 0, false))) return rc;
   }
@@ -3212,6 +3306,10 @@ int fs_run_kernel(void* h, int kernel, double dt) {
     case FS_K_FORCING:
       if (!s->forcing_on) return fail(FS_E_INVALID_ARG, "no forcing is set");
       rc = enqueue_forcing(s);
+      break;
+    case FS_K_ADVECT_DYE:
+      if (!s->dye[0]) return fail(FS_E_INVALID_ARG, "no dye: upload one or splat some first");
+      rc = enqueue_dye(s);
       break;
     case FS_K_GRADIENT:
       if (s->cfg.pressure_solver != FS_P_MULTIGRID)

@@ -24,12 +24,12 @@ PRESSURE = {"jacobi": 0, "red-black": 1, "multigrid": 2}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GUESS = {"zero": 0, "previous": 1, "learned": 2, "learned-coarse": 3, "learned-quarter": 4,
          "extrapolated": 5}
-FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4}
+FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4, "dye": 5}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
                  "advect_fused": 11, "forcing": 12}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6,
-          "forcing": 7}
+          "forcing": 7, "advect-dye": 8}
 
 
 class FsConfig(C.Structure):
@@ -53,7 +53,14 @@ class FsConfig(C.Structure):
         ("adv_fused", C.c_int),
         ("gated", C.c_int),
         ("graph", C.c_int),
+        ("dye_fade", C.c_double),
     ]
+
+
+class FsSplat(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("r", C.c_double), ("g", C.c_double), ("b", C.c_double), ("radius", C.c_double),
+                ("is_dye", C.c_int)]
 
 
 class FsStats(C.Structure):
@@ -142,6 +149,8 @@ _SIGNATURES = {
     "fs_read_velocity": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
     "fs_read_field": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
     "fs_step": (C.c_int, [_VP, C.c_double]),
+    "fs_step_splats": (C.c_int, [_VP, C.c_double, _VP, C.c_int]),
+    "fs_upload_dye": (C.c_int, [_VP, _VP, C.c_size_t]),
     "fs_step_cfl": (C.c_int, [_VP, C.c_double]),
     "fs_sync": (C.c_int, [_VP]),
     "fs_max_divergence": (C.c_int, [_VP, C.POINTER(C.c_double)]),

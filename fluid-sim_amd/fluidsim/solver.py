@@ -72,6 +72,8 @@ class GPUFluidSolver:
             raise ValueError(f"unknown guess mode {guess}")
         if confinement:
             raise ValueError("vorticity confinement is a display-only force and is not built here")
+        if not (0.0 <= dyeFade <= 1.0):
+            raise ValueError("dyeFade must be in [0, 1]")
         cfg = _lib.default_config()
         cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.nu = nx, ny, lx, ly, nu
         cfg.advection = _lib.ADVECTION[advection]
@@ -88,6 +90,7 @@ class GPUFluidSolver:
         cfg.adv_fused = 1 if advFused else 0
         cfg.gated = 1 if gated else 0
         cfg.graph = 1 if graph else 0
+        cfg.dye_fade = float(dyeFade)
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))
@@ -188,21 +191,39 @@ class GPUFluidSolver:
         if name not in _lib.FIELD:
             raise ValueError(f"unknown field {name}; have {', '.join(_lib.FIELD)}")
         dtype = np.float64 if name == "pressure" else np.float32
-        out = np.empty(self.nx * self.ny, dtype)
+        out = np.empty(self.nx * self.ny * (4 if name == "dye" else 1), dtype)
         check(self._L.fs_read_field(self.handle, _lib.FIELD[name], _ptr(out), out.nbytes))
         return out
 
     # ------------------------------------------------------------------ step
     def step(self, dt, splats=()):
-        """step(dt) (src/gpu/solver.js:518-591): enqueued, returns before the device
+        """step(dt, {splats}) (src/gpu/solver.js:518-591): enqueued, returns before the device
         finishes, as the reference's does. With a tolerance the projection is checked
         on the device (stats()["gate_misses"] counts steps that ended above it); a
         step checked on the host (the first after an upload) raises NotConverged if
-        the tolerance was not met within pressureIterations V-cycles."""
-        if splats:
-            raise ValueError("splats are a display-only interaction and are not built here")
+        the tolerance was not met within pressureIterations V-cycles.
+
+        splats: dicts {x, y, dx, dy, r, g, b, radius, isDye} as the reference's
+        (_writeSplat, src/gpu/solver.js:745-757); each keeps its own parameters."""
         self._push_solid()
-        check(self._L.fs_step(self.handle, float(dt)))
+        if splats:
+            arr = (_lib.FsSplat * len(splats))()
+            for a, sp in zip(arr, splats):
+                a.x, a.y = float(sp["x"]), float(sp["y"])
+                a.dx, a.dy = float(sp.get("dx", 0.0) or 0.0), float(sp.get("dy", 0.0) or 0.0)
+                a.r, a.g, a.b = (float(sp.get(c, 0.0) or 0.0) for c in ("r", "g", "b"))
+                a.radius = float(sp.get("radius", 0.2) or 0.2)
+                a.is_dye = 1 if sp.get("isDye") else 0
+            check(self._L.fs_step_splats(self.handle, float(dt), arr, len(splats)))
+        else:
+            check(self._L.fs_step(self.handle, float(dt)))
+
+    def uploadDye(self, rgba):
+        """uploadDye(rgba) (src/gpu/solver.js:317-323): nx*ny*4 values, rgba per cell centre."""
+        a = np.ascontiguousarray(rgba, np.float32).reshape(-1)
+        if a.size != 4 * self.nx * self.ny:
+            raise ValueError(f"expected {4 * self.nx * self.ny} values, got {a.size}")
+        check(self._L.fs_upload_dye(self.handle, _ptr(a), a.size))
 
     def stepCFL(self, cfl=0.5):
         """One step with dt = cfl / (max|u|/dx + max|v|/dy), computed on the device.

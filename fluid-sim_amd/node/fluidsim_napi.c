@@ -142,6 +142,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   if (prop_double(env, argv[0], "mgCoarse", &d)) c.mg_coarse = (int)d;
   if (prop_double(env, argv[0], "guess", &d)) c.guess = (int)d;
   if (prop_double(env, argv[0], "solidMaskWidth", &d)) c.solid_mask_width = d;
+  if (prop_double(env, argv[0], "dyeFade", &d)) c.dye_fade = d;
   if (prop_double(env, argv[0], "device", &d)) c.device = (int)d;
   void* s = NULL;
   int rc = fs_solver_create(&c, &s);
@@ -389,6 +390,7 @@ static napi_value js_read_field(napi_env env, napi_callback_info info) {
   if (!j) return NULL;
   j->field = (int)f;
   j->n = (size_t)dn;
+  if (j->field == FS_F_DYE) j->n *= 4;
   j->bytes = j->n * (j->field == FS_F_PRESSURE ? sizeof(double) : sizeof(float));
   j->a = malloc(j->bytes + 1);
   return start_job(env, argv[0], j);
@@ -486,6 +488,77 @@ static napi_value js_forcing_modes(napi_env env, napi_callback_info info) {
   int rc = fs_set_forcing_modes(h->solver, buf, buf + count, buf + 2 * count, (int)count, amp, drag, drift);
   unlock(h);
   free(buf);
+  if (rc) return throw_fs(env, rc);
+  return NULL;
+}
+
+/* stepSplats(h, dt, [{x, y, dx, dy, r, g, b, radius, isDye}]) */
+static napi_value js_step_splats(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  double dt;
+  NAPI_CALL(env, napi_get_value_double(env, argv[1], &dt));
+  uint32_t count = 0;
+  NAPI_CALL(env, napi_get_array_length(env, argv[2], &count));
+  fs_splat* sp = (fs_splat*)calloc(count + 1, sizeof(fs_splat));
+  if (!sp) {
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  for (uint32_t m = 0; m < count; m++) {
+    napi_value o;
+    if (napi_get_element(env, argv[2], m, &o) != napi_ok) {
+      free(sp);
+      napi_throw_type_error(env, NULL, "splats must be an array of objects");
+      return NULL;
+    }
+    fs_splat* p = &sp[m];
+    p->radius = 0.2;
+    prop_double(env, o, "x", &p->x);
+    prop_double(env, o, "y", &p->y);
+    prop_double(env, o, "dx", &p->dx);
+    prop_double(env, o, "dy", &p->dy);
+    prop_double(env, o, "r", &p->r);
+    prop_double(env, o, "g", &p->g);
+    prop_double(env, o, "b", &p->b);
+    double rad;
+    if (prop_double(env, o, "radius", &rad) && rad) p->radius = rad;
+    bool has = false;
+    napi_value d;
+    if (napi_has_named_property(env, o, "isDye", &has) == napi_ok && has &&
+        napi_get_named_property(env, o, "isDye", &d) == napi_ok) {
+      bool b = false;
+      napi_value bv;
+      if (napi_coerce_to_bool(env, d, &bv) == napi_ok && napi_get_value_bool(env, bv, &b) == napi_ok) p->is_dye = b;
+    }
+  }
+  lock(h);
+  int rc = fs_step_splats(h->solver, dt, sp, (int)count);
+  unlock(h);
+  free(sp);
+  if (rc) return throw_fs(env, rc);
+  return NULL;
+}
+
+/* uploadDye(h, Float32Array(nx*ny*4)) */
+static napi_value js_upload_dye(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  napi_typedarray_type t;
+  size_t len, off;
+  void* data;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off) != napi_ok || t != napi_float32_array) {
+    napi_throw_type_error(env, NULL, "the dye must be a Float32Array of nx*ny*4 rgba values");
+    return NULL;
+  }
+  lock(h);
+  int rc = fs_upload_dye(h->solver, (const float*)data, len);
+  unlock(h);
   if (rc) return throw_fs(env, rc);
   return NULL;
 }
@@ -706,7 +779,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"diagnostics", js_diag}, {"device", js_device},         {"setSolid", js_solid},
       {"solidImpulse", js_solid_impulse}, {"loadGuess", js_load_guess}, {"readField", js_read_field},
       {"energySpectrum", js_spectrum}, {"syncAsync", js_sync_async}, {"runKernel", js_run_kernel},
-      {"setForcingModes", js_forcing_modes},
+      {"setForcingModes", js_forcing_modes}, {"stepSplats", js_step_splats}, {"uploadDye", js_upload_dye},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

@@ -5,7 +5,8 @@
 // constructor options, defaults, methods and Promise-returning readbacks (the
 // copies run on a worker thread; calls on one solver keep their call order).
 // Deliberate differences: unknown selectors throw (the reference silently runs
-// Jacobi / semi-Lagrangian, src/gpu/solver.js:606, :725); vorticity confinement
+// Jacobi / semi-Lagrangian, src/gpu/solver.js:606, :725); each splat keeps its own
+// parameters (the reference binds the first one's, :546-553); vorticity confinement
 // and rendering are display features that are not built. setForcing(modes, opts)
 // is the reference's mode-list forcing; setBandForcing() adds the CPU solver's
 // seeded band forcing (src/cpu/solver.js:179-237). The reference's internal
@@ -23,9 +24,9 @@ const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-
   extrapolated: 5 };
 // fs_run_kernel ids (include/fluidsim.h)
 const KERNEL = { "advect-sl": 0, "advect-mc": 1, divergence: 2, diffuse: 3, project: 4, gradient: 5, solid: 6,
-  forcing: 7 };
+  forcing: 7, "advect-dye": 8 };
 // fs_read_field ids
-const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4 };
+const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4, dye: 5 };
 
 export async function createDevice({ requireTimestamps = false } = {}) {
   const d = addon.device();
@@ -65,6 +66,7 @@ export class GPUFluidSolver {
     }
     if (!(guess in GUESS)) throw new Error(`unknown guess mode ${guess}`);
     if (confinement > 0) throw new Error("vorticity confinement is display-only and not built here");
+    if (!(dyeFade >= 0 && dyeFade <= 1)) throw new Error("dyeFade must be in [0, 1]");
     this.device = device;
     this.nx = nx; this.ny = ny;
     this.dx = lx / nx; this.dy = ly / ny;
@@ -81,7 +83,7 @@ export class GPUFluidSolver {
       advection: ADVECTION[advection],
       pressureSolver: PRESSURE[pressureSolver],
       pressureIterations, tolerance, mgMinSize, mgPre, mgPost, mgCoarse,
-      guess: GUESS[guess], solidMaskWidth,
+      guess: GUESS[guess], solidMaskWidth, dyeFade,
     });
     if (device && device._solvers) device._solvers.add(this);
     // the reference's textures by name (src/gpu/solver.js:111-125, :287-292), for readTexture
@@ -93,6 +95,8 @@ export class GPUFluidSolver {
   get velNext() { return { format: "rgba32float", _field: "velocity" }; }
   get presCur() { return { format: "rgba32float", _field: "pressure" }; }
   get presNext() { return { format: "rgba32float", _field: "pressure" }; }
+  get dyeCur() { return { format: "rgba32float", _field: "dye" }; }
+  get dyeNext() { return { format: "rgba32float", _field: "dye" }; }
 
   get nu() { return this._nu; }
   set nu(v) { addon.setParam(this._h, "nu", v); this._nu = v; }
@@ -130,6 +134,8 @@ export class GPUFluidSolver {
     if (tex._field === "velocity") {
       const { u, v } = await addon.readVelocity(this._h, n);
       for (let i = 0; i < n; i++) { out[i * 4] = u[i]; out[i * 4 + 1] = v[i]; }
+    } else if (tex._field === "dye") {
+      out.set(await addon.readField(this._h, FIELD.dye, n));
     } else {
       const f = await addon.readField(this._h, FIELD[tex._field], n);
       for (let i = 0; i < n; i++) out[i * 4] = f[i];
@@ -159,11 +165,15 @@ export class GPUFluidSolver {
     return { k, E, counts };
   }
 
+  // step(dt, {splats}) (src/gpu/solver.js:518-591); each splat keeps its own parameters
   step(dt, { splats = [] } = {}) {
-    if (splats.length) throw new Error("splats are display-only and not built here");
     this._pushSolid();
-    addon.step(this._h, dt);
+    if (splats.length) addon.stepSplats(this._h, dt, splats);
+    else addon.step(this._h, dt);
   }
+
+  // uploadDye(rgba) (src/gpu/solver.js:317-323): Float32Array(nx*ny*4)
+  uploadDye(rgba) { addon.uploadDye(this._h, rgba); }
 
   stepCFL(cfl = 0.5) { this._pushSolid(); addon.stepCFL(this._h, cfl); }
 

@@ -67,7 +67,8 @@ enum {
   FS_F_V = 1,        /* float[nx*ny], current v (bottom faces) */
   FS_F_DIV = 2,      /* float[nx*ny], divergence written by the last divergence pass */
   FS_F_PRESSURE = 3, /* double[nx*ny], pressure of the last projection */
-  FS_F_RESIDUAL = 4  /* float[nx*ny], level-0 multigrid right-hand side of the last cycle */
+  FS_F_RESIDUAL = 4, /* float[nx*ny], level-0 multigrid right-hand side of the last cycle */
+  FS_F_DYE = 5       /* float[nx*ny*4], rgba dye at the cell centres (after fs_upload_dye or a dye splat) */
 };
 
 /* single kernels for per-kernel parity tests (tools/gputest.html:90-100, :121-129) */
@@ -79,7 +80,8 @@ enum {
   FS_K_PROJECT = 4,     /* one projection with the configured solver */
   FS_K_GRADIENT = 5,    /* velocity <- velocity - grad(pressure) */
   FS_K_SOLID = 6,       /* velocity <- direct forcing toward the disk's rigid motion */
-  FS_K_FORCING = 7      /* velocity <- forcing and drag of the current step index and time, dt */
+  FS_K_FORCING = 7,     /* velocity <- forcing and drag of the current step index and time, dt */
+  FS_K_ADVECT_DYE = 8   /* dye <- semi-Lagrangian(dye) through the velocity, faded and clamped, dt */
 };
 
 /* GPUFluidSolver constructor options, src/gpu/solver.js:64-98 */
@@ -115,7 +117,19 @@ typedef struct fs_config {
   int graph;                  /* 1 (default): a gated step's launches after its first replay as
                                  a captured hipGraph (one per V-cycle count, buffer parity and
                                  initial-pressure kind); 0: launched one by one. Same bits. */
+  double dye_fade;            /* dyeFade: the dye is multiplied by 1 - dye_fade each step; default 0 */
 } fs_config;
+
+/* one splat of step(dt, {splats}) (src/gpu/solver.js:546-554, _writeSplat :745-757): a
+ * Gaussian of `radius` about (x, y) adding (dx, dy) g to the velocity, or with is_dye
+ * (r, g, b) g to the dye */
+typedef struct fs_splat {
+  double x, y;                /* centre, world units */
+  double dx, dy;              /* velocity impulse (force splats) */
+  double r, g, b;             /* colour (dye splats) */
+  double radius;              /* Gaussian radius; the reference's default is 0.2 */
+  int is_dye;
+} fs_splat;
 
 /* per-step statistics (new: the reference reports none) */
 typedef struct fs_stats {
@@ -234,6 +248,14 @@ int fs_read_field(void* solver, int field, void* out, size_t bytes);
 
 /* step(dt) src/gpu/solver.js:518-591: enqueue one step on the solver's stream */
 int fs_step(void* solver, double dt);
+/* step(dt, {splats}) src/gpu/solver.js:518-591: the splats applied in order after the
+ * forcing (SPLAT src/gpu/shaders.js:474-511), each with its own parameters (the reference
+ * binds the first splat's for every pass, src/gpu/solver.js:546-553 against :757-768);
+ * a dye splat allocates the dye. Once a dye exists every step ends by advecting it
+ * (ADVECT_SCALAR src/gpu/shaders.js:234-266). */
+int fs_step_splats(void* solver, double dt, const fs_splat* splats, int count);
+/* uploadDye(rgba) src/gpu/solver.js:317-323: n = nx*ny*4 floats, rgba per cell centre */
+int fs_upload_dye(void* solver, const float* rgba, size_t n);
 /* step with dt = cfl / (max|u|/dx + max|v|/dy), the CFL definition of
  * src/core/grid.js:322-327, evaluated on the device from the last projection */
 int fs_step_cfl(void* solver, double cfl);

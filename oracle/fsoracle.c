@@ -355,6 +355,49 @@ void or_advect_scalar_mc(int nx, int ny, double lx, double ly, const double *u, 
   free(fwd); free(back);
 }
 
+/* ADVECT_SCALAR, src/gpu/shaders.js:234-266: traceBack from the cell centre, sampleC of
+ * each channel, fade, clamp to [0, 1] */
+void or_advect_dye(int nx, int ny, double lx, double ly, const double *u, const double *v, const double *dye,
+                   double *out, double dt, double fade) {
+  G g = {nx, ny, lx / nx, ly / ny};
+  size_t n = (size_t)nx * ny;
+  double *ch = (double *)malloc(sizeof(double) * n), *o = (double *)malloc(sizeof(double) * n);
+  for (int c = 0; c < 4; c++) {
+    for (size_t k = 0; k < n; k++) ch[k] = dye[4 * k + c];
+    or_advect_scalar_sl(nx, ny, lx, ly, u, v, ch, o, dt);
+    for (size_t k = 0; k < n; k++) {
+      double q = o[k] * (1.0 - fade);
+      out[4 * k + c] = q < 0 ? 0 : (q > 1 ? 1 : q);
+    }
+  }
+  (void)g;
+  free(ch);
+  free(o);
+}
+
+/* SPLAT, src/gpu/shaders.js:474-511 */
+void or_splat_apply(int nx, int ny, double lx, double ly, double *u, double *v, double *dye, const or_splat *sp) {
+  double dx = lx / nx, dy = ly / ny;
+  double r2 = sp->radius * sp->radius;
+  if (r2 < 1e-12) r2 = 1e-12;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      double px = (i + 0.5) * dx - sp->x, py = (j + 0.5) * dy - sp->y;
+      px -= lx * nearbyint(px / lx);
+      py -= ly * nearbyint(py / ly);
+      double w = exp(-(px * px + py * py) / r2);
+      size_t k = (size_t)j * nx + i;
+      if (sp->is_dye) {
+        dye[4 * k] += sp->r * w;
+        dye[4 * k + 1] += sp->g * w;
+        dye[4 * k + 2] += sp->b * w;
+      } else {
+        u[k] += sp->dx * w;
+        v[k] += sp->dy * w;
+      }
+    }
+}
+
 /* ------------------------------------------------------------------ diffusion */
 
 /* src/cpu/solver.js:56-69 */

@@ -173,6 +173,22 @@ void   or_mode_forcing(int nx, int ny, double lx, double ly, double *u, double *
                        const double *ky, const double *phase, double amp, double drag, double drift, double t,
                        double dt);
 
+/* the GPU's dye step (ADVECT_SCALAR src/gpu/shaders.js:234-266): each of the four channels
+ * of the cell-centred rgba dye (dye[4 k + c]) advected semi-Lagrangian through (u, v),
+ * times (1 - fade), clamped to [0, 1] */
+void   or_advect_dye(int nx, int ny, double lx, double ly, const double *u, const double *v, const double *dye,
+                     double *out, double dt, double fade);
+
+/* one splat (SPLAT src/gpu/shaders.js:474-511): g = exp(-|d|^2 / max(radius^2, 1e-12)) with
+ * d the periodic shortest separation (round half to even, as WGSL's round) of the cell
+ * centre from (x, y); a dye splat adds (r, g, b, 0) g to the rgba dye, a force splat
+ * adds (dx g, dy g) to the u and v faces of the cell */
+typedef struct {
+  double x, y, dx, dy, r, g, b, radius;
+  int is_dye;
+} or_splat;
+void   or_splat_apply(int nx, int ny, double lx, double ly, double *u, double *v, double *dye, const or_splat *s);
+
 /* one full step (src/cpu/solver.js:336-398); p_out may be NULL */
 int    or_step(const or_step_cfg *cfg, double *u, double *v, double *p_out, double dt, uint32_t force_seed);
 

@@ -19,6 +19,11 @@ ADV = {"semi-lagrangian": 0, "maccormack": 1}
 PROJ = {"fft": 0, "cg": 1, "jacobi": 2, "red-black": 3, "multigrid": 4, "none": 5}
 
 
+class Splat(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("dx", C.c_double), ("dy", C.c_double), ("r", C.c_double),
+                ("g", C.c_double), ("b", C.c_double), ("radius", C.c_double), ("is_dye", C.c_int)]
+
+
 class StepCfg(C.Structure):
     _fields_ = [
         ("nx", C.c_int), ("ny", C.c_int),
@@ -108,6 +113,8 @@ def lib():
         "or_band_modes": (i, [C.c_uint32, d, d, _I, _I, p]),
         "or_band_forcing": (None, [i, i, d, d, p, p, d, d, d, C.c_uint32, d]),
         "or_mode_forcing": (None, [i, i, d, d, p, p, i, p, p, p, d, d, d, d, d]),
+        "or_advect_dye": (None, [i, i, d, d, p, p, p, p, d, d]),
+        "or_splat_apply": (None, [i, i, d, d, p, p, p, C.POINTER(Splat)]),
         "or_linear_drag": (None, [i, p, p, d, d]),
         "or_taylor_green": (None, [i, i, d, d, d, d, p, p]),
         "or_velocity_error_tg": (d, [i, i, d, d, p, p, d, d]),
@@ -276,6 +283,22 @@ def band_forcing(nx, ny, u, v, kf, width, amp, seed, dt, lx=TWO_PI, ly=TWO_PI):
     v = _f(v).copy()
     lib().or_band_forcing(nx, ny, lx, ly, u, v, kf, width, amp, seed & 0xFFFFFFFF, dt)
     return u, v
+
+
+def advect_dye(nx, ny, u, v, dye, dt, fade=0.0, lx=TWO_PI, ly=TWO_PI):
+    out = np.empty(4 * nx * ny)
+    lib().or_advect_dye(nx, ny, lx, ly, _f(u), _f(v), _f(dye), out, dt, fade)
+    return out
+
+
+def splat(nx, ny, u, v, dye, s, lx=TWO_PI, ly=TWO_PI):
+    """One splat (dict x, y, dx, dy, r, g, b, radius, isDye) applied to copies of u, v, dye."""
+    u, v, dye = _f(u).copy(), _f(v).copy(), _f(dye).copy()
+    sp = Splat(float(s["x"]), float(s["y"]), float(s.get("dx", 0.0)), float(s.get("dy", 0.0)),
+               float(s.get("r", 0.0)), float(s.get("g", 0.0)), float(s.get("b", 0.0)),
+               float(s.get("radius", 0.2)), 1 if s.get("isDye") else 0)
+    lib().or_splat_apply(nx, ny, lx, ly, u, v, dye, C.byref(sp))
+    return u, v, dye
 
 
 def mode_forcing(nx, ny, u, v, modes, amplitude, drag, drift, t, dt, lx=TWO_PI, ly=TWO_PI):

@@ -141,6 +141,23 @@ async function main() {
     R.spectrumModes = Array.from(counts).reduce((a, b) => a + b, 0);
     gpu.destroy();
   }
+  // splats (each with its own parameters) and the dye through the shim
+  {
+    const gpu = new GPUFluidSolver(device, { n: N, nu: 0, pressureSolver: "multigrid", pressureIterations: 3,
+      dyeFade: 0.02 });
+    gpu.uploadVelocity(tu, tv);
+    const rgba = new Float32Array(N * N * 4);
+    for (let k = 0; k < N * N; k++) { rgba[k * 4] = (k % 7) / 7; rgba[k * 4 + 2] = 0.5; }
+    gpu.uploadDye(rgba);
+    const splats = [{ x: 1.0, y: 2.0, dx: 3.0, dy: -1.0, radius: 0.3 },
+      { x: 6.1, y: 0.1, r: 0.9, g: 0.2, b: 0.5, radius: 0.5, isDye: true }];
+    for (let k = 0; k < 3; k++) gpu.step(dt, { splats });
+    const raw = await gpu.readTexture(gpu.dyeCur);
+    save("dye.f32", raw);
+    const { u, v } = await gpu.readVelocity();
+    save("splat_u.f32", u); save("splat_v.f32", v);
+    gpu.destroy();
+  }
   writeFileSync(join(out, "results.json"), JSON.stringify(R));
   console.log(JSON.stringify({ ok: true }));
 }

@@ -179,3 +179,18 @@ def test_node_gputest_checks_equal_the_python_binding(tmp_path):
         assert same(g.readField("pressure"), _load(d, "tg_p.f64", np.float64))
         _, E, counts = g.energySpectrum()
     assert same(E, _load(d, "tg_E.f64", np.float64)) and R["spectrumModes"] == int(np.sum(counts))
+    rgba = np.zeros(N * N * 4, np.float32)
+    k = np.arange(N * N)
+    rgba[0::4] = (k % 7).astype(np.float32) / np.float32(7)
+    rgba[2::4] = 0.5
+    splats = [dict(x=1.0, y=2.0, dx=3.0, dy=-1.0, radius=0.3),
+              dict(x=6.1, y=0.1, r=0.9, g=0.2, b=0.5, radius=0.5, isDye=True)]
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver="multigrid", pressureIterations=3,
+                                 dyeFade=0.02) as g:
+        g.uploadVelocity(tu, tv)
+        g.uploadDye(rgba)
+        for _ in range(3):
+            g.step(dt, splats=splats)
+        assert same(g.readField("dye"), _load(d, "dye.f32", np.float32))
+        vel = g.readVelocity()
+    assert same(vel["u"], _load(d, "splat_u.f32", np.float32)) and same(vel["v"], _load(d, "splat_v.f32", np.float32))

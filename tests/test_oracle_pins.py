@@ -495,6 +495,53 @@ def test_mode_list_forcing_is_its_closed_form(O):
     assert np.max(np.abs(Gu - (damp * u0 + Fu))) < 1e-14 and np.max(np.abs(Gv - (damp * v0 + Fv))) < 1e-14
     assert np.max(np.abs(Gu - damp * (u0 + Fu))) > 1e-6
 
+
+def test_dye_step_fades_clamps_and_shifts_exactly(O):
+    # ADVECT_SCALAR (src/gpu/shaders.js:234-266): at rest the dye is only faded and
+    # clamped to [0, 1]; a uniform flow that moves every trace by whole cells shifts each
+    # channel exactly (brute force)
+    n, dt = 32, 0.1
+    h = TWO_PI / n
+    rng = np.random.default_rng(2)
+    dye = rng.uniform(-0.5, 1.5, 4 * n * n)
+    z = np.zeros(n * n)
+    out = O.advect_dye(n, n, z, z, dye, dt, fade=0.1)
+    assert np.max(np.abs(out - np.clip(dye * (1.0 - 0.1), 0.0, 1.0))) < 1e-14
+    d01 = rng.uniform(0, 1, 4 * n * n)
+    u = np.full(n * n, 2 * h / dt)      # two cells to the right
+    v = np.full(n * n, -1 * h / dt)     # one cell down
+    out = O.advect_dye(n, n, u, v, d01, dt)
+    want = np.roll(np.roll(d01.reshape(n, n, 4), 2, axis=1), -1, axis=0).reshape(-1)
+    assert np.max(np.abs(out - want)) < 1e-12
+
+
+def test_splat_is_a_periodic_gaussian_about_the_cell_centres(O):
+    # SPLAT (src/gpu/shaders.js:474-511): weight 1 at a cell centre the splat sits on,
+    # exp(-h^2/r^2) one cell away, the same across the periodic seam, dye splats add
+    # their colour and leave the velocity alone
+    n = 64
+    h = TWO_PI / n
+    z = np.zeros(n * n)
+    dz = np.zeros(4 * n * n)
+    r = 3 * h
+    s = dict(x=5.5 * h, y=0.5 * h, dx=2.0, dy=-1.0, radius=r)
+    u, v, dye = O.splat(n, n, z, z, dz, s)
+    U, V = u.reshape(n, n), v.reshape(n, n)
+    assert U[0, 5] == 2.0 and V[0, 5] == -1.0
+    assert abs(U[0, 6] - 2.0 * math.exp(-1 / 9)) < 1e-14 and abs(U[0, 4] - U[0, 6]) < 1e-14
+    assert abs(U[n - 1, 5] - U[1, 5]) < 1e-14          # across the seam in y
+    assert not dye.any()
+    # the weights integrate to pi r^2 over the cells (r >> h, r << L)
+    assert abs(U.sum() / 2.0 * h * h - math.pi * r * r) < 1e-9
+    s2 = dict(x=0.0, y=0.0, r=0.5, g=0.25, b=1.0, radius=r, isDye=True)
+    u, v, dye = O.splat(n, n, z, z, dz, s2)
+    D = dye.reshape(n, n, 4)
+    assert not u.any() and not v.any() and not D[:, :, 3].any()
+    # the splat on the corner of four cells: they share the largest weight
+    w = math.exp(-0.5 * h * h / (r * r))
+    for (j, i) in ((0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1)):
+        assert abs(D[j, i, 0] - 0.5 * w) < 1e-14 and abs(D[j, i, 2] - w) < 1e-14
+
 # ------------------------------------------------- immersed disk (src/cpu/solid.js)
 # restating test/solid.test.js:29-160 on the oracle's coupling
 
