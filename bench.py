@@ -49,14 +49,15 @@ KIND_TEXT = {
     "forcing": "band forcing + drag (k_forcing_sep), 16 B/cell: u, v in + out",
 }
 
-# HBM bytes per launch at 1024^2 measured by `rocprofv3 --pmc FETCH_SIZE` and
-# `--pmc WRITE_SIZE` runs of this script (--steps 20 --warmup 2 --no-roofline),
-# each scaled by what the counter reads for a plain copy of 256 MiB
-# (`bench.py --copy-calibration`: read = 2.0 x FETCH_SIZE KiB, write = 1.0 x
-# WRITE_SIZE KiB). For a class that runs on several levels, the mean over them.
-TRAFFIC_1024 = {
-    "up": 8.78e6, "down": 1.76e6, "advect_fused": 17.4e6, "gradient": 25.4e6, "down0": 36.7e6, "coarse": 0.0705e6,
+# The kernels of each class as the profiler names them, for the HBM counter passes.
+KIND_KERNELS = {
+    "advect_sl": r"k_advect_sl\b", "advect_mc": r"k_advect_mc\b", "diffuse": r"k_diffuse\b",
+    "down0": r"k_mgt_down0<", "down": r"k_mgt_down2?<", "coarse": r"k_mg_coarse_ml<", "up": r"k_mgt_up2?<",
+    "gradient": r"k_gradient64\b", "residual": r"k_residual64<", "rb": r"k_(mg_rb|rb_colour)<",
+    "advect_fused": r"k_advect_mc_diffuse\b", "forcing": r"k_forcing_(tables|sep)\b",
 }
+CAL_BYTES = 256 << 20   # one plain device copy of the counter calibration
+CAL_COPIES = 5
 
 
 # Workloads: the headline one (default) and the forced, mixed-precision one at 2048^2
@@ -85,7 +86,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--guess", default="zero", choices=["zero", "previous", "learned", "learned-coarse", "learned-quarter"])
+    ap.add_argument("--guess", default="zero",
+                    choices=["zero", "previous", "extrapolated", "learned", "learned-coarse", "learned-quarter"])
     ap.add_argument("--no-target", action="store_true",
                     help="skip the forced-1024^2 comparison of learned-guess BF16 multigrid with plain FP32 multigrid")
     ap.add_argument("--no-ablation", action="store_true",
@@ -98,6 +100,11 @@ def parse():
                     help="skip timing plain 1 GiB device copies for the roofline's copy_ceiling_gbs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (counter calibration) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) behind roofline.traffic")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run --warmup steps, then a profiled pass of 10 steps, print its launch counts and "
+                         "exit (the program the counter passes run)")
     return ap.parse_args()
 
 
@@ -148,6 +155,85 @@ def copy_ceiling():
     finally:
         hip.hipFree(a)
         hip.hipFree(b)
+
+
+def _counter_pass(counter, argv, timeout_s=150):
+    """Run `python3 bench.py <argv>` under `rocprofv3 --pmc <counter>` in a child process
+    group of its own; returns (its stdout, [(dispatch id, kernel name, value)]) or None."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    out = tempfile.mkdtemp(prefix="fs_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.join(ROOT, "bench.py")] + argv
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         start_new_session=True)
+    try:
+        text, _ = p.communicate(timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        print(f"# counter pass {counter} {' '.join(argv)}: no result in {timeout_s} s", file=sys.stderr, flush=True)
+        shutil.rmtree(out, ignore_errors=True)
+        return None
+    rows = []
+    for f in glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r.get("Dispatch_Id") or len(rows)), r.get("Kernel_Name", ""),
+                                 float(r["Counter_Value"])))
+    shutil.rmtree(out, ignore_errors=True)
+    if p.returncode != 0 or not rows:
+        print(f"# counter pass {counter} {' '.join(argv)}: exit {p.returncode}, {len(rows)} rows",
+              file=sys.stderr, flush=True)
+        return None
+    rows.sort()
+    return text, rows
+
+
+def hbm_traffic(a, kind):
+    """HBM bytes per launch of one kernel class, measured in two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE) over a profiled pass of the same workload run by this
+    script in a child process. Each counter is scaled by what it reads for plain
+    device-to-device copies of 256 MiB (--copy-calibration), as bytes read and written by
+    the copy over the counter's KiB. The class's launches are the last dispatches of its
+    kernels (the probe ends with its profiled pass), so the autotuner's trial launches at
+    create are left out. None if rocprofv3 is missing or a pass fails."""
+    import re
+    pat = KIND_KERNELS.get(kind)
+    if pat is None:
+        return None
+    probe = ["--traffic-probe", "--workload", a.workload, "--warmup", str(a.warmup), "--tol", str(a.tol),
+             "--guess", a.guess] + (["--n", str(a.n)] if a.n else []) + (["--precision", a.precision]
+                                                                        if a.precision else [])
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = _counter_pass(counter, ["--copy-calibration"])
+        run = _counter_pass(counter, probe)
+        if cal is None or run is None:
+            return None
+        cal_kib = sum(v for _, name, v in cal[1] if "copyBuffer" in name)
+        if cal_kib <= 0:
+            return None
+        factor = CAL_COPIES * CAL_BYTES / (cal_kib * 1024.0)
+        m = re.search(r"^# probe launches (\{.*\})$", run[0], re.M)
+        launches = json.loads(m.group(1)).get(kind, 0) if m else 0
+        mine = [v for _, name, v in run[1] if re.search(pat, name)]
+        if launches <= 0 or len(mine) < launches:
+            return None
+        res[counter] = (sum(mine[-launches:]) * 1024.0 * factor / launches, factor, launches)
+    return {"bytes": res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0], "read_bytes": res["FETCH_SIZE"][0],
+            "write_bytes": res["WRITE_SIZE"][0], "read_factor": res["FETCH_SIZE"][1],
+            "write_factor": res["WRITE_SIZE"][1], "launches": res["FETCH_SIZE"][2],
+            "counters": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, scaled by plain copies of 256 MiB"}
 
 
 GUESS_WEIGHTS = os.path.join(ROOT, "fluid-sim_amd", "weights", "guess_unet_256.fsgn")
@@ -372,6 +458,15 @@ def main():
     for _ in range(a.warmup):
         g.stepCFL(0.5)
     g.sync()
+    if a.traffic_probe:
+        g.profileEnable()
+        for _ in range(10):
+            g.stepCFL(0.5)
+        g.sync()
+        counts = {k: v[1] for k, v in g.profileRead().items()}
+        g.destroy()
+        print("# probe launches " + json.dumps(counts), flush=True)
+        return 0
     st0 = g.stats()
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -406,9 +501,14 @@ def main():
         ms, launches, nbytes = prof[dom]
         avg_s = ms / 1e3 / max(launches, 1)
         achieved = nbytes / (ms / 1e3) / 1e9
-        traffic = TRAFFIC_1024.get(dom) if (n == 1024 and wl["precision"] == "fp32") else None
+        detail = None if a.no_traffic else hbm_traffic(a, dom)
+        traffic = detail["bytes"] if detail else None
+        if detail:
+            print(f"# {dom} HBM traffic {traffic / 1e6:.2f} MB per launch (read {detail['read_bytes'] / 1e6:.2f}, "
+                  f"write {detail['write_bytes'] / 1e6:.2f}) against {nbytes / max(launches, 1) / 1e6:.2f} MB "
+                  "algorithmic", file=sys.stderr, flush=True)
         roof = {"bound": "hbm", "kernel": KIND_TEXT[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": detail,
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
                 "share_of_step": ms / psteps / step_ms,
@@ -423,7 +523,8 @@ def main():
     ablation = None
     if not a.no_ablation:
         ablation = []
-        others = [x for x in ("zero", "previous", "learned", "learned-coarse", "learned-quarter") if x != a.guess]
+        others = [x for x in ("zero", "previous", "extrapolated", "learned", "learned-coarse", "learned-quarter")
+                  if x != a.guess]
         for other in others:
             if other.startswith("learned") and (n % {"learned": 128, "learned-coarse": 256}.get(other, 512)
                                                 or not os.path.exists(GUESS_WEIGHTS)):
@@ -488,6 +589,11 @@ def main():
             "vcycles_per_step": cycles,
             "guess": a.guess,
             "stored_field_max_div": d["max_divergence"],
+            # timed steps whose projection ran gated on the device (no host read-back), those
+            # whose predicted V-cycles fell short of the tolerance, and step-graph replays
+            "gated_steps": st1["gated_steps"] - st0["gated_steps"],
+            "gate_misses": st1["gate_misses"] - st0["gate_misses"],
+            "graph_launches": st1["graph_launches"] - st0["graph_launches"],
             "tiles": [ln.split(" | ")[0] for ln in tiles.strip().splitlines()],
         },
         "roofline": roof,

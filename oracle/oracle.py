@@ -224,6 +224,13 @@ def diffuse_explicit(nx, ny, f, nu, dt, lx=TWO_PI, ly=TWO_PI):
     return out
 
 
+def diffuse_implicit(nx, ny, f, nu, dt, tol=1e-12, lx=TWO_PI, ly=TWO_PI):
+    """(I - nu dt L) out = f by conjugate gradients; returns (out, iterations)."""
+    out = np.empty(nx * ny)
+    it = lib().or_diffuse_implicit(nx, ny, lx, ly, _f(f), out, nu, dt, tol)
+    return out, it
+
+
 def poisson_fft(nx, ny, rhs, lx=TWO_PI, ly=TWO_PI):
     p = np.empty(nx * ny)
     rc = lib().or_poisson_fft(nx, ny, lx, ly, _f(rhs), p)

@@ -243,6 +243,36 @@ def test_explicit_diffusion_checkerboard_and_sine_factors(O):
     assert abs(out.mean() - f.mean()) < 1e-14
 
 
+def test_implicit_diffusion_inverts_each_fourier_mode(O):
+    # (I - nu dt L) out = f: every Fourier mode of f divided by 1 + 4a sin^2(kh/2) +
+    # 4b sin^2(lh/2) (the 5-point Laplacian's eigenvalue); checked on a seeded field
+    # through numpy's FFT, and a single sine mode by its closed factor
+    nx, ny = 32, 16
+    hx, hy = TWO_PI / nx, TWO_PI / ny
+    nu, dt = 0.7, 0.3
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((ny, nx))
+    out, it = O.diffuse_implicit(nx, ny, f.reshape(-1), nu, dt)
+    assert 0 < it < 400
+    kx = np.fft.fftfreq(nx) * nx
+    ky = np.fft.fftfreq(ny) * ny
+    den = (1 + 4 * nu * dt / hx ** 2 * np.sin(kx[None, :] * hx / 2) ** 2
+           + 4 * nu * dt / hy ** 2 * np.sin(ky[:, None] * hy / 2) ** 2)
+    want = np.real(np.fft.ifft2(np.fft.fft2(f) / den))
+    assert np.max(np.abs(out.reshape(ny, nx) - want)) < 1e-10
+    # a dropped factor 4, a wrong sign or an explicit step instead all miss by far more
+    for bad in (np.real(np.fft.ifft2(np.fft.fft2(f) / (2 * den - 1))),
+                np.real(np.fft.ifft2(np.fft.fft2(f) * (2 - den)))):
+        assert np.max(np.abs(bad - want)) > 1e-2
+    c = np.arange(nx) * hx
+    g = np.tile(np.sin(3 * c), ny)
+    fac = 1 + 4 * nu * dt / hx ** 2 * math.sin(3 * hx / 2) ** 2
+    out, _ = O.diffuse_implicit(nx, ny, g, nu, dt)
+    assert np.max(np.abs(out - g / fac)) < 1e-12
+    same, it0 = O.diffuse_implicit(nx, ny, g, 0.0, dt)
+    assert it0 == 0 and np.array_equal(same, g)
+
+
 # ------------------------------------------------------------------- advection
 
 @pytest.mark.parametrize("scheme", ["semi-lagrangian", "maccormack"])
