@@ -1067,4 +1067,143 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
   for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = st<SO>(bx[q]);
 }
 
+// ------------------------------------------------- exact coarsest-level solve
+// The periodic 5-point Laplacian of an n x n level (n <= COARSE_N) is T (x) I + I (x) T
+// with T the 1D second difference, and T = Q diag(lam) Q^T for the real Fourier basis
+// Q (orthonormal columns: the constant, cos and sin pairs, the alternating mode). So
+// L x = r - mean(r) is solved by x = Q ((Q^T r Q) * W) Q^T with W[a][b] = 1 / (lam_a +
+// lam_b) and W[0][0] = 0 (the zero-mean solution, as a pseudo-inverse gives). The four
+// 64 x 64 x 64 products run on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with
+// every operand split into a BF16 head and tail, x = h + l with h = bf16(x) and
+// l = bf16(x - h): a b ~ ha hb + ha lb + la hb accumulated in FP32 (the dropped la lb
+// and the tails' own rounding leave ~2^-16 relative per product). One 16 x 16 output
+// block per wavefront; Q and W come from the host (built in FP64), zero-padded to
+// 64 x 64 and laid out in the order the lanes read them, so n < 64 needs no special case.
+//
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8 (l / 16) + t] and
+// B[8 (l / 16) + t][l % 16], t = 0..7, and receives D[4 (l / 16) + i][l % 16] in its
+// i-th accumulator.
+constexpr int EXN = 64;   // padded level size
+constexpr int EXB = 72;   // LDS row pitch (BF16): 16-byte reads of 16 rows hit distinct banks
+constexpr int EXT = 1024; // 16 wavefronts, one 16 x 16 block of the 64 x 64 result each
+// Host-built operand table, in 16-byte units: Q[k][16 T + i] (T the block, i = l % 16,
+// k = 32 kk + 8 (l / 16) + t) heads then tails, Q[16 T + i][k] heads then tails (each
+// [T 4][lane 64][kk 2] x 8 BF16), then W at each lane's four outputs ([wave 16][lane 64]
+// x 4 FP32).
+constexpr int EX_TAB = 4 * 512 + 1024;
+typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 ex_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
+
+// x into LDS as head (at h[o]) and tail (at l[o])
+__device__ __forceinline__ void ex_put(__bf16* h, __bf16* l, int o, float x) {
+  const __bf16 hx = (__bf16)x;
+  h[o] = hx;
+  l[o] = (__bf16)(x - (float)hx);
+}
+
+// acc over the 64-long k range: heads and tails of the two operands of k-chunk kk
+__device__ __forceinline__ ex_f32x4 ex_mma3(ex_f32x4 acc, ex_bf16x8 ah, ex_bf16x8 al, ex_bf16x8 bh, ex_bf16x8 bl) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(EXT)
+k_mg_coarse_exact(float* __restrict__ xg, const S* __restrict__ rg, int n, const uint4* __restrict__ tab, Gate gate) {
+  if (gate_closed(gate)) return;
+  // two matrices, heads and tails; A/B alternate as input and output of the products
+  __shared__ __align__(16) __bf16 sah[EXN * EXB], sal[EXN * EXB], sbh[EXN * EXB], sbl[EXN * EXB];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int tm = w >> 2, tn = w & 3, m0 = 16 * tm, n0 = 16 * tn;
+  // the right-hand side first (four consecutive cells of one row per thread), then the
+  // Q operands in the order the products use them
+  const int ri = threadIdx.x >> 4, rj = 4 * (threadIdx.x & 15);
+  float rv[4];
+  if (n == EXN) {
+    if constexpr (sizeof(S) == 4) {
+      float4 t = *(const float4*)((const float*)rg + ri * EXN + rj);
+      rv[0] = t.x, rv[1] = t.y, rv[2] = t.z, rv[3] = t.w;
+    } else {
+      uint2 t = *(const uint2*)((const uint16_t*)rg + ri * EXN + rj);
+      rv[0] = __uint_as_float(t.x << 16), rv[1] = __uint_as_float(t.x & 0xffff0000u);
+      rv[2] = __uint_as_float(t.y << 16), rv[3] = __uint_as_float(t.y & 0xffff0000u);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; u++) rv[u] = (ri < n && rj + u < n) ? ld(rg[ri * n + rj + u]) : 0.f;
+  }
+  auto q8 = [&](int part, int T, int kk) { return __builtin_bit_cast(ex_bf16x8, tab[part * 512 + (T * 64 + lane) * 2 + kk]); };
+  ex_bf16x8 a1h[2], a1l[2], b2h[2], b2l[2], a3h[2], a3l[2], b4h[2], b4l[2];
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) a1h[kk] = q8(0, tm, kk), a1l[kk] = q8(1, tm, kk);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) b2h[kk] = q8(0, tn, kk), b2l[kk] = q8(1, tn, kk);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) a3h[kk] = q8(2, tm, kk), a3l[kk] = q8(3, tm, kk);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) b4h[kk] = q8(2, tn, kk), b4l[kk] = q8(3, tn, kk);
+  const float4 wv = ((const float4*)(tab + 2048))[w * 64 + lane];
+  // r transposed into A: RT[j][i] = r[i][j]
+#pragma unroll
+  for (int u = 0; u < 4; u++) ex_put(sah, sal, (rj + u) * EXB + ri, rv[u]);
+  __syncthreads();
+  // C1 = Q^T r (B operand RT rows), row-major into B
+  ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
+    acc = ex_mma3(acc, a1h[kk], a1l[kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
+  __syncthreads();
+  // C2 = (C1 Q) * W (A operand C1 rows), transposed into A: C2T[n][m]
+  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
+    acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b2h[kk], b2l[kk]);
+  }
+  {
+    const float c[4] = {acc[0] * wv.x, acc[1] * wv.y, acc[2] * wv.z, acc[3] * wv.w};
+    ex_bf16x4 h4, l4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const __bf16 hx = (__bf16)c[i];
+      h4[i] = hx;
+      l4[i] = (__bf16)(c[i] - (float)hx);
+    }
+    const int o = (n0 + li) * EXB + m0 + 4 * lg;
+    *(ex_bf16x4*)(sah + o) = h4;
+    *(ex_bf16x4*)(sal + o) = l4;
+  }
+  __syncthreads();
+  // C3 = Q C2 (B operand C2T rows), row-major into B
+  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
+    acc = ex_mma3(acc, a3h[kk], a3l[kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
+  __syncthreads();
+  // x = C3 Q^T (A operand C3 rows)
+  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
+    acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b4h[kk], b4l[kk]);
+  }
+  const int col = n0 + li;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = m0 + 4 * lg + i;
+    if (row < n && col < n) xg[row * n + col] = acc[i];
+  }
+}
+
 }  // namespace fsk

@@ -143,6 +143,7 @@ struct fs_solver_t {
   bool fused = false;        // LDS-tiled V-cycle in use
   bool bf16 = false;         // multigrid correction and right-hand sides stored as BF16
   float* pinv = nullptr;     // coarsest-level pseudo-inverse (null: run the sweeps)
+  float* ex_q = nullptr;     // exact coarse solve: the operand table (fsk::EX_TAB 16-byte units)
   int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
   double* rpart = nullptr;
@@ -685,7 +686,12 @@ int mg_vcycle(fs_solver_t* s) {
   }
   {
     Level& C = s->lv[Lc - 1];
-    if (C.n <= fsk::COARSE_N) {
+    if (s->ex_q) {
+      int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n + 16.0 * fsk::EX_TAB, fsk::k_mg_coarse_exact<float>,
+                      dim3(1), dim3(fsk::EXT), C.x, (const float*)C.rhs, C.n, (const uint4*)s->ex_q,
+                      fsk::gate_open());
+      if (rc) return rc;
+    } else if (C.n <= fsk::COARSE_N) {
       int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml<float>, dim3(1), dim3(fsk::CTH), C.x,
                       (const float*)C.rhs, level_table(s, Lc - 1, Lc), s->cfg.mg_pre, s->cfg.mg_post,
                       s->cfg.mg_coarse, (const float*)s->pinv, fsk::gate_open());
@@ -868,6 +874,10 @@ template <typename S>
 int launch_coarse_s(fs_solver_t* s, int from) {
   Level& C = s->lv[from];
   // rhs at the storage precision in, the result (the correction passed up) out in FP32
+  if (s->ex_q)
+    return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n + 16.0 * fsk::EX_TAB,
+                  fsk::k_mg_coarse_exact<S>, dim3(1), dim3(fsk::EXT), C.xb, as<S>((const float*)C.rhs), C.n,
+                  (const uint4*)s->ex_q, s->gate_v);
   return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
                 dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
                 s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv, s->gate_v);
@@ -1588,6 +1598,77 @@ int build_coarse_inverse(fs_solver_t* s) {
   return FS_OK;
 }
 
+// Tables of the exact coarsest-level solve (k_mg_coarse_exact), built in FP64: the
+// real Fourier basis Q of the periodic second difference on n nodes (column 0 the
+// constant, columns 2f-1 and 2f the cosine and sine of frequency f, column n-1 the
+// alternating mode when n is even; orthonormal) and W[a][b] = 1 / (lam_a + lam_b) with
+// lam of frequency f = -(4 / h^2) sin^2(pi f / n) and W[0][0] = 0, zero-padded to
+// 64 x 64. Q goes in as BF16 heads and tails in the order the lanes read them, W as
+// FP32 at each lane's outputs (fsk::EX_TAB 16-byte units).
+uint16_t bf16_bits(double x) {
+  float f = (float)x;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite input)
+  return (uint16_t)(u >> 16);
+}
+double bf16_value(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+int build_coarse_exact(fs_solver_t* s) {
+  const Level& C = s->lv.back();
+  const int n = C.n, E = fsk::EXN;
+  if (n > E) return fail(FS_E_INVALID_ARG, "exact coarse solve of a %d-cell level (at most %d)", n, E);
+  std::vector<double> Q((size_t)E * E, 0.0), lam(E, 0.0);
+  for (int c = 0; c < n; c++) {
+    int f = (c + 1) / 2;  // 0; 1, 1; 2, 2; ...; n/2 for the last column when n is even
+    bool alt = n % 2 == 0 && c == n - 1, sine = c > 0 && c % 2 == 0 && !alt;
+    double sh = std::sin(M_PI * f / n);
+    lam[c] = -4.0 * sh * sh / (C.dx * C.dx);
+    for (int j = 0; j < n; j++) {
+      double v;
+      if (c == 0) v = 1.0 / std::sqrt((double)n);
+      else if (alt) v = ((j & 1) ? -1.0 : 1.0) / std::sqrt((double)n);
+      else if (sine) v = std::sqrt(2.0 / n) * std::sin(2 * M_PI * f * j / n);
+      else v = std::sqrt(2.0 / n) * std::cos(2 * M_PI * f * j / n);
+      Q[(size_t)j * E + c] = v;
+    }
+  }
+  std::vector<uint16_t> qt((size_t)4 * 512 * 8);
+  for (int T = 0; T < 4; T++)
+    for (int lane = 0; lane < 64; lane++)
+      for (int kk = 0; kk < 2; kk++)
+        for (int t = 0; t < 8; t++) {
+          const int li = lane & 15, k = 32 * kk + 8 * (lane >> 4) + t;
+          const size_t o = ((size_t)(T * 64 + lane) * 2 + kk) * 8 + t;
+          const double a = Q[(size_t)k * E + 16 * T + li];  // Q[k][16 T + i]
+          const double b = Q[(size_t)(16 * T + li) * E + k];  // Q[16 T + i][k]
+          const uint16_t ah = bf16_bits(a), bh = bf16_bits(b);
+          qt[o] = ah;
+          qt[512 * 8 + o] = bf16_bits(a - bf16_value(ah));
+          qt[2 * 512 * 8 + o] = bh;
+          qt[3 * 512 * 8 + o] = bf16_bits(b - bf16_value(bh));
+        }
+  std::vector<float> wt((size_t)16 * 64 * 4);
+  for (int w = 0; w < 16; w++)
+    for (int lane = 0; lane < 64; lane++)
+      for (int i = 0; i < 4; i++) {
+        const int a = 16 * (w >> 2) + 4 * (lane >> 4) + i, b = 16 * (w & 3) + (lane & 15);
+        wt[((size_t)w * 64 + lane) * 4 + i] =
+            (a >= n || b >= n || (a == 0 && b == 0)) ? 0.f : (float)(1.0 / (lam[a] + lam[b]));
+      }
+  static_assert(fsk::EX_TAB * 16 == 4 * 512 * 8 * 2 + 16 * 64 * 4 * 4, "table layout");
+  int rc = dalloc(&s->ex_q, (size_t)fsk::EX_TAB * 4);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(s->ex_q, qt.data(), qt.size() * 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((char*)s->ex_q + qt.size() * 2, wt.data(), wt.size() * 4, hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
 int build_levels(fs_solver_t* s) {
   if (s->cfg.pressure_solver != FS_P_MULTIGRID) return FS_OK;
   if (s->nx != s->ny) return fail(FS_E_INVALID_ARG, "multigrid here assumes a square grid, got %dx%d", s->nx, s->ny);
@@ -1603,9 +1684,13 @@ int build_levels(fs_solver_t* s) {
     lvl.n = n;
     lvl.dx = L / n;
     s->lv.push_back(lvl);
+    if (s->cfg.mg_coarse_exact && n <= fsk::COARSE_N) break;  // solved exactly: no levels below
     if (n / 2 < minS || (n & 1) || ((n / 2) & 1)) break;
     n /= 2;
   }
+  if (s->cfg.mg_coarse_exact && s->lv.back().n > fsk::COARSE_N)
+    return fail(FS_E_INVALID_ARG, "the exact coarse solve needs a level of at most %d cells; the hierarchy of %d "
+                                  "with mg_min_size %d stops at %d", fsk::COARSE_N, s->nx, minS, s->lv.back().n);
   // fused path: grid-wide levels are even (restriction, tile parity), the tail from
   // the first level <= COARSE_N fits the one-workgroup kernel, and the sweep counts
   // have a compiled tile kernel
@@ -1632,6 +1717,7 @@ int build_levels(fs_solver_t* s) {
     // pressure; at the levels below, prolongation carries it up)
     if (s->fused && (rc = dalloc(&s->lv[l].xb, c))) return rc;
   }
+  if (s->cfg.mg_coarse_exact) return build_coarse_exact(s);
   return build_coarse_inverse(s);
 }
 
@@ -1769,6 +1855,7 @@ void release(fs_solver_t* s) {
   dfree(s->rpart);
   hfree(s->rpart_host);
   dfree(s->pinv);
+  dfree(s->ex_q);
   dfree(s->mpart);
   dfree(s->coef);
   hfree(s->dt_ring);
@@ -2543,6 +2630,7 @@ void fs_config_default(fs_config* c) {
   c->gated = 1;
   c->graph = 1;
   c->dye_fade = 0.0;
+  c->mg_coarse_exact = 0;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -2566,6 +2654,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (c.tolerance > 0 && c.pressure_solver != FS_P_MULTIGRID)
     return fail(FS_E_INVALID_ARG, "a residual tolerance needs the multigrid solver");
   if (c.mg_pre < 0 || c.mg_post < 0 || c.mg_coarse < 0) return fail(FS_E_INVALID_ARG, "negative sweep count");
+  if (c.mg_coarse_exact != 0 && c.mg_coarse_exact != 1)
+    return fail(FS_E_INVALID_ARG, "mg_coarse_exact must be 0 or 1, got %d", c.mg_coarse_exact);
   if (c.mg_precision == FS_MG_FP16)
     return fail(FS_E_UNSUPPORTED,
                 "FP16 multigrid storage is not built: the correction scales as h^2 |r| and underflows FP16 on "

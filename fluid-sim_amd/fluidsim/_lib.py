@@ -9,7 +9,7 @@ import os
 
 from . import build as _build
 
-ABI_VERSION = 2  # FS_ABI_VERSION of include/fluidsim.h these structures mirror
+ABI_VERSION = 3  # FS_ABI_VERSION of include/fluidsim.h these structures mirror
 
 FS_OK = 0
 FS_E_INVALID_ARG = -1
@@ -54,6 +54,7 @@ class FsConfig(C.Structure):
         ("gated", C.c_int),
         ("graph", C.c_int),
         ("dye_fade", C.c_double),
+        ("mg_coarse_exact", C.c_int),
     ]
 
 

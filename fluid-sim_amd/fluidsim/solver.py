@@ -48,8 +48,10 @@ class GPUFluidSolver:
     multigrid V(2,2) with 40 coarse sweeps and minSize 8. Extra options:
     tolerance (FP64 residual gate for multigrid; 0 = fixed V-cycle count),
     mgMinSize, mgPrecision, guess ("zero" | "previous"), gated (the tolerance
-    checked on the device, no host wait inside a step) and graph (gated steps
-    replayed as captured hipGraphs).
+    checked on the device, no host wait inside a step), graph (gated steps
+    replayed as captured hipGraphs) and mgCoarseSolve ("sweeps": the reference's
+    V-cycle down to mgMinSize; "exact": the hierarchy stops at its first level of at
+    most 64 x 64 cells and solves it exactly).
     """
 
     def __init__(self, device=0, *, n=256, nx=None, ny=None, lx=TWO_PI, ly=TWO_PI, nu=0.0,
@@ -57,7 +59,7 @@ class GPUFluidSolver:
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
                  mgPrecision="fp32", guess="zero", mgFused=True, autotune=True,
-                 advFused=True, gated=True, graph=True):
+                 advFused=True, gated=True, graph=True, mgCoarseSolve="sweeps"):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -74,6 +76,8 @@ class GPUFluidSolver:
             raise ValueError("vorticity confinement is a display-only force and is not built here")
         if not (0.0 <= dyeFade <= 1.0):
             raise ValueError("dyeFade must be in [0, 1]")
+        if mgCoarseSolve not in ("sweeps", "exact"):
+            raise ValueError(f"unknown coarse solve {mgCoarseSolve}; have sweeps, exact")
         cfg = _lib.default_config()
         cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.nu = nx, ny, lx, ly, nu
         cfg.advection = _lib.ADVECTION[advection]
@@ -91,6 +95,7 @@ class GPUFluidSolver:
         cfg.gated = 1 if gated else 0
         cfg.graph = 1 if graph else 0
         cfg.dye_fade = float(dyeFade)
+        cfg.mg_coarse_exact = 1 if mgCoarseSolve == "exact" else 0
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))

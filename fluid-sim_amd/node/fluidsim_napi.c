@@ -143,6 +143,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   if (prop_double(env, argv[0], "guess", &d)) c.guess = (int)d;
   if (prop_double(env, argv[0], "solidMaskWidth", &d)) c.solid_mask_width = d;
   if (prop_double(env, argv[0], "dyeFade", &d)) c.dye_fade = d;
+  if (prop_double(env, argv[0], "mgCoarseExact", &d)) c.mg_coarse_exact = (int)d;
   if (prop_double(env, argv[0], "device", &d)) c.device = (int)d;
   void* s = NULL;
   int rc = fs_solver_create(&c, &s);

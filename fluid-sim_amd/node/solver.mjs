@@ -57,6 +57,7 @@ export class GPUFluidSolver {
     pressureSolver = "jacobi", pressureIterations = 40,
     hasTimestamps = false,
     tolerance = 0, mgMinSize = 8, mgPre = 2, mgPost = 2, mgCoarse = 40, guess = "zero",
+    mgCoarseSolve = "sweeps",
   } = {}) {
     if (!(advection in ADVECTION)) {
       throw new Error(`unknown advection scheme ${advection}; have ${Object.keys(ADVECTION).join(", ")}`);
@@ -67,6 +68,9 @@ export class GPUFluidSolver {
     if (!(guess in GUESS)) throw new Error(`unknown guess mode ${guess}`);
     if (confinement > 0) throw new Error("vorticity confinement is display-only and not built here");
     if (!(dyeFade >= 0 && dyeFade <= 1)) throw new Error("dyeFade must be in [0, 1]");
+    if (mgCoarseSolve !== "sweeps" && mgCoarseSolve !== "exact") {
+      throw new Error(`unknown coarse solve ${mgCoarseSolve}; have sweeps, exact`);
+    }
     this.device = device;
     this.nx = nx; this.ny = ny;
     this.dx = lx / nx; this.dy = ly / ny;
@@ -83,7 +87,7 @@ export class GPUFluidSolver {
       advection: ADVECTION[advection],
       pressureSolver: PRESSURE[pressureSolver],
       pressureIterations, tolerance, mgMinSize, mgPre, mgPost, mgCoarse,
-      guess: GUESS[guess], solidMaskWidth, dyeFade,
+      guess: GUESS[guess], solidMaskWidth, dyeFade, mgCoarseExact: mgCoarseSolve === "exact" ? 1 : 0,
     });
     if (device && device._solvers) device._solvers.add(this);
     // the reference's textures by name (src/gpu/solver.js:111-125, :287-292), for readTexture

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 2
+#define FS_ABI_VERSION 3
 
 /* status codes */
 enum {
@@ -118,6 +118,12 @@ typedef struct fs_config {
                                  a captured hipGraph (one per V-cycle count, buffer parity and
                                  initial-pressure kind); 0: launched one by one. Same bits. */
   double dye_fade;            /* dyeFade: the dye is multiplied by 1 - dye_fade each step; default 0 */
+  int mg_coarse_exact;        /* 0 (default): the reference's V-cycle down to mg_min_size with
+                                 mg_coarse sweeps on the coarsest level; 1: the hierarchy stops at
+                                 its first level of at most 64 x 64 cells, which is solved exactly
+                                 (real Fourier basis, BF16 head + tail products on the matrix
+                                 cores, ~1e-5 relative) - not in the reference; changes the
+                                 bits, not the tolerance the gate enforces */
 } fs_config;
 
 /* one splat of step(dt, {splats}) (src/gpu/solver.js:546-554, _writeSplat :745-757): a

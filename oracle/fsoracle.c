@@ -530,6 +530,57 @@ int or_poisson_fft(int nx, int ny, double lx, double ly, const double *rhs, doub
   return 0;
 }
 
+/* The periodic 5-point problem L x = rhs - mean(rhs) on an n x n grid of spacing
+ * lx / n, solved exactly for any n: a separable complex DFT with directly evaluated
+ * twiddles, division by the eigenvalue -(4/h^2)(sin^2(pi kx/n) + sin^2(pi ky/n)), the
+ * inverse DFT. The solution has zero mean. O(n^3); the exact coarsest-level solve of
+ * the multigrid variant that stops at a level <= 64 (not in the reference). */
+static void dft_lines(int n, double *re, double *im, int stride, int step, int sign, const double *cs,
+                      const double *sn, double *tr, double *ti) {
+  for (int line = 0; line < n; line++) {
+    double *r = re + (size_t)line * step, *q = im + (size_t)line * step;
+    for (int k = 0; k < n; k++) {
+      double ar = 0, ai = 0;
+      for (int i = 0; i < n; i++) {
+        int m = (int)(((long)i * k) % n);
+        double c = cs[m], s_ = sign * sn[m];
+        double xr = r[(size_t)i * stride], xi = q[(size_t)i * stride];
+        ar += xr * c - xi * s_;
+        ai += xr * s_ + xi * c;
+      }
+      tr[k] = ar;
+      ti[k] = ai;
+    }
+    for (int k = 0; k < n; k++) { r[(size_t)k * stride] = tr[k]; q[(size_t)k * stride] = ti[k]; }
+  }
+}
+
+int or_poisson_dft(int n, double lx, const double *rhs, double *x) {
+  if (n < 1) return -1;
+  size_t N = (size_t)n * n;
+  double h = lx / n;
+  double *re = (double *)malloc(sizeof(double) * N), *im = (double *)calloc(N, sizeof(double));
+  double *cs = (double *)malloc(sizeof(double) * n), *sn = (double *)malloc(sizeof(double) * n);
+  double *tr = (double *)malloc(sizeof(double) * n), *ti = (double *)malloc(sizeof(double) * n);
+  for (int m = 0; m < n; m++) { cs[m] = cos(2 * M_PI * m / n); sn[m] = sin(2 * M_PI * m / n); }
+  memcpy(re, rhs, sizeof(double) * N);
+  dft_lines(n, re, im, 1, n, -1, cs, sn, tr, ti);  /* along x, one row per line */
+  dft_lines(n, re, im, n, 1, -1, cs, sn, tr, ti);  /* along y, one column per line */
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      double sx = sin(M_PI * i / n), sy = sin(M_PI * j / n);
+      double lam = -4.0 * (sx * sx + sy * sy) / (h * h);
+      size_t k = (size_t)j * n + i;
+      if (i == 0 && j == 0) { re[k] = 0; im[k] = 0; }
+      else { re[k] /= lam; im[k] /= lam; }
+    }
+  dft_lines(n, re, im, 1, n, +1, cs, sn, tr, ti);
+  dft_lines(n, re, im, n, 1, +1, cs, sn, tr, ti);
+  for (size_t k = 0; k < N; k++) x[k] = re[k] / (double)N;
+  free(re); free(im); free(cs); free(sn); free(tr); free(ti);
+  return 0;
+}
+
 static double dotv(const double *a, const double *b, size_t n) {
   double s = 0;
   for (size_t i = 0; i < n; i++) s += a[i] * b[i];
@@ -1039,11 +1090,20 @@ static void round_field(double *a, size_t n, int storage) {
  * Levels at or below coarse_n run unrounded (they stay on chip). The V-cycle is that of
  * src/gpu/multigrid.js:277-330; with no rounding the k-th p equals k plain
  * V-cycles (the cycle is affine in its right-hand side).
+ * coarse_exact: the hierarchy stops at its first level <= coarse_n, and that level is
+ * solved exactly (or_poisson_dft) instead of by sweeps and the levels below it.
  * rmax[it] = max|div - L p| before refinement step it, for it = 0..iterations. */
 int or_mg_refine(int n, double lx, const double *div, double *p, int iterations, int min_size, int pre,
-                 int post, int coarse, int coarse_n, int storage, double *rmax) {
+                 int post, int coarse, int coarse_n, int storage, int coarse_exact, double *rmax) {
   int L = or_mg_levels(n, min_size);
   if (L < 1) return -1;
+  if (coarse_exact) {
+    /* the hierarchy stops at its first level <= coarse_n, which is solved exactly */
+    int m = n, l = 1;
+    while (l < L && m > coarse_n) { m /= 2; l++; }
+    if (m > coarse_n) return -2;
+    L = l;
+  }
   MGL *lv = (MGL *)calloc(L, sizeof(MGL));
   double **xs = (double **)calloc(L, sizeof(double *));  /* stored pre-smoothed corrections */
   int m = n, cs = L;
@@ -1096,7 +1156,8 @@ int or_mg_refine(int n, double lx, const double *div, double *p, int iterations,
      * src/cpu/projection.js:91-102): stored rounding leaves it slightly non-zero,
      * and sweeps on an incompatible rhs drift by a growing constant */
     remove_mean(lv[L - 1].rhs, (size_t)lv[L - 1].n * lv[L - 1].n);
-    mg_smooth(&lv[L - 1], coarse);
+    if (coarse_exact) or_poisson_dft(lv[L - 1].n, lv[L - 1].n * lv[L - 1].dx, lv[L - 1].rhs, lv[L - 1].x);
+    else mg_smooth(&lv[L - 1], coarse);
     for (int i = L - 2; i >= cs; i--) {
       mg_prolong(&lv[i + 1], &lv[i]);
       mg_smooth(&lv[i], post);

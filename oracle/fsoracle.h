@@ -139,7 +139,8 @@ int    or_energy_spectrum(int nx, int ny, const double *cu, const double *cv, do
  * solver with the correction stored at that precision */
 double or_round_storage(double x, int storage);
 int    or_mg_refine(int n, double lx, const double *div, double *p, int iterations, int min_size, int pre,
-                    int post, int coarse, int coarse_n, int storage, double *rmax);
+                    int post, int coarse, int coarse_n, int storage, int coarse_exact, double *rmax);
+int    or_poisson_dft(int n, double lx, const double *rhs, double *x);
 
 /* immersed disk (src/cpu/solid.js:38-40, :101-114, :165-233) */
 double or_wrap_delta(double d, double l);

@@ -132,7 +132,8 @@ def lib():
         "or_cav_poisson_exact": (i, [i, p, p]),
         "or_cav_apply_coupling": (None, [i, p, p, C.POINTER(Disk), d, i, p]),
         "or_cav_step": (i, [i, d, d, i, C.c_void_p, d, i, d, p, p, C.c_void_p]),
-        "or_mg_refine": (i, [i, d, p, p, i, i, i, i, i, i, i, p]),
+        "or_mg_refine": (i, [i, d, p, p, i, i, i, i, i, i, i, i, p]),
+        "or_poisson_dft": (i, [i, d, p, p]),
         "or_disk_mask": (d, [C.POINTER(Disk), d, d, d, d, d]),
         "or_disk_velocity": (None, [C.POINTER(Disk), d, d, d, d, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "or_apply_coupling": (None, [i, i, d, d, p, p, C.POINTER(Disk), d, i, p]),
@@ -380,15 +381,23 @@ def round_storage(x, storage):
 
 
 def mg_refine(n, div, iterations, min_size=4, pre=2, post=2, coarse=40, coarse_n=64, storage="fp32", p=None,
-              lx=TWO_PI):
-    """GPU-style refinement projection; returns (p, max|div - L p| before each step 0..iterations)."""
+              lx=TWO_PI, coarse_exact=False):
+    """GPU-style refinement projection; returns (p, max|div - L p| before each step 0..iterations).
+    coarse_exact: the hierarchy stops at its first level <= coarse_n, solved exactly."""
     p = np.zeros(n * n) if p is None else _f(p).copy()
     rmax = np.zeros(iterations + 1)
     rc = lib().or_mg_refine(n, lx, _f(div), p, iterations, min_size, pre, post, coarse, coarse_n,
-                            STORAGE[storage], rmax)
+                            STORAGE[storage], int(coarse_exact), rmax)
     if rc < 1:
-        raise ValueError("no multigrid levels")
+        raise ValueError("no multigrid levels" if rc == -1 else "no level at or below coarse_n")
     return p, rmax
+
+
+def poisson_dft(n, rhs, lx=TWO_PI):
+    """Exact periodic 5-point solve of L x = rhs - mean(rhs) for any n (zero-mean x)."""
+    x = np.empty(n * n)
+    lib().or_poisson_dft(n, lx, _f(rhs), x)
+    return x
 
 
 # ------------------------------------------------------ lid-driven cavity (walls)

@@ -22,7 +22,7 @@ def test_library_builds_and_exports_every_declared_symbol():
     assert len(names) >= 25
     for n in names:
         assert hasattr(L, n), f"{n} declared in fluidsim.h but not exported"
-    assert L.fs_abi_version() == 2
+    assert L.fs_abi_version() == 3
 
 
 def test_python_binding_covers_every_declared_symbol():

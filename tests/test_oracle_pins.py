@@ -727,6 +727,43 @@ def test_refinement_without_rounding_equals_plain_vcycles(O, n, coarse_n):
     assert np.all(np.diff(np.log(rmax)) < -1.0)   # each step gains more than a factor e
 
 
+@pytest.mark.parametrize("n", [64, 48, 50, 16, 6])
+def test_poisson_dft_is_the_exact_zero_mean_inverse(O, n):
+    # L x = r - mean(r) to rounding, x of zero mean, for any n (the exact coarsest-level
+    # solve); equal to the FFT projection where that applies; a dropped factor of the
+    # eigenvalue, a wrong sign or a kept mean each fail it
+    rng = np.random.default_rng(n)
+    r = rng.standard_normal(n * n)
+    x = O.poisson_dft(n, r)
+    lap = O.laplacian(n, n, x)
+    assert np.max(np.abs(lap - (r - r.mean()))) < 1e-11 * np.max(np.abs(r))
+    assert abs(x.mean()) < 1e-13 * np.max(np.abs(x))
+    if n & (n - 1) == 0:
+        assert np.max(np.abs(x - O.poisson_fft(n, n, r))) < 1e-12 * np.max(np.abs(x))
+    for bad in (2 * x, -x, x + 0.1):
+        assert np.max(np.abs(O.laplacian(n, n, bad) - (r - r.mean()))) > 1e-3 or abs(bad.mean()) > 1e-3
+
+
+@pytest.mark.parametrize("n", [256, 96, 64])
+def test_exact_coarse_refinement_converges_faster_than_sweeps(O, n):
+    # the hierarchy stopped at its first level <= 64 and solved exactly: each V-cycle
+    # contracts at least as fast as the reference's cycle down to 4^2; a grid of at most
+    # 64^2 is solved in one refinement step
+    div = _zero_mean_div(O, n, 5)
+    _, r_ex = O.mg_refine(n, div, 4, min_size=4, coarse_exact=True)
+    _, r_sw = O.mg_refine(n, div, 4, min_size=4)
+    assert r_ex[0] == r_sw[0]
+    if n <= 64:
+        # exact but for the FP32 storage of the level-0 residual (2^-24 relative)
+        assert r_ex[1] < 2e-7 * r_ex[0]
+        assert r_sw[1] > 1e-3 * r_sw[0]
+    else:
+        assert np.all(r_ex[1:] <= r_sw[1:] * 1.02)
+        assert np.all(np.diff(np.log(r_ex)) < -1.0)
+    with pytest.raises(ValueError):
+        O.mg_refine(260, _zero_mean_div(O, 260, 1), 1, min_size=130, coarse_exact=True)
+
+
 def test_bf16_refinement_still_converges_to_the_gate(O):
     # BF16-stored corrections: 8-bit mantissa per step, FP64 residual; the iteration
     # keeps contracting and reaches a 1e-6 gate on a unit-scale rhs
