@@ -99,6 +99,78 @@ __device__ __forceinline__ void load_window(float* __restrict__ dst, const T* __
   }
 }
 
+// A window's loads held in registers: issue() starts them, commit() writes them to LDS
+// as FP32 (the values load_window stores). A kernel that issues every window it needs
+// before committing any pays one memory latency for all of them.
+template <int RW, int RH, int NT, typename T>
+struct Window {
+  static constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+  static constexpr bool PAIRS = sizeof(T) == 2 && RW % 2 == 0;
+  static constexpr int RW2 = RW / 2, CNT2 = CNT / 2, K2 = (CNT2 + NT - 1) / NT;
+  uint32_t w[PAIRS ? K : 1];  // BF16: two cells per word (even origin) or one cell's bits
+  float f[PAIRS ? 1 : K];
+  bool pairs;
+  __device__ __forceinline__ void issue(const T* __restrict__ src, int n, int ox, int oy) {
+    if constexpr (PAIRS) {
+      pairs = (ox & 1) == 0;
+      if (pairs) {
+#pragma unroll
+        for (int k = 0; k < K2; k++) {
+          const int q = threadIdx.x + k * NT;
+          if (q < CNT2) {
+            const int ly = q / RW2, lx = 2 * (q - ly * RW2);
+            w[k] = *(const uint32_t*)(src + (size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int q = threadIdx.x + k * NT;
+          if (q < CNT) {
+            const int ly = q / RW, lx = q - ly * RW;
+            w[k] = *(const uint16_t*)(src + (size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n));
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (q < CNT) {
+          const int ly = q / RW, lx = q - ly * RW;
+          f[k] = ld(src[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(float* __restrict__ dst) const {
+    if constexpr (PAIRS) {
+      if (pairs) {
+#pragma unroll
+        for (int k = 0; k < K2; k++) {
+          const int q = threadIdx.x + k * NT;
+          if (q < CNT2) {
+            dst[2 * q] = __uint_as_float(w[k] << 16);
+            dst[2 * q + 1] = __uint_as_float(w[k] & 0xffff0000u);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const int q = threadIdx.x + k * NT;
+          if (q < CNT) dst[q] = __uint_as_float(w[k] << 16);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (q < CNT) dst[q] = f[k];
+      }
+    }
+  }
+};
+
 // One red-black half-sweep on the LDS region [LO, RW-LO) x [LO, RH-LO), visiting
 // only the cells of `colour` (par0 = colour parity of LDS cell (0,0)). A colour pass
 // reads only the other colour, so all of a thread's reads are issued before its writes.
@@ -289,6 +361,22 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
   int t = xcd_tile(nblocks);
   int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
   int ox = tx0 - H, oy = ty0 - H;
+  // every load (p, e, the face velocities) in flight before the first LDS store
+  constexpr int CNTV = RW * RH, KV = (CNTV + NT - 1) / NT;
+  float uL[KV], uR[KV], vB[KV], vT[KV];
+#pragma unroll
+  for (int k = 0; k < KV; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CNTV) {
+      int ly = q / RW, lx = q - ly * RW;
+      int gi = wrapn(ox + lx, n), gj = wrapn(oy + ly, n);
+      int gip = gi + 1 == n ? 0 : gi + 1, gjp = gj + 1 == n ? 0 : gj + 1;
+      uL[k] = u[(size_t)gj * n + gi];
+      uR[k] = u[(size_t)gj * n + gip];
+      vB[k] = v[(size_t)gj * n + gi];
+      vT[k] = v[(size_t)gjp * n + gi];
+    }
+  }
   {
     constexpr int CNT = PW * PH, K = (CNT + NT - 1) / NT;
     double tp[K];
@@ -312,21 +400,7 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
   __syncthreads();
   double amax = 0.0;
   {
-    constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
-    float uL[K], uR[K], vB[K], vT[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      int q = threadIdx.x + k * NT;
-      if (q < CNT) {
-        int ly = q / RW, lx = q - ly * RW;
-        int gi = wrapn(ox + lx, n), gj = wrapn(oy + ly, n);
-        int gip = gi + 1 == n ? 0 : gi + 1, gjp = gj + 1 == n ? 0 : gj + 1;
-        uL[k] = u[(size_t)gj * n + gi];
-        uR[k] = u[(size_t)gj * n + gip];
-        vB[k] = v[(size_t)gj * n + gi];
-        vT[k] = v[(size_t)gjp * n + gi];
-      }
-    }
+    constexpr int CNT = CNTV, K = KV;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       int q = threadIdx.x + k * NT;
@@ -388,22 +462,19 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
   // lowest coarse node any region cell interpolates from (ox, oy are even)
   int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
   constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
-  constexpr bool PAIRS = sizeof(S) == 2;  // BF16 x: staged through LDS two cells per load
-  float xv[PAIRS ? 1 : K];
-  if constexpr (PAIRS) {
-    load_window<RW, RH, NT>(sx, xin, n, ox, oy);
-  } else {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      int q = threadIdx.x + k * NT;
-      if (q < CNT) {
-        int ly = q / RW, lx = q - ly * RW;
-        xv[k] = ld(xin[(size_t)wrapn(oy + ly, n) * n + wrapn(ox + lx, n)]);
-      }
-    }
-  }
-  load_window<CW, CH, NT>(sc, xc, cn, cox, coy);
-  load_window<RW, RH, NT>(sr, rhs, n, ox, oy);
+  // x, the coarse correction and rhs all in flight at once; FP32 x stays in registers,
+  // BF16 x is staged through LDS two cells per load
+  using WX = Window<RW, RH, NT, S>;
+  constexpr bool PAIRS = WX::PAIRS;
+  WX wx;
+  Window<CW, CH, NT, SO> wc;
+  Window<RW, RH, NT, S> wr;
+  wx.issue(xin, n, ox, oy);
+  wc.issue(xc, cn, cox, coy);
+  wr.issue(rhs, n, ox, oy);
+  if constexpr (PAIRS) wx.commit(sx);
+  wc.commit(sc);
+  wr.commit(sr);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -418,7 +489,7 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
       float a = sc[cy * CW + cx], b = sc[cy * CW + cx + 1];
       float c = sc[(cy + 1) * CW + cx], d = sc[(cy + 1) * CW + cx + 1];
       float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-      sx[q] = (PAIRS ? sx[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
+      sx[q] = (PAIRS ? sx[q] : wx.f[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
     }
   }
   __syncthreads();
@@ -463,25 +534,29 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   const int ox0 = tx0 - H, oy0 = ty0 - H;
   const int ox1 = (tx0 >> 1) - E, oy1 = (ty0 >> 1) - E;
   const int cox1 = (ox1 >> 1) - 1, coy1 = (oy1 >> 1) - 1;
+  // every window of both levels in flight at once; FP32 x stays in registers, BF16 x
+  // is staged through LDS two cells per load
+  using WX1 = Window<RW1, RH1, NT, S>;
+  using WX0 = Window<RW0, RH0, NT, S>;
+  WX1 wx1;
+  Window<CW1, CH1, NT, SO> wc1;
+  Window<RW1, RH1, NT, S> wr1;
+  WX0 wx0;
+  Window<RW0, RH0, NT, S> wr0;
+  wx1.issue(x1in, n1, ox1, oy1);
+  wc1.issue(xc2, n2, cox1, coy1);
+  wr1.issue(rhs1, n1, ox1, oy1);
+  wx0.issue(x0in, n0, ox0, oy0);
+  wr0.issue(rhs0, n0, ox0, oy0);
   // ---- level l + 1: x + prolong(correction of level l + 2), then the post-sweeps
   {
     constexpr int CNT = RW1 * RH1, K = (CNT + NT - 1) / NT;
-    constexpr bool PAIRS = sizeof(S) == 2;  // BF16 x: staged through LDS two cells per load
-    float xv[PAIRS ? 1 : K];
-    if constexpr (PAIRS) {
-      load_window<RW1, RH1, NT>(sx1, x1in, n1, ox1, oy1);
-    } else {
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int q = threadIdx.x + k * NT;
-        if (q < CNT) {
-          const int ly = q / RW1, lx = q - ly * RW1;
-          xv[k] = ld(x1in[(size_t)wrapn(oy1 + ly, n1) * n1 + wrapn(ox1 + lx, n1)]);
-        }
-      }
-    }
-    load_window<CW1, CH1, NT>(sc1, xc2, n2, cox1, coy1);
-    load_window<RW1, RH1, NT>(sr1, rhs1, n1, ox1, oy1);
+    constexpr bool PAIRS = WX1::PAIRS;
+    if constexpr (PAIRS) wx1.commit(sx1);
+    wc1.commit(sc1);
+    wr1.commit(sr1);
+    if constexpr (WX0::PAIRS) wx0.commit(sx0);
+    wr0.commit(sr0);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -496,7 +571,7 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
         const float a = sc1[cy * CW1 + cx], b = sc1[cy * CW1 + cx + 1];
         const float c = sc1[(cy + 1) * CW1 + cx], d = sc1[(cy + 1) * CW1 + cx + 1];
         const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-        sx1[q] = (PAIRS ? sx1[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
+        sx1[q] = (PAIRS ? sx1[q] : wx1.f[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
       }
     }
     __syncthreads();
@@ -505,22 +580,7 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   // ---- level l: x + prolong(the level-(l+1) correction, from LDS), then the post-sweeps
   {
     constexpr int CNT = RW0 * RH0, K = (CNT + NT - 1) / NT;
-    constexpr bool PAIRS = sizeof(S) == 2;
-    float xv[PAIRS ? 1 : K];
-    if constexpr (PAIRS) {
-      load_window<RW0, RH0, NT>(sx0, x0in, n0, ox0, oy0);
-    } else {
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        const int q = threadIdx.x + k * NT;
-        if (q < CNT) {
-          const int ly = q / RW0, lx = q - ly * RW0;
-          xv[k] = ld(x0in[(size_t)wrapn(oy0 + ly, n0) * n0 + wrapn(ox0 + lx, n0)]);
-        }
-      }
-    }
-    load_window<RW0, RH0, NT>(sr0, rhs0, n0, ox0, oy0);
-    __syncthreads();
+    constexpr bool PAIRS = WX0::PAIRS;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int q = threadIdx.x + k * NT;
@@ -535,7 +595,7 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
         const float a = rnd<SO>(sx1[cy * RW1 + cx]), b = rnd<SO>(sx1[cy * RW1 + cx + 1]);
         const float c = rnd<SO>(sx1[(cy + 1) * RW1 + cx]), d = rnd<SO>(sx1[(cy + 1) * RW1 + cx + 1]);
         const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
-        sx0[q] = (PAIRS ? sx0[q] : xv[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
+        sx0[q] = (PAIRS ? sx0[q] : wx0.f[PAIRS ? 0 : k]) + (top * (1.f - fy) + bot * fy);
       }
     }
     __syncthreads();
