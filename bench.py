@@ -530,6 +530,9 @@ def main():
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / psteps,
                 "algorithmic_bytes_per_launch": nbytes / max(launches, 1),
                 "share_of_step": ms / psteps / step_ms,
+                # every timed kernel class summed per step (the profiled pass launches one by
+                # one with events; its kernels are the timed steps' kernels), against ms_per_step
+                "kernels_ms_per_step": sum(v[0] for v in prof.values()) / psteps,
                 # plain 1 GiB device copies on the same GPU: the streaming ceiling in practice
                 "copy_ceiling_gbs": None if a.no_copy_ceiling else copy_ceiling(),
                 "all_kernels": {k: {"us_per_launch": v[0] / max(v[1], 1) * 1e3, "launches_per_step": v[1] / psteps,
@@ -620,6 +623,7 @@ def main():
             "gated_steps": st1["gated_steps"] - st0["gated_steps"],
             "gate_misses": st1["gate_misses"] - st0["gate_misses"],
             "graph_launches": st1["graph_launches"] - st0["graph_launches"],
+            "graph_captures": st1["graph_captures"] - st0["graph_captures"],
             "tiles": [ln.split(" | ")[0] for ln in tiles.strip().splitlines()],
         },
         "roofline": roof,

@@ -1469,24 +1469,26 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
   return enqueue_dye(s);
 }
 
-// The body as a graph, captured on first use of its key and replayed after.
-int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
+const fs_solver_t::StepGraph* find_graph(const fs_solver_t* s, const BodyKey& key) {
   for (const auto& g : s->graphs)
     if (g.k == key.k && g.a0 == key.a0 && g.zero_p == key.zero_p && g.cur == key.cur && g.keep == key.keep &&
-        g.hist == key.hist && g.dye == key.dye) {
-      HIPCHK(hipGraphLaunch(g.exec, s->stream));
-      s->graph_launches++;
-      s->cur = g.cur_after;
-      s->dye_cur = g.dye_after;
-      s->have_maxabs = true;
-      if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) s->have_hist = true;
-      return FS_OK;
-    }
-  if (s->graphs.size() >= 32) destroy_graphs(s);  // a drifting prediction: start afresh
+        g.hist == key.hist && g.dye == key.dye)
+      return &g;
+  return nullptr;
+}
+
+// Capture the body of `key` into a new cached graph without launching it. Capturing
+// enqueues nothing: the solver state the enqueue functions advance (velocity and dye
+// parity, the flags) is put back afterwards.
+int capture_body(fs_solver_t* s, const BodyKey& key) {
+  const int cur0 = s->cur, dye0 = s->dye_cur;
+  const bool maxabs0 = s->have_maxabs, hist0 = s->have_hist;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_gated_body(s, key);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  const int cur_after = s->cur, dye_after = s->dye_cur;
+  s->cur = cur0, s->dye_cur = dye0, s->have_maxabs = maxabs0, s->have_hist = hist0;
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc ? rc : fail(FS_E_DEVICE, "capturing the step failed: %s", hipGetErrorString(e));
@@ -1495,10 +1497,60 @@ int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(FS_E_DEVICE, "instantiating the step graph failed: %s", hipGetErrorString(e));
-  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, key.hist, key.dye, s->cur, s->dye_cur, exec});
+  s->graphs.push_back({key.k, key.a0, key.zero_p, key.cur, key.keep, key.hist, key.dye, cur_after, dye_after, exec});
   s->graph_captures++;
-  HIPCHK(hipGraphLaunch(exec, s->stream));
+  return FS_OK;
+}
+
+// The body as a graph, captured on first use of its key (unless captured ahead) and
+// replayed after.
+int launch_gated_body_graph(fs_solver_t* s, const BodyKey& key) {
+  const fs_solver_t::StepGraph* g = find_graph(s, key);
+  if (!g) {
+    if (s->graphs.size() >= 32) destroy_graphs(s);  // a drifting prediction: start afresh
+    int rc = capture_body(s, key);
+    if (rc) return rc;
+    g = &s->graphs.back();
+  }
+  HIPCHK(hipGraphLaunch(g->exec, s->stream));
   s->graph_launches++;
+  s->cur = g->cur_after;
+  s->dye_cur = g->dye_after;
+  s->have_maxabs = true;
+  if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) s->have_hist = true;
+  return FS_OK;
+}
+
+// The key of the next gated step body if its V-cycle prediction were `pred`.
+BodyKey gated_key(const fs_solver_t* s, int pred) {
+  const int maxc = std::max(0, s->cfg.pressure_iterations);
+  pred = std::max(0, std::min(pred, maxc));
+  BodyKey key{};
+  key.k = std::min(pred + GATE_SPARE, maxc);
+  key.a0 = s->pidx;
+  key.zero_p = zero_start(s) ? 1 : 0;
+  key.cur = s->cur;
+  key.hist = s->have_hist ? 1 : 0;
+  key.dye = s->dye[0] ? s->dye_cur : -1;
+  int a0e = s->pidx;
+  if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) a0e = 1 - (a0e & 1);
+  if (learned_guess(s)) a0e = 1 - (a0e & 1);
+  key.keep = (a0e + pred + 1) & 1;
+  return key;
+}
+
+// After a step is enqueued: capture one body the next step may need and that is not
+// cached yet (its current prediction, then one V-cycle more, then one fewer). A
+// capture on demand leaves the device idle while the host records and instantiates
+// it; ahead of time it overlaps the steps already queued.
+int capture_ahead(fs_solver_t* s) {
+  if (s->graphs.size() >= 24) return FS_OK;
+  for (int d : {0, 1, -1}) {
+    const int pred = s->last_pred + d;
+    if (pred < 0 || pred > s->cfg.pressure_iterations) continue;
+    const BodyKey key = gated_key(s, pred);
+    if (!find_graph(s, key)) return capture_body(s, key);
+  }
   return FS_OK;
 }
 
@@ -1512,23 +1564,12 @@ int step_gated(fs_solver_t* s, bool use_cfl, double value) {
     rc = consume_records(s, false);
   }
   if (rc) return rc;
-  const int maxc = std::max(0, s->cfg.pressure_iterations);
-  const bool learned = learned_guess(s);
-  if (learned && !s->guess_net.loaded) return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
-  const int pred = std::max(0, std::min(s->last_pred, maxc));
-  BodyKey key{};
-  key.k = std::min(pred + GATE_SPARE, maxc);
-  key.a0 = s->pidx;
-  key.zero_p = zero_start(s) ? 1 : 0;
-  key.cur = s->cur;
-  key.hist = s->have_hist ? 1 : 0;
-  key.dye = s->dye[0] ? s->dye_cur : -1;
-  int a0e = s->pidx;
-  if (s->cfg.guess == FS_GUESS_EXTRAPOLATED && !key.zero_p) a0e = 1 - (a0e & 1);
-  if (learned) a0e = 1 - (a0e & 1);
-  key.keep = (a0e + pred + 1) & 1;
+  if (learned_guess(s) && !s->guess_net.loaded)
+    return fail(FS_E_INVALID_ARG, "the learned guess has no weights loaded");
+  const BodyKey key = gated_key(s, s->last_pred);
   if ((rc = enqueue_head(s, use_cfl, value, key.k + 1))) return rc;
-  if (s->use_graph && !s->solid_on && s->splats.empty()) rc = launch_gated_body_graph(s, key);
+  const bool graph = s->use_graph && !s->solid_on && s->splats.empty();
+  if (graph) rc = launch_gated_body_graph(s, key);
   else rc = enqueue_gated_body(s, key);
   if (rc) return rc;
   hipEvent_t ev = nullptr;
@@ -1544,7 +1585,7 @@ int step_gated(fs_solver_t* s, bool use_cfl, double value) {
   s->have_p = true;
   s->steps++;
   s->gated_steps++;
-  return FS_OK;
+  return graph ? capture_ahead(s) : FS_OK;
 }
 
 
@@ -2628,7 +2669,7 @@ void fs_config_default(fs_config* c) {
   c->device = 0;
   c->mg_fused = 1;
   c->gated = 1;
-  c->graph = 1;
+  c->graph = 0;
   c->dye_fade = 0.0;
   c->mg_coarse_exact = 0;
 }

@@ -59,7 +59,7 @@ class GPUFluidSolver:
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
                  mgPrecision="fp32", guess="zero", mgFused=True, autotune=True,
-                 advFused=True, gated=True, graph=True, mgCoarseSolve="sweeps"):
+                 advFused=True, gated=True, graph=False, mgCoarseSolve="sweeps"):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx

@@ -114,9 +114,11 @@ typedef struct fs_config {
                                  the host (see fs_stats); 0: every projection reads its residuals
                                  back and adds V-cycles until the gate is met. Same bits when
                                  both run the same V-cycles. */
-  int graph;                  /* 1 (default): a gated step's launches after its first replay as
-                                 a captured hipGraph (one per V-cycle count, buffer parity and
-                                 initial-pressure kind); 0: launched one by one. Same bits. */
+  int graph;                  /* 1: a gated step's launches replay as a captured hipGraph (one
+                                 per V-cycle count, buffer parity and initial-pressure kind,
+                                 captured on first use or a step ahead); 0 (default): launched one
+                                 by one, which measured 2.5 % faster per step at 1024^2 (the host
+                                 stays ahead of the device either way). Same bits. */
   double dye_fade;            /* dyeFade: the dye is multiplied by 1 - dye_fade each step; default 0 */
   int mg_coarse_exact;        /* 0 (default): the reference's V-cycle down to mg_min_size with
                                  mg_coarse sweeps on the coarsest level; 1: the hierarchy stops at

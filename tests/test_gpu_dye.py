@@ -59,8 +59,9 @@ def test_dye_rides_the_projected_flow_and_fades(fs, O):
     X, Y = np.meshgrid((np.arange(n) + 0.5) * h, (np.arange(n) + 0.5) * h)
     rgba = np.stack([0.5 + 0.4 * np.sin(X), 0.5 + 0.4 * np.cos(2 * Y), 0.9 * np.exp(-((X - 3) ** 2 + (Y - 3) ** 2)),
                      np.zeros_like(X)], axis=-1).reshape(-1)
+    # the dye pass inside the captured step graph
     with fs.GPUFluidSolver(0, n=n, nu=0.01, advection="maccormack", pressureSolver="multigrid",
-                           pressureIterations=40, tolerance=1e-9, mgMinSize=4, dyeFade=fade) as g:
+                           pressureIterations=40, tolerance=1e-9, mgMinSize=4, dyeFade=fade, graph=True) as g:
         g.uploadVelocity(u0, v0)
         g.uploadDye(rgba)
         for _ in range(steps):

@@ -65,7 +65,8 @@ def test_gated_steps_equal_host_checked_steps(n, guess, forcing, prec):
     assert sr["gated_steps"] == 0
     assert se["gated_steps"] == steps - 1 and se["graph_launches"] == 0 and se["gate_misses"] == 0
     assert sg["gated_steps"] == steps - 1 and sg["graph_launches"] == steps - 1 and sg["gate_misses"] == 0
-    assert 1 <= sg["graph_captures"] <= steps - 1
+    # one capture on demand or ahead of time per step at most, besides the step's own
+    assert 1 <= sg["graph_captures"] <= 2 * (steps - 1)
     for other in (eager, graph):
         for a, b in zip(ref, other):
             assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
