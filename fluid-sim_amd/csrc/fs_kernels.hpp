@@ -378,6 +378,17 @@ __device__ __forceinline__ float bilerp_w(const AdvWin& a, int nx, int ny, float
   return fmaf(bot - top, fy, top);
 }
 
+// bilerp_w at a point whose fractions are exactly 1/2 (a face's other velocity
+// component, sampled at the face): the nodes (i0, j0) .. (i0 + 1, j0 + 1) indexed
+// directly, the same FMAs, so the same bits without the floor and fraction arithmetic
+template <bool FAST, bool WRAP = true>
+__device__ __forceinline__ float mid_w(const AdvWin& a, int nx, int ny, int i0, int j0) {
+  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, i0, j0);
+  float top = fmaf(q.y - q.x, 0.5f, q.x);
+  float bot = fmaf(q.w - q.z, 0.5f, q.z);
+  return fmaf(bot - top, 0.5f, top);
+}
+
 template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ void bounds4_w(const AdvWin& a, int nx, int ny, float gx, float gy, float& lo, float& hi) {
   float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)floorf(gx), (int)floorf(gy));
@@ -412,12 +423,12 @@ template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ void vel_u_face(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float& u1,
                                            float& v1) {
   u1 = node_w<FAST, WRAP>(u, nx, ny, i, j);
-  v1 = bilerp_w<FAST, WRAP>(v, nx, ny, i - 0.5f, j + 0.5f);
+  v1 = mid_w<FAST, WRAP>(v, nx, ny, i - 1, j);  // bilerp at (i - 1/2, j + 1/2)
 }
 template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ void vel_v_face(const AdvWin& u, const AdvWin& v, int nx, int ny, int i, int j, float& u1,
                                            float& v1) {
-  u1 = bilerp_w<FAST, WRAP>(u, nx, ny, i + 0.5f, j - 0.5f);
+  u1 = mid_w<FAST, WRAP>(u, nx, ny, i, j - 1);  // bilerp at (i + 1/2, j - 1/2)
   v1 = node_w<FAST, WRAP>(v, nx, ny, i, j);
 }
 
@@ -544,7 +555,7 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
     {  // u face at (i, j+1/2)
       float X = (float)i, Y = j + 0.5f;
       u1 = inu ? wu.w[wju * AUW + wiu] : u[(size_t)j * nx + i];
-      v1 = bilerp_w<FAST, WRAP>(wv, nx, ny, X - 0.5f, Y);
+      v1 = mid_w<FAST, WRAP>(wv, nx, ny, i - 1, j);  // bilerp at (X - 1/2, Y)
       trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<0, FAST, WRAP>(sfu, fox, foy, wu, wv, nx, ny, Xb, Yb - 0.5f, cx, cy);
       float fo = inf ? sfu[wjf * AFW + wif] : fwd_node_far<0>(wu, wv, nx, ny, i, j, cx, cy);
@@ -561,7 +572,7 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
     }
     {  // v face at (i+1/2, j)
       float X = i + 0.5f, Y = (float)j;
-      u1 = bilerp_w<FAST, WRAP>(wu, nx, ny, X, Y - 0.5f);
+      u1 = mid_w<FAST, WRAP>(wu, nx, ny, i, j - 1);  // bilerp at (X, Y - 1/2)
       v1 = inu ? wv.w[wju * AUW + wiu] : v[(size_t)j * nx + i];
       trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<1, FAST, WRAP>(sfv, fox, foy, wu, wv, nx, ny, Xb - 0.5f, Yb, cx, cy);
