@@ -1128,9 +1128,10 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
 }
 
 // ------------------------------------------------- exact coarsest-level solve
-// The periodic 5-point Laplacian of an n x n level (n <= COARSE_N) is T (x) I + I (x) T
-// with T the 1D second difference, and T = Q diag(lam) Q^T for the real Fourier basis
-// Q (orthonormal columns: the constant, cos and sin pairs, the alternating mode). So
+// The 5-point Laplacian of an n x n level (n <= COARSE_N) is T (x) I + I (x) T with T
+// the 1D second difference, and T = Q diag(lam) Q^T for an orthonormal eigenbasis Q:
+// the real Fourier basis on a periodic level, the DCT-II basis on a walled (zero-flux)
+// one; the host builds Q and W for either. So
 // L x = r - mean(r) is solved by x = Q ((Q^T r Q) * W) Q^T with W[a][b] = 1 / (lam_a +
 // lam_b) and W[0][0] = 0 (the zero-mean solution, as a pseudo-inverse gives). The four
 // 64 x 64 x 64 products run on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with
@@ -1169,10 +1170,17 @@ __device__ __forceinline__ ex_f32x4 ex_mma3(ex_f32x4 acc, ex_bf16x8 ah, ex_bf16x
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
 }
 
+// base: null for one grid, else one grid per workgroup at element offset base[blockIdx.x]
+// of xg and rg (a batch of cavities, each with its own level of n x n cells)
 template <typename S>
 __global__ void __launch_bounds__(EXT)
-k_mg_coarse_exact(float* __restrict__ xg, const S* __restrict__ rg, int n, const uint4* __restrict__ tab, Gate gate) {
+k_mg_coarse_exact(float* __restrict__ xg, const S* __restrict__ rg, int n, const long* __restrict__ base,
+                  const uint4* __restrict__ tab, Gate gate) {
   if (gate_closed(gate)) return;
+  if (base) {
+    xg += base[blockIdx.x];
+    rg += base[blockIdx.x];
+  }
   // two matrices, heads and tails; A/B alternate as input and output of the products
   __shared__ __align__(16) __bf16 sah[EXN * EXB], sal[EXN * EXB], sbh[EXN * EXB], sbl[EXN * EXB];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -688,8 +688,8 @@ int mg_vcycle(fs_solver_t* s) {
     Level& C = s->lv[Lc - 1];
     if (s->ex_q) {
       int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n + 16.0 * fsk::EX_TAB, fsk::k_mg_coarse_exact<float>,
-                      dim3(1), dim3(fsk::EXT), C.x, (const float*)C.rhs, C.n, (const uint4*)s->ex_q,
-                      fsk::gate_open());
+                      dim3(1), dim3(fsk::EXT), C.x, (const float*)C.rhs, C.n, (const long*)nullptr,
+                      (const uint4*)s->ex_q, fsk::gate_open());
       if (rc) return rc;
     } else if (C.n <= fsk::COARSE_N) {
       int rc = launch(s, PK_COARSE, 8.0 * C.n * C.n, fsk::k_mg_coarse_ml<float>, dim3(1), dim3(fsk::CTH), C.x,
@@ -877,7 +877,7 @@ int launch_coarse_s(fs_solver_t* s, int from) {
   if (s->ex_q)
     return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n + 16.0 * fsk::EX_TAB,
                   fsk::k_mg_coarse_exact<S>, dim3(1), dim3(fsk::EXT), C.xb, as<S>((const float*)C.rhs), C.n,
-                  (const uint4*)s->ex_q, s->gate_v);
+                  (const long*)nullptr, (const uint4*)s->ex_q, s->gate_v);
   return launch(s, PK_COARSE, (sizeof(S) + 4.0) * C.n * C.n, fsk::k_mg_coarse_ml<S, float>, dim3(1),
                 dim3(fsk::CTH), C.xb, as<S>((const float*)C.rhs), level_table(s, from, (int)s->lv.size()),
                 s->cfg.mg_pre, s->cfg.mg_post, s->cfg.mg_coarse, (const float*)s->pinv, s->gate_v);
@@ -1639,13 +1639,15 @@ int build_coarse_inverse(fs_solver_t* s) {
   return FS_OK;
 }
 
-// Tables of the exact coarsest-level solve (k_mg_coarse_exact), built in FP64: the
-// real Fourier basis Q of the periodic second difference on n nodes (column 0 the
-// constant, columns 2f-1 and 2f the cosine and sine of frequency f, column n-1 the
-// alternating mode when n is even; orthonormal) and W[a][b] = 1 / (lam_a + lam_b) with
-// lam of frequency f = -(4 / h^2) sin^2(pi f / n) and W[0][0] = 0, zero-padded to
-// 64 x 64. Q goes in as BF16 heads and tails in the order the lanes read them, W as
-// FP32 at each lane's outputs (fsk::EX_TAB 16-byte units).
+// Table of the exact coarsest-level solve (k_mg_coarse_exact), built in FP64 for an
+// n x n level of spacing h: an orthonormal eigenbasis Q of the 1D second difference and
+// W[a][b] = 1 / (lam_a + lam_b) with W[0][0] = 0, zero-padded to 64 x 64. Periodic: the
+// real Fourier basis (column 0 the constant, columns 2f-1 and 2f the cosine and sine of
+// frequency f, column n-1 the alternating mode when n is even), lam of frequency f =
+// -(4 / h^2) sin^2(pi f / n). Walls (cell-centred, zero flux): the DCT-II basis
+// cos(pi k (j + 1/2) / n), lam_k = -(4 / h^2) sin^2(pi k / (2 n)). Q goes in as BF16
+// heads and tails in the order the lanes read them, W as FP32 at each lane's outputs
+// (fsk::EX_TAB 16-byte units).
 uint16_t bf16_bits(double x) {
   float f = (float)x;
   uint32_t u;
@@ -1660,16 +1662,21 @@ double bf16_value(uint16_t b) {
   return f;
 }
 
-int build_coarse_exact(fs_solver_t* s) {
-  const Level& C = s->lv.back();
-  const int n = C.n, E = fsk::EXN;
-  if (n > E) return fail(FS_E_INVALID_ARG, "exact coarse solve of a %d-cell level (at most %d)", n, E);
+std::vector<uint8_t> exact_coarse_table(int n, double h, bool walls) {
+  const int E = fsk::EXN;
   std::vector<double> Q((size_t)E * E, 0.0), lam(E, 0.0);
   for (int c = 0; c < n; c++) {
-    int f = (c + 1) / 2;  // 0; 1, 1; 2, 2; ...; n/2 for the last column when n is even
-    bool alt = n % 2 == 0 && c == n - 1, sine = c > 0 && c % 2 == 0 && !alt;
-    double sh = std::sin(M_PI * f / n);
-    lam[c] = -4.0 * sh * sh / (C.dx * C.dx);
+    if (walls) {
+      const double sh = std::sin(M_PI * c / (2.0 * n));
+      lam[c] = -4.0 * sh * sh / (h * h);
+      const double a = c == 0 ? std::sqrt(1.0 / n) : std::sqrt(2.0 / n);
+      for (int j = 0; j < n; j++) Q[(size_t)j * E + c] = a * std::cos(M_PI * c * (j + 0.5) / n);
+      continue;
+    }
+    const int f = (c + 1) / 2;  // 0; 1, 1; 2, 2; ...; n/2 for the last column when n is even
+    const bool alt = n % 2 == 0 && c == n - 1, sine = c > 0 && c % 2 == 0 && !alt;
+    const double sh = std::sin(M_PI * f / n);
+    lam[c] = -4.0 * sh * sh / (h * h);
     for (int j = 0; j < n; j++) {
       double v;
       if (c == 0) v = 1.0 / std::sqrt((double)n);
@@ -1679,7 +1686,8 @@ int build_coarse_exact(fs_solver_t* s) {
       Q[(size_t)j * E + c] = v;
     }
   }
-  std::vector<uint16_t> qt((size_t)4 * 512 * 8);
+  std::vector<uint8_t> out((size_t)fsk::EX_TAB * 16);
+  uint16_t* qt = (uint16_t*)out.data();
   for (int T = 0; T < 4; T++)
     for (int lane = 0; lane < 64; lane++)
       for (int kk = 0; kk < 2; kk++)
@@ -1694,7 +1702,7 @@ int build_coarse_exact(fs_solver_t* s) {
           qt[2 * 512 * 8 + o] = bh;
           qt[3 * 512 * 8 + o] = bf16_bits(b - bf16_value(bh));
         }
-  std::vector<float> wt((size_t)16 * 64 * 4);
+  float* wt = (float*)(out.data() + 4 * 512 * 8 * 2);
   for (int w = 0; w < 16; w++)
     for (int lane = 0; lane < 64; lane++)
       for (int i = 0; i < 4; i++) {
@@ -1703,10 +1711,16 @@ int build_coarse_exact(fs_solver_t* s) {
             (a >= n || b >= n || (a == 0 && b == 0)) ? 0.f : (float)(1.0 / (lam[a] + lam[b]));
       }
   static_assert(fsk::EX_TAB * 16 == 4 * 512 * 8 * 2 + 16 * 64 * 4 * 4, "table layout");
+  return out;
+}
+
+int build_coarse_exact(fs_solver_t* s) {
+  const Level& C = s->lv.back();
+  if (C.n > fsk::EXN) return fail(FS_E_INVALID_ARG, "exact coarse solve of a %d-cell level (at most %d)", C.n, fsk::EXN);
+  const std::vector<uint8_t> t = exact_coarse_table(C.n, C.dx, false);
   int rc = dalloc(&s->ex_q, (size_t)fsk::EX_TAB * 4);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(s->ex_q, qt.data(), qt.size() * 2, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy((char*)s->ex_q + qt.size() * 2, wt.data(), wt.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->ex_q, t.data(), t.size(), hipMemcpyHostToDevice));
   return FS_OK;
 }
 
@@ -1964,6 +1978,7 @@ struct fs_batch_t {
   bool fused = false;        // LDS-tiled level kernels
   long* cbase = nullptr;     // per simulation: its 64 x 64 level
   float* pinv4 = nullptr;    // pseudo-inverse of the 4 x 4 Neumann Laplacian (h = 1/4)
+  float* ex_tab = nullptr;   // exact solve of every cavity's 64 x 64 level (DCT-II basis)
   double *rpart = nullptr, *rpart_host = nullptr;
   int rp_rows = 0;
   double* mpart = nullptr;
@@ -2077,9 +2092,15 @@ int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_o
                       cnt, (float)((double)s * s), b->lr + b->base[lg - 1])))
       return rc;
   }
-  if ((rc = blaunch(b, fsk::k_nm_coarse, dim3(b->nsim), dim3(1024), b->lxb, (const float*)b->lr,
-                    (const long*)b->cbase, 64, pre, post, (const float*)b->pinv4)))
+  if (b->ex_tab) {
+    // every cavity's 64 x 64 level solved exactly, one workgroup each
+    if ((rc = blaunch(b, fsk::k_mg_coarse_exact<float>, dim3(b->nsim), dim3(fsk::EXT), b->lxb, (const float*)b->lr,
+                      64, (const long*)b->cbase, (const uint4*)b->ex_tab, fsk::gate_open())))
+      return rc;
+  } else if ((rc = blaunch(b, fsk::k_nm_coarse, dim3(b->nsim), dim3(1024), b->lxb, (const float*)b->lr,
+                           (const long*)b->cbase, 64, pre, post, (const float*)b->pinv4))) {
     return rc;
+  }
   for (int lg = 7; lg <= b->max_lg; lg++) {
     const long cnt = b->count[lg];
     const int s = 1 << lg;
@@ -2337,6 +2358,7 @@ void batch_release(fs_batch_t* b) {
   dfree(b->lxb);
   dfree(b->cbase);
   dfree(b->pinv4);
+  dfree(b->ex_tab);
   dfree(b->rpart);
   hfree(b->rpart_host);
   dfree(b->mpart);
@@ -3568,6 +3590,7 @@ void fs_batch_config_default(fs_batch_config* c) {
   c->device = 0;
   c->mg_fused = 1;
   c->graph = 1;
+  c->mg_coarse_exact = 0;
   c->autotune = 1;
   c->tile = 0;
 }
@@ -3589,6 +3612,8 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
     return fail(FS_E_INVALID_ARG, "a batch takes the zero or the previous pressure guess");
   if (c.autotune < 0 || c.autotune > 2) return fail(FS_E_INVALID_ARG, "autotune must be 0, 1 or 2");
   if (c.tile < 0 || c.tile > TILE_KINDS) return fail(FS_E_INVALID_ARG, "tile must be in 0..%d", TILE_KINDS);
+  if (c.mg_coarse_exact != 0 && c.mg_coarse_exact != 1)
+    return fail(FS_E_INVALID_ARG, "mg_coarse_exact must be 0 or 1, got %d", c.mg_coarse_exact);
   for (int k = 0; k < count; k++) {
     const fs_cavity& q = sims[k];
     if (q.n < 64 || q.n > 4096 || (q.n & (q.n - 1)))
@@ -3677,6 +3702,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (!rc) rc = dalloc(&b->lxb, (size_t)lvl_cells);
   if (!rc) rc = dalloc(&b->cbase, (size_t)count);
   if (!rc) rc = dalloc(&b->pinv4, 256);
+  if (!rc && c.mg_coarse_exact) rc = dalloc(&b->ex_tab, (size_t)fsk::EX_TAB * 4);
   b->rp_rows = c.max_vcycles + 2;
   if (!rc) rc = dalloc(&b->rpart, (size_t)b->rp_rows * b->nblocks);
   if (!rc) rc = halloc(&b->rpart_host, (size_t)b->rp_rows * b->nblocks);
@@ -3687,6 +3713,10 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   HIPCHK(hipMemcpy(b->cbase, cbase.data(), sizeof(long) * count, hipMemcpyHostToDevice));
   std::vector<float> pinv = neumann_pinv4();
   HIPCHK(hipMemcpy(b->pinv4, pinv.data(), sizeof(float) * 256, hipMemcpyHostToDevice));
+  if (b->ex_tab) {
+    const std::vector<uint8_t> t = exact_coarse_table(64, 1.0 / 64, true);  // unit box, h = 1/64
+    HIPCHK(hipMemcpy(b->ex_tab, t.data(), t.size(), hipMemcpyHostToDevice));
+  }
   b->off = off;
   b->lvl_cells = lvl_cells;
   const int forced = c.tile >= 1 ? c.tile - 1 : -1;

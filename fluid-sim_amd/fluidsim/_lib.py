@@ -117,7 +117,7 @@ class FsBatchConfig(C.Structure):
     _fields_ = [("advection", C.c_int), ("tolerance", C.c_double), ("max_vcycles", C.c_int),
                 ("mg_pre", C.c_int), ("mg_post", C.c_int), ("coupling_passes", C.c_int),
                 ("mask_width", C.c_double), ("guess", C.c_int), ("device", C.c_int), ("mg_fused", C.c_int),
-                ("graph", C.c_int), ("autotune", C.c_int), ("tile", C.c_int)]
+                ("graph", C.c_int), ("autotune", C.c_int), ("tile", C.c_int), ("mg_coarse_exact", C.c_int)]
 
 
 class FsBatchStats(C.Structure):

@@ -39,7 +39,7 @@ class CavityBatch:
 
     def __init__(self, cavities, device=0, advection="maccormack", tolerance=1e-6, maxVcycles=60, mgPre=2,
                  mgPost=2, couplingPasses=2, maskWidth=1.5, guess="zero", mgFused=True, graph=True,
-                 autotune=1, tile=None):
+                 autotune=1, tile=None, coarseSolve="sweeps"):
         if advection not in _lib.ADVECTION:
             raise ValueError(f"unknown advection scheme {advection}")
         if guess not in ("zero", "previous"):
@@ -57,6 +57,9 @@ class CavityBatch:
         cfg.graph = 1 if graph else 0
         cfg.autotune = int(autotune)
         cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4}[tile]
+        if coarseSolve not in ("sweeps", "exact"):
+            raise ValueError(f"unknown coarse solve {coarseSolve}; have sweeps, exact")
+        cfg.mg_coarse_exact = 1 if coarseSolve == "exact" else 0
         arr = (FsCavity * len(cavities))()
         for a, c in zip(arr, cavities):
             a.n, a.lid, a.nu = int(c["n"]), float(c["lid"]), float(c["nu"])

@@ -219,6 +219,10 @@ typedef struct fs_batch_config {
                                  create (the choice changes the bits) */
   int tile;                   /* 0 (default): as autotune says; 1..4: every size class uses
                                  64x32, 32x32, 16x16 or 8x8 tiles (no tile timing). Same bits. */
+  int mg_coarse_exact;        /* 0 (default): every cavity's levels 64 .. 4 in one workgroup with
+                                 a 4 x 4 pseudo-inverse; 1: its 64 x 64 level solved exactly in the
+                                 cell-centred cosine (DCT-II) basis of the zero-flux Laplacian on
+                                 the matrix cores, one workgroup per cavity */
 } fs_batch_config;
 
 typedef struct fs_batch_stats {

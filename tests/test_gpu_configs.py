@@ -130,16 +130,19 @@ def test_shipped_weights_at_1024_match_torch_and_keep_the_gate(fs, O, threads):
         assert g.stats()["gate_misses"] == 0
 
 
-def test_mixed_size_batch_members_track_the_oracle(fs, O, threads):
+@pytest.mark.parametrize("coarse", ["sweeps", "exact"])
+def test_mixed_size_batch_members_track_the_oracle(fs, O, threads, coarse):
     # one cavity of each size in one batch; every member against the oracle's walled
-    # step (exact Neumann projection) on the batch's own per-cavity dt
+    # step (exact Neumann projection) on the batch's own per-cavity dt; the cavities'
+    # 64^2 levels by the one-workgroup V-cycle or solved exactly (DCT-II basis)
     cavs = [dict(n=512, lid=0.8, nu=4e-4, disk=dict(x=0.6, y=0.4, r=0.15)),
             dict(n=64, lid=2.0, nu=5e-3, disk=dict(x=0.5, y=0.5, r=0.1)),
             dict(n=256, lid=1.2, nu=6e-4, disk=None),
             dict(n=128, lid=1.5, nu=1e-3, disk=dict(x=0.3, y=0.6, r=0.08))]
     oracles = [O.Cavity(c["n"], U=c["lid"], nu=c["nu"], disk=c["disk"], passes=2) for c in cavs]
     steps = 6
-    with fs.CavityBatch(cavs, tolerance=1e-8) as b:
+    print(f"mixed batch coarse={coarse}", flush=True)
+    with fs.CavityBatch(cavs, tolerance=1e-8, coarseSolve=coarse) as b:
         for _ in range(steps):
             b.stepCFL(0.5)
             assert b.stats()["last_residual"] <= 1e-8
@@ -155,11 +158,13 @@ def test_mixed_size_batch_members_track_the_oracle(fs, O, threads):
         assert np.all(out["u"].reshape(n, n)[:, 0] == 0) and np.all(out["v"].reshape(n, n)[0, :] == 0)
 
 
-def test_256_cavity_workload_meets_the_gate_with_walls_and_no_divergence(fs, O):
+@pytest.mark.parametrize("coarse", ["sweeps", "exact"])
+def test_256_cavity_workload_meets_the_gate_with_walls_and_no_divergence(fs, O, coarse):
     # the batched workload: 256 cavities of 64^2 .. 512^2, each with its lid, viscosity
     # and pinned disk; three steps through the FP64 gate over every cavity
     cavs = fs.random_cavities(256, seed=1)
-    with fs.CavityBatch(cavs, tolerance=1e-6) as b:
+    print(f"256 cavities coarse={coarse}", flush=True)
+    with fs.CavityBatch(cavs, tolerance=1e-6, coarseSolve=coarse) as b:
         for _ in range(3):
             b.stepCFL(0.5)
             st = b.stats()
