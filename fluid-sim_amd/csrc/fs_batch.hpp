@@ -518,6 +518,32 @@ __device__ __forceinline__ void nm_load(float* __restrict__ dst, const float* __
   }
 }
 
+// nm_load split in two: issue() starts the loads into registers, commit() stores them,
+// so a kernel's windows are all in flight at once
+template <int RW, int RH, int NT>
+struct NmWindow {
+  static constexpr int CNT = RW * RH, K = (CNT + NT - 1) / NT;
+  float f[K];
+  __device__ __forceinline__ void issue(const float* __restrict__ src, int s, int ox, int oy) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        int ly = q / RW, lx = q - ly * RW;
+        int gx = ox + lx, gy = oy + ly;
+        f[k] = (gx >= 0 && gx < s && gy >= 0 && gy < s) ? src[(size_t)gy * s + gx] : 0.f;
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(float* __restrict__ dst) const {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = threadIdx.x + k * NT;
+      if (q < CNT) dst[q] = f[k];
+    }
+  }
+};
+
 // One colour half-sweep on the LDS region [LO, RW-LO) x [LO, RH-LO); cells outside
 // the box are skipped (they stay 0).
 template <int RW, int RH, int LO, int NT>
@@ -637,14 +663,28 @@ k_nmt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
   const int cs = s >> 1;
   const size_t gb = (size_t)g * s * s, cb = (size_t)g * cs * cs;
   const int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
-  // coarse footprint, indices clamped into the box
-  for (int q = threadIdx.x; q < CW * CH; q += NT) {
-    int ly = q / CW, lx = q - ly * CW;
-    int cx = min(max(cox + lx, 0), cs - 1), cy = min(max(coy + ly, 0), cs - 1);
-    sc[q] = xc[cb + (size_t)cy * cs + cx];
+  // coarse footprint (indices clamped into the box), x and rhs: all loads in flight at once
+  constexpr int KC = (CW * CH + NT - 1) / NT;
+  float fc[KC];
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CW * CH) {
+      int ly = q / CW, lx = q - ly * CW;
+      int cx = min(max(cox + lx, 0), cs - 1), cy = min(max(coy + ly, 0), cs - 1);
+      fc[k] = xc[cb + (size_t)cy * cs + cx];
+    }
   }
-  nm_load<RW, RH, NT>(sx, xin + gb, s, ox, oy);
-  nm_load<RW, RH, NT>(sr, rhs + gb, s, ox, oy);
+  NmWindow<RW, RH, NT> wx, wr;
+  wx.issue(xin + gb, s, ox, oy);
+  wr.issue(rhs + gb, s, ox, oy);
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    int q = threadIdx.x + k * NT;
+    if (q < CW * CH) sc[q] = fc[k];
+  }
+  wx.commit(sx);
+  wr.commit(sr);
   __syncthreads();
   for (int q = threadIdx.x; q < RW * RH; q += NT) {
     int ly = q / RW, lx = q - ly * RW;
