@@ -163,7 +163,7 @@ def copy_ceiling():
         hip.hipFree(b)
 
 
-def _counter_pass(counter, argv, timeout_s=150):
+def _counter_pass(counter, argv, timeout_s=90):
     """Run `python3 bench.py <argv>` under `rocprofv3 --pmc <counter>` in a child process
     group of its own; returns (its stdout, [(dispatch id, kernel name, value)]) or None."""
     import csv

@@ -1549,7 +1549,10 @@ int capture_ahead(fs_solver_t* s) {
     const int pred = s->last_pred + d;
     if (pred < 0 || pred > s->cfg.pressure_iterations) continue;
     const BodyKey key = gated_key(s, pred);
-    if (!find_graph(s, key)) return capture_body(s, key);
+    if (!find_graph(s, key)) {
+      (void)capture_body(s, key);  // speculative: a failure leaves the capture to the step that needs it
+      return FS_OK;
+    }
   }
   return FS_OK;
 }
