@@ -84,7 +84,7 @@ struct Level {
   float* rhs = nullptr;  // right-hand side (level 0: r)
   float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e), FP32 under every storage
   int tile[2] = {0, 0};  // fused path: tile shape (TileKind) of the down and up kernels
-  float tune_us[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
+  float tune_us[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
 };
 
 // Kernel classes timed by fs_profile_*: each profiled launch goes through
@@ -724,9 +724,11 @@ int mg_vcycle(fs_solver_t* s) {
 // yields a tile per CU; the autotuner replaces it with the fastest measured one.
 // Every shape computes the same bits (each cell sees the same operands in the same
 // order), so the choice affects time only.
-enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS };
-static const int kTileW[TILE_KINDS] = {64, 32, 16, 8}, kTileH[TILE_KINDS] = {32, 32, 16, 8};
-static const char* kTileName[TILE_KINDS] = {"64x32", "32x32", "16x16", "8x8"};
+// The cavity batch tunes over the first TILE_KINDS; the single grid also over 64 x 64
+// tiles of 1024 threads.
+enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS, TILE64SQ = TILE_KINDS, GRID_TILE_KINDS };
+static const int kTileW[GRID_TILE_KINDS] = {64, 32, 16, 8, 64}, kTileH[GRID_TILE_KINDS] = {32, 32, 16, 8, 64};
+static const char* kTileName[GRID_TILE_KINDS] = {"64x32", "32x32", "16x16", "8x8", "64x64"};
 
 inline int tile_count(int k, int n) { return ((n + kTileW[k] - 1) / kTileW[k]) * ((n + kTileH[k] - 1) / kTileH[k]); }
 
@@ -745,10 +747,11 @@ inline dim3 tiles_grid_t(int n, int* tiles_x, int* nblocks) {
 }
 
 #define DISPATCH_TILE(k, fn, SW, ...)                                                        \
-  ((k) == TILE64   ? fn<SW, fsk::Tile64>(__VA_ARGS__)                                         \
-   : (k) == TILE32 ? fn<SW, fsk::Tile32>(__VA_ARGS__)                                         \
-   : (k) == TILE16 ? fn<SW, fsk::Tile16>(__VA_ARGS__)                                         \
-                   : fn<SW, fsk::Tile8>(__VA_ARGS__))
+  ((k) == TILE64     ? fn<SW, fsk::Tile64>(__VA_ARGS__)                                       \
+   : (k) == TILE32   ? fn<SW, fsk::Tile32>(__VA_ARGS__)                                       \
+   : (k) == TILE16   ? fn<SW, fsk::Tile16>(__VA_ARGS__)                                       \
+   : (k) == TILE64SQ ? fn<SW, fsk::Tile64sq>(__VA_ARGS__)                                     \
+                     : fn<SW, fsk::Tile8>(__VA_ARGS__))
 
 template <typename S>
 inline S* as(float* p) {
@@ -852,9 +855,13 @@ int launch_up2_s(fs_solver_t* s, int l) {
 }
 // 8 x 8 tiles for the pair above the coarse solve (levels 256 and 128 over a 64
 // coarse level: more workgroups in flight; 16 x 16 took 0.4 us more per launch),
-// 32 x 32 for levels 1 and 0
+// 32 x 32 or (as tuned, on a grid of whole 64 x 64 tiles) 64 x 64 for levels 1 and 0
+bool up2_64sq_ok(const fs_solver_t* s) { return s->lv[0].n % 64 == 0; }
 template <int POST>
 int launch_up2(fs_solver_t* s, int l) {
+  if (l == 0 && s->lv[0].tile[1] == TILE64SQ && up2_64sq_ok(s))
+    return s->bf16 ? launch_up2_s<POST, fsk::Tile64sq, fsk::bf16>(s, l)
+                   : launch_up2_s<POST, fsk::Tile64sq, float>(s, l);
   if (l == 0)
     return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
   return s->bf16 ? launch_up2_s<POST, fsk::Tile8x256, fsk::bf16>(s, l)
@@ -1785,6 +1792,15 @@ int build_levels(fs_solver_t* s) {
 // per-tile partials go to a scratch row sized for the finest shape.
 constexpr int kTuneRounds = 3, kTuneLaunches = 10;
 
+// whether the V-cycle takes levels 1 and 0 up in one launch (enqueue_iteration's rule)
+bool up2_level0(const fs_solver_t* s) {
+  const int cs = s->coarse_start;
+  if (!s->fused || cs < 2 || s->lv[0].n % 32) return false;
+  const int pair = cs - 2 >= 1 && s->lv[cs - 1].n % 8 == 0 ? cs - 2 : cs;
+  const int l = pair < cs && s->lv[pair].n % 16 == 0 ? pair - 1 : cs - 1;
+  return l >= 1;
+}
+
 int autotune_tiles(fs_solver_t* s) {
   const int cs = s->coarse_start;
   if (!s->fused || cs == 0) return FS_OK;
@@ -1796,20 +1812,25 @@ int autotune_tiles(fs_solver_t* s) {
   hipEvent_t a = nullptr, b = nullptr;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventCreate");
   const int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
+  const bool up2_0 = up2_level0(s);
   for (int l = 0; l < cs && !rc; l++) {
     Level& L = s->lv[l];
     for (int dir = 0; dir < 2 && !rc; dir++) {
+      // level 0 goes up together with level 1 (k_mgt_up2) in 32 x 32 or 64 x 64 tiles
+      const bool pair0 = dir == 1 && l == 0 && up2_0;
       auto run = [&]() -> int {
+        if (pair0) return DISPATCH_SW(post, launch_up2, s, 0);
         if (dir == 1) return DISPATCH_SW(post, launch_up, s, l);
         if (l == 0) return DISPATCH_SW(pre, launch_down0, s, 0, 0, true, false);
         return DISPATCH_SW(pre, launch_down, s, l);
       };
       // three rounds over the shapes (order rotated each round), ten launches each
       // after an untimed one; a shape's time is its best round
-      for (int k = 0; k < TILE_KINDS; k++) L.tune_us[dir][k] = 0;
+      for (int k = 0; k < GRID_TILE_KINDS; k++) L.tune_us[dir][k] = 0;
       for (int round = 0; round < kTuneRounds && !rc; round++)
-        for (int j = 0; j < TILE_KINDS && !rc; j++) {
-          const int k = (j + round) % TILE_KINDS;
+        for (int j = 0; j < GRID_TILE_KINDS && !rc; j++) {
+          const int k = (j + round) % GRID_TILE_KINDS;
+          if (pair0 && k != TILE32 && !(k == TILE64SQ && up2_64sq_ok(s))) continue;
           L.tile[dir] = k;
           if ((rc = run())) break;
           if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
@@ -1822,7 +1843,7 @@ int autotune_tiles(fs_solver_t* s) {
           if (!rc && (L.tune_us[dir][k] == 0 || us < L.tune_us[dir][k])) L.tune_us[dir][k] = us;
         }
       int best = TILE32;
-      for (int k = 0; k < TILE_KINDS; k++)
+      for (int k = 0; k < GRID_TILE_KINDS; k++)
         if (L.tune_us[dir][k] > 0 && L.tune_us[dir][k] < L.tune_us[dir][best]) best = k;
       L.tile[dir] = best;
     }
@@ -2697,6 +2718,7 @@ void fs_config_default(fs_config* c) {
   c->graph = 0;
   c->dye_fade = 0.0;
   c->mg_coarse_exact = 0;
+  c->tile = 0;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -2722,6 +2744,8 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (c.mg_pre < 0 || c.mg_post < 0 || c.mg_coarse < 0) return fail(FS_E_INVALID_ARG, "negative sweep count");
   if (c.mg_coarse_exact != 0 && c.mg_coarse_exact != 1)
     return fail(FS_E_INVALID_ARG, "mg_coarse_exact must be 0 or 1, got %d", c.mg_coarse_exact);
+  if (c.tile < 0 || c.tile > GRID_TILE_KINDS)
+    return fail(FS_E_INVALID_ARG, "tile must be in 0..%d, got %d", GRID_TILE_KINDS, c.tile);
   if (c.mg_precision == FS_MG_FP16)
     return fail(FS_E_UNSUPPORTED,
                 "FP16 multigrid storage is not built: the correction scales as h^2 |r| and underflows FP16 on "
@@ -2807,7 +2831,13 @@ int fs_solver_create(const fs_config* cfg, void** out) {
     if (!rc && l.xb) rc = fill_f32(s, l.xb, (size_t)l.n * l.n, 0.f);
   }
   // on zero inputs every fused kernel writes zeros, so tuning leaves the state as it is
-  if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) rc = autotune_tiles(s);
+  if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.tile > 0) {
+    // one shape for every grid-wide level (the levels-1-and-0 launch takes 32 x 32 unless
+    // the shape is 64 x 64 and the grid is whole 64 x 64 tiles)
+    for (int l = 0; l < s->coarse_start; l++) s->lv[l].tile[0] = s->lv[l].tile[1] = c.tile - 1;
+  } else if (!rc && c.pressure_solver == FS_P_MULTIGRID && c.autotune) {
+    rc = autotune_tiles(s);
+  }
   if (!rc && c.pressure_solver == FS_P_MULTIGRID) rc = alloc_rpart(s, std::max(c.pressure_iterations, 1));
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "initial fill failed");
   if (rc) return bail(rc);
@@ -3489,10 +3519,11 @@ int fs_tile_report(void* h, char* buf, size_t len) {
   for (int l = 0; s->fused && l < s->coarse_start; l++) {
     const Level& L = s->lv[l];
     for (int dir = 0; dir < 2; dir++) {
-      snprintf(line, sizeof line, "level %d n=%d %s: %s", l, L.n, dir == 0 ? (l == 0 ? "down0" : "down") : "up",
+      snprintf(line, sizeof line, "level %d n=%d %s: %s", l, L.n,
+               dir == 0 ? (l == 0 ? "down0" : "down") : (l == 0 && up2_level0(s) ? "up (with level 1)" : "up"),
                kTileName[L.tile[dir]]);
       out += line;
-      for (int k = 0; k < TILE_KINDS; k++)
+      for (int k = 0; k < GRID_TILE_KINDS; k++)
         if (L.tune_us[dir][k] > 0) {
           snprintf(line, sizeof line, " | %s %.2f us", kTileName[k], L.tune_us[dir][k]);
           out += line;

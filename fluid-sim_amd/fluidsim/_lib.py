@@ -55,6 +55,7 @@ class FsConfig(C.Structure):
         ("graph", C.c_int),
         ("dye_fade", C.c_double),
         ("mg_coarse_exact", C.c_int),
+        ("tile", C.c_int),
     ]
 
 

@@ -59,7 +59,7 @@ class GPUFluidSolver:
                  pressureSolver="jacobi", pressureIterations=40, hasTimestamps=False,
                  tolerance=0.0, mgMinSize=8, mgPre=2, mgPost=2, mgCoarse=40,
                  mgPrecision="fp32", guess="zero", mgFused=True, autotune=True,
-                 advFused=True, gated=True, graph=False, mgCoarseSolve="sweeps"):
+                 advFused=True, gated=True, graph=False, mgCoarseSolve="sweeps", tile=None):
         if isinstance(device, dict):
             device = device.get("device", 0)
         nx = n if nx is None else nx
@@ -96,6 +96,7 @@ class GPUFluidSolver:
         cfg.graph = 1 if graph else 0
         cfg.dye_fade = float(dyeFade)
         cfg.mg_coarse_exact = 1 if mgCoarseSolve == "exact" else 0
+        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4, "64x64": 5}[tile]
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))

@@ -126,6 +126,9 @@ typedef struct fs_config {
                                  (real Fourier basis, BF16 head + tail products on the matrix
                                  cores, ~1e-5 relative) - not in the reference; changes the
                                  bits, not the tolerance the gate enforces */
+  int tile;                   /* 0 (default): the V-cycle tile shapes as the autotuner picks them;
+                                 1..5: every grid-wide level uses 64x32, 32x32, 16x16, 8x8 or
+                                 64x64 tiles (no tuning). Same bits. */
 } fs_config;
 
 /* one splat of step(dt, {splats}) (src/gpu/solver.js:546-554, _writeSplat :745-757): a

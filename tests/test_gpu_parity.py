@@ -544,6 +544,26 @@ def test_autotuned_tiles_give_the_same_bits(fs, O, n):
         assert np.array_equal(outs[0]["u"], o["u"]) and np.array_equal(outs[0]["v"], o["v"])
 
 
+@pytest.mark.parametrize("n,prec", [(256, "fp32"), (1024, "fp32"), (512, "bf16"), (192, "fp32")])
+def test_every_forced_tile_shape_gives_the_same_bits(fs, O, n, prec):
+    # each shape forced on every grid-wide level (64 x 64 also for levels 1 and 0 going up
+    # together when the grid is whole 64 x 64 tiles; 192 is not, and takes 32 x 32 there)
+    u0, v0 = O.js_random_velocity(n, 13)
+    outs = []
+    for tile in ("32x32", "64x32", "16x16", "8x8", "64x64"):
+        print(f"forced tile {tile} n={n} {prec}", flush=True)
+        with solver(fs, n=n, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
+                    mgMinSize=4, mgPrecision=prec, tile=tile, mgCoarseSolve="exact") as g:
+            g.uploadVelocity(u0, v0)
+            for _ in range(2):
+                g.step(2e-3)
+            outs.append((g.readVelocity(), g.readField("pressure")))
+            assert f": {tile}" in g.tileReport()
+    for o, p in outs[1:]:
+        assert np.array_equal(outs[0][0]["u"], o["u"]) and np.array_equal(outs[0][0]["v"], o["v"])
+        assert np.array_equal(outs[0][1], p)
+
+
 # ------------------------------------------------------ BF16 multigrid storage
 
 @pytest.mark.parametrize("n", [128, 256, 512])  # 512: the two-level down/up launches run
