@@ -38,6 +38,10 @@ using Tile16 = TileShape<16, 16, 128>;
 using Tile8 = TileShape<8, 8, 64>;
 using Tile64sq = TileShape<64, 64, 1024>;  // one tile per CU at 1024^2: a quarter of the
                                             // halo cells of 32 x 32 tiles recomputed
+// the same tiles with more threads (fewer cells each): shorter passes on the small,
+// latency-bound levels
+using Tile32w = TileShape<32, 32, 1024>;
+using Tile16w = TileShape<16, 16, 256>;
 using Tile8x256 = TileShape<8, 8, 256>;  // k_mgt_down2 and k_mgt_up2 above the coarse solve: the
                                          // level-l region is ~25 times the tile
 

@@ -84,7 +84,7 @@ struct Level {
   float* rhs = nullptr;  // right-hand side (level 0: r)
   float* xb = nullptr;   // fused path: correction after post-smoothing (level 0: e), FP32 under every storage
   int tile[2] = {0, 0};  // fused path: tile shape (TileKind) of the down and up kernels
-  float tune_us[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};  // autotuner timings per shape (0: not timed)
+  float tune_us[2][7] = {};  // autotuner timings per shape (0: not timed)
 };
 
 // Kernel classes timed by fs_profile_*: each profiled launch goes through
@@ -725,10 +725,13 @@ int mg_vcycle(fs_solver_t* s) {
 // Every shape computes the same bits (each cell sees the same operands in the same
 // order), so the choice affects time only.
 // The cavity batch tunes over the first TILE_KINDS; the single grid also over 64 x 64
-// tiles of 1024 threads.
-enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS, TILE64SQ = TILE_KINDS, GRID_TILE_KINDS };
-static const int kTileW[GRID_TILE_KINDS] = {64, 32, 16, 8, 64}, kTileH[GRID_TILE_KINDS] = {32, 32, 16, 8, 64};
-static const char* kTileName[GRID_TILE_KINDS] = {"64x32", "32x32", "16x16", "8x8", "64x64"};
+// tiles of 1024 threads and 32 x 32 / 16 x 16 tiles with four times / twice the threads.
+enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS, TILE64SQ = TILE_KINDS, TILE32W, TILE16W,
+                GRID_TILE_KINDS };
+static const int kTileW[GRID_TILE_KINDS] = {64, 32, 16, 8, 64, 32, 16};
+static const int kTileH[GRID_TILE_KINDS] = {32, 32, 16, 8, 64, 32, 16};
+static const char* kTileName[GRID_TILE_KINDS] = {"64x32", "32x32", "16x16", "8x8", "64x64", "32x32/1024",
+                                                 "16x16/256"};
 
 inline int tile_count(int k, int n) { return ((n + kTileW[k] - 1) / kTileW[k]) * ((n + kTileH[k] - 1) / kTileH[k]); }
 
@@ -751,6 +754,8 @@ inline dim3 tiles_grid_t(int n, int* tiles_x, int* nblocks) {
    : (k) == TILE32   ? fn<SW, fsk::Tile32>(__VA_ARGS__)                                       \
    : (k) == TILE16   ? fn<SW, fsk::Tile16>(__VA_ARGS__)                                       \
    : (k) == TILE64SQ ? fn<SW, fsk::Tile64sq>(__VA_ARGS__)                                     \
+   : (k) == TILE32W  ? fn<SW, fsk::Tile32w>(__VA_ARGS__)                                      \
+   : (k) == TILE16W  ? fn<SW, fsk::Tile16w>(__VA_ARGS__)                                      \
                      : fn<SW, fsk::Tile8>(__VA_ARGS__))
 
 template <typename S>

@@ -96,7 +96,8 @@ class GPUFluidSolver:
         cfg.graph = 1 if graph else 0
         cfg.dye_fade = float(dyeFade)
         cfg.mg_coarse_exact = 1 if mgCoarseSolve == "exact" else 0
-        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4, "64x64": 5}[tile]
+        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4, "64x64": 5,
+                                           "32x32/1024": 6, "16x16/256": 7}[tile]
         self._L = load()
         h = C.c_void_p()
         check(self._L.fs_solver_create(C.byref(cfg), C.byref(h)))

@@ -127,8 +127,9 @@ typedef struct fs_config {
                                  cores, ~1e-5 relative) - not in the reference; changes the
                                  bits, not the tolerance the gate enforces */
   int tile;                   /* 0 (default): the V-cycle tile shapes as the autotuner picks them;
-                                 1..5: every grid-wide level uses 64x32, 32x32, 16x16, 8x8 or
-                                 64x64 tiles (no tuning). Same bits. */
+                                 1..7: every grid-wide level uses 64x32, 32x32, 16x16, 8x8, 64x64
+                                 tiles, or 32x32 with 1024 / 16x16 with 256 threads (no tuning).
+                                 Same bits. */
 } fs_config;
 
 /* one splat of step(dt, {splats}) (src/gpu/solver.js:546-554, _writeSplat :745-757): a

@@ -550,7 +550,7 @@ def test_every_forced_tile_shape_gives_the_same_bits(fs, O, n, prec):
     # together when the grid is whole 64 x 64 tiles; 192 is not, and takes 32 x 32 there)
     u0, v0 = O.js_random_velocity(n, 13)
     outs = []
-    for tile in ("32x32", "64x32", "16x16", "8x8", "64x64"):
+    for tile in ("32x32", "64x32", "16x16", "8x8", "64x64", "32x32/1024", "16x16/256"):
         print(f"forced tile {tile} n={n} {prec}", flush=True)
         with solver(fs, n=n, advection="maccormack", pressureSolver="multigrid", pressureIterations=3,
                     mgMinSize=4, mgPrecision=prec, tile=tile, mgCoarseSolve="exact") as g:
