@@ -745,8 +745,9 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const float4 w = sr[m][ty + 4 * rr];
-      float cu = c.x * w.x - c.y * w.y;
-      float cv = c.z * w.z - c.w * w.w;
+      // cos(a + b) = cos a cos b - sin a sin b, one product fused
+      float cu = fmaf(c.x, w.x, -(c.y * w.y));
+      float cv = fmaf(c.z, w.z, -(c.w * w.w));
       fu[rr] = fmaf(md.cu, cu, fu[rr]);
       fv[rr] = fmaf(md.cv, cv, fv[rr]);
     }
