@@ -724,8 +724,8 @@ int mg_vcycle(fs_solver_t* s) {
 // yields a tile per CU; the autotuner replaces it with the fastest measured one.
 // Every shape computes the same bits (each cell sees the same operands in the same
 // order), so the choice affects time only.
-// The cavity batch tunes over the first TILE_KINDS; the single grid also over 64 x 64
-// tiles of 1024 threads and 32 x 32 / 16 x 16 tiles with four times / twice the threads.
+// Shapes: 64 x 32, 32 x 32, 16 x 16, 8 x 8 (the first TILE_KINDS), 64 x 64 tiles of 1024
+// threads, and 32 x 32 / 16 x 16 tiles with four times / twice the threads.
 enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS, TILE64SQ = TILE_KINDS, TILE32W, TILE16W,
                 GRID_TILE_KINDS };
 static const int kTileW[GRID_TILE_KINDS] = {64, 32, 16, 8, 64, 32, 16};
@@ -2028,7 +2028,7 @@ struct fs_batch_t {
   // level-kernel tile shape (TileKind) per size class 2^lg and direction (0 down, 1 up),
   // the autotuner's timings per shape (us, 0: not timed) and its sweep trials
   int tile[13][2] = {};
-  float tune_us[13][2][TILE_KINDS] = {};
+  float tune_us[13][2][GRID_TILE_KINDS] = {};
   struct SweepTrial {
     int pre, post;
     double ms_per_step;
@@ -2072,17 +2072,23 @@ int nmt_up_t(fs_batch_t* b, int sw, const float* x, float* xb, const float* r, c
   return blaunch(b, fsk::k_nmt_up<3, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
 }
 int nmt_down(fs_batch_t* b, int kind, int sw, float* x, const float* r, int s, long cnt, float* cr) {
-  return kind == TILE64 ? nmt_down_t<fsk::Tile64>(b, sw, x, r, s, cnt, cr)
-       : kind == TILE32 ? nmt_down_t<fsk::Tile32>(b, sw, x, r, s, cnt, cr)
-       : kind == TILE16 ? nmt_down_t<fsk::Tile16>(b, sw, x, r, s, cnt, cr)
-                        : nmt_down_t<fsk::Tile8>(b, sw, x, r, s, cnt, cr);
+  return kind == TILE64     ? nmt_down_t<fsk::Tile64>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE32   ? nmt_down_t<fsk::Tile32>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE16   ? nmt_down_t<fsk::Tile16>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE64SQ ? nmt_down_t<fsk::Tile64sq>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE32W  ? nmt_down_t<fsk::Tile32w>(b, sw, x, r, s, cnt, cr)
+       : kind == TILE16W  ? nmt_down_t<fsk::Tile16w>(b, sw, x, r, s, cnt, cr)
+                          : nmt_down_t<fsk::Tile8>(b, sw, x, r, s, cnt, cr);
 }
 int nmt_up(fs_batch_t* b, int kind, int sw, const float* x, float* xb, const float* r, const float* xc, int s,
            long cnt) {
-  return kind == TILE64 ? nmt_up_t<fsk::Tile64>(b, sw, x, xb, r, xc, s, cnt)
-       : kind == TILE32 ? nmt_up_t<fsk::Tile32>(b, sw, x, xb, r, xc, s, cnt)
-       : kind == TILE16 ? nmt_up_t<fsk::Tile16>(b, sw, x, xb, r, xc, s, cnt)
-                        : nmt_up_t<fsk::Tile8>(b, sw, x, xb, r, xc, s, cnt);
+  return kind == TILE64     ? nmt_up_t<fsk::Tile64>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE32   ? nmt_up_t<fsk::Tile32>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE16   ? nmt_up_t<fsk::Tile16>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE64SQ ? nmt_up_t<fsk::Tile64sq>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE32W  ? nmt_up_t<fsk::Tile32w>(b, sw, x, xb, r, xc, s, cnt)
+       : kind == TILE16W  ? nmt_up_t<fsk::Tile16w>(b, sw, x, xb, r, xc, s, cnt)
+                          : nmt_up_t<fsk::Tile8>(b, sw, x, xb, r, xc, s, cnt);
 }
 
 // refinement iteration `it`: residual of p64[pin] (+ the finest corrections),
@@ -2309,8 +2315,8 @@ int batch_autotune_tiles(fs_batch_t* b) {
                         : nmt_up(b, kind, b->cfg.mg_post, x, xb, r, xc, s, cnt);
       };
       for (int round = 0; round < kTuneRounds && !rc; round++)
-        for (int j = 0; j < TILE_KINDS && !rc; j++) {
-          const int k = (j + round) % TILE_KINDS;
+        for (int j = 0; j < GRID_TILE_KINDS && !rc; j++) {
+          const int k = (j + round) % GRID_TILE_KINDS;
           if ((rc = run(k))) break;
           if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord failed");
           for (int q = 0; q < kTuneLaunches && !rc; q++) rc = run(k);
@@ -2322,7 +2328,7 @@ int batch_autotune_tiles(fs_batch_t* b) {
           const float us = ms * 1000.f / kTuneLaunches;
           if (b->tune_us[lg][dir][k] == 0 || us < b->tune_us[lg][dir][k]) b->tune_us[lg][dir][k] = us;
         }
-      for (int k = 0; k < TILE_KINDS; k++)
+      for (int k = 0; k < GRID_TILE_KINDS; k++)
         if (b->tune_us[lg][dir][k] > 0 && b->tune_us[lg][dir][k] < b->tune_us[lg][dir][b->tile[lg][dir]])
           b->tile[lg][dir] = k;
     }
@@ -3650,7 +3656,7 @@ int fs_batch_create(const fs_batch_config* cfg, const fs_cavity* sims, int count
   if (c.guess != FS_GUESS_ZERO && c.guess != FS_GUESS_PREVIOUS)
     return fail(FS_E_INVALID_ARG, "a batch takes the zero or the previous pressure guess");
   if (c.autotune < 0 || c.autotune > 2) return fail(FS_E_INVALID_ARG, "autotune must be 0, 1 or 2");
-  if (c.tile < 0 || c.tile > TILE_KINDS) return fail(FS_E_INVALID_ARG, "tile must be in 0..%d", TILE_KINDS);
+  if (c.tile < 0 || c.tile > GRID_TILE_KINDS) return fail(FS_E_INVALID_ARG, "tile must be in 0..%d", GRID_TILE_KINDS);
   if (c.mg_coarse_exact != 0 && c.mg_coarse_exact != 1)
     return fail(FS_E_INVALID_ARG, "mg_coarse_exact must be 0 or 1, got %d", c.mg_coarse_exact);
   for (int k = 0; k < count; k++) {
@@ -3854,7 +3860,7 @@ int fs_batch_tune_report(void* h, char* buf, size_t len) {
       r += line;
       if (b->tune_us[lg][dir][0] > 0) {
         r += " [";
-        for (int k = 0; k < TILE_KINDS; k++) {
+        for (int k = 0; k < GRID_TILE_KINDS; k++) {
           snprintf(line, sizeof line, "%s%s %.1f us", k ? ", " : "", kTileName[k], b->tune_us[lg][dir][k]);
           r += line;
         }

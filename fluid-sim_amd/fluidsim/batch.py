@@ -56,7 +56,8 @@ class CavityBatch:
         cfg.mg_fused = 1 if mgFused else 0
         cfg.graph = 1 if graph else 0
         cfg.autotune = int(autotune)
-        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4}[tile]
+        cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4, "64x64": 5,
+                                           "32x32/1024": 6, "16x16/256": 7}[tile]
         if coarseSolve not in ("sweeps", "exact"):
             raise ValueError(f"unknown coarse solve {coarseSolve}; have sweeps, exact")
         cfg.mg_coarse_exact = 1 if coarseSolve == "exact" else 0

@@ -221,8 +221,9 @@ typedef struct fs_batch_config {
                                  each size class's level kernels timed at create (same bits);
                                  2: also (pre, post) in {1,2,3}^2 timed over steps from rest at
                                  create (the choice changes the bits) */
-  int tile;                   /* 0 (default): as autotune says; 1..4: every size class uses
-                                 64x32, 32x32, 16x16 or 8x8 tiles (no tile timing). Same bits. */
+  int tile;                   /* 0 (default): as autotune says; 1..7: every size class uses
+                                 64x32, 32x32, 16x16, 8x8, 64x64 tiles, or 32x32 with 1024 /
+                                 16x16 with 256 threads (no tile timing). Same bits. */
   int mg_coarse_exact;        /* 0 (default): every cavity's levels 64 .. 4 in one workgroup with
                                  a 4 x 4 pseudo-inverse; 1: its 64 x 64 level solved exactly in the
                                  cell-centred cosine (DCT-II) basis of the zero-flux Laplacian on

@@ -118,7 +118,7 @@ def test_graph_replay_gives_the_same_bits(fs, guess):
     assert 1 <= sg["graph_captures"] < steps          # heads are reused across steps
 
 
-@pytest.mark.parametrize("tile", ["64x32", "32x32", "16x16", "8x8"])
+@pytest.mark.parametrize("tile", ["64x32", "32x32", "16x16", "8x8", "64x64", "32x32/1024", "16x16/256"])
 def test_every_cavity_tile_shape_gives_the_same_bits(fs, tile):
     # what makes the tile autotuner safe: each shape against the one-pass kernels
     cavs = [dict(n=512, lid=1.0, nu=1e-3, disk=dict(x=0.4, y=0.5, r=0.1)),
