@@ -330,10 +330,19 @@ constexpr int ANT = 512;  // threads per tile
 constexpr int AUW = AT + 2 * AU, AFW = AT + 2 * AF, AMW = AT + 2;
 
 struct AdvWin {
-  const float* w;  // LDS window of side ws, origin (ox, oy) in unwrapped cell indices
-  int ox, oy, ws;
+  const float* w;  // LDS window of side AUW, origin (ox, oy) in unwrapped cell indices
+  int ox, oy;
+  int olin;        // oy * AUW + ox: the window position of cell (i, j) is j * AUW + i - olin
   const float* g;  // the field in global memory
 };
+
+// Window position of the node (fi, fj) given as floats holding integers (floors of
+// sample coordinates, |fi|, |fj| < 2^23): fj * W + fi is exact in FP32, so one FMA
+// and one conversion replace two conversions, two subtractions and a multiply.
+template <int W>
+__device__ __forceinline__ int lin_index(float fi, float fj, int olin) {
+  return (int)fmaf(fj, (float)W, fi) - olin;
+}
 
 // The four nodes (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1) at wrapped indices from
 // global memory: the rare path, kept out of line so the common one stays small.
@@ -360,11 +369,22 @@ __device__ __forceinline__ int win_index(int fi, int origin, int n) {
 template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ float4 nodes4_w(const AdvWin& a, int nx, int ny, int fi, int fj) {
   int li = WRAP ? win_index(fi, a.ox, nx) : fi - a.ox, lj = WRAP ? win_index(fj, a.oy, ny) : fj - a.oy;
-  if (FAST || ((unsigned)li < (unsigned)(a.ws - 1) && (unsigned)lj < (unsigned)(a.ws - 1))) {
-    const float* r0 = a.w + lj * a.ws + li;
-    return float4{r0[0], r0[1], r0[a.ws], r0[a.ws + 1]};
+  if (FAST || ((unsigned)li < (unsigned)(AUW - 1) && (unsigned)lj < (unsigned)(AUW - 1))) {
+    const float* r0 = a.w + lj * AUW + li;
+    return float4{r0[0], r0[1], r0[AUW], r0[AUW + 1]};
   }
   return nodes4_global(a.g, nx, ny, fi, fj);
+}
+
+// nodes4_w at floor values fi, fj (floats): on the fast inner path the window
+// position comes from lin_index
+template <bool FAST, bool WRAP>
+__device__ __forceinline__ float4 nodes4_wf(const AdvWin& a, int nx, int ny, float fi, float fj) {
+  if (FAST && !WRAP) {
+    const float* r0 = a.w + lin_index<AUW>(fi, fj, a.olin);
+    return float4{r0[0], r0[1], r0[AUW], r0[AUW + 1]};
+  }
+  return nodes4_w<FAST, WRAP>(a, nx, ny, (int)fi, (int)fj);
 }
 
 // bilerp() with the four nodes from the window when they are all inside it
@@ -372,7 +392,7 @@ template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ float bilerp_w(const AdvWin& a, int nx, int ny, float gx, float gy) {
   float fi = floorf(gx), fj = floorf(gy);
   float fx = gx - fi, fy = gy - fj;
-  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)fi, (int)fj);
+  float4 q = nodes4_wf<FAST, WRAP>(a, nx, ny, fi, fj);
   float top = fmaf(q.y - q.x, fx, q.x);
   float bot = fmaf(q.w - q.z, fx, q.z);
   return fmaf(bot - top, fy, top);
@@ -391,7 +411,7 @@ __device__ __forceinline__ float mid_w(const AdvWin& a, int nx, int ny, int i0, 
 
 template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ void bounds4_w(const AdvWin& a, int nx, int ny, float gx, float gy, float& lo, float& hi) {
-  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)floorf(gx), (int)floorf(gy));
+  float4 q = nodes4_wf<FAST, WRAP>(a, nx, ny, floorf(gx), floorf(gy));
   lo = fminf(fminf(q.x, q.y), fminf(q.z, q.w));
   hi = fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
 }
@@ -401,7 +421,7 @@ __device__ __forceinline__ void bounds4_w(const AdvWin& a, int nx, int ny, float
 template <bool FAST, bool WRAP = true>
 __device__ __forceinline__ float node_w(const AdvWin& a, int nx, int ny, int i, int j) {
   int li = WRAP ? win_index(i, a.ox, nx) : i - a.ox, lj = WRAP ? win_index(j, a.oy, ny) : j - a.oy;
-  if (FAST || ((unsigned)li < (unsigned)a.ws && (unsigned)lj < (unsigned)a.ws)) return a.w[lj * a.ws + li];
+  if (FAST || ((unsigned)li < (unsigned)AUW && (unsigned)lj < (unsigned)AUW)) return a.w[lj * AUW + li];
   return a.g[(size_t)wrapi(j, ny) * nx + wrapi(i, nx)];
 }
 
@@ -469,19 +489,12 @@ __device__ __forceinline__ float fwd_node_bounds(const AdvWin& u, const AdvWin& 
   const AdvWin& a = COMP == 0 ? u : v;
   float fi = floorf(gx), fj = floorf(gy);
   float fx = gx - fi, fy = gy - fj;
-  float4 q = nodes4_w<FAST, WRAP>(a, nx, ny, (int)fi, (int)fj);
+  float4 q = nodes4_wf<FAST, WRAP>(a, nx, ny, fi, fj);
   lo = fminf(fminf(q.x, q.y), fminf(q.z, q.w));
   hi = fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w));
   float top = fmaf(q.y - q.x, fx, q.x);
   float bot = fmaf(q.w - q.z, fx, q.z);
   return fmaf(bot - top, fy, top);
-}
-
-// forward value at one node, out of line (the rare path)
-template <int COMP>
-__device__ __noinline__ float fwd_node_far(const AdvWin u, const AdvWin v, int nx, int ny, int i, int j, float cx,
-                                           float cy) {
-  return fwd_node<COMP, false>(u, v, nx, ny, i, j, cx, cy);
 }
 
 // the four forward nodes around (i0, j0) at wrapped indices, as the global forward
@@ -503,14 +516,19 @@ __device__ __forceinline__ float back_w(const float* __restrict__ sf, int fox, i
                                         const AdvWin& v, int nx, int ny, float gx, float gy, float cx, float cy) {
   float fi = floorf(gx), fj = floorf(gy);
   float fx = gx - fi, fy = gy - fj;
-  int i0 = (int)fi, j0 = (int)fj;
-  int li = WRAP ? win_index(i0, fox, nx) : i0 - fox, lj = WRAP ? win_index(j0, foy, ny) : j0 - foy;
   float4 q;
-  if (FAST || ((unsigned)li < (unsigned)(AFW - 1) && (unsigned)lj < (unsigned)(AFW - 1))) {
-    const float* r0 = sf + lj * AFW + li;
+  if (FAST && !WRAP) {
+    const float* r0 = sf + lin_index<AFW>(fi, fj, foy * AFW + fox);
     q = float4{r0[0], r0[1], r0[AFW], r0[AFW + 1]};
   } else {
-    q = fwd_nodes4_far<COMP>(u, v, nx, ny, i0, j0, cx, cy);
+    int i0 = (int)fi, j0 = (int)fj;
+    int li = WRAP ? win_index(i0, fox, nx) : i0 - fox, lj = WRAP ? win_index(j0, foy, ny) : j0 - foy;
+    if (FAST || ((unsigned)li < (unsigned)(AFW - 1) && (unsigned)lj < (unsigned)(AFW - 1))) {
+      const float* r0 = sf + lj * AFW + li;
+      q = float4{r0[0], r0[1], r0[AFW], r0[AFW + 1]};
+    } else {
+      q = fwd_nodes4_far<COMP>(u, v, nx, ny, i0, j0, cx, cy);
+    }
   }
   float top = fmaf(q.y - q.x, fx, q.x);
   float bot = fmaf(q.w - q.z, fx, q.z);
@@ -519,10 +537,15 @@ __device__ __forceinline__ float back_w(const float* __restrict__ sf, int fox, i
 
 // Forward pass and corrected, limited pass of one tile (the diffusion follows in the
 // kernel). FAST: every sample is known to lie in its window.
+// LDS: the forward values on the AF ring (sfu, sfv) and, on the tile and its 1-cell
+// ring, one float4 per corrected node (sbd): first the limiter bounds of the forward
+// trace from that node (lo u, hi u, lo v, hi v), which only that node's correction
+// reads, then the corrected pair (u, v) written over them by the same thread. The
+// corrected node (lx, ly) of the 1-cell ring is the forward node (lx + 1, ly + 1):
+// both windows hold the same wrapped nodes in the same order.
 template <bool FAST, bool WRAP>
 __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, float* __restrict__ sfu,
                                             float* __restrict__ sfv, float4* __restrict__ sbd,
-                                            float* __restrict__ smu, float* __restrict__ smv,
                                             const float* __restrict__ u, const float* __restrict__ v, int nx, int ny,
                                             int tx0, int ty0, float cx, float cy) {
   const int uox = wu.ox, uoy = wu.oy;
@@ -533,11 +556,12 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
     int ly = q / AFW, lx = q - ly * AFW;
     int i = WRAP ? wrapi(fox + lx, nx) : fox + lx, j = WRAP ? wrapi(foy + ly, ny) : foy + ly;
     // the forward trace from a node is also the limiter's trace at that node: keep the
-    // bounds of its departure point (lo u, hi u, lo v, hi v)
+    // bounds of its departure point where the node is corrected
     float4 bd;
     sfu[q] = fwd_node_bounds<0, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy, bd.x, bd.y);
     sfv[q] = fwd_node_bounds<1, FAST, WRAP>(wu, wv, nx, ny, i, j, cx, cy, bd.z, bd.w);
-    sbd[q] = bd;
+    const unsigned mx = (unsigned)(lx - (AF - 1)), my = (unsigned)(ly - (AF - 1));
+    if (mx < (unsigned)AMW && my < (unsigned)AMW) sbd[my * AMW + mx] = bd;
   }
   __syncthreads();
   // ---- correction + limiter on the tile and a 1-cell ring
@@ -546,10 +570,10 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
     int i = WRAP ? wrapi(tx0 - 1 + lx, nx) : tx0 - 1 + lx, j = WRAP ? wrapi(ty0 - 1 + ly, ny) : ty0 - 1 + ly;
     // this node's place in the u, v window and in the forward window
     const int wiu = WRAP ? win_index(i, uox, nx) : i - uox, wju = WRAP ? win_index(j, uoy, ny) : j - uoy;
-    const int wif = WRAP ? win_index(i, fox, nx) : i - fox, wjf = WRAP ? win_index(j, foy, ny) : j - foy;
     const bool inu = FAST || ((unsigned)wiu < (unsigned)AUW && (unsigned)wju < (unsigned)AUW);
-    const bool inf = FAST || ((unsigned)wif < (unsigned)AFW && (unsigned)wjf < (unsigned)AFW);
-    float Xb, Yb, lo, hi, u1, v1;
+    const int qf = (ly + AF - 1) * AFW + lx + AF - 1;
+    const float4 bd = sbd[q];
+    float Xb, Yb, u1, v1, cu, cv;
     // the backward and the forward trace start from the same velocity at the face,
     // whose own component is the face's value
     {  // u face at (i, j+1/2)
@@ -558,17 +582,8 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
       v1 = mid_w<FAST, WRAP>(wv, nx, ny, i - 1, j);  // bilerp at (X - 1/2, Y)
       trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<0, FAST, WRAP>(sfu, fox, foy, wu, wv, nx, ny, Xb, Yb - 0.5f, cx, cy);
-      float fo = inf ? sfu[wjf * AFW + wif] : fwd_node_far<0>(wu, wv, nx, ny, i, j, cx, cy);
-      float c = fmaf(0.5f, u1 - back, fo);
-      if (inf) {
-        const float4 bd = sbd[wjf * AFW + wif];
-        lo = bd.x;
-        hi = bd.y;
-      } else {
-        trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
-        bounds4_w<FAST, WRAP>(wu, nx, ny, Xb, Yb - 0.5f, lo, hi);
-      }
-      smu[q] = c < lo ? lo : (c > hi ? hi : c);
+      float c = fmaf(0.5f, u1 - back, sfu[qf]);
+      cu = c < bd.x ? bd.x : (c > bd.y ? bd.y : c);
     }
     {  // v face at (i+1/2, j)
       float X = i + 0.5f, Y = (float)j;
@@ -576,30 +591,31 @@ __device__ __forceinline__ void advect_tile(const AdvWin& wu, const AdvWin& wv, 
       v1 = inu ? wv.w[wju * AUW + wiu] : v[(size_t)j * nx + i];
       trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, -cx, -cy, Xb, Yb);
       float back = back_w<1, FAST, WRAP>(sfv, fox, foy, wu, wv, nx, ny, Xb - 0.5f, Yb, cx, cy);
-      float fo = inf ? sfv[wjf * AFW + wif] : fwd_node_far<1>(wu, wv, nx, ny, i, j, cx, cy);
-      float c = fmaf(0.5f, v1 - back, fo);
-      if (inf) {
-        const float4 bd = sbd[wjf * AFW + wif];
-        lo = bd.z;
-        hi = bd.w;
-      } else {
-        trace2_w<FAST, WRAP>(wu, wv, nx, ny, X, Y, u1, v1, cx, cy, Xb, Yb);
-        bounds4_w<FAST, WRAP>(wv, nx, ny, Xb - 0.5f, Yb, lo, hi);
-      }
-      smv[q] = c < lo ? lo : (c > hi ? hi : c);
+      float c = fmaf(0.5f, v1 - back, sfv[qf]);
+      cv = c < bd.z ? bd.z : (c > bd.w ? bd.w : c);
     }
+    *(float2*)&sbd[q] = float2{cu, cv};
   }
 }
 
-__global__ void __launch_bounds__(ANT)
+// LDS of one tile, 40.5 KB, so that four tiles share a CU
+struct AdvLds {
+  float su[AUW * AUW], sv[AUW * AUW];
+  float sfu[AFW * AFW], sfv[AFW * AFW];
+  float4 sbd[AMW * AMW];  // limiter bounds, then the corrected (u, v), per corrected node
+  float wmax[2][ANT / 64];
+};
+static_assert(sizeof(AdvLds) <= 40960, "four advection tiles per CU");
+
+// At most 64 VGPRs (eight wavefronts per SIMD, four tiles per CU with the LDS above):
+// the fast paths need about 40; the general path's extra registers go to scratch.
+__global__ void __launch_bounds__(ANT) __attribute__((amdgpu_waves_per_eu(8)))
 k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou,
                     float* __restrict__ ov, int nx, int ny, int tiles_x, int nblocks,
                     const StepCoef* __restrict__ cf, int diffuse) {
-  __shared__ float su[AUW * AUW], sv[AUW * AUW];
-  __shared__ float sfu[AFW * AFW], sfv[AFW * AFW];
-  __shared__ float4 sbd[AFW * AFW];  // limiter bounds at each forward node's departure point
-  __shared__ float smu[AMW * AMW], smv[AMW * AMW];
-  __shared__ float wmax[2][ANT / 64];
+  __shared__ AdvLds L;
+  float* su = L.su;
+  float* sv = L.sv;
   int b = (int)blockIdx.x;
   if ((nblocks & 7) == 0) b = (b & 7) * (nblocks >> 3) + (b >> 3);
   const int tx0 = (b % tiles_x) * AT, ty0 = (b / tiles_x) * AT;
@@ -608,14 +624,30 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
   // the largest |u|, |v| of the window
   const int uox = tx0 - AU, uoy = ty0 - AU;
   float mu = 0.f, mv = 0.f;
-  for (int q = threadIdx.x; q < AUW * AUW; q += ANT) {
-    int ly = q / AUW, lx = q - ly * AUW;
-    size_t g = (size_t)wrapi(uoy + ly, ny) * nx + wrapi(uox + lx, nx);
-    float a = u[g], c = v[g];
-    su[q] = a;
-    sv[q] = c;
-    mu = fmaxf(mu, fabsf(a));
-    mv = fmaxf(mv, fabsf(c));
+  {
+    // every load of the window in flight before the first LDS store
+    constexpr int NL = (AUW * AUW + ANT - 1) / ANT;
+    float ua[NL], va[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int q = (int)threadIdx.x + k * ANT;
+      if (q < AUW * AUW) {
+        int ly = q / AUW, lx = q - ly * AUW;
+        size_t g = (size_t)wrapi(uoy + ly, ny) * nx + wrapi(uox + lx, nx);
+        ua[k] = u[g];
+        va[k] = v[g];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      const int q = (int)threadIdx.x + k * ANT;
+      if (q < AUW * AUW) {
+        su[q] = ua[k];
+        sv[q] = va[k];
+        mu = fmaxf(mu, fabsf(ua[k]));
+        mv = fmaxf(mv, fabsf(va[k]));
+      }
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -623,18 +655,18 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
     mv = fmaxf(mv, __shfl_xor(mv, o, 64));
   }
   if ((threadIdx.x & 63) == 0) {
-    wmax[0][threadIdx.x >> 6] = mu;
-    wmax[1][threadIdx.x >> 6] = mv;
+    L.wmax[0][threadIdx.x >> 6] = mu;
+    L.wmax[1][threadIdx.x >> 6] = mv;
   }
   __syncthreads();
-  mu = wmax[0][0];
-  mv = wmax[1][0];
+  mu = L.wmax[0][0];
+  mv = L.wmax[1][0];
 #pragma unroll
   for (int w = 1; w < ANT / 64; w++) {
-    mu = fmaxf(mu, wmax[0][w]);
-    mv = fmaxf(mv, wmax[1][w]);
+    mu = fmaxf(mu, L.wmax[0][w]);
+    mv = fmaxf(mv, L.wmax[1][w]);
   }
-  const AdvWin wu{su, uox, uoy, AUW, u}, wv{sv, uox, uoy, AUW, v};
+  const AdvWin wu{su, uox, uoy, uoy * AUW + uox, u}, wv{sv, uox, uoy, uoy * AUW + uox, v};
   // every velocity a trace reads comes from the window, so a trace moves at most
   // cx max|u| by cy max|v| cells; the rings hold every sample for moves up to one
   // cell (NaN fails the test and takes the general path)
@@ -642,24 +674,31 @@ k_advect_mc_diffuse(const float* __restrict__ u, const float* __restrict__ v, fl
   // a window that does not cross the periodic seam needs no wrapped indices
   const bool inner = uox >= 0 && uoy >= 0 && uox + AUW <= nx && uoy + AUW <= ny;
   if (fast && inner)
-    advect_tile<true, false>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<true, false>(wu, wv, L.sfu, L.sfv, L.sbd, u, v, nx, ny, tx0, ty0, cx, cy);
   else if (fast)
-    advect_tile<true, true>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<true, true>(wu, wv, L.sfu, L.sfv, L.sbd, u, v, nx, ny, tx0, ty0, cx, cy);
   else
-    advect_tile<false, true>(wu, wv, sfu, sfv, sbd, smu, smv, u, v, nx, ny, tx0, ty0, cx, cy);
+    advect_tile<false, true>(wu, wv, L.sfu, L.sfv, L.sbd, u, v, nx, ny, tx0, ty0, cx, cy);
   __syncthreads();
   // ---- diffusion (or the corrected field itself) on the tile
   const float a = cf->da, bb = cf->db;
+  const float4* sm = L.sbd;
   for (int q = threadIdx.x; q < AT * AT; q += ANT) {
     int iy = q / AT, ix = q - iy * AT;
     int i = tx0 + ix, j = ty0 + iy;
     if (i >= nx || j >= ny) continue;
     int m = (iy + 1) * AMW + ix + 1;
     size_t k = (size_t)j * nx + i;
-    float c = smu[m];
-    ou[k] = diffuse ? c + a * (smu[m - 1] - 2.f * c + smu[m + 1]) + bb * (smu[m - AMW] - 2.f * c + smu[m + AMW]) : c;
-    c = smv[m];
-    ov[k] = diffuse ? c + a * (smv[m - 1] - 2.f * c + smv[m + 1]) + bb * (smv[m - AMW] - 2.f * c + smv[m + AMW]) : c;
+    const float2 c = *(const float2*)&sm[m];
+    if (!diffuse) {
+      ou[k] = c.x;
+      ov[k] = c.y;
+      continue;
+    }
+    const float2 l = *(const float2*)&sm[m - 1], r = *(const float2*)&sm[m + 1];
+    const float2 d = *(const float2*)&sm[m - AMW], t = *(const float2*)&sm[m + AMW];
+    ou[k] = c.x + a * (l.x - 2.f * c.x + r.x) + bb * (d.x - 2.f * c.x + t.x);
+    ov[k] = c.y + a * (l.y - 2.f * c.y + r.y) + bb * (d.y - 2.f * c.y + t.y);
   }
 }
 
