@@ -444,6 +444,276 @@ k_mgt_down0(const float* __restrict__ u, const float* __restrict__ v, const doub
   tile_store_and_restrict<H, RW, TS>(sx, sr, x, crhs, n, tx0, ty0, ih2);
 }
 
+// ------------------------------------------- fixed thread -> cell-pair map passes
+
+// Thread t owns the cell pairs (2p, 2p + 1) of rows r, r + RG, r + 2 RG, ... of an
+// RW x RH LDS region, p = t % PC, r = t / PC. The red-black passes below update every
+// interior cell of one colour on the whole region instead of shrinking it by a cell per
+// half-sweep: a cell at distance d from the region's edge is exact after k half-sweeps
+// whenever d >= k, as in the shrinking passes, and the cells nearer the edge are never
+// read by a consumer. Each cell sees the same operands in the same order as
+// lds_half_sweep, so the results are the same bits; the indices are computed once per
+// launch instead of once per pass.
+template <int RW, int RH, int NT>
+struct PairMap {
+  static constexpr int PC = RW / 2, RG = NT / PC, ROWS = (RH + RG - 1) / RG;
+  static_assert(RW % 2 == 0 && RG >= 1, "pair map needs an even width and a row of pairs per group");
+  int base, r, p;
+  bool on;
+  __device__ __forceinline__ PairMap() {
+    const int t = threadIdx.x;
+    r = t / PC;
+    p = t - r * PC;
+    on = r < RG;
+    base = r * RW + 2 * p;
+  }
+  __device__ __forceinline__ bool row_ok(int k) const { return on && r + k * RG < RH; }
+};
+
+// Red pass from zero on the whole region: x = -h2 rhs / 4 on red cells, 0 on black.
+template <int RW, int RH, int NT>
+__device__ __forceinline__ void fm_from_zero(float* __restrict__ sx, const float* __restrict__ sr,
+                                             const PairMap<RW, RH, NT>& m, int par0, float h2) {
+  using M = PairMap<RW, RH, NT>;
+#pragma unroll
+  for (int k = 0; k < M::ROWS; k++) {
+    if (!m.row_ok(k)) continue;
+    const int c = m.base + k * M::RG * RW;
+    const bool red0 = ((m.r + k * M::RG + par0) & 1) == 0;  // is cell 2p red
+    const float2 r2 = *(const float2*)(sr + c);
+    float2 x2;
+    x2.x = red0 ? (0.f - h2 * r2.x) * 0.25f : 0.f;
+    x2.y = red0 ? 0.f : (0.f - h2 * r2.y) * 0.25f;
+    *(float2*)(sx + c) = x2;
+  }
+}
+
+// One colour pass on every interior cell of the colour (rows 1 .. RH-2, columns 1 .. RW-2).
+template <int RW, int RH, int NT>
+__device__ __forceinline__ void fm_half_sweep(float* __restrict__ sx, const float* __restrict__ sr,
+                                              const PairMap<RW, RH, NT>& m, int colour, int par0, float h2) {
+  using M = PairMap<RW, RH, NT>;
+  float acc[M::ROWS];
+  int cc[M::ROWS];
+  bool act[M::ROWS];
+#pragma unroll
+  for (int k = 0; k < M::ROWS; k++) {
+    const int ly = m.r + k * M::RG;
+    const int off = (colour + par0 + ly) & 1;  // the colour's cell of the pair
+    act[k] = m.row_ok(k) && ly >= 1 && ly <= RH - 2 && !(off == 0 && m.p == 0) && !(off == 1 && m.p == M::PC - 1);
+    const int c = m.base + k * M::RG * RW + off;
+    cc[k] = c;
+    if (act[k]) {
+      const float sum = sx[c - 1] + sx[c + 1] + sx[c - RW] + sx[c + RW];
+      acc[k] = (sum - h2 * sr[c]) * 0.25f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < M::ROWS; k++)
+    if (act[k]) sx[cc[k]] = acc[k];
+}
+
+// Pre-smoothing from zero with SW sweeps (2 SW half-sweeps: red from zero, then black,
+// red, ...), the last one followed by a barrier.
+template <int SW, int RW, int RH, int NT>
+__device__ __forceinline__ void fm_smooth_from_zero(float* __restrict__ sx, const float* __restrict__ sr, int par0,
+                                                    float h2) {
+  const PairMap<RW, RH, NT> m;
+  fm_from_zero<RW, RH, NT>(sx, sr, m, par0, h2);
+  __syncthreads();
+#pragma unroll
+  for (int s = 1; s < 2 * SW; s++) {
+    fm_half_sweep<RW, RH, NT>(sx, sr, m, s & 1, par0, h2);
+    __syncthreads();
+  }
+}
+
+template <typename S>
+__device__ __forceinline__ void store4(S* __restrict__ dst, float a, float b, float c, float d);
+template <>
+__device__ __forceinline__ void store4<float>(float* __restrict__ dst, float a, float b, float c, float d) {
+  *(float4*)dst = make_float4(a, b, c, d);
+}
+template <>
+__device__ __forceinline__ void store4<bf16>(bf16* __restrict__ dst, float a, float b, float c, float d) {
+  const uint32_t lo = (uint32_t)st<bf16>(a).b | ((uint32_t)st<bf16>(b).b << 16);
+  const uint32_t hi = (uint32_t)st<bf16>(c).b | ((uint32_t)st<bf16>(d).b << 16);
+  *(uint2*)dst = make_uint2(lo, hi);
+}
+
+// k_mgt_down0 with every global access 8 or 16 bytes wide and fixed thread maps: u, v, p
+// and the previous correction are staged in LDS (p + e in FP64), the FP64 residual is
+// evaluated four cells per thread from LDS, the passes use PairMap, and x leaves four
+// cells per store. Same operands in the same order as k_mgt_down0, so the same bits.
+// The host uses it when H = 2 SW is a multiple of 4 and n is a multiple of 4, TX and TY
+// (tiles are whole, quads and pairs never straddle the periodic seam, and every
+// 16-byte access is aligned); n * n < 2^31 (32-bit offsets).
+template <int SW, class TS, typename S>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_down0q(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
+             const float* __restrict__ e_prev, double* __restrict__ pout, S* __restrict__ r32, S* __restrict__ x,
+             S* __restrict__ crhs, int n, double idx, double idx2, float h2, float ih2, int tiles_x, int nblocks,
+             double* __restrict__ rpart, int residual_only, Gate gate) {
+  if (gate_closed(gate)) return;
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad accesses need H and TX multiples of 4");
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  constexpr int UQ = RW / 4 + 1, UW = 4 * UQ;  // u: columns ox .. ox + RW (one past the region for uR)
+  constexpr int VQ = RW / 4;                   // v: rows oy .. oy + RH (one past for vT)
+  constexpr int PP = RW / 2 + 2, PW = 2 * PP;  // p, e: columns ox - 2 .. ox + RW + 1 in pairs
+  constexpr int PH = RH + 2;                   // rows oy - 1 .. oy + RH
+  __shared__ __align__(16) double sp[PH * PW];
+  __shared__ __align__(16) float sxu[RH * UW];  // u window, then the correction x (row stride RW)
+  __shared__ __align__(16) float sv[(RH + 1) * RW];
+  __shared__ __align__(16) float sr[RH * RW];
+  __shared__ double wm[NT / 64];
+  float* __restrict__ su = sxu;
+  float* __restrict__ sx = sxu;
+  const int t = xcd_tile(nblocks);
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const int tid = threadIdx.x;
+  // ---- every load in flight before the first LDS store
+  constexpr int CU = UQ * RH, KU = (CU + NT - 1) / NT;
+  constexpr int CV = VQ * (RH + 1), KV = (CV + NT - 1) / NT;
+  constexpr int CP = PP * PH, KP = (CP + NT - 1) / NT;
+  float tu[KU][4], tv[KV][4];
+  double tp[KP][2];
+  float te[KP][2];
+#pragma unroll
+  for (int k = 0; k < KU; k++) {
+    const int q = tid + k * NT;
+    if (q < CU) {
+      const int ly = q / UQ, qx = q - ly * UQ;
+      const float4 w = *(const float4*)(u + wrapn(oy + ly, n) * n + wrapn(ox + 4 * qx, n));
+      tu[k][0] = w.x, tu[k][1] = w.y, tu[k][2] = w.z, tu[k][3] = w.w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KV; k++) {
+    const int q = tid + k * NT;
+    if (q < CV) {
+      const int ly = q / VQ, qx = q - ly * VQ;
+      const float4 w = *(const float4*)(v + wrapn(oy + ly, n) * n + wrapn(ox + 4 * qx, n));
+      tv[k][0] = w.x, tv[k][1] = w.y, tv[k][2] = w.z, tv[k][3] = w.w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const int q = tid + k * NT;
+    tp[k][0] = tp[k][1] = 0.0;
+    te[k][0] = te[k][1] = 0.f;
+    if (q < CP) {
+      const int ly = q / PP, qx = q - ly * PP;
+      const int g = wrapn(oy - 1 + ly, n) * n + wrapn(ox - 2 + 2 * qx, n);
+      if (p) {
+        const double2 w = *(const double2*)(p + g);
+        tp[k][0] = w.x, tp[k][1] = w.y;
+      }
+      if (e_prev) {
+        const float2 w = *(const float2*)(e_prev + g);
+        te[k][0] = w.x, te[k][1] = w.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KU; k++) {
+    const int q = tid + k * NT;
+    if (q < CU) *(float4*)(su + 4 * q) = make_float4(tu[k][0], tu[k][1], tu[k][2], tu[k][3]);
+  }
+#pragma unroll
+  for (int k = 0; k < KV; k++) {
+    const int q = tid + k * NT;
+    if (q < CV) *(float4*)(sv + 4 * q) = make_float4(tv[k][0], tv[k][1], tv[k][2], tv[k][3]);
+  }
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const int q = tid + k * NT;
+    if (q < CP) {
+      double2 s;
+      s.x = e_prev ? tp[k][0] + (double)te[k][0] : tp[k][0];
+      s.y = e_prev ? tp[k][1] + (double)te[k][1] : tp[k][1];
+      *(double2*)(sp + 2 * q) = s;
+    }
+  }
+  __syncthreads();
+  // ---- FP64 residual r = div u* - L p' on the region, four cells per thread
+  double amax = 0.0;
+  {
+    constexpr int CQ = VQ * RH, KQ = (CQ + NT - 1) / NT;
+#pragma unroll
+    for (int k = 0; k < KQ; k++) {
+      const int q = tid + k * NT;
+      if (q < CQ) {
+        const int ly = q / VQ, qx = q - ly * VQ, lx0 = 4 * qx;
+        const float4 ul = *(const float4*)(su + ly * UW + lx0);
+        const float ur4 = su[ly * UW + lx0 + 4];
+        const float4 vb = *(const float4*)(sv + ly * RW + lx0);
+        const float4 vt = *(const float4*)(sv + (ly + 1) * RW + lx0);
+        const double* pc = sp + (ly + 1) * PW + lx0 + 2;  // p' of cell (lx0, ly)
+        const double p0 = pc[0], p1 = pc[1], p2 = pc[2], p3 = pc[3];
+        auto res = [&](double pl, double c, double pr, double pb, double pt, float uL, float uR, float vB,
+                       float vT) {
+          const double lap = (pl - 2 * c + pr) * idx2 + (pb - 2 * c + pt) * idx2;
+          const double dv = ((double)uR - (double)uL) * idx + ((double)vT - (double)vB) * idx;
+          return dv - lap;
+        };
+        const double r0 = res(pc[-1], p0, p1, pc[-PW], pc[PW], ul.x, ul.y, vb.x, vt.x);
+        const double r1 = res(p0, p1, p2, pc[1 - PW], pc[1 + PW], ul.y, ul.z, vb.y, vt.y);
+        const double r2 = res(p1, p2, p3, pc[2 - PW], pc[2 + PW], ul.z, ul.w, vb.z, vt.z);
+        const double r3 = res(p2, p3, pc[4], pc[3 - PW], pc[3 + PW], ul.w, ur4, vb.w, vt.w);
+        *(float4*)(sr + ly * RW + lx0) =
+            make_float4(rnd<S>((float)r0), rnd<S>((float)r1), rnd<S>((float)r2), rnd<S>((float)r3));
+        if (lx0 >= H && lx0 < H + TX && ly >= H && ly < H + TY) {
+          const int g = (oy + ly) * n + ox + lx0;
+          *(double2*)(pout + g) = make_double2(p0, p1);
+          *(double2*)(pout + g + 2) = make_double2(p2, p3);
+          store4<S>(r32 + g, (float)r0, (float)r1, (float)r2, (float)r3);
+          amax = fmax(amax, fmax(fmax(nan_to_inf(fabs(r0)), nan_to_inf(fabs(r1))),
+                                 fmax(nan_to_inf(fabs(r2)), nan_to_inf(fabs(r3)))));
+        }
+      }
+    }
+  }
+  {
+    double m = wave_max(amax);
+    if ((tid & 63) == 0) wm[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      double b = wm[0];
+      for (int w = 1; w < NT / 64; w++) b = fmax(b, wm[w]);
+      rpart[t] = b;
+      gate_record(gate, b);
+    }
+  }
+  if (residual_only) return;
+  // ---- S sweeps from zero (the u window is no longer read: sx reuses its LDS)
+  fm_smooth_from_zero<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
+  // ---- x on the tile, four cells per store; the restricted residual of each 2x2 block
+  constexpr int XQ = TX / 4, CXQ = XQ * TY;
+  for (int q = tid; q < CXQ; q += NT) {
+    const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+    const float4 xv = *(const float4*)(sx + (iy + H) * RW + H + m4);
+    store4<S>(x + (ty0 + iy) * n + tx0 + m4, xv.x, xv.y, xv.z, xv.w);
+  }
+  constexpr int CX = TX / 2, CY = TY / 2;
+  const int cn = n >> 1;
+  for (int q = tid; q < CX * CY; q += NT) {
+    const int cy = q / CX, cx = q - cy * CX;
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int k = (2 * cy + b + H) * RW + (2 * cx + a + H);
+        const float lap = (sx[k - 1] + sx[k + 1] + sx[k - RW] + sx[k + RW] - 4.f * sx[k]) * ih2;
+        s += sr[k] - lap;
+      }
+    crhs[((ty0 >> 1) + cy) * cn + (tx0 >> 1) + cx] = st<S>(s * 0.25f);
+  }
+}
+
 // -------------------------------------------------------------------- up
 
 // x += bilinear prolongation of the coarse correction (PROLONG, src/gpu/multigrid.js:99-131),

@@ -779,6 +779,17 @@ int launch_down0_s(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_o
   const double es = sizeof(S);
   // u, v (8) + p (8) + previous correction (FP32) read; p (8) + r written; then x and the coarse rhs (1/4 cell)
   double bpc = 8 + (zero_p ? 0 : 8) + (it > 0 ? 4 : 0) + 8 + es + (residual_only ? 0 : es + 0.25 * es);
+  // 16-byte accesses and fixed thread maps when tiles are whole and quads stay aligned
+  if constexpr ((2 * PRE) % 4 == 0 && TS::TX % 4 == 0) {
+    const int n = L0.n;
+    if (n % 4 == 0 && n % TS::TX == 0 && n % TS::TY == 0 && n <= 32768)
+      return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0q<PRE, TS, S>, g, dim3(TS::NT),
+                    (const float*)s->u[c], (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
+                    (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
+                    as<S>(s->lv[1].rhs), L0.n, idx, idx * idx, (float)(L0.dx * L0.dx),
+                    (float)(1.0 / (L0.dx * L0.dx)), tx, nb, s->rpart + (size_t)it * s->rp_cols,
+                    residual_only ? 1 : 0, s->gate_r);
+  }
   return launch(s, PK_DOWN0, bpc * L0.n * L0.n, fsk::k_mgt_down0<PRE, TS, S>, g, dim3(TS::NT), (const float*)s->u[c],
                 (const float*)s->v[c], (const double*)(zero_p ? nullptr : s->p64[pin]),
                 (const float*)(it > 0 ? L0.xb : nullptr), s->p64[1 - pin], as<S>(L0.rhs), as<S>(L0.x),
