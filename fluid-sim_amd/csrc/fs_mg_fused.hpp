@@ -714,6 +714,209 @@ k_mgt_down0q(const float* __restrict__ u, const float* __restrict__ v, const dou
   }
 }
 
+// ------------------------------------------------- exact coarsest-level solve
+// The 5-point Laplacian of an n x n level (n <= COARSE_N) is T (x) I + I (x) T with T
+// the 1D second difference, and T = Q diag(lam) Q^T for an orthonormal eigenbasis Q:
+// the real Fourier basis on a periodic level, the DCT-II basis on a walled (zero-flux)
+// one; the host builds Q and W for either. So
+// L x = r - mean(r) is solved by x = Q ((Q^T r Q) * W) Q^T with W[a][b] = 1 / (lam_a +
+// lam_b) and W[0][0] = 0 (the zero-mean solution, as a pseudo-inverse gives). The four
+// 64 x 64 x 64 products run on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with
+// every operand split into a BF16 head and tail, x = h + l with h = bf16(x) and
+// l = bf16(x - h): a b ~ ha hb + ha lb + la hb accumulated in FP32 (the dropped la lb
+// and the tails' own rounding leave ~2^-16 relative per product). One 16 x 16 output
+// block per wavefront; Q and W come from the host (built in FP64), zero-padded to
+// 64 x 64 and laid out in the order the lanes read them, so n < 64 needs no special case.
+//
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8 (l / 16) + t] and
+// B[8 (l / 16) + t][l % 16], t = 0..7, and receives D[4 (l / 16) + i][l % 16] in its
+// i-th accumulator.
+constexpr int EXN = 64;   // padded level size
+constexpr int EXB = 72;   // LDS row pitch (BF16): 16-byte reads of 16 rows hit distinct banks
+constexpr int EXT = 1024; // 16 wavefronts, one 16 x 16 block of the 64 x 64 result each
+// Host-built operand table, in 16-byte units: Q[k][16 T + i] (T the block, i = l % 16,
+// k = 32 kk + 8 (l / 16) + t) heads then tails, Q[16 T + i][k] heads then tails (each
+// [T 4][lane 64][kk 2] x 8 BF16), then W at each lane's four outputs ([wave 16][lane 64]
+// x 4 FP32).
+constexpr int EX_TAB = 4 * 512 + 1024;
+typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 ex_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
+
+// x into LDS as head (at h[o]) and tail (at l[o])
+__device__ __forceinline__ void ex_put(__bf16* h, __bf16* l, int o, float x) {
+  const __bf16 hx = (__bf16)x;
+  h[o] = hx;
+  l[o] = (__bf16)(x - (float)hx);
+}
+
+// acc over the 64-long k range: heads and tails of the two operands of k-chunk kk
+__device__ __forceinline__ ex_f32x4 ex_mma3(ex_f32x4 acc, ex_bf16x8 ah, ex_bf16x8 al, ex_bf16x8 bh, ex_bf16x8 bl) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+}
+
+// The exact solve by the NT / 64 wavefronts of a workgroup, for any 16 x 16 output
+// blocks: products 1 and 2 on all 16 blocks, products 3 and 4 on the blocks in the row
+// blocks of rbm (bit tm) and, for the last product, the column blocks of cbm (bit tn);
+// put(row, col, x) receives every element of the last product's blocks (row, col < n).
+// Sixteen wavefronts compute block (w / 4, w % 4) of every product; four compute the
+// blocks (w, j) of product 1 and (j, w) of the others, j = 0..3. Each block is the same
+// MFMA sequence on the same operands either way, so the same bits.
+// LDS: the four 64 x EXB BF16 matrices sah, sal, sbh, sbl (A and B, heads and tails).
+template <int NT, int NS, typename S, class Put>
+__device__ __forceinline__ void ex_solve(const S* __restrict__ rg, int n, const uint4* __restrict__ tab,
+                                         __bf16* __restrict__ sah, __bf16* __restrict__ sal, __bf16* __restrict__ sbh,
+                                         __bf16* __restrict__ sbl, unsigned rbm, unsigned cbm, Put put) {
+  constexpr int NW = NT / 64, NJ = 16 / NW;
+  static_assert(NW == 16 || NW == 4, "the exact solve runs on 4 or 16 wavefronts");
+  static_assert(NS >= 1 && NS <= NJ, "row-block slots of products 3 and 4");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  // block j of wavefront w: product 1 (tm, tn) = P1, the others P2
+  auto tm1 = [&](int j) { return NW == 16 ? w >> 2 : w; };
+  auto tn1 = [&](int j) { return NW == 16 ? w & 3 : j; };
+  auto tm2 = [&](int j) { return NW == 16 ? w >> 2 : j; };
+  auto tn2 = [&](int j) { return NW == 16 ? w & 3 : w; };
+  // the right-hand side first (four consecutive cells of one row per thread), then the
+  // Q operands in the order the products use them
+  constexpr int KR = 1024 / NT;
+  float rv[KR][4];
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int q = threadIdx.x + k * NT, ri = q >> 4, rj = 4 * (q & 15);
+    if (n == EXN) {
+      if constexpr (sizeof(S) == 4) {
+        const float4 t = *(const float4*)((const float*)rg + ri * EXN + rj);
+        rv[k][0] = t.x, rv[k][1] = t.y, rv[k][2] = t.z, rv[k][3] = t.w;
+      } else {
+        const uint2 t = *(const uint2*)((const uint16_t*)rg + ri * EXN + rj);
+        rv[k][0] = __uint_as_float(t.x << 16), rv[k][1] = __uint_as_float(t.x & 0xffff0000u);
+        rv[k][2] = __uint_as_float(t.y << 16), rv[k][3] = __uint_as_float(t.y & 0xffff0000u);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) rv[k][u] = (ri < n && rj + u < n) ? ld(rg[ri * n + rj + u]) : 0.f;
+    }
+  }
+  auto q8 = [&](int part, int T, int kk) { return __builtin_bit_cast(ex_bf16x8, tab[part * 512 + (T * 64 + lane) * 2 + kk]); };
+  // operands of products 1 and 2 (fixed per wavefront), W at the lane's outputs
+  ex_bf16x8 a1h[2], a1l[2], b2h[2], b2l[2];
+  float4 wv[NJ];
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) a1h[kk] = q8(0, tm1(0), kk), a1l[kk] = q8(1, tm1(0), kk);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) b2h[kk] = q8(0, tn2(0), kk), b2l[kk] = q8(1, tn2(0), kk);
+#pragma unroll
+  for (int j = 0; j < NJ; j++) wv[j] = ((const float4*)(tab + 2048))[(4 * tm2(j) + tn2(j)) * 64 + lane];
+  // r transposed into A: RT[j][i] = r[i][j]
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const int q = threadIdx.x + k * NT, ri = q >> 4, rj = 4 * (q & 15);
+#pragma unroll
+    for (int u = 0; u < 4; u++) ex_put(sah, sal, (rj + u) * EXB + ri, rv[k][u]);
+  }
+  __syncthreads();
+  // C1 = Q^T r (B operand RT rows), row-major into B
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int m0 = 16 * tm1(j), n0 = 16 * tn1(j);
+    ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
+      acc = ex_mma3(acc, a1h[kk], a1l[kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
+  }
+  // the operands of products 3 and 4 load while the products run: the row blocks of
+  // rbm this wavefront computes, at most NS of them (slot tm -1: none)
+  int stm[NS];
+  {
+    int k = 0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const bool on = (rbm >> tm2(j)) & 1;
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+        if (q == k) stm[q] = on ? tm2(j) : -1;
+      k += on ? 1 : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NS; q++)
+      if (q >= k) stm[q] = -1;
+  }
+  ex_bf16x8 a3h[NS][2], a3l[NS][2], b4h[2], b4l[2];
+#pragma unroll
+  for (int q = 0; q < NS; q++)
+    if (stm[q] >= 0)
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++) a3h[q][kk] = q8(2, stm[q], kk), a3l[q][kk] = q8(3, stm[q], kk);
+  if ((cbm >> tn2(0)) & 1)
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) b4h[kk] = q8(2, tn2(0), kk), b4l[kk] = q8(3, tn2(0), kk);
+  __syncthreads();
+  // C2 = (C1 Q) * W (A operand C1 rows), transposed into A: C2T[n][m]
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int m0 = 16 * tm2(j), n0 = 16 * tn2(j);
+    ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
+      acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b2h[kk], b2l[kk]);
+    }
+    const float c[4] = {acc[0] * wv[j].x, acc[1] * wv[j].y, acc[2] * wv[j].z, acc[3] * wv[j].w};
+    ex_bf16x4 h4, l4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const __bf16 hx = (__bf16)c[i];
+      h4[i] = hx;
+      l4[i] = (__bf16)(c[i] - (float)hx);
+    }
+    const int o = (n0 + li) * EXB + m0 + 4 * lg;
+    *(ex_bf16x4*)(sah + o) = h4;
+    *(ex_bf16x4*)(sal + o) = l4;
+  }
+  __syncthreads();
+  // C3 = Q C2 (B operand C2T rows), row-major into B, on the row blocks of rbm
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    if (stm[q] < 0) continue;
+    const int m0 = 16 * stm[q], n0 = 16 * tn2(0);
+    ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
+      acc = ex_mma3(acc, a3h[q][kk], a3l[q][kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
+  }
+  __syncthreads();
+  // x = C3 Q^T (A operand C3 rows) on the blocks of rbm x cbm
+  if (!((cbm >> tn2(0)) & 1)) return;
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    if (stm[q] < 0) continue;
+    const int m0 = 16 * stm[q], n0 = 16 * tn2(0);
+    ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
+      acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b4h[kk], b4l[kk]);
+    }
+    const int col = n0 + li;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = m0 + 4 * lg + i;
+      if (row < n && col < n) put(row, col, acc[i]);
+    }
+  }
+}
+
 // -------------------------------------------------------------------- up
 
 // x += bilinear prolongation of the coarse correction (PROLONG, src/gpu/multigrid.js:99-131),
@@ -784,11 +987,17 @@ k_mgt_up(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__
 // interpolates from, then the level-l up-sweep with those values read from LDS. The
 // level-(l+1) correction is used as the separate launch would store it (FP32) and is
 // not stored; the level-l correction equals the separate launches' bit for bit.
-template <int SW, class TS, typename S, typename SO>
+//
+// FOLD: level l + 2 is the 64 x 64 level solved exactly, and every workgroup solves it
+// itself (ex_solve on the level's rhs, rhs2) instead of reading the correction a
+// k_mg_coarse_exact launch would store: products 3 and 4 only on the output blocks that
+// cover the workgroup's footprint window, whose values go straight to LDS. The same
+// MFMA sequence per block, so the same bits, and one launch fewer per V-cycle.
+template <int SW, class TS, typename S, typename SO, bool FOLD = false>
 __global__ void __launch_bounds__(TS::NT)
 k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __restrict__ xc2,
           const S* __restrict__ x0in, SO* __restrict__ x0out, const S* __restrict__ rhs0, int n0, float h20,
-          float h21, int tiles_x, int nblocks, Gate gate) {
+          float h21, int tiles_x, int nblocks, Gate gate, const S* __restrict__ rhs2, const uint4* __restrict__ tab) {
   if (gate_closed(gate)) return;
   constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
   constexpr int H = 2 * SW;
@@ -804,6 +1013,8 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   __shared__ float sc1[CH1 * CW1];
   __shared__ float sx0[RH0 * RW0];
   __shared__ float sr0[RH0 * RW0];
+  constexpr int EXM = FOLD ? EXN * EXB : 1;
+  __shared__ __align__(16) __bf16 sah[EXM], sal[EXM], sbh[EXM], sbl[EXM];
   const int n1 = n0 >> 1, n2 = n1 >> 1;
   const int t = xcd_tile(nblocks);
   const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
@@ -820,16 +1031,30 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   WX0 wx0;
   Window<RW0, RH0, NT, S> wr0;
   wx1.issue(x1in, n1, ox1, oy1);
-  wc1.issue(xc2, n2, cox1, coy1);
+  if constexpr (!FOLD) wc1.issue(xc2, n2, cox1, coy1);
   wr1.issue(rhs1, n1, ox1, oy1);
   wx0.issue(x0in, n0, ox0, oy0);
   wr0.issue(rhs0, n0, ox0, oy0);
+  if constexpr (FOLD) {
+    // the footprint window of level l + 2 (n2 = 64): its row and column blocks, then
+    // the solve with the window's values put in sc1
+    const int wr0_ = coy1 & (EXN - 1), wc0_ = cox1 & (EXN - 1);
+    unsigned rbm = 0, cbm = 0;
+    for (int r = 0; r < CH1; r++) rbm |= 1u << (((wr0_ + r) & (EXN - 1)) >> 4);
+    for (int c = 0; c < CW1; c++) cbm |= 1u << (((wc0_ + c) & (EXN - 1)) >> 4);
+    // a window of CH1 rows touches at most CH1 / 16 + 2 of the four row blocks
+    constexpr int NS = NT == 1024 ? 1 : ((CH1 + 15) / 16 + 1 < 4 ? (CH1 + 15) / 16 + 1 : 4);
+    ex_solve<NT, NS, S>(rhs2, EXN, tab, sah, sal, sbh, sbl, rbm, cbm, [&](int row, int col, float x) {
+      const int lr = (row - wr0_) & (EXN - 1), lc = (col - wc0_) & (EXN - 1);
+      if (lr < CH1 && lc < CW1) sc1[lr * CW1 + lc] = x;
+    });
+  }
   // ---- level l + 1: x + prolong(correction of level l + 2), then the post-sweeps
   {
     constexpr int CNT = RW1 * RH1, K = (CNT + NT - 1) / NT;
     constexpr bool PAIRS = WX1::PAIRS;
     if constexpr (PAIRS) wx1.commit(sx1);
-    wc1.commit(sc1);
+    if constexpr (!FOLD) wc1.commit(sc1);
     wr1.commit(sr1);
     if constexpr (WX0::PAIRS) wx0.commit(sx0);
     wr0.commit(sr0);
@@ -1403,49 +1628,6 @@ k_mg_coarse_ml(SO* __restrict__ xg, const S* __restrict__ rg, const LevelTable t
   for (int q = threadIdx.x; q < n0 * n0; q += CTH) xg[q] = st<SO>(bx[q]);
 }
 
-// ------------------------------------------------- exact coarsest-level solve
-// The 5-point Laplacian of an n x n level (n <= COARSE_N) is T (x) I + I (x) T with T
-// the 1D second difference, and T = Q diag(lam) Q^T for an orthonormal eigenbasis Q:
-// the real Fourier basis on a periodic level, the DCT-II basis on a walled (zero-flux)
-// one; the host builds Q and W for either. So
-// L x = r - mean(r) is solved by x = Q ((Q^T r Q) * W) Q^T with W[a][b] = 1 / (lam_a +
-// lam_b) and W[0][0] = 0 (the zero-mean solution, as a pseudo-inverse gives). The four
-// 64 x 64 x 64 products run on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with
-// every operand split into a BF16 head and tail, x = h + l with h = bf16(x) and
-// l = bf16(x - h): a b ~ ha hb + ha lb + la hb accumulated in FP32 (the dropped la lb
-// and the tails' own rounding leave ~2^-16 relative per product). One 16 x 16 output
-// block per wavefront; Q and W come from the host (built in FP64), zero-padded to
-// 64 x 64 and laid out in the order the lanes read them, so n < 64 needs no special case.
-//
-// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8 (l / 16) + t] and
-// B[8 (l / 16) + t][l % 16], t = 0..7, and receives D[4 (l / 16) + i][l % 16] in its
-// i-th accumulator.
-constexpr int EXN = 64;   // padded level size
-constexpr int EXB = 72;   // LDS row pitch (BF16): 16-byte reads of 16 rows hit distinct banks
-constexpr int EXT = 1024; // 16 wavefronts, one 16 x 16 block of the 64 x 64 result each
-// Host-built operand table, in 16-byte units: Q[k][16 T + i] (T the block, i = l % 16,
-// k = 32 kk + 8 (l / 16) + t) heads then tails, Q[16 T + i][k] heads then tails (each
-// [T 4][lane 64][kk 2] x 8 BF16), then W at each lane's four outputs ([wave 16][lane 64]
-// x 4 FP32).
-constexpr int EX_TAB = 4 * 512 + 1024;
-typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 ex_bf16x4 __attribute__((ext_vector_type(4)));
-typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
-
-// x into LDS as head (at h[o]) and tail (at l[o])
-__device__ __forceinline__ void ex_put(__bf16* h, __bf16* l, int o, float x) {
-  const __bf16 hx = (__bf16)x;
-  h[o] = hx;
-  l[o] = (__bf16)(x - (float)hx);
-}
-
-// acc over the 64-long k range: heads and tails of the two operands of k-chunk kk
-__device__ __forceinline__ ex_f32x4 ex_mma3(ex_f32x4 acc, ex_bf16x8 ah, ex_bf16x8 al, ex_bf16x8 bh, ex_bf16x8 bl) {
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-}
-
 // base: null for one grid, else one grid per workgroup at element offset base[blockIdx.x]
 // of xg and rg (a batch of cavities, each with its own level of n x n cells)
 template <typename S>
@@ -1459,95 +1641,8 @@ k_mg_coarse_exact(float* __restrict__ xg, const S* __restrict__ rg, int n, const
   }
   // two matrices, heads and tails; A/B alternate as input and output of the products
   __shared__ __align__(16) __bf16 sah[EXN * EXB], sal[EXN * EXB], sbh[EXN * EXB], sbl[EXN * EXB];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int li = lane & 15, lg = lane >> 4;
-  const int tm = w >> 2, tn = w & 3, m0 = 16 * tm, n0 = 16 * tn;
-  // the right-hand side first (four consecutive cells of one row per thread), then the
-  // Q operands in the order the products use them
-  const int ri = threadIdx.x >> 4, rj = 4 * (threadIdx.x & 15);
-  float rv[4];
-  if (n == EXN) {
-    if constexpr (sizeof(S) == 4) {
-      float4 t = *(const float4*)((const float*)rg + ri * EXN + rj);
-      rv[0] = t.x, rv[1] = t.y, rv[2] = t.z, rv[3] = t.w;
-    } else {
-      uint2 t = *(const uint2*)((const uint16_t*)rg + ri * EXN + rj);
-      rv[0] = __uint_as_float(t.x << 16), rv[1] = __uint_as_float(t.x & 0xffff0000u);
-      rv[2] = __uint_as_float(t.y << 16), rv[3] = __uint_as_float(t.y & 0xffff0000u);
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < 4; u++) rv[u] = (ri < n && rj + u < n) ? ld(rg[ri * n + rj + u]) : 0.f;
-  }
-  auto q8 = [&](int part, int T, int kk) { return __builtin_bit_cast(ex_bf16x8, tab[part * 512 + (T * 64 + lane) * 2 + kk]); };
-  ex_bf16x8 a1h[2], a1l[2], b2h[2], b2l[2], a3h[2], a3l[2], b4h[2], b4l[2];
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) a1h[kk] = q8(0, tm, kk), a1l[kk] = q8(1, tm, kk);
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) b2h[kk] = q8(0, tn, kk), b2l[kk] = q8(1, tn, kk);
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) a3h[kk] = q8(2, tm, kk), a3l[kk] = q8(3, tm, kk);
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) b4h[kk] = q8(2, tn, kk), b4l[kk] = q8(3, tn, kk);
-  const float4 wv = ((const float4*)(tab + 2048))[w * 64 + lane];
-  // r transposed into A: RT[j][i] = r[i][j]
-#pragma unroll
-  for (int u = 0; u < 4; u++) ex_put(sah, sal, (rj + u) * EXB + ri, rv[u]);
-  __syncthreads();
-  // C1 = Q^T r (B operand RT rows), row-major into B
-  ex_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) {
-    const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
-    acc = ex_mma3(acc, a1h[kk], a1l[kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
-  __syncthreads();
-  // C2 = (C1 Q) * W (A operand C1 rows), transposed into A: C2T[n][m]
-  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) {
-    const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
-    acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b2h[kk], b2l[kk]);
-  }
-  {
-    const float c[4] = {acc[0] * wv.x, acc[1] * wv.y, acc[2] * wv.z, acc[3] * wv.w};
-    ex_bf16x4 h4, l4;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const __bf16 hx = (__bf16)c[i];
-      h4[i] = hx;
-      l4[i] = (__bf16)(c[i] - (float)hx);
-    }
-    const int o = (n0 + li) * EXB + m0 + 4 * lg;
-    *(ex_bf16x4*)(sah + o) = h4;
-    *(ex_bf16x4*)(sal + o) = l4;
-  }
-  __syncthreads();
-  // C3 = Q C2 (B operand C2T rows), row-major into B
-  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) {
-    const int o = (n0 + li) * EXB + 32 * kk + 8 * lg;
-    acc = ex_mma3(acc, a3h[kk], a3l[kk], *(const ex_bf16x8*)(sah + o), *(const ex_bf16x8*)(sal + o));
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) ex_put(sbh, sbl, (m0 + 4 * lg + i) * EXB + n0 + li, acc[i]);
-  __syncthreads();
-  // x = C3 Q^T (A operand C3 rows)
-  acc = ex_f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kk = 0; kk < 2; kk++) {
-    const int o = (m0 + li) * EXB + 32 * kk + 8 * lg;
-    acc = ex_mma3(acc, *(const ex_bf16x8*)(sbh + o), *(const ex_bf16x8*)(sbl + o), b4h[kk], b4l[kk]);
-  }
-  const int col = n0 + li;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int row = m0 + 4 * lg + i;
-    if (row < n && col < n) xg[row * n + col] = acc[i];
-  }
+  ex_solve<EXT, 1, S>(rg, n, tab, sah, sal, sbh, sbl, 0xfu, 0xfu,
+                   [&](int row, int col, float x) { xg[row * n + col] = x; });
 }
 
 }  // namespace fsk

@@ -144,6 +144,7 @@ struct fs_solver_t {
   bool bf16 = false;         // multigrid correction and right-hand sides stored as BF16
   float* pinv = nullptr;     // coarsest-level pseudo-inverse (null: run the sweeps)
   float* ex_q = nullptr;     // exact coarse solve: the operand table (fsk::EX_TAB 16-byte units)
+  bool fold_exact = true;    // the exact solve inside the up launch above it (fold_coarse())
   int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
   double* rpart = nullptr;
@@ -856,18 +857,32 @@ int launch_up_s(fs_solver_t* s, int l) {
                 as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
                 (float)(F.dx * F.dx), tx, nb, s->gate_v);
 }
-// Levels l + 1 and l up in one launch (k_mgt_up2), tiles TS of level l.
-template <int POST, class TS, typename S>
-int launch_up2_s(fs_solver_t* s, int l) {
+// Levels l + 1 and l up in one launch (k_mgt_up2), tiles TS of level l. FOLD: level
+// l + 2 is the 64 x 64 exactly solved level, solved inside the launch (fold_coarse).
+template <int POST, class TS, typename S, bool FOLD>
+int launch_up2_f(fs_solver_t* s, int l) {
   Level& F = s->lv[l];
   Level& C = s->lv[l + 1];
+  Level& C2 = s->lv[l + 2];
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
-  // level l + 1: x, rhs, the level-(l + 2) correction read; level l: x, rhs read, correction written
-  const double bytes = (2.0 * sizeof(S) + 1.0) * C.n * C.n + (2.0 * sizeof(S) + 4.0) * F.n * F.n;
-  return launch(s, PK_UP, bytes, fsk::k_mgt_up2<POST, TS, S, float>, g, dim3(TS::NT), as<S>((const float*)C.x),
-                as<S>((const float*)C.rhs), (const float*)s->lv[l + 2].xb, as<S>((const float*)F.x), F.xb,
-                as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v);
+  // level l + 1: x, rhs, the level-(l + 2) correction (or its rhs and the solve's table)
+  // read; level l: x, rhs read, correction written
+  const double bytes = (2.0 * sizeof(S) + (FOLD ? 0.0 : 1.0)) * C.n * C.n + (2.0 * sizeof(S) + 4.0) * F.n * F.n +
+                       (FOLD ? sizeof(S) * C2.n * C2.n + 16.0 * fsk::EX_TAB : 0.0);
+  return launch(s, PK_UP, bytes, fsk::k_mgt_up2<POST, TS, S, float, FOLD>, g, dim3(TS::NT),
+                as<S>((const float*)C.x), as<S>((const float*)C.rhs), (const float*)C2.xb, as<S>((const float*)F.x),
+                F.xb, as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v,
+                as<S>((const float*)C2.rhs), (const uint4*)s->ex_q);
+}
+// The exact 64 x 64 solve of the coarse level runs inside the up launch above it (every
+// workgroup solves it for its own footprint) instead of as a launch of its own.
+bool fold_coarse(const fs_solver_t* s, int l) {
+  return s->fold_exact && s->ex_q && l + 2 == s->coarse_start && s->lv[l + 2].n == fsk::EXN;
+}
+template <int POST, class TS, typename S>
+int launch_up2_s(fs_solver_t* s, int l) {
+  return fold_coarse(s, l) ? launch_up2_f<POST, TS, S, true>(s, l) : launch_up2_f<POST, TS, S, false>(s, l);
 }
 // 8 x 8 tiles for the pair above the coarse solve (levels 256 and 128 over a 64
 // coarse level: more workgroups in flight; 16 x 16 took 0.4 us more per launch),
@@ -880,6 +895,9 @@ int launch_up2(fs_solver_t* s, int l) {
                    : launch_up2_s<POST, fsk::Tile64sq, float>(s, l);
   if (l == 0)
     return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
+  // with the exact solve folded in, 16 x 16 tiles: a quarter of the workgroups repeat it
+  if (fold_coarse(s, l))
+    return s->bf16 ? launch_up2_s<POST, fsk::Tile16w, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile16w, float>(s, l);
   return s->bf16 ? launch_up2_s<POST, fsk::Tile8x256, fsk::bf16>(s, l)
                  : launch_up2_s<POST, fsk::Tile8x256, float>(s, l);
 }
@@ -949,7 +967,9 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
   for (int l = 1; l < pair; l++)
     if ((rc = DISPATCH_SW(pre, launch_down, s, l))) return rc;
   if (pair < cs && (rc = DISPATCH_SW(pre, launch_down2, s, pair))) return rc;
-  if ((rc = launch_coarse_ml(s, cs))) return rc;
+  // the exact coarse solve runs in the first up launch when that launch is a k_mgt_up2
+  const int up2_first = pair < cs && s->lv[pair].n % 16 == 0 ? pair : (cs == 2 && s->lv[0].n % 32 == 0 ? 0 : -1);
+  if (!(up2_first >= 0 && fold_coarse(s, up2_first)) && (rc = launch_coarse_ml(s, cs))) return rc;
   // and come back up in one launch (level l tiled by 16 x 16 exactly); levels 1 and 0,
   // when neither is in that pair, also go up in one launch (32 x 32 tiles of level 0)
   int l = cs - 1;
@@ -1820,8 +1840,10 @@ bool up2_level0(const fs_solver_t* s) {
 int autotune_tiles(fs_solver_t* s) {
   const int cs = s->coarse_start;
   if (!s->fused || cs == 0) return FS_OK;
+  // the level-0 down-sweep is timed as the iterations after the first launch it, reading
+  // the pressure and the previous correction; its block maxima go to row 1 of these partials
   double* tmp = nullptr;
-  int rc = dalloc(&tmp, (size_t)tile_count(TILE8, s->lv[0].n));
+  int rc = dalloc(&tmp, (size_t)s->rp_cols + tile_count(TILE8, s->lv[0].n));
   if (rc) return rc;
   double* saved = s->rpart;
   s->rpart = tmp;
@@ -1837,7 +1859,7 @@ int autotune_tiles(fs_solver_t* s) {
       auto run = [&]() -> int {
         if (pair0) return DISPATCH_SW(post, launch_up2, s, 0);
         if (dir == 1) return DISPATCH_SW(post, launch_up, s, l);
-        if (l == 0) return DISPATCH_SW(pre, launch_down0, s, 0, 0, true, false);
+        if (l == 0) return DISPATCH_SW(pre, launch_down0, s, 1, 0, false, false);
         return DISPATCH_SW(pre, launch_down, s, l);
       };
       // three rounds over the shapes (order rotated each round), ten launches each
