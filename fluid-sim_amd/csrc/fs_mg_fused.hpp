@@ -1108,6 +1108,175 @@ k_mgt_up2(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __re
   }
 }
 
+// A window of whole quads, RW / 4 per row (RW and the origin ox multiples of 4 and n a
+// multiple of 4, so a quad never straddles the periodic seam): 16-byte (FP32) or 8-byte
+// (BF16) loads issued into registers, committed to LDS as FP32, the values Window holds.
+template <int RW, int RH, int NT, typename T>
+struct QWindow {
+  static constexpr int RQ = RW / 4, CNT = RQ * RH, K = (CNT + NT - 1) / NT;
+  float f[K][4];
+  __device__ __forceinline__ void issue(const T* __restrict__ src, int n, int ox, int oy) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) {
+        const int ly = q / RQ, qx = q - ly * RQ;
+        const T* a = src + wrapn(oy + ly, n) * n + wrapn(ox + 4 * qx, n);
+        if constexpr (sizeof(T) == 4) {
+          const float4 w = *(const float4*)a;
+          f[k][0] = w.x, f[k][1] = w.y, f[k][2] = w.z, f[k][3] = w.w;
+        } else {
+          const uint2 w = *(const uint2*)a;
+          f[k][0] = __uint_as_float(w.x << 16), f[k][1] = __uint_as_float(w.x & 0xffff0000u);
+          f[k][2] = __uint_as_float(w.y << 16), f[k][3] = __uint_as_float(w.y & 0xffff0000u);
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(float* __restrict__ dst) const {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) *(float4*)(dst + 4 * q) = make_float4(f[k][0], f[k][1], f[k][2], f[k][3]);
+    }
+  }
+};
+
+// x + the bilinear prolongation (PROLONG) of a coarse LDS array cw (row stride CWS,
+// origin (cox, coy) in coarse cells) for the four cells (i .. i + 3, j), i even: they
+// interpolate from the coarse columns i/2 - 1 .. i/2 + 2 with fx = 3/4, 1/4, 3/4, 1/4.
+// R rounds a coarse value as stored (the level above keeps its correction in SO). Each
+// cell's arithmetic is that of the one-cell loop, in the same order.
+template <int CWS, bool R, typename SO>
+__device__ __forceinline__ float4 prolong_quad(const float* __restrict__ cw, int cox, int coy, int i, int j,
+                                               float4 x) {
+  const int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+  const float fy = (j & 1) ? 0.25f : 0.75f;
+  const float* r0 = cw + (j0 - coy) * CWS + (i >> 1) - 1 - cox;
+  const float* r1 = r0 + CWS;
+  auto g = [&](float v) { return R ? rnd<SO>(v) : v; };
+  const float a0 = g(r0[0]), a1 = g(r0[1]), a2 = g(r0[2]), a3 = g(r0[3]);
+  const float b0 = g(r1[0]), b1 = g(r1[1]), b2 = g(r1[2]), b3 = g(r1[3]);
+  auto bl = [&](float a, float b, float c, float d, float fx) {
+    const float top = a * (1.f - fx) + b * fx, bot = c * (1.f - fx) + d * fx;
+    return top * (1.f - fy) + bot * fy;
+  };
+  return make_float4(x.x + bl(a0, a1, b0, b1, 0.75f), x.y + bl(a1, a2, b1, b2, 0.25f),
+                     x.z + bl(a1, a2, b1, b2, 0.75f), x.w + bl(a2, a3, b2, b3, 0.25f));
+}
+
+// Post-smoothing: 2 SW half-sweeps (red, black, ...) on the whole region with PairMap,
+// each followed by a barrier.
+template <int SW, int RW, int RH, int NT>
+__device__ __forceinline__ void fm_post_sweeps(float* __restrict__ sx, const float* __restrict__ sr, int par0,
+                                               float h2) {
+  const PairMap<RW, RH, NT> m;
+#pragma unroll
+  for (int s = 0; s < 2 * SW; s++) {
+    fm_half_sweep<RW, RH, NT>(sx, sr, m, s & 1, par0, h2);
+    __syncthreads();
+  }
+}
+
+// k_mgt_up2 with quad accesses (H = 2 SW a multiple of 4; the host launches k_mgt_up2
+// only on grids of whole tiles whose sizes are multiples of 16): x and rhs of both levels
+// staged through QWindow, the prolongation four cells at a time from the eight coarse
+// nodes they share, PairMap passes, x leaving four cells per store. Same bits.
+template <int SW, class TS, typename S, typename SO, bool FOLD = false>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_up2q(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __restrict__ xc2,
+           const S* __restrict__ x0in, SO* __restrict__ x0out, const S* __restrict__ rhs0, int n0, float h20,
+           float h21, int tiles_x, int nblocks, Gate gate, const S* __restrict__ rhs2, const uint4* __restrict__ tab) {
+  if (gate_closed(gate)) return;
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW0 = TX + 2 * H, RH0 = TY + 2 * H;         // level-l region
+  constexpr int CW0 = RW0 / 2 + 3, CH0 = RH0 / 2 + 3;       // its level-(l+1) footprint
+  constexpr int D = H / 2 + 1;                              // footprint origin = tile origin / 2 - D
+  constexpr int E = (D + H + 1) & ~1;                       // level-(l+1) region origin = tile origin / 2 - E (even)
+  constexpr int RW1 = (E - D + CW0 + H + 1) & ~1, RH1 = (E - D + CH0 + H + 1) & ~1;
+  constexpr int CW1 = RW1 / 2 + 3, CH1 = RH1 / 2 + 3;       // level-(l+2) footprint of the level-(l+1) region
+  static_assert(TX % 8 == 0 && TY % 8 == 0 && H % 4 == 0 && E % 4 == 0 && RW1 % 4 == 0 && RW0 % 4 == 0,
+                "quad windows");
+  __shared__ __align__(16) float sx1[RH1 * RW1];
+  __shared__ __align__(16) float sr1[RH1 * RW1];
+  __shared__ float sc1[CH1 * CW1];
+  __shared__ __align__(16) float sx0[RH0 * RW0];
+  __shared__ __align__(16) float sr0[RH0 * RW0];
+  constexpr int EXM = FOLD ? EXN * EXB : 1;
+  __shared__ __align__(16) __bf16 sah[EXM], sal[EXM], sbh[EXM], sbl[EXM];
+  const int n1 = n0 >> 1, n2 = n1 >> 1;
+  const int t = xcd_tile(nblocks);
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox0 = tx0 - H, oy0 = ty0 - H;
+  const int ox1 = (tx0 >> 1) - E, oy1 = (ty0 >> 1) - E;
+  const int cox1 = (ox1 >> 1) - 1, coy1 = (oy1 >> 1) - 1;
+  // every window of both levels in flight at once
+  QWindow<RW1, RH1, NT, S> wx1, wr1;
+  QWindow<RW0, RH0, NT, S> wx0, wr0;
+  Window<CW1, CH1, NT, SO> wc1;
+  wx1.issue(x1in, n1, ox1, oy1);
+  if constexpr (!FOLD) wc1.issue(xc2, n2, cox1, coy1);
+  wr1.issue(rhs1, n1, ox1, oy1);
+  wx0.issue(x0in, n0, ox0, oy0);
+  wr0.issue(rhs0, n0, ox0, oy0);
+  if constexpr (FOLD) {
+    // the footprint window of level l + 2 (n2 = 64): its row and column blocks, then
+    // the solve with the window's values put in sc1
+    const int wr0_ = coy1 & (EXN - 1), wc0_ = cox1 & (EXN - 1);
+    unsigned rbm = 0, cbm = 0;
+    for (int r = 0; r < CH1; r++) rbm |= 1u << (((wr0_ + r) & (EXN - 1)) >> 4);
+    for (int c = 0; c < CW1; c++) cbm |= 1u << (((wc0_ + c) & (EXN - 1)) >> 4);
+    // a window of CH1 rows touches at most CH1 / 16 + 2 of the four row blocks
+    constexpr int NS = NT == 1024 ? 1 : ((CH1 + 15) / 16 + 1 < 4 ? (CH1 + 15) / 16 + 1 : 4);
+    ex_solve<NT, NS, S>(rhs2, EXN, tab, sah, sal, sbh, sbl, rbm, cbm, [&](int row, int col, float x) {
+      const int lr = (row - wr0_) & (EXN - 1), lc = (col - wc0_) & (EXN - 1);
+      if (lr < CH1 && lc < CW1) sc1[lr * CW1 + lc] = x;
+    });
+  }
+  if constexpr (!FOLD) wc1.commit(sc1);
+  wr1.commit(sr1);
+  wr0.commit(sr0);
+  __syncthreads();
+  // ---- level l + 1: x + prolong(correction of level l + 2), then the post-sweeps
+  {
+    using W = QWindow<RW1, RH1, NT, S>;
+#pragma unroll
+    for (int k = 0; k < W::K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < W::CNT) {
+        const int ly = q / W::RQ, lx0 = 4 * (q - ly * W::RQ);
+        const float4 xv = make_float4(wx1.f[k][0], wx1.f[k][1], wx1.f[k][2], wx1.f[k][3]);
+        *(float4*)(sx1 + ly * RW1 + lx0) = prolong_quad<CW1, false, SO>(sc1, cox1, coy1, ox1 + lx0, oy1 + ly, xv);
+      }
+    }
+    __syncthreads();
+    fm_post_sweeps<SW, RW1, RH1, NT>(sx1, sr1, (ox1 + oy1) & 1, h21);
+  }
+  // ---- level l: x + prolong(the level-(l+1) correction, from LDS), then the post-sweeps
+  {
+    using W = QWindow<RW0, RH0, NT, S>;
+#pragma unroll
+    for (int k = 0; k < W::K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < W::CNT) {
+        const int ly = q / W::RQ, lx0 = 4 * (q - ly * W::RQ);
+        const float4 xv = make_float4(wx0.f[k][0], wx0.f[k][1], wx0.f[k][2], wx0.f[k][3]);
+        // the level-(l+1) nodes sit at sx1 column i0 - ox1, inside its valid interior
+        *(float4*)(sx0 + ly * RW0 + lx0) = prolong_quad<RW1, true, SO>(sx1, ox1, oy1, ox0 + lx0, oy0 + ly, xv);
+      }
+    }
+    __syncthreads();
+    fm_post_sweeps<SW, RW0, RH0, NT>(sx0, sr0, (ox0 + oy0) & 1, h20);
+    constexpr int XQ = TX / 4;
+    for (int q = threadIdx.x; q < XQ * TY; q += NT) {
+      const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+      const float4 xv = *(const float4*)(sx0 + (iy + H) * RW0 + H + m4);
+      store4<SO>(x0out + (ty0 + iy) * n0 + tx0 + m4, xv.x, xv.y, xv.z, xv.w);
+    }
+  }
+}
+
 // ----------------------------------------------------------- coarse levels
 
 // All levels from n0 (<= COARSE_N) down to the coarsest, in one workgroup:

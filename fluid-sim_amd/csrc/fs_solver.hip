@@ -870,10 +870,16 @@ int launch_up2_f(fs_solver_t* s, int l) {
   // read; level l: x, rhs read, correction written
   const double bytes = (2.0 * sizeof(S) + (FOLD ? 0.0 : 1.0)) * C.n * C.n + (2.0 * sizeof(S) + 4.0) * F.n * F.n +
                        (FOLD ? sizeof(S) * C2.n * C2.n + 16.0 * fsk::EX_TAB : 0.0);
-  return launch(s, PK_UP, bytes, fsk::k_mgt_up2<POST, TS, S, float, FOLD>, g, dim3(TS::NT),
-                as<S>((const float*)C.x), as<S>((const float*)C.rhs), (const float*)C2.xb, as<S>((const float*)F.x),
-                F.xb, as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v,
-                as<S>((const float*)C2.rhs), (const uint4*)s->ex_q);
+  // quad accesses when the halo 2 POST is a multiple of 4 (levels l and l + 1 are whole
+  // multiples of 16 wherever k_mgt_up2 runs)
+  auto kern = [] {
+    if constexpr ((2 * POST) % 4 == 0) return fsk::k_mgt_up2q<POST, TS, S, float, FOLD>;
+    else return fsk::k_mgt_up2<POST, TS, S, float, FOLD>;
+  }();
+  return launch(s, PK_UP, bytes, kern, g, dim3(TS::NT), as<S>((const float*)C.x), as<S>((const float*)C.rhs),
+                (const float*)C2.xb, as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), F.n,
+                (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v, as<S>((const float*)C2.rhs),
+                (const uint4*)s->ex_q);
 }
 // The exact 64 x 64 solve of the coarse level runs inside the up launch above it (every
 // workgroup solves it for its own footprint) instead of as a launch of its own.
