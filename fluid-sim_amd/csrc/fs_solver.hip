@@ -144,6 +144,7 @@ struct fs_solver_t {
   bool bf16 = false;         // multigrid correction and right-hand sides stored as BF16
   float* pinv = nullptr;     // coarsest-level pseudo-inverse (null: run the sweeps)
   float* ex_q = nullptr;     // exact coarse solve: the operand table (fsk::EX_TAB 16-byte units)
+  bool spare = false;        // launches being enqueued belong to a gated body's spare V-cycle
   bool fold_exact = true;    // the exact solve inside the up launch above it (fold_coarse())
   int coarse_start = 0;      // first level solved by the one-workgroup coarse kernel
   // residual partials: rpart[it * rp_cols + block] = block max |r| of iteration it
@@ -823,7 +824,11 @@ int launch_down_t(fs_solver_t* s, int l) {
 }
 template <int PRE>
 int launch_down(fs_solver_t* s, int l) {
-  return DISPATCH_TILE(s->lv[l].tile[0], launch_down_t, PRE, s, l);
+  // a spare V-cycle, gated off unless the prediction fell short, takes the shape with the
+  // fewest workgroups (a gated-off launch costs its dispatch, which grows with them)
+  const int n = s->lv[l].n;
+  const int k = s->spare && n % 64 == 0 ? (int)TILE64SQ : s->lv[l].tile[0];
+  return DISPATCH_TILE(k, launch_down_t, PRE, s, l);
 }
 
 // Levels l and l + 1 down in one launch (k_mgt_down2), tiles of 8 x 8 cells of level l + 1.
@@ -952,8 +957,11 @@ int enqueue_residual(fs_solver_t* s, int it, int pin, bool zero_p, const float* 
   return s->bf16 ? enqueue_residual_s<fsk::bf16>(s, it, pin, zero_p, e) : enqueue_residual_s<float>(s, it, pin, zero_p, e);
 }
 
-// Residual `it`, then (unless residual_only) the V-cycle on its correction.
-int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only) {
+// Residual `it`, then (unless residual_only) the V-cycle on its correction. split: the
+// residual pass runs residual-only and the level-0 pre-smoothing is a k_mgt_down launch
+// on the stored residual under the V-cycle's gate (the same bits), so that a pass that
+// meets the gate, as a gated body's predicted last one usually does, skips its smoothing.
+int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residual_only, bool split = false) {
   int rc;
   if (!s->fused) {
     if ((rc = enqueue_residual(s, it, pin, zero_p, it > 0 ? s->lv[0].x : nullptr))) return rc;
@@ -965,8 +973,9 @@ int enqueue_iteration(fs_solver_t* s, int it, int pin, bool zero_p, bool residua
     return residual_only ? FS_OK : launch_coarse_ml(s, 0);
   }
   int pre = s->cfg.mg_pre, post = s->cfg.mg_post;
-  if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only))) return rc;
+  if ((rc = DISPATCH_SW(pre, launch_down0, s, it, pin, zero_p, residual_only || split))) return rc;
   if (residual_only) return FS_OK;
+  if (split && (rc = DISPATCH_SW(pre, launch_down, s, 0))) return rc;
   // the last two grid-wide levels above the coarse solve go down in one launch
   // (level l + 1 tiled by 8 x 8 exactly)
   const int pair = cs - 2 >= 1 && s->lv[cs - 1].n % 8 == 0 ? cs - 2 : cs;
@@ -1494,10 +1503,13 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
   if ((rc = enqueue_guess(s, a0, zero_p))) return rc;
   if (key.hist != (hist0 ? 1 : 0)) return fail(FS_E_DEVICE, "step body key out of date");
   const double tol = s->cfg.tolerance;
+  // iteration k - 1's residual pass is the predicted last one, and its V-cycle the spare
   for (int it = 0; it <= key.k && !rc; it++) {
     s->gate_r = fsk::Gate{s->gate_flag, tol, it > 0 ? it - 1 : -1, it};
     s->gate_v = fsk::Gate{s->gate_flag, tol, it, -1};
-    rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, it == key.k);
+    s->spare = it == key.k - 1;
+    rc = enqueue_iteration(s, it, (a0 + it) & 1, it == 0 && zero_p, it == key.k, it == key.k - 1);
+    s->spare = false;
   }
   s->gate_r = s->gate_v = fsk::gate_open();
   if (rc) return rc;
