@@ -908,7 +908,8 @@ int launch_up2(fs_solver_t* s, int l) {
     return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
   // with the exact solve folded in, 16 x 16 tiles: a quarter of the workgroups repeat it
   if (fold_coarse(s, l))
-    return s->bf16 ? launch_up2_s<POST, fsk::Tile16w, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile16w, float>(s, l);
+    return s->bf16 ? launch_up2_s<POST, fsk::Tile16x1024, fsk::bf16>(s, l)
+                   : launch_up2_s<POST, fsk::Tile16x1024, float>(s, l);
   return s->bf16 ? launch_up2_s<POST, fsk::Tile8x256, fsk::bf16>(s, l)
                  : launch_up2_s<POST, fsk::Tile8x256, float>(s, l);
 }
