@@ -44,8 +44,9 @@ using Tile32w = TileShape<32, 32, 1024>;
 using Tile16w = TileShape<16, 16, 256>;
 using Tile16x1024 = TileShape<16, 16, 1024>;  // the pair above an exactly solved 64 x 64 level, which
                                                // every workgroup solves (one output block per wavefront)
-using Tile8x256 = TileShape<8, 8, 256>;  // k_mgt_down2 and k_mgt_up2 above the coarse solve: the
-                                         // level-l region is ~25 times the tile
+using Tile8x1024 = TileShape<8, 8, 1024>;  // k_mgt_down2 and k_mgt_up2 above the coarse solve: the
+                                           // level-l region is ~25 times the tile, and these launches
+                                           // are latency-bound: short passes over many threads
 
 // Blocks are issued round-robin over the 8 XCDs; give each XCD a contiguous band of
 // tiles so neighbouring tiles (which share halos) share an L2.

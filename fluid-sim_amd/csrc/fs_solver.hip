@@ -831,10 +831,11 @@ int launch_down(fs_solver_t* s, int l) {
   return DISPATCH_TILE(k, launch_down_t, PRE, s, l);
 }
 
-// Levels l and l + 1 down in one launch (k_mgt_down2), tiles of 8 x 8 cells of level l + 1.
+// Levels l and l + 1 down in one launch (k_mgt_down2), tiles of 8 x 8 cells of level l + 1
+// (1024 threads: the launch is latency-bound, and more threads make its passes shorter).
 template <int PRE, typename S>
 int launch_down2_s(fs_solver_t* s, int l) {
-  using TS = fsk::Tile8x256;
+  using TS = fsk::Tile8x1024;
   Level& F = s->lv[l];
   Level& C = s->lv[l + 1];
   int tx, nb;
@@ -895,9 +896,10 @@ template <int POST, class TS, typename S>
 int launch_up2_s(fs_solver_t* s, int l) {
   return fold_coarse(s, l) ? launch_up2_f<POST, TS, S, true>(s, l) : launch_up2_f<POST, TS, S, false>(s, l);
 }
-// 8 x 8 tiles for the pair above the coarse solve (levels 256 and 128 over a 64
-// coarse level: more workgroups in flight; 16 x 16 took 0.4 us more per launch),
-// 32 x 32 or (as tuned, on a grid of whole 64 x 64 tiles) 64 x 64 for levels 1 and 0
+// 8 x 8 tiles of 1024 threads for the pair above the coarse solve (levels 256 and 128
+// over a 64 coarse level: more workgroups in flight than 16 x 16, short passes), 16 x 16
+// of 1024 threads when that pair also solves the coarse level itself, 32 x 32 or (as
+// tuned, on a grid of whole 64 x 64 tiles) 64 x 64 for levels 1 and 0
 bool up2_64sq_ok(const fs_solver_t* s) { return s->lv[0].n % 64 == 0; }
 template <int POST>
 int launch_up2(fs_solver_t* s, int l) {
@@ -910,8 +912,8 @@ int launch_up2(fs_solver_t* s, int l) {
   if (fold_coarse(s, l))
     return s->bf16 ? launch_up2_s<POST, fsk::Tile16x1024, fsk::bf16>(s, l)
                    : launch_up2_s<POST, fsk::Tile16x1024, float>(s, l);
-  return s->bf16 ? launch_up2_s<POST, fsk::Tile8x256, fsk::bf16>(s, l)
-                 : launch_up2_s<POST, fsk::Tile8x256, float>(s, l);
+  return s->bf16 ? launch_up2_s<POST, fsk::Tile8x1024, fsk::bf16>(s, l)
+                 : launch_up2_s<POST, fsk::Tile8x1024, float>(s, l);
 }
 
 template <int POST, class TS>
