@@ -92,7 +92,7 @@ struct Level {
 // itself (not the gap before it), and carries its algorithmic byte count.
 enum {
   PK_ADVECT_SL = 0, PK_ADVECT_MC, PK_DIFFUSE, PK_DOWN0, PK_DOWN, PK_COARSE, PK_UP, PK_GRADIENT, PK_RESIDUAL,
-  PK_RB, PK_GUESS, PK_ADVECT_FUSED, PK_FORCING, PK_COUNT
+  PK_RB, PK_GUESS, PK_ADVECT_FUSED, PK_FORCING, PK_UP_COARSE, PK_COUNT
 };
 
 struct Profile {
@@ -882,7 +882,8 @@ int launch_up2_f(fs_solver_t* s, int l) {
     if constexpr ((2 * POST) % 4 == 0) return fsk::k_mgt_up2q<POST, TS, S, float, FOLD>;
     else return fsk::k_mgt_up2<POST, TS, S, float, FOLD>;
   }();
-  return launch(s, PK_UP, bytes, kern, g, dim3(TS::NT), as<S>((const float*)C.x), as<S>((const float*)C.rhs),
+  return launch(s, FOLD ? PK_UP_COARSE : PK_UP, bytes, kern, g, dim3(TS::NT), as<S>((const float*)C.x),
+                as<S>((const float*)C.rhs),
                 (const float*)C2.xb, as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), F.n,
                 (float)(F.dx * F.dx), (float)(C.dx * C.dx), tx, nb, s->gate_v, as<S>((const float*)C2.rhs),
                 (const uint4*)s->ex_q);
