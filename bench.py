@@ -43,7 +43,8 @@ KIND_TEXT = {
     "down0": "level-0 residual in FP64 + pre-smoothing + restriction (k_mgt_down0), 37-45 B/cell",
     "down": "level>0 pre-smoothing + restriction (k_mgt_down), 9 B/cell of the level",
     "coarse": "coarse levels in one workgroup (k_mg_coarse_ml), 8 B/cell of its top level",
-    "up": "prolongation + post-smoothing (k_mgt_up), 13 B/cell of the level",
+    "up": "prolongation + post-smoothing (k_mgt_up, k_mgt_up2: two levels), 13 B/cell of the level",
+    "up_coarse": "two levels up with the exact 64^2 coarse solve in every workgroup (k_mgt_up2, folded)",
     "gradient": "FP64 pressure-gradient subtraction (k_gradient64), 24 B/cell",
     "residual": "FP64 residual only (k_residual64), 20-32 B/cell",
     "rb": "red-black colour sweep (k_mg_rb), 6 B/cell",
@@ -55,7 +56,8 @@ KIND_TEXT = {
 # The kernels of each class as the profiler names them, for the HBM counter passes.
 KIND_KERNELS = {
     "advect_sl": r"k_advect_sl\b", "advect_mc": r"k_advect_mc\b", "diffuse": r"k_diffuse\b",
-    "down0": r"k_mgt_down0<", "down": r"k_mgt_down2?<", "coarse": r"k_mg_coarse_ml<", "up": r"k_mgt_up2?<",
+    "down0": r"k_mgt_down0q?<", "down": r"k_mgt_down2?<", "coarse": r"k_mg_coarse_(ml|exact)<",
+    "up": r"k_mgt_up(<|2q?<[^(]*false>)", "up_coarse": r"k_mgt_up2q?<[^(]*true>",
     "gradient": r"k_gradient64\b", "residual": r"k_residual64<", "rb": r"k_(mg_rb|rb_colour)<",
     "advect_fused": r"k_advect_mc_diffuse\b", "forcing": r"k_forcing_(tables|sep)\b",
 }

@@ -27,7 +27,7 @@ GUESS = {"zero": 0, "previous": 1, "learned": 2, "learned-coarse": 3, "learned-q
 FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4, "dye": 5}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
-                 "advect_fused": 11, "forcing": 12}
+                 "advect_fused": 11, "forcing": 12, "up_coarse": 13}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6,
           "forcing": 7, "advect-dye": 8}
 

@@ -354,7 +354,8 @@ enum {
   FS_PK_GUESS = 10,  /* 3x3 layers of the learned guess */
   FS_PK_ADVECT_FUSED = 11,  /* MacCormack + diffusion in one launch */
   FS_PK_FORCING = 12,       /* band forcing + drag */
-  FS_PK_COUNT = 13
+  FS_PK_UP_COARSE = 13,     /* two levels up with the exact 64^2 coarse solve inside the launch */
+  FS_PK_COUNT = 14
 };
 /* Time every launch of the classes in `mask` (bit k = class k; 0 = off) with HIP
  * events recorded at the dispatch's own start and end on the solver's stream, and
