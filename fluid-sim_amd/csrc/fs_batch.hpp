@@ -70,6 +70,11 @@ __device__ __forceinline__ float cv_at(const float* __restrict__ v, int n, int i
 
 // a[idx] with a 32-bit byte offset (idx < 2^30), so a block-uniform base stays in
 // scalar registers and the load takes a scalar base plus a 32-bit lane offset
+// x / k for the neighbour count k of a Neumann cell (2, 3 or 4): an interior cell's
+// x / 4 is x * 0.25 exactly (a power-of-two scaling rounds once, either way), which
+// saves the division sequence on all but the boundary cells; same bits.
+__device__ __forceinline__ float nm_div(float x, int k) { return k == 4 ? x * 0.25f : x / (float)k; }
+
 __device__ __forceinline__ float ld_u32(const float* __restrict__ a, int idx) {
   return *(const float*)((const char*)a + (unsigned)((unsigned)idx * 4u));
 }
@@ -438,7 +443,7 @@ k_nm_rb(float* __restrict__ x, const float* __restrict__ rhs, int lg, long count
   if (i < s - 1) { sum += x_zero ? 0.f : xg[c + 1]; k++; }
   if (j > 0) { sum += x_zero ? 0.f : xg[c - s]; k++; }
   if (j < s - 1) { sum += x_zero ? 0.f : xg[c + s]; k++; }
-  xg[c] = (sum - h2 * rhs[(g << (2 * lg)) + c]) / (float)k;
+  xg[c] = nm_div(sum - h2 * rhs[(g << (2 * lg)) + c], k);
 }
 
 // The black half of a from-zero first sweep reads red values only; when x_zero is set
@@ -567,7 +572,7 @@ __device__ __forceinline__ void nm_half_sweep(float* __restrict__ sx, const floa
         kk[k] = c;
         int nk = 4 - (gi == 0) - (gi == s - 1) - (gj == 0) - (gj == s - 1);
         float sum = sx[c - 1] + sx[c + 1] + sx[c - RW] + sx[c + RW];
-        acc[k] = (sum - h2 * sr[c]) / (float)nk;
+        acc[k] = nm_div(sum - h2 * sr[c], nk);
       }
     }
   }
@@ -613,7 +618,7 @@ k_nmt_down(float* __restrict__ x, const float* __restrict__ rhs, int s, int tile
     bool in = gi >= 0 && gi < s && gj >= 0 && gj < s;
     bool red = ((lx + ly + par0) & 1) == 0;
     int nk = 4 - (gi == 0) - (gi == s - 1) - (gj == 0) - (gj == s - 1);
-    sx[q] = (in && red) ? (0.f - h2 * sr[q]) / (float)nk : 0.f;
+    sx[q] = (in && red) ? nm_div(0.f - h2 * sr[q], nk) : 0.f;
   }
   __syncthreads();
   NmSweeps<RW, RH, 1, 2 * SW, 0, NT>::run(sx, sr, par0, h2, ox, oy, s);
@@ -751,7 +756,7 @@ k_nm_coarse(float* __restrict__ xall, const float* __restrict__ rall, const long
       if (i < s - 1) { sum += zero ? 0.f : x[c + 1]; kk++; }
       if (j > 0) { sum += zero ? 0.f : x[c - s]; kk++; }
       if (j < s - 1) { sum += zero ? 0.f : x[c + s]; kk++; }
-      x[c] = (sum - h2 * r[c]) / (float)kk;
+      x[c] = nm_div(sum - h2 * r[c], kk);
     }
     __syncthreads();
   };
