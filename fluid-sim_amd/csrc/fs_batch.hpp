@@ -711,6 +711,230 @@ k_nmt_up(const float* __restrict__ xin, float* __restrict__ xout, const float* _
   }
 }
 
+// ---- the level kernels with quad accesses and fixed thread maps (SW even). The
+// region origin is a multiple of 4 and so is s, so a quad of four cells is wholly inside
+// or wholly outside the box; the passes use PairMap over the whole region (cells outside
+// the box are never updated and hold 0, as in the kernels above), every in-box cell
+// seeing the same operands in the same order: the same bits.
+
+// A window of whole quads of grid g (s x s), zero outside the box.
+template <int RW, int RH, int NT>
+struct NmQWindow {
+  static constexpr int RQ = RW / 4, CNT = RQ * RH, K = (CNT + NT - 1) / NT;
+  float f[K][4];
+  __device__ __forceinline__ void issue(const float* __restrict__ src, int s, int ox, int oy) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      f[k][0] = f[k][1] = f[k][2] = f[k][3] = 0.f;
+      if (q < CNT) {
+        const int ly = q / RQ, gx = ox + 4 * (q - ly * RQ), gy = oy + ly;
+        if (gx >= 0 && gx < s && gy >= 0 && gy < s) {
+          const float4 w = *(const float4*)(src + gy * s + gx);
+          f[k][0] = w.x, f[k][1] = w.y, f[k][2] = w.z, f[k][3] = w.w;
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(float* __restrict__ dst) const {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < CNT) *(float4*)(dst + 4 * q) = make_float4(f[k][0], f[k][1], f[k][2], f[k][3]);
+    }
+  }
+};
+
+__device__ __forceinline__ int nm_count(int gi, int gj, int s) {
+  return 4 - (gi == 0) - (gi == s - 1) - (gj == 0) - (gj == s - 1);
+}
+
+// one colour pass of nm_half_sweep on every in-box interior cell of the region
+template <int RW, int RH, int NT>
+__device__ __forceinline__ void nm_fm_half_sweep(float* __restrict__ sx, const float* __restrict__ sr,
+                                                 const PairMap<RW, RH, NT>& m, int colour, int par0, float h2, int ox,
+                                                 int oy, int s) {
+  using M = PairMap<RW, RH, NT>;
+  float acc[M::ROWS];
+  int cc[M::ROWS];
+  bool act[M::ROWS];
+#pragma unroll
+  for (int k = 0; k < M::ROWS; k++) {
+    const int ly = m.r + k * M::RG;
+    const int off = (colour + par0 + ly) & 1;
+    const int lx = 2 * m.p + off;
+    const int gi = ox + lx, gj = oy + ly;
+    act[k] = m.row_ok(k) && ly >= 1 && ly <= RH - 2 && lx >= 1 && lx <= RW - 2 && gi >= 0 && gi < s && gj >= 0 &&
+             gj < s;
+    const int c = m.base + k * M::RG * RW + off;
+    cc[k] = c;
+    if (act[k]) {
+      const float sum = sx[c - 1] + sx[c + 1] + sx[c - RW] + sx[c + RW];
+      acc[k] = nm_div(sum - h2 * sr[c], nm_count(gi, gj, s));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < M::ROWS; k++)
+    if (act[k]) sx[cc[k]] = acc[k];
+}
+
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
+k_nmt_downq(float* __restrict__ x, const float* __restrict__ rhs, int s, int tiles_x, int tpg, float h2, float ih2,
+            float* __restrict__ crhs) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT, H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad windows");
+  __shared__ __align__(16) float sx[RH * RW];
+  __shared__ __align__(16) float sr[RH * RW];
+  const int bid = xcd_tile((int)gridDim.x), g = bid / tpg, t = bid - g * tpg;
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const size_t gb = (size_t)g * s * s;
+  {
+    NmQWindow<RW, RH, NT> wr;
+    wr.issue(rhs + gb, s, ox, oy);
+    wr.commit(sr);
+  }
+  __syncthreads();
+  const int par0 = (ox + oy) & 1;
+  const PairMap<RW, RH, NT> m;
+  // red from zero on the whole region (black and outside cells 0)
+  {
+    using M = PairMap<RW, RH, NT>;
+#pragma unroll
+    for (int k = 0; k < M::ROWS; k++) {
+      if (!m.row_ok(k)) continue;
+      const int ly = m.r + k * M::RG, c = m.base + k * M::RG * RW;
+      const int gj = oy + ly, gi = ox + 2 * m.p;
+      const bool rowin = gj >= 0 && gj < s;
+      const bool red0 = ((ly + par0) & 1) == 0;  // is cell 2p red
+      const float2 r2 = *(const float2*)(sr + c);
+      float2 x2;
+      x2.x = (rowin && gi >= 0 && gi < s && red0) ? nm_div(0.f - h2 * r2.x, nm_count(gi, gj, s)) : 0.f;
+      x2.y = (rowin && gi + 1 >= 0 && gi + 1 < s && !red0) ? nm_div(0.f - h2 * r2.y, nm_count(gi + 1, gj, s)) : 0.f;
+      *(float2*)(sx + c) = x2;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int hs = 1; hs < 2 * SW; hs++) {
+    nm_fm_half_sweep<RW, RH, NT>(sx, sr, m, hs & 1, par0, h2, ox, oy, s);
+    __syncthreads();
+  }
+  constexpr int XQ = TX / 4;
+  for (int q = threadIdx.x; q < XQ * TY; q += NT) {
+    const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+    *(float4*)(x + gb + (size_t)(ty0 + iy) * s + tx0 + m4) = *(const float4*)(sx + (iy + H) * RW + H + m4);
+  }
+  constexpr int CX = TX / 2, CY = TY / 2;
+  const int cs = s >> 1;
+  const size_t cb = (size_t)g * cs * cs;
+  for (int q = threadIdx.x; q < CX * CY; q += NT) {
+    const int cy = q / CX, cx = q - cy * CX;
+    const int I = (tx0 >> 1) + cx, J = (ty0 >> 1) + cy;
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int i = 2 * I + a, j = 2 * J + b;
+        const int k = (2 * cy + b + H) * RW + (2 * cx + a + H);
+        const float xc = sx[k];
+        float sum = 0.f;
+        if (i > 0) sum += sx[k - 1] - xc;
+        if (i < s - 1) sum += sx[k + 1] - xc;
+        if (j > 0) sum += sx[k - RW] - xc;
+        if (j < s - 1) sum += sx[k + RW] - xc;
+        acc += sr[k] - sum * ih2;
+      }
+    crhs[cb + (size_t)J * cs + I] = acc * 0.25f;
+  }
+}
+
+template <int SW, class TS>
+__global__ void __launch_bounds__(TS::NT)
+k_nmt_upq(const float* __restrict__ xin, float* __restrict__ xout, const float* __restrict__ rhs,
+          const float* __restrict__ xc, int s, int tiles_x, int tpg, float h2) {
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT, H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  constexpr int CW = RW / 2 + 3, CH = RH / 2 + 3;
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad windows");
+  __shared__ __align__(16) float sx[RH * RW];
+  __shared__ __align__(16) float sr[RH * RW];
+  __shared__ float sc[CH * CW];
+  const int bid = xcd_tile((int)gridDim.x), g = bid / tpg, t = bid - g * tpg;
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const int cs = s >> 1;
+  const size_t gb = (size_t)g * s * s, cb = (size_t)g * cs * cs;
+  const int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
+  // coarse footprint (indices clamped into the box), x and rhs: all loads in flight at once
+  constexpr int KC = (CW * CH + NT - 1) / NT;
+  float fc[KC];
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    const int q = threadIdx.x + k * NT;
+    if (q < CW * CH) {
+      const int ly = q / CW, lx = q - ly * CW;
+      const int cx = min(max(cox + lx, 0), cs - 1), cy = min(max(coy + ly, 0), cs - 1);
+      fc[k] = xc[cb + (size_t)cy * cs + cx];
+    }
+  }
+  NmQWindow<RW, RH, NT> wx, wr;
+  wx.issue(xin + gb, s, ox, oy);
+  wr.issue(rhs + gb, s, ox, oy);
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    const int q = threadIdx.x + k * NT;
+    if (q < CW * CH) sc[q] = fc[k];
+  }
+  wr.commit(sr);
+  __syncthreads();
+  // x + the prolongation, four cells per thread (a quad is wholly in or out of the box)
+  {
+    using W = NmQWindow<RW, RH, NT>;
+#pragma unroll
+    for (int k = 0; k < W::K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < W::CNT) {
+        const int ly = q / W::RQ, lx0 = 4 * (q - ly * W::RQ);
+        const int j = oy + ly, i4 = ox + lx0;
+        float o[4] = {wx.f[k][0], wx.f[k][1], wx.f[k][2], wx.f[k][3]};
+        if (j >= 0 && j < s && i4 >= 0 && i4 < s) {
+          const int j0 = (j & 1) ? (j >> 1) : (j >> 1) - 1;
+          const float fy = (j & 1) ? 0.25f : 0.75f;
+          const int y0 = max(j0, 0) - coy, y1 = min(j0 + 1, cs - 1) - coy;
+#pragma unroll
+          for (int a = 0; a < 4; a++) {
+            const int i = i4 + a;
+            const int i0 = (i & 1) ? (i >> 1) : (i >> 1) - 1;
+            const float fx = (i & 1) ? 0.25f : 0.75f;
+            const int x0 = max(i0, 0) - cox, x1 = min(i0 + 1, cs - 1) - cox;
+            const float pa = sc[y0 * CW + x0], pb = sc[y0 * CW + x1], pc = sc[y1 * CW + x0], pd = sc[y1 * CW + x1];
+            const float top = pa * (1.f - fx) + pb * fx, bot = pc * (1.f - fx) + pd * fx;
+            o[a] = o[a] + (top * (1.f - fy) + bot * fy);
+          }
+        }
+        *(float4*)(sx + ly * RW + lx0) = make_float4(o[0], o[1], o[2], o[3]);
+      }
+    }
+  }
+  __syncthreads();
+  const PairMap<RW, RH, NT> m;
+  const int par0 = (ox + oy) & 1;
+#pragma unroll
+  for (int hs = 0; hs < 2 * SW; hs++) {
+    nm_fm_half_sweep<RW, RH, NT>(sx, sr, m, hs & 1, par0, h2, ox, oy, s);
+    __syncthreads();
+  }
+  constexpr int XQ = TX / 4;
+  for (int q = threadIdx.x; q < XQ * TY; q += NT) {
+    const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+    *(float4*)(xout + gb + (size_t)(ty0 + iy) * s + tx0 + m4) = *(const float4*)(sx + (iy + H) * RW + H + m4);
+  }
+}
+
 // The levels from s0 <= 64 down to 4x4 of one simulation per workgroup, in LDS:
 // pre-smoothing from zero, restriction, the coarsest 4x4 solved with the
 // pseudo-inverse of its Neumann Laplacian (after removing the rhs mean), then

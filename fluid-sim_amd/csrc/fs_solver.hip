@@ -2114,7 +2114,7 @@ int nmt_down_t(fs_batch_t* b, int sw, float* x, const float* r, int s, long cnt,
   const dim3 g((unsigned)(cnt * tpg)), t(TS::NT);
   const float h2 = 1.f / ((float)s * s), ih2 = (float)((double)s * s);
   if (sw == 1) return blaunch(b, fsk::k_nmt_down<1, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
-  if (sw == 2) return blaunch(b, fsk::k_nmt_down<2, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
+  if (sw == 2) return blaunch(b, fsk::k_nmt_downq<2, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
   return blaunch(b, fsk::k_nmt_down<3, TS>, g, t, x, r, s, tiles_x, tpg, h2, ih2, cr);
 }
 template <class TS>
@@ -2123,7 +2123,7 @@ int nmt_up_t(fs_batch_t* b, int sw, const float* x, float* xb, const float* r, c
   const dim3 g((unsigned)(cnt * tpg)), t(TS::NT);
   const float h2 = 1.f / ((float)s * s);
   if (sw == 1) return blaunch(b, fsk::k_nmt_up<1, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
-  if (sw == 2) return blaunch(b, fsk::k_nmt_up<2, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
+  if (sw == 2) return blaunch(b, fsk::k_nmt_upq<2, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
   return blaunch(b, fsk::k_nmt_up<3, TS>, g, t, x, xb, r, xc, s, tiles_x, tpg, h2);
 }
 int nmt_down(fs_batch_t* b, int kind, int sw, float* x, const float* r, int s, long cnt, float* cr) {
