@@ -1300,6 +1300,177 @@ k_mgt_up2q(const S* __restrict__ x1in, const S* __restrict__ rhs1, const SO* __r
   }
 }
 
+// Quad-access, fixed-map forms of the single-level down and up and of the two-level
+// down (H = 2 SW a multiple of 4; the host uses them on whole tiles with n a multiple of
+// 4): the same operands in the same order as k_mgt_down, k_mgt_up and k_mgt_down2.
+
+template <int SW, class TS, typename S>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_downq(S* __restrict__ x, const S* __restrict__ rhs, int n, float h2, float ih2, S* __restrict__ crhs,
+            int tiles_x, int nblocks, Gate gate) {
+  if (gate_closed(gate)) return;
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad windows");
+  __shared__ __align__(16) float sx[RH * RW];
+  __shared__ __align__(16) float sr[RH * RW];
+  const int t = xcd_tile(nblocks);
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  {
+    QWindow<RW, RH, NT, S> wr;
+    wr.issue(rhs, n, ox, oy);
+    wr.commit(sr);
+  }
+  __syncthreads();
+  fm_smooth_from_zero<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
+  constexpr int XQ = TX / 4;
+  for (int q = threadIdx.x; q < XQ * TY; q += NT) {
+    const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+    const float4 xv = *(const float4*)(sx + (iy + H) * RW + H + m4);
+    store4<S>(x + (ty0 + iy) * n + tx0 + m4, xv.x, xv.y, xv.z, xv.w);
+  }
+  constexpr int CX = TX / 2, CY = TY / 2;
+  const int cn = n >> 1;
+  for (int q = threadIdx.x; q < CX * CY; q += NT) {
+    const int cy = q / CX, cx = q - cy * CX;
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int k = (2 * cy + b + H) * RW + (2 * cx + a + H);
+        const float lap = (sx[k - 1] + sx[k + 1] + sx[k - RW] + sx[k + RW] - 4.f * sx[k]) * ih2;
+        s += sr[k] - lap;
+      }
+    crhs[((ty0 >> 1) + cy) * cn + (tx0 >> 1) + cx] = st<S>(s * 0.25f);
+  }
+}
+
+template <int SW, class TS, typename S, typename SO = S>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_upq(const S* __restrict__ xin, SO* __restrict__ xout, const S* __restrict__ rhs,
+          const SO* __restrict__ xc, int n, float h2, int tiles_x, int nblocks, Gate gate) {
+  if (gate_closed(gate)) return;
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  constexpr int CW = RW / 2 + 3, CH = RH / 2 + 3;  // coarse footprint of the region
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad windows");
+  __shared__ __align__(16) float sx[RH * RW];
+  __shared__ __align__(16) float sr[RH * RW];
+  __shared__ float sc[CH * CW];
+  const int t = xcd_tile(nblocks);
+  const int tx0 = (t % tiles_x) * TX, ty0 = (t / tiles_x) * TY;
+  const int ox = tx0 - H, oy = ty0 - H;
+  const int cn = n >> 1;
+  const int cox = (ox >> 1) - 1, coy = (oy >> 1) - 1;
+  QWindow<RW, RH, NT, S> wx, wr;
+  Window<CW, CH, NT, SO> wc;
+  wx.issue(xin, n, ox, oy);
+  wc.issue(xc, cn, cox, coy);
+  wr.issue(rhs, n, ox, oy);
+  wc.commit(sc);
+  wr.commit(sr);
+  __syncthreads();
+  {
+    using W = QWindow<RW, RH, NT, S>;
+#pragma unroll
+    for (int k = 0; k < W::K; k++) {
+      const int q = threadIdx.x + k * NT;
+      if (q < W::CNT) {
+        const int ly = q / W::RQ, lx0 = 4 * (q - ly * W::RQ);
+        const float4 xv = make_float4(wx.f[k][0], wx.f[k][1], wx.f[k][2], wx.f[k][3]);
+        *(float4*)(sx + ly * RW + lx0) = prolong_quad<CW, false, SO>(sc, cox, coy, ox + lx0, oy + ly, xv);
+      }
+    }
+  }
+  __syncthreads();
+  fm_post_sweeps<SW, RW, RH, NT>(sx, sr, (ox + oy) & 1, h2);
+  constexpr int XQ = TX / 4;
+  for (int q = threadIdx.x; q < XQ * TY; q += NT) {
+    const int iy = q / XQ, m4 = 4 * (q - iy * XQ);
+    const float4 xv = *(const float4*)(sx + (iy + H) * RW + H + m4);
+    store4<SO>(xout + (ty0 + iy) * n + tx0 + m4, xv.x, xv.y, xv.z, xv.w);
+  }
+}
+
+template <int SW, class TS, typename S>
+__global__ void __launch_bounds__(TS::NT)
+k_mgt_down2q(S* __restrict__ x0, const S* __restrict__ rhs0, int n0, float h20, float ih20, S* __restrict__ x1,
+             S* __restrict__ rhs1, float h21, float ih21, S* __restrict__ crhs, int tiles_x, int nblocks,
+             Gate gate) {
+  if (gate_closed(gate)) return;
+  constexpr int TX = TS::TX, TY = TS::TY, NT = TS::NT;
+  constexpr int H = 2 * SW;
+  constexpr int RW1 = TX + 2 * H, RH1 = TY + 2 * H;            // level l+1 region
+  constexpr int RW0 = 2 * RW1 + 2 * H, RH0 = 2 * RH1 + 2 * H;  // level l region: residual on 2 RW1 x 2 RH1
+  static_assert(H % 4 == 0 && TX % 4 == 0, "quad windows");
+  __shared__ __align__(16) float sx0[RH0 * RW0];
+  __shared__ __align__(16) float sr0[RH0 * RW0];
+  __shared__ __align__(16) float sx1[RH1 * RW1];
+  __shared__ __align__(16) float sr1[RH1 * RW1];
+  const int n1 = n0 >> 1;
+  const int t = xcd_tile(nblocks);
+  const int tx1 = (t % tiles_x) * TX, ty1 = (t / tiles_x) * TY;
+  const int ox1 = tx1 - H, oy1 = ty1 - H;          // level-(l+1) coordinate of sx1 (0, 0)
+  const int ox0 = 2 * ox1 - H, oy0 = 2 * oy1 - H;  // level-l coordinate of sx0 (0, 0)
+  {
+    QWindow<RW0, RH0, NT, S> wr;
+    wr.issue(rhs0, n0, ox0, oy0);
+    wr.commit(sr0);
+  }
+  __syncthreads();
+  // after 2 SW half-sweeps from zero, x is exact on [H - 1, RW0 - H + 1) and the
+  // residual on [H, RW0 - H), the 2 RW1 x 2 RH1 block under the level-(l+1) region
+  fm_smooth_from_zero<SW, RW0, RH0, NT>(sx0, sr0, (ox0 + oy0) & 1, h20);
+  constexpr int XQ0 = 2 * TX / 4;
+  for (int q = threadIdx.x; q < XQ0 * 2 * TY; q += NT) {
+    const int iy = q / XQ0, m4 = 4 * (q - iy * XQ0);
+    const float4 xv = *(const float4*)(sx0 + (3 * H + iy) * RW0 + 3 * H + m4);
+    store4<S>(x0 + (2 * ty1 + iy) * n0 + 2 * tx1 + m4, xv.x, xv.y, xv.z, xv.w);
+  }
+  for (int q = threadIdx.x; q < RW1 * RH1; q += NT) {
+    const int cy = q / RW1, cx = q - cy * RW1;
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int k = (2 * cy + b + H) * RW0 + (2 * cx + a + H);
+        const float lap = (sx0[k - 1] + sx0[k + 1] + sx0[k - RW0] + sx0[k + RW0] - 4.f * sx0[k]) * ih20;
+        s += sr0[k] - lap;
+      }
+    sr1[q] = rnd<S>(s * 0.25f);  // as the level-(l+1) sweep would load it
+    if (cx >= H && cx < H + TX && cy >= H && cy < H + TY)
+      rhs1[(size_t)(oy1 + cy) * n1 + ox1 + cx] = st<S>(s * 0.25f);
+  }
+  __syncthreads();
+  fm_smooth_from_zero<SW, RW1, RH1, NT>(sx1, sr1, (ox1 + oy1) & 1, h21);
+  constexpr int XQ1 = TX / 4;
+  for (int q = threadIdx.x; q < XQ1 * TY; q += NT) {
+    const int iy = q / XQ1, m4 = 4 * (q - iy * XQ1);
+    const float4 xv = *(const float4*)(sx1 + (iy + H) * RW1 + H + m4);
+    store4<S>(x1 + (ty1 + iy) * n1 + tx1 + m4, xv.x, xv.y, xv.z, xv.w);
+  }
+  constexpr int CX = TX / 2, CY = TY / 2;
+  const int cn = n1 >> 1;
+  for (int q = threadIdx.x; q < CX * CY; q += NT) {
+    const int cy = q / CX, cx = q - cy * CX;
+    float s = 0.f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int k = (2 * cy + b + H) * RW1 + (2 * cx + a + H);
+        const float lap = (sx1[k - 1] + sx1[k + 1] + sx1[k - RW1] + sx1[k + RW1] - 4.f * sx1[k]) * ih21;
+        s += sr1[k] - lap;
+      }
+    crhs[((ty1 >> 1) + cy) * cn + (tx1 >> 1) + cx] = st<S>(s * 0.25f);
+  }
+}
+
 // ----------------------------------------------------------- coarse levels
 
 // All levels from n0 (<= COARSE_N) down to the coarsest, in one workgroup:

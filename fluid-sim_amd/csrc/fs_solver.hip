@@ -814,7 +814,13 @@ int launch_down_s(fs_solver_t* s, int l) {
   int tx, nb;
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
   // rhs read; x and the coarse rhs (1 per 4 cells) written
-  return launch(s, PK_DOWN, 2.25 * sizeof(S) * F.n * F.n, fsk::k_mgt_down<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
+  // quad accesses when the halo 2 PRE is a multiple of 4 and the level is whole tiles
+  auto kern = [&] {
+    if constexpr ((2 * PRE) % 4 == 0 && TS::TX % 4 == 0)
+      if (F.n % 4 == 0 && F.n % TS::TX == 0 && F.n % TS::TY == 0) return fsk::k_mgt_downq<PRE, TS, S>;
+    return fsk::k_mgt_down<PRE, TS, S>;
+  }();
+  return launch(s, PK_DOWN, 2.25 * sizeof(S) * F.n * F.n, kern, g, dim3(TS::NT), as<S>(F.x),
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)),
                 as<S>(s->lv[l + 1].rhs), tx, nb, s->gate_v);
 }
@@ -842,7 +848,12 @@ int launch_down2_s(fs_solver_t* s, int l) {
   dim3 g = tiles_grid_t<TS>(C.n, &tx, &nb);
   // level l: rhs read, x written; level l + 1: x and rhs written; level l + 2 rhs written
   const double bytes = 2.0 * sizeof(S) * F.n * F.n + 2.25 * sizeof(S) * C.n * C.n;
-  return launch(s, PK_DOWN, bytes, fsk::k_mgt_down2<PRE, TS, S>, g, dim3(TS::NT), as<S>(F.x),
+  auto kern = [&] {
+    if constexpr ((2 * PRE) % 4 == 0)
+      if (C.n % TS::TX == 0 && C.n % TS::TY == 0) return fsk::k_mgt_down2q<PRE, TS, S>;
+    return fsk::k_mgt_down2<PRE, TS, S>;
+  }();
+  return launch(s, PK_DOWN, bytes, kern, g, dim3(TS::NT), as<S>(F.x),
                 as<S>((const float*)F.rhs), F.n, (float)(F.dx * F.dx), (float)(1.0 / (F.dx * F.dx)), as<S>(C.x),
                 as<S>(C.rhs), (float)(C.dx * C.dx), (float)(1.0 / (C.dx * C.dx)), as<S>(s->lv[l + 2].rhs), tx, nb,
                 s->gate_v);
@@ -859,7 +870,12 @@ int launch_up_s(fs_solver_t* s, int l) {
   dim3 g = tiles_grid_t<TS>(F.n, &tx, &nb);
   // x, rhs (storage precision) and the coarse correction (FP32, 1 per 4 cells) read;
   // the correction written in FP32 (see Level::xb)
-  return launch(s, PK_UP, (2.0 * sizeof(S) + 5.0) * F.n * F.n, fsk::k_mgt_up<POST, TS, S, float>, g, dim3(TS::NT),
+  auto kern = [&] {
+    if constexpr ((2 * POST) % 4 == 0 && TS::TX % 4 == 0)
+      if (F.n % 4 == 0 && F.n % TS::TX == 0 && F.n % TS::TY == 0) return fsk::k_mgt_upq<POST, TS, S, float>;
+    return fsk::k_mgt_up<POST, TS, S, float>;
+  }();
+  return launch(s, PK_UP, (2.0 * sizeof(S) + 5.0) * F.n * F.n, kern, g, dim3(TS::NT),
                 as<S>((const float*)F.x), F.xb, as<S>((const float*)F.rhs), (const float*)s->lv[l + 1].xb, F.n,
                 (float)(F.dx * F.dx), tx, nb, s->gate_v);
 }
