@@ -274,16 +274,22 @@ k_cav_diffuse_solid(const float* __restrict__ u, const float* __restrict__ v, fl
 // stored, r goes (FP32) to the simulation's finest multigrid rhs, block max |r| to part.
 // One wavefront per 256-cell block: lane l takes the 4 consecutive cells 4l .. 4l + 3 of
 // it (one row; sizes are multiples of 64, arena offsets multiples of 4096 cells), with
-// 16-byte loads and stores; each cell's arithmetic is the one-cell formula's.
-__global__ void __launch_bounds__(64)
+// 16-byte loads and stores; each cell's arithmetic is the one-cell formula's. A
+// workgroup holds CAV_WPB consecutive blocks, one per wavefront (a launch of one
+// wavefront per workgroup over 20 M cells is bound by dispatching its workgroups).
+constexpr int CAV_WPB = 4;
+__global__ void __launch_bounds__(64 * CAV_WPB)
 k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const double* __restrict__ p,
                  const float* __restrict__ xe, double* __restrict__ pout, float* __restrict__ rhs,
-                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
-  const int blk = xcd_tile((int)gridDim.x);
+                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part,
+                 int nblocks) {
+  const int blk = xcd_tile((int)gridDim.x) * CAV_WPB + (int)(threadIdx.x >> 6);
+  if (blk >= nblocks) return;
+  const int lane = (int)threadIdx.x & 63;
   const int k = blk0[nsim + blk];
   const CavSim& S = sims[k];
   const int n = S.n;
-  const int local = (blk - blk0[k]) * 256 + 4 * (int)threadIdx.x;
+  const int local = (blk - blk0[k]) * 256 + 4 * lane;
   const int j = local >> S.lg, i = local & (n - 1);
   const long c = S.off + local, cl = S.lvl + local;
   // p + e at 4 cells of row jj (0.0 + e with no p, as the one-cell formula)
@@ -347,19 +353,23 @@ k_cav_residual64(const float* __restrict__ u, const float* __restrict__ v, const
   *(double2*)(pout + c + 2) = double2{out[2], out[3]};
   *(float4*)(rhs + cl) = float4{r4[0], r4[1], r4[2], r4[3]};
   amax = wave_max(amax);
-  if (threadIdx.x == 0) part[blk] = amax;
+  if (lane == 0) part[blk] = amax;
 }
 
 // u -= grad p on the interior faces; block max |u|, |v| for the next step's dt.
-// One wavefront per 256-cell block, 4 cells of a row per lane (as k_cav_residual64).
-__global__ void __launch_bounds__(64)
+// One wavefront per 256-cell block, 4 cells of a row per lane, CAV_WPB blocks per
+// workgroup (as k_cav_residual64).
+__global__ void __launch_bounds__(64 * CAV_WPB)
 k_cav_gradient64(float* __restrict__ u, float* __restrict__ v, const double* __restrict__ p,
-                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part) {
-  const int blk = xcd_tile((int)gridDim.x);
+                 const CavSim* __restrict__ sims, const int* __restrict__ blk0, int nsim, double* __restrict__ part,
+                 int nblocks) {
+  const int blk = xcd_tile((int)gridDim.x) * CAV_WPB + (int)(threadIdx.x >> 6);
+  if (blk >= nblocks) return;
+  const int lane = (int)threadIdx.x & 63;
   const int k = blk0[nsim + blk];
   const CavSim& S = sims[k];
   const int n = S.n;
-  const int local = (blk - blk0[k]) * 256 + 4 * (int)threadIdx.x;
+  const int local = (blk - blk0[k]) * 256 + 4 * lane;
   const int j = local >> S.lg, i = local & (n - 1);
   const long c = S.off + local;
   const double dn = (double)n;
@@ -393,7 +403,7 @@ k_cav_gradient64(float* __restrict__ u, float* __restrict__ v, const double* __r
   }
   a = wave_max(a);
   b = wave_max(b);
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     part[2 * blk] = a;
     part[2 * blk + 1] = b;
   }

@@ -2167,10 +2167,11 @@ int nmt_up(fs_batch_t* b, int kind, int sw, const float* x, float* xb, const flo
 int batch_iteration(fs_batch_t* b, int it, int pin, bool zero_p, bool residual_only) {
   int rc;
   const int c = b->cur;
-  if ((rc = blaunch(b, fsk::k_cav_residual64, dim3(b->nblocks), dim3(64), (const float*)b->u[c],
+  if ((rc = blaunch(b, fsk::k_cav_residual64, dim3((b->nblocks + fsk::CAV_WPB - 1) / fsk::CAV_WPB),
+                    dim3(64 * fsk::CAV_WPB), (const float*)b->u[c],
                     (const float*)b->v[c], (const double*)(zero_p ? nullptr : b->p64[pin]),
                     (const float*)(it > 0 ? b->lxb : nullptr), b->p64[1 - pin], b->lr, (const fsk::CavSim*)b->sims,
-                    (const int*)b->blk0, b->nsim, b->rpart + (size_t)it * b->nblocks)))
+                    (const int*)b->blk0, b->nsim, b->rpart + (size_t)it * b->nblocks, b->nblocks)))
     return rc;
   if (residual_only) return FS_OK;
   const int pre = b->cfg.mg_pre, post = b->cfg.mg_post;
@@ -3863,9 +3864,9 @@ int fs_batch_step_cfl(void* h, double cfl) {
   int cycles = 0;
   int prc = batch_finish_projection(b, c, a0, zero_p, &cycles);
   if (prc && prc != FS_E_NOT_CONVERGED) return prc;
-  if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3(b->nblocks), dim3(64), b->u[b->cur], b->v[b->cur],
-                    (const double*)b->p64[b->pidx], (const fsk::CavSim*)b->sims, (const int*)b->blk0, b->nsim,
-                    b->mpart)))
+  if ((rc = blaunch(b, fsk::k_cav_gradient64, dim3((b->nblocks + fsk::CAV_WPB - 1) / fsk::CAV_WPB),
+                    dim3(64 * fsk::CAV_WPB), b->u[b->cur], b->v[b->cur], (const double*)b->p64[b->pidx],
+                    (const fsk::CavSim*)b->sims, (const int*)b->blk0, b->nsim, b->mpart, b->nblocks)))
     return rc;
   b->have_p = true;
   b->last_vcycles = cycles;
