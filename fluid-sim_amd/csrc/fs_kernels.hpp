@@ -293,6 +293,118 @@ k_splat(float* __restrict__ u, float* __restrict__ v, float4* __restrict__ dye, 
   if (dye) dye[k] = c;
 }
 
+// ----------------------------------------------------------- display: render and bloom
+//
+// Display only: these read the fields and write pixels that nothing in the step reads.
+// Pixels are rgba half floats (IEEE binary16, round to nearest even), as the reference's
+// rgba16float targets. The WGSL built-ins are spelled out: smoothstep(e0, e1, x) =
+// t^2 (3 - 2t) with t = clamp((x - e0) / (e1 - e0), 0, 1), mix(a, b, t) = a (1 - t) + b t.
+
+struct __align__(8) Half4 {
+  _Float16 x, y, z, w;
+};
+__device__ __forceinline__ float smoothstep_f(float e0, float e1, float x) {
+  float t = (x - e0) / (e1 - e0);
+  t = fminf(fmaxf(t, 0.f), 1.f);
+  return t * t * (3.f - 2.f * t);
+}
+__device__ __forceinline__ float mix_f(float a, float b, float t) { return a * (1.f - t) + b * t; }
+__device__ __forceinline__ float3 mix3(float3 a, float3 b, float t) {
+  return float3{mix_f(a.x, b.x, t), mix_f(a.y, b.y, t), mix_f(a.z, b.z, t)};
+}
+__device__ __forceinline__ float3 scale3(float3 a, float s) { return float3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return float3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+// Rec. 709 luminance, summed left to right
+__device__ __forceinline__ float lum709(float3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+__device__ __forceinline__ Half4 to_half4(float3 c) {
+  return Half4{(_Float16)c.x, (_Float16)c.y, (_Float16)c.z, (_Float16)1.0f};
+}
+__device__ __forceinline__ float3 rgb_of(Half4 h) { return float3{(float)h.x, (float)h.y, (float)h.z}; }
+
+// RENDER (src/gpu/shaders.js:677-780): the corner vorticity (v_x - u_y at the four
+// corners of the cell, averaged to its centre) through a signed neon ramp (mode < 0.5), or
+// the face speed |(u, v)| of the cell (mode >= 0.5), both with the auto-exposure `scale`;
+// a white-hot term from the speed; a luminance lift and a little colour from the dye
+// (dye may be null: no dye yet); non-finite channels painted 0. Periodic neighbours.
+__global__ void __launch_bounds__(BX * BY)
+k_render(const float* __restrict__ u, const float* __restrict__ v, const float4* __restrict__ dye,
+         Half4* __restrict__ out, int nx, int ny, float dx, float dy, float scale, float mode) {
+  const int2 bk = xcd_block2();
+  const int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  auto U = [&](int a, int b) { return u[(size_t)wrapi(b, ny) * nx + wrapi(a, nx)]; };
+  auto V = [&](int a, int b) { return v[(size_t)wrapi(b, ny) * nx + wrapi(a, nx)]; };
+  auto curl = [&](int a, int b) { return (V(a, b) - V(a - 1, b)) / dx - (U(a, b) - U(a, b - 1)) / dy; };
+  const float w = 0.25f * (curl(i, j) + curl(i + 1, j) + curl(i, j + 1) + curl(i + 1, j + 1));
+  const float cu = U(i, j), cv = V(i, j);
+  const float speed = sqrtf(cu * cu + cv * cv);
+  const float3 MAGENTA{1.00f, 0.10f, 0.62f}, CYAN{0.10f, 0.92f, 1.00f}, VIOLET{0.16f, 0.03f, 0.42f},
+      VOID_{0.004f, 0.003f, 0.016f};
+  float3 col;
+  if (mode < 0.5f) {
+    const float sig = tanhf(fminf(fmaxf(w * scale, -12.f), 12.f));
+    const float mag = fabsf(sig);
+    const float3 hue = sig >= 0.f ? MAGENTA : CYAN;
+    const float3 ramp = mix3(mix3(VOID_, VIOLET, smoothstep_f(0.f, 0.22f, mag)), hue, smoothstep_f(0.12f, 0.62f, mag));
+    col = scale3(ramp, 0.30f + 4.6f * mag * mag);
+  } else {
+    const float t = tanhf(fminf(fmaxf(speed * scale, -12.f), 12.f));
+    const float3 hue = mix3(VIOLET, CYAN, smoothstep_f(0.15f, 0.95f, t));
+    col = scale3(mix3(VOID_, hue, smoothstep_f(0.f, 0.6f, t)), 0.35f + 2.9f * t * t);
+  }
+  const float sp = tanhf(fminf(fmaxf(speed * scale * 0.45f, 0.f), 12.f));
+  const float sp2 = sp * sp;
+  col = add3(col, scale3(float3{0.55f, 0.78f, 1.0f}, (sp2 * sp2 * sp) * 0.85f));
+  const float4 d = dye ? dye[(size_t)j * nx + i] : float4{0.f, 0.f, 0.f, 0.f};
+  const float3 drgb{d.x, d.y, d.z};
+  col = add3(add3(col, scale3(scale3(col, lum709(drgb)), 1.5f)), scale3(drgb, 0.20f));
+  col = float3{col.x == col.x ? col.x : 0.f, col.y == col.y ? col.y : 0.f, col.z == col.z ? col.z : 0.f};
+  out[(size_t)j * nx + i] = to_half4(col);
+}
+
+// BLOOM_BRIGHT (src/gpu/shaders.js:783-812): the 2x2 box average of the render at half
+// resolution (bw x bh pixels, the render's row stride nx), kept by a soft knee
+// smoothstep(thr, thr + 0.6, luminance).
+__global__ void __launch_bounds__(BX * BY)
+k_bloom_bright(const Half4* __restrict__ src, Half4* __restrict__ dst, int bw, int bh, int nx, float thr) {
+  const int2 bk = xcd_block2();
+  const int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  if (i >= bw || j >= bh) return;
+  const size_t r0 = (size_t)(2 * j) * nx + 2 * i, r1 = r0 + nx;
+  float3 acc = rgb_of(src[r0]);
+  acc = add3(acc, rgb_of(src[r0 + 1]));
+  acc = add3(acc, rgb_of(src[r1]));
+  acc = add3(acc, rgb_of(src[r1 + 1]));
+  acc = scale3(acc, 0.25f);
+  const float knee = smoothstep_f(thr, thr + 0.6f, lum709(acc));
+  dst[(size_t)j * bw + i] = to_half4(scale3(acc, knee));
+}
+
+// BLOOM_BLUR (src/gpu/shaders.js:816-843): one axis (0 = x, 1 = y) of the separable
+// 13-tap Gaussian (sigma ~ 3), normalised by its weights, taps clamped to the image (the
+// image is not periodic).
+__global__ void __launch_bounds__(BX * BY)
+k_bloom_blur(const Half4* __restrict__ src, Half4* __restrict__ dst, int bw, int bh, int axis) {
+  const int2 bk = xcd_block2();
+  const int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
+  if (i >= bw || j >= bh) return;
+  constexpr float W[7] = {0.1964f, 0.1747f, 0.1210f, 0.0656f, 0.0278f, 0.0092f, 0.0024f};
+  auto at = [&](int a, int b) {
+    a = a < 0 ? 0 : (a > bw - 1 ? bw - 1 : a);
+    b = b < 0 ? 0 : (b > bh - 1 ? bh - 1 : b);
+    return rgb_of(src[(size_t)b * bw + a]);
+  };
+  const int ox = axis ? 0 : 1, oy = axis ? 1 : 0;
+  float3 acc = scale3(at(i, j), W[0]);
+  float norm = W[0];
+#pragma unroll
+  for (int k = 1; k < 7; k++) {
+    acc = add3(acc, scale3(add3(at(i + ox * k, j + oy * k), at(i - ox * k, j - oy * k)), W[k]));
+    norm = norm + 2.f * W[k];
+  }
+  dst[(size_t)j * bw + i] = to_half4(float3{acc.x / norm, acc.y / norm, acc.z / norm});
+}
+
 // Explicit 5-point diffusion of both components (src/cpu/solver.js:56-69;
 // DIFFUSE src/gpu/shaders.js:401-425); a = nu dt/dx^2, b = nu dt/dy^2.
 __global__ void __launch_bounds__(BX * BY)

@@ -136,6 +136,12 @@ struct fs_solver_t {
   bool have_p = false;       // a previous pressure exists (warm start)
   float4* dye[2] = {nullptr, nullptr};  // rgba dye at the cell centres (allocated on first use)
   int dye_cur = 0;
+  // render(): the rgba half-float image (nx*ny) and the two half-resolution bloom
+  // buffers ((nx/2)*(ny/2)), allocated on the first render; bloom_cur holds the last
+  // bloom pass's output
+  fsk::Half4* out_img = nullptr;
+  fsk::Half4* bloom[2] = {nullptr, nullptr};
+  int bloom_cur = 0;
   std::vector<fsk::Splat> splats;         // splats of the step being enqueued
   double* p_hist = nullptr;  // FS_GUESS_EXTRAPOLATED: the pressure of the step before the last
   bool have_hist = false;    // p_hist holds it
@@ -1986,6 +1992,9 @@ void release(fs_solver_t* s) {
   dfree(s->p_hist);
   dfree(s->dye[0]);
   dfree(s->dye[1]);
+  dfree(s->out_img);
+  dfree(s->bloom[0]);
+  dfree(s->bloom[1]);
   hfree(s->ring);
   release_guess(s);
   dfree(s->spec_Z);
@@ -2992,6 +3001,16 @@ int fs_read_field(void* h, int field, void* out, size_t bytes) {
       src = s->dye[s->dye_cur];
       need = s->n * 4 * sizeof(float);
       break;
+    case FS_F_OUTPUT:
+      if (!s->out_img) return fail(FS_E_INVALID_ARG, "nothing rendered yet");
+      src = s->out_img;
+      need = s->n * sizeof(fsk::Half4);
+      break;
+    case FS_F_BLOOM:
+      if (!s->out_img) return fail(FS_E_INVALID_ARG, "nothing rendered yet");
+      src = s->bloom[s->bloom_cur];
+      need = (size_t)(s->nx >> 1) * (size_t)(s->ny >> 1) * sizeof(fsk::Half4);
+      break;
     case FS_F_PRESSURE:
       need = s->n * sizeof(double);
       if (s->cfg.pressure_solver != FS_P_MULTIGRID) {
@@ -3090,6 +3109,43 @@ int fs_upload_dye(void* h, const float* rgba, size_t n) {
   if ((rc = set_device(s)) || (rc = ensure_dye(s))) return rc;
   HIPCHK(hipMemcpyAsync(s->dye[s->dye_cur], rgba, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
   return sync_stream(s);
+}
+
+int fs_render(void* h, double mode, double scale, double bloom_threshold, int bloom_passes) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (s->nx < 2 || s->ny < 2) return fail(FS_E_INVALID_ARG, "render needs at least 2 x 2 cells");
+  if (!std::isfinite(mode) || !std::isfinite(scale) || !std::isfinite(bloom_threshold))
+    return fail(FS_E_INVALID_ARG, "render parameters must be finite");
+  if ((rc = set_device(s))) return rc;
+  const int bw = s->nx >> 1, bh = s->ny >> 1;
+  const size_t nb = (size_t)bw * (size_t)bh;
+  if (!s->out_img) {
+    if ((rc = dalloc(&s->out_img, s->n)) || (rc = dalloc(&s->bloom[0], nb)) || (rc = dalloc(&s->bloom[1], nb)))
+      return rc;
+    // a render with no bloom pass leaves the bloom as it was: zero before the first
+    HIPCHK(hipMemsetAsync(s->bloom[0], 0, nb * sizeof(fsk::Half4), s->stream));
+    s->bloom_cur = 0;
+  }
+  fsk::k_render<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
+      s->u[s->cur], s->v[s->cur], s->dye[0] ? s->dye[s->dye_cur] : nullptr, s->out_img, s->nx, s->ny,
+      (float)s->dx, (float)s->dy, (float)scale, (float)mode);
+  LAUNCHCHK();
+  if (bloom_passes > 0) {
+    fsk::k_bloom_bright<<<grid2(bw, bh), BLK, 0, s->stream>>>(s->out_img, s->bloom[0], bw, bh, s->nx,
+                                                             (float)bloom_threshold);
+    LAUNCHCHK();
+    s->bloom_cur = 0;
+    for (int k = 0; k < bloom_passes; k++)
+      for (int axis = 0; axis < 2; axis++) {
+        fsk::k_bloom_blur<<<grid2(bw, bh), BLK, 0, s->stream>>>(s->bloom[s->bloom_cur], s->bloom[1 - s->bloom_cur],
+                                                               bw, bh, axis);
+        LAUNCHCHK();
+        s->bloom_cur = 1 - s->bloom_cur;
+      }
+  }
+  return FS_OK;
 }
 
 int fs_step_cfl(void* h, double cfl) {

@@ -24,7 +24,7 @@ PRESSURE = {"jacobi": 0, "red-black": 1, "multigrid": 2}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GUESS = {"zero": 0, "previous": 1, "learned": 2, "learned-coarse": 3, "learned-quarter": 4,
          "extrapolated": 5}
-FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4, "dye": 5}
+FIELD = {"u": 0, "v": 1, "div": 2, "pressure": 3, "residual": 4, "dye": 5, "output": 6, "bloom": 7}
 PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down": 4, "coarse": 5, "up": 6,
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
                  "advect_fused": 11, "forcing": 12, "up_coarse": 13}
@@ -153,6 +153,7 @@ _SIGNATURES = {
     "fs_step": (C.c_int, [_VP, C.c_double]),
     "fs_step_splats": (C.c_int, [_VP, C.c_double, _VP, C.c_int]),
     "fs_upload_dye": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "fs_render": (C.c_int, [_VP, C.c_double, C.c_double, C.c_double, C.c_int]),
     "fs_step_cfl": (C.c_int, [_VP, C.c_double]),
     "fs_sync": (C.c_int, [_VP]),
     "fs_max_divergence": (C.c_int, [_VP, C.POINTER(C.c_double)]),

@@ -194,13 +194,26 @@ class GPUFluidSolver:
         return {"u": u, "v": v}
 
     def readField(self, name: str):
-        """readTexture(tex) (src/gpu/solver.js:336-353) by field name."""
+        """readTexture(tex) (src/gpu/solver.js:336-353) by field name. "output" and "bloom"
+        (the last render()) come back as float16 rgba, (ny, nx, 4) and (ny/2, nx/2, 4)."""
         if name not in _lib.FIELD:
             raise ValueError(f"unknown field {name}; have {', '.join(_lib.FIELD)}")
-        dtype = np.float64 if name == "pressure" else np.float32
-        out = np.empty(self.nx * self.ny * (4 if name == "dye" else 1), dtype)
+        if name in ("output", "bloom"):
+            h, w = (self.ny, self.nx) if name == "output" else (self.ny >> 1, self.nx >> 1)
+            out = np.empty((h, w, 4), np.float16)
+        else:
+            dtype = np.float64 if name == "pressure" else np.float32
+            out = np.empty(self.nx * self.ny * (4 if name == "dye" else 1), dtype)
         check(self._L.fs_read_field(self.handle, _lib.FIELD[name], _ptr(out), out.nbytes))
         return out
+
+    def render(self, mode=0, scale=1.5, bloomThreshold=0.75, bloomPasses=2):
+        """render({mode, scale, bloomThreshold, bloomPasses}) (src/gpu/solver.js:771-823):
+        the display image of the current velocity and dye and its bloom, enqueued; read
+        them with readField("output") and readField("bloom"). Nothing it computes feeds
+        back into the solve. Passes run while k < bloomPasses, as the reference's loop."""
+        passes = max(0, math.ceil(bloomPasses))
+        check(self._L.fs_render(self.handle, float(mode), float(scale), float(bloomThreshold), int(passes)))
 
     # ------------------------------------------------------------------ step
     def step(self, dt, splats=()):

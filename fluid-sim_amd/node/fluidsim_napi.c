@@ -271,6 +271,9 @@ static void job_execute(napi_env env, void* data) {
   release_turn(j->h);
 }
 
+/* the render targets: rgba half floats, returned as their uint16 bits */
+static int is_half_field(int field) { return field == FS_F_OUTPUT || field == FS_F_BLOOM; }
+
 static napi_value copy_typed(napi_env env, napi_typedarray_type t, size_t count, size_t esize, const void* src) {
   napi_value ab, arr;
   void* p;
@@ -297,6 +300,7 @@ static void job_complete(napi_env env, napi_status status, void* data) {
       }
       case J_READ_FIELD:
         result = j->field == FS_F_PRESSURE ? copy_typed(env, napi_float64_array, j->n, sizeof(double), j->a)
+                 : is_half_field(j->field) ? copy_typed(env, napi_uint16_array, j->n, sizeof(uint16_t), j->a)
                                            : copy_typed(env, napi_float32_array, j->n, sizeof(float), j->a);
         break;
       case J_MAX_DIV:
@@ -380,7 +384,8 @@ static napi_value js_read(napi_env env, napi_callback_info info) {
   return start_job(env, argv[0], j);
 }
 
-/* readField(h, field, n) -> Promise<Float32Array | Float64Array (pressure)> */
+/* readField(h, field, n) -> Promise<Float32Array | Float64Array (pressure) | Uint16Array
+ * (output, bloom: n is their count of half floats)> */
 static napi_value js_read_field(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return NULL;
@@ -392,7 +397,8 @@ static napi_value js_read_field(napi_env env, napi_callback_info info) {
   j->field = (int)f;
   j->n = (size_t)dn;
   if (j->field == FS_F_DYE) j->n *= 4;
-  j->bytes = j->n * (j->field == FS_F_PRESSURE ? sizeof(double) : sizeof(float));
+  j->bytes = j->n * (j->field == FS_F_PRESSURE ? sizeof(double) : is_half_field(j->field) ? sizeof(uint16_t)
+                                                                                            : sizeof(float));
   j->a = malloc(j->bytes + 1);
   return start_job(env, argv[0], j);
 }
@@ -592,6 +598,24 @@ static napi_value js_step_cfl(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* render(h, mode, scale, bloomThreshold, bloomPasses) */
+static napi_value js_render(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  if (!get_args(env, info, 5, argv)) return NULL;
+  Handle* h = get_handle(env, argv[0]);
+  if (!h) return NULL;
+  double mode, scale, thr, passes;
+  NAPI_CALL(env, napi_get_value_double(env, argv[1], &mode));
+  NAPI_CALL(env, napi_get_value_double(env, argv[2], &scale));
+  NAPI_CALL(env, napi_get_value_double(env, argv[3], &thr));
+  NAPI_CALL(env, napi_get_value_double(env, argv[4], &passes));
+  lock(h);
+  int rc = fs_render(h->solver, mode, scale, thr, (int)passes);
+  unlock(h);
+  if (rc) return throw_fs(env, rc);
+  return NULL;
+}
+
 static napi_value js_sync(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -781,6 +805,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"solidImpulse", js_solid_impulse}, {"loadGuess", js_load_guess}, {"readField", js_read_field},
       {"energySpectrum", js_spectrum}, {"syncAsync", js_sync_async}, {"runKernel", js_run_kernel},
       {"setForcingModes", js_forcing_modes}, {"stepSplats", js_step_splats}, {"uploadDye", js_upload_dye},
+      {"render", js_render},
   };
   for (size_t k = 0; k < sizeof fns / sizeof fns[0]; k++) {
     napi_value f;

@@ -26,7 +26,7 @@ const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-
 const KERNEL = { "advect-sl": 0, "advect-mc": 1, divergence: 2, diffuse: 3, project: 4, gradient: 5, solid: 6,
   forcing: 7, "advect-dye": 8 };
 // fs_read_field ids
-const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4, dye: 5 };
+const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4, dye: 5, output: 6, bloom: 7 };
 
 export async function createDevice({ requireTimestamps = false } = {}) {
   const d = addon.device();
@@ -92,7 +92,10 @@ export class GPUFluidSolver {
     if (device && device._solvers) device._solvers.add(this);
     // the reference's textures by name (src/gpu/solver.js:111-125, :287-292), for readTexture
     this.div = { format: "rgba32float", _field: "div" };
-    this.output = { format: "rgba16float", _field: null };
+    this.output = { format: "rgba16float", _field: "output" };
+    // half resolution; after render() bloom[0] holds the last blur pass (the reference
+    // reverses the pair after each pass, src/gpu/solver.js:808-821)
+    this.bloom = [{ format: "rgba16float", _field: "bloom" }, { format: "rgba16float", _field: null }];
   }
 
   get velCur() { return { format: "rgba32float", _field: "velocity" }; }
@@ -147,10 +150,24 @@ export class GPUFluidSolver {
     return out;
   }
 
-  // a field by name: "u", "v", "div", "pressure" (Float64Array), "residual"
+  // a field by name: "u", "v", "div", "pressure" (Float64Array), "residual", "dye"; the
+  // render targets "output" and "bloom" as Uint16Array of rgba IEEE half floats
   readField(name) {
     if (!(name in FIELD)) throw new Error(`unknown field ${name}; have ${Object.keys(FIELD).join(", ")}`);
-    return addon.readField(this._h, FIELD[name], this.nx * this.ny);
+    const count = name === "output" ? 4 * this.nx * this.ny
+      : name === "bloom" ? 4 * (this.nx >> 1) * (this.ny >> 1) : this.nx * this.ny;
+    return addon.readField(this._h, FIELD[name], count);
+  }
+
+  // render({mode, scale, bloomThreshold, bloomPasses}) (src/gpu/solver.js:771-823): the
+  // display image of the current velocity and dye into `output`, then the bloom chain;
+  // returns `output`. Its pixels are rgba16float, which readTexture refuses as the
+  // reference's does; readField("output") / readField("bloom") return their bits.
+  render({ mode = 0, scale = 1.5, bloomThreshold = 0.75, bloomPasses = 2 } = {}) {
+    let passes = 0;
+    for (let k = 0; k < bloomPasses; k++) passes++;
+    addon.render(this._h, mode, scale, bloomThreshold, passes);
+    return this.output;
   }
 
   // one kernel by name (replaces gputest's _dispatch of internals, tools/gputest.html:90-129)

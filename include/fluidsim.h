@@ -68,7 +68,11 @@ enum {
   FS_F_DIV = 2,      /* float[nx*ny], divergence written by the last divergence pass */
   FS_F_PRESSURE = 3, /* double[nx*ny], pressure of the last projection */
   FS_F_RESIDUAL = 4, /* float[nx*ny], level-0 multigrid right-hand side of the last cycle */
-  FS_F_DYE = 5       /* float[nx*ny*4], rgba dye at the cell centres (after fs_upload_dye or a dye splat) */
+  FS_F_DYE = 5,      /* float[nx*ny*4], rgba dye at the cell centres (after fs_upload_dye or a dye splat) */
+  FS_F_OUTPUT = 6,   /* uint16[nx*ny*4], the last fs_render image: rgba IEEE half floats (the
+                        reference's `output`, rgba16float; its readTexture refuses that format) */
+  FS_F_BLOOM = 7     /* uint16[(nx/2)*(ny/2)*4], the last bloom pass's half-resolution rgba
+                        half floats (the reference's bloom[0]) */
 };
 
 /* single kernels for per-kernel parity tests (tools/gputest.html:90-100, :121-129) */
@@ -273,6 +277,13 @@ int fs_step(void* solver, double dt);
 int fs_step_splats(void* solver, double dt, const fs_splat* splats, int count);
 /* uploadDye(rgba) src/gpu/solver.js:317-323: n = nx*ny*4 floats, rgba per cell centre */
 int fs_upload_dye(void* solver, const float* rgba, size_t n);
+/* render({mode, scale, bloomThreshold, bloomPasses}) src/gpu/solver.js:771-823 (defaults
+ * 0, 1.5, 0.75, 2): the display image of the current velocity and dye (RENDER
+ * src/gpu/shaders.js:677-780; mode < 0.5 colours the vorticity, else the speed), then,
+ * when bloom_passes > 0, the bright pass at half resolution and bloom_passes rounds of
+ * the separable blur in x and y (BLOOM_BRIGHT :783-812, BLOOM_BLUR :816-843). Enqueued
+ * on the solver's stream; read back with FS_F_OUTPUT and FS_F_BLOOM. Needs nx, ny >= 2. */
+int fs_render(void* solver, double mode, double scale, double bloom_threshold, int bloom_passes);
 /* step with dt = cfl / (max|u|/dx + max|v|/dy), the CFL definition of
  * src/core/grid.js:322-327, evaluated on the device from the last projection */
 int fs_step_cfl(void* solver, double cfl);

@@ -156,6 +156,13 @@ async function main() {
     save("dye.f32", raw);
     const { u, v } = await gpu.readVelocity();
     save("splat_u.f32", u); save("splat_v.f32", v);
+    // render() as index.html:250-251 calls it; its half-float output refuses readTexture
+    const img = gpu.render({ mode: 0, scale: 1.5, bloomThreshold: 0.62, bloomPasses: 2 });
+    let threw = false;
+    try { await gpu.readTexture(img); } catch (e) { threw = true; }
+    R.renderOutputRejected = threw && img === gpu.output;
+    save("render.u16", await gpu.readField("output"));
+    save("bloom.u16", await gpu.readField("bloom"));
     gpu.destroy();
   }
   writeFileSync(join(out, "results.json"), JSON.stringify(R));

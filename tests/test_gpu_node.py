@@ -193,4 +193,10 @@ def test_node_gputest_checks_equal_the_python_binding(tmp_path):
             g.step(dt, splats=splats)
         assert same(g.readField("dye"), _load(d, "dye.f32", np.float32))
         vel = g.readVelocity()
+        g.render(mode=0, scale=1.5, bloomThreshold=0.62, bloomPasses=2)
+        img = g.readField("output")
+        bloom = g.readField("bloom")
     assert same(vel["u"], _load(d, "splat_u.f32", np.float32)) and same(vel["v"], _load(d, "splat_v.f32", np.float32))
+    assert R["renderOutputRejected"] is True
+    assert same(img.view(np.uint16).reshape(-1), _load(d, "render.u16", np.uint16))
+    assert same(bloom.view(np.uint16).reshape(-1), _load(d, "bloom.u16", np.uint16))
