@@ -104,6 +104,9 @@ def parse():
                     help="forced2048: time-averaged E(k) (device spectrum, Parseval) over a second run and the "
                          "field error against the f64 oracle over --validate-steps steps")
     ap.add_argument("--validate-steps", type=int, default=20)
+    ap.add_argument("--display", type=int, default=0, metavar="CALLS",
+                    help="after the timed steps, time CALLS render() calls with index.html's parameters, then "
+                         "run 20 steps with vorticity confinement on (for a kernel trace of k_confine)")
     ap.add_argument("--no-copy-ceiling", action="store_true",
                     help="skip timing plain 1 GiB device copies for the roofline's copy_ceiling_gbs")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -452,6 +455,30 @@ def target_comparison(a, fluidsim, initial):
     return out
 
 
+def display_timing(g, n, calls):
+    """render() as index.html:250-251 calls it (vorticity ramp, bloom threshold 0.62, two
+    blur rounds), `calls` times back to back; then 20 steps with vorticity confinement on."""
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        g.render(mode=0, scale=1.5, bloomThreshold=0.62, bloomPasses=2)
+    g.sync()
+    per = (time.perf_counter() - t0) / calls
+    # algorithmic bytes: the render reads u, v and writes an rgba half pixel (16 B per cell,
+    # no dye here); the bright pass reads 4 and writes 1 half pixel per half-resolution
+    # pixel (40 B); four blur passes read and write one each (16 B)
+    hb = (n // 2) * (n // 2)
+    nbytes = 16 * n * n + 40 * hb + 4 * 16 * hb
+    g.confinement = 1.0
+    for _ in range(20):
+        g.stepCFL(0.5)
+    g.sync()
+    g.confinement = 0.0
+    return {"render_ms": per * 1e3, "gbs": nbytes / per / 1e9, "calls": calls,
+            "algorithmic_bytes": nbytes, "launches": 6,
+            "params": "mode 0, scale 1.5, bloomThreshold 0.62, bloomPasses 2 (index.html:250-251)"}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -542,6 +569,11 @@ def main():
                 "all_kernels": {k: {"us_per_launch": v[0] / max(v[1], 1) * 1e3, "launches_per_step": v[1] / psteps,
                                     "gbs": v[2] / (v[0] / 1e3) / 1e9 if v[0] else 0.0}
                                 for k, v in prof.items() if v[1]}}
+    display = None
+    if a.display > 0:
+        display = display_timing(g, n, a.display)
+        print(f"# render + bloom: {display['render_ms'] * 1e3:.1f} us per call, {display['gbs']:.0f} GB/s "
+              "algorithmic", file=sys.stderr, flush=True)
     g.destroy()
 
     # ablation: the same workload and timing with the other initial pressure guesses
@@ -636,6 +668,8 @@ def main():
         "target": target,
         "report": report,
     }
+    if display:
+        line["display"] = display
     print(json.dumps(line), flush=True)
     return 0
 

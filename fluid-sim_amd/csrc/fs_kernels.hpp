@@ -293,6 +293,73 @@ k_splat(float* __restrict__ u, float* __restrict__ v, float4* __restrict__ dye, 
   if (dye) dye[k] = c;
 }
 
+// Corner-vorticity kernels (confinement, render) work on tiles of VT_X x VT_Y cells, two
+// rows per thread of a BX x BY block: u and v on the tile and a ring are staged in LDS
+// once, and the corner vorticity v_x - u_y of every corner the tile reads is computed
+// once, with the same operations per corner as a thread computing it alone.
+constexpr int VT_X = BX, VT_Y = 2 * BY;
+inline dim3 vt_grid(int nx, int ny) {
+  return dim3((unsigned)((nx + VT_X - 1) / VT_X), (unsigned)((ny + VT_Y - 1) / VT_Y));
+}
+// the corner (a, b): (v(a, b) - v(a - 1, b)) / dx - (u(a, b) - u(a, b - 1)) / dy
+__device__ __forceinline__ float corner_curl(float vab, float vlb, float uab, float uab1, float dx, float dy) {
+  return (vab - vlb) / dx - (uab - uab1) / dy;
+}
+// u, v of the cells (i0 - R + lx, j0 - R + ly), W x H of each, wrapped (periodic)
+template <int W, int H, int R>
+__device__ __forceinline__ void stage_uv(const float* __restrict__ u, const float* __restrict__ v, float* su,
+                                         float* sv, int nx, int ny, int i0, int j0) {
+  for (int q = threadIdx.x + threadIdx.y * BX; q < W * H; q += BX * BY) {
+    const int ly = q / W, lx = q - ly * W;
+    const size_t g = (size_t)wrapi(j0 - R + ly, ny) * nx + wrapi(i0 - R + lx, nx);
+    su[q] = u[g];
+    sv[q] = v[g];
+  }
+}
+
+// Vorticity confinement as the reference's GPU step applies it (VORTICITY_CONFINEMENT,
+// src/gpu/shaders.js:437-467; a non-physical force, off unless eps > 0): the corner
+// vorticity w at (i, j) (the lower-left corner of cell (i, j)), the unit vector n along
+// the central difference of |w| over the four neighbouring corners (|grad| + 1e-20
+// guards a uniform flow), and the force eps dx dt (n_y w, -n_x w) added to the cell's
+// own u and v faces. Reads (u, v), writes (ou, ov); periodic; launched on vt_grid.
+__global__ void __launch_bounds__(BX * BY)
+k_confine(const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ ou, float* __restrict__ ov,
+          int nx, int ny, float dx, float dy, float eps, const StepCoef* __restrict__ cf) {
+  // window: cells i0-2 .. i0+VT_X, j0-2 .. j0+VT_Y; corners i0-1 .. i0+VT_X, j0-1 .. j0+VT_Y
+  constexpr int W = VT_X + 3, H = VT_Y + 3, CW = VT_X + 2, CH = VT_Y + 2;
+  __shared__ float su[W * H], sv[W * H], sc[CW * CH];
+  const int2 bk = xcd_block2();
+  const int i0 = bk.x * VT_X, j0 = bk.y * VT_Y;
+  stage_uv<W, H, 2>(u, v, su, sv, nx, ny, i0, j0);
+  __syncthreads();
+  for (int q = threadIdx.x + threadIdx.y * BX; q < CW * CH; q += BX * BY) {
+    const int cy = q / CW, cx = q - cy * CW;  // corner (i0 - 1 + cx, j0 - 1 + cy): window (cx + 1, cy + 1)
+    const int k = (cy + 1) * W + cx + 1;
+    sc[q] = corner_curl(sv[k], sv[k - 1], su[k], su[k - W], dx, dy);
+  }
+  __syncthreads();
+  const float s = eps * dx * cf->fdt;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int li = threadIdx.x, lj = threadIdx.y + r * BY;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= nx || j >= ny) continue;
+    const int c = (lj + 1) * CW + li + 1;  // corner (i, j)
+    const float w = sc[c];
+    const float wl = fabsf(sc[c - 1]), wr = fabsf(sc[c + 1]);
+    const float wb = fabsf(sc[c - CW]), wt = fabsf(sc[c + CW]);
+    float gx = (wr - wl) / (2.f * dx), gy = (wt - wb) / (2.f * dy);
+    const float m = sqrtf(gx * gx + gy * gy) + 1e-20f;
+    gx = gx / m;
+    gy = gy / m;
+    const int k = (lj + 2) * W + li + 2;
+    const size_t g = (size_t)j * nx + i;
+    ou[g] = su[k] + s * (gy * w);
+    ov[g] = sv[k] + s * (-gx * w);
+  }
+}
+
 // ----------------------------------------------------------- display: render and bloom
 //
 // Display only: these read the fields and write pixels that nothing in the step reads.
@@ -325,41 +392,59 @@ __device__ __forceinline__ float3 rgb_of(Half4 h) { return float3{(float)h.x, (f
 // corners of the cell, averaged to its centre) through a signed neon ramp (mode < 0.5), or
 // the face speed |(u, v)| of the cell (mode >= 0.5), both with the auto-exposure `scale`;
 // a white-hot term from the speed; a luminance lift and a little colour from the dye
-// (dye may be null: no dye yet); non-finite channels painted 0. Periodic neighbours.
+// (dye may be null: no dye yet); non-finite channels painted 0. Periodic neighbours;
+// launched on vt_grid.
 __global__ void __launch_bounds__(BX * BY)
 k_render(const float* __restrict__ u, const float* __restrict__ v, const float4* __restrict__ dye,
          Half4* __restrict__ out, int nx, int ny, float dx, float dy, float scale, float mode) {
+  // window: cells i0-1 .. i0+VT_X, j0-1 .. j0+VT_Y; corners i0 .. i0+VT_X, j0 .. j0+VT_Y
+  constexpr int W = VT_X + 2, H = VT_Y + 2, CW = VT_X + 1, CH = VT_Y + 1;
+  __shared__ float su[W * H], sv[W * H], sc[CW * CH];
   const int2 bk = xcd_block2();
-  const int i = bk.x * BX + threadIdx.x, j = bk.y * BY + threadIdx.y;
-  if (i >= nx || j >= ny) return;
-  auto U = [&](int a, int b) { return u[(size_t)wrapi(b, ny) * nx + wrapi(a, nx)]; };
-  auto V = [&](int a, int b) { return v[(size_t)wrapi(b, ny) * nx + wrapi(a, nx)]; };
-  auto curl = [&](int a, int b) { return (V(a, b) - V(a - 1, b)) / dx - (U(a, b) - U(a, b - 1)) / dy; };
-  const float w = 0.25f * (curl(i, j) + curl(i + 1, j) + curl(i, j + 1) + curl(i + 1, j + 1));
-  const float cu = U(i, j), cv = V(i, j);
-  const float speed = sqrtf(cu * cu + cv * cv);
+  const int i0 = bk.x * VT_X, j0 = bk.y * VT_Y;
+  stage_uv<W, H, 1>(u, v, su, sv, nx, ny, i0, j0);
+  __syncthreads();
+  for (int q = threadIdx.x + threadIdx.y * BX; q < CW * CH; q += BX * BY) {
+    const int cy = q / CW, cx = q - cy * CW;  // corner (i0 + cx, j0 + cy): window (cx + 1, cy + 1)
+    const int k = (cy + 1) * W + cx + 1;
+    sc[q] = corner_curl(sv[k], sv[k - 1], su[k], su[k - W], dx, dy);
+  }
+  __syncthreads();
   const float3 MAGENTA{1.00f, 0.10f, 0.62f}, CYAN{0.10f, 0.92f, 1.00f}, VIOLET{0.16f, 0.03f, 0.42f},
       VOID_{0.004f, 0.003f, 0.016f};
-  float3 col;
-  if (mode < 0.5f) {
-    const float sig = tanhf(fminf(fmaxf(w * scale, -12.f), 12.f));
-    const float mag = fabsf(sig);
-    const float3 hue = sig >= 0.f ? MAGENTA : CYAN;
-    const float3 ramp = mix3(mix3(VOID_, VIOLET, smoothstep_f(0.f, 0.22f, mag)), hue, smoothstep_f(0.12f, 0.62f, mag));
-    col = scale3(ramp, 0.30f + 4.6f * mag * mag);
-  } else {
-    const float t = tanhf(fminf(fmaxf(speed * scale, -12.f), 12.f));
-    const float3 hue = mix3(VIOLET, CYAN, smoothstep_f(0.15f, 0.95f, t));
-    col = scale3(mix3(VOID_, hue, smoothstep_f(0.f, 0.6f, t)), 0.35f + 2.9f * t * t);
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int li = threadIdx.x, lj = threadIdx.y + r * BY;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= nx || j >= ny) continue;
+    const int c = lj * CW + li;
+    const float w = 0.25f * (sc[c] + sc[c + 1] + sc[c + CW] + sc[c + CW + 1]);
+    const int k = (lj + 1) * W + li + 1;
+    const float cu = su[k], cv = sv[k];
+    const float speed = sqrtf(cu * cu + cv * cv);
+    float3 col;
+    if (mode < 0.5f) {
+      const float sig = tanhf(fminf(fmaxf(w * scale, -12.f), 12.f));
+      const float mag = fabsf(sig);
+      const float3 hue = sig >= 0.f ? MAGENTA : CYAN;
+      const float3 ramp =
+          mix3(mix3(VOID_, VIOLET, smoothstep_f(0.f, 0.22f, mag)), hue, smoothstep_f(0.12f, 0.62f, mag));
+      col = scale3(ramp, 0.30f + 4.6f * mag * mag);
+    } else {
+      const float t = tanhf(fminf(fmaxf(speed * scale, -12.f), 12.f));
+      const float3 hue = mix3(VIOLET, CYAN, smoothstep_f(0.15f, 0.95f, t));
+      col = scale3(mix3(VOID_, hue, smoothstep_f(0.f, 0.6f, t)), 0.35f + 2.9f * t * t);
+    }
+    const float sp = tanhf(fminf(fmaxf(speed * scale * 0.45f, 0.f), 12.f));
+    const float sp2 = sp * sp;
+    col = add3(col, scale3(float3{0.55f, 0.78f, 1.0f}, (sp2 * sp2 * sp) * 0.85f));
+    const size_t g = (size_t)j * nx + i;
+    const float4 d = dye ? dye[g] : float4{0.f, 0.f, 0.f, 0.f};
+    const float3 drgb{d.x, d.y, d.z};
+    col = add3(add3(col, scale3(scale3(col, lum709(drgb)), 1.5f)), scale3(drgb, 0.20f));
+    col = float3{col.x == col.x ? col.x : 0.f, col.y == col.y ? col.y : 0.f, col.z == col.z ? col.z : 0.f};
+    out[g] = to_half4(col);
   }
-  const float sp = tanhf(fminf(fmaxf(speed * scale * 0.45f, 0.f), 12.f));
-  const float sp2 = sp * sp;
-  col = add3(col, scale3(float3{0.55f, 0.78f, 1.0f}, (sp2 * sp2 * sp) * 0.85f));
-  const float4 d = dye ? dye[(size_t)j * nx + i] : float4{0.f, 0.f, 0.f, 0.f};
-  const float3 drgb{d.x, d.y, d.z};
-  col = add3(add3(col, scale3(scale3(col, lum709(drgb)), 1.5f)), scale3(drgb, 0.20f));
-  col = float3{col.x == col.x ? col.x : 0.f, col.y == col.y ? col.y : 0.f, col.z == col.z ? col.z : 0.f};
-  out[(size_t)j * nx + i] = to_half4(col);
 }
 
 // BLOOM_BRIGHT (src/gpu/shaders.js:783-812): the 2x2 box average of the render at half

@@ -557,6 +557,19 @@ int enqueue_splats(fs_solver_t* s) {
   return FS_OK;
 }
 
+// Vorticity confinement after the splats and before the disk and the projection
+// (src/gpu/solver.js:558-568), when its strength is positive.
+int enqueue_confinement(fs_solver_t* s) {
+  if (!(s->cfg.confinement > 0)) return FS_OK;
+  const int c = s->cur;
+  fsk::k_confine<<<fsk::vt_grid(s->nx, s->ny), BLK, 0, s->stream>>>(s->u[c], s->v[c], s->u[1 - c], s->v[1 - c], s->nx,
+                                                             s->ny, (float)s->dx, (float)s->dy,
+                                                             (float)s->cfg.confinement, (const fsk::StepCoef*)s->coef);
+  LAUNCHCHK();
+  s->cur = 1 - c;
+  return FS_OK;
+}
+
 // The dye through the projected velocity (the step's last pass, src/gpu/solver.js:575-584).
 int enqueue_dye(fs_solver_t* s) {
   if (!s->dye[0]) return FS_OK;
@@ -1424,6 +1437,7 @@ int do_step(fs_solver_t* s) {
   }
   if ((rc = enqueue_forcing(s))) return rc;
   if ((rc = enqueue_splats(s))) return rc;
+  if ((rc = enqueue_confinement(s))) return rc;
   if ((rc = enqueue_solid(s))) return rc;
   int cycles = 0;
   int prc = enqueue_projection(s, &cycles);
@@ -1522,6 +1536,7 @@ int enqueue_gated_body(fs_solver_t* s, const BodyKey& key) {
   }
   if ((rc = enqueue_forcing(s))) return rc;
   if ((rc = enqueue_splats(s))) return rc;
+  if ((rc = enqueue_confinement(s))) return rc;
   if ((rc = enqueue_solid(s))) return rc;
   int a0 = key.a0;
   bool zero_p = key.zero_p != 0;
@@ -2811,6 +2826,7 @@ void fs_config_default(fs_config* c) {
   c->dye_fade = 0.0;
   c->mg_coarse_exact = 0;
   c->tile = 0;
+  c->confinement = 0.0;
 }
 
 int fs_solver_create(const fs_config* cfg, void** out) {
@@ -2834,6 +2850,7 @@ int fs_solver_create(const fs_config* cfg, void** out) {
   if (c.tolerance > 0 && c.pressure_solver != FS_P_MULTIGRID)
     return fail(FS_E_INVALID_ARG, "a residual tolerance needs the multigrid solver");
   if (c.mg_pre < 0 || c.mg_post < 0 || c.mg_coarse < 0) return fail(FS_E_INVALID_ARG, "negative sweep count");
+  if (!std::isfinite(c.confinement)) return fail(FS_E_INVALID_ARG, "confinement must be finite");
   if (c.mg_coarse_exact != 0 && c.mg_coarse_exact != 1)
     return fail(FS_E_INVALID_ARG, "mg_coarse_exact must be 0 or 1, got %d", c.mg_coarse_exact);
   if (c.tile < 0 || c.tile > GRID_TILE_KINDS)
@@ -3128,7 +3145,7 @@ int fs_render(void* h, double mode, double scale, double bloom_threshold, int bl
     HIPCHK(hipMemsetAsync(s->bloom[0], 0, nb * sizeof(fsk::Half4), s->stream));
     s->bloom_cur = 0;
   }
-  fsk::k_render<<<grid2(s->nx, s->ny), BLK, 0, s->stream>>>(
+  fsk::k_render<<<fsk::vt_grid(s->nx, s->ny), BLK, 0, s->stream>>>(
       s->u[s->cur], s->v[s->cur], s->dye[0] ? s->dye[s->dye_cur] : nullptr, s->out_img, s->nx, s->ny,
       (float)s->dx, (float)s->dy, (float)scale, (float)mode);
   LAUNCHCHK();
@@ -3265,6 +3282,19 @@ int fs_set_nu(void* h, double nu) {
     destroy_graphs(s);
   }
   s->cfg.nu = nu;
+  return FS_OK;
+}
+
+int fs_set_confinement(void* h, double eps) {
+  int rc = check_solver(h);
+  if (rc) return rc;
+  if (!std::isfinite(eps)) return fail(FS_E_INVALID_ARG, "confinement must be finite");
+  fs_solver_t* s = (fs_solver_t*)h;
+  if (eps != s->cfg.confinement) {  // captured bodies hold the strength
+    if ((rc = set_device(s)) || (rc = sync_stream(s))) return rc;
+    destroy_graphs(s);
+  }
+  s->cfg.confinement = eps;
   return FS_OK;
 }
 
@@ -3599,7 +3629,7 @@ int fs_run_kernel(void* h, int kernel, double dt) {
   fs_solver_t* s = (fs_solver_t*)h;
   if ((rc = set_device(s))) return rc;
   if (kernel == FS_K_ADVECT_SL || kernel == FS_K_ADVECT_MC || kernel == FS_K_DIFFUSE || kernel == FS_K_FORCING ||
-      kernel == FS_K_ADVECT_DYE) {
+      kernel == FS_K_ADVECT_DYE || kernel == FS_K_CONFINE) {
     if (!(dt >= 0) || !std::isfinite(dt)) return fail(FS_E_INVALID_ARG, "dt must be finite and >= 0");
     if ((rc = enqueue_head(s, false, dt, 0, false))) return rc;
   }
@@ -3635,6 +3665,10 @@ int fs_run_kernel(void* h, int kernel, double dt) {
     case FS_K_ADVECT_DYE:
       if (!s->dye[0]) return fail(FS_E_INVALID_ARG, "no dye: upload one or splat some first");
       rc = enqueue_dye(s);
+      break;
+    case FS_K_CONFINE:
+      if (!(s->cfg.confinement > 0)) return fail(FS_E_INVALID_ARG, "confinement is off (strength <= 0)");
+      rc = enqueue_confinement(s);
       break;
     case FS_K_GRADIENT:
       if (s->cfg.pressure_solver != FS_P_MULTIGRID)

@@ -29,7 +29,7 @@ PROFILE_KINDS = {"advect_sl": 0, "advect_mc": 1, "diffuse": 2, "down0": 3, "down
                  "gradient": 7, "residual": 8, "rb": 9, "guess": 10,
                  "advect_fused": 11, "forcing": 12, "up_coarse": 13}
 KERNEL = {"advect-sl": 0, "advect-mc": 1, "divergence": 2, "diffuse": 3, "project": 4, "gradient": 5, "solid": 6,
-          "forcing": 7, "advect-dye": 8}
+          "forcing": 7, "advect-dye": 8, "confine": 9}
 
 
 class FsConfig(C.Structure):
@@ -56,6 +56,7 @@ class FsConfig(C.Structure):
         ("dye_fade", C.c_double),
         ("mg_coarse_exact", C.c_int),
         ("tile", C.c_int),
+        ("confinement", C.c_double),
     ]
 
 
@@ -154,6 +155,7 @@ _SIGNATURES = {
     "fs_step_splats": (C.c_int, [_VP, C.c_double, _VP, C.c_int]),
     "fs_upload_dye": (C.c_int, [_VP, _VP, C.c_size_t]),
     "fs_render": (C.c_int, [_VP, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "fs_set_confinement": (C.c_int, [_VP, C.c_double]),
     "fs_step_cfl": (C.c_int, [_VP, C.c_double]),
     "fs_sync": (C.c_int, [_VP]),
     "fs_max_divergence": (C.c_int, [_VP, C.POINTER(C.c_double)]),

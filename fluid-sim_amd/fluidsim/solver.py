@@ -72,8 +72,6 @@ class GPUFluidSolver:
             raise ValueError(f"unknown multigrid precision {mgPrecision}")
         if guess not in _lib.GUESS:
             raise ValueError(f"unknown guess mode {guess}")
-        if confinement:
-            raise ValueError("vorticity confinement is a display-only force and is not built here")
         if not (0.0 <= dyeFade <= 1.0):
             raise ValueError("dyeFade must be in [0, 1]")
         if mgCoarseSolve not in ("sweeps", "exact"):
@@ -95,6 +93,7 @@ class GPUFluidSolver:
         cfg.gated = 1 if gated else 0
         cfg.graph = 1 if graph else 0
         cfg.dye_fade = float(dyeFade)
+        cfg.confinement = float(confinement)
         cfg.mg_coarse_exact = 1 if mgCoarseSolve == "exact" else 0
         cfg.tile = 0 if tile is None else {"64x32": 1, "32x32": 2, "16x16": 3, "8x8": 4, "64x64": 5,
                                            "32x32/1024": 6, "16x16/256": 7}[tile]
@@ -106,6 +105,7 @@ class GPUFluidSolver:
         self.dx, self.dy = lx / nx, ly / ny
         self.lx, self.ly = lx, ly
         self._nu = nu
+        self._confinement = float(confinement)
         self.advection = advection
         self.pressureSolver = pressureSolver
         self._pressureIterations = int(pressureIterations)
@@ -127,6 +127,15 @@ class GPUFluidSolver:
     def nu(self, value):
         check(self._L.fs_set_nu(self.handle, float(value)))
         self._nu = float(value)
+
+    @property
+    def confinement(self):
+        return self._confinement
+
+    @confinement.setter
+    def confinement(self, value):  # mutable, as index.html:471 uses it; <= 0 is off
+        check(self._L.fs_set_confinement(self.handle, float(value)))
+        self._confinement = float(value)
 
     @property
     def pressureIterations(self):

@@ -143,6 +143,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   if (prop_double(env, argv[0], "guess", &d)) c.guess = (int)d;
   if (prop_double(env, argv[0], "solidMaskWidth", &d)) c.solid_mask_width = d;
   if (prop_double(env, argv[0], "dyeFade", &d)) c.dye_fade = d;
+  if (prop_double(env, argv[0], "confinement", &d)) c.confinement = d;
   if (prop_double(env, argv[0], "mgCoarseExact", &d)) c.mg_coarse_exact = (int)d;
   if (prop_double(env, argv[0], "device", &d)) c.device = (int)d;
   void* s = NULL;
@@ -640,7 +641,7 @@ static napi_value js_set(napi_env env, napi_callback_info info) {
   double val;
   NAPI_CALL(env, napi_get_value_double(env, argv[2], &val));
   int which = !strcmp(key, "pressureIterations") ? 0 : !strcmp(key, "tolerance") ? 1 : !strcmp(key, "nu") ? 2
-              : !strcmp(key, "guess") ? 3 : -1;
+              : !strcmp(key, "guess") ? 3 : !strcmp(key, "confinement") ? 4 : -1;
   if (which < 0) {
     napi_throw_error(env, NULL, "unknown parameter");
     return NULL;
@@ -649,7 +650,8 @@ static napi_value js_set(napi_env env, napi_callback_info info) {
   int rc = which == 0 ? fs_set_pressure_iterations(h->solver, (int)val)
            : which == 1 ? fs_set_tolerance(h->solver, val)
            : which == 2 ? fs_set_nu(h->solver, val)
-                        : fs_set_guess(h->solver, (int)val);
+           : which == 3 ? fs_set_guess(h->solver, (int)val)
+                        : fs_set_confinement(h->solver, val);
   unlock(h);
   if (rc) return throw_fs(env, rc);
   return NULL;

@@ -6,8 +6,9 @@
 // copies run on a worker thread; calls on one solver keep their call order).
 // Deliberate differences: unknown selectors throw (the reference silently runs
 // Jacobi / semi-Lagrangian, src/gpu/solver.js:606, :725); each splat keeps its own
-// parameters (the reference binds the first one's, :546-553); vorticity confinement
-// and rendering are display features that are not built. setForcing(modes, opts)
+// parameters (the reference binds the first one's, :546-553); render() writes
+// `output` and `bloom` as the reference's, but there is no canvas to present them
+// (readField("output") returns their half-float bits). setForcing(modes, opts)
 // is the reference's mode-list forcing; setBandForcing() adds the CPU solver's
 // seeded band forcing (src/cpu/solver.js:179-237). The reference's internal
 // textures are named objects here (velCur, velNext, presCur, div): readTexture()
@@ -24,7 +25,7 @@ const GUESS = { zero: 0, previous: 1, learned: 2, "learned-coarse": 3, "learned-
   extrapolated: 5 };
 // fs_run_kernel ids (include/fluidsim.h)
 const KERNEL = { "advect-sl": 0, "advect-mc": 1, divergence: 2, diffuse: 3, project: 4, gradient: 5, solid: 6,
-  forcing: 7, "advect-dye": 8 };
+  forcing: 7, "advect-dye": 8, confine: 9 };
 // fs_read_field ids
 const FIELD = { u: 0, v: 1, div: 2, pressure: 3, residual: 4, dye: 5, output: 6, bloom: 7 };
 
@@ -66,7 +67,6 @@ export class GPUFluidSolver {
       throw new Error(`unknown pressure solver ${pressureSolver}; have ${Object.keys(PRESSURE).join(", ")}`);
     }
     if (!(guess in GUESS)) throw new Error(`unknown guess mode ${guess}`);
-    if (confinement > 0) throw new Error("vorticity confinement is display-only and not built here");
     if (!(dyeFade >= 0 && dyeFade <= 1)) throw new Error("dyeFade must be in [0, 1]");
     if (mgCoarseSolve !== "sweeps" && mgCoarseSolve !== "exact") {
       throw new Error(`unknown coarse solve ${mgCoarseSolve}; have sweeps, exact`);
@@ -75,6 +75,7 @@ export class GPUFluidSolver {
     this.nx = nx; this.ny = ny;
     this.dx = lx / nx; this.dy = ly / ny;
     this._nu = nu;
+    this._confinement = confinement;
     this.advection = advection;
     this.pressureSolver = pressureSolver;
     this._pressureIterations = pressureIterations;
@@ -87,7 +88,7 @@ export class GPUFluidSolver {
       advection: ADVECTION[advection],
       pressureSolver: PRESSURE[pressureSolver],
       pressureIterations, tolerance, mgMinSize, mgPre, mgPost, mgCoarse,
-      guess: GUESS[guess], solidMaskWidth, dyeFade, mgCoarseExact: mgCoarseSolve === "exact" ? 1 : 0,
+      guess: GUESS[guess], solidMaskWidth, dyeFade, confinement, mgCoarseExact: mgCoarseSolve === "exact" ? 1 : 0,
     });
     if (device && device._solvers) device._solvers.add(this);
     // the reference's textures by name (src/gpu/solver.js:111-125, :287-292), for readTexture
@@ -107,6 +108,10 @@ export class GPUFluidSolver {
 
   get nu() { return this._nu; }
   set nu(v) { addon.setParam(this._h, "nu", v); this._nu = v; }
+
+  // vorticity confinement (src/gpu/solver.js:558-568), mutable as index.html:471 uses it
+  get confinement() { return this._confinement; }
+  set confinement(v) { addon.setParam(this._h, "confinement", v); this._confinement = v; }
 
   // mutable after construction, as index.html:472 uses it
   get pressureIterations() { return this._pressureIterations; }

@@ -85,7 +85,8 @@ enum {
   FS_K_GRADIENT = 5,    /* velocity <- velocity - grad(pressure) */
   FS_K_SOLID = 6,       /* velocity <- direct forcing toward the disk's rigid motion */
   FS_K_FORCING = 7,     /* velocity <- forcing and drag of the current step index and time, dt */
-  FS_K_ADVECT_DYE = 8   /* dye <- semi-Lagrangian(dye) through the velocity, faded and clamped, dt */
+  FS_K_ADVECT_DYE = 8,  /* dye <- semi-Lagrangian(dye) through the velocity, faded and clamped, dt */
+  FS_K_CONFINE = 9      /* velocity <- velocity + vorticity-confinement force, dt, the configured strength */
 };
 
 /* GPUFluidSolver constructor options, src/gpu/solver.js:64-98 */
@@ -134,6 +135,8 @@ typedef struct fs_config {
                                  1..7: every grid-wide level uses 64x32, 32x32, 16x16, 8x8, 64x64
                                  tiles, or 32x32 with 1024 / 16x16 with 256 threads (no tuning).
                                  Same bits. */
+  double confinement;         /* confinement: vorticity-confinement strength (src/gpu/solver.js:67,
+                                 :558-568); the force runs in each step when > 0; default 0 */
 } fs_config;
 
 /* one splat of step(dt, {splats}) (src/gpu/solver.js:546-554, _writeSplat :745-757): a
@@ -299,6 +302,8 @@ int fs_poisson_residual_after(void* solver, int iterations, double* out);
 int fs_set_pressure_iterations(void* solver, int iterations);
 int fs_set_tolerance(void* solver, double tolerance);
 int fs_set_nu(void* solver, double nu);
+/* the confinement field (index.html:471 mutates it); <= 0 turns the force off */
+int fs_set_confinement(void* solver, double eps);
 int fs_set_guess(void* solver, int guess);
 
 /* learned initial pressure guess (new: the reference zeroes p every step,

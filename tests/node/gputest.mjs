@@ -165,6 +165,19 @@ async function main() {
     save("bloom.u16", await gpu.readField("bloom"));
     gpu.destroy();
   }
+  // vorticity confinement from the constructor, then through the setter (index.html:471)
+  {
+    const gpu = new GPUFluidSolver(device, { n: N, nu: 0, pressureSolver: "multigrid", pressureIterations: 3,
+      confinement: 1.2 });
+    gpu.uploadVelocity(tu, tv);
+    for (let k = 0; k < 3; k++) gpu.step(dt);
+    gpu.confinement = 2.5;
+    for (let k = 0; k < 2; k++) gpu.step(dt);
+    const { u, v } = await gpu.readVelocity();
+    save("conf_u.f32", u); save("conf_v.f32", v);
+    R.confinement = gpu.confinement;
+    gpu.destroy();
+  }
   writeFileSync(join(out, "results.json"), JSON.stringify(R));
   console.log(JSON.stringify({ ok: true }));
 }

@@ -200,3 +200,14 @@ def test_node_gputest_checks_equal_the_python_binding(tmp_path):
     assert R["renderOutputRejected"] is True
     assert same(img.view(np.uint16).reshape(-1), _load(d, "render.u16", np.uint16))
     assert same(bloom.view(np.uint16).reshape(-1), _load(d, "bloom.u16", np.uint16))
+    with fluidsim.GPUFluidSolver(0, n=N, nu=0.0, pressureSolver="multigrid", pressureIterations=3,
+                                 confinement=1.2) as g:
+        g.uploadVelocity(tu, tv)
+        for _ in range(3):
+            g.step(dt)
+        g.confinement = 2.5
+        for _ in range(2):
+            g.step(dt)
+        vel = g.readVelocity()
+    assert R["confinement"] == 2.5
+    assert same(vel["u"], _load(d, "conf_u.f32", np.float32)) and same(vel["v"], _load(d, "conf_v.f32", np.float32))
