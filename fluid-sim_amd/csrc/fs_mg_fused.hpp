@@ -32,7 +32,7 @@ template <int TX_, int TY_, int NT_>
 struct TileShape {
   static constexpr int TX = TX_, TY = TY_, NT = NT_;
 };
-using Tile64 = TileShape<64, 32, 256>;
+using Tile64 = TileShape<64, 32, 512>;  // with 256 threads the level-0 down-sweep at 2048^2 took 9 % longer
 using Tile32 = TileShape<32, 32, 256>;
 using Tile16 = TileShape<16, 16, 128>;
 using Tile8 = TileShape<8, 8, 64>;

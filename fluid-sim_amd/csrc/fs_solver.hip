@@ -745,8 +745,9 @@ int mg_vcycle(fs_solver_t* s) {
 // yields a tile per CU; the autotuner replaces it with the fastest measured one.
 // Every shape computes the same bits (each cell sees the same operands in the same
 // order), so the choice affects time only.
-// Shapes: 64 x 32, 32 x 32, 16 x 16, 8 x 8 (the first TILE_KINDS), 64 x 64 tiles of 1024
-// threads, and 32 x 32 / 16 x 16 tiles with four times / twice the threads.
+// Shapes: 64 x 32 (512 threads), 32 x 32 (256), 16 x 16 (128), 8 x 8 (64) (the first
+// TILE_KINDS), 64 x 64 tiles of 1024 threads, and 32 x 32 / 16 x 16 tiles with four times /
+// twice the threads.
 enum TileKind { TILE64 = 0, TILE32, TILE16, TILE8, TILE_KINDS, TILE64SQ = TILE_KINDS, TILE32W, TILE16W,
                 GRID_TILE_KINDS };
 static const int kTileW[GRID_TILE_KINDS] = {64, 32, 16, 8, 64, 32, 16};
