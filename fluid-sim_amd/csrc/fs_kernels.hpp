@@ -916,14 +916,14 @@ struct ModeSet {
 
 // Evaluated separably: with A = frac(tx x) and B = frac(ty y + phase) reduced to
 // [0, 1) turns in FP64 (so large wavenumbers keep full FP32 accuracy),
-// cos 2pi(A + B) = cos 2piA cos 2piB - sin 2piA sin 2piB: 4 (nx + ny) sincospi per
-// mode and step instead of 2 nx ny cospi. A block of FX x FY cells stages its
-// columns' and rows' entries of the tables in LDS.
-constexpr int FX = 64, FY = 16;
+// c cos 2pi(A + B) = (c cos 2piA) cos 2piB + (-c sin 2piA) sin 2piB: 4 (nx + ny) sincospi
+// per mode and step instead of 2 nx ny cospi, and two fused multiply-adds per mode and
+// face. A block of FX x FY cells stages its columns' and rows' entries of the tables in LDS.
+constexpr int FX = 64, FY = 32;
 
-// Per-step tables of the separable forcing: col[m][i] = (cos 2piA, sin 2piA) at the u
-// face (x = i) and at the v face (x = i + 1/2); row[m][j] = (cos 2piB, sin 2piB) at the
-// u face (y = j + 1/2) and at the v face (y = j).
+// Per-step tables of the separable forcing: col[m][i] = (cu cos 2piA, -cu sin 2piA) at the
+// u face (x = i) and (cv cos 2piA, -cv sin 2piA) at the v face (x = i + 1/2);
+// row[m][j] = (cos 2piB, sin 2piB) at the u face (y = j + 1/2) and at the v face (y = j).
 __global__ void __launch_bounds__(256)
 k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restrict__ col, float4* __restrict__ row,
                  const StepCoef* __restrict__ cf) {
@@ -947,45 +947,72 @@ k_forcing_tables(const ModeSet* __restrict__ ms, int nx, int ny, float4* __restr
   float ca, sa, cb, sb;
   sincospif(2.f * (float)a, &sa, &ca);
   sincospif(2.f * (float)b, &sb, &cb);
-  if (r < nx) col[(size_t)m * nx + r] = float4{ca, sa, cb, sb};
+  if (r < nx) col[(size_t)m * nx + r] = float4{md.cu * ca, -(md.cu * sa), md.cv * cb, -(md.cv * sb)};
   else row[(size_t)m * ny + (r - nx)] = float4{ca, sa, cb, sb};
 }
 
+// The forcing and drag of FX x FY cells per block of 256 threads (a column each, FY / 4
+// rows apart by 4): the field's and the tables' loads are all issued first, then the
+// tables are staged in LDS, then the sum over the modes in mode order, then the drag.
 __global__ void __launch_bounds__(256)
 k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet* __restrict__ ms,
               const float4* __restrict__ col, const float4* __restrict__ row, const StepCoef* __restrict__ cf) {
   __shared__ float4 sc[MAX_MODES][FX], sr[MAX_MODES][FY];
   const int i0 = blockIdx.x * FX, j0 = blockIdx.y * FY;
   const int nm = ms->n;
-  for (int q = threadIdx.x; q < nm * (FX + FY); q += 256) {
-    const int m = q / (FX + FY), r = q - m * (FX + FY);
-    if (r < FX) {
-      if (i0 + r < nx) sc[m][r] = col[(size_t)m * nx + i0 + r];
-    } else if (j0 + r - FX < ny) {
-      sr[m][r - FX] = row[(size_t)m * ny + j0 + r - FX];
+  const int tx = threadIdx.x & (FX - 1), ty = threadIdx.x / FX;  // 4 rows of threads
+  const int i = i0 + tx;
+  constexpr int R = FY / 4;  // rows per thread: ty, ty + 4, ...
+  float ur[R], vr[R];
+#pragma unroll
+  for (int rr = 0; rr < R; rr++) {
+    const int j = j0 + ty + 4 * rr;
+    ur[rr] = vr[rr] = 0.f;
+    if (i < nx && j < ny) {
+      const size_t k = (size_t)j * nx + i;
+      ur[rr] = u[k];
+      vr[rr] = v[k];
+    }
+  }
+  // every table load in flight before the first LDS store
+  constexpr int SPT = (MAX_MODES * (FX + FY) + 255) / 256;
+  float4 t[SPT];
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+    const int q = (int)threadIdx.x + k * 256;
+    t[k] = float4{0.f, 0.f, 0.f, 0.f};
+    if (q < nm * (FX + FY)) {
+      const int m = q / (FX + FY), r = q - m * (FX + FY);
+      if (r < FX) {
+        if (i0 + r < nx) t[k] = col[(size_t)m * nx + i0 + r];
+      } else if (j0 + r - FX < ny) {
+        t[k] = row[(size_t)m * ny + j0 + r - FX];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+    const int q = (int)threadIdx.x + k * 256;
+    if (q < nm * (FX + FY)) {
+      const int m = q / (FX + FY), r = q - m * (FX + FY);
+      if (r < FX) sc[m][r] = t[k];
+      else sr[m][r - FX] = t[k];
     }
   }
   __syncthreads();
-  const int tx = threadIdx.x & (FX - 1), ty = threadIdx.x / FX;  // 4 rows of threads
-  const int i = i0 + tx;
   if (i >= nx) return;
   const float fdt = cf->fdt, drag = cf->drag;
   const bool after_drag = ms->after_drag != 0;
-  constexpr int R = FY / 4;  // rows per thread: ty, ty + 4, ...
   float fu[R], fv[R];
 #pragma unroll
   for (int rr = 0; rr < R; rr++) fu[rr] = fv[rr] = 0.f;
   for (int m = 0; m < nm; m++) {
-    const Mode& md = ms->m[m];
     const float4 c = sc[m][tx];
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const float4 w = sr[m][ty + 4 * rr];
-      // cos(a + b) = cos a cos b - sin a sin b, one product fused
-      float cu = fmaf(c.x, w.x, -(c.y * w.y));
-      float cv = fmaf(c.z, w.z, -(c.w * w.w));
-      fu[rr] = fmaf(md.cu, cu, fu[rr]);
-      fv[rr] = fmaf(md.cv, cv, fv[rr]);
+      fu[rr] = fmaf(c.y, w.y, fmaf(c.x, w.x, fu[rr]));
+      fv[rr] = fmaf(c.w, w.w, fmaf(c.z, w.z, fv[rr]));
     }
   }
 #pragma unroll
@@ -994,11 +1021,11 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
     if (j >= ny) break;
     const size_t k = (size_t)j * nx + i;
     if (after_drag) {
-      u[k] = u[k] * drag + fu[rr] * fdt;
-      v[k] = v[k] * drag + fv[rr] * fdt;
+      u[k] = ur[rr] * drag + fu[rr] * fdt;
+      v[k] = vr[rr] * drag + fv[rr] * fdt;
     } else {
-      u[k] = (u[k] + fu[rr] * fdt) * drag;
-      v[k] = (v[k] + fv[rr] * fdt) * drag;
+      u[k] = (ur[rr] + fu[rr] * fdt) * drag;
+      v[k] = (vr[rr] + fv[rr] * fdt) * drag;
     }
   }
 }
