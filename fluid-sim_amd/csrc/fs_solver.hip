@@ -931,7 +931,10 @@ bool fold_coarse(const fs_solver_t* s, int l) {
 }
 template <int POST, class TS, typename S>
 int launch_up2_s(fs_solver_t* s, int l) {
-  return fold_coarse(s, l) ? launch_up2_f<POST, TS, S, true>(s, l) : launch_up2_f<POST, TS, S, false>(s, l);
+  // the folded solve runs on 4 or 16 wavefronts
+  if constexpr (TS::NT == 256 || TS::NT == 1024)
+    if (fold_coarse(s, l)) return launch_up2_f<POST, TS, S, true>(s, l);
+  return launch_up2_f<POST, TS, S, false>(s, l);
 }
 // 8 x 8 tiles of 1024 threads for the pair above the coarse solve (levels 256 and 128
 // over a 64 coarse level: more workgroups in flight than 16 x 16, short passes), 16 x 16
@@ -943,6 +946,8 @@ int launch_up2(fs_solver_t* s, int l) {
   if (l == 0 && s->lv[0].tile[1] == TILE64SQ && up2_64sq_ok(s))
     return s->bf16 ? launch_up2_s<POST, fsk::Tile64sq, fsk::bf16>(s, l)
                    : launch_up2_s<POST, fsk::Tile64sq, float>(s, l);
+  if (l == 0 && s->lv[0].tile[1] == TILE64 && up2_64sq_ok(s) && !fold_coarse(s, 0))
+    return s->bf16 ? launch_up2_s<POST, fsk::Tile64, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile64, float>(s, l);
   if (l == 0)
     return s->bf16 ? launch_up2_s<POST, fsk::Tile32, fsk::bf16>(s, l) : launch_up2_s<POST, fsk::Tile32, float>(s, l);
   // with the exact solve folded in, 16 x 16 tiles: a quarter of the workgroups repeat it
@@ -1914,7 +1919,7 @@ int autotune_tiles(fs_solver_t* s) {
   for (int l = 0; l < cs && !rc; l++) {
     Level& L = s->lv[l];
     for (int dir = 0; dir < 2 && !rc; dir++) {
-      // level 0 goes up together with level 1 (k_mgt_up2) in 32 x 32 or 64 x 64 tiles
+      // level 0 goes up together with level 1 (k_mgt_up2) in 32 x 32, 64 x 32 or 64 x 64 tiles
       const bool pair0 = dir == 1 && l == 0 && up2_0;
       auto run = [&]() -> int {
         if (pair0) return DISPATCH_SW(post, launch_up2, s, 0);
@@ -1928,7 +1933,9 @@ int autotune_tiles(fs_solver_t* s) {
       for (int round = 0; round < kTuneRounds && !rc; round++)
         for (int j = 0; j < GRID_TILE_KINDS && !rc; j++) {
           const int k = (j + round) % GRID_TILE_KINDS;
-          if (pair0 && k != TILE32 && !(k == TILE64SQ && up2_64sq_ok(s))) continue;
+          if (pair0 && k != TILE32 && !(k == TILE64SQ && up2_64sq_ok(s)) &&
+              !(k == TILE64 && up2_64sq_ok(s) && !fold_coarse(s, 0)))
+            continue;
           L.tile[dir] = k;
           if ((rc = run())) break;
           if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(FS_E_DEVICE, "hipEventRecord");
