@@ -1030,6 +1030,99 @@ k_forcing_sep(float* __restrict__ u, float* __restrict__ v, int nx, int ny, cons
   }
 }
 
+// The same forcing on the matrix cores, for grids whose sides are multiples of 64: the
+// field is a product F = A B with K = 2 (modes) <= 64, A[j][2m] = cos 2piB_m(j),
+// A[j][2m + 1] = sin 2piB_m(j) from the row tables and B[2m][i], B[2m + 1][i] the column
+// tables' coefficient-folded cosine and sine, for u and for v. Each operand value is split
+// into a BF16 head and tail (x = h + l, h = bf16(x), l = bf16(x - h)) and a b is summed as
+// ha hb + ha lb + la hb in FP32 (v_mfma_f32_16x16x32_bf16), ~2^-16 relative per term: the
+// force to ~1e-5 relative, far below the FP32 rounding of the velocity it is added to.
+// A block of four wavefronts covers 64 x 64 cells, each wavefront 16 rows by four 16 x 16
+// output blocks; lane l of the MFMA supplies A[l % 16][8 (l / 16) + t], B[8 (l / 16) + t][l % 16]
+// (t = 0..7) and receives D[4 (l / 16) + q][l % 16] (q = 0..3).
+typedef __bf16 fk_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float fk_f32x4 __attribute__((ext_vector_type(4)));
+constexpr int FMX = 64;  // cells per block side
+__device__ __forceinline__ void fk_split(const float (&x)[8], fk_bf16x8& h, fk_bf16x8& l) {
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const __bf16 hx = (__bf16)x[t];
+    h[t] = hx;
+    l[t] = (__bf16)(x[t] - (float)hx);
+  }
+}
+__device__ __forceinline__ fk_f32x4 fk_mma3(fk_f32x4 acc, fk_bf16x8 ah, fk_bf16x8 al, fk_bf16x8 bh, fk_bf16x8 bl) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(256)
+k_forcing_mma(float* __restrict__ u, float* __restrict__ v, int nx, int ny, const ModeSet* __restrict__ ms,
+              const float4* __restrict__ col, const float4* __restrict__ row, const StepCoef* __restrict__ cf) {
+  const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int i0 = (int)blockIdx.x * FMX, jw = (int)blockIdx.y * FMX + 16 * w;  // the wavefront's rows jw .. jw + 15
+  const int nm = ms->n;
+  // the field this lane updates: rows jw + 4 lg + q, columns i0 + 16 c + li
+  float ur[4][4], vr[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const size_t k = (size_t)(jw + 4 * lg + q) * nx + i0 + 16 * c + li;
+      ur[c][q] = u[k];
+      vr[c][q] = v[k];
+    }
+  // A fragments of row jw + li: modes 16 kk + 4 lg + 0..3, (cos, sin) each, for u and v
+  fk_bf16x8 auh[2], aul[2], avh[2], avl[2];
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++) {
+    float xu[8], xv[8];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int m = 16 * kk + 4 * lg + t;
+      const float4 r = m < nm ? row[(size_t)m * ny + jw + li] : float4{0.f, 0.f, 0.f, 0.f};
+      xu[2 * t] = r.x, xu[2 * t + 1] = r.y, xv[2 * t] = r.z, xv[2 * t + 1] = r.w;
+    }
+    fk_split(xu, auh[kk], aul[kk]);
+    fk_split(xv, avh[kk], avl[kk]);
+  }
+  const float fdt = cf->fdt, drag = cf->drag;
+  const bool after_drag = ms->after_drag != 0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int i = i0 + 16 * c + li;
+    fk_f32x4 fu = {0.f, 0.f, 0.f, 0.f}, fv = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+      float xu[8], xv[8];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int m = 16 * kk + 4 * lg + t;
+        const float4 b = m < nm ? col[(size_t)m * nx + i] : float4{0.f, 0.f, 0.f, 0.f};
+        xu[2 * t] = b.x, xu[2 * t + 1] = b.y, xv[2 * t] = b.z, xv[2 * t + 1] = b.w;
+      }
+      fk_bf16x8 buh, bul, bvh, bvl;
+      fk_split(xu, buh, bul);
+      fk_split(xv, bvh, bvl);
+      fu = fk_mma3(fu, auh[kk], aul[kk], buh, bul);
+      fv = fk_mma3(fv, avh[kk], avl[kk], bvh, bvl);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const size_t k = (size_t)(jw + 4 * lg + q) * nx + i;
+      if (after_drag) {
+        u[k] = ur[c][q] * drag + fu[q] * fdt;
+        v[k] = vr[c][q] * drag + fv[q] * fdt;
+      } else {
+        u[k] = (ur[c][q] + fu[q] * fdt) * drag;
+        v[k] = (vr[c][q] + fv[q] * fdt) * drag;
+      }
+    }
+  }
+}
+
 // --------------------------------------------------- projection, FP32 paths
 
 // Divergence at centres (src/core/grid.js:191-206; DIVERGENCE src/gpu/shaders.js:269-288).

@@ -537,6 +537,11 @@ int enqueue_forcing(fs_solver_t* s) {
                    (const fsk::ModeSet*)s->modes_dev, s->nx, s->ny, s->ftab_col, s->ftab_row,
                    (const fsk::StepCoef*)s->coef)))
     return rc;
+  if (s->nx % fsk::FMX == 0 && s->ny % fsk::FMX == 0)  // the field as a product on the matrix cores
+    return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_mma,
+                  dim3((unsigned)(s->nx / fsk::FMX), (unsigned)(s->ny / fsk::FMX)), dim3(256), s->u[c], s->v[c],
+                  s->nx, s->ny, (const fsk::ModeSet*)s->modes_dev, (const float4*)s->ftab_col,
+                  (const float4*)s->ftab_row, (const fsk::StepCoef*)s->coef);
   const dim3 g((unsigned)((s->nx + fsk::FX - 1) / fsk::FX), (unsigned)((s->ny + fsk::FY - 1) / fsk::FY));
   return launch(s, PK_FORCING, 16.0 * s->n, fsk::k_forcing_sep, g, dim3(256), s->u[c], s->v[c], s->nx, s->ny,
                 (const fsk::ModeSet*)s->modes_dev, (const float4*)s->ftab_col, (const float4*)s->ftab_row,

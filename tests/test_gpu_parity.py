@@ -261,10 +261,12 @@ MODES = [dict(kx=3, ky=-2, phase=0.4), dict(kx=0, ky=5, phase=2.0), dict(kx=0, k
          dict(kx=-4, ky=1, phase=5.5), dict(kx=6, ky=4, phase=3.3)]
 
 
-def test_mode_list_forcing_kernel_matches_oracle(fs, O):
+@pytest.mark.parametrize("n", [128, 96])
+def test_mode_list_forcing_kernel_matches_oracle(fs, O, n):
     # setForcing(modes, {amplitude, drag, phaseDrift}) (src/gpu/solver.js:370-414) applied
-    # once at t = 0 against the oracle's FORCING (src/gpu/shaders.js:532-566)
-    n, dt = 128, 0.02
+    # once at t = 0 against the oracle's FORCING (src/gpu/shaders.js:532-566); 128: the
+    # field as a product on the matrix cores, 96 (not a multiple of 64): the VALU sum
+    dt = 0.02
     from fluidsim import initial
     u0, v0 = initial.decaying_turbulence(n, seed=2)
     with solver(fs, n=n) as g:
