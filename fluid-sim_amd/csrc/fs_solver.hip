@@ -3152,8 +3152,13 @@ int fs_render(void* h, double mode, double scale, double bloom_threshold, int bl
   const int bw = s->nx >> 1, bh = s->ny >> 1;
   const size_t nb = (size_t)bw * (size_t)bh;
   if (!s->out_img) {
-    if ((rc = dalloc(&s->out_img, s->n)) || (rc = dalloc(&s->bloom[0], nb)) || (rc = dalloc(&s->bloom[1], nb)))
+    if ((rc = dalloc(&s->out_img, s->n)) || (rc = dalloc(&s->bloom[0], nb)) || (rc = dalloc(&s->bloom[1], nb))) {
+      dfree(s->out_img);  // all three or none
+      dfree(s->bloom[0]);
+      dfree(s->bloom[1]);
+      s->out_img = s->bloom[0] = s->bloom[1] = nullptr;
       return rc;
+    }
     // a render with no bloom pass leaves the bloom as it was: zero before the first
     HIPCHK(hipMemsetAsync(s->bloom[0], 0, nb * sizeof(fsk::Half4), s->stream));
     s->bloom_cur = 0;

@@ -221,7 +221,7 @@ class GPUFluidSolver:
         the display image of the current velocity and dye and its bloom, enqueued; read
         them with readField("output") and readField("bloom"). Nothing it computes feeds
         back into the solve. Passes run while k < bloomPasses, as the reference's loop."""
-        passes = max(0, math.ceil(bloomPasses))
+        passes = math.ceil(bloomPasses) if bloomPasses > 0 else 0  # NaN or <= 0: no pass
         check(self._L.fs_render(self.handle, float(mode), float(scale), float(bloomThreshold), int(passes)))
 
     # ------------------------------------------------------------------ step
